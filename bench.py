@@ -1,0 +1,151 @@
+#!/usr/bin/env python3
+"""Residual + Jacobian throughput of the ternary tabulated EAM workload on one MI355X.
+
+One step = one residual vector plus the full analytic Jacobian at one
+parameter vector over the whole dataset (500 configurations, 32-2048 atoms,
+240 spline knots), with every input resident in HBM. The metric is
+configs*params/s = C * D / (seconds per step).
+
+Prints one JSON line with the step throughput, the roofline figure of the
+dominant kernel (k_jac_main: FP32 Jacobian force rows), and a CPU baseline:
+the FP64 oracle (a restatement of the reference's oneTBB calculator: central
+differences, two residual evaluations per column, configurations in parallel)
+timed on a bounded sample of columns on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--configs", type=int, default=500)
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="target CPU time of the baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    return ap.parse_args()
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(w, x, seconds, threads):
+    """FD Jacobian of the oracle on a bounded sample of columns."""
+    import oracle
+    oracle.set_threads(threads)
+    o = oracle.Oracle(w.dataset, w.model)
+    t0 = time.time()
+    o.prepare()
+    t_prep = time.time() - t0
+    D = o.param_count
+    t_res = o.time_residual(x, 1)
+    k = int(max(1, min(D, (seconds / max(t_res, 1e-6) - 1) / 2)))
+    t_sample = o.time_fd_jacobian(x, k)  # 1 residual + 2k residuals
+    t_per_res = t_sample / (2 * k + 1)
+    t_unit = t_per_res * (2 * D + 1)
+    C = w.dataset.n_configs
+    return {"value": C * D / t_unit, "unit": "configs*params/s", "cores": threads, "kind": "port",
+            "sample": f"oracle FP64 central-difference Jacobian (reference algorithm, h=1e-5): "
+                      f"1 residual + {k} of {D} columns over all {C} configs ({2 * k + 1} residual "
+                      f"evaluations, {t_sample:.1f} s); full residual+Jacobian extrapolated to "
+                      f"{2 * D + 1} evaluations = {t_unit:.1f} s; neighbour list + site priming "
+                      f"{t_prep:.1f} s untimed"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU: --gpus {a.gpus} is not supported (the sharded design is "
+              "described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+
+    import forcesmith_amd as fa
+    from forcesmith_amd import datasets
+
+    log("generating the ternary EAM workload (500 configs, seed 2)")
+    w = datasets.config2(n_configs=a.configs)
+    ds = w.dataset
+    g = fa.GpuCalculator(ds, w.model, precision=a.precision)
+    t0 = time.time()
+    P = g.prepare()
+    log(f"neighbour list: {P} pairs in {time.time() - t0:.2f} s")
+    lpa = g.autotune()
+    log(f"autotuned lanes per atom: {lpa}")
+    x = g.gather_params()
+    C, D, N, R = ds.n_configs, g.param_count, ds.n_atoms, g.residual_count
+    lib, h = g._lib, g._h
+
+    def step():
+        g.scatter_params(x)  # new parameter vector: slopes, residual and Jacobian all recomputed
+        g._check(lib.fs_eval_residual(h, None, None))
+        g._check(lib.fs_eval_jacobian(h, None))
+
+    for _ in range(a.warmup):
+        step()
+    g.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        step()
+    g.synchronize()
+    dt = time.perf_counter() - t0
+    ms = 1e3 * dt / a.steps
+    value = C * D * a.steps / dt
+    log(f"step {ms:.3f} ms -> {value:.4g} configs*params/s")
+
+    # kernel times with HIP events on the context stream
+    t_res = g.time_step(0, 10)
+    t_jac = g.time_step(1, 10)
+    t_main = g.time_step(4, 10)
+    t_ne = g.time_step(2, 5)
+    # algorithmic bytes of k_jac_main per launch: pair records (16 B/pair) and
+    # the FP32 force rows it writes (3N x D x 4 B)
+    b_main = 16 * P + 4 * 3 * N * D
+    achieved = b_main / (t_main * 1e-3) / 1e9
+    b_alg = 16 * P + 28 * N + 56 * C + 8 * R + 4 * R * D
+    out = {
+        "metric": "residual+Jacobian configs*params/s",
+        "value": value,
+        "unit": "configs*params/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32" if a.precision == "mixed" else "f64",
+        "data": "synthetic (seeded ternary analytic EAM truth + noise)",
+        "config": {"workload": "ternary tabulated EAM, 500 configs of 32-2048 atoms at 0.06/A^3, "
+                               "12 tables x 20 knots (D=240), residual + full Jacobian",
+                   "configs": C, "atoms": N, "pairs": P, "params": D, "rows": R,
+                   "precision": a.precision, "lanes_per_atom": lpa},
+        "roofline": {"bound": "hbm", "kernel": "k_jac_main", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": b_main, "ms_per_launch": t_main},
+        "kernels_ms": {"residual": t_res, "jacobian": t_jac, "jac_main": t_main, "normal_equations": t_ne},
+        "step_bytes_alg": b_alg,
+        "step_hbm_frac": b_alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+    }
+    if not a.no_cpu_baseline:
+        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        log(f"CPU baseline on {threads} threads")
+        out["cpu_baseline"] = cpu_baseline(w, x, a.cpu_seconds, threads)
+        out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

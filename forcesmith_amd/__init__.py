@@ -1,1 +1,387 @@
-"""MI355X-native force-matching evaluator (placeholder; filled in below)."""
+"""MI355X-native force-matching evaluator.
+
+Python host mirror of the reference's force-calculator surface over the
+C ABI of ``libforcesmith_amd.so`` (declared in ``include/forcesmith_amd.h``).
+``GpuCalculator`` mirrors the erased ``ForceCalculator`` of the reference
+(include/forcesmith/force/force_calculator.hpp:84-159): ``prepare``,
+``eval_forces``, ``eval_cached``, ``param_count``, ``gather_params``,
+``scatter_params``, ``gather_bounds``, ``max_cutoff``; ``residual`` and
+``jacobian`` mirror ``ForcesmithFunctor::operator()`` / ``df``
+(src/optimization/forcesmith_functor.cpp:221-292).
+
+There is no CPU fallback: importing works anywhere, but constructing a
+calculator requires the built HIP library and a HIP device and raises
+otherwise.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+
+import numpy as np
+
+from .model import Dataset, SWModel, Table, TabulatedModel, knot_grid, pair_ordinal  # noqa: F401
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libforcesmith_amd.so")
+
+FS_EAM, FS_ADP, FS_SW = 0, 1, 2
+FS_MIXED, FS_FP64 = 0, 1
+STATUS = {0: "FS_OK", -1: "FS_ERR_ARG", -2: "FS_ERR_STATE", -3: "FS_ERR_HIP", -4: "FS_ERR_NOMEM",
+          -5: "FS_ERR_UNSUPPORTED", -6: "FS_ERR_NUMERIC"}
+
+# every symbol include/forcesmith_amd.h declares
+EXPORTS = (
+    "fs_abi_version", "fs_ctx_create", "fs_ctx_retain", "fs_ctx_release", "fs_last_error",
+    "fs_upload_dataset", "fs_set_tabulated_model", "fs_set_sw_model", "fs_set_weights",
+    "fs_param_count", "fs_gather_params", "fs_scatter_params", "fs_gather_bounds", "fs_max_cutoff",
+    "fs_residual_count", "fs_row_offsets", "fs_prepare", "fs_neighbor_count", "fs_get_neighbor_list",
+    "fs_eval_forces", "fs_eval_config", "fs_eval_residual", "fs_eval_jacobian", "fs_normal_equations",
+    "fs_lm_minimize", "fs_population_cost", "fs_de_minimize", "fs_device_buffer", "fs_synchronize",
+    "fs_set_launch", "fs_autotune", "fs_time_step", "fs_mlp_forward", "fs_mlp_loss_grad",
+)
+
+
+class FsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class fs_opts(ct.Structure):
+    _fields_ = [("device", ct.c_int32), ("precision", ct.c_int32), ("reserved", ct.c_int32 * 6)]
+
+
+class fs_de_opts(ct.Structure):
+    _fields_ = [("np", ct.c_int32), ("generations", ct.c_int32), ("F", ct.c_double), ("CR", ct.c_double),
+                ("seed", ct.c_uint64), ("surrogate", ct.c_int32), ("oversample", ct.c_int32),
+                ("retrain_every", ct.c_int32), ("hidden", ct.c_int32), ("target_cost", ct.c_double),
+                ("rank_tau_min", ct.c_double), ("reserved", ct.c_int32 * 4)]
+
+
+_lib = None
+_dp = ct.POINTER(ct.c_double)
+_fp = ct.POINTER(ct.c_float)
+_i32p = ct.POINTER(ct.c_int32)
+_i64p = ct.POINTER(ct.c_int64)
+_u8p = ct.POINTER(ct.c_uint8)
+
+
+def load_library(path: str = LIB_PATH):
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} is missing: build it with `python -m forcesmith_amd.build` "
+                                "(there is no CPU fallback)")
+    lib = ct.CDLL(path)
+    vp = ct.c_void_p
+    sig = {
+        "fs_abi_version": (ct.c_int, []),
+        "fs_ctx_create": (ct.c_int, [ct.POINTER(fs_opts), ct.POINTER(vp)]),
+        "fs_ctx_retain": (ct.c_int, [vp]),
+        "fs_ctx_release": (ct.c_int, [vp]),
+        "fs_last_error": (ct.c_char_p, [vp]),
+        "fs_upload_dataset": (ct.c_int, [vp, ct.c_int32, _i64p, _i32p, _dp, _dp, _dp, _dp, _dp]),
+        "fs_set_tabulated_model": (ct.c_int, [vp, ct.c_int32, ct.c_int32, ct.c_int32, _i32p, _dp, _dp, _dp,
+                                              _u8p]),
+        "fs_set_sw_model": (ct.c_int, [vp, ct.c_int32, _dp, _u8p, _dp, _u8p]),
+        "fs_set_weights": (ct.c_int, [vp, ct.c_double, ct.c_double]),
+        "fs_param_count": (ct.c_int, [vp, _i32p]),
+        "fs_gather_params": (ct.c_int, [vp, _dp]),
+        "fs_scatter_params": (ct.c_int, [vp, _dp]),
+        "fs_gather_bounds": (ct.c_int, [vp, _dp, _dp]),
+        "fs_max_cutoff": (ct.c_int, [vp, _dp]),
+        "fs_residual_count": (ct.c_int, [vp, _i64p]),
+        "fs_row_offsets": (ct.c_int, [vp, _i64p]),
+        "fs_prepare": (ct.c_int, [vp]),
+        "fs_neighbor_count": (ct.c_int, [vp, _i64p]),
+        "fs_get_neighbor_list": (ct.c_int, [vp, _i64p, _i32p, _i32p, _dp]),
+        "fs_eval_forces": (ct.c_int, [vp, _dp, _dp, _dp, _dp]),
+        "fs_eval_config": (ct.c_int, [vp, ct.c_int32, _dp, _dp, _dp, _dp]),
+        "fs_eval_residual": (ct.c_int, [vp, _dp, _dp]),
+        "fs_eval_jacobian": (ct.c_int, [vp, _fp]),
+        "fs_normal_equations": (ct.c_int, [vp, _dp, _dp, _dp]),
+        "fs_lm_minimize": (ct.c_int, [vp, _dp, ct.c_int32, _dp, _i32p]),
+        "fs_population_cost": (ct.c_int, [vp, _dp, ct.c_int32, _dp]),
+        "fs_de_minimize": (ct.c_int, [vp, ct.POINTER(fs_de_opts), _dp, _dp, _dp, _dp, _dp, _i32p]),
+        "fs_device_buffer": (ct.c_int, [vp, ct.c_int32, ct.POINTER(vp), _i64p]),
+        "fs_synchronize": (ct.c_int, [vp]),
+        "fs_set_launch": (ct.c_int, [vp, ct.c_int32, ct.c_int32]),
+        "fs_autotune": (ct.c_int, [vp, _i32p]),
+        "fs_time_step": (ct.c_int, [vp, ct.c_int32, ct.c_int32, _dp]),
+        "fs_mlp_forward": (ct.c_int, [vp, ct.c_int32, ct.c_int32, ct.c_int32, _fp, _fp, _fp]),
+        "fs_mlp_loss_grad": (ct.c_int, [vp, ct.c_int32, ct.c_int32, ct.c_int32, _fp, _fp, _fp, _fp, _fp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _p(a, typ):
+    return a.ctypes.data_as(typ) if a is not None else ct.cast(None, typ)
+
+
+class GpuCalculator:
+    """A dataset plus a model resident on one HIP device (one fs_ctx)."""
+
+    def __init__(self, dataset: Dataset, model, precision: str = "mixed", device: int = 0,
+                 energy_weight=None, stress_weight=None):
+        lib = load_library()
+        self._lib = lib
+        opts = fs_opts(device, FS_FP64 if precision == "fp64" else FS_MIXED)
+        h = ct.c_void_p()
+        rc = lib.fs_ctx_create(ct.byref(opts), ct.byref(h))
+        if rc != 0:
+            raise FsError(rc, lib.fs_last_error(None).decode())
+        self._h = h
+        self.precision = precision
+        self.dataset = dataset
+        self.kind = model.kind
+        off = np.ascontiguousarray(dataset.atom_off, dtype=np.int64)
+        self._check(lib.fs_upload_dataset(h, dataset.n_configs, _p(off, _i64p),
+                                          _p(np.ascontiguousarray(dataset.types, np.int32), _i32p),
+                                          _p(np.ascontiguousarray(dataset.pos, np.float64), _dp),
+                                          _p(np.ascontiguousarray(dataset.ref_force, np.float64), _dp),
+                                          _p(np.ascontiguousarray(dataset.cell, np.float64), _dp),
+                                          _p(np.ascontiguousarray(dataset.ref_energy, np.float64), _dp),
+                                          _p(np.ascontiguousarray(dataset.ref_stress, np.float64), _dp)))
+        self.set_model(model)
+        ew = model.energy_weight if energy_weight is None else energy_weight
+        sw = model.stress_weight if stress_weight is None else stress_weight
+        self._check(lib.fs_set_weights(h, float(ew), float(sw)))
+        self.n_atoms = dataset.n_atoms
+        self.n_configs = dataset.n_configs
+
+    def _check(self, rc):
+        if rc != 0:
+            raise FsError(rc, self._lib.fs_last_error(self._h).decode())
+
+    def set_model(self, model):
+        lib, h = self._lib, self._h
+        if model.kind == "sw":
+            self._check(lib.fs_set_sw_model(h, model.ntypes, _p(np.ascontiguousarray(model.pair_params), _dp),
+                                            _p(np.ascontiguousarray(model.pair_fixed, np.uint8), _u8p),
+                                            _p(np.ascontiguousarray(model.lam), _dp),
+                                            _p(np.ascontiguousarray(model.lam_fixed, np.uint8), _u8p)))
+        else:
+            kind = FS_EAM if model.kind == "eam" else FS_ADP
+            n = np.array([t.n for t in model.tables], dtype=np.int32)
+            rmin = np.array([t.rmin for t in model.tables], dtype=np.float64)
+            rmax = np.array([t.rmax for t in model.tables], dtype=np.float64)
+            y = np.concatenate([t.y for t in model.tables]).astype(np.float64)
+            fx = np.concatenate([t.fixed for t in model.tables]).astype(np.uint8)
+            self._check(lib.fs_set_tabulated_model(h, kind, model.ntypes, len(model.tables), _p(n, _i32p),
+                                                   _p(rmin, _dp), _p(rmax, _dp), _p(y, _dp), _p(fx, _u8p)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.fs_ctx_release(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- CFittable surface ----
+    @property
+    def param_count(self) -> int:
+        v = ct.c_int32()
+        self._check(self._lib.fs_param_count(self._h, ct.byref(v)))
+        return int(v.value)
+
+    def gather_params(self):
+        x = np.zeros(self.param_count)
+        self._check(self._lib.fs_gather_params(self._h, _p(x, _dp)))
+        return x
+
+    def scatter_params(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        self._check(self._lib.fs_scatter_params(self._h, _p(x, _dp)))
+
+    def gather_bounds(self):
+        lo, hi = np.zeros(self.param_count), np.zeros(self.param_count)
+        self._check(self._lib.fs_gather_bounds(self._h, _p(lo, _dp), _p(hi, _dp)))
+        return lo, hi
+
+    @property
+    def max_cutoff(self) -> float:
+        v = ct.c_double()
+        self._check(self._lib.fs_max_cutoff(self._h, ct.byref(v)))
+        return v.value
+
+    @property
+    def residual_count(self) -> int:
+        v = ct.c_int64()
+        self._check(self._lib.fs_residual_count(self._h, ct.byref(v)))
+        return int(v.value)
+
+    def row_offsets(self):
+        r = np.zeros(self.n_configs + 1, dtype=np.int64)
+        self._check(self._lib.fs_row_offsets(self._h, _p(r, _i64p)))
+        return r
+
+    # ---- prepare / neighbour list ----
+    def prepare(self) -> int:
+        self._check(self._lib.fs_prepare(self._h))
+        v = ct.c_int64()
+        self._check(self._lib.fs_neighbor_count(self._h, ct.byref(v)))
+        return int(v.value)
+
+    def neighbor_list(self):
+        v = ct.c_int64()
+        self._check(self._lib.fs_neighbor_count(self._h, ct.byref(v)))
+        P = int(v.value)
+        off = np.zeros(self.n_atoms + 1, dtype=np.int64)
+        j = np.zeros(P, dtype=np.int32)
+        img = np.zeros((P, 3), dtype=np.int32)
+        d = np.zeros((P, 3))
+        self._check(self._lib.fs_get_neighbor_list(self._h, _p(off, _i64p), _p(j, _i32p), _p(img, _i32p),
+                                                   _p(d, _dp)))
+        return off, j, img, d
+
+    # ---- evaluation ----
+    def eval_forces(self):
+        F = np.zeros((self.n_atoms, 3))
+        E = np.zeros(self.n_configs)
+        S = np.zeros((self.n_configs, 3, 3))
+        L = np.zeros(self.n_configs)
+        self._check(self._lib.fs_eval_forces(self._h, _p(F, _dp), _p(E, _dp), _p(S, _dp), _p(L, _dp)))
+        return F, E, S, L
+
+    def eval_cached(self, config: int):
+        na = int(self.dataset.atom_off[config + 1] - self.dataset.atom_off[config])
+        F = np.zeros((na, 3))
+        E, L = ct.c_double(), ct.c_double()
+        S = np.zeros((3, 3))
+        self._check(self._lib.fs_eval_config(self._h, int(config), _p(F, _dp), ct.byref(E), _p(S, _dp),
+                                             ct.byref(L)))
+        return F, E.value, S, L.value
+
+    def residual(self, x=None, with_cost=False):
+        if x is not None:
+            self.scatter_params(x)
+        F = np.zeros(self.residual_count)
+        c = ct.c_double()
+        self._check(self._lib.fs_eval_residual(self._h, _p(F, _dp), ct.byref(c)))
+        return (F, c.value) if with_cost else F
+
+    def cost(self, x=None) -> float:
+        if x is not None:
+            self.scatter_params(x)
+        c = ct.c_double()
+        self._check(self._lib.fs_eval_residual(self._h, ct.cast(None, _dp), ct.byref(c)))
+        return c.value
+
+    def jacobian(self, x=None):
+        if x is not None:
+            self.scatter_params(x)
+        J = np.zeros((self.residual_count, self.param_count), dtype=np.float32)
+        self._check(self._lib.fs_eval_jacobian(self._h, _p(J, _fp)))
+        return J
+
+    def normal_equations(self):
+        D = self.param_count
+        A, g, c = np.zeros((D, D)), np.zeros(D), ct.c_double()
+        self._check(self._lib.fs_normal_equations(self._h, _p(A, _dp), _p(g, _dp), ct.byref(c)))
+        return A, g, c.value
+
+    def lm(self, x, max_iter):
+        x = np.ascontiguousarray(x, dtype=np.float64).copy()
+        hist = np.zeros(max_iter)
+        acc = ct.c_int32()
+        self._check(self._lib.fs_lm_minimize(self._h, _p(x, _dp), int(max_iter), _p(hist, _dp), ct.byref(acc)))
+        return x, hist, int(acc.value)
+
+    def population_cost(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        c = np.zeros(X.shape[0])
+        self._check(self._lib.fs_population_cost(self._h, _p(X, _dp), X.shape[0], _p(c, _dp)))
+        return c
+
+    def de(self, x, lo, hi, np_, generations, F=0.65, CR=0.5, seed=42, surrogate=False, oversample=4,
+           retrain_every=10, hidden=256, target_cost=0.0, rank_tau_min=0.3):
+        o = fs_de_opts(int(np_), int(generations), float(F), float(CR), int(seed), int(bool(surrogate)),
+                       int(oversample), int(retrain_every), int(hidden), float(target_cost),
+                       float(rank_tau_min))
+        x = np.ascontiguousarray(x, dtype=np.float64).copy()
+        best = ct.c_double()
+        hist = np.zeros(generations)
+        gens = ct.c_int32()
+        self._check(self._lib.fs_de_minimize(self._h, ct.byref(o), _p(np.ascontiguousarray(lo, np.float64), _dp),
+                                             _p(np.ascontiguousarray(hi, np.float64), _dp), _p(x, _dp),
+                                             ct.byref(best), _p(hist, _dp), ct.byref(gens)))
+        return x, best.value, hist[:gens.value]
+
+    # ---- launch shape / timing ----
+    def set_launch(self, lanes_per_atom: int, blocks_per_cu: int = 8):
+        self._check(self._lib.fs_set_launch(self._h, int(lanes_per_atom), int(blocks_per_cu)))
+
+    def autotune(self) -> int:
+        v = ct.c_int32()
+        self._check(self._lib.fs_autotune(self._h, ct.byref(v)))
+        return int(v.value)
+
+    def time_step(self, what: int = 0, reps: int = 10) -> float:
+        v = ct.c_double()
+        self._check(self._lib.fs_time_step(self._h, int(what), int(reps), ct.byref(v)))
+        return v.value
+
+    def synchronize(self):
+        self._check(self._lib.fs_synchronize(self._h))
+
+    # ---- MLP surrogate kernels (host buffers) ----
+    def mlp_forward(self, params, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        n, d = X.shape
+        h = mlp_hidden(len(params), d)
+        y = np.zeros(n, dtype=np.float32)
+        self._check(self._lib.fs_mlp_forward(self._h, n, d, h, _p(np.ascontiguousarray(params, np.float32), _fp),
+                                             _p(X, _fp), _p(y, _fp)))
+        return y
+
+    def mlp_loss_grad(self, params, X, t):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        n, d = X.shape
+        h = mlp_hidden(len(params), d)
+        g = np.zeros(len(params), dtype=np.float32)
+        loss = ct.c_float()
+        self._check(self._lib.fs_mlp_loss_grad(self._h, n, d, h, _p(np.ascontiguousarray(params, np.float32), _fp),
+                                               _p(X, _fp), _p(np.ascontiguousarray(t, np.float32), _fp),
+                                               _p(g, _fp), ct.byref(loss)))
+        return loss.value, g
+
+
+def mlp_hidden(n_params: int, d: int) -> int:
+    """Width h of the D -> h -> h -> 1 MLP with n_params parameters."""
+    # n = h*d + h + h*h + h + h + 1
+    a, b, c = 1.0, d + 3.0, 1.0 - n_params
+    h = int(round((-b + (b * b - 4 * a * c) ** 0.5) / 2))
+    assert h * d + h + h * h + 3 * h + 1 == n_params
+    return h
+
+
+def mlp_pack(W1, b1, W2, b2, W3, b3):
+    return np.concatenate([np.ravel(W1), b1, np.ravel(W2), b2, W3, [b3]]).astype(np.float32)
+
+
+def mlp_unpack(p, d, h):
+    o = 0
+    W1 = p[o:o + h * d].reshape(h, d); o += h * d
+    b1 = p[o:o + h]; o += h
+    W2 = p[o:o + h * h].reshape(h, h); o += h * h
+    b2 = p[o:o + h]; o += h
+    W3 = p[o:o + h]; o += h
+    return W1, b1, W2, b2, W3, float(p[o])
+
+
+def stress_voigt(S):
+    """3x3 stresses -> residual order xx, yy, zz, xy, xz, yz."""
+    S = np.asarray(S)
+    return np.stack([S[..., 0, 0], S[..., 1, 1], S[..., 2, 2], S[..., 0, 1], S[..., 0, 2], S[..., 1, 2]], -1)

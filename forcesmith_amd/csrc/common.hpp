@@ -1,0 +1,251 @@
+// Internal types shared by the host runtime and the HIP kernels of
+// libforcesmith_amd. Host code and kernels are compiled for gfx950 only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/forcesmith_amd.h"
+
+namespace fsa {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define FS_HIP(call)                                                            \
+  do {                                                                          \
+    hipError_t e_ = (call);                                                     \
+    if (e_ != hipSuccess)                                                       \
+      throw ::fsa::Error(FS_ERR_HIP, std::string(#call) + ": " +               \
+                                         hipGetErrorString(e_));                \
+  } while (0)
+
+#define FS_REQUIRE(cond, code, msg)                                            \
+  do {                                                                          \
+    if (!(cond)) throw ::fsa::Error((code), (msg));                             \
+  } while (0)
+
+// Device buffer owned by a context; grows, never shrinks.
+template <class T> struct DBuf {
+  T *p = nullptr;
+  size_t cap = 0;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = n = 0;
+  }
+  T *alloc(size_t count) {
+    if (count > cap) {
+      release();
+      if (count) {
+        hipError_t e = hipMalloc(reinterpret_cast<void **>(&p), count * sizeof(T));
+        if (e != hipSuccess) {
+          p = nullptr;
+          throw Error(FS_ERR_NOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) +
+                                        " bytes failed: " + hipGetErrorString(e));
+        }
+      }
+      cap = count;
+    }
+    n = count;
+    return p;
+  }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+// ---------------------------------------------------------------------------
+// Pair record flags. A pair is stored as float4 {dx, dy, dz, bits} (mixed
+// precision) plus a uint32 word 'bits' and double3 d (exact path):
+//   bits[0:22)  neighbour index local to its configuration
+//   bits[22:25) neighbour type (ntypes <= 8)
+//   bits[25..]  gates computed once from the FP64 distance at prepare time,
+//               the counterpart of the reference's site priming
+//               (prime_site = in_range ? prepare_site : none, kernels.hpp:14-16)
+// ---------------------------------------------------------------------------
+constexpr uint32_t kJBits = 22;
+constexpr uint32_t kJMask = (1u << kJBits) - 1u;
+constexpr uint32_t kTShift = 22;
+constexpr uint32_t kTMask = 7u;
+constexpr uint32_t kFlagPhi = 1u << 25;   // r inside span of pair table (ti,tj)
+constexpr uint32_t kFlagGj = 1u << 26;    // r inside span of density table tj
+constexpr uint32_t kFlagGi = 1u << 27;    // r inside span of density table ti
+constexpr uint32_t kFlagDip = 1u << 28;   // ADP dipole table
+constexpr uint32_t kFlagQuad = 1u << 29;  // ADP quadrupole table
+constexpr uint32_t kFlagEdge = 1u << 30;  // some role has r exactly at a span end
+constexpr uint32_t kFlagTiny = 1u << 31;  // r < 1e-14: skipped by the force pass
+
+__host__ __device__ inline uint32_t pj_local(uint32_t b) { return b & kJMask; }
+__host__ __device__ inline int pj_type(uint32_t b) { return int((b >> kTShift) & kTMask); }
+
+// Uniform-grid table descriptor. x_k = x0 + k*h evaluated on the host the way
+// the reference builds its grid; the exact x array lives in tab_x.
+struct TableDesc {
+  int32_t n;      // knots
+  int32_t off;    // first knot in the flat knot arrays
+  int32_t ivl;    // first interval in the packed interval arrays
+  int32_t linear; // n < 4: piecewise linear (no slopes)
+  double x0, xn, h, inv_h;
+  float x0f, inv_hf, hf, pad;
+};
+
+struct CfgGrid {
+  int32_t nc[3];  // cells per axis
+  int32_t K[3];   // cell search half-width per axis
+  int32_t s[3];   // reference image shells per axis
+  int32_t cell0;  // first cell of this configuration
+};
+
+constexpr int kMaxTypes = 8;
+
+struct Context;
+struct Mlp;
+struct SurrogateState;
+
+// ---- kernel launchers (defined in the .hip translation units) -------------
+void launch_neighbor_build(Context &c);
+void launch_pair_flags(Context &c);
+void launch_slopes(Context &c);
+void launch_scatter_params(Context &c, const double *d_x);
+void launch_eval(Context &c, bool want_forces);
+void launch_cost(Context &c, double *d_out);
+void launch_jacobian(Context &c);
+void launch_normal_equations(Context &c);
+int lm_minimize_device(Context &c, double *x, int max_iter, double *hist);
+void launch_population_cost(Context &c, const double *d_X, int np, double *d_cost);
+void launch_sw_rev(Context &c);
+void launch_sw_eval(Context &c);
+
+struct Context {
+  std::mutex lock;
+  std::string err;
+  int refs = 1;
+  int device = 0;
+  int precision = FS_MIXED;
+  hipStream_t stream = nullptr;
+  int cu_count = 256;
+  int lanes_per_atom = 16;
+  int blocks_per_cu = 8;
+
+  // ---- dataset (host mirror + device) ------------------------------------
+  int C = 0;
+  int64_t N = 0;
+  std::vector<int64_t> h_atom_off;
+  std::vector<int32_t> h_types, h_cfg_of;
+  std::vector<double> h_pos, h_ref_force, h_cell, h_ref_energy, h_ref_stress;
+  std::vector<double> h_box, h_inv, h_vol;
+  DBuf<int64_t> atom_off, row_off;
+  DBuf<int32_t> types, cfg_of;
+  DBuf<double> pos, ref_force, ref_energy, ref_stress, box, inv, vol;
+
+  // tiles: a tile is up to atoms_per_tile consecutive atoms of one config
+  std::vector<int32_t> h_tile_cfg, h_tile_a0, h_tile_na, h_cfg_tile0;
+  DBuf<int32_t> tile_cfg, tile_a0, tile_na, cfg_tile0;
+  int ntiles = 0;
+  int tile_atoms = 0;
+
+  // ---- model --------------------------------------------------------------
+  bool have_model = false;
+  int kind = FS_EAM;
+  int ntypes = 1, npair = 1, ntab = 0;
+  std::vector<TableDesc> h_tabs;
+  std::vector<double> h_x, h_y; // knot abscissae / values (all tables)
+  std::vector<uint8_t> h_fixed;
+  std::vector<int32_t> h_col_of;   // knot -> free column or -1
+  std::vector<int32_t> h_knot_of;  // free column -> knot
+  int D = 0, nknots = 0, nivl = 0;
+  DBuf<TableDesc> tabs;
+  DBuf<double> tab_x, tab_y, tab_s, tab_S; // S: n*5 band of ds/dy per table
+  DBuf<float4> ivl32;                       // per interval {y_k, s_k, y_k+1, s_k+1}
+  DBuf<int32_t> col_of, knot_of;
+  // SW
+  std::vector<double> h_sw, h_lam;
+  std::vector<uint8_t> h_sw_fixed, h_lam_fixed;
+  DBuf<double> sw, lam;
+  double sw_nl_rc = -1.0;
+  DBuf<double> x_dev;
+  bool params_dirty = true;
+  double energy_weight = 1.0, stress_weight = 0.0;
+  int64_t R = 0;
+  std::vector<int64_t> h_row_off;
+
+  // ---- neighbour list ------------------------------------------------------
+  bool have_nl = false;
+  double nl_rc = -1.0;
+  int64_t P = 0;
+  std::vector<CfgGrid> h_grid;
+  DBuf<CfgGrid> grid;
+  DBuf<int64_t> pair_off;
+  DBuf<float4> pair_rec;
+  DBuf<uint32_t> pair_bits;
+  DBuf<double> pair_d;
+  DBuf<double> pair_r64; // FP64 distance per pair (energy values, density)
+  DBuf<int32_t> pair_img;
+  DBuf<int32_t> rev;         // reverse pair index (SW gather)
+  DBuf<double> wpos;
+  DBuf<int32_t> cell_of, cell_count, cell_start, cell_atoms, counts;
+  DBuf<uint64_t> keys;
+  DBuf<uint8_t> scan_tmp;
+
+  // ---- evaluation state ----------------------------------------------------
+  DBuf<double> rho, Fp, Fpp, e_atom, lim_atom, clamp_over;
+  DBuf<float> Fp32;
+  DBuf<double> mu, lamb; // ADP per-atom dipole (3) and quadrupole (6)
+  DBuf<double> forces, tile_part, cfg_E, cfg_S, cfg_L;
+  DBuf<double> residual, cost, cost_part;
+  DBuf<double> pairbuf; // SW triplet forces per pair
+  bool forces_valid = false;
+  // Jacobian
+  DBuf<float> J;
+  DBuf<float> H;        // per-atom d rho / d knot (density columns) scaled
+  DBuf<double> jac_part; // per-tile energy/stress/limit partial columns
+  DBuf<double> JtJ, Jtr, jtj_part;
+  // Jacobian work buffers: tiles of <= 64 atoms, column descriptors, per-atom
+  // density histograms and embedding windows, SW tangents
+  DBuf<int32_t> jt_cfg, jt_a0, jt_na, jt_ct0;
+  int njt = 0;
+  DBuf<int4> jcols;
+  DBuf<float> Hs, emb_d, emb_v;
+  DBuf<int32_t> emb_k, sw_csw, sw_clam;
+  DBuf<double> sw_own, sw_ptan, sw_etan;
+  DBuf<double> lm_M, lm_dx, lm_scal;
+  Mlp *mlp = nullptr;              // owned; see mlp.hpp
+  SurrogateState *surr = nullptr;  // owned; see de.cpp
+  // per-kernel timing of the main Jacobian kernel (HIP events, context stream)
+  hipEvent_t jev0 = nullptr, jev1 = nullptr;
+  bool time_jac_main = false;
+  double jac_main_ms = 0.0;
+  int jac_main_n = 0;
+
+  ~Context();
+  void time_main_begin() {
+    if (time_jac_main) FS_HIP(hipEventRecord(jev0, stream));
+  }
+  void time_main_end() {
+    if (!time_jac_main) return;
+    FS_HIP(hipEventRecord(jev1, stream));
+    FS_HIP(hipEventSynchronize(jev1));
+    float ms = 0.f;
+    FS_HIP(hipEventElapsedTime(&ms, jev0, jev1));
+    jac_main_ms += ms;
+    ++jac_main_n;
+  }
+  void fail_if_no_dataset() const {
+    FS_REQUIRE(C > 0, FS_ERR_STATE, "no dataset uploaded");
+  }
+  void fail_if_no_model() const { FS_REQUIRE(have_model, FS_ERR_STATE, "no model set"); }
+};
+
+} // namespace fsa

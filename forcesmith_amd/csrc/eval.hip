@@ -1,0 +1,713 @@
+// Residual path: makima slope refresh, EAM/ADP two-pass kernels
+// (density -> clamp + embedding -> force/stress), per-configuration FP64
+// reduction and residual assembly.
+//
+// Reference: EAMForceCalculator::eval_forces (src/force/eam_force.cpp:45-102),
+// ADPForceCalculator::eval_forces (src/force/adp_force.cpp:73-135),
+// clamp_embedding / accumulate_pair (include/forcesmith/force/kernels.hpp:
+// 19-38), with_eval_scope (eval_scope.hpp:14-31), write_config_residuals
+// (include/forcesmith/optimization/residual_layout.hpp:37-54).
+//
+// Layout: atoms are processed in tiles of consecutive atoms of one
+// configuration; a block handles one tile per iteration of a persistent loop,
+// LPA lanes per atom stride over the atom's CSR neighbour segment (coalesced
+// 16-byte pair records), the atom's sums are reduced across its lanes with
+// cross-lane shuffles, and the tile's energy/stress/limit are reduced in LDS
+// in atom order and written as one FP64 partial per tile. Forces are
+// gathered per atom: no atomics anywhere, every result is deterministic.
+#include "common.hpp"
+#include "spline.hpp"
+
+#include <algorithm>
+#include <type_traits>
+
+namespace fsa {
+
+namespace {
+
+constexpr double kPenalty = 100.0 * 10.0; // kDummyWeight * 10 (kernels.hpp:24)
+
+// ---------------------------------------------------------------------------
+// makima slopes + band of d s / d y per table (spline.cpp:43-103)
+// ---------------------------------------------------------------------------
+__device__ inline double dmb(const double *x, int q, int m) {
+  const double h = x[q + 1] - x[q];
+  return m == q ? -1.0 / h : (m == q + 1 ? 1.0 / h : 0.0);
+}
+__device__ inline double dM(const double *x, int n, int q, int m) {
+  if (q >= 0 && q <= n - 2) return dmb(x, q, m);
+  if (q == -1) return 2.0 * dmb(x, 0, m) - dmb(x, 1, m);
+  if (q == -2) return 2.0 * (2.0 * dmb(x, 0, m) - dmb(x, 1, m)) - dmb(x, 0, m);
+  if (q == n - 1) return 2.0 * dmb(x, n - 2, m) - dmb(x, n - 3, m);
+  return 2.0 * (2.0 * dmb(x, n - 2, m) - dmb(x, n - 3, m)) - dmb(x, n - 2, m);
+}
+__device__ inline double Mext(const double *x, const double *y, int n, int q) {
+  auto sec = [&](int k) { return (y[k + 1] - y[k]) / (x[k + 1] - x[k]); };
+  if (q >= 0 && q <= n - 2) return sec(q);
+  if (q == -1) return 2 * sec(0) - sec(1);
+  if (q == -2) {
+    const double m1 = 2 * sec(0) - sec(1);
+    return 2 * m1 - sec(0);
+  }
+  if (q == n - 1) return 2 * sec(n - 2) - sec(n - 3);
+  const double m1 = 2 * sec(n - 2) - sec(n - 3);
+  return 2 * m1 - sec(n - 2);
+}
+__device__ inline double sgnd(double u) { return double((u > 0) - (u < 0)); }
+
+__device__ void table_slopes(const TableDesc &T, const double *x, const double *y, double *s,
+                             double *S, float4 *ivl) {
+  const int n = T.n;
+  if (n >= 4) {
+    double m2 = (y[3] - y[2]) / (x[3] - x[2]);
+    double m1 = (y[2] - y[1]) / (x[2] - x[1]);
+    double m0 = (y[1] - y[0]) / (x[1] - x[0]);
+    double mm1 = 2 * m0 - m1;
+    double mm2 = 2 * mm1 - m0;
+    double w1 = fabs(m1 - m0) + fabs(m1 + m0) / 2;
+    double w2 = fabs(mm1 - mm2) + fabs(mm1 + mm2) / 2;
+    s[0] = (w1 * mm1 + w2 * m0) / (w1 + w2);
+    if (isnan(s[0])) s[0] = 0;
+    w1 = fabs(m2 - m1) + fabs(m2 + m1) / 2;
+    w2 = fabs(m0 - mm1) + fabs(m0 + mm1) / 2;
+    s[1] = (w1 * m0 + w2 * m1) / (w1 + w2);
+    if (isnan(s[1])) s[1] = 0;
+    for (int i = 2; i < n - 2; ++i) {
+      double mim2 = (y[i - 1] - y[i - 2]) / (x[i - 1] - x[i - 2]);
+      double mim1 = (y[i] - y[i - 1]) / (x[i] - x[i - 1]);
+      double mi = (y[i + 1] - y[i]) / (x[i + 1] - x[i]);
+      double mip1 = (y[i + 2] - y[i + 1]) / (x[i + 2] - x[i + 1]);
+      w1 = fabs(mip1 - mi) + fabs(mip1 + mi) / 2;
+      w2 = fabs(mim1 - mim2) + fabs(mim1 + mim2) / 2;
+      s[i] = (w1 * mim1 + w2 * mi) / (w1 + w2);
+      if (isnan(s[i])) s[i] = 0;
+    }
+    double mnm4 = (y[n - 3] - y[n - 4]) / (x[n - 3] - x[n - 4]);
+    double mnm3 = (y[n - 2] - y[n - 3]) / (x[n - 2] - x[n - 3]);
+    double mnm2 = (y[n - 1] - y[n - 2]) / (x[n - 1] - x[n - 2]);
+    double mnm1 = 2 * mnm2 - mnm3;
+    double mn = 2 * mnm1 - mnm2;
+    w1 = fabs(mnm1 - mnm2) + fabs(mnm1 + mnm2) / 2;
+    w2 = fabs(mnm3 - mnm4) + fabs(mnm3 + mnm4) / 2;
+    s[n - 2] = (w1 * mnm3 + w2 * mnm2) / (w1 + w2);
+    if (isnan(s[n - 2])) s[n - 2] = 0;
+    w1 = fabs(mn - mnm1) + fabs(mn + mnm1) / 2;
+    w2 = fabs(mnm2 - mnm3) + fabs(mnm2 + mnm3) / 2;
+    s[n - 1] = (w1 * mnm2 + w2 * mnm1) / (w1 + w2);
+    if (isnan(s[n - 1])) s[n - 1] = 0;
+    // band of the slope jacobian: S[k][m], m in [k-2, k+2]
+    for (int k = 0; k < n; ++k) {
+      const double A = Mext(x, y, n, k - 1), B = Mext(x, y, n, k), Cn = Mext(x, y, n, k + 1),
+                   P = Mext(x, y, n, k - 2);
+      const double W1 = fabs(Cn - B) + fabs(Cn + B) / 2;
+      const double W2 = fabs(A - P) + fabs(A + P) / 2;
+      const double W = W1 + W2;
+      for (int q = 0; q < 5; ++q) S[5 * k + q] = 0.0;
+      if (!(W > 0)) continue;
+      const double sv = (W1 * A + W2 * B) / W;
+      const double dw1C = sgnd(Cn - B) + sgnd(Cn + B) / 2, dw1B = -sgnd(Cn - B) + sgnd(Cn + B) / 2;
+      const double dw2A = sgnd(A - P) + sgnd(A + P) / 2, dw2P = -sgnd(A - P) + sgnd(A + P) / 2;
+      const double ds1 = (A - sv) / W, ds2 = (B - sv) / W;
+      const double gP = ds2 * dw2P, gA = W1 / W + ds2 * dw2A, gB = W2 / W + ds1 * dw1B,
+                   gC = ds1 * dw1C;
+      for (int q = 0; q < 5; ++q) {
+        const int m = k - 2 + q;
+        if (m < 0 || m >= n) continue;
+        S[5 * k + q] = gP * dM(x, n, k - 2, m) + gA * dM(x, n, k - 1, m) +
+                       gB * dM(x, n, k, m) + gC * dM(x, n, k + 1, m);
+      }
+    }
+  } else {
+    for (int k = 0; k < n; ++k) {
+      s[k] = 0.0;
+      for (int q = 0; q < 5; ++q) S[5 * k + q] = 0.0;
+    }
+  }
+  for (int k = 0; k + 1 < n; ++k)
+    ivl[T.ivl + k] = make_float4(float(y[k]), float(s[k]), float(y[k + 1]), float(s[k + 1]));
+}
+
+__global__ void k_slopes(int ntab, const TableDesc *__restrict__ tabs, const double *__restrict__ x,
+                         const double *__restrict__ y, double *__restrict__ s,
+                         double *__restrict__ S, float4 *__restrict__ ivl) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntab) return;
+  const TableDesc T = tabs[t];
+  table_slopes(T, x + T.off, y + T.off, s + T.off, S + 5 * T.off, ivl);
+}
+
+__global__ void k_scatter(int D, const double *__restrict__ xv, const int32_t *__restrict__ knot_of,
+                          double *__restrict__ y) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < D) y[knot_of[c]] = xv[c];
+}
+
+// ---------------------------------------------------------------------------
+// two-pass EAM / ADP
+// ---------------------------------------------------------------------------
+struct EvalArgs {
+  int ntiles;
+  const int32_t *tile_cfg, *tile_a0, *tile_na;
+  const int64_t *atom_off, *pair_off, *row_off;
+  const int32_t *types;
+  const float4 *rec;
+  const uint32_t *bits;
+  const double *pd;
+  const double *r64;
+  const TableDesc *tabs;
+  int ntab, nivl;
+  const float4 *ivl32;
+  const double *tx, *ty, *ts;
+  int ntypes, npair;
+  const double *ref_force;
+  double *rho, *over, *Fp;
+  float *Fp32;
+  double *e_atom, *lim, *mu, *lamb;
+  double *forces, *residual, *tile_part;
+};
+
+template <int LPA, class T> __device__ inline T sub_sum(T v) {
+#pragma unroll
+  for (int m = LPA / 2; m > 0; m >>= 1) v += __shfl_xor(v, m, LPA);
+  return v;
+}
+
+__device__ inline void stage_tables(const EvalArgs &a, TableDesc *sT, float4 *sI) {
+  const float4 *src = reinterpret_cast<const float4 *>(a.tabs);
+  float4 *dst = reinterpret_cast<float4 *>(sT);
+  const int nq = a.ntab * int(sizeof(TableDesc) / sizeof(float4));
+  for (int k = threadIdx.x; k < nq; k += blockDim.x) dst[k] = src[k];
+  for (int k = threadIdx.x; k < a.nivl; k += blockDim.x) sI[k] = a.ivl32[k];
+  __syncthreads();
+}
+
+// table lookups -------------------------------------------------------------
+struct Tabs {
+  const TableDesc *T;
+  const float4 *I;
+  const double *x, *y, *s;
+  __device__ void eval64(int t, double r, double &v, double &d) const {
+    const TableDesc D = T[t];
+    spl_site64(D, x + D.off, y + D.off, s + D.off, r, v, d);
+  }
+  __device__ void eval32(int t, float r, float &v, float &d) const {
+    const TableDesc D = T[t];
+    float tt;
+    const int k = spl_interval32(D, r, tt);
+    spl_eval32(D, I[D.ivl + k], tt, v, d);
+  }
+  __device__ void deriv32(int t, float r, float &d) const {
+    const TableDesc D = T[t];
+    float tt;
+    const int k = spl_interval32(D, r, tt);
+    spl_deriv32(D, I[D.ivl + k], tt, d);
+  }
+};
+
+template <class R> struct Pair {
+  uint32_t b;
+  R d0, d1, d2, r;
+};
+
+template <bool F64> __device__ inline Pair<double> load_pair64(const EvalArgs &a, int64_t p) {
+  Pair<double> q;
+  q.b = a.bits[p];
+  q.d0 = a.pd[3 * p];
+  q.d1 = a.pd[3 * p + 1];
+  q.d2 = a.pd[3 * p + 2];
+  double r2 = q.d0 * q.d0 + q.d1 * q.d1;
+  r2 = r2 + q.d2 * q.d2;
+  q.r = sqrt(r2);
+  return q;
+}
+
+// --- pass 1: density (+ ADP moments) ---------------------------------------
+template <class R> struct DensAcc {
+  R rho = 0, m0 = 0, m1 = 0, m2 = 0, l[6] = {0, 0, 0, 0, 0, 0};
+};
+
+template <int KIND, bool EXACT, class R>
+__device__ inline void density_pair(const Tabs &tb, const EvalArgs &a, int ti, uint32_t b, R d0, R d1,
+                                    R d2, R r, DensAcc<R> &acc) {
+  if (KIND == FS_ADP && (b & kFlagTiny)) return;
+  const int tj = pj_type(b);
+  if (b & kFlagGj) {
+    R v, d;
+    if (EXACT) {
+      double vv, dd;
+      tb.eval64(a.npair + tj, double(r), vv, dd);
+      v = R(vv);
+    } else {
+      float vv, dd;
+      tb.eval32(a.npair + tj, float(r), vv, dd);
+      v = R(vv);
+    }
+    acc.rho += v;
+    (void)d;
+  }
+  if (KIND == FS_ADP) {
+    const int o = ord_pair(ti, tj, a.ntypes);
+    if (b & kFlagDip) {
+      R u;
+      if (EXACT) {
+        double vv, dd;
+        tb.eval64(a.npair + 2 * a.ntypes + o, double(r), vv, dd);
+        u = R(vv);
+      } else {
+        float vv, dd;
+        tb.eval32(a.npair + 2 * a.ntypes + o, float(r), vv, dd);
+        u = R(vv);
+      }
+      acc.m0 += u * d0;
+      acc.m1 += u * d1;
+      acc.m2 += u * d2;
+    }
+    if (b & kFlagQuad) {
+      R w;
+      if (EXACT) {
+        double vv, dd;
+        tb.eval64(2 * a.npair + 2 * a.ntypes + o, double(r), vv, dd);
+        w = R(vv);
+      } else {
+        float vv, dd;
+        tb.eval32(2 * a.npair + 2 * a.ntypes + o, float(r), vv, dd);
+        w = R(vv);
+      }
+      acc.l[0] += w * (d0 * d0);
+      acc.l[1] += w * (d1 * d1);
+      acc.l[2] += w * (d2 * d2);
+      acc.l[3] += w * (d0 * d1);
+      acc.l[4] += w * (d0 * d2);
+      acc.l[5] += w * (d1 * d2);
+    }
+  }
+}
+
+template <int KIND, bool F64, int LPA>
+__global__ __launch_bounds__(256) void k_density(EvalArgs a) {
+  // Densities (and ADP moments) feed the energy rows directly, whose 1e-6
+  // cost bar is dominated by |E| >> |E - E_ref|: the per-pair values are
+  // evaluated in FP64 in both precision modes, from the FP64 distance.
+  extern __shared__ float4 smem[];
+  TableDesc *sT = reinterpret_cast<TableDesc *>(smem);
+  float4 *sI = smem + a.ntab * int(sizeof(TableDesc) / sizeof(float4));
+  stage_tables(a, sT, sI);
+  const Tabs tb{sT, sI, a.tx, a.ty, a.ts};
+  const int sub = threadIdx.x / LPA, lane = threadIdx.x % LPA;
+  for (int tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
+    if (sub >= a.tile_na[tile]) continue;
+    const int64_t i = int64_t(a.tile_a0[tile]) + sub;
+    const int ti = a.types[i];
+    const int64_t p0 = a.pair_off[i], p1 = a.pair_off[i + 1];
+    DensAcc<double> acc;
+    for (int64_t p = p0 + lane; p < p1; p += LPA) {
+      if constexpr (KIND == FS_ADP) {
+        const Pair<double> q = load_pair64<true>(a, p);
+        density_pair<KIND, true, double>(tb, a, ti, q.b, q.d0, q.d1, q.d2, q.r, acc);
+      } else {
+        const uint32_t b = a.bits[p];
+        if (b & kFlagGj) {
+          double v, d;
+          tb.eval64(a.npair + pj_type(b), a.r64[p], v, d);
+          acc.rho += v;
+        }
+      }
+    }
+    acc.rho = sub_sum<LPA>(acc.rho);
+    if (KIND == FS_ADP) {
+      acc.m0 = sub_sum<LPA>(acc.m0);
+      acc.m1 = sub_sum<LPA>(acc.m1);
+      acc.m2 = sub_sum<LPA>(acc.m2);
+      for (int k = 0; k < 6; ++k) acc.l[k] = sub_sum<LPA>(acc.l[k]);
+    }
+    if (lane == 0) {
+      // clamp_embedding + F(rho) in FP64 (kernels.hpp:19-31, eam_force.cpp:68-73)
+      const TableDesc Te = sT[a.npair + a.ntypes + ti];
+      double rh = double(acc.rho), lim = 0.0, over = 0.0;
+      if (rh > Te.xn) {
+        const double dd = rh - Te.xn;
+        lim += kPenalty * dd * dd;
+        over = 2.0 * kPenalty * dd;
+        rh = Te.xn;
+      } else if (rh < Te.x0) {
+        const double dd = Te.x0 - rh;
+        lim += kPenalty * dd * dd;
+        over = -2.0 * kPenalty * dd;
+        rh = Te.x0;
+      }
+      double F, dF;
+      spl_uncached64(Te, a.tx + Te.off, a.ty + Te.off, a.ts + Te.off, rh, F, dF);
+      double e = F;
+      if (KIND == FS_ADP) {
+        const double m0 = acc.m0, m1 = acc.m1, m2 = acc.m2;
+        e += 0.5 * (m0 * m0 + m1 * m1 + m2 * m2);
+        const double L0 = acc.l[0], L1 = acc.l[1], L2 = acc.l[2], L3 = acc.l[3], L4 = acc.l[4],
+                     L5 = acc.l[5];
+        const double tr = L0 + L1 + L2;
+        const double fro = L0 * L0 + L1 * L1 + L2 * L2 + 2.0 * (L3 * L3 + L4 * L4 + L5 * L5);
+        e += 0.5 * (fro - tr * tr / 3.0);
+        a.mu[3 * i] = m0;
+        a.mu[3 * i + 1] = m1;
+        a.mu[3 * i + 2] = m2;
+        for (int k = 0; k < 6; ++k) a.lamb[6 * i + k] = double(acc.l[k]);
+      }
+      a.rho[i] = rh;
+      a.over[i] = over;
+      a.Fp[i] = dF;
+      a.Fp32[i] = float(dF);
+      a.e_atom[i] = e;
+      a.lim[i] = lim;
+    }
+  }
+}
+
+// --- pass 2: forces, pair energy, virial -------------------------------------
+template <class R> struct AtomState {
+  R Fp;
+  R mu[3];
+  R l[6];
+};
+
+template <class R> __device__ inline void quad_terms(const R *L, R d0, R d1, R d2, R &nu, R xi[3]) {
+  // L = (xx, yy, zz, xy, xz, yz); quad_nu / quad_xi (adp_force.hpp:63-68)
+  const R Md0 = L[0] * d0 + L[3] * d1 + L[4] * d2;
+  const R Md1 = L[3] * d0 + L[1] * d1 + L[5] * d2;
+  const R Md2 = L[4] * d0 + L[5] * d1 + L[2] * d2;
+  const R tr = L[0] + L[1] + L[2];
+  const R dd = d0 * d0 + d1 * d1 + d2 * d2;
+  nu += (d0 * Md0 + d1 * Md1 + d2 * Md2) - dd / R(3) * tr;
+  const R t3 = tr / R(3);
+  xi[0] += Md0 - t3 * d0;
+  xi[1] += Md1 - t3 * d1;
+  xi[2] += Md2 - t3 * d2;
+}
+
+template <class R> struct ForceAcc {
+  R f0 = 0, f1 = 0, f2 = 0;
+  double ep = 0.0;
+  double sg[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+template <int KIND, bool EXACT, class R>
+__device__ inline void force_pair(const Tabs &tb, const EvalArgs &a, int ti, int64_t cbase,
+                                  const AtomState<R> &si, uint32_t b, R d0, R d1, R d2, R r,
+                                  ForceAcc<R> &acc, double r64 = 0.0) {
+  if (b & kFlagTiny) return;
+  const int tj = pj_type(b);
+  const int64_t jg = cbase + pj_local(b);
+  const R inv_r = R(1) / r;
+  R phi = 0, dphi = 0, dgj = 0, dgi = 0;
+  const int o = ord_pair(ti, tj, a.ntypes);
+  auto ev = [&](int t, R &v, R &d) {
+    if (EXACT) {
+      double vv, dd;
+      tb.eval64(t, double(r), vv, dd);
+      v = R(vv);
+      d = R(dd);
+    } else {
+      float vv, dd;
+      tb.eval32(t, float(r), vv, dd);
+      v = R(vv);
+      d = R(dd);
+    }
+  };
+  R tmp;
+  if (b & kFlagPhi) ev(o, phi, dphi);
+  if (b & kFlagGj) ev(a.npair + tj, tmp, dgj);
+  if (b & kFlagGi) ev(a.npair + ti, tmp, dgi);
+  const R Fpj = EXACT ? R(a.Fp[jg]) : R(a.Fp32[jg]);
+  const R fs = (dphi + si.Fp * dgj + Fpj * dgi) * inv_r;
+  R f0 = fs * d0, f1 = fs * d1, f2 = fs * d2;
+  if (KIND == FS_ADP) {
+    R muj[3], lj[6];
+    for (int k = 0; k < 3; ++k) muj[k] = R(a.mu[3 * jg + k]);
+    for (int k = 0; k < 6; ++k) lj[k] = R(a.lamb[6 * jg + k]);
+    if (b & kFlagDip) {
+      R u, du;
+      ev(a.npair + 2 * a.ntypes + o, u, du);
+      const R doti = si.mu[0] * d0 + si.mu[1] * d1 + si.mu[2] * d2;
+      const R dotj = muj[0] * d0 + muj[1] * d1 + muj[2] * d2;
+      const R sd = du * inv_r * (doti - dotj);
+      f0 += sd * d0 + u * (si.mu[0] - muj[0]);
+      f1 += sd * d1 + u * (si.mu[1] - muj[1]);
+      f2 += sd * d2 + u * (si.mu[2] - muj[2]);
+    }
+    if (b & kFlagQuad) {
+      R w, dw;
+      ev(2 * a.npair + 2 * a.ntypes + o, w, dw);
+      R nu = 0, xi[3] = {0, 0, 0};
+      quad_terms(si.l, d0, d1, d2, nu, xi);
+      quad_terms(lj, d0, d1, d2, nu, xi);
+      const R sq = dw * inv_r * nu;
+      f0 += sq * d0 + R(2) * w * xi[0];
+      f1 += sq * d1 + R(2) * w * xi[1];
+      f2 += sq * d2 + R(2) * w * xi[2];
+    }
+  }
+  acc.f0 += f0;
+  acc.f1 += f1;
+  acc.f2 += f2;
+  if (!EXACT && (b & kFlagPhi)) {
+    // pair energy value in FP64 from the FP64 distance (see k_density)
+    double pv, pd;
+    tb.eval64(o, r64, pv, pd);
+    acc.ep += 0.5 * pv;
+  } else {
+    acc.ep += 0.5 * double(phi);
+  }
+  const double hd[3] = {0.5 * double(d0), 0.5 * double(d1), 0.5 * double(d2)};
+  const double fd[3] = {double(f0), double(f1), double(f2)};
+#pragma unroll
+  for (int p = 0; p < 3; ++p)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) acc.sg[3 * p + q] -= hd[p] * fd[q];
+}
+
+constexpr int kPart = 11; // energy, limit, 9 virial components
+
+template <int KIND, bool F64, int LPA>
+__global__ __launch_bounds__(256) void k_force(EvalArgs a) {
+  using R = typename std::conditional<F64, double, float>::type;
+  constexpr int APB = 256 / LPA;
+  extern __shared__ float4 smem[];
+  TableDesc *sT = reinterpret_cast<TableDesc *>(smem);
+  float4 *sI = smem + a.ntab * int(sizeof(TableDesc) / sizeof(float4));
+  __shared__ double sRed[APB][kPart];
+  stage_tables(a, sT, sI);
+  const Tabs tb{sT, sI, a.tx, a.ty, a.ts};
+  const int sub = threadIdx.x / LPA, lane = threadIdx.x % LPA;
+  for (int tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
+    const int na = a.tile_na[tile];
+    if (sub < na) {
+      const int64_t i = int64_t(a.tile_a0[tile]) + sub;
+      const int c = a.tile_cfg[tile];
+      const int64_t cbase = a.atom_off[c];
+      const int ti = a.types[i];
+      AtomState<R> si;
+      si.Fp = F64 ? R(a.Fp[i]) : R(a.Fp32[i]);
+      if (KIND == FS_ADP) {
+        for (int k = 0; k < 3; ++k) si.mu[k] = R(a.mu[3 * i + k]);
+        for (int k = 0; k < 6; ++k) si.l[k] = R(a.lamb[6 * i + k]);
+      }
+      ForceAcc<R> acc;
+      const int64_t p0 = a.pair_off[i], p1 = a.pair_off[i + 1];
+      for (int64_t p = p0 + lane; p < p1; p += LPA) {
+        if constexpr (F64) {
+          const Pair<double> q = load_pair64<true>(a, p);
+          force_pair<KIND, true, double>(tb, a, ti, cbase, si, q.b, q.d0, q.d1, q.d2, q.r, acc);
+        } else {
+          const float4 q = a.rec[p];
+          const uint32_t b = __float_as_uint(q.w);
+          if (b & kFlagEdge) {
+            const Pair<double> e = load_pair64<true>(a, p);
+            AtomState<double> sd;
+            sd.Fp = a.Fp[i];
+            if (KIND == FS_ADP) {
+              for (int k = 0; k < 3; ++k) sd.mu[k] = a.mu[3 * i + k];
+              for (int k = 0; k < 6; ++k) sd.l[k] = a.lamb[6 * i + k];
+            }
+            ForceAcc<double> t;
+            force_pair<KIND, true, double>(tb, a, ti, cbase, sd, e.b, e.d0, e.d1, e.d2, e.r, t);
+            acc.f0 += R(t.f0);
+            acc.f1 += R(t.f1);
+            acc.f2 += R(t.f2);
+            acc.ep += t.ep;
+            for (int k = 0; k < 9; ++k) acc.sg[k] += t.sg[k];
+          } else {
+            const float r = sqrtf(q.x * q.x + q.y * q.y + q.z * q.z);
+            force_pair<KIND, false, float>(tb, a, ti, cbase, si, b, q.x, q.y, q.z, r, acc, a.r64[p]);
+          }
+        }
+      }
+      acc.f0 = sub_sum<LPA>(acc.f0);
+      acc.f1 = sub_sum<LPA>(acc.f1);
+      acc.f2 = sub_sum<LPA>(acc.f2);
+      acc.ep = sub_sum<LPA>(acc.ep);
+      for (int k = 0; k < 9; ++k) acc.sg[k] = sub_sum<LPA>(acc.sg[k]);
+      if (lane == 0) {
+        const double F[3] = {double(acc.f0), double(acc.f1), double(acc.f2)};
+        const int64_t row = a.row_off[c] + 3 * (i - cbase);
+        for (int k = 0; k < 3; ++k) {
+          a.forces[3 * i + k] = F[k];
+          a.residual[row + k] = F[k] - a.ref_force[3 * i + k];
+        }
+        sRed[sub][0] = a.e_atom[i] + acc.ep;
+        sRed[sub][1] = a.lim[i];
+        for (int k = 0; k < 9; ++k) sRed[sub][2 + k] = acc.sg[k];
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x < kPart) {
+      double s = 0.0;
+      for (int q = 0; q < na && q < APB; ++q) s += sRed[q][threadIdx.x];
+      a.tile_part[int64_t(tile) * kPart + threadIdx.x] = s;
+    }
+    __syncthreads();
+  }
+}
+
+// --- per-configuration finalisation ------------------------------------------
+__global__ void k_finalize(int C, const int32_t *__restrict__ cfg_tile0,
+                           const double *__restrict__ part, const double *__restrict__ vol,
+                           const int64_t *__restrict__ row_off,
+                           const int64_t *__restrict__ atom_off,
+                           const double *__restrict__ ref_energy,
+                           const double *__restrict__ ref_stress, double ew, double sw,
+                           double *__restrict__ residual, double *__restrict__ E,
+                           double *__restrict__ S, double *__restrict__ L) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= C) return;
+  double acc[kPart];
+  for (int k = 0; k < kPart; ++k) acc[k] = 0.0;
+  for (int t = cfg_tile0[c]; t < cfg_tile0[c + 1]; ++t)
+    for (int k = 0; k < kPart; ++k) acc[k] += part[int64_t(t) * kPart + k];
+  const double V = vol[c];
+  double sig[9];
+  for (int k = 0; k < 9; ++k) sig[k] = acc[2 + k] / V;
+  int64_t row = row_off[c] + 3 * (atom_off[c + 1] - atom_off[c]);
+  residual[row++] = ew * (acc[0] - ref_energy[c]);
+  if (sw > 0.0) {
+    const int vo[6] = {0, 4, 8, 1, 2, 5};
+    for (int v = 0; v < 6; ++v) residual[row++] = sw * (sig[vo[v]] - ref_stress[6 * c + v]);
+  }
+  residual[row] = acc[1];
+  E[c] = acc[0];
+  L[c] = acc[1];
+  for (int k = 0; k < 9; ++k) S[9 * c + k] = sig[k];
+}
+
+// --- |F|^2, deterministic two-level sum ---------------------------------------
+__global__ void k_sumsq(int64_t n, const double *__restrict__ v, double *__restrict__ part) {
+  __shared__ double s[256];
+  double acc = 0.0;
+  for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < n;
+       k += int64_t(gridDim.x) * blockDim.x)
+    acc += v[k] * v[k];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+__global__ void k_sum_final(int n, const double *__restrict__ part, double *__restrict__ out) {
+  __shared__ double s[256];
+  double acc = 0.0;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) acc += part[k];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = s[0];
+}
+
+EvalArgs make_args(Context &c) {
+  EvalArgs a;
+  a.ntiles = c.ntiles;
+  a.tile_cfg = c.tile_cfg.p;
+  a.tile_a0 = c.tile_a0.p;
+  a.tile_na = c.tile_na.p;
+  a.atom_off = c.atom_off.p;
+  a.pair_off = c.pair_off.p;
+  a.row_off = c.row_off.p;
+  a.types = c.types.p;
+  a.rec = c.pair_rec.p;
+  a.bits = c.pair_bits.p;
+  a.pd = c.pair_d.p;
+  a.r64 = c.pair_r64.p;
+  a.tabs = c.tabs.p;
+  a.ntab = c.ntab;
+  a.nivl = c.nivl;
+  a.ivl32 = c.ivl32.p;
+  a.tx = c.tab_x.p;
+  a.ty = c.tab_y.p;
+  a.ts = c.tab_s.p;
+  a.ntypes = c.ntypes;
+  a.npair = c.npair;
+  a.ref_force = c.ref_force.p;
+  a.rho = c.rho.p;
+  a.over = c.clamp_over.p;
+  a.Fp = c.Fp.p;
+  a.Fp32 = c.Fp32.p;
+  a.e_atom = c.e_atom.p;
+  a.lim = c.lim_atom.p;
+  a.mu = c.mu.p;
+  a.lamb = c.lamb.p;
+  a.forces = c.forces.p;
+  a.residual = c.residual.p;
+  a.tile_part = c.tile_part.p;
+  return a;
+}
+
+template <int KIND, bool F64, int LPA> void run_eval(Context &c, const EvalArgs &a, size_t lds,
+                                                      int grid) {
+  hipLaunchKernelGGL((k_density<KIND, F64, LPA>), dim3(grid), dim3(256), lds, c.stream, a);
+  FS_HIP(hipGetLastError());
+  hipLaunchKernelGGL((k_force<KIND, F64, LPA>), dim3(grid), dim3(256), lds, c.stream, a);
+  FS_HIP(hipGetLastError());
+}
+
+template <int KIND, bool F64> void dispatch_lpa(Context &c, const EvalArgs &a, size_t lds, int grid) {
+  switch (c.lanes_per_atom) {
+  case 4: run_eval<KIND, F64, 4>(c, a, lds, grid); break;
+  case 8: run_eval<KIND, F64, 8>(c, a, lds, grid); break;
+  case 16: run_eval<KIND, F64, 16>(c, a, lds, grid); break;
+  case 32: run_eval<KIND, F64, 32>(c, a, lds, grid); break;
+  default: run_eval<KIND, F64, 64>(c, a, lds, grid); break;
+  }
+}
+
+} // namespace
+
+void launch_scatter_params(Context &c, const double *d_x) {
+  if (c.D > 0)
+    hipLaunchKernelGGL(k_scatter, dim3((c.D + 255) / 256), dim3(256), 0, c.stream, c.D, d_x,
+                       c.knot_of.p, c.tab_y.p);
+  FS_HIP(hipGetLastError());
+}
+
+void launch_slopes(Context &c) {
+  if (c.kind == FS_SW || c.ntab == 0) return;
+  hipLaunchKernelGGL(k_slopes, dim3((c.ntab + 63) / 64), dim3(64), 0, c.stream, c.ntab, c.tabs.p,
+                     c.tab_x.p, c.tab_y.p, c.tab_s.p, c.tab_S.p, c.ivl32.p);
+  FS_HIP(hipGetLastError());
+}
+
+void launch_eval(Context &c, bool) {
+  if (c.kind == FS_SW) {
+    launch_sw_eval(c);
+  } else {
+    const EvalArgs a = make_args(c);
+    const size_t lds = size_t(c.ntab) * sizeof(TableDesc) + size_t(c.nivl) * sizeof(float4);
+    const int grid = std::max(1, std::min(c.ntiles, c.blocks_per_cu * c.cu_count));
+    if (c.ntiles > 0) {
+      const bool f64 = c.precision == FS_FP64;
+      if (c.kind == FS_EAM) {
+        if (f64) dispatch_lpa<FS_EAM, true>(c, a, lds, grid);
+        else dispatch_lpa<FS_EAM, false>(c, a, lds, grid);
+      } else {
+        if (f64) dispatch_lpa<FS_ADP, true>(c, a, lds, grid);
+        else dispatch_lpa<FS_ADP, false>(c, a, lds, grid);
+      }
+    }
+  }
+  hipLaunchKernelGGL(k_finalize, dim3((c.C + 127) / 128), dim3(128), 0, c.stream, c.C,
+                     c.cfg_tile0.p, c.tile_part.p, c.vol.p, c.row_off.p, c.atom_off.p,
+                     c.ref_energy.p, c.ref_stress.p, c.energy_weight, c.stress_weight,
+                     c.residual.p, c.cfg_E.p, c.cfg_S.p, c.cfg_L.p);
+  FS_HIP(hipGetLastError());
+}
+
+void launch_cost(Context &c, double *d_out) {
+  const int nb = 512;
+  c.cost_part.alloc(nb);
+  hipLaunchKernelGGL(k_sumsq, dim3(nb), dim3(256), 0, c.stream, c.R, c.residual.p, c.cost_part.p);
+  FS_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, c.stream, nb, c.cost_part.p, d_out);
+  FS_HIP(hipGetLastError());
+}
+
+} // namespace fsa

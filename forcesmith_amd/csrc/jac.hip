@@ -1,0 +1,920 @@
+// Analytic Jacobian dF/dx on the GPU.
+//
+// Tabulated EAM: per-knot columns through the makima slope chain. The chain
+// rule is the one of the reference's analytic-table Jacobian
+// (EAMForceCalculator::write_param_jacobian, src/force/eam_force.cpp:108-279;
+// add_force_column / add_radial_bond, include/forcesmith/force/
+// param_jacobian.hpp:91-121) with spline partials in place of param_grad /
+// dderiv_dparam:
+//   d(value, deriv)(r)/dy_m = b0[m==k] + b1[m==k+1] + bs0 dS_k/dy_m + bs1 dS_k+1/dy_m
+// where b are the Hermite basis coefficients of the interval k holding r and
+// dS/dy is the band of the makima slope jacobian refreshed with the knots.
+// The reference itself has no such columns for tabulated tables: it falls
+// back to central differences (fd_jacobian.hpp:37-49).
+//
+// Kernels:
+//   k_jac_dens   one wave per atom: H_i[c] = d rho_i / d y_c over the density
+//                columns, Hs_i = F''(rho_i) H_i (0 when the density is clamped),
+//                and the embedding window of atom i.
+//   k_jac_main   one wave per atom, lanes own columns: phase A stages one
+//                record per neighbour pair in LDS (64 pairs at a time), phase B
+//                walks the records and accumulates every column's force and
+//                virial partials; FP32 force rows are written, energy / virial
+//                / limit partials are reduced per tile in FP64.
+//   k_jac_rows   per configuration: energy, stress and limit rows from the
+//                tile partials, in tile order (deterministic, no atomics).
+// Stillinger-Weber: forward-mode tangents of the reference's pair and triplet
+// formulas (stiweb_force.cpp:24-54, kernels.hpp:40-68), one column per grid
+// row, with the triplet j/k forces gathered through the reverse pair index.
+#include "common.hpp"
+#include "spline.hpp"
+
+#include <algorithm>
+
+namespace fsa {
+
+namespace {
+
+constexpr int kCPL = 4;        // columns per lane (D <= 256)
+constexpr int kJacTile = 64;   // atoms per Jacobian tile
+constexpr int kJP = 8;         // partial rows: energy, 6 virial, limit
+enum { CK_PAIR = 0, CK_DENS = 1, CK_EMB = 2, CK_DIP = 3, CK_QUAD = 4, CK_NONE = 7 };
+
+struct ColDesc {
+  int32_t tab;  // table
+  int32_t m;    // knot index within the table
+  int32_t kind; // CK_*
+  int32_t dc;   // density-column index (kind == CK_DENS) or -1
+};
+
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <class T> __device__ inline T wave_sum(T v) {
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
+
+// S band of table T at (k, m): d s_k / d y_m, zero outside |m - k| <= 2
+__device__ inline float sband(const float *sS, const TableDesc &T, int k, int m) {
+  const int q = m - k + 2;
+  return (q >= 0 && q < 5 && k >= 0 && k < T.n) ? sS[(T.off + k) * 5 + q] : 0.0f;
+}
+
+__device__ inline float band(const float *sS, const TableDesc &T, int k, int m, const float c[4]) {
+  float v = c[2] * sband(sS, T, k, m) + c[3] * sband(sS, T, k + 1, m);
+  if (m == k) v += c[0];
+  if (m == k + 1) v += c[1];
+  return v;
+}
+
+// interval + value / derivative basis of a pair table at r (cached-site
+// semantics: linear site only at the span ends or for n < 4 tables)
+__device__ inline int pair_basis(const TableDesc &T, float r, float bv[4], float bd[4]) {
+  float t;
+  const int k = spl_interval32(T, r, t);
+  if (T.linear) {
+    bv[0] = 1.0f - t;
+    bv[1] = t;
+    bv[2] = bv[3] = 0.0f;
+    bd[0] = -T.inv_hf;
+    bd[1] = T.inv_hf;
+    bd[2] = bd[3] = 0.0f;
+    return k;
+  }
+  const float h = T.hf, omt = 1.0f - t;
+  bv[0] = omt * omt * (1.0f + 2.0f * t);
+  bv[1] = t * t * (3.0f - 2.0f * t);
+  bv[2] = omt * omt * (t * h);
+  bv[3] = t * t * (h * (t - 1.0f));
+  bd[0] = -6.0f * t * omt * T.inv_hf;
+  bd[1] = 6.0f * t * omt * T.inv_hf;
+  bd[2] = omt * (1.0f - 3.0f * t);
+  bd[3] = t * (3.0f * t - 2.0f);
+  return k;
+}
+
+struct JacArgs {
+  int64_t N;
+  int C, D, Dd, dens_c0;
+  int ntypes, npair, ntab;
+  int ntiles;
+  const int32_t *tile_cfg, *tile_a0, *tile_na, *cfg_tile0;
+  const int64_t *atom_off, *pair_off, *row_off;
+  const int32_t *types;
+  const float4 *rec;
+  const uint32_t *bits;
+  const TableDesc *tabs;
+  const double *tx, *ty, *ts, *tS;
+  const float4 *ivl32;
+  const ColDesc *cols;
+  const double *rho, *over, *Fp;
+  const float *Fp32;
+  float *H, *Hs;          // [N x Dd]
+  int32_t *emb_k;         // [N]
+  float *emb_d, *emb_v;   // [N x 4]
+  double *Fpp;            // [N]
+  float *J;               // [R x D]
+  double *part;           // [ntiles x kJP x D]
+  const double *vol;
+  double ew, sw;
+};
+
+// LDS staging of descriptors and float S bands
+__device__ inline void stage(const JacArgs &a, TableDesc *sT, float *sS, int nk) {
+  const float4 *src = reinterpret_cast<const float4 *>(a.tabs);
+  float4 *dst = reinterpret_cast<float4 *>(sT);
+  const int nq = a.ntab * int(sizeof(TableDesc) / sizeof(float4));
+  for (int k = threadIdx.x; k < nq; k += blockDim.x) dst[k] = src[k];
+  for (int k = threadIdx.x; k < nk * 5; k += blockDim.x) sS[k] = float(a.tS[k]);
+  __syncthreads();
+}
+
+struct DRec {
+  int32_t tab, k;
+  float v[4];
+};
+
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_jac_dens(JacArgs a, int nknots) {
+  extern __shared__ float4 smem[];
+  TableDesc *sT = reinterpret_cast<TableDesc *>(smem);
+  float *sS = reinterpret_cast<float *>(smem + a.ntab * int(sizeof(TableDesc) / sizeof(float4)));
+  DRec *sR = reinterpret_cast<DRec *>(sS + ((nknots * 5 + 3) & ~3));
+  stage(a, sT, sS, nknots);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  DRec *myR = sR + wave * 64;
+  const int64_t gw = int64_t(blockIdx.x) * 4 + wave;
+  const int64_t nw = int64_t(gridDim.x) * 4;
+  // my density columns
+  int ctab[kCPL], cm[kCPL];
+  for (int s = 0; s < kCPL; ++s) {
+    const int dc = lane + 64 * s;
+    if (dc < a.Dd) {
+      const ColDesc cd = a.cols[a.dens_c0 + dc];
+      ctab[s] = cd.tab;
+      cm[s] = cd.m;
+    } else {
+      ctab[s] = -1;
+      cm[s] = 0;
+    }
+  }
+  for (int64_t i = gw; i < a.N; i += nw) {
+    const int64_t p0 = a.pair_off[i], p1 = a.pair_off[i + 1];
+    float H[kCPL] = {0, 0, 0, 0};
+    for (int64_t pc = p0; pc < p1; pc += 64) {
+      const int64_t p = pc + lane;
+      DRec r;
+      r.tab = -1;
+      r.k = 0;
+      if (p < p1) {
+        const float4 q = a.rec[p];
+        const uint32_t b = __float_as_uint(q.w);
+        if (b & kFlagGj) {
+          const float rr = sqrtf(q.x * q.x + q.y * q.y + q.z * q.z);
+          r.tab = a.npair + pj_type(b);
+          float bd[4];
+          r.k = pair_basis(sT[r.tab], rr, r.v, bd);
+        }
+      }
+      myR[lane] = r;
+      wave_sync();
+      const int np = int(min<int64_t>(64, p1 - pc));
+      for (int q = 0; q < np; ++q) {
+        const DRec r2 = myR[q];
+        if (r2.tab < 0) continue;
+        const TableDesc T = sT[r2.tab];
+#pragma unroll
+        for (int s = 0; s < kCPL; ++s)
+          if (ctab[s] == r2.tab) H[s] += band(sS, T, r2.k, cm[s], r2.v);
+      }
+      wave_sync();
+    }
+    // embedding at the clamped density (lane 0), broadcast F'' and the clamp
+    const int ti = a.types[i];
+    const TableDesc Te = sT[a.npair + a.ntypes + ti];
+    const double rb = a.rho[i];
+    const bool clamped = a.over[i] != 0.0;
+    double F2 = 0.0;
+    if (lane == 0) {
+      const double *x = a.tx + Te.off, *y = a.ty + Te.off, *s = a.ts + Te.off;
+      const int n = Te.n;
+      int k;
+      bool lin;
+      if (rb <= x[0]) {
+        k = 0;
+        lin = true;
+      } else if (rb >= x[n - 1]) {
+        k = n - 2;
+        lin = true;
+      } else {
+        k = spl_locate(x, n, rb);
+        lin = Te.linear != 0;
+      }
+      const double dx = x[k + 1] - x[k];
+      float dcoef[4], vcoef[4];
+      if (lin) {
+        const double anchor = (rb >= x[n - 1] && rb > x[0]) ? x[n - 1] : x[k];
+        const double f = (rb - anchor) / dx;
+        if (rb >= x[n - 1] && rb > x[0]) {
+          vcoef[0] = float(-f);
+          vcoef[1] = float(1.0 + f);
+        } else {
+          vcoef[0] = float(1.0 - f);
+          vcoef[1] = float(f);
+        }
+        vcoef[2] = vcoef[3] = 0.0f;
+        dcoef[0] = float(-1.0 / dx);
+        dcoef[1] = float(1.0 / dx);
+        dcoef[2] = dcoef[3] = 0.0f;
+        F2 = 0.0;
+      } else {
+        double bv[4], bd[4], b2[4];
+        herm_basis64(x[k], dx, rb, bv, bd);
+        herm_basis2_64(x[k], dx, rb, b2);
+        for (int q = 0; q < 4; ++q) {
+          vcoef[q] = float(bv[q]);
+          dcoef[q] = float(bd[q]);
+        }
+        F2 = b2[0] * y[k] + b2[1] * y[k + 1] + b2[2] * s[k] + b2[3] * s[k + 1];
+      }
+      a.emb_k[i] = k;
+      for (int q = 0; q < 4; ++q) {
+        a.emb_d[4 * i + q] = dcoef[q];
+        a.emb_v[4 * i + q] = vcoef[q];
+      }
+      a.Fpp[i] = F2;
+    }
+    F2 = __shfl(F2, 0, 64);
+    const float hs = clamped ? 0.0f : float(F2);
+#pragma unroll
+    for (int s = 0; s < kCPL; ++s) {
+      const int dc = lane + 64 * s;
+      if (dc < a.Dd) {
+        a.H[i * a.Dd + dc] = H[s];
+        a.Hs[i * a.Dd + dc] = hs * H[s];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+struct PRec {
+  int32_t jg, flags;
+  int32_t kphi, kgj, kgi, kfj;
+  float u[3], d[3];
+  float bphi[4], vphi[4], bgj[4], bgi[4], ebj[4];
+  float dgj, dgi, fpj, pad;
+};
+
+__global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots) {
+  extern __shared__ float4 smem[];
+  TableDesc *sT = reinterpret_cast<TableDesc *>(smem);
+  float *sS = reinterpret_cast<float *>(smem + a.ntab * int(sizeof(TableDesc) / sizeof(float4)));
+  PRec *sR = reinterpret_cast<PRec *>(sS + ((nknots * 5 + 3) & ~3));
+  __shared__ double sPart[kJP][256];
+  stage(a, sT, sS, nknots);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  PRec *myR = sR + wave * 64;
+  // lane-owned columns
+  int ccol[kCPL], ctab[kCPL], cm[kCPL], ckind[kCPL], cdc[kCPL];
+  for (int s = 0; s < kCPL; ++s) {
+    const int c = lane + 64 * s;
+    if (c < a.D) {
+      const ColDesc cd = a.cols[c];
+      ccol[s] = c;
+      ctab[s] = cd.tab;
+      cm[s] = cd.m;
+      ckind[s] = cd.kind;
+      cdc[s] = cd.dc;
+    } else {
+      ccol[s] = -1;
+      ctab[s] = -1;
+      cm[s] = 0;
+      ckind[s] = CK_NONE;
+      cdc[s] = -1;
+    }
+  }
+  for (int tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
+    const int na = a.tile_na[tile];
+    const int c = a.tile_cfg[tile];
+    const int64_t cbase = a.atom_off[c];
+    double tp[kCPL][kJP];
+#pragma unroll
+    for (int s = 0; s < kCPL; ++s)
+#pragma unroll
+      for (int q = 0; q < kJP; ++q) tp[s][q] = 0.0;
+    for (int li = wave; li < na; li += 4) {
+      const int64_t i = int64_t(a.tile_a0[tile]) + li;
+      const int ti = a.types[i];
+      const float Fpi = a.Fp32[i];
+      const int64_t p0 = a.pair_off[i], p1 = a.pair_off[i + 1];
+      float acf[kCPL][3], acs[kCPL][6], ace[kCPL];
+#pragma unroll
+      for (int s = 0; s < kCPL; ++s) {
+        acf[s][0] = acf[s][1] = acf[s][2] = 0.0f;
+        for (int q = 0; q < 6; ++q) acs[s][q] = 0.0f;
+        ace[s] = 0.0f;
+      }
+      float G[3] = {0, 0, 0}, Gs[6] = {0, 0, 0, 0, 0, 0};
+      for (int64_t pc = p0; pc < p1; pc += 64) {
+        const int64_t p = pc + lane;
+        PRec r;
+        r.flags = 0;
+        if (p < p1) {
+          const float4 q = a.rec[p];
+          const uint32_t b = __float_as_uint(q.w);
+          if (!(b & kFlagTiny)) {
+            const float rr = sqrtf(q.x * q.x + q.y * q.y + q.z * q.z);
+            const float inv_r = 1.0f / rr;
+            const int tj = pj_type(b);
+            r.flags = int32_t(b | 1u);
+            r.jg = int32_t(cbase + pj_local(b));
+            r.d[0] = q.x;
+            r.d[1] = q.y;
+            r.d[2] = q.z;
+            r.u[0] = q.x * inv_r;
+            r.u[1] = q.y * inv_r;
+            r.u[2] = q.z * inv_r;
+            float dummy[4];
+            r.kphi = (b & kFlagPhi) ? pair_basis(sT[ord_pair(ti, tj, a.ntypes)], rr, r.vphi, r.bphi) : -9;
+            r.dgj = 0.0f;
+            r.dgi = 0.0f;
+            r.kgj = -9;
+            r.kgi = -9;
+            if (b & kFlagGj) {
+              const TableDesc T = sT[a.npair + tj];
+              r.kgj = pair_basis(T, rr, dummy, r.bgj);
+              float tt, v;
+              const int k = spl_interval32(T, rr, tt);
+              spl_eval32(T, a.ivl32[T.ivl + k], tt, v, r.dgj);
+            }
+            if (b & kFlagGi) {
+              const TableDesc T = sT[a.npair + ti];
+              r.kgi = pair_basis(T, rr, dummy, r.bgi);
+              float tt, v;
+              const int k = spl_interval32(T, rr, tt);
+              spl_eval32(T, a.ivl32[T.ivl + k], tt, v, r.dgi);
+            }
+            r.fpj = a.Fp32[r.jg];
+            r.kfj = a.emb_k[r.jg];
+            for (int q2 = 0; q2 < 4; ++q2) r.ebj[q2] = a.emb_d[4 * int64_t(r.jg) + q2];
+            // G_i = sum_j g'_tj(r) u, virial counterpart sum_j g'_tj d_a u_b
+            G[0] += r.dgj * r.u[0];
+            G[1] += r.dgj * r.u[1];
+            G[2] += r.dgj * r.u[2];
+            Gs[0] += r.dgj * r.d[0] * r.u[0];
+            Gs[1] += r.dgj * r.d[1] * r.u[1];
+            Gs[2] += r.dgj * r.d[2] * r.u[2];
+            Gs[3] += r.dgj * r.d[0] * r.u[1];
+            Gs[4] += r.dgj * r.d[0] * r.u[2];
+            Gs[5] += r.dgj * r.d[1] * r.u[2];
+          }
+        }
+        myR[lane] = r;
+        wave_sync();
+        const int np = int(min<int64_t>(64, p1 - pc));
+        for (int q = 0; q < np; ++q) {
+          if (myR[q].flags == 0) continue;
+          const int tj = pj_type(uint32_t(myR[q].flags));
+          const int tphi = ord_pair(ti, tj, a.ntypes);
+          const int tgj = a.npair + tj, tgi = a.npair + ti;
+          const int temb = a.npair + a.ntypes + tj;
+          const int jg = myR[q].jg;
+          const float dgi = myR[q].dgi;
+#pragma unroll
+          for (int s = 0; s < kCPL; ++s) {
+            if (ckind[s] == CK_NONE) continue;
+            float contrib = 0.0f, en = 0.0f;
+            const int T = ctab[s];
+            if (ckind[s] == CK_PAIR) {
+              if (T == tphi && myR[q].kphi >= 0) {
+                const TableDesc D = sT[T];
+                contrib = band(sS, D, myR[q].kphi, cm[s], myR[q].bphi);
+                en = 0.5f * band(sS, D, myR[q].kphi, cm[s], myR[q].vphi);
+              }
+            } else if (ckind[s] == CK_DENS) {
+              const TableDesc D = sT[T];
+              if (T == tgj && myR[q].kgj >= 0) contrib += Fpi * band(sS, D, myR[q].kgj, cm[s], myR[q].bgj);
+              if (T == tgi && myR[q].kgi >= 0)
+                contrib += myR[q].fpj * band(sS, D, myR[q].kgi, cm[s], myR[q].bgi);
+              contrib += a.Hs[int64_t(jg) * a.Dd + cdc[s]] * dgi;
+            } else if (ckind[s] == CK_EMB) {
+              if (T == temb) contrib = band(sS, sT[T], myR[q].kfj, cm[s], myR[q].ebj) * dgi;
+            }
+            const float *u = myR[q].u;
+            const float *d = myR[q].d;
+            acf[s][0] += contrib * u[0];
+            acf[s][1] += contrib * u[1];
+            acf[s][2] += contrib * u[2];
+            acs[s][0] += contrib * d[0] * u[0];
+            acs[s][1] += contrib * d[1] * u[1];
+            acs[s][2] += contrib * d[2] * u[2];
+            acs[s][3] += contrib * d[0] * u[1];
+            acs[s][4] += contrib * d[0] * u[2];
+            acs[s][5] += contrib * d[1] * u[2];
+            ace[s] += en;
+          }
+        }
+        wave_sync();
+      }
+      for (int q = 0; q < 3; ++q) G[q] = wave_sum(G[q]);
+      for (int q = 0; q < 6; ++q) Gs[q] = wave_sum(Gs[q]);
+      // per-atom terms
+      const bool clamped = a.over[i] != 0.0;
+      const double Fpi64 = a.Fp[i], over = a.over[i];
+      const int kfi = a.emb_k[i];
+      float ebi[4], evi[4];
+      for (int q = 0; q < 4; ++q) {
+        ebi[q] = a.emb_d[4 * i + q];
+        evi[q] = a.emb_v[4 * i + q];
+      }
+      const int temb_i = a.npair + a.ntypes + ti;
+      const int64_t row = a.row_off[c] + 3 * (i - cbase);
+#pragma unroll
+      for (int s = 0; s < kCPL; ++s) {
+        if (ckind[s] == CK_NONE) continue;
+        float g = 0.0f;
+        double e_extra = 0.0, l_extra = 0.0;
+        if (ckind[s] == CK_DENS) {
+          g = a.Hs[i * a.Dd + cdc[s]];
+          const double h = a.H[i * a.Dd + cdc[s]];
+          if (clamped) l_extra = over * h;
+          else e_extra = Fpi64 * h;
+        } else if (ckind[s] == CK_EMB && ctab[s] == temb_i) {
+          const TableDesc D = sT[ctab[s]];
+          g = band(sS, D, kfi, cm[s], ebi);
+          e_extra = band(sS, D, kfi, cm[s], evi);
+        }
+        for (int q = 0; q < 3; ++q) acf[s][q] += g * G[q];
+        for (int q = 0; q < 6; ++q) acs[s][q] += g * Gs[q];
+        const int col = ccol[s];
+        for (int q = 0; q < 3; ++q) a.J[(row + q) * a.D + col] = acf[s][q];
+        tp[s][0] += double(ace[s]) + e_extra;
+        for (int q = 0; q < 6; ++q) tp[s][1 + q] += double(acs[s][q]);
+        tp[s][7] += l_extra;
+      }
+    }
+    // reduce the four waves' partials in wave order, write the tile partial
+    for (int w = 0; w < 4; ++w) {
+      if (wave == w) {
+#pragma unroll
+        for (int s = 0; s < kCPL; ++s) {
+          const int col = lane + 64 * s;
+          if (col >= 256) continue;
+          for (int q = 0; q < kJP; ++q) sPart[q][col] = (w == 0 ? 0.0 : sPart[q][col]) + tp[s][q];
+        }
+      }
+      __syncthreads();
+    }
+    for (int k = threadIdx.x; k < kJP * a.D; k += blockDim.x) {
+      const int q = k / a.D, col = k % a.D;
+      a.part[(int64_t(tile) * kJP + q) * a.D + col] = sPart[q][col];
+    }
+    __syncthreads();
+  }
+}
+
+// energy / stress / limit rows of each configuration
+__global__ void k_jac_rows(JacArgs a) {
+  const int c = blockIdx.x;
+  const int64_t na = a.atom_off[c + 1] - a.atom_off[c];
+  const int64_t e_row = a.row_off[c] + 3 * na;
+  const bool ws = a.sw > 0.0;
+  const double iv = 1.0 / a.vol[c];
+  for (int col = threadIdx.x; col < a.D; col += blockDim.x) {
+    double acc[kJP];
+    for (int q = 0; q < kJP; ++q) acc[q] = 0.0;
+    for (int t = a.cfg_tile0[c]; t < a.cfg_tile0[c + 1]; ++t)
+      for (int q = 0; q < kJP; ++q) acc[q] += a.part[(int64_t(t) * kJP + q) * a.D + col];
+    int64_t row = e_row;
+    a.J[row++ * a.D + col] = float(a.ew * acc[0]);
+    if (ws)
+      for (int v = 0; v < 6; ++v) a.J[row++ * a.D + col] = float(-0.5 * a.sw * acc[1 + v] * iv);
+    a.J[row * a.D + col] = float(acc[7]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Stillinger-Weber tangents
+// ---------------------------------------------------------------------------
+struct SwT {
+  double A, B, p, q, delta, a1, gamma, a2;
+};
+__device__ inline SwT lsw(const double *sw, int s) {
+  const double *q = sw + 8 * s;
+  return SwT{q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7]};
+}
+
+// values and tangents w.r.t. field f (0..5) of (v2, dv2)
+__device__ inline void v2_tan(double r, const SwT &p, int f, double &v2, double &dv2, double &tv,
+                              double &tdv) {
+  v2 = dv2 = tv = tdv = 0.0;
+  if (r >= p.a1) return;
+  const double dd = r - p.a1;
+  const double e = exp(p.delta / dd);
+  if (e == 0.0) return;
+  const double rp = pow(r, -p.p), rq = pow(r, -p.q);
+  const double rp1 = pow(r, -p.p - 1.0), rq1 = pow(r, -p.q - 1.0);
+  const double lr = log(r);
+  const double poly = p.A * rp - p.B * rq;
+  const double dpoly = -p.A * p.p * rp1 + p.B * p.q * rq1;
+  const double g = -p.delta / (dd * dd);
+  v2 = poly * e;
+  dv2 = dpoly * e + poly * e * g;
+  double P = 0.0, dP = 0.0, E = 0.0, Gt = 0.0;
+  switch (f) {
+  case 0: P = rp; dP = -p.p * rp1; break;
+  case 1: P = -rq; dP = p.q * rq1; break;
+  case 2: P = -p.A * lr * rp; dP = p.A * rp1 * (p.p * lr - 1.0); break;
+  case 3: P = p.B * lr * rq; dP = p.B * rq1 * (1.0 - p.q * lr); break;
+  case 4: E = e / dd; Gt = -1.0 / (dd * dd); break;
+  case 5: E = e * p.delta / (dd * dd); Gt = -2.0 * p.delta / (dd * dd * dd); break;
+  default: break;
+  }
+  tv = P * e + poly * E;
+  tdv = dP * e + dpoly * E + P * e * g + poly * E * g + poly * e * Gt;
+}
+
+__device__ inline void h_tan(double r, const SwT &p, int f, double &h, double &dh, double &th,
+                             double &tdh) {
+  h = dh = th = tdh = 0.0;
+  if (r >= p.a2) return;
+  const double dd = r - p.a2;
+  const double e = exp(p.gamma / dd);
+  if (e == 0.0) return;
+  const double g = -p.gamma / (dd * dd);
+  h = e;
+  dh = e * g;
+  double H = 0.0, Gt = 0.0;
+  if (f == 6) {
+    H = e / dd;
+    Gt = -1.0 / (dd * dd);
+  } else if (f == 7) {
+    H = e * p.gamma / (dd * dd);
+    Gt = -2.0 * p.gamma / (dd * dd * dd);
+  }
+  th = H;
+  tdh = H * g + e * Gt;
+}
+
+__device__ inline void tb(const double *d1, const double *d2, double i1, double i2, double c,
+                          double f1, double df1, double f2, double df2, double g, double dg,
+                          double *fi, double *fj, double *fk) {
+  const double i12 = i1 * i2;
+  const double A = c * i1 * i1 - i12, B = c * i2 * i2 - i12;
+  const double s1 = df1 * i1 * f2 * g, s2 = f1 * df2 * i2 * g, s3 = f1 * f2 * dg;
+  for (int x = 0; x < 3; ++x) {
+    fi[x] = s1 * d1[x] + s2 * d2[x] - s3 * (A * d1[x] + B * d2[x]);
+    fj[x] = -s1 * d1[x] - s3 * (i12 * d2[x] - c * i1 * i1 * d1[x]);
+    fk[x] = -s2 * d2[x] - s3 * (i12 * d1[x] - c * i2 * i2 * d2[x]);
+  }
+}
+
+struct SwJArgs {
+  int64_t N;
+  int D, ntypes, npair;
+  const int64_t *pair_off;
+  const int32_t *types;
+  const double *pd;
+  const uint32_t *bits;
+  const int32_t *rev;
+  const double *sw, *lam;
+  const int32_t *col_sw;  // npair*8 -> column or -1
+  const int32_t *col_lam; // nlam -> column or -1
+  double *own;            // [D][N][3]
+  double *ptan;           // [D][P][3]
+  double *etan;           // [D][N][10]: energy + 9 virial
+  int64_t P;
+  // tiles
+  int ntiles;
+  const int32_t *tile_cfg, *tile_a0, *tile_na;
+  const int64_t *atom_off, *row_off;
+  float *J;
+  double *part;
+};
+
+__device__ inline double pr64(const double *pd, int64_t p) {
+  const double d0 = pd[3 * p], d1 = pd[3 * p + 1], d2 = pd[3 * p + 2];
+  double r2 = d0 * d0 + d1 * d1;
+  r2 = r2 + d2 * d2;
+  return sqrt(r2);
+}
+
+__global__ __launch_bounds__(128) void k_sw_jac_centre(SwJArgs a) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int col = blockIdx.y;
+  if (i >= a.N) return;
+  const int ti = a.types[i];
+  const int64_t p0 = a.pair_off[i], p1 = a.pair_off[i + 1];
+  double F[3] = {0, 0, 0}, E = 0.0, sg[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  double *pt = a.ptan + int64_t(col) * a.P * 3;
+  for (int64_t p = p0; p < p1; ++p) pt[3 * p] = pt[3 * p + 1] = pt[3 * p + 2] = 0.0;
+  for (int64_t p = p0; p < p1; ++p) {
+    const uint32_t b = a.bits[p];
+    if (b & kFlagTiny) continue;
+    const int slot = ord_pair(ti, pj_type(b), a.ntypes);
+    int f = -1;
+    for (int q = 0; q < 6; ++q)
+      if (a.col_sw[8 * slot + q] == col) f = q;
+    if (f < 0) continue;
+    const double r = pr64(a.pd, p);
+    double v2, dv2, tv, tdv;
+    v2_tan(r, lsw(a.sw, slot), f, v2, dv2, tv, tdv);
+    const double s = tdv / r;
+    const double *d = a.pd + 3 * p;
+    const double df[3] = {s * d[0], s * d[1], s * d[2]};
+    for (int x = 0; x < 3; ++x) F[x] += df[x];
+    E += 0.5 * tv;
+    for (int x = 0; x < 3; ++x)
+      for (int y = 0; y < 3; ++y) sg[3 * x + y] -= 0.5 * d[x] * df[y];
+  }
+  for (int64_t pj = p0; pj < p1; ++pj) {
+    const uint32_t bj = a.bits[pj];
+    if (bj & kFlagTiny) continue;
+    const int tj = pj_type(bj);
+    const int s1 = ord_pair(ti, tj, a.ntypes);
+    const double r1 = pr64(a.pd, pj);
+    double h1, dh1, th, tdh;
+    h_tan(r1, lsw(a.sw, s1), -1, h1, dh1, th, tdh);
+    if (h1 == 0.0 && dh1 == 0.0) continue;
+    const double i1 = 1.0 / r1;
+    const double *d1 = a.pd + 3 * pj;
+    for (int64_t pk = pj + 1; pk < p1; ++pk) {
+      const uint32_t bk = a.bits[pk];
+      if (bk & kFlagTiny) continue;
+      const int tk = pj_type(bk);
+      const int s2 = ord_pair(ti, tk, a.ntypes);
+      const double r2 = pr64(a.pd, pk);
+      double h2, dh2;
+      h_tan(r2, lsw(a.sw, s2), -1, h2, dh2, th, tdh);
+      if (h2 == 0.0 && dh2 == 0.0) continue;
+      const double i2 = 1.0 / r2;
+      const double *d2 = a.pd + 3 * pk;
+      const int li = ti * a.npair + ord_pair(tj, tk, a.ntypes);
+      const double lam = a.lam[li];
+      const double cs = (d1[0] * d2[0] + d1[1] * d2[1] + d1[2] * d2[2]) * i1 * i2;
+      const double cp = cs + 1.0 / 3.0;
+      const double w = lam * cp * cp, dw = 2.0 * lam * cp;
+      // tangents of (h1, dh1, h2, dh2, w, dw) w.r.t. this column
+      double t1 = 0, td1 = 0, t2 = 0, td2 = 0, tg = 0, tdg = 0;
+      bool any = false;
+      for (int f = 6; f < 8; ++f) {
+        if (a.col_sw[8 * s1 + f] == col) {
+          double hh, dd, a1_, b1_;
+          h_tan(r1, lsw(a.sw, s1), f, hh, dd, a1_, b1_);
+          t1 += a1_;
+          td1 += b1_;
+          any = true;
+        }
+        if (a.col_sw[8 * s2 + f] == col) {
+          double hh, dd, a2_, b2_;
+          h_tan(r2, lsw(a.sw, s2), f, hh, dd, a2_, b2_);
+          t2 += a2_;
+          td2 += b2_;
+          any = true;
+        }
+      }
+      if (a.col_lam[li] == col) {
+        tg = cp * cp;
+        tdg = 2.0 * cp;
+        any = true;
+      }
+      if (!any) continue;
+      double fi[3], fj[3], fk[3], xi[3], xj[3], xk[3];
+      tb(d1, d2, i1, i2, cs, t1, td1, h2, dh2, w, dw, fi, fj, fk);
+      tb(d1, d2, i1, i2, cs, h1, dh1, t2, td2, w, dw, xi, xj, xk);
+      for (int x = 0; x < 3; ++x) { fi[x] += xi[x]; fj[x] += xj[x]; fk[x] += xk[x]; }
+      tb(d1, d2, i1, i2, cs, h1, dh1, h2, dh2, tg, tdg, xi, xj, xk);
+      for (int x = 0; x < 3; ++x) { fi[x] += xi[x]; fj[x] += xj[x]; fk[x] += xk[x]; }
+      E += t1 * h2 * w + h1 * t2 * w + h1 * h2 * tg;
+      for (int x = 0; x < 3; ++x) {
+        F[x] += fi[x];
+        pt[3 * pj + x] += fj[x];
+        pt[3 * pk + x] += fk[x];
+      }
+      for (int x = 0; x < 3; ++x)
+        for (int y = 0; y < 3; ++y) sg[3 * x + y] += d1[x] * fj[y] + d2[x] * fk[y];
+    }
+  }
+  double *o = a.own + (int64_t(col) * a.N + i) * 3;
+  for (int x = 0; x < 3; ++x) o[x] = F[x];
+  double *e = a.etan + (int64_t(col) * a.N + i) * 10;
+  e[0] = E;
+  for (int k = 0; k < 9; ++k) e[1 + k] = sg[k];
+}
+
+// one block per tile, x = atoms of the tile, y = column
+__global__ __launch_bounds__(64) void k_sw_jac_tiles(SwJArgs a) {
+  __shared__ double red[64][kJP];
+  const int tile = blockIdx.x, col = blockIdx.y, t = threadIdx.x;
+  const int na = a.tile_na[tile];
+  if (t < na) {
+    const int64_t i = int64_t(a.tile_a0[tile]) + t;
+    const int c = a.tile_cfg[tile];
+    const double *o = a.own + (int64_t(col) * a.N + i) * 3;
+    double F[3] = {o[0], o[1], o[2]};
+    const double *pt = a.ptan + int64_t(col) * a.P * 3;
+    for (int64_t p = a.pair_off[i]; p < a.pair_off[i + 1]; ++p) {
+      const int32_t q = a.rev[p];
+      if (q < 0) continue;
+      for (int x = 0; x < 3; ++x) F[x] += pt[3 * int64_t(q) + x];
+    }
+    const int64_t row = a.row_off[c] + 3 * (i - a.atom_off[c]);
+    for (int x = 0; x < 3; ++x) a.J[(row + x) * a.D + col] = float(F[x]);
+    const double *e = a.etan + (int64_t(col) * a.N + i) * 10;
+    red[t][0] = e[0];
+    // stored as -2 x virial so the row kernel's -w_S/(2V) scaling yields w_S * virial / V
+    const int vo[6] = {0, 4, 8, 1, 2, 5};
+    for (int v = 0; v < 6; ++v) red[t][1 + v] = -2.0 * e[1 + vo[v]];
+    red[t][7] = 0.0;
+  }
+  __syncthreads();
+  if (t < kJP) {
+    double s = 0.0;
+    for (int q = 0; q < na; ++q) s += red[q][t];
+    a.part[(int64_t(tile) * kJP + t) * a.D + col] = s;
+  }
+}
+
+inline unsigned nb(int64_t n, int b) { return unsigned((n + b - 1) / b); }
+
+} // namespace
+
+void launch_jacobian(Context &c) {
+  FS_REQUIRE(c.kind != FS_ADP, FS_ERR_UNSUPPORTED, "ADP Jacobian columns are not available yet");
+  FS_REQUIRE(c.D <= 64 * kCPL, FS_ERR_UNSUPPORTED, "more than 256 free parameters");
+  const int64_t R = c.R;
+  c.J.alloc(size_t(std::max<int64_t>(R * c.D, 1)));
+  // Jacobian tiles: <= 64 consecutive atoms of one configuration
+  if (c.njt == 0) {
+    std::vector<int32_t> tcfg, ta0, tna, ct0(c.C + 1, 0);
+    for (int k = 0; k < c.C; ++k) {
+      ct0[k] = int32_t(tcfg.size());
+      for (int64_t a = c.h_atom_off[k]; a < c.h_atom_off[k + 1]; a += kJacTile) {
+        tcfg.push_back(k);
+        ta0.push_back(int32_t(a));
+        tna.push_back(int32_t(std::min<int64_t>(kJacTile, c.h_atom_off[k + 1] - a)));
+      }
+    }
+    ct0[c.C] = int32_t(tcfg.size());
+    const int n = int(tcfg.size());
+    c.jt_cfg.alloc(std::max(n, 1));
+    c.jt_a0.alloc(std::max(n, 1));
+    c.jt_na.alloc(std::max(n, 1));
+    c.jt_ct0.alloc(c.C + 1);
+    FS_HIP(hipMemcpyAsync(c.jt_cfg.p, tcfg.data(), 4 * n, hipMemcpyHostToDevice, c.stream));
+    FS_HIP(hipMemcpyAsync(c.jt_a0.p, ta0.data(), 4 * n, hipMemcpyHostToDevice, c.stream));
+    FS_HIP(hipMemcpyAsync(c.jt_na.p, tna.data(), 4 * n, hipMemcpyHostToDevice, c.stream));
+    FS_HIP(hipMemcpyAsync(c.jt_ct0.p, ct0.data(), 4 * (c.C + 1), hipMemcpyHostToDevice, c.stream));
+    FS_HIP(hipStreamSynchronize(c.stream));
+    c.njt = n;
+  }
+  const int nt = c.njt;
+  c.jac_part.alloc(size_t(std::max(nt, 1)) * kJP * std::max(c.D, 1));
+
+  JacArgs a{};
+  a.N = c.N;
+  a.C = c.C;
+  a.D = c.D;
+  a.ntypes = c.ntypes;
+  a.npair = c.npair;
+  a.ntab = c.ntab;
+  a.ntiles = nt;
+  a.tile_cfg = c.jt_cfg.p;
+  a.tile_a0 = c.jt_a0.p;
+  a.tile_na = c.jt_na.p;
+  a.cfg_tile0 = c.jt_ct0.p;
+  a.atom_off = c.atom_off.p;
+  a.pair_off = c.pair_off.p;
+  a.row_off = c.row_off.p;
+  a.types = c.types.p;
+  a.rec = c.pair_rec.p;
+  a.bits = c.pair_bits.p;
+  a.J = c.J.p;
+  a.part = c.jac_part.p;
+  a.vol = c.vol.p;
+  a.ew = c.energy_weight;
+  a.sw = c.stress_weight;
+
+  if (c.kind == FS_SW) {
+    SwJArgs s{};
+    s.N = c.N;
+    s.D = c.D;
+    s.ntypes = c.ntypes;
+    s.npair = c.npair;
+    s.pair_off = c.pair_off.p;
+    s.types = c.types.p;
+    s.pd = c.pair_d.p;
+    s.bits = c.pair_bits.p;
+    s.rev = c.rev.p;
+    s.sw = c.sw.p;
+    s.lam = c.lam.p;
+    std::vector<int32_t> csw(8 * c.npair), clam(c.h_lam.size());
+    int o = 0;
+    for (size_t k = 0; k < csw.size(); ++k) csw[k] = c.h_sw_fixed[k] ? -1 : o++;
+    for (size_t k = 0; k < clam.size(); ++k) clam[k] = c.h_lam_fixed[k] ? -1 : o++;
+    c.sw_csw.alloc(csw.size());
+    c.sw_clam.alloc(std::max<size_t>(clam.size(), 1));
+    FS_HIP(hipMemcpyAsync(c.sw_csw.p, csw.data(), 4 * csw.size(), hipMemcpyHostToDevice, c.stream));
+    FS_HIP(hipMemcpyAsync(c.sw_clam.p, clam.data(), 4 * clam.size(), hipMemcpyHostToDevice, c.stream));
+    s.col_sw = c.sw_csw.p;
+    s.col_lam = c.sw_clam.p;
+    s.P = c.P;
+    c.sw_own.alloc(size_t(c.D) * c.N * 3 + 1);
+    c.sw_ptan.alloc(size_t(c.D) * std::max<int64_t>(c.P, 1) * 3);
+    c.sw_etan.alloc(size_t(c.D) * c.N * 10 + 1);
+    s.own = c.sw_own.p;
+    s.ptan = c.sw_ptan.p;
+    s.etan = c.sw_etan.p;
+    s.ntiles = nt;
+    s.tile_cfg = c.jt_cfg.p;
+    s.tile_a0 = c.jt_a0.p;
+    s.tile_na = c.jt_na.p;
+    s.atom_off = c.atom_off.p;
+    s.row_off = c.row_off.p;
+    s.J = c.J.p;
+    s.part = c.jac_part.p;
+    if (c.N > 0 && c.D > 0) {
+      c.time_main_begin();
+      hipLaunchKernelGGL(k_sw_jac_centre, dim3(nb(c.N, 128), c.D), dim3(128), 0, c.stream, s);
+      FS_HIP(hipGetLastError());
+      c.time_main_end();
+      hipLaunchKernelGGL(k_sw_jac_tiles, dim3(std::max(nt, 1), c.D), dim3(64), 0, c.stream, s);
+      FS_HIP(hipGetLastError());
+    }
+  } else {
+    // column descriptors
+    std::vector<ColDesc> cols(std::max(c.D, 1));
+    int dens_c0 = -1, Dd = 0;
+    for (int t = 0; t < c.ntab; ++t) {
+      const TableDesc &T = c.h_tabs[t];
+      int kind = CK_PAIR;
+      if (t >= c.npair && t < c.npair + c.ntypes) kind = CK_DENS;
+      else if (t >= c.npair + c.ntypes && t < c.npair + 2 * c.ntypes) kind = CK_EMB;
+      for (int m = 0; m < T.n; ++m) {
+        const int col = c.h_col_of[T.off + m];
+        if (col < 0) continue;
+        ColDesc cd{t, m, kind, -1};
+        if (kind == CK_DENS) {
+          if (dens_c0 < 0) dens_c0 = col;
+          cd.dc = Dd++;
+        }
+        cols[col] = cd;
+      }
+    }
+    if (dens_c0 < 0) dens_c0 = 0;
+    static_assert(sizeof(ColDesc) == sizeof(int4), "column descriptor layout");
+    c.jcols.alloc(cols.size());
+    FS_HIP(hipMemcpyAsync(c.jcols.p, cols.data(), sizeof(ColDesc) * cols.size(), hipMemcpyHostToDevice,
+                          c.stream));
+    FS_HIP(hipStreamSynchronize(c.stream));
+    a.Dd = Dd;
+    a.dens_c0 = dens_c0;
+    a.cols = reinterpret_cast<const ColDesc *>(c.jcols.p);
+    a.tabs = c.tabs.p;
+    a.tx = c.tab_x.p;
+    a.ty = c.tab_y.p;
+    a.ts = c.tab_s.p;
+    a.tS = c.tab_S.p;
+    a.ivl32 = c.ivl32.p;
+    a.rho = c.rho.p;
+    a.over = c.clamp_over.p;
+    a.Fp = c.Fp.p;
+    a.Fp32 = c.Fp32.p;
+    c.H.alloc(size_t(std::max<int64_t>(c.N * std::max(Dd, 1), 1)));
+    c.Hs.alloc(size_t(std::max<int64_t>(c.N * std::max(Dd, 1), 1)));
+    c.emb_d.alloc(4 * size_t(std::max<int64_t>(c.N, 1)));
+    c.emb_v.alloc(4 * size_t(std::max<int64_t>(c.N, 1)));
+    c.emb_k.alloc(size_t(std::max<int64_t>(c.N, 1)));
+    a.H = c.H.p;
+    a.Hs = c.Hs.p;
+    a.emb_d = c.emb_d.p;
+    a.emb_v = c.emb_v.p;
+    a.emb_k = c.emb_k.p;
+    a.Fpp = c.Fpp.p;
+    const size_t base = size_t(c.ntab) * sizeof(TableDesc) + ((size_t(c.nknots) * 5 + 3) & ~size_t(3)) * 4;
+    const size_t lds_dens = base + 4 * 64 * sizeof(DRec);
+    const size_t lds_main = base + 4 * 64 * sizeof(PRec);
+    FS_REQUIRE(lds_main + kJP * 256 * 8 <= 64 * 1024, FS_ERR_UNSUPPORTED,
+               "Jacobian LDS budget exceeded");
+    if (c.N > 0) {
+      const unsigned g1 = unsigned(std::min<int64_t>((c.N + 3) / 4, int64_t(c.cu_count) * 16));
+      hipLaunchKernelGGL(k_jac_dens, dim3(g1), dim3(256), lds_dens, c.stream, a, c.nknots);
+      FS_HIP(hipGetLastError());
+      const unsigned g2 = unsigned(std::max(1, std::min(nt, c.cu_count * 8)));
+      c.time_main_begin();
+      hipLaunchKernelGGL(k_jac_main, dim3(g2), dim3(256), lds_main, c.stream, a, c.nknots);
+      FS_HIP(hipGetLastError());
+      c.time_main_end();
+    }
+  }
+  hipLaunchKernelGGL(k_jac_rows, dim3(c.C), dim3(256), 0, c.stream, a);
+  FS_HIP(hipGetLastError());
+}
+
+} // namespace fsa

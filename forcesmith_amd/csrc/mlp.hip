@@ -1,0 +1,242 @@
+// MLP surrogate for differential evolution: D -> H -> H -> 1, GELU (tanh
+// form), BF16 matrix-core products with FP32 accumulation, FP32 master
+// weights and Adam. No counterpart in the reference: the surrogate screens DE
+// trial vectors; the exact GPU cost still judges every accepted member.
+//
+// GEMM: C[M x N] = op(A)[M x K] * op(B)[K x N]; a 256-thread block owns a
+// 64 x 64 tile of C, each wave a 32 x 32 quarter as 2 x 2 tiles of
+// v_mfma_f32_16x16x32_bf16. A and B tiles are converted to BF16 while staged
+// in LDS (A row-major [64][32], B transposed [64][32]) so every lane reads its
+// eight consecutive K values with one 16-byte LDS load.
+#include "common.hpp"
+#include "mlp.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace fsa {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+constexpr int kBM = 64, kBN = 64, kBK = 32;
+enum { EPI_STORE = 0, EPI_BIAS_GELU = 1 };
+
+__device__ inline float gelu_f(float u) {
+  const float k = 0.7978845608028654f;
+  return 0.5f * u * (1.0f + tanhf(k * (u + 0.044715f * u * u * u)));
+}
+__device__ inline float gelu_df(float u) {
+  const float k = 0.7978845608028654f;
+  const float th = tanhf(k * (u + 0.044715f * u * u * u));
+  return 0.5f * (1.0f + th) + 0.5f * u * (1.0f - th * th) * k * (1.0f + 3.0f * 0.044715f * u * u);
+}
+
+template <bool TA, bool TB, int EPI>
+__global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, const float *__restrict__ A,
+                                                   int lda, const float *__restrict__ B, int ldb,
+                                                   float *__restrict__ C, int ldc,
+                                                   const float *__restrict__ bias,
+                                                   float *__restrict__ H) {
+  __shared__ __attribute__((aligned(16))) __bf16 sA[kBM][kBK + 8];
+  __shared__ __attribute__((aligned(16))) __bf16 sB[kBN][kBK + 8];
+  const int m0 = blockIdx.y * kBM, n0 = blockIdx.x * kBN;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < K; k0 += kBK) {
+    for (int e = threadIdx.x; e < kBM * kBK; e += 256) {
+      const int r = e / kBK, kk = e % kBK;
+      const int m = m0 + r, k = k0 + kk;
+      float v = 0.f;
+      if (m < M && k < K) v = TA ? A[int64_t(k) * lda + m] : A[int64_t(m) * lda + k];
+      sA[r][kk] = __bf16(v);
+    }
+    for (int e = threadIdx.x; e < kBN * kBK; e += 256) {
+      const int c = e / kBK, kk = e % kBK;
+      const int n = n0 + c, k = k0 + kk;
+      float v = 0.f;
+      if (n < N && k < K) v = TB ? B[int64_t(n) * ldb + k] : B[int64_t(k) * ldb + n];
+      sB[c][kk] = __bf16(v);
+    }
+    __syncthreads();
+    const int kq = 8 * (lane >> 4);
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      const bf16x8 fa = *reinterpret_cast<const bf16x8 *>(&sA[wm + 16 * a + (lane & 15)][kq]);
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const bf16x8 fb = *reinterpret_cast<const bf16x8 *>(&sB[wn + 16 * b + (lane & 15)][kq]);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa, fb, acc[a][b], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int m = m0 + wm + 16 * a + 4 * (lane >> 4) + i;
+        const int n = n0 + wn + 16 * b + (lane & 15);
+        if (m >= M || n >= N) continue;
+        float v = acc[a][b][i];
+        if (EPI == EPI_BIAS_GELU) {
+          v += bias[n];
+          C[int64_t(m) * ldc + n] = v;
+          H[int64_t(m) * ldc + n] = gelu_f(v);
+        } else {
+          C[int64_t(m) * ldc + n] = v;
+        }
+      }
+}
+
+// y[b] = H2[b] . W3 + b3  (one wave per row)
+__global__ void k_head(int B, int Hn, const float *__restrict__ H2, const float *__restrict__ W3,
+                       const float *__restrict__ b3, float *__restrict__ y) {
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (row >= B) return;
+  float s = 0.f;
+  for (int k = lane; k < Hn; k += 64) s += H2[int64_t(row) * Hn + k] * W3[k];
+  for (int m = 32; m > 0; m >>= 1) s += __shfl_xor(s, m, 64);
+  if (lane == 0) y[row] = s + b3[0];
+}
+
+// gy = (y - t)/B; dZ2 = gy * W3 * gelu'(Z2); loss partial
+__global__ void k_head_back(int B, int Hn, const float *__restrict__ y, const float *__restrict__ t,
+                            const float *__restrict__ W3, const float *__restrict__ Z2,
+                            float *__restrict__ gy, float *__restrict__ dZ2) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= int64_t(B) * Hn) return;
+  const int b = int(e / Hn), k = int(e % Hn);
+  const float g = (y[b] - t[b]) / float(B);
+  if (k == 0) gy[b] = g;
+  dZ2[e] = g * W3[k] * gelu_df(Z2[e]);
+}
+
+__global__ void k_gelu_back(int64_t n, const float *__restrict__ Z, float *__restrict__ dH) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e < n) dH[e] *= gelu_df(Z[e]);
+}
+
+// column sums over rows: out[c] = sum_b A[b*ld + c] (* w[b] if w)
+__global__ void k_colsum(int B, int Ncol, const float *__restrict__ A, const float *__restrict__ w,
+                         float *__restrict__ out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= Ncol) return;
+  float s = 0.f;
+  for (int b = 0; b < B; ++b) s += A[int64_t(b) * Ncol + c] * (w ? w[b] : 1.0f);
+  out[c] = s;
+}
+
+__global__ void k_loss(int B, const float *__restrict__ y, const float *__restrict__ t,
+                       float *__restrict__ out) {
+  __shared__ float s[256];
+  float a = 0.f;
+  for (int b = threadIdx.x; b < B; b += blockDim.x) {
+    const float r = y[b] - t[b];
+    a += 0.5f * r * r;
+  }
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = s[0] / float(B);
+}
+
+__global__ void k_adam(int64_t n, float *__restrict__ p, const float *__restrict__ g,
+                       float *__restrict__ m, float *__restrict__ v, float lr, float b1, float b2,
+                       float c1, float c2, float eps) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const float gg = g[e];
+  m[e] = b1 * m[e] + (1.f - b1) * gg;
+  v[e] = b2 * v[e] + (1.f - b2) * gg * gg;
+  p[e] -= lr * (m[e] / c1) / (sqrtf(v[e] / c2) + eps);
+}
+
+inline unsigned cdiv(int64_t a, int b) { return unsigned((a + b - 1) / b); }
+
+template <bool TA, bool TB, int EPI>
+void gemm(hipStream_t s, int M, int N, int K, const float *A, int lda, const float *B, int ldb,
+          float *C, int ldc, const float *bias = nullptr, float *H = nullptr) {
+  if (M <= 0 || N <= 0) return;
+  dim3 grid(cdiv(N, kBN), cdiv(M, kBM));
+  hipLaunchKernelGGL((k_gemm_bf16<TA, TB, EPI>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C,
+                     ldc, bias, H);
+  FS_HIP(hipGetLastError());
+}
+
+} // namespace
+
+void mlp_init(Mlp &m, int D, int Hn) {
+  m.D = D;
+  m.Hn = Hn;
+  m.cap = 0;
+  const size_t np = m.n_params();
+  m.P.alloc(np);
+  m.G.alloc(np);
+  m.Mo.alloc(np);
+  m.Vo.alloc(np);
+  m.step = 0;
+}
+
+void mlp_forward(Mlp &m, hipStream_t s, int B, const float *X, float *y_out) {
+  m.ensure(B);
+  gemm<false, true, EPI_BIAS_GELU>(s, B, m.Hn, m.D, X, m.D, m.W1(), m.D, m.Z1.p, m.Hn, m.b1(),
+                                   m.H1.p);
+  gemm<false, true, EPI_BIAS_GELU>(s, B, m.Hn, m.Hn, m.H1.p, m.Hn, m.W2(), m.Hn, m.Z2.p, m.Hn,
+                                   m.b2(), m.H2.p);
+  hipLaunchKernelGGL(k_head, dim3(cdiv(B, 4)), dim3(256), 0, s, B, m.Hn, m.H2.p, m.W3(), m.b3(),
+                     y_out ? y_out : m.y.p);
+  FS_HIP(hipGetLastError());
+}
+
+// gradient of mean 0.5 (y - t)^2 into m.G; returns nothing (loss in m.loss)
+void mlp_backward(Mlp &m, hipStream_t s, int B, const float *X, const float *t) {
+  const int Hn = m.Hn, D = m.D;
+  mlp_forward(m, s, B, X, m.y.p);
+  hipLaunchKernelGGL(k_loss, dim3(1), dim3(256), 0, s, B, m.y.p, t, m.loss.p);
+  // dZ2 in tmp
+  hipLaunchKernelGGL(k_head_back, dim3(cdiv(int64_t(B) * Hn, 256)), dim3(256), 0, s, B, Hn, m.y.p, t,
+                     m.W3(), m.Z2.p, m.gy.p, m.tmp.p);
+  float *gW1 = m.G.p, *gb1 = gW1 + size_t(Hn) * D, *gW2 = gb1 + Hn, *gb2 = gW2 + size_t(Hn) * Hn,
+        *gW3 = gb2 + Hn, *gb3 = gW3 + Hn;
+  // gW3 = sum_b gy_b H2_b ; gb3 = sum_b gy_b
+  hipLaunchKernelGGL(k_colsum, dim3(cdiv(Hn, 256)), dim3(256), 0, s, B, Hn, m.H2.p, m.gy.p, gW3);
+  hipLaunchKernelGGL(k_colsum, dim3(1), dim3(256), 0, s, B, 1, m.gy.p, (const float *)nullptr, gb3);
+  // gW2 = dZ2^T H1 ; gb2 = colsum dZ2
+  gemm<true, false, EPI_STORE>(s, Hn, Hn, B, m.tmp.p, Hn, m.H1.p, Hn, gW2, Hn);
+  hipLaunchKernelGGL(k_colsum, dim3(cdiv(Hn, 256)), dim3(256), 0, s, B, Hn, m.tmp.p,
+                     (const float *)nullptr, gb2);
+  // dH1 = dZ2 W2 -> Z2 buffer reused; dZ1 = dH1 * gelu'(Z1)
+  gemm<false, false, EPI_STORE>(s, B, Hn, Hn, m.tmp.p, Hn, m.W2(), Hn, m.H2.p, Hn);
+  hipLaunchKernelGGL(k_gelu_back, dim3(cdiv(int64_t(B) * Hn, 256)), dim3(256), 0, s,
+                     int64_t(B) * Hn, m.Z1.p, m.H2.p);
+  // gW1 = dZ1^T X ; gb1 = colsum dZ1
+  gemm<true, false, EPI_STORE>(s, Hn, D, B, m.H2.p, Hn, X, D, gW1, D);
+  hipLaunchKernelGGL(k_colsum, dim3(cdiv(Hn, 256)), dim3(256), 0, s, B, Hn, m.H2.p,
+                     (const float *)nullptr, gb1);
+  FS_HIP(hipGetLastError());
+}
+
+void mlp_adam(Mlp &m, hipStream_t s, float lr) {
+  ++m.step;
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = 1.f - std::pow(b1, float(m.step)), c2 = 1.f - std::pow(b2, float(m.step));
+  const int64_t n = int64_t(m.n_params());
+  hipLaunchKernelGGL(k_adam, dim3(cdiv(n, 256)), dim3(256), 0, s, n, m.P.p, m.G.p, m.Mo.p, m.Vo.p, lr,
+                     b1, b2, c1, c2, 1e-8f);
+  FS_HIP(hipGetLastError());
+}
+
+} // namespace fsa

@@ -1,0 +1,242 @@
+// Host drivers above the kernels: Jacobian / normal equations, the
+// Levenberg-Marquardt loop, population costs and differential evolution,
+// kernel timing with HIP events on the context stream, and the launch-shape
+// autotuner. Each step's arithmetic runs on the GPU; the host only carries
+// scalars (costs, the LM damping, DE selection) between launches.
+#include "common.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <limits>
+
+namespace fsa {
+
+void evaluate_for_solver(Context &c);
+void rebuild_tiles(Context &c);
+void ensure_ready(Context &c);
+void launch_diag_scale(Context &c);
+void launch_damped_solve(Context &c, double mu);
+int de_run(Context &c, const fs_de_opts *o, const double *lo, const double *hi, double *x,
+           double *best, double *hist, int *gens);
+
+namespace {
+
+void scatter_host(Context &c, const double *x) {
+  int o = 0;
+  if (c.kind == FS_SW) {
+    for (size_t k = 0; k < c.h_sw.size(); ++k)
+      if (!c.h_sw_fixed[k]) c.h_sw[k] = x[o++];
+    for (size_t k = 0; k < c.h_lam.size(); ++k)
+      if (!c.h_lam_fixed[k]) c.h_lam[k] = x[o++];
+  } else {
+    for (int k = 0; k < c.D; ++k) c.h_y[c.h_knot_of[k]] = x[k];
+  }
+  c.params_dirty = true;
+  c.forces_valid = false;
+}
+
+void gather_host(const Context &c, double *x) {
+  int o = 0;
+  if (c.kind == FS_SW) {
+    for (size_t k = 0; k < c.h_sw.size(); ++k)
+      if (!c.h_sw_fixed[k]) x[o++] = c.h_sw[k];
+    for (size_t k = 0; k < c.h_lam.size(); ++k)
+      if (!c.h_lam_fixed[k]) x[o++] = c.h_lam[k];
+  } else {
+    for (int k = 0; k < c.D; ++k) x[k] = c.h_y[c.h_knot_of[k]];
+  }
+}
+
+double cost_now(Context &c) {
+  evaluate_for_solver(c);
+  launch_cost(c, c.cost.p);
+  double v = 0.0;
+  FS_HIP(hipMemcpyAsync(&v, c.cost.p, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  FS_HIP(hipStreamSynchronize(c.stream));
+  return v;
+}
+
+} // namespace
+
+int fs_jacobian_impl(Context &c, float *J) {
+  evaluate_for_solver(c);
+  launch_jacobian(c);
+  if (J)
+    FS_HIP(hipMemcpyAsync(J, c.J.p, sizeof(float) * size_t(c.R) * c.D, hipMemcpyDeviceToHost,
+                          c.stream));
+  FS_HIP(hipStreamSynchronize(c.stream));
+  return FS_OK;
+}
+
+int fs_normal_impl(Context &c, double *JtJ, double *Jtr, double *cost) {
+  evaluate_for_solver(c);
+  launch_jacobian(c);
+  launch_normal_equations(c);
+  if (JtJ)
+    FS_HIP(hipMemcpyAsync(JtJ, c.JtJ.p, sizeof(double) * size_t(c.D) * c.D, hipMemcpyDeviceToHost,
+                          c.stream));
+  if (Jtr)
+    FS_HIP(hipMemcpyAsync(Jtr, c.Jtr.p, sizeof(double) * c.D, hipMemcpyDeviceToHost, c.stream));
+  if (cost) {
+    launch_cost(c, c.cost.p);
+    FS_HIP(hipMemcpyAsync(cost, c.cost.p, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  }
+  FS_HIP(hipStreamSynchronize(c.stream));
+  return FS_OK;
+}
+
+// Levenberg-Marquardt on the normal equations: one Jacobian per outer
+// iteration, inner damping loop until the gain ratio exceeds 1e-4 (as
+// MINPACK's lmpar loop inside Eigen's minimizeOneStep, solver.cpp:67-85),
+// Marquardt diagonal scaling, Nielsen damping update. The same algorithm is
+// restated by the CPU oracle.
+int fs_lm_impl(Context &c, double *x, int max_iter, double *hist, int *accepted) {
+  const int D = c.D;
+  *accepted = 0;
+  if (D == 0) return FS_OK;
+  std::vector<double> xn(D), dx(D);
+  scatter_host(c, x);
+  double cost = cost_now(c);
+  launch_jacobian(c);
+  launch_normal_equations(c);
+  launch_diag_scale(c);
+  double scal[3];
+  FS_HIP(hipMemcpyAsync(scal, c.lm_scal.p, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  FS_HIP(hipStreamSynchronize(c.stream));
+  double mu = 1e-3 * scal[0], nu = 2.0;
+  for (int it = 0; it < max_iter; ++it) {
+    bool ok = false;
+    for (int inner = 0; inner < 16; ++inner) {
+      launch_damped_solve(c, mu);
+      FS_HIP(hipMemcpyAsync(scal, c.lm_scal.p, sizeof(double) * 3, hipMemcpyDeviceToHost, c.stream));
+      FS_HIP(hipMemcpyAsync(dx.data(), c.lm_dx.p, sizeof(double) * D, hipMemcpyDeviceToHost,
+                            c.stream));
+      FS_HIP(hipStreamSynchronize(c.stream));
+      if (scal[1] != 0.0) {
+        mu *= nu;
+        nu *= 2.0;
+        continue;
+      }
+      for (int k = 0; k < D; ++k) xn[k] = x[k] + dx[k];
+      scatter_host(c, xn.data());
+      const double cn = cost_now(c);
+      const double pred = scal[2];
+      const double rho = (cost - cn) / pred;
+      if (std::isfinite(cn) && pred > 0.0 && rho > 1e-4) {
+        std::copy(xn.begin(), xn.end(), x);
+        cost = cn;
+        const double t = 2.0 * rho - 1.0;
+        const double f = 1.0 - t * t * t;
+        mu *= f > 1.0 / 3.0 ? f : 1.0 / 3.0;
+        nu = 2.0;
+        ok = true;
+        break;
+      }
+      mu *= nu;
+      nu *= 2.0;
+    }
+    if (hist) hist[it] = cost;
+    if (!ok) {
+      for (int r = it + 1; r < max_iter && hist; ++r) hist[r] = cost;
+      break;
+    }
+    ++*accepted;
+    if (it + 1 < max_iter) {
+      // the residual at x was just evaluated, the per-atom state matches x
+      launch_jacobian(c);
+      launch_normal_equations(c);
+      launch_diag_scale(c);
+    }
+  }
+  scatter_host(c, x);
+  evaluate_for_solver(c);
+  FS_HIP(hipStreamSynchronize(c.stream));
+  return FS_OK;
+}
+
+int fs_population_impl(Context &c, const double *X, int np, double *cost) {
+  const int D = c.D;
+  std::vector<double> x0(D);
+  gather_host(c, x0.data());
+  for (int m = 0; m < np; ++m) {
+    scatter_host(c, X + size_t(m) * D);
+    cost[m] = cost_now(c);
+  }
+  scatter_host(c, x0.data());
+  return FS_OK;
+}
+
+int fs_de_impl(Context &c, const fs_de_opts *o, const double *lo, const double *hi, double *x,
+               double *best, double *hist, int *gens) {
+  return de_run(c, o, lo, hi, x, best, hist, gens);
+}
+
+// what: 0 = residual (density + force + finalize), 1 = Jacobian (all its
+// kernels), 2 = normal equations, 3 = residual + Jacobian, 4 = the main
+// Jacobian kernel alone (events around that launch only)
+double time_step_impl(Context &c, int what, int reps) {
+  ensure_ready(c);
+  evaluate_for_solver(c);
+  if (what == 4) {
+    if (!c.jev0) FS_HIP(hipEventCreate(&c.jev0));
+    if (!c.jev1) FS_HIP(hipEventCreate(&c.jev1));
+    launch_jacobian(c); // warm-up
+    c.time_jac_main = true;
+    c.jac_main_ms = 0.0;
+    c.jac_main_n = 0;
+    try {
+      for (int r = 0; r < reps; ++r) launch_jacobian(c);
+    } catch (...) {
+      c.time_jac_main = false;
+      throw;
+    }
+    c.time_jac_main = false;
+    return c.jac_main_n ? c.jac_main_ms / c.jac_main_n : 0.0;
+  }
+  hipEvent_t e0, e1;
+  FS_HIP(hipEventCreate(&e0));
+  FS_HIP(hipEventCreate(&e1));
+  auto body = [&]() {
+    if (what == 0) launch_eval(c, true);
+    else if (what == 1) launch_jacobian(c);
+    else if (what == 2) launch_normal_equations(c);
+    else {
+      launch_eval(c, true);
+      launch_jacobian(c);
+    }
+  };
+  body(); // warm-up
+  FS_HIP(hipEventRecord(e0, c.stream));
+  for (int r = 0; r < reps; ++r) body();
+  FS_HIP(hipEventRecord(e1, c.stream));
+  FS_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  FS_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c.forces_valid = true;
+  return double(ms) / reps;
+}
+
+int autotune_impl(Context &c) {
+  ensure_ready(c);
+  const int cand[] = {4, 8, 16, 32, 64};
+  int best = c.lanes_per_atom;
+  double best_ms = std::numeric_limits<double>::infinity();
+  for (int lpa : cand) {
+    c.lanes_per_atom = lpa;
+    rebuild_tiles(c);
+    c.forces_valid = false;
+    const double ms = time_step_impl(c, 0, 5);
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = lpa;
+    }
+  }
+  c.lanes_per_atom = best;
+  rebuild_tiles(c);
+  c.forces_valid = false;
+  return FS_OK;
+}
+
+} // namespace fsa

@@ -1,0 +1,158 @@
+// Device-side makima spline evaluation on a uniform knot grid.
+//
+// Two forms, both restating src/potentials/spline.cpp / spline.hpp:
+//  * FP64 "site" form for pair tables (cached-site semantics, spline.hpp:
+//    47-98 with prepare_site spline.cpp:281-324): upper_bound interval,
+//    linear site at r <= x0 or r >= xn, Hermite basis dotted with
+//    (y_i, y_i+1, s_i, s_i+1).
+//  * FP64 uncached form for the embedding F(rho) (spline.cpp:215-279),
+//    extrapolating with the end secant outside the span.
+//  * FP32 form for the mixed-precision bulk: interval from the uniform grid,
+//    coefficients {y_k, s_k, y_k+1, s_k+1} read as one float4 from LDS.
+#pragma once
+
+#include "common.hpp"
+
+namespace fsa {
+
+__device__ inline int spl_locate(const double *x, int n, double r) {
+  int lo = 0, hi = n; // first index with x > r
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (x[mid] > r) hi = mid;
+    else lo = mid + 1;
+  }
+  int i = lo;
+  if (i > 0) --i;
+  if (i > n - 2) i = n - 2;
+  return i;
+}
+
+// Hermite basis of value and first/second r-derivative w.r.t.
+// (y_i, y_i+1, s_i, s_i+1).
+__device__ inline void herm_basis64(double x0, double dx, double r, double v[4], double d[4]) {
+  const double t = (r - x0) / dx;
+  const double omt = 1.0 - t;
+  v[0] = omt * omt * (1.0 + 2.0 * t);
+  v[1] = t * t * (3.0 - 2.0 * t);
+  v[2] = omt * omt * (r - x0);
+  v[3] = t * t * (dx * (t - 1.0));
+  d[0] = -6.0 * t * omt / dx;
+  d[1] = 6.0 * t * omt / dx;
+  d[2] = omt * (1.0 - 3.0 * t);
+  d[3] = t * (3.0 * t - 2.0);
+}
+
+__device__ inline void herm_basis2_64(double x0, double dx, double r, double d2[4]) {
+  const double t = (r - x0) / dx;
+  d2[0] = -6.0 * (1.0 - 2.0 * t) / (dx * dx);
+  d2[1] = 6.0 * (1.0 - 2.0 * t) / (dx * dx);
+  d2[2] = (6.0 * t - 4.0) / dx;
+  d2[3] = (6.0 * t - 2.0) / dx;
+}
+
+// cached-site evaluation (r already known to be inside [x0, xn])
+__device__ inline void spl_site64(const TableDesc &T, const double *x, const double *y,
+                                  const double *s, double r, double &v, double &d) {
+  const int n = T.n;
+  int i;
+  bool lin;
+  if (r <= x[0]) {
+    i = 0;
+    lin = true;
+  } else if (r >= x[n - 1]) {
+    i = n - 2;
+    lin = true;
+  } else {
+    i = spl_locate(x, n, r);
+    lin = T.linear != 0;
+  }
+  const double dx = x[i + 1] - x[i];
+  if (lin) {
+    const double inv_dx = 1.0 / dx;
+    const double off = r - x[i];
+    v = y[i] + off * inv_dx * (y[i + 1] - y[i]);
+    d = (y[i + 1] - y[i]) * inv_dx;
+    return;
+  }
+  double bv[4], bd[4];
+  herm_basis64(x[i], dx, r, bv, bd);
+  const double yi = y[i], yi1 = y[i + 1], si = s[i], si1 = s[i + 1];
+  v = bv[0] * yi + bv[1] * yi1 + bv[2] * si + bv[3] * si1;
+  d = bd[0] * yi + bd[1] * yi1 + bd[2] * si + bd[3] * si1;
+}
+
+// uncached evaluation with linear extrapolation (embedding F(rho))
+__device__ inline void spl_uncached64(const TableDesc &T, const double *x, const double *y,
+                                      const double *s, double r, double &v, double &d) {
+  const int n = T.n;
+  if (r <= x[0]) {
+    const double slope = (y[1] - y[0]) / (x[1] - x[0]);
+    v = y[0] + slope * (r - x[0]);
+    d = slope;
+  } else if (r >= x[n - 1]) {
+    const double slope = (y[n - 1] - y[n - 2]) / (x[n - 1] - x[n - 2]);
+    v = y[n - 1] + slope * (r - x[n - 1]);
+    d = slope;
+  } else if (T.linear) {
+    const int i = spl_locate(x, n, r);
+    const double t = (r - x[i]) / (x[i + 1] - x[i]);
+    v = y[i] + t * (y[i + 1] - y[i]);
+    d = (y[i + 1] - y[i]) / (x[i + 1] - x[i]);
+  } else {
+    const int i = spl_locate(x, n, r);
+    const double x0 = x[i], dx = x[i + 1] - x[i];
+    const double t = (r - x0) / dx;
+    const double omt = 1.0 - t;
+    const double a = y[i] * (1.0 + 2.0 * t) + s[i] * (r - x0);
+    const double da = y[i] * 2.0 / dx + s[i];
+    const double b = y[i + 1] * (3.0 - 2.0 * t) + dx * s[i + 1] * (t - 1.0);
+    const double db = -y[i + 1] * 2.0 / dx + s[i + 1];
+    v = omt * omt * a + t * t * b;
+    d = -2.0 * omt / dx * a + omt * omt * da + 2.0 * t / dx * b + t * t * db;
+  }
+}
+
+// FP32 bulk evaluation; q = {y_k, s_k, y_k+1, s_k+1} of interval k
+__device__ inline int spl_interval32(const TableDesc &T, float r, float &t) {
+  const float u = (r - T.x0f) * T.inv_hf;
+  int k = int(floorf(u));
+  k = k < 0 ? 0 : (k > T.n - 2 ? T.n - 2 : k);
+  t = u - float(k);
+  return k;
+}
+
+__device__ inline void spl_eval32(const TableDesc &T, float4 q, float t, float &v, float &d) {
+  if (T.linear) {
+    v = q.x + t * (q.z - q.x);
+    d = (q.z - q.x) * T.inv_hf;
+    return;
+  }
+  const float h = T.hf;
+  const float omt = 1.0f - t;
+  v = omt * omt * (q.x * (1.0f + 2.0f * t) + q.y * (h * t)) +
+      t * t * (q.z * (3.0f - 2.0f * t) + h * q.w * (t - 1.0f));
+  d = 6.0f * t * omt * (q.z - q.x) * T.inv_hf + omt * (1.0f - 3.0f * t) * q.y +
+      t * (3.0f * t - 2.0f) * q.w;
+}
+
+__device__ inline void spl_deriv32(const TableDesc &T, float4 q, float t, float &d) {
+  if (T.linear) {
+    d = (q.z - q.x) * T.inv_hf;
+    return;
+  }
+  const float omt = 1.0f - t;
+  d = 6.0f * t * omt * (q.z - q.x) * T.inv_hf + omt * (1.0f - 3.0f * t) * q.y +
+      t * (3.0f * t - 2.0f) * q.w;
+}
+
+__device__ inline int ord_pair(int a, int b, int nt) {
+  if (a > b) {
+    const int t = a;
+    a = b;
+    b = t;
+  }
+  return a * nt - a * (a - 1) / 2 + (b - a);
+}
+
+} // namespace fsa

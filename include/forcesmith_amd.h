@@ -1,0 +1,200 @@
+/*
+ * forcesmith_amd: MI355X-native force-matching evaluator, C ABI.
+ *
+ * The drop-in boundary for the reference's force-calculator customisation
+ * point (docs/extension.md:252-321; erased surface CForceCalcTE,
+ * include/forcesmith/force/force_calculator.hpp:62-81). A reference-side
+ * adapter type holds an fs_ctx* (shared, ref-counted: the reference deep-copies
+ * calculators per FD column and per thread, forcesmith_functor.cpp:73,146,178)
+ * and forwards each erased method to the entry point named below.
+ *
+ * Conventions
+ *  - Every function returns 0 on success or a negative FS_ERR_* code; the
+ *    message is available from fs_last_error(ctx). No exception crosses the
+ *    ABI. Output buffers are caller-owned host memory unless stated; device
+ *    memory is owned by the context.
+ *  - Calls on one context are serialised by an internal mutex, so the
+ *    reference's concurrent per-configuration calls
+ *    (forcesmith_functor.cpp:43-51,127-131) are safe. Each context owns one
+ *    HIP stream on its device.
+ *  - cell[9c + 3k + a]: component a of lattice vector k of configuration c
+ *    (the reference's "X","Y","Z", stored as box columns,
+ *    src/io/config_reader.cpp:58-66). Stress in and out of the residual is in
+ *    the order xx, yy, zz, xy, xz, yz (include/forcesmith/core/voigt.hpp:9-13);
+ *    full 3x3 stresses are row-major.
+ */
+#ifndef FORCESMITH_AMD_H
+#define FORCESMITH_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FS_ABI_VERSION 1
+
+enum fs_status {
+  FS_OK = 0,
+  FS_ERR_ARG = -1,         /* invalid argument or shape */
+  FS_ERR_STATE = -2,       /* call out of order (no dataset / model / prepare) */
+  FS_ERR_HIP = -3,         /* HIP runtime error */
+  FS_ERR_NOMEM = -4,       /* device allocation failed */
+  FS_ERR_UNSUPPORTED = -5, /* feature not available for this model kind */
+  FS_ERR_NUMERIC = -6      /* non-finite result or failed factorisation */
+};
+
+enum fs_model_kind { FS_EAM = 0, FS_ADP = 1, FS_SW = 2 };
+
+/* FS_MIXED: FP32 pair/density/spline arithmetic, FP64 per-configuration
+ * energy/stress accumulation, FP64 normal equations and LM solve.
+ * FS_FP64: every step in FP64 (parity mode). */
+enum fs_precision { FS_MIXED = 0, FS_FP64 = 1 };
+
+typedef struct fs_opts {
+  int32_t device;    /* HIP device ordinal */
+  int32_t precision; /* enum fs_precision */
+  int32_t reserved[6];
+} fs_opts;
+
+typedef struct fs_ctx fs_ctx;
+
+int fs_abi_version(void);
+
+/* Context lifetime. fs_ctx_retain/fs_ctx_release implement the shared handle
+ * the reference's value-semantics calculator copies need
+ * (force_calculator.hpp:84-104: copy constructor/assignment). */
+int fs_ctx_create(const fs_opts *opts, fs_ctx **out);
+int fs_ctx_retain(fs_ctx *ctx);
+int fs_ctx_release(fs_ctx *ctx);
+const char *fs_last_error(const fs_ctx *ctx);
+
+/* Dataset: replaces the std::span<Configuration> handed to prepare()
+ * (CForceCalcTE has_prepare, force_calculator.hpp:74) and the per-config
+ * reference data read by write_config_residuals (residual_layout.hpp:37-54). */
+int fs_upload_dataset(fs_ctx *ctx, int32_t n_configs, const int64_t *atom_off,
+                      const int32_t *types, const double *pos,
+                      const double *ref_force, const double *cell,
+                      const double *ref_energy, const double *ref_stress);
+
+/* Tabulated EAM/ADP model: tables in the reference declaration order
+ * (eam_force.hpp:24-32, adp_force.hpp:37-51), knot grid of make_tabulated
+ * (src/io/potential_reader.cpp:162-191). Replaces constructing the
+ * EAMForceCalculator / ADPForceCalculator value. */
+int fs_set_tabulated_model(fs_ctx *ctx, int32_t kind, int32_t ntypes,
+                           int32_t ntab, const int32_t *n_knots,
+                           const double *rmin, const double *rmax,
+                           const double *y, const uint8_t *fixed);
+/* Stillinger-Weber: pair slots of (A,B,p,q,delta,a1,gamma,a2) and lambda[T*P]
+ * (stiweb_force.hpp:19-62). */
+int fs_set_sw_model(fs_ctx *ctx, int32_t ntypes, const double *pair_params,
+                    const uint8_t *pair_fixed, const double *lambda,
+                    const uint8_t *lambda_fixed);
+/* OptimizerOptions energy_weight / stress_weight (optimizer.hpp:11-15). */
+int fs_set_weights(fs_ctx *ctx, double energy_weight, double stress_weight);
+
+/* CFittable surface (fit_params.hpp:24-30; TableParams
+ * force_calculator_concept.hpp:171-208). */
+int fs_param_count(fs_ctx *ctx, int32_t *count);
+int fs_gather_params(fs_ctx *ctx, double *x);
+int fs_scatter_params(fs_ctx *ctx, const double *x);
+int fs_gather_bounds(fs_ctx *ctx, double *lo, double *hi);
+int fs_max_cutoff(fs_ctx *ctx, double *rcut);     /* max_cutoff() */
+int fs_residual_count(fs_ctx *ctx, int64_t *rows); /* count_residuals */
+int fs_row_offsets(fs_ctx *ctx, int64_t *row_off); /* functor row_offset_ */
+
+/* prepare(): GPU cell-list neighbour build whose pair set, order and FP64
+ * displacements equal the reference brute force (neighbor_list.cpp:59-92). */
+int fs_prepare(fs_ctx *ctx);
+int fs_neighbor_count(fs_ctx *ctx, int64_t *pairs);
+int fs_get_neighbor_list(fs_ctx *ctx, int64_t *pair_off, int32_t *j,
+                         int32_t *img, double *d);
+
+/* eval_forces over every configuration (EAMForceCalculator::eval_forces
+ * eam_force.cpp:45-102 and siblings): forces[3N], energy[C], stress9[9C]
+ * (virial / volume, eval_scope.hpp:14-31), limit[C]. Any pointer may be NULL. */
+int fs_eval_forces(fs_ctx *ctx, double *forces, double *energy,
+                   double *stress9, double *limit);
+/* eval_cached(cache_index, forces, energy, stress) for one configuration
+ * (force_calculator.hpp:75-76) from the last fs_eval_forces/residual. */
+int fs_eval_config(fs_ctx *ctx, int32_t config, double *forces,
+                   double *energy, double *stress9, double *limit);
+/* Residual vector F(x) at the current parameters (ForcesmithFunctor::
+ * operator(), forcesmith_functor.cpp:221-231); F may be NULL (device only).
+ * cost (may be NULL) receives |F|^2. */
+int fs_eval_residual(fs_ctx *ctx, double *F, double *cost);
+/* Jacobian dF/dx, analytic per-knot (per-parameter for SW), FP32, row-major
+ * R x D. Replaces the central-difference fallback of
+ * ForcesmithFunctor::df (forcesmith_functor.cpp:252-292; fd_jacobian.hpp:
+ * 37-49) and eval_cached_jacobian (force_calculator.hpp:77-78). J may be
+ * NULL (device only). */
+int fs_eval_jacobian(fs_ctx *ctx, float *J);
+/* FP64 normal equations of the last residual/Jacobian: JtJ[D*D], Jtr[D]. */
+int fs_normal_equations(fs_ctx *ctx, double *JtJ, double *Jtr, double *cost);
+
+/* Levenberg-Marquardt on the GPU (FP64 normal equations, damped Cholesky);
+ * replaces Solver::minimize with EigenLMSolver (solver.cpp:67-85).
+ * x is read and written; cost_hist[max_iter] (may be NULL). */
+int fs_lm_minimize(fs_ctx *ctx, double *x, int32_t max_iter,
+                   double *cost_hist, int32_t *accepted);
+
+/* Exact cost |F(x_m)|^2 of np parameter vectors X[np*D] (row-major), the
+ * per-member cost of BoostDESolver (solver.cpp:143-146), batched. */
+int fs_population_cost(fs_ctx *ctx, const double *X, int32_t np,
+                       double *cost);
+
+typedef struct fs_de_opts {
+  int32_t np;          /* population size (reference: 15 * D) */
+  int32_t generations; /* max generations */
+  double F;            /* mutation factor */
+  double CR;           /* crossover probability */
+  uint64_t seed;
+  int32_t surrogate;   /* 0 = plain exact-judged DE, 1 = MLP-screened */
+  int32_t oversample;  /* trial candidates per member when screened (>= 1) */
+  int32_t retrain_every;
+  int32_t hidden;      /* MLP width */
+  double target_cost;  /* stop when best <= target (<= 0: run all) */
+  double rank_tau_min; /* fallback when Kendall tau of the surrogate drops */
+  int32_t reserved[4];
+} fs_de_opts;
+/* Differential evolution with the whole population evaluated per batched
+ * launch; replaces BoostDESolver::minimize (solver.cpp:113-155). lo/hi: box;
+ * x: initial guess in, best member out; hist[generations] best cost. */
+int fs_de_minimize(fs_ctx *ctx, const fs_de_opts *opts, const double *lo,
+                   const double *hi, double *x, double *best_cost,
+                   double *hist, int32_t *generations_run);
+
+/* Device views and tuning (no reference counterpart: measurement hooks). */
+enum fs_buffer {
+  FS_BUF_RESIDUAL = 0, /* double[R] */
+  FS_BUF_JACOBIAN = 1, /* float[R*D] */
+  FS_BUF_JTJ = 2,      /* double[D*D] */
+  FS_BUF_JTR = 3       /* double[D] */
+};
+int fs_device_buffer(fs_ctx *ctx, int32_t which, void **ptr, int64_t *bytes);
+int fs_synchronize(fs_ctx *ctx);
+/* Launch shape: lanes per atom (4..64), blocks per CU for persistent grids.
+ * fs_autotune times the candidates on the resident dataset and keeps the
+ * fastest. */
+int fs_set_launch(fs_ctx *ctx, int32_t lanes_per_atom, int32_t blocks_per_cu);
+int fs_autotune(fs_ctx *ctx, int32_t *lanes_per_atom);
+/* Average kernel time (ms) of the named kernel family over the last
+ * fs_time_* window, measured with HIP events on the context stream. */
+int fs_time_step(fs_ctx *ctx, int32_t what, int32_t reps, double *ms_per_rep);
+
+/* MLP surrogate kernels on host buffers (no reference counterpart; the
+ * surrogate is new functionality that screens DE trial vectors). params is
+ * one FP32 buffer [W1 (h x d) | b1 (h) | W2 (h x h) | b2 (h) | W3 (h) | b3];
+ * products run as BF16 matrix-core operations with FP32 accumulation, GELU in
+ * its tanh form. fs_mlp_loss_grad returns the gradient of
+ * mean 0.5 (y - t)^2 in the same layout. */
+int fs_mlp_forward(fs_ctx *ctx, int32_t n, int32_t d, int32_t h,
+                   const float *params, const float *X, float *y);
+int fs_mlp_loss_grad(fs_ctx *ctx, int32_t n, int32_t d, int32_t h,
+                     const float *params, const float *X, const float *t,
+                     float *grad, float *loss);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

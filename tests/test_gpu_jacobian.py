@@ -1,0 +1,99 @@
+"""GPU analytic Jacobian, FP64 normal equations and LM against the oracle.
+
+Bars: the FP32 Jacobian within max |J - J_oracle| / (1 + |J_oracle|) <= 2e-4
+(the reference FD metric, test_param_jacobian.cpp:181-193, at FP32 rounding);
+J^T J and J^T F within 1e-4 of the oracle's FP64 products relative to their
+largest entry; LM cost history non-increasing and its final cost within 1e-3
+of the oracle LM after the same number of iterations.
+"""
+import numpy as np
+import pytest
+
+import forcesmith_amd as fa
+import oracle
+from forcesmith_amd import datasets as dsm
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def wl():
+    w4 = dsm.config4(n_configs=6)
+    w4.model.stress_weight = 0.4
+    return {
+        "eam1": dsm.config1(n_configs=3),
+        "eam3s": dsm.config2(n_configs=6, nmin=32, nmax=300),
+        "sw": w4,
+    }
+
+
+def metric(a, b):
+    return float(np.max(np.abs(a - b) / (1.0 + np.abs(b))))
+
+
+@pytest.mark.parametrize("name", ["eam1", "eam3s", "sw"])
+def test_jacobian_matches_oracle(wl, name):
+    w = wl[name]
+    print(f"jac {name}", flush=True)
+    o = oracle.Oracle(w.dataset, w.model)
+    o.prepare()
+    x = o.gather()
+    Jo = o.jacobian(x)
+    g = fa.GpuCalculator(w.dataset, w.model)
+    g.prepare()
+    Jg = g.jacobian(x).astype(np.float64)
+    m = metric(Jg, Jo)
+    print(f"[{name}] J metric {m:.3e} shape {Jg.shape}", flush=True)
+    assert Jg.shape == Jo.shape
+    assert m <= 2e-4
+
+
+def test_jacobian_rows_by_kind(wl):
+    # force, energy, stress and limit rows are each exercised
+    w = wl["eam3s"]
+    o = oracle.Oracle(w.dataset, w.model)
+    o.prepare()
+    x = o.gather()
+    Jo = o.jacobian(x)
+    g = fa.GpuCalculator(w.dataset, w.model)
+    g.prepare()
+    Jg = g.jacobian(x).astype(np.float64)
+    ro = o.row_offsets()
+    sizes = w.dataset.config_sizes()
+    for c in range(len(sizes)):
+        e = ro[c] + 3 * sizes[c]
+        for r in range(e, ro[c + 1]):
+            assert metric(Jg[r], Jo[r]) <= 2e-4, (c, r - e)
+
+
+def test_normal_equations(wl):
+    w = wl["eam1"]
+    o = oracle.Oracle(w.dataset, w.model)
+    o.prepare()
+    x = o.gather()
+    Jo = o.jacobian(x)
+    Fo = o.residual(x)
+    g = fa.GpuCalculator(w.dataset, w.model)
+    g.prepare()
+    A, b, c = g.normal_equations()
+    Ao, bo = Jo.T @ Jo, Jo.T @ Fo
+    assert np.max(np.abs(A - Ao)) <= 1e-4 * np.max(np.abs(Ao))
+    assert np.max(np.abs(b - bo)) <= 1e-4 * np.max(np.abs(bo))
+    assert np.array_equal(A, A.T)
+    assert abs(c - Fo @ Fo) <= 1e-6 * (Fo @ Fo)
+
+
+def test_lm_tracks_oracle(wl):
+    w = wl["eam1"]
+    o = oracle.Oracle(w.dataset, w.model)
+    o.prepare()
+    x0 = o.gather()
+    xo, ho, _ = o.lm(x0, 8)
+    g = fa.GpuCalculator(w.dataset, w.model)
+    g.prepare()
+    xg, hg, acc = g.lm(x0, 8)
+    print("oracle", ho, "\ngpu", hg, flush=True)
+    assert acc > 0
+    assert np.all(np.diff(hg) <= 0)
+    assert abs(hg[-1] - ho[-1]) <= 1e-3 * ho[-1]
+    assert abs(g.cost(xg) - hg[-1]) <= 1e-6 * hg[-1]
