@@ -67,8 +67,8 @@ def cpu_baseline(w, x, seconds, threads):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU: --gpus {a.gpus} is not supported (the sharded design is "
-              "described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU: --gpus {a.gpus} is not supported; this build runs the "
+              "evaluator on a single device", file=sys.stderr)
         sys.exit(2)
     import numpy as np
 

@@ -363,7 +363,7 @@ def mlp_hidden(n_params: int, d: int) -> int:
     # n = h*d + h + h*h + h + h + 1
     a, b, c = 1.0, d + 3.0, 1.0 - n_params
     h = int(round((-b + (b * b - 4 * a * c) ** 0.5) / 2))
-    assert h * d + h + h * h + 3 * h + 1 == n_params
+    assert h * d + h * h + 3 * h + 1 == n_params
     return h
 
 

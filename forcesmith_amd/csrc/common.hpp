@@ -98,7 +98,7 @@ struct TableDesc {
   int32_t ivl;    // first interval in the packed interval arrays
   int32_t linear; // n < 4: piecewise linear (no slopes)
   double x0, xn, h, inv_h;
-  float x0f, inv_hf, hf, pad;
+  float x0f, inv_hf, hf, inv_hlo; // FP32 grid; inv_hf + inv_hlo carries 1/h to ~1e-14
 };
 
 struct CfgGrid {
@@ -168,7 +168,8 @@ struct Context {
   int D = 0, nknots = 0, nivl = 0;
   DBuf<TableDesc> tabs;
   DBuf<double> tab_x, tab_y, tab_s, tab_S; // S: n*5 band of ds/dy per table
-  DBuf<float4> ivl32;                       // per interval {y_k, s_k, y_k+1, s_k+1}
+  DBuf<float4> ivl32; // per interval k: power form {a, b, c, d} as FP32 hi at 2k, FP32 lo at 2k+1
+  DBuf<double> ivl64; // per interval k: the same power form in FP64 at 4k .. 4k+3
   DBuf<int32_t> col_of, knot_of;
   // SW
   std::vector<double> h_sw, h_lam;

@@ -385,6 +385,7 @@ int fs_set_tabulated_model(fs_ctx *ctx, int32_t kind, int32_t ntypes, int32_t nt
       T.x0f = float(T.x0);
       T.inv_hf = float(T.inv_h);
       T.hf = float(h);
+      T.inv_hlo = float(T.inv_h - double(T.inv_hf));
       for (int k = 0; k < n; ++k) {
         c.h_y.push_back(y[off + k]);
         c.h_fixed.push_back(fixed[off + k] ? 1 : 0);
@@ -394,7 +395,7 @@ int fs_set_tabulated_model(fs_ctx *ctx, int32_t kind, int32_t ntypes, int32_t nt
     }
     c.nknots = off;
     c.nivl = ivl;
-    FS_REQUIRE(size_t(ntab) * sizeof(TableDesc) + size_t(ivl) * sizeof(float4) <= 48 * 1024,
+    FS_REQUIRE(size_t(ntab) * sizeof(TableDesc) + 2 * size_t(ivl) * sizeof(float4) <= 48 * 1024,
                FS_ERR_UNSUPPORTED, "knot tables exceed the 48 KiB LDS staging budget");
     c.h_col_of.assign(off, -1);
     c.h_knot_of.clear();
@@ -409,7 +410,8 @@ int fs_set_tabulated_model(fs_ctx *ctx, int32_t kind, int32_t ntypes, int32_t nt
     c.tab_y.alloc(off);
     c.tab_s.alloc(off);
     c.tab_S.alloc(size_t(off) * 5);
-    c.ivl32.alloc(std::max(ivl, 1));
+    c.ivl32.alloc(2 * size_t(std::max(ivl, 1)));
+    c.ivl64.alloc(4 * size_t(std::max(ivl, 1)));
     upload(c, c.col_of, c.h_col_of);
     upload(c, c.knot_of, c.h_knot_of);
     c.have_model = true;

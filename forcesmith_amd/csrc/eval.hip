@@ -56,7 +56,7 @@ __device__ inline double Mext(const double *x, const double *y, int n, int q) {
 __device__ inline double sgnd(double u) { return double((u > 0) - (u < 0)); }
 
 __device__ void table_slopes(const TableDesc &T, const double *x, const double *y, double *s,
-                             double *S, float4 *ivl) {
+                             double *S, float4 *ivl, double *p64) {
   const int n = T.n;
   if (n >= 4) {
     double m2 = (y[3] - y[2]) / (x[3] - x[2]);
@@ -123,17 +123,42 @@ __device__ void table_slopes(const TableDesc &T, const double *x, const double *
       for (int q = 0; q < 5; ++q) S[5 * k + q] = 0.0;
     }
   }
-  for (int k = 0; k + 1 < n; ++k)
-    ivl[T.ivl + k] = make_float4(float(y[k]), float(s[k]), float(y[k + 1]), float(s[k + 1]));
+  // power form a + b t + c t^2 + d t^3 in t = (r - x_k) / h, formed in FP64;
+  // the FP32 copy is a hi/lo pair per coefficient
+  for (int k = 0; k + 1 < n; ++k) {
+    const double y0 = y[k], y1 = y[k + 1];
+    double q[4];
+    if (T.linear) {
+      q[0] = y0;
+      q[1] = y1 - y0;
+      q[2] = q[3] = 0.0;
+    } else {
+      const double dx = x[k + 1] - x[k];
+      const double b = dx * s[k], e = dx * s[k + 1];
+      q[0] = y0;
+      q[1] = b;
+      q[2] = 3.0 * (y1 - y0) - 2.0 * b - e;
+      q[3] = 2.0 * (y0 - y1) + b + e;
+    }
+    float hi[4], lo[4];
+    for (int m = 0; m < 4; ++m) {
+      hi[m] = float(q[m]);
+      lo[m] = float(q[m] - double(hi[m]));
+      p64[4 * (T.ivl + k) + m] = q[m];
+    }
+    ivl[2 * (T.ivl + k)] = make_float4(hi[0], hi[1], hi[2], hi[3]);
+    ivl[2 * (T.ivl + k) + 1] = make_float4(lo[0], lo[1], lo[2], lo[3]);
+  }
 }
 
 __global__ void k_slopes(int ntab, const TableDesc *__restrict__ tabs, const double *__restrict__ x,
                          const double *__restrict__ y, double *__restrict__ s,
-                         double *__restrict__ S, float4 *__restrict__ ivl) {
+                         double *__restrict__ S, float4 *__restrict__ ivl,
+                         double *__restrict__ p64) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ntab) return;
   const TableDesc T = tabs[t];
-  table_slopes(T, x + T.off, y + T.off, s + T.off, S + 5 * T.off, ivl);
+  table_slopes(T, x + T.off, y + T.off, s + T.off, S + 5 * T.off, ivl, p64);
 }
 
 __global__ void k_scatter(int D, const double *__restrict__ xv, const int32_t *__restrict__ knot_of,
@@ -157,6 +182,7 @@ struct EvalArgs {
   const TableDesc *tabs;
   int ntab, nivl;
   const float4 *ivl32;
+  const double *ivl64;
   const double *tx, *ty, *ts;
   int ntypes, npair;
   const double *ref_force;
@@ -177,7 +203,7 @@ __device__ inline void stage_tables(const EvalArgs &a, TableDesc *sT, float4 *sI
   float4 *dst = reinterpret_cast<float4 *>(sT);
   const int nq = a.ntab * int(sizeof(TableDesc) / sizeof(float4));
   for (int k = threadIdx.x; k < nq; k += blockDim.x) dst[k] = src[k];
-  for (int k = threadIdx.x; k < a.nivl; k += blockDim.x) sI[k] = a.ivl32[k];
+  for (int k = threadIdx.x; k < 2 * a.nivl; k += blockDim.x) sI[k] = a.ivl32[k];
   __syncthreads();
 }
 
@@ -185,22 +211,38 @@ __device__ inline void stage_tables(const EvalArgs &a, TableDesc *sT, float4 *sI
 struct Tabs {
   const TableDesc *T;
   const float4 *I;
-  const double *x, *y, *s;
+  const double *x, *y, *s, *P;
+  // FP64 cached-site evaluation (parity mode and exact edge pairs)
   __device__ void eval64(int t, double r, double &v, double &d) const {
     const TableDesc D = T[t];
     spl_site64(D, x + D.off, y + D.off, s + D.off, r, v, d);
   }
-  __device__ void eval32(int t, float r, float &v, float &d) const {
+  // FP32 value and derivative, located from the FP64 distance
+  __device__ void eval32(int t, double r, float &v, float &d) const {
     const TableDesc D = T[t];
-    float tt;
-    const int k = spl_interval32(D, r, tt);
-    spl_eval32(D, I[D.ivl + k], tt, v, d);
+    double u;
+    const int k = spl_interval64(D, r, u);
+    spl_eval32(D, I, k, float(u), v, d);
   }
-  __device__ void deriv32(int t, float r, float &d) const {
+  __device__ void deriv32(int t, double r, float &d) const {
     const TableDesc D = T[t];
-    float tt;
-    const int k = spl_interval32(D, r, tt);
-    spl_deriv32(D, I[D.ivl + k], tt, d);
+    double u;
+    const int k = spl_interval64(D, r, u);
+    spl_deriv32(D, I, k, float(u), d);
+  }
+  // FP32 derivative with the FP64 value of the same interval
+  __device__ void deriv32_value64(int t, double r, float &d, double &v) const {
+    const TableDesc D = T[t];
+    double u;
+    const int k = spl_interval64(D, r, u);
+    spl_deriv32(D, I, k, float(u), d);
+    v = spl_value64(D, P, k, u);
+  }
+  __device__ double value64(int t, double r) const {
+    const TableDesc D = T[t];
+    double u;
+    const int k = spl_interval64(D, r, u);
+    return spl_value64(D, P, k, u);
   }
 };
 
@@ -238,9 +280,7 @@ __device__ inline void density_pair(const Tabs &tb, const EvalArgs &a, int ti, u
       tb.eval64(a.npair + tj, double(r), vv, dd);
       v = R(vv);
     } else {
-      float vv, dd;
-      tb.eval32(a.npair + tj, float(r), vv, dd);
-      v = R(vv);
+      v = R(tb.value64(a.npair + tj, double(r)));
     }
     acc.rho += v;
     (void)d;
@@ -255,7 +295,7 @@ __device__ inline void density_pair(const Tabs &tb, const EvalArgs &a, int ti, u
         u = R(vv);
       } else {
         float vv, dd;
-        tb.eval32(a.npair + 2 * a.ntypes + o, float(r), vv, dd);
+        tb.eval32(a.npair + 2 * a.ntypes + o, double(r), vv, dd);
         u = R(vv);
       }
       acc.m0 += u * d0;
@@ -270,7 +310,7 @@ __device__ inline void density_pair(const Tabs &tb, const EvalArgs &a, int ti, u
         w = R(vv);
       } else {
         float vv, dd;
-        tb.eval32(2 * a.npair + 2 * a.ntypes + o, float(r), vv, dd);
+        tb.eval32(2 * a.npair + 2 * a.ntypes + o, double(r), vv, dd);
         w = R(vv);
       }
       acc.l[0] += w * (d0 * d0);
@@ -292,7 +332,7 @@ __global__ __launch_bounds__(256) void k_density(EvalArgs a) {
   TableDesc *sT = reinterpret_cast<TableDesc *>(smem);
   float4 *sI = smem + a.ntab * int(sizeof(TableDesc) / sizeof(float4));
   stage_tables(a, sT, sI);
-  const Tabs tb{sT, sI, a.tx, a.ty, a.ts};
+  const Tabs tb{sT, sI, a.tx, a.ty, a.ts, a.ivl64};
   const int sub = threadIdx.x / LPA, lane = threadIdx.x % LPA;
   for (int tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
     if (sub >= a.tile_na[tile]) continue;
@@ -307,9 +347,13 @@ __global__ __launch_bounds__(256) void k_density(EvalArgs a) {
       } else {
         const uint32_t b = a.bits[p];
         if (b & kFlagGj) {
-          double v, d;
-          tb.eval64(a.npair + pj_type(b), a.r64[p], v, d);
-          acc.rho += v;
+          if constexpr (F64) {
+            double v, d;
+            tb.eval64(a.npair + pj_type(b), a.r64[p], v, d);
+            acc.rho += v;
+          } else {
+            acc.rho += tb.value64(a.npair + pj_type(b), a.r64[p]);
+          }
         }
       }
     }
@@ -406,15 +450,33 @@ __device__ inline void force_pair(const Tabs &tb, const EvalArgs &a, int ti, int
       d = R(dd);
     } else {
       float vv, dd;
-      tb.eval32(t, float(r), vv, dd);
+      tb.eval32(t, r64, vv, dd);
       v = R(vv);
       d = R(dd);
     }
   };
-  R tmp;
-  if (b & kFlagPhi) ev(o, phi, dphi);
-  if (b & kFlagGj) ev(a.npair + tj, tmp, dgj);
-  if (b & kFlagGi) ev(a.npair + ti, tmp, dgi);
+  auto dg = [&](int t, R &d) {
+    if (EXACT) {
+      double vv, dd;
+      tb.eval64(t, double(r), vv, dd);
+      d = R(dd);
+    } else {
+      float dd;
+      tb.deriv32(t, r64, dd);
+      d = R(dd);
+    }
+  };
+  double phi64 = 0.0;
+  if (b & kFlagPhi) {
+    if (EXACT) ev(o, phi, dphi);
+    else {
+      float dd;
+      tb.deriv32_value64(o, r64, dd, phi64);
+      dphi = R(dd);
+    }
+  }
+  if (b & kFlagGj) dg(a.npair + tj, dgj);
+  if (b & kFlagGi) dg(a.npair + ti, dgi);
   const R Fpj = EXACT ? R(a.Fp[jg]) : R(a.Fp32[jg]);
   const R fs = (dphi + si.Fp * dgj + Fpj * dgi) * inv_r;
   R f0 = fs * d0, f1 = fs * d1, f2 = fs * d2;
@@ -447,14 +509,8 @@ __device__ inline void force_pair(const Tabs &tb, const EvalArgs &a, int ti, int
   acc.f0 += f0;
   acc.f1 += f1;
   acc.f2 += f2;
-  if (!EXACT && (b & kFlagPhi)) {
-    // pair energy value in FP64 from the FP64 distance (see k_density)
-    double pv, pd;
-    tb.eval64(o, r64, pv, pd);
-    acc.ep += 0.5 * pv;
-  } else {
-    acc.ep += 0.5 * double(phi);
-  }
+  // pair energy value in FP64 from the FP64 distance in both modes
+  acc.ep += EXACT ? 0.5 * double(phi) : 0.5 * phi64;
   const double hd[3] = {0.5 * double(d0), 0.5 * double(d1), 0.5 * double(d2)};
   const double fd[3] = {double(f0), double(f1), double(f2)};
 #pragma unroll
@@ -474,7 +530,7 @@ __global__ __launch_bounds__(256) void k_force(EvalArgs a) {
   float4 *sI = smem + a.ntab * int(sizeof(TableDesc) / sizeof(float4));
   __shared__ double sRed[APB][kPart];
   stage_tables(a, sT, sI);
-  const Tabs tb{sT, sI, a.tx, a.ty, a.ts};
+  const Tabs tb{sT, sI, a.tx, a.ty, a.ts, a.ivl64};
   const int sub = threadIdx.x / LPA, lane = threadIdx.x % LPA;
   for (int tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
     const int na = a.tile_na[tile];
@@ -622,6 +678,7 @@ EvalArgs make_args(Context &c) {
   a.ntab = c.ntab;
   a.nivl = c.nivl;
   a.ivl32 = c.ivl32.p;
+  a.ivl64 = c.ivl64.p;
   a.tx = c.tab_x.p;
   a.ty = c.tab_y.p;
   a.ts = c.tab_s.p;
@@ -672,7 +729,7 @@ void launch_scatter_params(Context &c, const double *d_x) {
 void launch_slopes(Context &c) {
   if (c.kind == FS_SW || c.ntab == 0) return;
   hipLaunchKernelGGL(k_slopes, dim3((c.ntab + 63) / 64), dim3(64), 0, c.stream, c.ntab, c.tabs.p,
-                     c.tab_x.p, c.tab_y.p, c.tab_s.p, c.tab_S.p, c.ivl32.p);
+                     c.tab_x.p, c.tab_y.p, c.tab_s.p, c.tab_S.p, c.ivl32.p, c.ivl64.p);
   FS_HIP(hipGetLastError());
 }
 
@@ -681,7 +738,7 @@ void launch_eval(Context &c, bool) {
     launch_sw_eval(c);
   } else {
     const EvalArgs a = make_args(c);
-    const size_t lds = size_t(c.ntab) * sizeof(TableDesc) + size_t(c.nivl) * sizeof(float4);
+    const size_t lds = size_t(c.ntab) * sizeof(TableDesc) + 2 * size_t(c.nivl) * sizeof(float4);
     const int grid = std::max(1, std::min(c.ntiles, c.blocks_per_cu * c.cu_count));
     if (c.ntiles > 0) {
       const bool f64 = c.precision == FS_FP64;

@@ -351,14 +351,14 @@ __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots) {
               r.kgj = pair_basis(T, rr, dummy, r.bgj);
               float tt, v;
               const int k = spl_interval32(T, rr, tt);
-              spl_eval32(T, a.ivl32[T.ivl + k], tt, v, r.dgj);
+              spl_eval32(T, a.ivl32, k, tt, v, r.dgj);
             }
             if (b & kFlagGi) {
               const TableDesc T = sT[a.npair + ti];
               r.kgi = pair_basis(T, rr, dummy, r.bgi);
               float tt, v;
               const int k = spl_interval32(T, rr, tt);
-              spl_eval32(T, a.ivl32[T.ivl + k], tt, v, r.dgi);
+              spl_eval32(T, a.ivl32, k, tt, v, r.dgi);
             }
             r.fpj = a.Fp32[r.jg];
             r.kfj = a.emb_k[r.jg];

@@ -8,7 +8,8 @@
 //  * FP64 uncached form for the embedding F(rho) (spline.cpp:215-279),
 //    extrapolating with the end secant outside the span.
 //  * FP32 form for the mixed-precision bulk: interval from the uniform grid,
-//    coefficients {y_k, s_k, y_k+1, s_k+1} read as one float4 from LDS.
+//    power-form coefficients {a, b, c, d} of the interval (formed in FP64,
+//    held as FP32 hi/lo pairs) read from LDS; linear intervals carry c = d = 0.
 #pragma once
 
 #include "common.hpp"
@@ -113,7 +114,11 @@ __device__ inline void spl_uncached64(const TableDesc &T, const double *x, const
   }
 }
 
-// FP32 bulk evaluation; q = {y_k, s_k, y_k+1, s_k+1} of interval k
+// FP32 bulk evaluation. Interval k of table T holds the power form
+// v = a + b t + c t^2 + d t^3, t = (r - x_k) / h, split as FP32 hi + lo
+// pairs (I[2k], I[2k+1]) so that the coefficients are exact to ~1e-14 while
+// every operation is FP32: rounding the coefficients themselves would bias all
+// pairs of an interval alike, a systematic ~1e-7 relative force error.
 __device__ inline int spl_interval32(const TableDesc &T, float r, float &t) {
   const float u = (r - T.x0f) * T.inv_hf;
   int k = int(floorf(u));
@@ -122,28 +127,38 @@ __device__ inline int spl_interval32(const TableDesc &T, float r, float &t) {
   return k;
 }
 
-__device__ inline void spl_eval32(const TableDesc &T, float4 q, float t, float &v, float &d) {
-  if (T.linear) {
-    v = q.x + t * (q.z - q.x);
-    d = (q.z - q.x) * T.inv_hf;
-    return;
-  }
-  const float h = T.hf;
-  const float omt = 1.0f - t;
-  v = omt * omt * (q.x * (1.0f + 2.0f * t) + q.y * (h * t)) +
-      t * t * (q.z * (3.0f - 2.0f * t) + h * q.w * (t - 1.0f));
-  d = 6.0f * t * omt * (q.z - q.x) * T.inv_hf + omt * (1.0f - 3.0f * t) * q.y +
-      t * (3.0f * t - 2.0f) * q.w;
+// interval from the FP64 distance: the FP32 offset t carries no rounding of
+// x_0 or 1/h into the pair's position on the grid
+__device__ inline int spl_interval64(const TableDesc &T, double r, double &t) {
+  const double u = (r - T.x0) * T.inv_h;
+  int k = int(floor(u));
+  k = k < 0 ? 0 : (k > T.n - 2 ? T.n - 2 : k);
+  t = u - double(k);
+  return k;
 }
 
-__device__ inline void spl_deriv32(const TableDesc &T, float4 q, float t, float &d) {
-  if (T.linear) {
-    d = (q.z - q.x) * T.inv_hf;
-    return;
-  }
-  const float omt = 1.0f - t;
-  d = 6.0f * t * omt * (q.z - q.x) * T.inv_hf + omt * (1.0f - 3.0f * t) * q.y +
-      t * (3.0f * t - 2.0f) * q.w;
+__device__ inline float spl_dscale(const TableDesc &T, float dt) {
+  return fmaf(dt, T.inv_hf, dt * T.inv_hlo);
+}
+
+__device__ inline void spl_eval32(const TableDesc &T, const float4 *I, int k, float t, float &v,
+                                  float &d) {
+  const float4 q = I[2 * (T.ivl + k)], e = I[2 * (T.ivl + k) + 1];
+  v = (q.x + t * (q.y + t * (q.z + t * q.w))) + (e.x + t * (e.y + t * (e.z + t * e.w)));
+  const float dt = (q.y + t * (2.0f * q.z + 3.0f * t * q.w)) + (e.y + t * (2.0f * e.z + 3.0f * t * e.w));
+  d = spl_dscale(T, dt);
+}
+
+__device__ inline void spl_deriv32(const TableDesc &T, const float4 *I, int k, float t, float &d) {
+  const float4 q = I[2 * (T.ivl + k)], e = I[2 * (T.ivl + k) + 1];
+  const float dt = (q.y + t * (2.0f * q.z + 3.0f * t * q.w)) + (e.y + t * (2.0f * e.z + 3.0f * t * e.w));
+  d = spl_dscale(T, dt);
+}
+
+// FP64 value on the same power form (energy and density terms)
+__device__ inline double spl_value64(const TableDesc &T, const double *P, int k, double t) {
+  const double *q = P + 4 * size_t(T.ivl + k);
+  return q[0] + t * (q[1] + t * (q[2] + t * q[3]));
 }
 
 __device__ inline int ord_pair(int a, int b, int nt) {
