@@ -218,6 +218,9 @@ struct Context {
   DBuf<int32_t> jt_cfg, jt_a0, jt_na, jt_ct0;
   int njt = 0;
   DBuf<int4> jcols;
+  DBuf<float4> jrec; // pair records sorted by (partner type, r) per atom
+  DBuf<float> jr;    // their FP32 distances
+  bool jsort_valid = false;
   DBuf<float> Hs, emb_d, emb_v;
   DBuf<int32_t> emb_k, sw_csw, sw_clam;
   DBuf<double> sw_own, sw_ptan, sw_etan;

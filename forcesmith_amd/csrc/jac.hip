@@ -108,6 +108,8 @@ struct JacArgs {
   const int32_t *types;
   const float4 *rec;
   const uint32_t *bits;
+  const float4 *jrec; // pairs sorted by (partner type, r) per atom
+  const float *jr;    // their FP32 distances
   const TableDesc *tabs;
   const double *tx, *ty, *ts, *tS;
   const float4 *ivl32;
@@ -134,20 +136,80 @@ __device__ inline void stage(const JacArgs &a, TableDesc *sT, float *sS, int nk)
   __syncthreads();
 }
 
-struct DRec {
-  int32_t tab, k;
-  float v[4];
+// ---------------------------------------------------------------------------
+// Jacobian pair order. Every column's pairs are those whose interval k on the
+// column's table lies in [m - 3, m + 2]; with an atom's pairs sorted by
+// (partner type, r) they form one contiguous run per partner type, found by
+// binary search on the FP32 interval index (monotone in r). The FP32
+// distance stored here is the one every Jacobian kernel uses, so the search
+// and the evaluation agree exactly.
+__global__ void k_jac_sort(int64_t N, const int64_t *__restrict__ pair_off,
+                           const float4 *__restrict__ rec, float4 *__restrict__ jrec,
+                           float *__restrict__ jr) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int64_t p0 = pair_off[i], p1 = pair_off[i + 1];
+  for (int64_t p = p0; p < p1; ++p) {
+    const float4 q = rec[p];
+    const float r = sqrtf(q.x * q.x + q.y * q.y + q.z * q.z);
+    const int t = pj_type(__float_as_uint(q.w));
+    int64_t s = p;
+    while (s > p0) {
+      const int ts = pj_type(__float_as_uint(jrec[s - 1].w));
+      if (ts < t || (ts == t && jr[s - 1] <= r)) break;
+      jrec[s] = jrec[s - 1];
+      jr[s] = jr[s - 1];
+      --s;
+    }
+    jrec[s] = q;
+    jr[s] = r;
+  }
+}
+
+// per-wave staging of up to 64 pairs of one atom, structure of arrays
+struct Stage {
+  float r[64], u0[64], u1[64], u2[64];
+  float fpj[64], dgi[64];
+  int32_t jg[64], bits[64], kfj[64];
+  float eb[4][64];
 };
+
+// first index in [lo, hi) whose interval on table T is >= kmin
+__device__ inline int first_k_ge(const float *sr, int lo, int hi, const TableDesc &T, int kmin) {
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    float t;
+    if (spl_interval32(T, sr[mid], t) >= kmin) hi = mid;
+    else lo = mid + 1;
+  }
+  return lo;
+}
+
+// run of pairs in [s0, s1) whose interval on T lies in the band of knot m
+__device__ inline void band_run(const float *sr, int s0, int s1, const TableDesc &T, int m, int &lo,
+                                int &hi) {
+  lo = first_k_ge(sr, s0, s1, T, m - 3);
+  hi = first_k_ge(sr, lo, s1, T, m + 3);
+}
+
+// segment starts of the partner types in the staged chunk (invalid lanes
+// carry type kMaxTypes and sort last)
+__device__ inline void chunk_segments(int ty, int ntypes, int seg[kMaxTypes + 1]) {
+#pragma unroll
+  for (int t = 0; t <= kMaxTypes; ++t)
+    seg[t] = t <= ntypes ? int(__popcll(__ballot(ty < t))) : seg[ntypes];
+}
 
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_jac_dens(JacArgs a, int nknots) {
   extern __shared__ float4 smem[];
   TableDesc *sT = reinterpret_cast<TableDesc *>(smem);
   float *sS = reinterpret_cast<float *>(smem + a.ntab * int(sizeof(TableDesc) / sizeof(float4)));
-  DRec *sR = reinterpret_cast<DRec *>(sS + ((nknots * 5 + 3) & ~3));
+  float *sRall = sS + ((nknots * 5 + 3) & ~3);
   stage(a, sT, sS, nknots);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  DRec *myR = sR + wave * 64;
+  float *sr = sRall + wave * 128;
+  uint32_t *sb = reinterpret_cast<uint32_t *>(sr + 64);
   const int64_t gw = int64_t(blockIdx.x) * 4 + wave;
   const int64_t nw = int64_t(gridDim.x) * 4;
   // my density columns
@@ -168,29 +230,30 @@ __global__ __launch_bounds__(256) void k_jac_dens(JacArgs a, int nknots) {
     float H[kCPL] = {0, 0, 0, 0};
     for (int64_t pc = p0; pc < p1; pc += 64) {
       const int64_t p = pc + lane;
-      DRec r;
-      r.tab = -1;
-      r.k = 0;
+      int ty = kMaxTypes;
       if (p < p1) {
-        const float4 q = a.rec[p];
-        const uint32_t b = __float_as_uint(q.w);
-        if (b & kFlagGj) {
-          const float rr = sqrtf(q.x * q.x + q.y * q.y + q.z * q.z);
-          r.tab = a.npair + pj_type(b);
-          float bd[4];
-          r.k = pair_basis(sT[r.tab], rr, r.v, bd);
-        }
+        const uint32_t b = __float_as_uint(a.jrec[p].w);
+        sr[lane] = a.jr[p];
+        sb[lane] = b;
+        ty = pj_type(b);
       }
-      myR[lane] = r;
+      int seg[kMaxTypes + 1];
+      chunk_segments(ty, a.ntypes, seg);
       wave_sync();
-      const int np = int(min<int64_t>(64, p1 - pc));
-      for (int q = 0; q < np; ++q) {
-        const DRec r2 = myR[q];
-        if (r2.tab < 0) continue;
-        const TableDesc T = sT[r2.tab];
 #pragma unroll
-        for (int s = 0; s < kCPL; ++s)
-          if (ctab[s] == r2.tab) H[s] += band(sS, T, r2.k, cm[s], r2.v);
+      for (int s = 0; s < kCPL; ++s) {
+        if (ctab[s] < 0) continue;
+        const int t = ctab[s] - a.npair;
+        const TableDesc T = sT[ctab[s]];
+        int lo, hi;
+        band_run(sr, seg[t], seg[t + 1], T, cm[s], lo, hi);
+        for (int q = lo; q < hi; ++q) {
+          const uint32_t b = sb[q];
+          if (!(b & kFlagGj) || (b & kFlagTiny)) continue;
+          float bv[4], bd[4];
+          const int k = pair_basis(T, sr[q], bv, bd);
+          H[s] += band(sS, T, k, cm[s], bv);
+        }
       }
       wave_sync();
     }
@@ -263,24 +326,34 @@ __global__ __launch_bounds__(256) void k_jac_dens(JacArgs a, int nknots) {
 }
 
 // ---------------------------------------------------------------------------
-struct PRec {
-  int32_t jg, flags;
-  int32_t kphi, kgj, kgi, kfj;
-  float u[3], d[3];
-  float bphi[4], vphi[4], bgj[4], bgi[4], ebj[4];
-  float dgj, dgi, fpj, pad;
+// force / virial / energy partials of every column for one atom per wave
+struct ColAcc {
+  float f[3], s[6], e;
 };
+
+__device__ inline void add_pair(ColAcc &A, float c, float r, float u0, float u1, float u2) {
+  A.f[0] += c * u0;
+  A.f[1] += c * u1;
+  A.f[2] += c * u2;
+  const float cr = c * r;
+  A.s[0] += cr * u0 * u0;
+  A.s[1] += cr * u1 * u1;
+  A.s[2] += cr * u2 * u2;
+  A.s[3] += cr * u0 * u1;
+  A.s[4] += cr * u0 * u2;
+  A.s[5] += cr * u1 * u2;
+}
 
 __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots) {
   extern __shared__ float4 smem[];
   TableDesc *sT = reinterpret_cast<TableDesc *>(smem);
   float *sS = reinterpret_cast<float *>(smem + a.ntab * int(sizeof(TableDesc) / sizeof(float4)));
-  PRec *sR = reinterpret_cast<PRec *>(sS + ((nknots * 5 + 3) & ~3));
+  Stage *sAll = reinterpret_cast<Stage *>(sS + ((nknots * 5 + 3) & ~3));
   __shared__ double sPart[kJP][256];
   stage(a, sT, sS, nknots);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  PRec *myR = sR + wave * 64;
-  // lane-owned columns
+  Stage &P = sAll[wave];
+  // lane-owned columns; pair tables carry their two types in dc
   int ccol[kCPL], ctab[kCPL], cm[kCPL], ckind[kCPL], cdc[kCPL];
   for (int s = 0; s < kCPL; ++s) {
     const int c = lane + 64 * s;
@@ -313,111 +386,149 @@ __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots) {
       const int ti = a.types[i];
       const float Fpi = a.Fp32[i];
       const int64_t p0 = a.pair_off[i], p1 = a.pair_off[i + 1];
-      float acf[kCPL][3], acs[kCPL][6], ace[kCPL];
+      ColAcc A[kCPL];
 #pragma unroll
       for (int s = 0; s < kCPL; ++s) {
-        acf[s][0] = acf[s][1] = acf[s][2] = 0.0f;
-        for (int q = 0; q < 6; ++q) acs[s][q] = 0.0f;
-        ace[s] = 0.0f;
+        for (int q = 0; q < 3; ++q) A[s].f[q] = 0.0f;
+        for (int q = 0; q < 6; ++q) A[s].s[q] = 0.0f;
+        A[s].e = 0.0f;
       }
       float G[3] = {0, 0, 0}, Gs[6] = {0, 0, 0, 0, 0, 0};
+      // partner type of each lane-owned pair column for this centre type
+      int ptt[kCPL];
+#pragma unroll
+      for (int s = 0; s < kCPL; ++s) {
+        ptt[s] = -1;
+        if (ckind[s] == CK_PAIR) {
+          const int ta = cdc[s] & 0xff, tb = (cdc[s] >> 8) & 0xff;
+          ptt[s] = ti == ta ? tb : (ti == tb ? ta : -1);
+        }
+      }
       for (int64_t pc = p0; pc < p1; pc += 64) {
         const int64_t p = pc + lane;
-        PRec r;
-        r.flags = 0;
+        int ty = kMaxTypes;
         if (p < p1) {
-          const float4 q = a.rec[p];
+          const float4 q = a.jrec[p];
           const uint32_t b = __float_as_uint(q.w);
+          const float rr = a.jr[p];
+          const float inv_r = (b & kFlagTiny) ? 0.0f : 1.0f / rr;
+          const int tj = pj_type(b);
+          const int32_t jg = int32_t(cbase + pj_local(b));
+          ty = tj;
+          const float u0 = q.x * inv_r, u1 = q.y * inv_r, u2 = q.z * inv_r;
+          P.r[lane] = rr;
+          P.u0[lane] = u0;
+          P.u1[lane] = u1;
+          P.u2[lane] = u2;
+          P.bits[lane] = int32_t(b);
+          P.jg[lane] = jg;
+          P.fpj[lane] = a.Fp32[jg];
+          P.kfj[lane] = a.emb_k[jg];
+          const float4 e = reinterpret_cast<const float4 *>(a.emb_d)[jg];
+          P.eb[0][lane] = e.x;
+          P.eb[1][lane] = e.y;
+          P.eb[2][lane] = e.z;
+          P.eb[3][lane] = e.w;
+          float dgj = 0.0f, dgi = 0.0f;
           if (!(b & kFlagTiny)) {
-            const float rr = sqrtf(q.x * q.x + q.y * q.y + q.z * q.z);
-            const float inv_r = 1.0f / rr;
-            const int tj = pj_type(b);
-            r.flags = int32_t(b | 1u);
-            r.jg = int32_t(cbase + pj_local(b));
-            r.d[0] = q.x;
-            r.d[1] = q.y;
-            r.d[2] = q.z;
-            r.u[0] = q.x * inv_r;
-            r.u[1] = q.y * inv_r;
-            r.u[2] = q.z * inv_r;
-            float dummy[4];
-            r.kphi = (b & kFlagPhi) ? pair_basis(sT[ord_pair(ti, tj, a.ntypes)], rr, r.vphi, r.bphi) : -9;
-            r.dgj = 0.0f;
-            r.dgi = 0.0f;
-            r.kgj = -9;
-            r.kgi = -9;
             if (b & kFlagGj) {
               const TableDesc T = sT[a.npair + tj];
-              r.kgj = pair_basis(T, rr, dummy, r.bgj);
               float tt, v;
               const int k = spl_interval32(T, rr, tt);
-              spl_eval32(T, a.ivl32, k, tt, v, r.dgj);
+              spl_eval32(T, a.ivl32, k, tt, v, dgj);
             }
             if (b & kFlagGi) {
               const TableDesc T = sT[a.npair + ti];
-              r.kgi = pair_basis(T, rr, dummy, r.bgi);
               float tt, v;
               const int k = spl_interval32(T, rr, tt);
-              spl_eval32(T, a.ivl32, k, tt, v, r.dgi);
+              spl_eval32(T, a.ivl32, k, tt, v, dgi);
             }
-            r.fpj = a.Fp32[r.jg];
-            r.kfj = a.emb_k[r.jg];
-            for (int q2 = 0; q2 < 4; ++q2) r.ebj[q2] = a.emb_d[4 * int64_t(r.jg) + q2];
-            // G_i = sum_j g'_tj(r) u, virial counterpart sum_j g'_tj d_a u_b
-            G[0] += r.dgj * r.u[0];
-            G[1] += r.dgj * r.u[1];
-            G[2] += r.dgj * r.u[2];
-            Gs[0] += r.dgj * r.d[0] * r.u[0];
-            Gs[1] += r.dgj * r.d[1] * r.u[1];
-            Gs[2] += r.dgj * r.d[2] * r.u[2];
-            Gs[3] += r.dgj * r.d[0] * r.u[1];
-            Gs[4] += r.dgj * r.d[0] * r.u[2];
-            Gs[5] += r.dgj * r.d[1] * r.u[2];
           }
+          P.dgi[lane] = dgi;
+          // G_i = sum_j g'_tj(r) u, virial counterpart sum_j g'_tj r u u
+          G[0] += dgj * u0;
+          G[1] += dgj * u1;
+          G[2] += dgj * u2;
+          const float gr = dgj * rr;
+          Gs[0] += gr * u0 * u0;
+          Gs[1] += gr * u1 * u1;
+          Gs[2] += gr * u2 * u2;
+          Gs[3] += gr * u0 * u1;
+          Gs[4] += gr * u0 * u2;
+          Gs[5] += gr * u1 * u2;
         }
-        myR[lane] = r;
+        int seg[kMaxTypes + 1];
+        chunk_segments(ty, a.ntypes, seg);
         wave_sync();
         const int np = int(min<int64_t>(64, p1 - pc));
-        for (int q = 0; q < np; ++q) {
-          if (myR[q].flags == 0) continue;
-          const int tj = pj_type(uint32_t(myR[q].flags));
-          const int tphi = ord_pair(ti, tj, a.ntypes);
-          const int tgj = a.npair + tj, tgi = a.npair + ti;
-          const int temb = a.npair + a.ntypes + tj;
-          const int jg = myR[q].jg;
-          const float dgi = myR[q].dgi;
 #pragma unroll
-          for (int s = 0; s < kCPL; ++s) {
-            if (ckind[s] == CK_NONE) continue;
-            float contrib = 0.0f, en = 0.0f;
-            const int T = ctab[s];
-            if (ckind[s] == CK_PAIR) {
-              if (T == tphi && myR[q].kphi >= 0) {
-                const TableDesc D = sT[T];
-                contrib = band(sS, D, myR[q].kphi, cm[s], myR[q].bphi);
-                en = 0.5f * band(sS, D, myR[q].kphi, cm[s], myR[q].vphi);
-              }
-            } else if (ckind[s] == CK_DENS) {
-              const TableDesc D = sT[T];
-              if (T == tgj && myR[q].kgj >= 0) contrib += Fpi * band(sS, D, myR[q].kgj, cm[s], myR[q].bgj);
-              if (T == tgi && myR[q].kgi >= 0)
-                contrib += myR[q].fpj * band(sS, D, myR[q].kgi, cm[s], myR[q].bgi);
-              contrib += a.Hs[int64_t(jg) * a.Dd + cdc[s]] * dgi;
-            } else if (ckind[s] == CK_EMB) {
-              if (T == temb) contrib = band(sS, sT[T], myR[q].kfj, cm[s], myR[q].ebj) * dgi;
+        for (int s = 0; s < kCPL; ++s) {
+          const int kind = ckind[s];
+          if (kind == CK_NONE) continue;
+          const TableDesc T = sT[ctab[s]];
+          const int m = cm[s];
+          if (kind == CK_PAIR) {
+            if (ptt[s] < 0) continue;
+            int lo, hi;
+            band_run(P.r, seg[ptt[s]], seg[ptt[s] + 1], T, m, lo, hi);
+            for (int q = lo; q < hi; ++q) {
+              const uint32_t b = uint32_t(P.bits[q]);
+              if (!(b & kFlagPhi) || (b & kFlagTiny)) continue;
+              float bv[4], bd[4];
+              const float r = P.r[q];
+              const int k = pair_basis(T, r, bv, bd);
+              add_pair(A[s], band(sS, T, k, m, bd), r, P.u0[q], P.u1[q], P.u2[q]);
+              A[s].e += 0.5f * band(sS, T, k, m, bv);
             }
-            const float *u = myR[q].u;
-            const float *d = myR[q].d;
-            acf[s][0] += contrib * u[0];
-            acf[s][1] += contrib * u[1];
-            acf[s][2] += contrib * u[2];
-            acs[s][0] += contrib * d[0] * u[0];
-            acs[s][1] += contrib * d[1] * u[1];
-            acs[s][2] += contrib * d[2] * u[2];
-            acs[s][3] += contrib * d[0] * u[1];
-            acs[s][4] += contrib * d[0] * u[2];
-            acs[s][5] += contrib * d[1] * u[2];
-            ace[s] += en;
+          } else if (kind == CK_DENS) {
+            const int t = ctab[s] - a.npair;
+            int lo, hi;
+            // F'_i g'_tj(r) with tj == t
+            band_run(P.r, seg[t], seg[t + 1], T, m, lo, hi);
+            for (int q = lo; q < hi; ++q) {
+              const uint32_t b = uint32_t(P.bits[q]);
+              if (!(b & kFlagGj) || (b & kFlagTiny)) continue;
+              float bv[4], bd[4];
+              const float r = P.r[q];
+              const int k = pair_basis(T, r, bv, bd);
+              add_pair(A[s], Fpi * band(sS, T, k, m, bd), r, P.u0[q], P.u1[q], P.u2[q]);
+            }
+            // F'_j g'_ti(r) with ti == t, every partner type
+            if (t == ti) {
+              for (int tt = 0; tt < a.ntypes; ++tt) {
+                band_run(P.r, seg[tt], seg[tt + 1], T, m, lo, hi);
+                for (int q = lo; q < hi; ++q) {
+                  const uint32_t b = uint32_t(P.bits[q]);
+                  if (!(b & kFlagGi) || (b & kFlagTiny)) continue;
+                  float bv[4], bd[4];
+                  const float r = P.r[q];
+                  const int k = pair_basis(T, r, bv, bd);
+                  add_pair(A[s], P.fpj[q] * band(sS, T, k, m, bd), r, P.u0[q], P.u1[q], P.u2[q]);
+                }
+              }
+            }
+            // F''_j H_j g'_ti(r): every pair, rows of Hs gathered four at a time
+            const float *Hs = a.Hs + cdc[s];
+            for (int q0 = 0; q0 < np; q0 += 4) {
+              float h[4];
+#pragma unroll
+              for (int v = 0; v < 4; ++v)
+                h[v] = q0 + v < np ? Hs[int64_t(P.jg[q0 + v]) * a.Dd] : 0.0f;
+#pragma unroll
+              for (int v = 0; v < 4; ++v) {
+                const int q = q0 + v;
+                if (q < np) add_pair(A[s], h[v] * P.dgi[q], P.r[q], P.u0[q], P.u1[q], P.u2[q]);
+              }
+            }
+          } else if (kind == CK_EMB) {
+            // F'_j through the embedding window of j, tj == t
+            const int t = ctab[s] - a.npair - a.ntypes;
+            for (int q = seg[t]; q < seg[t + 1]; ++q) {
+              const int kf = P.kfj[q];
+              if (kf < m - 3 || kf > m + 2) continue;
+              const float eb[4] = {P.eb[0][q], P.eb[1][q], P.eb[2][q], P.eb[3][q]};
+              add_pair(A[s], band(sS, T, kf, m, eb) * P.dgi[q], P.r[q], P.u0[q], P.u1[q], P.u2[q]);
+            }
           }
         }
         wave_sync();
@@ -450,12 +561,12 @@ __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots) {
           g = band(sS, D, kfi, cm[s], ebi);
           e_extra = band(sS, D, kfi, cm[s], evi);
         }
-        for (int q = 0; q < 3; ++q) acf[s][q] += g * G[q];
-        for (int q = 0; q < 6; ++q) acs[s][q] += g * Gs[q];
+        for (int q = 0; q < 3; ++q) A[s].f[q] += g * G[q];
+        for (int q = 0; q < 6; ++q) A[s].s[q] += g * Gs[q];
         const int col = ccol[s];
-        for (int q = 0; q < 3; ++q) a.J[(row + q) * a.D + col] = acf[s][q];
-        tp[s][0] += double(ace[s]) + e_extra;
-        for (int q = 0; q < 6; ++q) tp[s][1 + q] += double(acs[s][q]);
+        for (int q = 0; q < 3; ++q) a.J[(row + q) * a.D + col] = A[s].f[q];
+        tp[s][0] += double(A[s].e) + e_extra;
+        for (int q = 0; q < 6; ++q) tp[s][1 + q] += double(A[s].s[q]);
         tp[s][7] += l_extra;
       }
     }
@@ -850,6 +961,9 @@ void launch_jacobian(Context &c) {
   } else {
     // column descriptors
     std::vector<ColDesc> cols(std::max(c.D, 1));
+    std::vector<std::pair<int, int>> pair_types(std::max(c.npair, 1));
+    for (int ta = 0; ta < c.ntypes; ++ta)
+      for (int tb = ta; tb < c.ntypes; ++tb) pair_types[ord_pair(ta, tb, c.ntypes)] = {ta, tb};
     int dens_c0 = -1, Dd = 0;
     for (int t = 0; t < c.ntab; ++t) {
       const TableDesc &T = c.h_tabs[t];
@@ -860,6 +974,7 @@ void launch_jacobian(Context &c) {
         const int col = c.h_col_of[T.off + m];
         if (col < 0) continue;
         ColDesc cd{t, m, kind, -1};
+        if (kind == CK_PAIR) cd.dc = pair_types[t].first | (pair_types[t].second << 8);
         if (kind == CK_DENS) {
           if (dens_c0 < 0) dens_c0 = col;
           cd.dc = Dd++;
@@ -898,10 +1013,20 @@ void launch_jacobian(Context &c) {
     a.emb_k = c.emb_k.p;
     a.Fpp = c.Fpp.p;
     const size_t base = size_t(c.ntab) * sizeof(TableDesc) + ((size_t(c.nknots) * 5 + 3) & ~size_t(3)) * 4;
-    const size_t lds_dens = base + 4 * 64 * sizeof(DRec);
-    const size_t lds_main = base + 4 * 64 * sizeof(PRec);
+    const size_t lds_dens = base + 4 * 128 * sizeof(float);
+    const size_t lds_main = base + 4 * sizeof(Stage);
     FS_REQUIRE(lds_main + kJP * 256 * 8 <= 64 * 1024, FS_ERR_UNSUPPORTED,
                "Jacobian LDS budget exceeded");
+    if (!c.jsort_valid && c.N > 0) {
+      c.jrec.alloc(size_t(std::max<int64_t>(c.P, 1)));
+      c.jr.alloc(size_t(std::max<int64_t>(c.P, 1)));
+      hipLaunchKernelGGL(k_jac_sort, dim3(nb(c.N, 128)), dim3(128), 0, c.stream, c.N, c.pair_off.p,
+                         c.pair_rec.p, c.jrec.p, c.jr.p);
+      FS_HIP(hipGetLastError());
+      c.jsort_valid = true;
+    }
+    a.jrec = c.jrec.p;
+    a.jr = c.jr.p;
     if (c.N > 0) {
       const unsigned g1 = unsigned(std::min<int64_t>((c.N + 3) / 4, int64_t(c.cu_count) * 16));
       hipLaunchKernelGGL(k_jac_dens, dim3(g1), dim3(256), lds_dens, c.stream, a, c.nknots);

@@ -372,6 +372,7 @@ void launch_pair_flags(Context &c) {
   a.r64 = c.pair_r64.p;
   hipLaunchKernelGGL(k_pair_flags, dim3(nblk(c.N, 128)), dim3(128), 0, c.stream, a);
   FS_HIP(hipGetLastError());
+  c.jsort_valid = false;
 }
 
 } // namespace fsa

@@ -161,7 +161,7 @@ __device__ inline double spl_value64(const TableDesc &T, const double *P, int k,
   return q[0] + t * (q[1] + t * (q[2] + t * q[3]));
 }
 
-__device__ inline int ord_pair(int a, int b, int nt) {
+__host__ __device__ inline int ord_pair(int a, int b, int nt) {
   if (a > b) {
     const int t = a;
     a = b;
