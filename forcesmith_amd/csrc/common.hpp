@@ -217,7 +217,7 @@ struct Context {
   // density histograms and embedding windows, SW tangents
   DBuf<int32_t> jt_cfg, jt_a0, jt_na, jt_ct0;
   int njt = 0;
-  DBuf<int4> jcols;
+  DBuf<int4> jcols, jslots;
   DBuf<float4> jrec; // pair records sorted by (partner type, r) per atom
   DBuf<float> jr;    // their FP32 distances
   bool jsort_valid = false;

@@ -13,14 +13,17 @@
 // back to central differences (fd_jacobian.hpp:37-49).
 //
 // Kernels:
+//   k_jac_sort   once per neighbour list: each atom's pairs sorted by
+//                (partner type, FP32 r), so that every column's pairs form one
+//                contiguous window per partner type.
 //   k_jac_dens   one wave per atom: H_i[c] = d rho_i / d y_c over the density
 //                columns, Hs_i = F''(rho_i) H_i (0 when the density is clamped),
 //                and the embedding window of atom i.
-//   k_jac_main   one wave per atom, lanes own columns: phase A stages one
-//                record per neighbour pair in LDS (64 pairs at a time), phase B
-//                walks the records and accumulates every column's force and
-//                virial partials; FP32 force rows are written, energy / virial
-//                / limit partials are reduced per tile in FP64.
+//   k_jac_main   one wave per atom, lanes own columns (grouped by kind into
+//                slots): each 64-pair chunk stages per-pair band coefficients
+//                of every role in LDS, then each column sums its window; FP32
+//                force rows are written, energy / virial / limit partials are
+//                reduced per tile in FP64.
 //   k_jac_rows   per configuration: energy, stress and limit rows from the
 //                tile partials, in tile order (deterministic, no atomics).
 // Stillinger-Weber: forward-mode tangents of the reference's pair and triplet
@@ -166,30 +169,29 @@ __global__ void k_jac_sort(int64_t N, const int64_t *__restrict__ pair_off,
   }
 }
 
-// per-wave staging of up to 64 pairs of one atom, structure of arrays
-struct Stage {
-  float r[64], u0[64], u1[64], u2[64];
-  float fpj[64], dgi[64];
-  int32_t jg[64], bits[64], kfj[64];
-  float eb[4][64];
-};
+// Band coefficients of one spline lookup: c[d] is the derivative of
+// b . (y_k, y_k+1, s_k, s_k+1) with respect to knot y_(k-2+d), d = 0..5,
+// through the makima slope band dS/dy (zero for linear tables).
+__device__ inline void band6(const float *sS, const TableDesc &T, int k, const float b[4], float w,
+                             float c[6]) {
+  const float *S0 = sS + (T.off + k) * 5;
+  const float *S1 = S0 + 5;
+  c[0] = w * (b[2] * S0[0]);
+  c[1] = w * (b[2] * S0[1] + b[3] * S1[0]);
+  c[2] = w * (b[0] + b[2] * S0[2] + b[3] * S1[1]);
+  c[3] = w * (b[1] + b[2] * S0[3] + b[3] * S1[2]);
+  c[4] = w * (b[2] * S0[4] + b[3] * S1[3]);
+  c[5] = w * (b[3] * S1[4]);
+}
 
-// first index in [lo, hi) whose interval on table T is >= kmin
-__device__ inline int first_k_ge(const float *sr, int lo, int hi, const TableDesc &T, int kmin) {
+// first index in [lo, hi) of a non-decreasing interval array with k >= kmin
+__device__ inline int first_ge(const int32_t *sk, int lo, int hi, int kmin) {
   while (lo < hi) {
     const int mid = (lo + hi) >> 1;
-    float t;
-    if (spl_interval32(T, sr[mid], t) >= kmin) hi = mid;
+    if (sk[mid] >= kmin) hi = mid;
     else lo = mid + 1;
   }
   return lo;
-}
-
-// run of pairs in [s0, s1) whose interval on T lies in the band of knot m
-__device__ inline void band_run(const float *sr, int s0, int s1, const TableDesc &T, int m, int &lo,
-                                int &hi) {
-  lo = first_k_ge(sr, s0, s1, T, m - 3);
-  hi = first_k_ge(sr, lo, s1, T, m + 3);
 }
 
 // segment starts of the partner types in the staged chunk (invalid lanes
@@ -201,15 +203,20 @@ __device__ inline void chunk_segments(int ty, int ntypes, int seg[kMaxTypes + 1]
 }
 
 // ---------------------------------------------------------------------------
+// H_i = d rho_i / d y over the density columns, from staged band coefficients
+struct DensStage {
+  int32_t k[64];
+  float c[6][64];
+};
+
 __global__ __launch_bounds__(256) void k_jac_dens(JacArgs a, int nknots) {
   extern __shared__ float4 smem[];
   TableDesc *sT = reinterpret_cast<TableDesc *>(smem);
   float *sS = reinterpret_cast<float *>(smem + a.ntab * int(sizeof(TableDesc) / sizeof(float4)));
-  float *sRall = sS + ((nknots * 5 + 3) & ~3);
+  DensStage *sAll = reinterpret_cast<DensStage *>(sS + ((nknots * 5 + 3) & ~3));
   stage(a, sT, sS, nknots);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  float *sr = sRall + wave * 128;
-  uint32_t *sb = reinterpret_cast<uint32_t *>(sr + 64);
+  DensStage &P = sAll[wave];
   const int64_t gw = int64_t(blockIdx.x) * 4 + wave;
   const int64_t nw = int64_t(gridDim.x) * 4;
   // my density columns
@@ -233,9 +240,16 @@ __global__ __launch_bounds__(256) void k_jac_dens(JacArgs a, int nknots) {
       int ty = kMaxTypes;
       if (p < p1) {
         const uint32_t b = __float_as_uint(a.jrec[p].w);
-        sr[lane] = a.jr[p];
-        sb[lane] = b;
+        const float r = a.jr[p];
         ty = pj_type(b);
+        const TableDesc T = sT[a.npair + ty];
+        float bv[4], bd[4];
+        const int k = pair_basis(T, r, bv, bd);
+        float c[6];
+        band6(sS, T, k, bv, ((b & kFlagGj) && !(b & kFlagTiny)) ? 1.0f : 0.0f, c);
+        P.k[lane] = k;
+#pragma unroll
+        for (int d = 0; d < 6; ++d) P.c[d][lane] = c[d];
       }
       int seg[kMaxTypes + 1];
       chunk_segments(ty, a.ntypes, seg);
@@ -244,16 +258,10 @@ __global__ __launch_bounds__(256) void k_jac_dens(JacArgs a, int nknots) {
       for (int s = 0; s < kCPL; ++s) {
         if (ctab[s] < 0) continue;
         const int t = ctab[s] - a.npair;
-        const TableDesc T = sT[ctab[s]];
-        int lo, hi;
-        band_run(sr, seg[t], seg[t + 1], T, cm[s], lo, hi);
-        for (int q = lo; q < hi; ++q) {
-          const uint32_t b = sb[q];
-          if (!(b & kFlagGj) || (b & kFlagTiny)) continue;
-          float bv[4], bd[4];
-          const int k = pair_basis(T, sr[q], bv, bd);
-          H[s] += band(sS, T, k, cm[s], bv);
-        }
+        const int m = cm[s];
+        const int lo = first_ge(P.k, seg[t], seg[t + 1], m - 3);
+        const int hi = first_ge(P.k, lo, seg[t + 1], m + 3);
+        for (int q = lo; q < hi; ++q) H[s] += P.c[m - P.k[q] + 2][q];
       }
       wave_sync();
     }
@@ -326,7 +334,25 @@ __global__ __launch_bounds__(256) void k_jac_dens(JacArgs a, int nknots) {
 }
 
 // ---------------------------------------------------------------------------
-// force / virial / energy partials of every column for one atom per wave
+// k_jac_main: one wave per atom, lanes own columns, columns grouped by kind
+// into slots (one slot holds pair, density or embedding columns only, so a
+// slot's loop does not diverge). Per 64-pair chunk every lane first stages
+// its pair: unit vector, distance, and for each role of the pair (pair table,
+// density of j weighted by F'_i, density of i weighted by F'_j, embedding of
+// j weighted by g'_ti(r)) the interval index and the six band coefficients.
+// A column then walks only its window of pairs and reads one coefficient per
+// pair.
+struct JStage {
+  float r[64], u0[64], u1[64], u2[64], dgi[64];
+  int32_t jg[64], kphi[64], kgj[64], kgi[64], kfj[64];
+  float cphi[6][64], ephi[6][64], cgj[6][64], cgi[6][64], cemb[6][64];
+};
+
+struct SlotDesc {
+  int32_t tab, m, kind, dc; // as ColDesc; dc = density column or pair-table types
+  int32_t col, pad0, pad1, pad2;
+};
+
 struct ColAcc {
   float f[3], s[6], e;
 };
@@ -344,60 +370,52 @@ __device__ inline void add_pair(ColAcc &A, float c, float r, float u0, float u1,
   A.s[5] += cr * u1 * u2;
 }
 
-__global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots) {
+template <int NS>
+__global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots, const SlotDesc *slots) {
   extern __shared__ float4 smem[];
   TableDesc *sT = reinterpret_cast<TableDesc *>(smem);
   float *sS = reinterpret_cast<float *>(smem + a.ntab * int(sizeof(TableDesc) / sizeof(float4)));
-  Stage *sAll = reinterpret_cast<Stage *>(sS + ((nknots * 5 + 3) & ~3));
+  JStage *sAll = reinterpret_cast<JStage *>(sS + ((nknots * 5 + 3) & ~3));
   __shared__ double sPart[kJP][256];
   stage(a, sT, sS, nknots);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  Stage &P = sAll[wave];
-  // lane-owned columns; pair tables carry their two types in dc
-  int ccol[kCPL], ctab[kCPL], cm[kCPL], ckind[kCPL], cdc[kCPL];
-  for (int s = 0; s < kCPL; ++s) {
-    const int c = lane + 64 * s;
-    if (c < a.D) {
-      const ColDesc cd = a.cols[c];
-      ccol[s] = c;
-      ctab[s] = cd.tab;
-      cm[s] = cd.m;
-      ckind[s] = cd.kind;
-      cdc[s] = cd.dc;
-    } else {
-      ccol[s] = -1;
-      ctab[s] = -1;
-      cm[s] = 0;
-      ckind[s] = CK_NONE;
-      cdc[s] = -1;
-    }
+  JStage &P = sAll[wave];
+  int ccol[NS], ctab[NS], cm[NS], ckind[NS], cdc[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const SlotDesc sd = slots[s * 64 + lane];
+    ccol[s] = sd.col;
+    ctab[s] = sd.tab;
+    cm[s] = sd.m;
+    ckind[s] = sd.kind;
+    cdc[s] = sd.dc;
   }
   for (int tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
     const int na = a.tile_na[tile];
     const int c = a.tile_cfg[tile];
     const int64_t cbase = a.atom_off[c];
-    double tp[kCPL][kJP];
+    float tp[NS][kJP];
 #pragma unroll
-    for (int s = 0; s < kCPL; ++s)
+    for (int s = 0; s < NS; ++s)
 #pragma unroll
-      for (int q = 0; q < kJP; ++q) tp[s][q] = 0.0;
+      for (int q = 0; q < kJP; ++q) tp[s][q] = 0.0f;
     for (int li = wave; li < na; li += 4) {
       const int64_t i = int64_t(a.tile_a0[tile]) + li;
       const int ti = a.types[i];
       const float Fpi = a.Fp32[i];
       const int64_t p0 = a.pair_off[i], p1 = a.pair_off[i + 1];
-      ColAcc A[kCPL];
+      ColAcc A[NS];
 #pragma unroll
-      for (int s = 0; s < kCPL; ++s) {
+      for (int s = 0; s < NS; ++s) {
         for (int q = 0; q < 3; ++q) A[s].f[q] = 0.0f;
         for (int q = 0; q < 6; ++q) A[s].s[q] = 0.0f;
         A[s].e = 0.0f;
       }
       float G[3] = {0, 0, 0}, Gs[6] = {0, 0, 0, 0, 0, 0};
-      // partner type of each lane-owned pair column for this centre type
-      int ptt[kCPL];
+      // partner type of each pair column for this centre type (-1: not ti's table)
+      int ptt[NS];
 #pragma unroll
-      for (int s = 0; s < kCPL; ++s) {
+      for (int s = 0; s < NS; ++s) {
         ptt[s] = -1;
         if (ckind[s] == CK_PAIR) {
           const int ta = cdc[s] & 0xff, tb = (cdc[s] >> 8) & 0xff;
@@ -411,7 +429,8 @@ __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots) {
           const float4 q = a.jrec[p];
           const uint32_t b = __float_as_uint(q.w);
           const float rr = a.jr[p];
-          const float inv_r = (b & kFlagTiny) ? 0.0f : 1.0f / rr;
+          const bool tiny = (b & kFlagTiny) != 0;
+          const float inv_r = tiny ? 0.0f : 1.0f / rr;
           const int tj = pj_type(b);
           const int32_t jg = int32_t(cbase + pj_local(b));
           ty = tj;
@@ -420,31 +439,67 @@ __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots) {
           P.u0[lane] = u0;
           P.u1[lane] = u1;
           P.u2[lane] = u2;
-          P.bits[lane] = int32_t(b);
           P.jg[lane] = jg;
-          P.fpj[lane] = a.Fp32[jg];
-          P.kfj[lane] = a.emb_k[jg];
-          const float4 e = reinterpret_cast<const float4 *>(a.emb_d)[jg];
-          P.eb[0][lane] = e.x;
-          P.eb[1][lane] = e.y;
-          P.eb[2][lane] = e.z;
-          P.eb[3][lane] = e.w;
-          float dgj = 0.0f, dgi = 0.0f;
-          if (!(b & kFlagTiny)) {
-            if (b & kFlagGj) {
-              const TableDesc T = sT[a.npair + tj];
-              float tt, v;
-              const int k = spl_interval32(T, rr, tt);
-              spl_eval32(T, a.ivl32, k, tt, v, dgj);
+          const float fpj = a.Fp32[jg];
+          float bv[4], bd[4], cc[6], ce[6];
+          // pair table phi_(ti,tj)
+          {
+            const TableDesc T = sT[ord_pair(ti, tj, a.ntypes)];
+            const int k = pair_basis(T, rr, bv, bd);
+            const float w = ((b & kFlagPhi) && !tiny) ? 1.0f : 0.0f;
+            band6(sS, T, k, bd, w, cc);
+            band6(sS, T, k, bv, 0.5f * w, ce);
+            P.kphi[lane] = k;
+#pragma unroll
+            for (int d = 0; d < 6; ++d) {
+              P.cphi[d][lane] = cc[d];
+              P.ephi[d][lane] = ce[d];
             }
-            if (b & kFlagGi) {
-              const TableDesc T = sT[a.npair + ti];
+          }
+          // density of j, weight F'_i; also g'_tj(r) for G_i
+          float dgj = 0.0f;
+          {
+            const TableDesc T = sT[a.npair + tj];
+            const int k = pair_basis(T, rr, bv, bd);
+            const bool on = (b & kFlagGj) && !tiny;
+            band6(sS, T, k, bd, on ? Fpi : 0.0f, cc);
+            P.kgj[lane] = k;
+#pragma unroll
+            for (int d = 0; d < 6; ++d) P.cgj[d][lane] = cc[d];
+            if (on) {
               float tt, v;
-              const int k = spl_interval32(T, rr, tt);
-              spl_eval32(T, a.ivl32, k, tt, v, dgi);
+              const int kk = spl_interval32(T, rr, tt);
+              spl_eval32(T, a.ivl32, kk, tt, v, dgj);
+            }
+          }
+          // density of i, weight F'_j; g'_ti(r) weights the embedding / H terms
+          float dgi = 0.0f;
+          {
+            const TableDesc T = sT[a.npair + ti];
+            const int k = pair_basis(T, rr, bv, bd);
+            const bool on = (b & kFlagGi) && !tiny;
+            band6(sS, T, k, bd, on ? fpj : 0.0f, cc);
+            P.kgi[lane] = k;
+#pragma unroll
+            for (int d = 0; d < 6; ++d) P.cgi[d][lane] = cc[d];
+            if (on) {
+              float tt, v;
+              const int kk = spl_interval32(T, rr, tt);
+              spl_eval32(T, a.ivl32, kk, tt, v, dgi);
             }
           }
           P.dgi[lane] = dgi;
+          // embedding of j at its density, weight g'_ti(r)
+          {
+            const TableDesc T = sT[a.npair + a.ntypes + tj];
+            const int kf = a.emb_k[jg];
+            const float4 e = reinterpret_cast<const float4 *>(a.emb_d)[jg];
+            const float eb[4] = {e.x, e.y, e.z, e.w};
+            band6(sS, T, kf, eb, dgi, cc);
+            P.kfj[lane] = kf;
+#pragma unroll
+            for (int d = 0; d < 6; ++d) P.cemb[d][lane] = cc[d];
+          }
           // G_i = sum_j g'_tj(r) u, virial counterpart sum_j g'_tj r u u
           G[0] += dgj * u0;
           G[1] += dgj * u1;
@@ -462,60 +517,49 @@ __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots) {
         wave_sync();
         const int np = int(min<int64_t>(64, p1 - pc));
 #pragma unroll
-        for (int s = 0; s < kCPL; ++s) {
+        for (int s = 0; s < NS; ++s) {
           const int kind = ckind[s];
           if (kind == CK_NONE) continue;
-          const TableDesc T = sT[ctab[s]];
           const int m = cm[s];
           if (kind == CK_PAIR) {
             if (ptt[s] < 0) continue;
-            int lo, hi;
-            band_run(P.r, seg[ptt[s]], seg[ptt[s] + 1], T, m, lo, hi);
+            const int s1 = seg[ptt[s] + 1];
+            const int lo = first_ge(P.kphi, seg[ptt[s]], s1, m - 3);
+            const int hi = first_ge(P.kphi, lo, s1, m + 3);
             for (int q = lo; q < hi; ++q) {
-              const uint32_t b = uint32_t(P.bits[q]);
-              if (!(b & kFlagPhi) || (b & kFlagTiny)) continue;
-              float bv[4], bd[4];
-              const float r = P.r[q];
-              const int k = pair_basis(T, r, bv, bd);
-              add_pair(A[s], band(sS, T, k, m, bd), r, P.u0[q], P.u1[q], P.u2[q]);
-              A[s].e += 0.5f * band(sS, T, k, m, bv);
+              const int d = m - P.kphi[q] + 2;
+              add_pair(A[s], P.cphi[d][q], P.r[q], P.u0[q], P.u1[q], P.u2[q]);
+              A[s].e += P.ephi[d][q];
             }
           } else if (kind == CK_DENS) {
             const int t = ctab[s] - a.npair;
-            int lo, hi;
-            // F'_i g'_tj(r) with tj == t
-            band_run(P.r, seg[t], seg[t + 1], T, m, lo, hi);
-            for (int q = lo; q < hi; ++q) {
-              const uint32_t b = uint32_t(P.bits[q]);
-              if (!(b & kFlagGj) || (b & kFlagTiny)) continue;
-              float bv[4], bd[4];
-              const float r = P.r[q];
-              const int k = pair_basis(T, r, bv, bd);
-              add_pair(A[s], Fpi * band(sS, T, k, m, bd), r, P.u0[q], P.u1[q], P.u2[q]);
+            // F'_i g'_tj(r), tj == t
+            {
+              const int s1 = seg[t + 1];
+              const int lo = first_ge(P.kgj, seg[t], s1, m - 3);
+              const int hi = first_ge(P.kgj, lo, s1, m + 3);
+              for (int q = lo; q < hi; ++q)
+                add_pair(A[s], P.cgj[m - P.kgj[q] + 2][q], P.r[q], P.u0[q], P.u1[q], P.u2[q]);
             }
-            // F'_j g'_ti(r) with ti == t, every partner type
+            // F'_j g'_ti(r), ti == t, every partner type
             if (t == ti) {
               for (int tt = 0; tt < a.ntypes; ++tt) {
-                band_run(P.r, seg[tt], seg[tt + 1], T, m, lo, hi);
-                for (int q = lo; q < hi; ++q) {
-                  const uint32_t b = uint32_t(P.bits[q]);
-                  if (!(b & kFlagGi) || (b & kFlagTiny)) continue;
-                  float bv[4], bd[4];
-                  const float r = P.r[q];
-                  const int k = pair_basis(T, r, bv, bd);
-                  add_pair(A[s], P.fpj[q] * band(sS, T, k, m, bd), r, P.u0[q], P.u1[q], P.u2[q]);
-                }
+                const int s1 = seg[tt + 1];
+                const int lo = first_ge(P.kgi, seg[tt], s1, m - 3);
+                const int hi = first_ge(P.kgi, lo, s1, m + 3);
+                for (int q = lo; q < hi; ++q)
+                  add_pair(A[s], P.cgi[m - P.kgi[q] + 2][q], P.r[q], P.u0[q], P.u1[q], P.u2[q]);
               }
             }
-            // F''_j H_j g'_ti(r): every pair, rows of Hs gathered four at a time
+            // F''_j H_j g'_ti(r): every pair, rows of Hs gathered eight at a time
             const float *Hs = a.Hs + cdc[s];
-            for (int q0 = 0; q0 < np; q0 += 4) {
-              float h[4];
+            for (int q0 = 0; q0 < np; q0 += 8) {
+              float h[8];
 #pragma unroll
-              for (int v = 0; v < 4; ++v)
+              for (int v = 0; v < 8; ++v)
                 h[v] = q0 + v < np ? Hs[int64_t(P.jg[q0 + v]) * a.Dd] : 0.0f;
 #pragma unroll
-              for (int v = 0; v < 4; ++v) {
+              for (int v = 0; v < 8; ++v) {
                 const int q = q0 + v;
                 if (q < np) add_pair(A[s], h[v] * P.dgi[q], P.r[q], P.u0[q], P.u1[q], P.u2[q]);
               }
@@ -524,10 +568,9 @@ __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots) {
             // F'_j through the embedding window of j, tj == t
             const int t = ctab[s] - a.npair - a.ntypes;
             for (int q = seg[t]; q < seg[t + 1]; ++q) {
-              const int kf = P.kfj[q];
-              if (kf < m - 3 || kf > m + 2) continue;
-              const float eb[4] = {P.eb[0][q], P.eb[1][q], P.eb[2][q], P.eb[3][q]};
-              add_pair(A[s], band(sS, T, kf, m, eb) * P.dgi[q], P.r[q], P.u0[q], P.u1[q], P.u2[q]);
+              const int d = m - P.kfj[q] + 2;
+              if (d < 0 || d > 5) continue;
+              add_pair(A[s], P.cemb[d][q], P.r[q], P.u0[q], P.u1[q], P.u2[q]);
             }
           }
         }
@@ -547,15 +590,14 @@ __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots) {
       const int temb_i = a.npair + a.ntypes + ti;
       const int64_t row = a.row_off[c] + 3 * (i - cbase);
 #pragma unroll
-      for (int s = 0; s < kCPL; ++s) {
+      for (int s = 0; s < NS; ++s) {
         if (ckind[s] == CK_NONE) continue;
-        float g = 0.0f;
-        double e_extra = 0.0, l_extra = 0.0;
+        float g = 0.0f, e_extra = 0.0f, l_extra = 0.0f;
         if (ckind[s] == CK_DENS) {
           g = a.Hs[i * a.Dd + cdc[s]];
           const double h = a.H[i * a.Dd + cdc[s]];
-          if (clamped) l_extra = over * h;
-          else e_extra = Fpi64 * h;
+          if (clamped) l_extra = float(over * h);
+          else e_extra = float(Fpi64 * h);
         } else if (ckind[s] == CK_EMB && ctab[s] == temb_i) {
           const TableDesc D = sT[ctab[s]];
           g = band(sS, D, kfi, cm[s], ebi);
@@ -565,19 +607,20 @@ __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots) {
         for (int q = 0; q < 6; ++q) A[s].s[q] += g * Gs[q];
         const int col = ccol[s];
         for (int q = 0; q < 3; ++q) a.J[(row + q) * a.D + col] = A[s].f[q];
-        tp[s][0] += double(A[s].e) + e_extra;
-        for (int q = 0; q < 6; ++q) tp[s][1 + q] += double(A[s].s[q]);
+        tp[s][0] += A[s].e + e_extra;
+        for (int q = 0; q < 6; ++q) tp[s][1 + q] += A[s].s[q];
         tp[s][7] += l_extra;
       }
     }
-    // reduce the four waves' partials in wave order, write the tile partial
+    // reduce the four waves' partials in wave order (FP64), write the tile partial
     for (int w = 0; w < 4; ++w) {
       if (wave == w) {
 #pragma unroll
-        for (int s = 0; s < kCPL; ++s) {
-          const int col = lane + 64 * s;
-          if (col >= 256) continue;
-          for (int q = 0; q < kJP; ++q) sPart[q][col] = (w == 0 ? 0.0 : sPart[q][col]) + tp[s][q];
+        for (int s = 0; s < NS; ++s) {
+          if (ckind[s] == CK_NONE) continue;
+          const int col = ccol[s];
+          for (int q = 0; q < kJP; ++q)
+            sPart[q][col] = (w == 0 ? 0.0 : sPart[q][col]) + double(tp[s][q]);
         }
       }
       __syncthreads();
@@ -1013,10 +1056,32 @@ void launch_jacobian(Context &c) {
     a.emb_k = c.emb_k.p;
     a.Fpp = c.Fpp.p;
     const size_t base = size_t(c.ntab) * sizeof(TableDesc) + ((size_t(c.nknots) * 5 + 3) & ~size_t(3)) * 4;
-    const size_t lds_dens = base + 4 * 128 * sizeof(float);
-    const size_t lds_main = base + 4 * sizeof(Stage);
+    const size_t lds_dens = base + 4 * sizeof(DensStage);
+    const size_t lds_main = base + 4 * sizeof(JStage);
     FS_REQUIRE(lds_main + kJP * 256 * 8 <= 64 * 1024, FS_ERR_UNSUPPORTED,
                "Jacobian LDS budget exceeded");
+    // lane slots: pair, density and embedding columns each start a new slot
+    std::vector<SlotDesc> slots;
+    for (int kind : {int(CK_PAIR), int(CK_DENS), int(CK_EMB)}) {
+      size_t n = 0;
+      for (int col = 0; col < c.D; ++col) {
+        if (cols[col].kind != kind) continue;
+        if (n % 64 == 0) slots.resize(slots.size() + 64, SlotDesc{-1, 0, CK_NONE, -1, 0, 0, 0, 0});
+        const ColDesc &cd = cols[col];
+        slots[slots.size() - 64 + n % 64] = SlotDesc{cd.tab, cd.m, cd.kind, cd.dc, col, 0, 0, 0};
+        ++n;
+      }
+    }
+    const int ns = int(slots.size() / 64);
+    FS_REQUIRE(ns <= 6, FS_ERR_UNSUPPORTED, "Jacobian columns need more than six lane slots");
+    const int NS = ns <= 3 ? 3 : (ns <= 4 ? 4 : 6);
+    slots.resize(size_t(NS) * 64, SlotDesc{-1, 0, CK_NONE, -1, 0, 0, 0, 0});
+    static_assert(sizeof(SlotDesc) == 2 * sizeof(int4), "slot descriptor layout");
+    c.jslots.alloc(slots.size() * 2);
+    FS_HIP(hipMemcpyAsync(c.jslots.p, slots.data(), sizeof(SlotDesc) * slots.size(),
+                          hipMemcpyHostToDevice, c.stream));
+    FS_HIP(hipStreamSynchronize(c.stream));
+    const SlotDesc *d_slots = reinterpret_cast<const SlotDesc *>(c.jslots.p);
     if (!c.jsort_valid && c.N > 0) {
       c.jrec.alloc(size_t(std::max<int64_t>(c.P, 1)));
       c.jr.alloc(size_t(std::max<int64_t>(c.P, 1)));
@@ -1033,7 +1098,12 @@ void launch_jacobian(Context &c) {
       FS_HIP(hipGetLastError());
       const unsigned g2 = unsigned(std::max(1, std::min(nt, c.cu_count * 8)));
       c.time_main_begin();
-      hipLaunchKernelGGL(k_jac_main, dim3(g2), dim3(256), lds_main, c.stream, a, c.nknots);
+      if (NS == 3)
+        hipLaunchKernelGGL(k_jac_main<3>, dim3(g2), dim3(256), lds_main, c.stream, a, c.nknots, d_slots);
+      else if (NS == 4)
+        hipLaunchKernelGGL(k_jac_main<4>, dim3(g2), dim3(256), lds_main, c.stream, a, c.nknots, d_slots);
+      else
+        hipLaunchKernelGGL(k_jac_main<6>, dim3(g2), dim3(256), lds_main, c.stream, a, c.nknots, d_slots);
       FS_HIP(hipGetLastError());
       c.time_main_end();
     }
