@@ -342,6 +342,12 @@ __global__ __launch_bounds__(256) void k_jac_dens(JacArgs a, int nknots) {
 // j weighted by g'_ti(r)) the interval index and the six band coefficients.
 // A column then walks only its window of pairs and reads one coefficient per
 // pair.
+// Virial rows only need per-configuration sums, and the full list holds every
+// pair from both ends with the same distance and table: the F'_j g'_ti term of
+// pair (i,j) equals the F'_i g'_tj term of (j,i), and the H and embedding
+// terms of pair (i,j) equal atom j's per-atom terms through
+// Gs_j = sum_i g'_ti(r) r u u. Those pair loops therefore accumulate forces
+// only, and the matching per-atom / F'_i terms carry their virial twice.
 struct JStage {
   float r[64], u0[64], u1[64], u2[64], dgi[64];
   int32_t jg[64], kphi[64], kgj[64], kgi[64], kfj[64];
@@ -356,6 +362,22 @@ struct SlotDesc {
 struct ColAcc {
   float f[3], s[6], e;
 };
+
+__device__ inline void add_force(ColAcc &A, float c, float u0, float u1, float u2) {
+  A.f[0] += c * u0;
+  A.f[1] += c * u1;
+  A.f[2] += c * u2;
+}
+
+__device__ inline void add_virial(ColAcc &A, float c, float r, float u0, float u1, float u2) {
+  const float cr = c * r;
+  A.s[0] += cr * u0 * u0;
+  A.s[1] += cr * u1 * u1;
+  A.s[2] += cr * u2 * u2;
+  A.s[3] += cr * u0 * u1;
+  A.s[4] += cr * u0 * u2;
+  A.s[5] += cr * u1 * u2;
+}
 
 __device__ inline void add_pair(ColAcc &A, float c, float r, float u0, float u1, float u2) {
   A.f[0] += c * u0;
@@ -376,7 +398,9 @@ __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots, const S
   TableDesc *sT = reinterpret_cast<TableDesc *>(smem);
   float *sS = reinterpret_cast<float *>(smem + a.ntab * int(sizeof(TableDesc) / sizeof(float4)));
   JStage *sAll = reinterpret_cast<JStage *>(sS + ((nknots * 5 + 3) & ~3));
-  __shared__ double sPart[kJP][256];
+  // the tile-end reduction reuses the staging area of the four waves
+  static_assert(4 * sizeof(JStage) >= sizeof(double) * kJP * 256, "reduction buffer");
+  double(*sPart)[256] = reinterpret_cast<double(*)[256]>(sAll);
   stage(a, sT, sS, nknots);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   JStage &P = sAll[wave];
@@ -538,8 +562,11 @@ __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots, const S
               const int s1 = seg[t + 1];
               const int lo = first_ge(P.kgj, seg[t], s1, m - 3);
               const int hi = first_ge(P.kgj, lo, s1, m + 3);
-              for (int q = lo; q < hi; ++q)
-                add_pair(A[s], P.cgj[m - P.kgj[q] + 2][q], P.r[q], P.u0[q], P.u1[q], P.u2[q]);
+              for (int q = lo; q < hi; ++q) {
+                const float cq = P.cgj[m - P.kgj[q] + 2][q];
+                add_force(A[s], cq, P.u0[q], P.u1[q], P.u2[q]);
+                add_virial(A[s], 2.0f * cq, P.r[q], P.u0[q], P.u1[q], P.u2[q]);
+              }
             }
             // F'_j g'_ti(r), ti == t, every partner type
             if (t == ti) {
@@ -548,7 +575,7 @@ __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots, const S
                 const int lo = first_ge(P.kgi, seg[tt], s1, m - 3);
                 const int hi = first_ge(P.kgi, lo, s1, m + 3);
                 for (int q = lo; q < hi; ++q)
-                  add_pair(A[s], P.cgi[m - P.kgi[q] + 2][q], P.r[q], P.u0[q], P.u1[q], P.u2[q]);
+                  add_force(A[s], P.cgi[m - P.kgi[q] + 2][q], P.u0[q], P.u1[q], P.u2[q]);
               }
             }
             // F''_j H_j g'_ti(r): every pair, rows of Hs gathered eight at a time
@@ -561,7 +588,7 @@ __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots, const S
 #pragma unroll
               for (int v = 0; v < 8; ++v) {
                 const int q = q0 + v;
-                if (q < np) add_pair(A[s], h[v] * P.dgi[q], P.r[q], P.u0[q], P.u1[q], P.u2[q]);
+                if (q < np) add_force(A[s], h[v] * P.dgi[q], P.u0[q], P.u1[q], P.u2[q]);
               }
             }
           } else if (kind == CK_EMB) {
@@ -570,7 +597,7 @@ __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots, const S
             for (int q = seg[t]; q < seg[t + 1]; ++q) {
               const int d = m - P.kfj[q] + 2;
               if (d < 0 || d > 5) continue;
-              add_pair(A[s], P.cemb[d][q], P.r[q], P.u0[q], P.u1[q], P.u2[q]);
+              add_force(A[s], P.cemb[d][q], P.u0[q], P.u1[q], P.u2[q]);
             }
           }
         }
@@ -604,7 +631,8 @@ __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots, const S
           e_extra = band(sS, D, kfi, cm[s], evi);
         }
         for (int q = 0; q < 3; ++q) A[s].f[q] += g * G[q];
-        for (int q = 0; q < 6; ++q) A[s].s[q] += g * Gs[q];
+        // twice: the virial of the mirrored pair terms (see the pair loop)
+        for (int q = 0; q < 6; ++q) A[s].s[q] += 2.0f * g * Gs[q];
         const int col = ccol[s];
         for (int q = 0; q < 3; ++q) a.J[(row + q) * a.D + col] = A[s].f[q];
         tp[s][0] += A[s].e + e_extra;
@@ -613,6 +641,7 @@ __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots, const S
       }
     }
     // reduce the four waves' partials in wave order (FP64), write the tile partial
+    __syncthreads(); // every wave is done with its staging area
     for (int w = 0; w < 4; ++w) {
       if (wave == w) {
 #pragma unroll
@@ -1058,8 +1087,7 @@ void launch_jacobian(Context &c) {
     const size_t base = size_t(c.ntab) * sizeof(TableDesc) + ((size_t(c.nknots) * 5 + 3) & ~size_t(3)) * 4;
     const size_t lds_dens = base + 4 * sizeof(DensStage);
     const size_t lds_main = base + 4 * sizeof(JStage);
-    FS_REQUIRE(lds_main + kJP * 256 * 8 <= 64 * 1024, FS_ERR_UNSUPPORTED,
-               "Jacobian LDS budget exceeded");
+    FS_REQUIRE(lds_main <= 64 * 1024, FS_ERR_UNSUPPORTED, "Jacobian LDS budget exceeded");
     // lane slots: pair, density and embedding columns each start a new slot
     std::vector<SlotDesc> slots;
     for (int kind : {int(CK_PAIR), int(CK_DENS), int(CK_EMB)}) {
