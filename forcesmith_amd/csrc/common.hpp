@@ -124,7 +124,9 @@ void launch_cost(Context &c, double *d_out);
 void launch_jacobian(Context &c);
 void launch_normal_equations(Context &c);
 int lm_minimize_device(Context &c, double *x, int max_iter, double *hist);
-void launch_population_cost(Context &c, const double *d_X, int np, double *d_cost);
+// batched |F(x_m)|^2 over host parameter vectors X[np][D]; false when the
+// model / precision has no batched kernels (the caller then loops members)
+bool population_cost_batched(Context &c, const double *X, int np, double *cost);
 void launch_sw_rev(Context &c);
 void launch_sw_eval(Context &c);
 
@@ -225,6 +227,13 @@ struct Context {
   DBuf<int32_t> emb_k, sw_csw, sw_clam;
   DBuf<double> sw_own, sw_ptan, sw_etan;
   DBuf<double> lm_M, lm_dx, lm_scal;
+  // batched population cost: member tables, embedding derivatives, partials
+  DBuf<double> pop_X, pop_y, pop_s, pop_S, pop_i64, pop_Fp, pop_partE, pop_partF, pop_cmc, pop_cost;
+  DBuf<float4> pop_i32;
+  DBuf<float> pop_Fp32;
+  DBuf<int32_t> pop_grp;
+  std::vector<int32_t> h_pop_grp;
+  int pop_grp_tiles = -1;
   Mlp *mlp = nullptr;              // owned; see mlp.hpp
   SurrogateState *surr = nullptr;  // owned; see de.cpp
   // per-kernel timing of the main Jacobian kernel (HIP events, context stream)

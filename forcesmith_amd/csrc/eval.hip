@@ -171,6 +171,7 @@ __global__ void k_scatter(int D, const double *__restrict__ xv, const int32_t *_
 // two-pass EAM / ADP
 // ---------------------------------------------------------------------------
 struct EvalArgs {
+  int64_t N;
   int ntiles;
   const int32_t *tile_cfg, *tile_a0, *tile_na;
   const int64_t *atom_off, *pair_off, *row_off;
@@ -660,8 +661,259 @@ __global__ void k_sum_final(int n, const double *__restrict__ part, double *__re
   if (threadIdx.x == 0) out[0] = s[0];
 }
 
+// ---------------------------------------------------------------------------
+// Batched population cost (EAM, mixed precision): |F(x_m)|^2 for a batch of
+// parameter vectors in three launches instead of one residual per member.
+// Block b works on member b / G over tile group b % G; the G = 8 groups are
+// contiguous tile ranges, one per XCD under the round-robin block dispatch,
+// so the blocks of the members in flight on an XCD share that group's pair
+// records in its L2. Each block keeps its member's FP32 interval table in LDS
+// for its whole tile range.
+constexpr int kPopG = 8;
+constexpr int kPopE = 2;  // density pass: embedding energy, limit
+constexpr int kPopF = 11; // force pass: |F - F_ref|^2, pair energy, 9 virial
+
+struct PopArgs {
+  int nb;                            // members in this batch
+  int nknots, nivl;
+  const double *y, *s;               // [nb][nknots] member knots and slopes
+  const float4 *ivl32;               // [nb][2 nivl] FP32 hi/lo interval forms
+  const double *ivl64;               // [nb][4 nivl] FP64 interval forms
+  double *Fp;                        // [nb][N] F'(rho) FP64
+  float *Fp32;                       // [nb][N] F'(rho) FP32
+  const int32_t *grp_t0;             // [G+1] tile range of each group
+  double *partE;                     // [nb][C][G][kPopE]
+  double *partF;                     // [nb][C][G][kPopF]
+  int C;
+};
+
+// per-member tables: knots = base knots with the free ones from X[m]
+__global__ void k_pop_tables(int ntab, int D, int nknots, const TableDesc *__restrict__ tabs,
+                             const double *__restrict__ x, const double *__restrict__ y0,
+                             const int32_t *__restrict__ knot_of, const double *__restrict__ X,
+                             double *__restrict__ y, double *__restrict__ s, double *__restrict__ S,
+                             float4 *__restrict__ ivl, double *__restrict__ p64, int nivl) {
+  const int m = blockIdx.x;
+  double *ym = y + size_t(m) * nknots;
+  for (int k = threadIdx.x; k < nknots; k += blockDim.x) ym[k] = y0[k];
+  __syncthreads();
+  for (int c = threadIdx.x; c < D; c += blockDim.x) ym[knot_of[c]] = X[size_t(m) * D + c];
+  __syncthreads();
+  for (int t = threadIdx.x; t < ntab; t += blockDim.x) {
+    const TableDesc T = tabs[t];
+    table_slopes(T, x + T.off, ym + T.off, s + size_t(m) * nknots + T.off,
+                 S + size_t(m) * 5 * nknots + 5 * T.off, ivl + size_t(m) * 2 * nivl,
+                 p64 + size_t(m) * 4 * nivl);
+  }
+}
+
+__device__ inline void pop_flush(double *dst, const double *v, int n) {
+  for (int k = 0; k < n; ++k) dst[k] = v[k];
+}
+
+template <int LPA>
+__global__ __launch_bounds__(256) void k_pop_density(EvalArgs a, PopArgs p) {
+  constexpr int APB = 256 / LPA;
+  extern __shared__ float4 smem[];
+  TableDesc *sT = reinterpret_cast<TableDesc *>(smem);
+  __shared__ double sRed[APB][kPopE];
+  const int g = blockIdx.x % kPopG, m = blockIdx.x / kPopG;
+  for (int k = threadIdx.x; k < a.ntab; k += blockDim.x) sT[k] = a.tabs[k];
+  __syncthreads();
+  const double *P64 = p.ivl64 + size_t(m) * 4 * p.nivl;
+  const double *ym = p.y + size_t(m) * p.nknots, *sm = p.s + size_t(m) * p.nknots;
+  const int sub = threadIdx.x / LPA, lane = threadIdx.x % LPA;
+  double run[kPopE] = {0.0, 0.0};
+  int run_c = -1;
+  for (int tile = p.grp_t0[g]; tile < p.grp_t0[g + 1]; ++tile) {
+    const int na = a.tile_na[tile];
+    const int c = a.tile_cfg[tile];
+    if (sub < na) {
+      const int64_t i = int64_t(a.tile_a0[tile]) + sub;
+      const int ti = a.types[i];
+      double rho = 0.0;
+      for (int64_t q = a.pair_off[i] + lane; q < a.pair_off[i + 1]; q += LPA) {
+        const uint32_t b = a.bits[q];
+        if (b & kFlagGj) {
+          const TableDesc D = sT[a.npair + pj_type(b)];
+          double u;
+          const int k = spl_interval64(D, a.r64[q], u);
+          rho += spl_value64(D, P64, k, u);
+        }
+      }
+      rho = sub_sum<LPA>(rho);
+      if (lane == 0) {
+        const TableDesc Te = sT[a.npair + a.ntypes + ti];
+        double rh = rho, lim = 0.0;
+        if (rh > Te.xn) {
+          const double dd = rh - Te.xn;
+          lim = kPenalty * dd * dd;
+          rh = Te.xn;
+        } else if (rh < Te.x0) {
+          const double dd = Te.x0 - rh;
+          lim = kPenalty * dd * dd;
+          rh = Te.x0;
+        }
+        double F, dF;
+        spl_uncached64(Te, a.tx + Te.off, ym + Te.off, sm + Te.off, rh, F, dF);
+        p.Fp[size_t(m) * a.N + i] = dF;
+        p.Fp32[size_t(m) * a.N + i] = float(dF);
+        sRed[sub][0] = F;
+        sRed[sub][1] = lim;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      if (c != run_c) {
+        if (run_c >= 0) pop_flush(p.partE + ((size_t(m) * p.C + run_c) * kPopG + g) * kPopE, run, kPopE);
+        run[0] = run[1] = 0.0;
+        run_c = c;
+      }
+      for (int q = 0; q < na && q < APB; ++q) {
+        run[0] += sRed[q][0];
+        run[1] += sRed[q][1];
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && run_c >= 0)
+    pop_flush(p.partE + ((size_t(m) * p.C + run_c) * kPopG + g) * kPopE, run, kPopE);
+}
+
+template <int LPA>
+__global__ __launch_bounds__(256) void k_pop_force(EvalArgs a, PopArgs p) {
+  constexpr int APB = 256 / LPA;
+  extern __shared__ float4 smem[];
+  TableDesc *sT = reinterpret_cast<TableDesc *>(smem);
+  float4 *sI = smem + a.ntab * int(sizeof(TableDesc) / sizeof(float4));
+  __shared__ double sRed[APB][kPopF];
+  const int g = blockIdx.x % kPopG, m = blockIdx.x / kPopG;
+  {
+    const float4 *src = reinterpret_cast<const float4 *>(a.tabs);
+    float4 *dst = reinterpret_cast<float4 *>(sT);
+    const int nq = a.ntab * int(sizeof(TableDesc) / sizeof(float4));
+    for (int k = threadIdx.x; k < nq; k += blockDim.x) dst[k] = src[k];
+    const float4 *mi = p.ivl32 + size_t(m) * 2 * p.nivl;
+    for (int k = threadIdx.x; k < 2 * p.nivl; k += blockDim.x) sI[k] = mi[k];
+    __syncthreads();
+  }
+  const Tabs tb{sT, sI, a.tx, p.y + size_t(m) * p.nknots, p.s + size_t(m) * p.nknots,
+                p.ivl64 + size_t(m) * 4 * p.nivl};
+  EvalArgs am = a; // the member's embedding derivatives
+  am.Fp = p.Fp + size_t(m) * a.N;
+  am.Fp32 = p.Fp32 + size_t(m) * a.N;
+  const int sub = threadIdx.x / LPA, lane = threadIdx.x % LPA;
+  double run[kPopF];
+  for (int k = 0; k < kPopF; ++k) run[k] = 0.0;
+  int run_c = -1;
+  for (int tile = p.grp_t0[g]; tile < p.grp_t0[g + 1]; ++tile) {
+    const int na = a.tile_na[tile];
+    const int c = a.tile_cfg[tile];
+    if (sub < na) {
+      const int64_t i = int64_t(a.tile_a0[tile]) + sub;
+      const int64_t cbase = a.atom_off[c];
+      const int ti = a.types[i];
+      AtomState<float> si;
+      si.Fp = am.Fp32[i];
+      ForceAcc<float> acc;
+      for (int64_t q = a.pair_off[i] + lane; q < a.pair_off[i + 1]; q += LPA) {
+        const float4 rec = a.rec[q];
+        const uint32_t b = __float_as_uint(rec.w);
+        if (b & kFlagEdge) {
+          const Pair<double> e = load_pair64<true>(a, q);
+          AtomState<double> sd;
+          sd.Fp = am.Fp[i];
+          ForceAcc<double> t;
+          force_pair<FS_EAM, true, double>(tb, am, ti, cbase, sd, e.b, e.d0, e.d1, e.d2, e.r, t);
+          acc.f0 += float(t.f0);
+          acc.f1 += float(t.f1);
+          acc.f2 += float(t.f2);
+          acc.ep += t.ep;
+          for (int k = 0; k < 9; ++k) acc.sg[k] += t.sg[k];
+        } else {
+          const float r = sqrtf(rec.x * rec.x + rec.y * rec.y + rec.z * rec.z);
+          force_pair<FS_EAM, false, float>(tb, am, ti, cbase, si, b, rec.x, rec.y, rec.z, r, acc,
+                                           a.r64[q]);
+        }
+      }
+      acc.f0 = sub_sum<LPA>(acc.f0);
+      acc.f1 = sub_sum<LPA>(acc.f1);
+      acc.f2 = sub_sum<LPA>(acc.f2);
+      acc.ep = sub_sum<LPA>(acc.ep);
+      for (int k = 0; k < 9; ++k) acc.sg[k] = sub_sum<LPA>(acc.sg[k]);
+      if (lane == 0) {
+        const double F[3] = {double(acc.f0), double(acc.f1), double(acc.f2)};
+        double r2 = 0.0;
+        for (int k = 0; k < 3; ++k) {
+          const double d = F[k] - a.ref_force[3 * i + k];
+          r2 += d * d;
+        }
+        sRed[sub][0] = r2;
+        sRed[sub][1] = acc.ep;
+        for (int k = 0; k < 9; ++k) sRed[sub][2 + k] = acc.sg[k];
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      if (c != run_c) {
+        if (run_c >= 0) pop_flush(p.partF + ((size_t(m) * p.C + run_c) * kPopG + g) * kPopF, run, kPopF);
+        for (int k = 0; k < kPopF; ++k) run[k] = 0.0;
+        run_c = c;
+      }
+      for (int q = 0; q < na && q < APB; ++q)
+        for (int k = 0; k < kPopF; ++k) run[k] += sRed[q][k];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && run_c >= 0)
+    pop_flush(p.partF + ((size_t(m) * p.C + run_c) * kPopG + g) * kPopF, run, kPopF);
+}
+
+// cost of every (member, configuration): the residual rows of
+// write_config_residuals (residual_layout.hpp:37-54), squared and summed
+__global__ void k_pop_finalize(int nb, int C, const int32_t *__restrict__ grp_t0,
+                               const int32_t *__restrict__ cfg_tile0, const double *__restrict__ partE,
+                               const double *__restrict__ partF, const double *__restrict__ vol,
+                               const double *__restrict__ ref_energy,
+                               const double *__restrict__ ref_stress, double ew, double sw,
+                               double *__restrict__ cost_mc) {
+  const int64_t id = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (id >= int64_t(nb) * C) return;
+  const int m = int(id / C), c = int(id % C);
+  double e = 0.0, lim = 0.0, r2 = 0.0, vir[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  const int t0 = cfg_tile0[c], t1 = cfg_tile0[c + 1];
+  for (int g = 0; g < kPopG; ++g) {
+    if (!(grp_t0[g] < t1 && grp_t0[g + 1] > t0)) continue; // group holds no tile of c
+    const double *pe = partE + ((size_t(m) * C + c) * kPopG + g) * kPopE;
+    const double *pf = partF + ((size_t(m) * C + c) * kPopG + g) * kPopF;
+    e += pe[0] + pf[1];
+    lim += pe[1];
+    r2 += pf[0];
+    for (int k = 0; k < 9; ++k) vir[k] += pf[2 + k];
+  }
+  const double er = ew * (e - ref_energy[c]);
+  double cost = r2 + er * er + lim * lim;
+  if (sw > 0.0) {
+    const int vo[6] = {0, 4, 8, 1, 2, 5};
+    for (int v = 0; v < 6; ++v) {
+      const double sr = sw * (vir[vo[v]] / vol[c] - ref_stress[6 * c + v]);
+      cost += sr * sr;
+    }
+  }
+  cost_mc[id] = cost;
+}
+
+__global__ void k_pop_sum(int nb, int C, const double *__restrict__ cost_mc, double *__restrict__ cost) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nb) return;
+  double s = 0.0;
+  for (int c = 0; c < C; ++c) s += cost_mc[size_t(m) * C + c];
+  cost[m] = s;
+}
+
 EvalArgs make_args(Context &c) {
   EvalArgs a;
+  a.N = c.N;
   a.ntiles = c.ntiles;
   a.tile_cfg = c.tile_cfg.p;
   a.tile_a0 = c.tile_a0.p;
@@ -756,6 +1008,93 @@ void launch_eval(Context &c, bool) {
                      c.ref_energy.p, c.ref_stress.p, c.energy_weight, c.stress_weight,
                      c.residual.p, c.cfg_E.p, c.cfg_S.p, c.cfg_L.p);
   FS_HIP(hipGetLastError());
+}
+
+bool population_cost_batched(Context &c, const double *X, int np, double *cost) {
+  if (c.kind != FS_EAM || c.precision != FS_MIXED || c.ntiles == 0 || c.D == 0 || np <= 0)
+    return false;
+  const int G = kPopG;
+  if (c.h_pop_grp.size() != size_t(G + 1) || c.pop_grp_tiles != c.ntiles) {
+    c.h_pop_grp.assign(G + 1, 0);
+    for (int g = 0; g <= G; ++g) c.h_pop_grp[g] = int32_t(int64_t(g) * c.ntiles / G);
+    c.pop_grp.alloc(G + 1);
+    FS_HIP(hipMemcpyAsync(c.pop_grp.p, c.h_pop_grp.data(), 4 * (G + 1), hipMemcpyHostToDevice,
+                          c.stream));
+    c.pop_grp_tiles = c.ntiles;
+  }
+  // batch size: per member 12 N bytes of embedding derivatives, 104 C G bytes
+  // of partials and its tables; about 4 GiB per batch
+  const size_t per = 12 * size_t(c.N) + size_t(c.C) * G * 8 * (kPopE + kPopF) +
+                     size_t(c.nknots) * 8 * 7 + size_t(c.nivl) * 64 + size_t(c.C) * 8 + 8 * c.D;
+  const int cap = int(std::max<size_t>(1, std::min<size_t>(size_t(np), (size_t(4) << 30) / per)));
+  c.pop_X.alloc(size_t(cap) * c.D);
+  c.pop_y.alloc(size_t(cap) * c.nknots);
+  c.pop_s.alloc(size_t(cap) * c.nknots);
+  c.pop_S.alloc(size_t(cap) * 5 * c.nknots);
+  c.pop_i32.alloc(size_t(cap) * 2 * c.nivl);
+  c.pop_i64.alloc(size_t(cap) * 4 * c.nivl);
+  c.pop_Fp.alloc(size_t(cap) * c.N);
+  c.pop_Fp32.alloc(size_t(cap) * c.N);
+  c.pop_partE.alloc(size_t(cap) * c.C * G * kPopE);
+  c.pop_partF.alloc(size_t(cap) * c.C * G * kPopF);
+  c.pop_cmc.alloc(size_t(cap) * c.C);
+  c.pop_cost.alloc(size_t(cap));
+  EvalArgs a = make_args(c);
+  const size_t lds_d = size_t(c.ntab) * sizeof(TableDesc);
+  const size_t lds_f = lds_d + 2 * size_t(c.nivl) * sizeof(float4);
+  for (int m0 = 0; m0 < np; m0 += cap) {
+    const int nb = std::min(cap, np - m0);
+    FS_HIP(hipMemcpyAsync(c.pop_X.p, X + size_t(m0) * c.D, sizeof(double) * nb * c.D,
+                          hipMemcpyHostToDevice, c.stream));
+    hipLaunchKernelGGL(k_pop_tables, dim3(nb), dim3(64), 0, c.stream, c.ntab, c.D, c.nknots,
+                       c.tabs.p, c.tab_x.p, c.tab_y.p, c.knot_of.p, c.pop_X.p, c.pop_y.p, c.pop_s.p,
+                       c.pop_S.p, c.pop_i32.p, c.pop_i64.p, c.nivl);
+    FS_HIP(hipGetLastError());
+    PopArgs p{};
+    p.nb = nb;
+    p.nknots = c.nknots;
+    p.nivl = c.nivl;
+    p.y = c.pop_y.p;
+    p.s = c.pop_s.p;
+    p.ivl32 = c.pop_i32.p;
+    p.ivl64 = c.pop_i64.p;
+    p.Fp = c.pop_Fp.p;
+    p.Fp32 = c.pop_Fp32.p;
+    p.grp_t0 = c.pop_grp.p;
+    p.partE = c.pop_partE.p;
+    p.partF = c.pop_partF.p;
+    p.C = c.C;
+    const dim3 grid(unsigned(nb) * G);
+    switch (c.lanes_per_atom) {
+#define FS_POP(L)                                                                                   \
+  case L:                                                                                           \
+    hipLaunchKernelGGL(k_pop_density<L>, grid, dim3(256), lds_d, c.stream, a, p);                   \
+    FS_HIP(hipGetLastError());                                                                      \
+    hipLaunchKernelGGL(k_pop_force<L>, grid, dim3(256), lds_f, c.stream, a, p);                     \
+    break;
+      FS_POP(4)
+      FS_POP(8)
+      FS_POP(16)
+      FS_POP(32)
+      FS_POP(64)
+#undef FS_POP
+    default: throw Error(FS_ERR_STATE, "unsupported lanes per atom");
+    }
+    FS_HIP(hipGetLastError());
+    const int64_t nmc = int64_t(nb) * c.C;
+    hipLaunchKernelGGL(k_pop_finalize, dim3(unsigned((nmc + 127) / 128)), dim3(128), 0, c.stream, nb,
+                       c.C, c.pop_grp.p, c.cfg_tile0.p, c.pop_partE.p, c.pop_partF.p, c.vol.p,
+                       c.ref_energy.p, c.ref_stress.p, c.energy_weight, c.stress_weight,
+                       c.pop_cmc.p);
+    FS_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_pop_sum, dim3(unsigned((nb + 127) / 128)), dim3(128), 0, c.stream, nb, c.C,
+                       c.pop_cmc.p, c.pop_cost.p);
+    FS_HIP(hipGetLastError());
+    FS_HIP(hipMemcpyAsync(cost + m0, c.pop_cost.p, sizeof(double) * nb, hipMemcpyDeviceToHost,
+                          c.stream));
+  }
+  FS_HIP(hipStreamSynchronize(c.stream));
+  return true;
 }
 
 void launch_cost(Context &c, double *d_out) {

@@ -156,6 +156,8 @@ int fs_lm_impl(Context &c, double *x, int max_iter, double *hist, int *accepted)
 
 int fs_population_impl(Context &c, const double *X, int np, double *cost) {
   const int D = c.D;
+  ensure_ready(c);
+  if (population_cost_batched(c, X, np, cost)) return FS_OK;
   std::vector<double> x0(D);
   gather_host(c, x0.data());
   for (int m = 0; m < np; ++m) {

@@ -1,0 +1,110 @@
+"""Batched population cost and the DE driver on the GPU against the FP64 oracle.
+
+The batched kernels (members x tile groups, one launch pair per batch) must
+give every member's cost within the mixed-precision cost bar (1e-6 relative)
+of the oracle, and agree with the single-member residual path to rounding
+(1e-10 relative). FP64 DE uses the oracle's counter RNG, so with FP64 costs
+its trajectory equals the oracle's.
+"""
+import numpy as np
+import pytest
+
+import forcesmith_amd as fa
+import oracle
+from forcesmith_amd import datasets as dsm
+
+pytestmark = pytest.mark.gpu
+
+
+def _print(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def wl():
+    return {"eam1": dsm.config1(n_configs=3), "eam3": dsm.config2(n_configs=10, nmin=32, nmax=300)}
+
+
+def members(x0, n, seed, clamp_member=None):
+    rng = np.random.default_rng(seed)
+    X = [x0.copy()]
+    for _ in range(n - 1):
+        X.append(x0 + rng.normal(size=x0.size) * (0.02 * np.abs(x0) + 1e-3))
+    if clamp_member is not None:
+        X.append(clamp_member)
+    return np.array(X)
+
+
+@pytest.mark.parametrize("name", ["eam1", "eam3"])
+def test_batched_population_matches_oracle(wl, name):
+    w = wl[name]
+    _print(f"population {name}")
+    o = oracle.Oracle(w.dataset, w.model)
+    o.prepare()
+    g = fa.GpuCalculator(w.dataset, w.model)
+    g.prepare()
+    x0 = g.gather_params()
+    # a member whose density tables are tripled pushes rho outside the
+    # embedding span: the limit rows are exercised
+    xc = x0.copy()
+    off = 0
+    for t, tab in enumerate(w.model.tables):
+        nfree = tab.free_count
+        if w.model.npair <= t < w.model.npair + w.model.ntypes:
+            xc[off:off + nfree] *= 3.0
+        off += nfree
+    X = members(x0, 17, 5, xc)
+    cg = g.population_cost(X)
+    co = o.population_cost(X)
+    rel = np.abs(cg - co) / co
+    _print(f"[{name}] members {len(X)} max rel vs oracle {rel.max():.3e}; clamp member cost {co[-1]:.4e}")
+    assert rel.max() <= 1e-6
+    assert co[-1] > 10 * co[0]  # the clamp member really is penalised
+    # the single-member residual path, same precision
+    cs = np.array([g.cost(x) for x in X])
+    assert np.max(np.abs(cg - cs) / cs) <= 1e-10
+
+
+def test_population_deterministic(wl):
+    w = wl["eam3"]
+    g = fa.GpuCalculator(w.dataset, w.model)
+    g.prepare()
+    X = members(g.gather_params(), 9, 11)
+    a, b = g.population_cost(X), g.population_cost(X)
+    assert np.array_equal(a, b)
+
+
+def test_de_fp64_matches_oracle(wl):
+    w = wl["eam1"]
+    _print("de fp64 vs oracle")
+    o = oracle.Oracle(w.dataset, w.model)
+    o.prepare()
+    g = fa.GpuCalculator(w.dataset, w.model, precision="fp64")
+    g.prepare()
+    x0 = g.gather_params()
+    lo, hi = x0 - 0.05 * np.abs(x0) - 1e-3, x0 + 0.05 * np.abs(x0) + 1e-3
+    xo, bo, ho = o.de(x0, lo, hi, 16, 5, seed=42)
+    xg, bg, hg = g.de(x0, lo, hi, 16, 5, seed=42)
+    _print(f"best oracle {bo:.12e} gpu {bg:.12e}")
+    assert abs(bg - bo) <= 1e-10 * bo
+    assert np.allclose(xg, xo, rtol=0, atol=1e-12 * (1 + np.abs(xo).max()))
+    assert np.allclose(hg, ho, rtol=1e-10)
+
+
+@pytest.mark.parametrize("surrogate", [False, True])
+def test_de_mixed_batched(wl, surrogate):
+    w = wl["eam3"]
+    _print(f"de mixed surrogate={surrogate}")
+    g = fa.GpuCalculator(w.dataset, w.model)
+    g.prepare()
+    x0 = g.gather_params()
+    lo, hi = x0 - 0.05 * np.abs(x0) - 1e-3, x0 + 0.05 * np.abs(x0) + 1e-3
+    x, best, hist = g.de(x0, lo, hi, 48, 6, seed=7, surrogate=surrogate, oversample=4,
+                         retrain_every=2, hidden=64)
+    assert np.all(x >= lo) and np.all(x <= hi)
+    assert np.all(np.diff(hist) <= 0)
+    assert best <= g.cost(np.clip(x0, lo, hi)) * (1 + 1e-12)
+    assert abs(best - g.cost(x)) <= 1e-9 * best  # the exact cost judges the reported best
+    x2, best2, _ = g.de(x0, lo, hi, 48, 6, seed=7, surrogate=surrogate, oversample=4,
+                        retrain_every=2, hidden=64)
+    assert best2 == best and np.array_equal(x, x2)
