@@ -127,6 +127,9 @@ int lm_minimize_device(Context &c, double *x, int max_iter, double *hist);
 // batched |F(x_m)|^2 over host parameter vectors X[np][D]; false when the
 // model / precision has no batched kernels (the caller then loops members)
 bool population_cost_batched(Context &c, const double *X, int np, double *cost);
+bool population_supported(const Context &c);
+void population_prepare(Context &c, int np);
+void population_launch(Context &c, const double *d_X, int np, double *d_cost);
 void launch_sw_rev(Context &c);
 void launch_sw_eval(Context &c);
 

@@ -1010,9 +1010,20 @@ void launch_eval(Context &c, bool) {
   FS_HIP(hipGetLastError());
 }
 
-bool population_cost_batched(Context &c, const double *X, int np, double *cost) {
-  if (c.kind != FS_EAM || c.precision != FS_MIXED || c.ntiles == 0 || c.D == 0 || np <= 0)
-    return false;
+bool population_supported(const Context &c) {
+  return c.kind == FS_EAM && c.precision == FS_MIXED && c.ntiles > 0 && c.D > 0;
+}
+
+// members per batch: per member 12 N bytes of embedding derivatives, the
+// per-(config, group) partials and its tables; about 4 GiB per batch
+int population_batch_cap(const Context &c, int np) {
+  const size_t per = 12 * size_t(c.N) + size_t(c.C) * kPopG * 8 * (kPopE + kPopF) +
+                     size_t(c.nknots) * 8 * 7 + size_t(c.nivl) * 64 + size_t(c.C) * 8 + 8 * c.D;
+  return int(std::max<size_t>(1, std::min<size_t>(size_t(std::max(np, 1)), (size_t(4) << 30) / per)));
+}
+
+// every allocation of a population evaluation of np members; no launch
+void population_prepare(Context &c, int np) {
   const int G = kPopG;
   if (c.h_pop_grp.size() != size_t(G + 1) || c.pop_grp_tiles != c.ntiles) {
     c.h_pop_grp.assign(G + 1, 0);
@@ -1020,14 +1031,10 @@ bool population_cost_batched(Context &c, const double *X, int np, double *cost) 
     c.pop_grp.alloc(G + 1);
     FS_HIP(hipMemcpyAsync(c.pop_grp.p, c.h_pop_grp.data(), 4 * (G + 1), hipMemcpyHostToDevice,
                           c.stream));
+    FS_HIP(hipStreamSynchronize(c.stream));
     c.pop_grp_tiles = c.ntiles;
   }
-  // batch size: per member 12 N bytes of embedding derivatives, 104 C G bytes
-  // of partials and its tables; about 4 GiB per batch
-  const size_t per = 12 * size_t(c.N) + size_t(c.C) * G * 8 * (kPopE + kPopF) +
-                     size_t(c.nknots) * 8 * 7 + size_t(c.nivl) * 64 + size_t(c.C) * 8 + 8 * c.D;
-  const int cap = int(std::max<size_t>(1, std::min<size_t>(size_t(np), (size_t(4) << 30) / per)));
-  c.pop_X.alloc(size_t(cap) * c.D);
+  const int cap = population_batch_cap(c, np);
   c.pop_y.alloc(size_t(cap) * c.nknots);
   c.pop_s.alloc(size_t(cap) * c.nknots);
   c.pop_S.alloc(size_t(cap) * 5 * c.nknots);
@@ -1038,17 +1045,22 @@ bool population_cost_batched(Context &c, const double *X, int np, double *cost) 
   c.pop_partE.alloc(size_t(cap) * c.C * G * kPopE);
   c.pop_partF.alloc(size_t(cap) * c.C * G * kPopF);
   c.pop_cmc.alloc(size_t(cap) * c.C);
-  c.pop_cost.alloc(size_t(cap));
+}
+
+// |F(x_m)|^2 of the np device vectors d_X[np][D] into d_cost[np]: launches
+// only (no allocation, copy or synchronisation, so a stream capture can
+// record it); population_prepare(c, np) must have run
+void population_launch(Context &c, const double *d_X, int np, double *d_cost) {
+  const int G = kPopG;
+  const int cap = population_batch_cap(c, np);
   EvalArgs a = make_args(c);
   const size_t lds_d = size_t(c.ntab) * sizeof(TableDesc);
   const size_t lds_f = lds_d + 2 * size_t(c.nivl) * sizeof(float4);
   for (int m0 = 0; m0 < np; m0 += cap) {
     const int nb = std::min(cap, np - m0);
-    FS_HIP(hipMemcpyAsync(c.pop_X.p, X + size_t(m0) * c.D, sizeof(double) * nb * c.D,
-                          hipMemcpyHostToDevice, c.stream));
     hipLaunchKernelGGL(k_pop_tables, dim3(nb), dim3(64), 0, c.stream, c.ntab, c.D, c.nknots,
-                       c.tabs.p, c.tab_x.p, c.tab_y.p, c.knot_of.p, c.pop_X.p, c.pop_y.p, c.pop_s.p,
-                       c.pop_S.p, c.pop_i32.p, c.pop_i64.p, c.nivl);
+                       c.tabs.p, c.tab_x.p, c.tab_y.p, c.knot_of.p, d_X + size_t(m0) * c.D, c.pop_y.p,
+                       c.pop_s.p, c.pop_S.p, c.pop_i32.p, c.pop_i64.p, c.nivl);
     FS_HIP(hipGetLastError());
     PopArgs p{};
     p.nb = nb;
@@ -1088,11 +1100,20 @@ bool population_cost_batched(Context &c, const double *X, int np, double *cost) 
                        c.pop_cmc.p);
     FS_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_pop_sum, dim3(unsigned((nb + 127) / 128)), dim3(128), 0, c.stream, nb, c.C,
-                       c.pop_cmc.p, c.pop_cost.p);
+                       c.pop_cmc.p, d_cost + m0);
     FS_HIP(hipGetLastError());
-    FS_HIP(hipMemcpyAsync(cost + m0, c.pop_cost.p, sizeof(double) * nb, hipMemcpyDeviceToHost,
-                          c.stream));
   }
+}
+
+bool population_cost_batched(Context &c, const double *X, int np, double *cost) {
+  if (!population_supported(c) || np <= 0) return false;
+  population_prepare(c, np);
+  c.pop_X.alloc(size_t(np) * c.D);
+  c.pop_cost.alloc(size_t(np));
+  FS_HIP(hipMemcpyAsync(c.pop_X.p, X, sizeof(double) * size_t(np) * c.D, hipMemcpyHostToDevice,
+                        c.stream));
+  population_launch(c, c.pop_X.p, np, c.pop_cost.p);
+  FS_HIP(hipMemcpyAsync(cost, c.pop_cost.p, sizeof(double) * np, hipMemcpyDeviceToHost, c.stream));
   FS_HIP(hipStreamSynchronize(c.stream));
   return true;
 }

@@ -108,3 +108,48 @@ def test_de_mixed_batched(wl, surrogate):
     x2, best2, _ = g.de(x0, lo, hi, 48, 6, seed=7, surrogate=surrogate, oversample=4,
                         retrain_every=2, hidden=64)
     assert best2 == best and np.array_equal(x, x2)
+
+
+def test_de_device_driver_matches_oracle(wl):
+    """The device-resident driver (hipGraph-replayed generations) draws the
+    oracle's trials; with costs within the mixed bar the selections agree."""
+    w = wl["eam1"]
+    o = oracle.Oracle(w.dataset, w.model)
+    o.prepare()
+    g = fa.GpuCalculator(w.dataset, w.model)
+    g.prepare()
+    x0 = g.gather_params()
+    lo, hi = x0 - 0.05 * np.abs(x0) - 1e-3, x0 + 0.05 * np.abs(x0) + 1e-3
+    xo, bo, ho = o.de(x0, lo, hi, 16, 5, seed=3)
+    xg, bg, hg = g.de(x0, lo, hi, 16, 5, seed=3)
+    _print(f"device DE best {bg:.10e} oracle {bo:.10e}")
+    assert np.array_equal(xg, xo)
+    assert abs(bg - bo) <= 1e-6 * bo
+    assert np.allclose(hg, ho, rtol=1e-6)
+
+
+def test_graph_replay_equals_eager_generations(wl):
+    """Plain DE replays one captured generation; the screened driver with a
+    single candidate runs the same kernels eagerly: identical trajectories."""
+    w = wl["eam3"]
+    g = fa.GpuCalculator(w.dataset, w.model)
+    g.prepare()
+    x0 = g.gather_params()
+    lo, hi = x0 - 0.1 * np.abs(x0) - 1e-3, x0 + 0.1 * np.abs(x0) + 1e-3
+    xa, ba, ha = g.de(x0, lo, hi, 40, 20, seed=9)
+    xb, bb, hb = g.de(x0, lo, hi, 40, 20, seed=9, surrogate=True, oversample=1)
+    assert np.array_equal(xa, xb) and ba == bb and np.array_equal(ha, hb)
+    assert np.all(np.diff(ha) <= 0)
+
+
+def test_de_target_stops_early(wl):
+    w = wl["eam3"]
+    g = fa.GpuCalculator(w.dataset, w.model)
+    g.prepare()
+    x0 = g.gather_params()
+    lo, hi = x0 - 0.1 * np.abs(x0) - 1e-3, x0 + 0.1 * np.abs(x0) + 1e-3
+    _, _, ha = g.de(x0, lo, hi, 40, 40, seed=9)
+    target = ha[9]
+    _, bt, ht = g.de(x0, lo, hi, 40, 40, seed=9, target_cost=target)
+    assert len(ht) == int(np.argmax(ha <= target)) + 1
+    assert bt <= target
