@@ -34,6 +34,18 @@ def parse():
                     help="target CPU time of the baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    # differential-evolution measurement (separate from the default line)
+    ap.add_argument("--de", choices=["plain", "surrogate", "both"], default=None,
+                    help="measure DE wall time to the plain run's final cost instead")
+    ap.add_argument("--de-np", type=int, default=3600)
+    ap.add_argument("--de-generations", type=int, default=1000)
+    ap.add_argument("--de-box", type=float, default=0.2,
+                    help="box half-width as a fraction of |x0| (plus 1e-3)")
+    ap.add_argument("--de-target", type=float, default=0.0,
+                    help="surrogate run: the plain run's final cost")
+    ap.add_argument("--de-plain-seconds", type=float, default=0.0,
+                    help="surrogate run: the plain run's wall time")
+    ap.add_argument("--de-lanes", type=int, default=4)
     return ap.parse_args()
 
 
@@ -64,8 +76,64 @@ def cpu_baseline(w, x, seconds, threads):
                       f"{t_prep:.1f} s untimed"}
 
 
+def de_measure(a):
+    """Surrogate-screened DE vs plain batched DE on the ternary EAM workload
+    (D = 240, NP = 15 D, F = 0.65, CR = 0.5, seed 42): wall time of the
+    screened run until its best cost is <= (1 + 1e-6) x the plain run's final
+    cost, over the plain run's wall time."""
+    import numpy as np
+
+    import forcesmith_amd as fa
+    from forcesmith_amd import datasets
+
+    log(f"generating the ternary EAM workload ({a.configs} configs, seed 2)")
+    w = datasets.config2(n_configs=a.configs)
+    g = fa.GpuCalculator(w.dataset, w.model)
+    g.prepare()
+    g.set_launch(a.de_lanes)
+    x0 = g.gather_params()
+    lo = x0 - a.de_box * np.abs(x0) - 1e-3
+    hi = x0 + a.de_box * np.abs(x0) + 1e-3
+    common = dict(F=0.65, CR=0.5, seed=42, progress_every=25)
+    out = {"metric": "DE wall-time to target cost vs plain GPU", "unit": "x", "higher_is_better": False,
+           "n_gpus": 1, "data": "synthetic (seeded ternary analytic EAM truth + noise)", "dtype": "fp32",
+           "config": {"workload": f"ternary tabulated EAM, {a.configs} configs, D={x0.size}",
+                      "np": a.de_np, "generations": a.de_generations, "F": 0.65, "CR": 0.5, "seed": 42,
+                      "box": f"x0 +- ({a.de_box}|x0| + 1e-3)", "lanes_per_atom": a.de_lanes,
+                      "surrogate": {"hidden": 256, "oversample": 4, "retrain_every": 10}}}
+    t_plain, c_plain = a.de_plain_seconds, a.de_target
+    if a.de in ("plain", "both"):
+        t0 = time.perf_counter()
+        _, c_plain, hp = g.de(x0, lo, hi, a.de_np, a.de_generations, **common)
+        t_plain = time.perf_counter() - t0
+        out["plain"] = {"seconds": t_plain, "final_cost": c_plain, "generations": len(hp),
+                        "ms_per_generation": 1e3 * t_plain / max(len(hp), 1)}
+        log(f"plain: {len(hp)} generations {t_plain:.1f} s final {c_plain:.9e}")
+    if a.de in ("surrogate", "both"):
+        if c_plain <= 0:
+            raise SystemExit("--de surrogate needs --de-target (the plain run's final cost)")
+        target = c_plain * (1 + 1e-6)
+        t0 = time.perf_counter()
+        _, c_s, hs = g.de(x0, lo, hi, a.de_np, a.de_generations, surrogate=True, oversample=4,
+                          retrain_every=10, hidden=256, target_cost=target, **common)
+        t_s = time.perf_counter() - t0
+        out["surrogate"] = {"seconds": t_s, "final_cost": c_s, "generations": len(hs),
+                            "reached": bool(c_s <= target),
+                            "ms_per_generation": 1e3 * t_s / max(len(hs), 1)}
+        log(f"surrogate: {len(hs)} generations {t_s:.1f} s best {c_s:.9e} reached {c_s <= target}")
+        if t_plain > 0:
+            out["value"] = t_s / t_plain if c_s <= target else None
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
+    if a.de:
+        if a.gpus != 1:
+            print("bench.py measures one GPU", file=sys.stderr)
+            sys.exit(2)
+        de_measure(a)
+        return
     if a.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {a.gpus} is not supported; this build runs the "
               "evaluator on a single device", file=sys.stderr)

@@ -56,7 +56,7 @@ class fs_de_opts(ct.Structure):
     _fields_ = [("np", ct.c_int32), ("generations", ct.c_int32), ("F", ct.c_double), ("CR", ct.c_double),
                 ("seed", ct.c_uint64), ("surrogate", ct.c_int32), ("oversample", ct.c_int32),
                 ("retrain_every", ct.c_int32), ("hidden", ct.c_int32), ("target_cost", ct.c_double),
-                ("rank_tau_min", ct.c_double), ("reserved", ct.c_int32 * 4)]
+                ("rank_tau_min", ct.c_double), ("progress_every", ct.c_int32), ("reserved", ct.c_int32 * 3)]
 
 
 _lib = None
@@ -306,10 +306,10 @@ class GpuCalculator:
         return c
 
     def de(self, x, lo, hi, np_, generations, F=0.65, CR=0.5, seed=42, surrogate=False, oversample=4,
-           retrain_every=10, hidden=256, target_cost=0.0, rank_tau_min=0.3):
+           retrain_every=10, hidden=256, target_cost=0.0, rank_tau_min=0.3, progress_every=0):
         o = fs_de_opts(int(np_), int(generations), float(F), float(CR), int(seed), int(bool(surrogate)),
                        int(oversample), int(retrain_every), int(hidden), float(target_cost),
-                       float(rank_tau_min))
+                       float(rank_tau_min), int(progress_every))
         x = np.ascontiguousarray(x, dtype=np.float64).copy()
         best = ct.c_double()
         hist = np.zeros(generations)

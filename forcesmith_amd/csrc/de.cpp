@@ -13,7 +13,9 @@
 #include "mlp.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <limits>
 
 namespace fsa {
@@ -545,6 +547,16 @@ int de_run_device(Context &c, const fs_de_opts *o, const double *lo, const doubl
 
   std::vector<double> h(std::max(G, 1));
   int done = 0;
+  const auto t_start = std::chrono::steady_clock::now();
+  auto progress = [&](int gdone) {
+    if (o->progress_every <= 0 || gdone % o->progress_every != 0) return;
+    double b = 0.0;
+    FS_HIP(hipMemcpyAsync(&b, d.hist.p + (gdone - 1), 8, hipMemcpyDeviceToHost, st));
+    FS_HIP(hipStreamSynchronize(st));
+    const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+    std::fprintf(stderr, "de: generation %d best %.9e elapsed %.1f s\n", gdone, b, el);
+    std::fflush(stderr);
+  };
   const int chunk = 16; // generations between history reads (early-stop checks)
   auto reached = [&](int upto) {
     if (o->target_cost <= 0.0) return -1;
@@ -574,7 +586,10 @@ int de_run_device(Context &c, const fs_de_opts *o, const double *lo, const doubl
     try {
       while (done < G && stop < 0) {
         const int n = std::min(chunk, G - done);
-        for (int k = 0; k < n; ++k) FS_HIP(hipGraphLaunch(exec, st));
+        for (int k = 0; k < n; ++k) {
+          FS_HIP(hipGraphLaunch(exec, st));
+          progress(done + k + 1);
+        }
         done += n;
         stop = reached(done);
       }
@@ -640,6 +655,7 @@ int de_run_device(Context &c, const fs_de_opts *o, const double *lo, const doubl
         }
       }
       ++done;
+      progress(done);
       if (o->target_cost > 0.0) {
         double b = 0.0;
         FS_HIP(hipMemcpyAsync(&b, d.hist.p + (done - 1), 8, hipMemcpyDeviceToHost, st));

@@ -707,23 +707,20 @@ __global__ void k_pop_tables(int ntab, int D, int nknots, const TableDesc *__res
   }
 }
 
-__device__ inline void pop_flush(double *dst, const double *v, int n) {
-  for (int k = 0; k < n; ++k) dst[k] = v[k];
-}
-
 template <int LPA>
 __global__ __launch_bounds__(256) void k_pop_density(EvalArgs a, PopArgs p) {
   constexpr int APB = 256 / LPA;
   extern __shared__ float4 smem[];
   TableDesc *sT = reinterpret_cast<TableDesc *>(smem);
+  double *P64 = reinterpret_cast<double *>(smem + a.ntab * int(sizeof(TableDesc) / sizeof(float4)));
   __shared__ double sRed[APB][kPopE];
   const int g = blockIdx.x % kPopG, m = blockIdx.x / kPopG;
   for (int k = threadIdx.x; k < a.ntab; k += blockDim.x) sT[k] = a.tabs[k];
+  for (int k = threadIdx.x; k < 4 * p.nivl; k += blockDim.x) P64[k] = p.ivl64[size_t(m) * 4 * p.nivl + k];
   __syncthreads();
-  const double *P64 = p.ivl64 + size_t(m) * 4 * p.nivl;
   const double *ym = p.y + size_t(m) * p.nknots, *sm = p.s + size_t(m) * p.nknots;
   const int sub = threadIdx.x / LPA, lane = threadIdx.x % LPA;
-  double run[kPopE] = {0.0, 0.0};
+  double run = 0.0; // component threadIdx.x (< kPopE) of the running config partial
   int run_c = -1;
   for (int tile = p.grp_t0[g]; tile < p.grp_t0[g + 1]; ++tile) {
     const int na = a.tile_na[tile];
@@ -763,21 +760,18 @@ __global__ __launch_bounds__(256) void k_pop_density(EvalArgs a, PopArgs p) {
       }
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < kPopE) {
       if (c != run_c) {
-        if (run_c >= 0) pop_flush(p.partE + ((size_t(m) * p.C + run_c) * kPopG + g) * kPopE, run, kPopE);
-        run[0] = run[1] = 0.0;
+        if (run_c >= 0) p.partE[((size_t(m) * p.C + run_c) * kPopG + g) * kPopE + threadIdx.x] = run;
+        run = 0.0;
         run_c = c;
       }
-      for (int q = 0; q < na && q < APB; ++q) {
-        run[0] += sRed[q][0];
-        run[1] += sRed[q][1];
-      }
+      for (int q = 0; q < na && q < APB; ++q) run += sRed[q][threadIdx.x];
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0 && run_c >= 0)
-    pop_flush(p.partE + ((size_t(m) * p.C + run_c) * kPopG + g) * kPopE, run, kPopE);
+  if (threadIdx.x < kPopE && run_c >= 0)
+    p.partE[((size_t(m) * p.C + run_c) * kPopG + g) * kPopE + threadIdx.x] = run;
 }
 
 template <int LPA>
@@ -795,16 +789,17 @@ __global__ __launch_bounds__(256) void k_pop_force(EvalArgs a, PopArgs p) {
     for (int k = threadIdx.x; k < nq; k += blockDim.x) dst[k] = src[k];
     const float4 *mi = p.ivl32 + size_t(m) * 2 * p.nivl;
     for (int k = threadIdx.x; k < 2 * p.nivl; k += blockDim.x) sI[k] = mi[k];
+    double *sP = reinterpret_cast<double *>(sI + 2 * p.nivl);
+    for (int k = threadIdx.x; k < 4 * p.nivl; k += blockDim.x) sP[k] = p.ivl64[size_t(m) * 4 * p.nivl + k];
     __syncthreads();
   }
   const Tabs tb{sT, sI, a.tx, p.y + size_t(m) * p.nknots, p.s + size_t(m) * p.nknots,
-                p.ivl64 + size_t(m) * 4 * p.nivl};
+                reinterpret_cast<const double *>(sI + 2 * p.nivl)};
   EvalArgs am = a; // the member's embedding derivatives
   am.Fp = p.Fp + size_t(m) * a.N;
   am.Fp32 = p.Fp32 + size_t(m) * a.N;
   const int sub = threadIdx.x / LPA, lane = threadIdx.x % LPA;
-  double run[kPopF];
-  for (int k = 0; k < kPopF; ++k) run[k] = 0.0;
+  double run = 0.0; // component threadIdx.x (< kPopF) of the running config partial
   int run_c = -1;
   for (int tile = p.grp_t0[g]; tile < p.grp_t0[g + 1]; ++tile) {
     const int na = a.tile_na[tile];
@@ -854,19 +849,18 @@ __global__ __launch_bounds__(256) void k_pop_force(EvalArgs a, PopArgs p) {
       }
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < kPopF) {
       if (c != run_c) {
-        if (run_c >= 0) pop_flush(p.partF + ((size_t(m) * p.C + run_c) * kPopG + g) * kPopF, run, kPopF);
-        for (int k = 0; k < kPopF; ++k) run[k] = 0.0;
+        if (run_c >= 0) p.partF[((size_t(m) * p.C + run_c) * kPopG + g) * kPopF + threadIdx.x] = run;
+        run = 0.0;
         run_c = c;
       }
-      for (int q = 0; q < na && q < APB; ++q)
-        for (int k = 0; k < kPopF; ++k) run[k] += sRed[q][k];
+      for (int q = 0; q < na && q < APB; ++q) run += sRed[q][threadIdx.x];
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0 && run_c >= 0)
-    pop_flush(p.partF + ((size_t(m) * p.C + run_c) * kPopG + g) * kPopF, run, kPopF);
+  if (threadIdx.x < kPopF && run_c >= 0)
+    p.partF[((size_t(m) * p.C + run_c) * kPopG + g) * kPopF + threadIdx.x] = run;
 }
 
 // cost of every (member, configuration): the residual rows of
@@ -1054,8 +1048,9 @@ void population_launch(Context &c, const double *d_X, int np, double *d_cost) {
   const int G = kPopG;
   const int cap = population_batch_cap(c, np);
   EvalArgs a = make_args(c);
-  const size_t lds_d = size_t(c.ntab) * sizeof(TableDesc);
-  const size_t lds_f = lds_d + 2 * size_t(c.nivl) * sizeof(float4);
+  const size_t lds_t = size_t(c.ntab) * sizeof(TableDesc);
+  const size_t lds_d = lds_t + 4 * size_t(c.nivl) * sizeof(double);
+  const size_t lds_f = lds_t + 2 * size_t(c.nivl) * sizeof(float4) + 4 * size_t(c.nivl) * sizeof(double);
   for (int m0 = 0; m0 < np; m0 += cap) {
     const int nb = std::min(cap, np - m0);
     hipLaunchKernelGGL(k_pop_tables, dim3(nb), dim3(64), 0, c.stream, c.ntab, c.D, c.nknots,

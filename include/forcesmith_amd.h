@@ -155,7 +155,9 @@ typedef struct fs_de_opts {
   int32_t hidden;      /* MLP width */
   double target_cost;  /* stop when best <= target (<= 0: run all) */
   double rank_tau_min; /* fallback when Kendall tau of the surrogate drops */
-  int32_t reserved[4];
+  int32_t progress_every; /* print generation, best cost and elapsed time to
+                             stderr every n generations (0: silent) */
+  int32_t reserved[3];
 } fs_de_opts;
 /* Differential evolution with the whole population evaluated per batched
  * launch; replaces BoostDESolver::minimize (solver.cpp:113-155). lo/hi: box;
