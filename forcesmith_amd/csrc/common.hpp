@@ -229,6 +229,7 @@ struct Context {
   DBuf<float> Hs, emb_d, emb_v;
   DBuf<int32_t> emb_k, sw_csw, sw_clam;
   DBuf<double> sw_own, sw_ptan, sw_etan;
+  DBuf<float> adp_tan; // ADP Jacobian tangents [N][10][D]
   DBuf<double> lm_M, lm_dx, lm_scal;
   // batched population cost: member tables, embedding derivatives, partials
   DBuf<double> pop_X, pop_y, pop_s, pop_S, pop_i64, pop_Fp, pop_partE, pop_partF, pop_cmc, pop_cost;

@@ -123,6 +123,8 @@ struct JacArgs {
   int32_t *emb_k;         // [N]
   float *emb_d, *emb_v;   // [N x 4]
   double *Fpp;            // [N]
+  const double *mu, *lamb; // ADP per-atom dipole (3) and quadrupole (6)
+  float *tan;             // ADP tangents [N][10][D]
   float *J;               // [R x D]
   double *part;           // [ntiles x kJP x D]
   const double *vol;
@@ -202,6 +204,63 @@ __device__ inline void chunk_segments(int ty, int ntypes, int seg[kMaxTypes + 1]
     seg[t] = t <= ntypes ? int(__popcll(__ballot(ty < t))) : seg[ntypes];
 }
 
+// Embedding window of atom i at its clamped density: interval, FP32 value and
+// derivative basis (for the embedding columns) and F''(rho) (FP64), with the
+// cached-site rules of the embedding lookup (linear outside the span).
+__device__ double emb_window(const JacArgs &a, const TableDesc *sT, int64_t i) {
+  const int ti = a.types[i];
+  const TableDesc Te = sT[a.npair + a.ntypes + ti];
+  const double rb = a.rho[i];
+  const double *x = a.tx + Te.off, *y = a.ty + Te.off, *s = a.ts + Te.off;
+  const int n = Te.n;
+  int k;
+  bool lin;
+  if (rb <= x[0]) {
+    k = 0;
+    lin = true;
+  } else if (rb >= x[n - 1]) {
+    k = n - 2;
+    lin = true;
+  } else {
+    k = spl_locate(x, n, rb);
+    lin = Te.linear != 0;
+  }
+  const double dx = x[k + 1] - x[k];
+  float dcoef[4], vcoef[4];
+  double F2 = 0.0;
+  if (lin) {
+    const double anchor = (rb >= x[n - 1] && rb > x[0]) ? x[n - 1] : x[k];
+    const double f = (rb - anchor) / dx;
+    if (rb >= x[n - 1] && rb > x[0]) {
+      vcoef[0] = float(-f);
+      vcoef[1] = float(1.0 + f);
+    } else {
+      vcoef[0] = float(1.0 - f);
+      vcoef[1] = float(f);
+    }
+    vcoef[2] = vcoef[3] = 0.0f;
+    dcoef[0] = float(-1.0 / dx);
+    dcoef[1] = float(1.0 / dx);
+    dcoef[2] = dcoef[3] = 0.0f;
+  } else {
+    double bv[4], bd[4], b2[4];
+    herm_basis64(x[k], dx, rb, bv, bd);
+    herm_basis2_64(x[k], dx, rb, b2);
+    for (int q = 0; q < 4; ++q) {
+      vcoef[q] = float(bv[q]);
+      dcoef[q] = float(bd[q]);
+    }
+    F2 = b2[0] * y[k] + b2[1] * y[k + 1] + b2[2] * s[k] + b2[3] * s[k + 1];
+  }
+  a.emb_k[i] = k;
+  for (int q = 0; q < 4; ++q) {
+    a.emb_d[4 * i + q] = dcoef[q];
+    a.emb_v[4 * i + q] = vcoef[q];
+  }
+  a.Fpp[i] = F2;
+  return F2;
+}
+
 // ---------------------------------------------------------------------------
 // H_i = d rho_i / d y over the density columns, from staged band coefficients
 struct DensStage {
@@ -266,60 +325,9 @@ __global__ __launch_bounds__(256) void k_jac_dens(JacArgs a, int nknots) {
       wave_sync();
     }
     // embedding at the clamped density (lane 0), broadcast F'' and the clamp
-    const int ti = a.types[i];
-    const TableDesc Te = sT[a.npair + a.ntypes + ti];
-    const double rb = a.rho[i];
     const bool clamped = a.over[i] != 0.0;
     double F2 = 0.0;
-    if (lane == 0) {
-      const double *x = a.tx + Te.off, *y = a.ty + Te.off, *s = a.ts + Te.off;
-      const int n = Te.n;
-      int k;
-      bool lin;
-      if (rb <= x[0]) {
-        k = 0;
-        lin = true;
-      } else if (rb >= x[n - 1]) {
-        k = n - 2;
-        lin = true;
-      } else {
-        k = spl_locate(x, n, rb);
-        lin = Te.linear != 0;
-      }
-      const double dx = x[k + 1] - x[k];
-      float dcoef[4], vcoef[4];
-      if (lin) {
-        const double anchor = (rb >= x[n - 1] && rb > x[0]) ? x[n - 1] : x[k];
-        const double f = (rb - anchor) / dx;
-        if (rb >= x[n - 1] && rb > x[0]) {
-          vcoef[0] = float(-f);
-          vcoef[1] = float(1.0 + f);
-        } else {
-          vcoef[0] = float(1.0 - f);
-          vcoef[1] = float(f);
-        }
-        vcoef[2] = vcoef[3] = 0.0f;
-        dcoef[0] = float(-1.0 / dx);
-        dcoef[1] = float(1.0 / dx);
-        dcoef[2] = dcoef[3] = 0.0f;
-        F2 = 0.0;
-      } else {
-        double bv[4], bd[4], b2[4];
-        herm_basis64(x[k], dx, rb, bv, bd);
-        herm_basis2_64(x[k], dx, rb, b2);
-        for (int q = 0; q < 4; ++q) {
-          vcoef[q] = float(bv[q]);
-          dcoef[q] = float(bd[q]);
-        }
-        F2 = b2[0] * y[k] + b2[1] * y[k + 1] + b2[2] * s[k] + b2[3] * s[k + 1];
-      }
-      a.emb_k[i] = k;
-      for (int q = 0; q < 4; ++q) {
-        a.emb_d[4 * i + q] = dcoef[q];
-        a.emb_v[4 * i + q] = vcoef[q];
-      }
-      a.Fpp[i] = F2;
-    }
+    if (lane == 0) F2 = emb_window(a, sT, i);
     F2 = __shfl(F2, 0, 64);
     const float hs = clamped ? 0.0f : float(F2);
 #pragma unroll
@@ -349,9 +357,11 @@ __global__ __launch_bounds__(256) void k_jac_dens(JacArgs a, int nknots) {
 // Gs_j = sum_i g'_ti(r) r u u. Those pair loops therefore accumulate forces
 // only, and the matching per-atom / F'_i terms carry their virial twice.
 struct JStage {
-  float r[64], u0[64], u1[64], u2[64], dgi[64];
+  float4 g[64];          // unit vector and distance {u0, u1, u2, r}
+  float2 cphi[6][64];    // pair-table band: {force coefficient, half energy}
+  float dgi[64];
   int32_t jg[64], kphi[64], kgj[64], kgi[64], kfj[64];
-  float cphi[6][64], ephi[6][64], cgj[6][64], cgi[6][64], cemb[6][64];
+  float cgj[6][64], cgi[6][64], cemb[6][64];
 };
 
 struct SlotDesc {
@@ -459,10 +469,7 @@ __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots, const S
           const int32_t jg = int32_t(cbase + pj_local(b));
           ty = tj;
           const float u0 = q.x * inv_r, u1 = q.y * inv_r, u2 = q.z * inv_r;
-          P.r[lane] = rr;
-          P.u0[lane] = u0;
-          P.u1[lane] = u1;
-          P.u2[lane] = u2;
+          P.g[lane] = make_float4(u0, u1, u2, rr);
           P.jg[lane] = jg;
           const float fpj = a.Fp32[jg];
           float bv[4], bd[4], cc[6], ce[6];
@@ -475,10 +482,7 @@ __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots, const S
             band6(sS, T, k, bv, 0.5f * w, ce);
             P.kphi[lane] = k;
 #pragma unroll
-            for (int d = 0; d < 6; ++d) {
-              P.cphi[d][lane] = cc[d];
-              P.ephi[d][lane] = ce[d];
-            }
+            for (int d = 0; d < 6; ++d) P.cphi[d][lane] = make_float2(cc[d], ce[d]);
           }
           // density of j, weight F'_i; also g'_tj(r) for G_i
           float dgj = 0.0f;
@@ -551,9 +555,10 @@ __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots, const S
             const int lo = first_ge(P.kphi, seg[ptt[s]], s1, m - 3);
             const int hi = first_ge(P.kphi, lo, s1, m + 3);
             for (int q = lo; q < hi; ++q) {
-              const int d = m - P.kphi[q] + 2;
-              add_pair(A[s], P.cphi[d][q], P.r[q], P.u0[q], P.u1[q], P.u2[q]);
-              A[s].e += P.ephi[d][q];
+              const float2 ce = P.cphi[m - P.kphi[q] + 2][q];
+              const float4 gq = P.g[q];
+              add_pair(A[s], ce.x, gq.w, gq.x, gq.y, gq.z);
+              A[s].e += ce.y;
             }
           } else if (kind == CK_DENS) {
             const int t = ctab[s] - a.npair;
@@ -564,8 +569,9 @@ __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots, const S
               const int hi = first_ge(P.kgj, lo, s1, m + 3);
               for (int q = lo; q < hi; ++q) {
                 const float cq = P.cgj[m - P.kgj[q] + 2][q];
-                add_force(A[s], cq, P.u0[q], P.u1[q], P.u2[q]);
-                add_virial(A[s], 2.0f * cq, P.r[q], P.u0[q], P.u1[q], P.u2[q]);
+                const float4 gq = P.g[q];
+                add_force(A[s], cq, gq.x, gq.y, gq.z);
+                add_virial(A[s], 2.0f * cq, gq.w, gq.x, gq.y, gq.z);
               }
             }
             // F'_j g'_ti(r), ti == t, every partner type
@@ -574,8 +580,10 @@ __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots, const S
                 const int s1 = seg[tt + 1];
                 const int lo = first_ge(P.kgi, seg[tt], s1, m - 3);
                 const int hi = first_ge(P.kgi, lo, s1, m + 3);
-                for (int q = lo; q < hi; ++q)
-                  add_force(A[s], P.cgi[m - P.kgi[q] + 2][q], P.u0[q], P.u1[q], P.u2[q]);
+                for (int q = lo; q < hi; ++q) {
+                  const float4 gq = P.g[q];
+                  add_force(A[s], P.cgi[m - P.kgi[q] + 2][q], gq.x, gq.y, gq.z);
+                }
               }
             }
             // F''_j H_j g'_ti(r): every pair, rows of Hs gathered eight at a time
@@ -588,7 +596,10 @@ __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots, const S
 #pragma unroll
               for (int v = 0; v < 8; ++v) {
                 const int q = q0 + v;
-                if (q < np) add_force(A[s], h[v] * P.dgi[q], P.u0[q], P.u1[q], P.u2[q]);
+                if (q < np) {
+                  const float4 gq = P.g[q];
+                  add_force(A[s], h[v] * P.dgi[q], gq.x, gq.y, gq.z);
+                }
               }
             }
           } else if (kind == CK_EMB) {
@@ -597,7 +608,8 @@ __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots, const S
             for (int q = seg[t]; q < seg[t + 1]; ++q) {
               const int d = m - P.kfj[q] + 2;
               if (d < 0 || d > 5) continue;
-              add_force(A[s], P.cemb[d][q], P.u0[q], P.u1[q], P.u2[q]);
+              const float4 gq = P.g[q];
+              add_force(A[s], P.cemb[d][q], gq.x, gq.y, gq.z);
             }
           }
         }
@@ -659,6 +671,401 @@ __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots, const S
       a.part[(int64_t(tile) * kJP + q) * a.D + col] = sPart[q][col];
     }
     __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// ADP Jacobian: forward-mode tangents per column, one wave per atom, lanes
+// own columns (kind-homogeneous slots). The chain is the one of the
+// reference's ADP parameter Jacobian (ADPForceCalculator::write_param_jacobian,
+// src/force/adp_force.cpp:141-385) with spline band coefficients:
+//   pass 1 (k_adp_tan1): d rho_i, d mu_i (3), d lambda_i (6) per column;
+//   pass 2 (k_adp_tan2): d F'_i = F''_i d rho_i (0 when clamped) + the
+//   embedding band, the pair-table, density, embedding, dipole and quadrupole
+//   terms of every pair's force, the energy (F' d rho + mu.d mu +
+//   lambda:d lambda - tr lambda tr d lambda / 3 + half d phi), virial and
+//   limit (over d rho) tangents.
+// ---------------------------------------------------------------------------
+constexpr int kTan = 10; // d rho, d mu (3), d lambda (xx, yy, zz, xy, xz, yz)
+
+struct AStage1 {
+  float d0[64], d1[64], d2[64];
+  int32_t tg[64], tu[64], tw[64], kg[64], ku[64], kw[64];
+  float cg[6][64], cu[6][64], cw[6][64];
+};
+
+template <int NS>
+__global__ __launch_bounds__(64) void k_adp_tan1(JacArgs a, int nknots, const SlotDesc *slots) {
+  extern __shared__ float4 smem[];
+  TableDesc *sT = reinterpret_cast<TableDesc *>(smem);
+  float *sS = reinterpret_cast<float *>(smem + a.ntab * int(sizeof(TableDesc) / sizeof(float4)));
+  AStage1 &P = *reinterpret_cast<AStage1 *>(sS + ((nknots * 5 + 3) & ~3));
+  stage(a, sT, sS, nknots);
+  const int lane = threadIdx.x;
+  const int dip0 = a.npair + 2 * a.ntypes, quad0 = 2 * a.npair + 2 * a.ntypes;
+  int ccol[NS], ctab[NS], cm[NS], ckind[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const SlotDesc sd = slots[s * 64 + lane];
+    ccol[s] = sd.col;
+    ctab[s] = sd.tab;
+    cm[s] = sd.m;
+    ckind[s] = sd.kind;
+  }
+  for (int64_t i = blockIdx.x; i < a.N; i += gridDim.x) {
+    const int ti = a.types[i];
+    const int64_t p0 = a.pair_off[i], p1 = a.pair_off[i + 1];
+    float T[NS][kTan];
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+#pragma unroll
+      for (int q = 0; q < kTan; ++q) T[s][q] = 0.0f;
+    for (int64_t pc = p0; pc < p1; pc += 64) {
+      const int64_t p = pc + lane;
+      if (p < p1) {
+        const float4 q = a.rec[p];
+        const uint32_t b = __float_as_uint(q.w);
+        const bool tiny = (b & kFlagTiny) != 0;
+        const float r = sqrtf(q.x * q.x + q.y * q.y + q.z * q.z);
+        const int tj = pj_type(b), o = ord_pair(ti, tj, a.ntypes);
+        P.d0[lane] = q.x;
+        P.d1[lane] = q.y;
+        P.d2[lane] = q.z;
+        float bv[4], bd[4], c[6];
+        const int tg = (!tiny && (b & kFlagGj)) ? a.npair + tj : -1;
+        const int tu = (!tiny && (b & kFlagDip)) ? dip0 + o : -1;
+        const int tw = (!tiny && (b & kFlagQuad)) ? quad0 + o : -1;
+        P.tg[lane] = tg;
+        P.tu[lane] = tu;
+        P.tw[lane] = tw;
+        if (tg >= 0) {
+          P.kg[lane] = pair_basis(sT[tg], r, bv, bd);
+          band6(sS, sT[tg], P.kg[lane], bv, 1.0f, c);
+          for (int d = 0; d < 6; ++d) P.cg[d][lane] = c[d];
+        }
+        if (tu >= 0) {
+          P.ku[lane] = pair_basis(sT[tu], r, bv, bd);
+          band6(sS, sT[tu], P.ku[lane], bv, 1.0f, c);
+          for (int d = 0; d < 6; ++d) P.cu[d][lane] = c[d];
+        }
+        if (tw >= 0) {
+          P.kw[lane] = pair_basis(sT[tw], r, bv, bd);
+          band6(sS, sT[tw], P.kw[lane], bv, 1.0f, c);
+          for (int d = 0; d < 6; ++d) P.cw[d][lane] = c[d];
+        }
+      }
+      wave_sync();
+      const int np = int(min<int64_t>(64, p1 - pc));
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        const int kind = ckind[s], m = cm[s], tab = ctab[s];
+        if (kind == CK_DENS) {
+          for (int q = 0; q < np; ++q) {
+            if (P.tg[q] != tab) continue;
+            const int d = m - P.kg[q] + 2;
+            if (d >= 0 && d < 6) T[s][0] += P.cg[d][q];
+          }
+        } else if (kind == CK_DIP) {
+          for (int q = 0; q < np; ++q) {
+            if (P.tu[q] != tab) continue;
+            const int d = m - P.ku[q] + 2;
+            if (d < 0 || d > 5) continue;
+            const float v = P.cu[d][q];
+            T[s][1] += v * P.d0[q];
+            T[s][2] += v * P.d1[q];
+            T[s][3] += v * P.d2[q];
+          }
+        } else if (kind == CK_QUAD) {
+          for (int q = 0; q < np; ++q) {
+            if (P.tw[q] != tab) continue;
+            const int d = m - P.kw[q] + 2;
+            if (d < 0 || d > 5) continue;
+            const float v = P.cw[d][q], x = P.d0[q], y = P.d1[q], z = P.d2[q];
+            T[s][4] += v * (x * x);
+            T[s][5] += v * (y * y);
+            T[s][6] += v * (z * z);
+            T[s][7] += v * (x * y);
+            T[s][8] += v * (x * z);
+            T[s][9] += v * (y * z);
+          }
+        }
+      }
+      wave_sync();
+    }
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      if (ckind[s] == CK_NONE) continue;
+      for (int q = 0; q < kTan; ++q) a.tan[(i * kTan + q) * a.D + ccol[s]] = T[s][q];
+    }
+    if (lane == 0) (void)emb_window(a, sT, i);
+  }
+}
+
+struct AStage2 {
+  float r[64], d0[64], d1[64], d2[64];
+  float dgj[64], dgi[64], u[64], du[64], w[64], dw[64];
+  float fpj[64], fppj[64], muj[3][64], nu[64], xi[3][64];
+  int32_t jg[64], tj[64], kfj[64], tp[64], tgj[64], tgi[64], tu[64], tw[64];
+  int32_t kp[64], kgj[64], kgi[64], ku[64], kw[64];
+  float ebj[4][64];
+  float cpd[6][64], cpv[6][64], cgj[6][64], cgi[6][64], cud[6][64], cuv[6][64], cwd[6][64],
+      cwv[6][64];
+};
+
+// nu(L, d) and xi(L, d) of a symmetric L stored as (xx, yy, zz, xy, xz, yz)
+// (quad_nu / quad_xi, adp_force.hpp:63-68)
+__device__ inline float qnu(const float *L, float x, float y, float z, float xi[3]) {
+  const float Mx = L[0] * x + L[3] * y + L[4] * z;
+  const float My = L[3] * x + L[1] * y + L[5] * z;
+  const float Mz = L[4] * x + L[5] * y + L[2] * z;
+  const float tr3 = (L[0] + L[1] + L[2]) / 3.0f;
+  xi[0] = Mx - tr3 * x;
+  xi[1] = My - tr3 * y;
+  xi[2] = Mz - tr3 * z;
+  return (x * Mx + y * My + z * Mz) - (x * x + y * y + z * z) * tr3;
+}
+
+template <int NS>
+__global__ __launch_bounds__(64) void k_adp_tan2(JacArgs a, int nknots, const SlotDesc *slots) {
+  extern __shared__ float4 smem[];
+  TableDesc *sT = reinterpret_cast<TableDesc *>(smem);
+  float *sS = reinterpret_cast<float *>(smem + a.ntab * int(sizeof(TableDesc) / sizeof(float4)));
+  AStage2 &P = *reinterpret_cast<AStage2 *>(sS + ((nknots * 5 + 3) & ~3));
+  stage(a, sT, sS, nknots);
+  const int lane = threadIdx.x;
+  const int dip0 = a.npair + 2 * a.ntypes, quad0 = 2 * a.npair + 2 * a.ntypes;
+  const int emb0 = a.npair + a.ntypes;
+  int ccol[NS], ctab[NS], cm[NS], ckind[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const SlotDesc sd = slots[s * 64 + lane];
+    ccol[s] = sd.col;
+    ctab[s] = sd.tab;
+    cm[s] = sd.m;
+    ckind[s] = sd.kind;
+  }
+  const int tile = blockIdx.x;
+  const int na = a.tile_na[tile];
+  const int c = a.tile_cfg[tile];
+  const int64_t cbase = a.atom_off[c];
+  float tp[NS][kJP];
+#pragma unroll
+  for (int s = 0; s < NS; ++s)
+#pragma unroll
+    for (int q = 0; q < kJP; ++q) tp[s][q] = 0.0f;
+  for (int li = 0; li < na; ++li) {
+    const int64_t i = int64_t(a.tile_a0[tile]) + li;
+    const int ti = a.types[i];
+    const bool clamped = a.over[i] != 0.0;
+    const float Fpi = a.Fp32[i], Fppi = clamped ? 0.0f : float(a.Fpp[i]);
+    const float mui[3] = {float(a.mu[3 * i]), float(a.mu[3 * i + 1]), float(a.mu[3 * i + 2])};
+    float Li[6];
+    for (int q = 0; q < 6; ++q) Li[q] = float(a.lamb[6 * i + q]);
+    const int kfi = a.emb_k[i];
+    float ebi[4], evi[4];
+    for (int q = 0; q < 4; ++q) {
+      ebi[q] = a.emb_d[4 * i + q];
+      evi[q] = a.emb_v[4 * i + q];
+    }
+    // own tangents and the per-atom parts
+    float Ti[NS][kTan], dFi[NS], F[NS][4], V[NS][6];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      for (int q = 0; q < kTan; ++q)
+        Ti[s][q] = ckind[s] == CK_NONE ? 0.0f : a.tan[(i * kTan + q) * a.D + ccol[s]];
+      float e = 0.0f, lim = 0.0f;
+      dFi[s] = Fppi * Ti[s][0];
+      if (ckind[s] == CK_DENS) {
+        if (clamped) lim = float(a.over[i]) * Ti[s][0];
+        else e = float(a.Fp[i]) * Ti[s][0];
+      } else if (ckind[s] == CK_EMB && ctab[s] == emb0 + ti) {
+        dFi[s] += band(sS, sT[ctab[s]], kfi, cm[s], ebi);
+        e = band(sS, sT[ctab[s]], kfi, cm[s], evi);
+      } else if (ckind[s] == CK_DIP) {
+        e = mui[0] * Ti[s][1] + mui[1] * Ti[s][2] + mui[2] * Ti[s][3];
+      } else if (ckind[s] == CK_QUAD) {
+        const float *dl = Ti[s] + 4;
+        const float dot = Li[0] * dl[0] + Li[1] * dl[1] + Li[2] * dl[2] +
+                          2.0f * (Li[3] * dl[3] + Li[4] * dl[4] + Li[5] * dl[5]);
+        e = dot - (Li[0] + Li[1] + Li[2]) * (dl[0] + dl[1] + dl[2]) / 3.0f;
+      }
+      for (int q = 0; q < 3; ++q) F[s][q] = 0.0f;
+      F[s][3] = e;
+      for (int q = 0; q < 6; ++q) V[s][q] = 0.0f;
+      tp[s][7] += lim;
+    }
+    const int64_t p0 = a.pair_off[i], p1 = a.pair_off[i + 1];
+    for (int64_t pc = p0; pc < p1; pc += 64) {
+      const int64_t p = pc + lane;
+      if (p < p1) {
+        const float4 q4 = a.rec[p];
+        const uint32_t b = __float_as_uint(q4.w);
+        const bool tiny = (b & kFlagTiny) != 0;
+        const float r = sqrtf(q4.x * q4.x + q4.y * q4.y + q4.z * q4.z);
+        const int tj = pj_type(b), o = ord_pair(ti, tj, a.ntypes);
+        const int64_t jg = cbase + pj_local(b);
+        P.r[lane] = tiny ? 1.0f : r;
+        P.d0[lane] = q4.x;
+        P.d1[lane] = q4.y;
+        P.d2[lane] = q4.z;
+        P.jg[lane] = int32_t(jg);
+        P.tj[lane] = tj;
+        const bool jcl = a.over[jg] != 0.0;
+        P.fpj[lane] = a.Fp32[jg];
+        P.fppj[lane] = jcl ? 0.0f : float(a.Fpp[jg]);
+        P.kfj[lane] = a.emb_k[jg];
+        for (int q = 0; q < 4; ++q) P.ebj[q][lane] = a.emb_d[4 * jg + q];
+        const float muj[3] = {float(a.mu[3 * jg]), float(a.mu[3 * jg + 1]), float(a.mu[3 * jg + 2])};
+        float Lj[6];
+        for (int q = 0; q < 6; ++q) Lj[q] = float(a.lamb[6 * jg + q]);
+        for (int q = 0; q < 3; ++q) P.muj[q][lane] = muj[q];
+        float xa[3], xb[3];
+        const float nu = qnu(Li, q4.x, q4.y, q4.z, xa) + qnu(Lj, q4.x, q4.y, q4.z, xb);
+        P.nu[lane] = nu;
+        for (int q = 0; q < 3; ++q) P.xi[q][lane] = xa[q] + xb[q];
+        float bv[4], bd[4], cc[6], cv[6];
+        auto role = [&](bool on, int tab, int32_t &tslot, int32_t &kslot, float (*cd)[64],
+                        float (*cvv)[64], float &val, float &der) {
+          tslot = on ? tab : -1;
+          val = der = 0.0f;
+          if (!on) return;
+          const TableDesc T = sT[tab];
+          kslot = pair_basis(T, r, bv, bd);
+          band6(sS, T, kslot, bd, 1.0f, cc);
+          for (int d = 0; d < 6; ++d) cd[d][lane] = cc[d];
+          if (cvv) {
+            band6(sS, T, kslot, bv, 1.0f, cv);
+            for (int d = 0; d < 6; ++d) cvv[d][lane] = cv[d];
+          }
+          float tt;
+          const int kk = spl_interval32(T, r, tt);
+          spl_eval32(T, a.ivl32, kk, tt, val, der);
+        };
+        float v0, d0;
+        role(!tiny && (b & kFlagPhi), o, P.tp[lane], P.kp[lane], P.cpd, P.cpv, v0, d0);
+        role(!tiny && (b & kFlagGj), a.npair + tj, P.tgj[lane], P.kgj[lane], P.cgj, nullptr, v0, d0);
+        P.dgj[lane] = d0;
+        role(!tiny && (b & kFlagGi), a.npair + ti, P.tgi[lane], P.kgi[lane], P.cgi, nullptr, v0, d0);
+        P.dgi[lane] = d0;
+        role(!tiny && (b & kFlagDip), dip0 + o, P.tu[lane], P.ku[lane], P.cud, P.cuv, v0, d0);
+        P.u[lane] = v0;
+        P.du[lane] = d0;
+        role(!tiny && (b & kFlagQuad), quad0 + o, P.tw[lane], P.kw[lane], P.cwd, P.cwv, v0, d0);
+        P.w[lane] = v0;
+        P.dw[lane] = d0;
+        if (tiny) P.jg[lane] = -1;
+      }
+      wave_sync();
+      const int np = int(min<int64_t>(64, p1 - pc));
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        const int kind = ckind[s];
+        if (kind == CK_NONE) continue;
+        const int tab = ctab[s], m = cm[s], col = ccol[s];
+        for (int q = 0; q < np; ++q) {
+          const int jg = P.jg[q];
+          if (jg < 0) continue;
+          const float x = P.d0[q], y = P.d1[q], z = P.d2[q], ir = 1.0f / P.r[q];
+          float fs = 0.0f, e = 0.0f, f[3] = {0.0f, 0.0f, 0.0f};
+          if (kind == CK_PAIR) {
+            if (P.tp[q] != tab) continue;
+            const int d = m - P.kp[q] + 2;
+            if (d < 0 || d > 5) continue;
+            fs = P.cpd[d][q];
+            e = 0.5f * P.cpv[d][q];
+          } else if (kind == CK_DENS || kind == CK_EMB) {
+            float dFj = 0.0f;
+            if (kind == CK_DENS) {
+              if (P.tgj[q] == tab) {
+                const int d = m - P.kgj[q] + 2;
+                if (d >= 0 && d < 6) fs += Fpi * P.cgj[d][q];
+              }
+              if (P.tgi[q] == tab) {
+                const int d = m - P.kgi[q] + 2;
+                if (d >= 0 && d < 6) fs += P.fpj[q] * P.cgi[d][q];
+              }
+              dFj = P.fppj[q] * a.tan[(int64_t(jg) * kTan) * a.D + col];
+            } else if (P.tj[q] == tab - emb0) {
+              const float eb[4] = {P.ebj[0][q], P.ebj[1][q], P.ebj[2][q], P.ebj[3][q]};
+              dFj = band(sS, sT[tab], P.kfj[q], m, eb);
+            }
+            fs += dFi[s] * P.dgj[q] + dFj * P.dgi[q];
+          } else if (kind == CK_DIP) {
+            const float dmx = mui[0] - P.muj[0][q], dmy = mui[1] - P.muj[1][q], dmz = mui[2] - P.muj[2][q];
+            if (P.tu[q] == tab) {
+              const int d = m - P.ku[q] + 2;
+              if (d >= 0 && d < 6) {
+                const float sc = P.cud[d][q] * ir * (dmx * x + dmy * y + dmz * z);
+                const float v = P.cuv[d][q];
+                f[0] += sc * x + v * dmx;
+                f[1] += sc * y + v * dmy;
+                f[2] += sc * z + v * dmz;
+              }
+            }
+            if (P.tu[q] >= 0) {
+              const float *tj = a.tan + (int64_t(jg) * kTan) * a.D + col;
+              const float gx = Ti[s][1] - tj[1 * a.D], gy = Ti[s][2] - tj[2 * a.D],
+                          gz = Ti[s][3] - tj[3 * a.D];
+              const float sc = P.du[q] * ir * (gx * x + gy * y + gz * z);
+              const float u = P.u[q];
+              f[0] += sc * x + u * gx;
+              f[1] += sc * y + u * gy;
+              f[2] += sc * z + u * gz;
+            }
+          } else if (kind == CK_QUAD) {
+            if (P.tw[q] == tab) {
+              const int d = m - P.kw[q] + 2;
+              if (d >= 0 && d < 6) {
+                const float sc = P.cwd[d][q] * ir * P.nu[q];
+                const float v2 = 2.0f * P.cwv[d][q];
+                f[0] += sc * x + v2 * P.xi[0][q];
+                f[1] += sc * y + v2 * P.xi[1][q];
+                f[2] += sc * z + v2 * P.xi[2][q];
+              }
+            }
+            if (P.tw[q] >= 0) {
+              const float *tj = a.tan + (int64_t(jg) * kTan) * a.D + col;
+              float dl[6];
+              for (int k = 0; k < 6; ++k) dl[k] = Ti[s][4 + k] + tj[(4 + k) * a.D];
+              float xq[3];
+              const float sc = P.dw[q] * ir * qnu(dl, x, y, z, xq);
+              const float w2 = 2.0f * P.w[q];
+              f[0] += sc * x + w2 * xq[0];
+              f[1] += sc * y + w2 * xq[1];
+              f[2] += sc * z + w2 * xq[2];
+            }
+          }
+          const float sr = fs * ir;
+          f[0] += sr * x;
+          f[1] += sr * y;
+          f[2] += sr * z;
+          F[s][0] += f[0];
+          F[s][1] += f[1];
+          F[s][2] += f[2];
+          F[s][3] += e;
+          V[s][0] += x * f[0];
+          V[s][1] += y * f[1];
+          V[s][2] += z * f[2];
+          V[s][3] += x * f[1];
+          V[s][4] += x * f[2];
+          V[s][5] += y * f[2];
+        }
+      }
+      wave_sync();
+    }
+    const int64_t row = a.row_off[c] + 3 * (i - cbase);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      if (ckind[s] == CK_NONE) continue;
+      for (int q = 0; q < 3; ++q) a.J[(row + q) * a.D + ccol[s]] = F[s][q];
+      tp[s][0] += F[s][3];
+      for (int q = 0; q < 6; ++q) tp[s][1 + q] += V[s][q];
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    if (ckind[s] == CK_NONE) continue;
+    for (int q = 0; q < kJP; ++q) a.part[(int64_t(tile) * kJP + q) * a.D + ccol[s]] = double(tp[s][q]);
   }
 }
 
@@ -929,7 +1336,6 @@ inline unsigned nb(int64_t n, int b) { return unsigned((n + b - 1) / b); }
 } // namespace
 
 void launch_jacobian(Context &c) {
-  FS_REQUIRE(c.kind != FS_ADP, FS_ERR_UNSUPPORTED, "ADP Jacobian columns are not available yet");
   FS_REQUIRE(c.D <= 64 * kCPL, FS_ERR_UNSUPPORTED, "more than 256 free parameters");
   const int64_t R = c.R;
   c.J.alloc(size_t(std::max<int64_t>(R * c.D, 1)));
@@ -1042,6 +1448,8 @@ void launch_jacobian(Context &c) {
       int kind = CK_PAIR;
       if (t >= c.npair && t < c.npair + c.ntypes) kind = CK_DENS;
       else if (t >= c.npair + c.ntypes && t < c.npair + 2 * c.ntypes) kind = CK_EMB;
+      else if (t >= c.npair + 2 * c.ntypes && t < 2 * c.npair + 2 * c.ntypes) kind = CK_DIP;
+      else if (t >= 2 * c.npair + 2 * c.ntypes) kind = CK_QUAD;
       for (int m = 0; m < T.n; ++m) {
         const int col = c.h_col_of[T.off + m];
         if (col < 0) continue;
@@ -1090,7 +1498,7 @@ void launch_jacobian(Context &c) {
     FS_REQUIRE(lds_main <= 64 * 1024, FS_ERR_UNSUPPORTED, "Jacobian LDS budget exceeded");
     // lane slots: pair, density and embedding columns each start a new slot
     std::vector<SlotDesc> slots;
-    for (int kind : {int(CK_PAIR), int(CK_DENS), int(CK_EMB)}) {
+    for (int kind : {int(CK_PAIR), int(CK_DENS), int(CK_EMB), int(CK_DIP), int(CK_QUAD)}) {
       size_t n = 0;
       for (int col = 0; col < c.D; ++col) {
         if (cols[col].kind != kind) continue;
@@ -1101,8 +1509,9 @@ void launch_jacobian(Context &c) {
       }
     }
     const int ns = int(slots.size() / 64);
-    FS_REQUIRE(ns <= 6, FS_ERR_UNSUPPORTED, "Jacobian columns need more than six lane slots");
-    const int NS = ns <= 3 ? 3 : (ns <= 4 ? 4 : 6);
+    const bool adp = c.kind == FS_ADP;
+    FS_REQUIRE(ns <= (adp ? 8 : 6), FS_ERR_UNSUPPORTED, "Jacobian columns need too many lane slots");
+    const int NS = adp ? (ns <= 4 ? 4 : 8) : (ns <= 3 ? 3 : (ns <= 4 ? 4 : 6));
     slots.resize(size_t(NS) * 64, SlotDesc{-1, 0, CK_NONE, -1, 0, 0, 0, 0});
     static_assert(sizeof(SlotDesc) == 2 * sizeof(int4), "slot descriptor layout");
     c.jslots.alloc(slots.size() * 2);
@@ -1120,7 +1529,29 @@ void launch_jacobian(Context &c) {
     }
     a.jrec = c.jrec.p;
     a.jr = c.jr.p;
-    if (c.N > 0) {
+    if (adp) {
+      c.adp_tan.alloc(std::max<size_t>(size_t(c.N) * kTan * c.D, 1));
+      a.tan = c.adp_tan.p;
+      a.mu = c.mu.p;
+      a.lamb = c.lamb.p;
+      const size_t lds1 = base + sizeof(AStage1), lds2 = base + sizeof(AStage2);
+      FS_REQUIRE(lds2 <= 64 * 1024, FS_ERR_UNSUPPORTED, "ADP Jacobian LDS budget exceeded");
+      if (c.N > 0) {
+        const unsigned g1 = unsigned(std::min<int64_t>(c.N, int64_t(c.cu_count) * 32));
+        c.time_main_begin();
+        if (NS == 4) {
+          hipLaunchKernelGGL(k_adp_tan1<4>, dim3(g1), dim3(64), lds1, c.stream, a, c.nknots, d_slots);
+          FS_HIP(hipGetLastError());
+          hipLaunchKernelGGL(k_adp_tan2<4>, dim3(nt), dim3(64), lds2, c.stream, a, c.nknots, d_slots);
+        } else {
+          hipLaunchKernelGGL(k_adp_tan1<8>, dim3(g1), dim3(64), lds1, c.stream, a, c.nknots, d_slots);
+          FS_HIP(hipGetLastError());
+          hipLaunchKernelGGL(k_adp_tan2<8>, dim3(nt), dim3(64), lds2, c.stream, a, c.nknots, d_slots);
+        }
+        FS_HIP(hipGetLastError());
+        c.time_main_end();
+      }
+    } else if (c.N > 0) {
       const unsigned g1 = unsigned(std::min<int64_t>((c.N + 3) / 4, int64_t(c.cu_count) * 16));
       hipLaunchKernelGGL(k_jac_dens, dim3(g1), dim3(256), lds_dens, c.stream, a, c.nknots);
       FS_HIP(hipGetLastError());

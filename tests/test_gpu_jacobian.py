@@ -20,10 +20,17 @@ pytestmark = pytest.mark.gpu
 def wl():
     w4 = dsm.config4(n_configs=6)
     w4.model.stress_weight = 0.4
+    # ADP with pair and density knots freed as well (240 columns of all five
+    # table kinds) next to the C3 free set (dipole, quadrupole, embedding)
+    w5 = dsm.config3(n_configs=2)
+    for t in w5.model.tables[:w5.model.npair + w5.model.ntypes]:
+        t.fixed[:] = False
     return {
         "eam1": dsm.config1(n_configs=3),
         "eam3s": dsm.config2(n_configs=6, nmin=32, nmax=300),
         "sw": w4,
+        "adp2": dsm.config3(n_configs=3),
+        "adp2all": w5,
     }
 
 
@@ -31,7 +38,7 @@ def metric(a, b):
     return float(np.max(np.abs(a - b) / (1.0 + np.abs(b))))
 
 
-@pytest.mark.parametrize("name", ["eam1", "eam3s", "sw"])
+@pytest.mark.parametrize("name", ["eam1", "eam3s", "sw", "adp2", "adp2all"])
 def test_jacobian_matches_oracle(wl, name):
     w = wl[name]
     print(f"jac {name}", flush=True)
@@ -97,3 +104,53 @@ def test_lm_tracks_oracle(wl):
     assert np.all(np.diff(hg) <= 0)
     assert abs(hg[-1] - ho[-1]) <= 1e-3 * ho[-1]
     assert abs(g.cost(xg) - hg[-1]) <= 1e-6 * hg[-1]
+
+
+def test_lm_50_iterations_single_element():
+    """The single-element FCC workload's run: one residual + Jacobian, then 50
+    LM iterations; the GPU and oracle drivers share the algorithm, so the
+    cost histories agree to the FP32-Jacobian level and both descend."""
+    w = dsm.config1()
+    o = oracle.Oracle(w.dataset, w.model)
+    o.prepare()
+    x0 = o.gather()
+    xo, ho, ao = o.lm(x0, 50)
+    g = fa.GpuCalculator(w.dataset, w.model)
+    g.prepare()
+    xg, hg, ag = g.lm(x0, 50)
+    print(f"LM 50: oracle {ho[0]:.6e} -> {ho[-1]:.6e} ({ao} accepted); "
+          f"gpu {hg[0]:.6e} -> {hg[-1]:.6e} ({ag} accepted)", flush=True)
+    assert np.all(np.diff(hg) <= 0)
+    assert hg[-1] < 0.1 * hg[0]
+    assert abs(hg[-1] - ho[-1]) <= 1e-2 * ho[-1]
+    assert abs(g.cost(xg) - hg[-1]) <= 1e-6 * hg[-1]
+
+
+def test_lm_stillinger_weber(wl):
+    """Batched LM on SW with analytic parameter columns (tangents)."""
+    w = wl["sw"]
+    o = oracle.Oracle(w.dataset, w.model)
+    o.prepare()
+    x0 = o.gather() * 1.02
+    xo, ho, _ = o.lm(x0, 10)
+    g = fa.GpuCalculator(w.dataset, w.model)
+    g.prepare()
+    xg, hg, acc = g.lm(x0, 10)
+    print(f"SW LM: oracle {ho[0]:.6e} -> {ho[-1]:.6e}; gpu {hg[0]:.6e} -> {hg[-1]:.6e}", flush=True)
+    assert acc > 0 and np.all(np.diff(hg) <= 0)
+    assert abs(hg[-1] - ho[-1]) <= 1e-3 * ho[-1]
+    assert np.allclose(xg, xo, rtol=1e-3, atol=1e-6)
+
+
+def test_lm_adp(wl):
+    w = wl["adp2"]
+    o = oracle.Oracle(w.dataset, w.model)
+    o.prepare()
+    x0 = o.gather()
+    xo, ho, _ = o.lm(x0, 5)
+    g = fa.GpuCalculator(w.dataset, w.model)
+    g.prepare()
+    xg, hg, acc = g.lm(x0, 5)
+    print(f"ADP LM: oracle {ho[0]:.6e} -> {ho[-1]:.6e}; gpu {hg[0]:.6e} -> {hg[-1]:.6e}", flush=True)
+    assert acc > 0 and np.all(np.diff(hg) <= 0)
+    assert abs(hg[-1] - ho[-1]) <= 1e-3 * ho[-1]
