@@ -357,11 +357,9 @@ __global__ __launch_bounds__(256) void k_jac_dens(JacArgs a, int nknots) {
 // Gs_j = sum_i g'_ti(r) r u u. Those pair loops therefore accumulate forces
 // only, and the matching per-atom / F'_i terms carry their virial twice.
 struct JStage {
-  float4 g[64];          // unit vector and distance {u0, u1, u2, r}
-  float2 cphi[6][64];    // pair-table band: {force coefficient, half energy}
-  float dgi[64];
+  float r[64], u0[64], u1[64], u2[64], dgi[64];
   int32_t jg[64], kphi[64], kgj[64], kgi[64], kfj[64];
-  float cgj[6][64], cgi[6][64], cemb[6][64];
+  float cphi[6][64], ephi[6][64], cgj[6][64], cgi[6][64], cemb[6][64];
 };
 
 struct SlotDesc {
@@ -469,7 +467,10 @@ __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots, const S
           const int32_t jg = int32_t(cbase + pj_local(b));
           ty = tj;
           const float u0 = q.x * inv_r, u1 = q.y * inv_r, u2 = q.z * inv_r;
-          P.g[lane] = make_float4(u0, u1, u2, rr);
+          P.r[lane] = rr;
+          P.u0[lane] = u0;
+          P.u1[lane] = u1;
+          P.u2[lane] = u2;
           P.jg[lane] = jg;
           const float fpj = a.Fp32[jg];
           float bv[4], bd[4], cc[6], ce[6];
@@ -482,7 +483,10 @@ __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots, const S
             band6(sS, T, k, bv, 0.5f * w, ce);
             P.kphi[lane] = k;
 #pragma unroll
-            for (int d = 0; d < 6; ++d) P.cphi[d][lane] = make_float2(cc[d], ce[d]);
+            for (int d = 0; d < 6; ++d) {
+              P.cphi[d][lane] = cc[d];
+              P.ephi[d][lane] = ce[d];
+            }
           }
           // density of j, weight F'_i; also g'_tj(r) for G_i
           float dgj = 0.0f;
@@ -555,10 +559,9 @@ __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots, const S
             const int lo = first_ge(P.kphi, seg[ptt[s]], s1, m - 3);
             const int hi = first_ge(P.kphi, lo, s1, m + 3);
             for (int q = lo; q < hi; ++q) {
-              const float2 ce = P.cphi[m - P.kphi[q] + 2][q];
-              const float4 gq = P.g[q];
-              add_pair(A[s], ce.x, gq.w, gq.x, gq.y, gq.z);
-              A[s].e += ce.y;
+              const int d = m - P.kphi[q] + 2;
+              add_pair(A[s], P.cphi[d][q], P.r[q], P.u0[q], P.u1[q], P.u2[q]);
+              A[s].e += P.ephi[d][q];
             }
           } else if (kind == CK_DENS) {
             const int t = ctab[s] - a.npair;
@@ -569,9 +572,8 @@ __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots, const S
               const int hi = first_ge(P.kgj, lo, s1, m + 3);
               for (int q = lo; q < hi; ++q) {
                 const float cq = P.cgj[m - P.kgj[q] + 2][q];
-                const float4 gq = P.g[q];
-                add_force(A[s], cq, gq.x, gq.y, gq.z);
-                add_virial(A[s], 2.0f * cq, gq.w, gq.x, gq.y, gq.z);
+                add_force(A[s], cq, P.u0[q], P.u1[q], P.u2[q]);
+                add_virial(A[s], 2.0f * cq, P.r[q], P.u0[q], P.u1[q], P.u2[q]);
               }
             }
             // F'_j g'_ti(r), ti == t, every partner type
@@ -580,10 +582,8 @@ __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots, const S
                 const int s1 = seg[tt + 1];
                 const int lo = first_ge(P.kgi, seg[tt], s1, m - 3);
                 const int hi = first_ge(P.kgi, lo, s1, m + 3);
-                for (int q = lo; q < hi; ++q) {
-                  const float4 gq = P.g[q];
-                  add_force(A[s], P.cgi[m - P.kgi[q] + 2][q], gq.x, gq.y, gq.z);
-                }
+                for (int q = lo; q < hi; ++q)
+                  add_force(A[s], P.cgi[m - P.kgi[q] + 2][q], P.u0[q], P.u1[q], P.u2[q]);
               }
             }
             // F''_j H_j g'_ti(r): every pair, rows of Hs gathered eight at a time
@@ -596,10 +596,7 @@ __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots, const S
 #pragma unroll
               for (int v = 0; v < 8; ++v) {
                 const int q = q0 + v;
-                if (q < np) {
-                  const float4 gq = P.g[q];
-                  add_force(A[s], h[v] * P.dgi[q], gq.x, gq.y, gq.z);
-                }
+                if (q < np) add_force(A[s], h[v] * P.dgi[q], P.u0[q], P.u1[q], P.u2[q]);
               }
             }
           } else if (kind == CK_EMB) {
@@ -608,8 +605,7 @@ __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots, const S
             for (int q = seg[t]; q < seg[t + 1]; ++q) {
               const int d = m - P.kfj[q] + 2;
               if (d < 0 || d > 5) continue;
-              const float4 gq = P.g[q];
-              add_force(A[s], P.cemb[d][q], gq.x, gq.y, gq.z);
+              add_force(A[s], P.cemb[d][q], P.u0[q], P.u1[q], P.u2[q]);
             }
           }
         }
