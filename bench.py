@@ -46,6 +46,9 @@ def parse():
     ap.add_argument("--de-plain-seconds", type=float, default=0.0,
                     help="surrogate run: the plain run's wall time")
     ap.add_argument("--de-lanes", type=int, default=4)
+    ap.add_argument("--pmc-driver", action="store_true",
+                    help="counter-collection driver: a 1 GiB copy (calibration) then three "
+                         "residual + Jacobian evaluations; prints no JSON")
     return ap.parse_args()
 
 
@@ -126,8 +129,54 @@ def de_measure(a):
     print(json.dumps(out), flush=True)
 
 
+def measured_traffic(kernel="k_jac_main"):
+    """HBM bytes per launch of the kernel from its committed two-pass PMC
+    summary (FETCH_SIZE and WRITE_SIZE, each scaled by a 1 GiB copy)."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"{kernel}_traffic_eam_ternary_500cfg.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
+def pmc_driver(a):
+    """Workload for rocprofv3 --pmc passes: the copy kernel of known size that
+    scales FETCH_SIZE / WRITE_SIZE, then the Jacobian kernels."""
+    import torch
+
+    n = 1 << 28  # 2^28 FP32 = 1 GiB read and 1 GiB written per dispatch
+    x = torch.ones(n, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        torch.add(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    del x, y
+    torch.cuda.empty_cache()
+    log("calibration copies done")
+    import forcesmith_amd as fa
+    from forcesmith_amd import datasets
+
+    w = datasets.config2(n_configs=a.configs)
+    g = fa.GpuCalculator(w.dataset, w.model)
+    g.prepare()
+    g.set_launch(8)
+    lib, h = g._lib, g._h
+    for _ in range(3):
+        g._check(lib.fs_eval_residual(h, None, None))
+        g._check(lib.fs_eval_jacobian(h, None))
+        g.synchronize()
+    log("residual + Jacobian evaluations done")
+
+
 def main():
     a = parse()
+    if a.pmc_driver:
+        pmc_driver(a)
+        return
     if a.de:
         if a.gpus != 1:
             print("bench.py measures one GPU", file=sys.stderr)
@@ -201,7 +250,10 @@ def main():
                    "configs": C, "atoms": N, "pairs": P, "params": D, "rows": R,
                    "precision": a.precision, "lanes_per_atom": lpa},
         "roofline": {"bound": "hbm", "kernel": "k_jac_main", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic() if C == 500 else None,
+                     "traffic_source": "profiles/k_jac_main_traffic_eam_ternary_500cfg.md (rocprofv3 "
+                                       "FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
                      "bytes_per_launch": b_main, "ms_per_launch": t_main},
         "kernels_ms": {"residual": t_res, "jacobian": t_jac, "jac_main": t_main, "normal_equations": t_ne},
         "step_bytes_alg": b_alg,

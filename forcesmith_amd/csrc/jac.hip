@@ -41,6 +41,7 @@ namespace {
 constexpr int kCPL = 4;        // columns per lane (D <= 256)
 constexpr int kJacTile = 64;   // atoms per Jacobian tile
 constexpr int kJP = 8;         // partial rows: energy, 6 virial, limit
+constexpr int kXcds = 8;       // MI355X accelerator complex dies (one L2 each)
 enum { CK_PAIR = 0, CK_DENS = 1, CK_EMB = 2, CK_DIP = 3, CK_QUAD = 4, CK_NONE = 7 };
 
 struct ColDesc {
@@ -422,7 +423,15 @@ __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots, const S
     ckind[s] = sd.kind;
     cdc[s] = sd.dc;
   }
-  for (int tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
+  // XCD-aware tiles: blocks are dispatched round-robin over the 8 XCDs, so
+  // block b runs on XCD b % 8; each XCD takes one contiguous eighth of the
+  // tiles (whole configurations), which keeps the neighbours' Hs rows that the
+  // density-chain term gathers in that XCD's L2
+  const int ng = min(kXcds, int(gridDim.x)); // tile ranges; every range gets a block
+  const int xcd = blockIdx.x % ng, slot = blockIdx.x / ng;
+  const int nslot = (int(gridDim.x) - xcd + ng - 1) / ng;
+  const int t_end = int(int64_t(xcd + 1) * a.ntiles / ng);
+  for (int tile = int(int64_t(xcd) * a.ntiles / ng) + slot; tile < t_end; tile += nslot) {
     const int na = a.tile_na[tile];
     const int c = a.tile_cfg[tile];
     const int64_t cbase = a.atom_off[c];
