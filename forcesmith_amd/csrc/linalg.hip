@@ -1,8 +1,8 @@
 // FP64 normal equations and the damped LM solve on the GPU.
 //
 //   JtJ = J^T J and g = J^T F from the FP32 Jacobian and the FP64 residual,
-//   with every product and sum in FP64: split-K partial tiles reduced in a
-//   fixed chunk order (deterministic).
+//   with every product and sum in FP64 on the matrix cores (one pass over J,
+//   split-K over row chunks reduced in a fixed chunk order: deterministic).
 //   (JtJ + mu * diag(d)) dx = -g by an in-place Cholesky factorisation and two
 //   triangular solves in one workgroup, plus the predicted reduction
 //   dx . (mu d dx - g) the LM gain ratio needs.
@@ -16,92 +16,162 @@ namespace fsa {
 
 namespace {
 
-constexpr int kT = 64;     // output tile edge
-constexpr int kRB = 32;    // rows per LDS stage
+typedef double d4v __attribute__((ext_vector_type(4)));
 
-__global__ __launch_bounds__(256) void k_jtj_partial(const float *__restrict__ J, int64_t R, int D,
-                                                     int nchunk, const int2 *__restrict__ tiles,
-                                                     double *__restrict__ part) {
-  __shared__ float sA[kRB][kT];
-  __shared__ float sB[kRB][kT];
-  const int chunk = blockIdx.x, tp = blockIdx.y;
-  const int2 tt = tiles[tp];
-  const int64_t rows_per = (R + nchunk - 1) / nchunk;
-  const int64_t r0 = int64_t(chunk) * rows_per;
-  const int64_t r1 = std::min<int64_t>(R, r0 + rows_per);
-  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
-  double acc[4][4];
+constexpr int kMT = 16; // MFMA tile edge (v_mfma_f64_16x16x4f64)
+
+// Upper-triangle 16x16 tile pairs (I <= J) of an nt x nt tile grid, row-major.
+__host__ __device__ inline int tile_pair(int I, int J, int nt) { return I * nt - I * (I - 1) / 2 + (J - I); }
+
+// One pass over a chunk of Jacobian rows on the FP64 matrix cores. Wave w
+// owns tile rows I0 = w and I1 = nt - 1 - w of the upper triangle of J^T J
+// (nt + 1 tiles per wave, so the waves are balanced and two waves share a
+// SIMD within its 512 registers), four rows per step, plus the Jt r entries of
+// its columns. Operand layout of v_mfma_f64_16x16x4f64 (measured on gfx950):
+// lane l holds A[l % 16][l / 16] and B[l / 16][l % 16], and after the product
+// D[4 v + l / 16][l % 16] in element v. A is the transposed column tile I of
+// four rows and B the column tile J of the same rows: the FP32 entries are
+// widened to FP64 and every product and sum is FP64. Partials per chunk are
+// reduced in chunk order (deterministic).
+constexpr int kMaxWaveTiles = 17; // nt + 1 for nt <= 16
+constexpr int kNeRows = 64;       // Jacobian rows per LDS stage
+
+// LDS row pitch: D rounded up to 32 floats plus 8, so the four rows one MFMA
+// step reads start 8 banks apart and every bank serves exactly two lanes
+__host__ __device__ inline int ne_pitch(int D) { return ((D + 31) / 32) * 32 + 8; }
+
+__global__ __launch_bounds__(512) void k_ne_mfma(const float *__restrict__ J,
+                                                 const double *__restrict__ F, int64_t R, int D,
+                                                 int nt, int nchunk, double *__restrict__ part) {
+  extern __shared__ float sJ[]; // [kNeRows][pitch]
+  __shared__ double sF[kNeRows];
+  const int pitch = ne_pitch(D);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int I0 = w, I1 = nt - 1 - w;
+  const bool live = I0 <= I1;               // waves past the middle row only load
+  const int n0 = live ? nt - I0 : 0;        // tiles of row I0
+  const int n1 = I1 > I0 ? nt - I1 : 0;     // tiles of row I1 (none when it is I0)
+  const int ntp = nt * (nt + 1) / 2;
+  const int64_t rows_per = (((R + nchunk - 1) / nchunk) + 3) & ~int64_t(3);
+  const int64_t r0 = int64_t(blockIdx.x) * rows_per;
+  const int64_t r1 = r0 + rows_per < R ? r0 + rows_per : R;
+  const int kr = l >> 4, cl = l & 15;
+  const int c0 = kMT * I0 + cl, c1 = kMT * I1 + cl;
+  d4v acc[kMaxWaveTiles];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int j = 0; j < kMaxWaveTiles; ++j) acc[j] = d4v{0.0, 0.0, 0.0, 0.0};
+  double g0 = 0.0, g1 = 0.0;
+  // pad columns D .. pitch stay zero for the whole pass
+  for (int e = threadIdx.x; e < kNeRows * (pitch - D); e += blockDim.x)
+    sJ[(e / (pitch - D)) * pitch + D + e % (pitch - D)] = 0.0f;
+  const bool vec = (D & 3) == 0; // rows start 16-byte aligned: float4 staging
+  for (int64_t sb = r0; sb < r1; sb += kNeRows) {
+    const int nr = int(r1 - sb < kNeRows ? r1 - sb : kNeRows);
+    // stage nr rows; a short last stage is zero-padded to kNeRows rows
+    if (vec) {
+      const int q4 = D >> 2;
+      const float4 *src = reinterpret_cast<const float4 *>(J + sb * D);
+      for (int e = threadIdx.x; e < nr * q4; e += blockDim.x) {
+        const int rr = e / q4, cc = 4 * (e % q4);
+        const float4 v = src[e];
+        float *d = sJ + rr * pitch + cc;
+        d[0] = v.x;
+        d[1] = v.y;
+        d[2] = v.z;
+        d[3] = v.w;
+      }
+    } else {
+      for (int e = threadIdx.x; e < nr * D; e += blockDim.x) {
+        const int rr = e / D, cc = e % D;
+        sJ[rr * pitch + cc] = J[(sb + rr) * D + cc];
+      }
+    }
+    for (int e = threadIdx.x; e < (kNeRows - nr) * D; e += blockDim.x)
+      sJ[(nr + e / D) * pitch + e % D] = 0.0f;
+    for (int e = threadIdx.x; e < kNeRows; e += blockDim.x) sF[e] = e < nr ? F[sb + e] : 0.0;
+    __syncthreads();
+    if (live) {
+      for (int rb = 0; rb < nr; rb += 4) {
+        const float *row = sJ + (rb + kr) * pitch;
+        const double f = sF[rb + kr];
+        const double a0 = double(row[c0]);
+        const double a1 = n1 > 0 ? double(row[c1]) : 0.0;
+        g0 += a0 * f;
+        g1 += a1 * f;
 #pragma unroll
-    for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
-  const int ca = tt.x * kT, cb = tt.y * kT;
-  for (int64_t rb = r0; rb < r1; rb += kRB) {
-    for (int k = threadIdx.x; k < kRB * kT; k += 256) {
-      const int rr = k / kT, cc = k % kT;
-      const int64_t row = rb + rr;
-      const bool ok = row < r1;
-      sA[rr][cc] = (ok && ca + cc < D) ? J[row * D + ca + cc] : 0.0f;
-      sB[rr][cc] = (ok && cb + cc < D) ? J[row * D + cb + cc] : 0.0f;
+        for (int j = 0; j < kMaxWaveTiles; ++j) {
+          if (j < n0 + n1) { // uniform across the wave
+            const int Jt = j < n0 ? I0 + j : I1 + (j - n0);
+            const double b = double(row[kMT * Jt + cl]);
+            acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(j < n0 ? a0 : a1, b, acc[j], 0, 0, 0);
+          }
+        }
+      }
     }
     __syncthreads();
-#pragma unroll 4
-    for (int k = 0; k < kRB; ++k) {
-      double av[4], bv[4];
+  }
+  if (!live) return;
+  // Jt r: the four lanes of one column hold its four row-phase sums
+  g0 += __shfl_xor(g0, 16, 64);
+  g0 += __shfl_xor(g0, 32, 64);
+  g1 += __shfl_xor(g1, 16, 64);
+  g1 += __shfl_xor(g1, 32, 64);
+  double *o = part + int64_t(blockIdx.x) * (int64_t(ntp) * 256 + D);
 #pragma unroll
-      for (int a = 0; a < 4; ++a) av[a] = double(sA[k][4 * ty + a]);
+  for (int j = 0; j < kMaxWaveTiles; ++j) {
+    if (j < n0 + n1) {
+      const int I = j < n0 ? I0 : I1;
+      const int Jt = j < n0 ? I0 + j : I1 + (j - n0);
+      double *t = o + int64_t(tile_pair(I, Jt, nt)) * 256;
 #pragma unroll
-      for (int b = 0; b < 4; ++b) bv[b] = double(sB[k][4 * tx + b]);
-#pragma unroll
-      for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] += av[a] * bv[b];
+      for (int v = 0; v < 4; ++v) t[(4 * v + kr) * kMT + cl] = acc[j][v];
     }
-    __syncthreads();
   }
-  double *o = part + (int64_t(tp) * nchunk + chunk) * (kT * kT);
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) o[(4 * ty + a) * kT + 4 * tx + b] = acc[a][b];
-}
-
-__global__ void k_jtj_reduce(int D, int nchunk, const int2 *__restrict__ tiles,
-                             const double *__restrict__ part, double *__restrict__ A) {
-  const int tp = blockIdx.x;
-  const int2 tt = tiles[tp];
-  for (int e = threadIdx.x; e < kT * kT; e += blockDim.x) {
-    const int a = e / kT, b = e % kT;
-    const int ia = tt.x * kT + a, ib = tt.y * kT + b;
-    if (ia >= D || ib >= D) continue;
-    double s = 0.0;
-    const double *p = part + int64_t(tp) * nchunk * (kT * kT) + e;
-    for (int k = 0; k < nchunk; ++k) s += p[int64_t(k) * (kT * kT)];
-    A[int64_t(ia) * D + ib] = s;
-    A[int64_t(ib) * D + ia] = s;
+  if (kr == 0) {
+    if (c0 < D) o[int64_t(ntp) * 256 + c0] = g0;
+    if (n1 > 0 && c1 < D) o[int64_t(ntp) * 256 + c1] = g1;
   }
 }
 
-__global__ void k_jtr_partial(const float *__restrict__ J, const double *__restrict__ F, int64_t R,
-                              int D, int nchunk, double *__restrict__ part) {
-  const int chunk = blockIdx.x;
-  const int64_t rows_per = (R + nchunk - 1) / nchunk;
-  const int64_t r0 = int64_t(chunk) * rows_per;
-  const int64_t r1 = std::min<int64_t>(R, r0 + rows_per);
-  for (int col = threadIdx.x; col < D; col += blockDim.x) {
-    double s = 0.0;
-    for (int64_t r = r0; r < r1; ++r) s += double(J[r * D + col]) * F[r];
-    part[int64_t(chunk) * D + col] = s;
-  }
+// First level of the chunk reduction: group gi sums chunks [gi*per_g, ...)
+// in order; the second level (k_ne_reduce) sums the groups in order.
+constexpr int kNeGroups = 16;
+__global__ void k_ne_reduce1(int64_t per, int nchunk, const double *__restrict__ part,
+                             double *__restrict__ part2) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int gi = blockIdx.y;
+  if (e >= per) return;
+  const int cpg = (nchunk + kNeGroups - 1) / kNeGroups;
+  const int k0 = gi * cpg, k1 = k0 + cpg < nchunk ? k0 + cpg : nchunk;
+  double s = 0.0;
+  for (int k = k0; k < k1; ++k) s += part[int64_t(k) * per + e];
+  part2[int64_t(gi) * per + e] = s;
 }
 
-__global__ void k_jtr_reduce(int D, int nchunk, const double *__restrict__ part,
-                             double *__restrict__ g) {
-  for (int col = blockIdx.x * blockDim.x + threadIdx.x; col < D; col += gridDim.x * blockDim.x) {
-    double s = 0.0;
-    for (int k = 0; k < nchunk; ++k) s += part[int64_t(k) * D + col];
-    g[col] = s;
+// Sum the chunk partials in chunk order; both triangles of J^T J are written
+// from the upper one (diagonal tiles from their i <= j half), so the matrix is
+// exactly symmetric.
+__global__ void k_ne_reduce(int D, int nt, int nchunk, const double *__restrict__ part,
+                            double *__restrict__ A, double *__restrict__ g) {
+  const int ntp = nt * (nt + 1) / 2;
+  const int64_t per = int64_t(ntp) * 256 + D;
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= per) return;
+  double s = 0.0;
+  for (int k = 0; k < nchunk; ++k) s += part[int64_t(k) * per + e];
+  if (e >= int64_t(ntp) * 256) {
+    g[e - int64_t(ntp) * 256] = s;
+    return;
   }
+  const int tp = int(e / 256), w = int(e % 256), i = w / kMT, j = w % kMT;
+  int I = 0;
+  while (tile_pair(I, nt - 1, nt) < tp) ++I; // tile row of pair tp
+  const int Jt = I + (tp - tile_pair(I, I, nt));
+  const int a = kMT * I + i, b = kMT * Jt + j;
+  if (a >= D || b >= D) return;
+  if (I == Jt && i > j) return;
+  A[int64_t(a) * D + b] = s;
+  A[int64_t(b) * D + a] = s;
 }
 
 // Marquardt scaling d_k = max(A_kk, 1e-12 max_k A_kk) (1 when all zero);
@@ -200,29 +270,23 @@ void launch_normal_equations(Context &c) {
   c.JtJ.alloc(size_t(std::max(D, 1)) * std::max(D, 1));
   c.Jtr.alloc(std::max(D, 1));
   if (D == 0) return;
-  const int nt = (D + kT - 1) / kT;
-  std::vector<int2> tiles;
-  for (int a = 0; a < nt; ++a)
-    for (int b = a; b < nt; ++b) tiles.push_back(make_int2(a, b));
-  const int ntp = int(tiles.size());
-  const int nchunk = int(std::max<int64_t>(1, std::min<int64_t>(512, (R + 4095) / 4096)));
-  static_assert(sizeof(int2) == 8, "int2");
-  c.jtj_part.alloc(size_t(ntp) * nchunk * kT * kT + size_t(nchunk) * D + 2 * ntp);
-  int2 *d_tiles = reinterpret_cast<int2 *>(c.jtj_part.p + size_t(ntp) * nchunk * kT * kT +
-                                           size_t(nchunk) * D);
-  FS_HIP(hipMemcpyAsync(d_tiles, tiles.data(), sizeof(int2) * ntp, hipMemcpyHostToDevice, c.stream));
-  hipLaunchKernelGGL(k_jtj_partial, dim3(nchunk, ntp), dim3(256), 0, c.stream, c.J.p, R, D, nchunk,
-                     d_tiles, c.jtj_part.p);
+  FS_REQUIRE(D <= 16 * kMT, FS_ERR_UNSUPPORTED, "normal equations support at most 256 parameters");
+  const int nt = (D + kMT - 1) / kMT;
+  const int ntp = nt * (nt + 1) / 2;
+  // two chunks of rows per CU (at least 64 rows per chunk)
+  const int nchunk = int(std::max<int64_t>(1, std::min<int64_t>(2 * c.cu_count, (R + 63) / 64)));
+  const int64_t per = int64_t(ntp) * 256 + D;
+  c.jtj_part.alloc(size_t(nchunk + kNeGroups) * per);
+  double *part2 = c.jtj_part.p + size_t(nchunk) * per;
+  const size_t lds = size_t(kNeRows) * ne_pitch(D) * sizeof(float);
+  hipLaunchKernelGGL(k_ne_mfma, dim3(nchunk), dim3(64 * ((nt + 1) / 2)), lds, c.stream, c.J.p,
+                     c.residual.p, R, D, nt, nchunk, c.jtj_part.p);
   FS_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_jtj_reduce, dim3(ntp), dim3(256), 0, c.stream, D, nchunk, d_tiles,
-                     c.jtj_part.p, c.JtJ.p);
+  hipLaunchKernelGGL(k_ne_reduce1, dim3(unsigned((per + 255) / 256), kNeGroups), dim3(256), 0,
+                     c.stream, per, nchunk, c.jtj_part.p, part2);
   FS_HIP(hipGetLastError());
-  double *gpart = c.jtj_part.p + size_t(ntp) * nchunk * kT * kT;
-  hipLaunchKernelGGL(k_jtr_partial, dim3(nchunk), dim3(256), 0, c.stream, c.J.p, c.residual.p, R,
-                     D, nchunk, gpart);
-  FS_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_jtr_reduce, dim3((D + 255) / 256), dim3(256), 0, c.stream, D, nchunk, gpart,
-                     c.Jtr.p);
+  hipLaunchKernelGGL(k_ne_reduce, dim3(unsigned((per + 255) / 256)), dim3(256), 0, c.stream, D, nt,
+                     kNeGroups, part2, c.JtJ.p, c.Jtr.p);
   FS_HIP(hipGetLastError());
 }
 
