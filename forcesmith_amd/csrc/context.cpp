@@ -395,7 +395,8 @@ int fs_set_tabulated_model(fs_ctx *ctx, int32_t kind, int32_t ntypes, int32_t nt
     }
     c.nknots = off;
     c.nivl = ivl;
-    FS_REQUIRE(size_t(ntab) * sizeof(TableDesc) + 2 * size_t(ivl) * sizeof(float4) <= 48 * 1024,
+    FS_REQUIRE(size_t(ntab) * sizeof(TableDesc) + 2 * size_t(ivl) * sizeof(float4) +
+                       4 * size_t(ivl) * sizeof(double) <= 48 * 1024,
                FS_ERR_UNSUPPORTED, "knot tables exceed the 48 KiB LDS staging budget");
     c.h_col_of.assign(off, -1);
     c.h_knot_of.clear();

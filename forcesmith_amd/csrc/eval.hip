@@ -55,110 +55,91 @@ __device__ inline double Mext(const double *x, const double *y, int n, int q) {
 }
 __device__ inline double sgnd(double u) { return double((u > 0) - (u < 0)); }
 
-__device__ void table_slopes(const TableDesc &T, const double *x, const double *y, double *s,
-                             double *S, float4 *ivl, double *p64) {
-  const int n = T.n;
-  if (n >= 4) {
-    double m2 = (y[3] - y[2]) / (x[3] - x[2]);
-    double m1 = (y[2] - y[1]) / (x[2] - x[1]);
-    double m0 = (y[1] - y[0]) / (x[1] - x[0]);
-    double mm1 = 2 * m0 - m1;
-    double mm2 = 2 * mm1 - m0;
-    double w1 = fabs(m1 - m0) + fabs(m1 + m0) / 2;
-    double w2 = fabs(mm1 - mm2) + fabs(mm1 + mm2) / 2;
-    s[0] = (w1 * mm1 + w2 * m0) / (w1 + w2);
-    if (isnan(s[0])) s[0] = 0;
-    w1 = fabs(m2 - m1) + fabs(m2 + m1) / 2;
-    w2 = fabs(m0 - mm1) + fabs(m0 + mm1) / 2;
-    s[1] = (w1 * m0 + w2 * m1) / (w1 + w2);
-    if (isnan(s[1])) s[1] = 0;
-    for (int i = 2; i < n - 2; ++i) {
-      double mim2 = (y[i - 1] - y[i - 2]) / (x[i - 1] - x[i - 2]);
-      double mim1 = (y[i] - y[i - 1]) / (x[i] - x[i - 1]);
-      double mi = (y[i + 1] - y[i]) / (x[i + 1] - x[i]);
-      double mip1 = (y[i + 2] - y[i + 1]) / (x[i + 2] - x[i + 1]);
-      w1 = fabs(mip1 - mi) + fabs(mip1 + mi) / 2;
-      w2 = fabs(mim1 - mim2) + fabs(mim1 + mim2) / 2;
-      s[i] = (w1 * mim1 + w2 * mi) / (w1 + w2);
-      if (isnan(s[i])) s[i] = 0;
-    }
-    double mnm4 = (y[n - 3] - y[n - 4]) / (x[n - 3] - x[n - 4]);
-    double mnm3 = (y[n - 2] - y[n - 3]) / (x[n - 2] - x[n - 3]);
-    double mnm2 = (y[n - 1] - y[n - 2]) / (x[n - 1] - x[n - 2]);
-    double mnm1 = 2 * mnm2 - mnm3;
-    double mn = 2 * mnm1 - mnm2;
-    w1 = fabs(mnm1 - mnm2) + fabs(mnm1 + mnm2) / 2;
-    w2 = fabs(mnm3 - mnm4) + fabs(mnm3 + mnm4) / 2;
-    s[n - 2] = (w1 * mnm3 + w2 * mnm2) / (w1 + w2);
-    if (isnan(s[n - 2])) s[n - 2] = 0;
-    w1 = fabs(mn - mnm1) + fabs(mn + mnm1) / 2;
-    w2 = fabs(mnm2 - mnm3) + fabs(mnm2 + mnm3) / 2;
-    s[n - 1] = (w1 * mnm2 + w2 * mnm1) / (w1 + w2);
-    if (isnan(s[n - 1])) s[n - 1] = 0;
-    // band of the slope jacobian: S[k][m], m in [k-2, k+2]
-    for (int k = 0; k < n; ++k) {
-      const double A = Mext(x, y, n, k - 1), B = Mext(x, y, n, k), Cn = Mext(x, y, n, k + 1),
-                   P = Mext(x, y, n, k - 2);
-      const double W1 = fabs(Cn - B) + fabs(Cn + B) / 2;
-      const double W2 = fabs(A - P) + fabs(A + P) / 2;
-      const double W = W1 + W2;
-      for (int q = 0; q < 5; ++q) S[5 * k + q] = 0.0;
-      if (!(W > 0)) continue;
-      const double sv = (W1 * A + W2 * B) / W;
-      const double dw1C = sgnd(Cn - B) + sgnd(Cn + B) / 2, dw1B = -sgnd(Cn - B) + sgnd(Cn + B) / 2;
-      const double dw2A = sgnd(A - P) + sgnd(A + P) / 2, dw2P = -sgnd(A - P) + sgnd(A + P) / 2;
-      const double ds1 = (A - sv) / W, ds2 = (B - sv) / W;
-      const double gP = ds2 * dw2P, gA = W1 / W + ds2 * dw2A, gB = W2 / W + ds1 * dw1B,
-                   gC = ds1 * dw1C;
-      for (int q = 0; q < 5; ++q) {
-        const int m = k - 2 + q;
-        if (m < 0 || m >= n) continue;
-        S[5 * k + q] = gP * dM(x, n, k - 2, m) + gA * dM(x, n, k - 1, m) +
-                       gB * dM(x, n, k, m) + gC * dM(x, n, k + 1, m);
-      }
-    }
-  } else {
-    for (int k = 0; k < n; ++k) {
-      s[k] = 0.0;
-      for (int q = 0; q < 5; ++q) S[5 * k + q] = 0.0;
-    }
-  }
-  // power form a + b t + c t^2 + d t^3 in t = (r - x_k) / h, formed in FP64;
-  // the FP32 copy is a hi/lo pair per coefficient
-  for (int k = 0; k + 1 < n; ++k) {
-    const double y0 = y[k], y1 = y[k + 1];
-    double q[4];
-    if (T.linear) {
-      q[0] = y0;
-      q[1] = y1 - y0;
-      q[2] = q[3] = 0.0;
-    } else {
-      const double dx = x[k + 1] - x[k];
-      const double b = dx * s[k], e = dx * s[k + 1];
-      q[0] = y0;
-      q[1] = b;
-      q[2] = 3.0 * (y1 - y0) - 2.0 * b - e;
-      q[3] = 2.0 * (y0 - y1) + b + e;
-    }
-    float hi[4], lo[4];
-    for (int m = 0; m < 4; ++m) {
-      hi[m] = float(q[m]);
-      lo[m] = float(q[m] - double(hi[m]));
-      p64[4 * (T.ivl + k) + m] = q[m];
-    }
-    ivl[2 * (T.ivl + k)] = make_float4(hi[0], hi[1], hi[2], hi[3]);
-    ivl[2 * (T.ivl + k) + 1] = make_float4(lo[0], lo[1], lo[2], lo[3]);
+// makima slope of knot k and its row of the slope band dS_k/dy_(k-2..k+2),
+// through the extended secants (quadratic extrapolation at both ends), so
+// every knot follows one formula and the knots of a table run in parallel
+__device__ void knot_slope(int n, int k, const double *x, const double *y, double &sk, double *Sk) {
+  const double A = Mext(x, y, n, k - 1), B = Mext(x, y, n, k), Cn = Mext(x, y, n, k + 1),
+               P = Mext(x, y, n, k - 2);
+  const double W1 = fabs(Cn - B) + fabs(Cn + B) / 2;
+  const double W2 = fabs(A - P) + fabs(A + P) / 2;
+  const double W = W1 + W2;
+  sk = (W1 * A + W2 * B) / W;
+  if (isnan(sk)) sk = 0;
+  for (int q = 0; q < 5; ++q) Sk[q] = 0.0;
+  if (!(W > 0)) return;
+  const double sv = sk;
+  const double dw1C = sgnd(Cn - B) + sgnd(Cn + B) / 2, dw1B = -sgnd(Cn - B) + sgnd(Cn + B) / 2;
+  const double dw2A = sgnd(A - P) + sgnd(A + P) / 2, dw2P = -sgnd(A - P) + sgnd(A + P) / 2;
+  const double ds1 = (A - sv) / W, ds2 = (B - sv) / W;
+  const double gP = ds2 * dw2P, gA = W1 / W + ds2 * dw2A, gB = W2 / W + ds1 * dw1B, gC = ds1 * dw1C;
+  for (int q = 0; q < 5; ++q) {
+    const int m = k - 2 + q;
+    if (m < 0 || m >= n) continue;
+    Sk[q] = gP * dM(x, n, k - 2, m) + gA * dM(x, n, k - 1, m) + gB * dM(x, n, k, m) +
+            gC * dM(x, n, k + 1, m);
   }
 }
 
-__global__ void k_slopes(int ntab, const TableDesc *__restrict__ tabs, const double *__restrict__ x,
-                         const double *__restrict__ y, double *__restrict__ s,
-                         double *__restrict__ S, float4 *__restrict__ ivl,
+// power form a + b t + c t^2 + d t^3 of interval k in t = (r - x_k) / h,
+// formed in FP64; the FP32 copy is a hi/lo pair per coefficient
+__device__ void interval_form(const TableDesc &T, int k, const double *x, const double *y,
+                              const double *s, float4 *ivl, double *p64) {
+  const double y0 = y[k], y1 = y[k + 1];
+  double q[4];
+  if (T.linear) {
+    q[0] = y0;
+    q[1] = y1 - y0;
+    q[2] = q[3] = 0.0;
+  } else {
+    const double dx = x[k + 1] - x[k];
+    const double b = dx * s[k], e = dx * s[k + 1];
+    q[0] = y0;
+    q[1] = b;
+    q[2] = 3.0 * (y1 - y0) - 2.0 * b - e;
+    q[3] = 2.0 * (y0 - y1) + b + e;
+  }
+  float hi[4], lo[4];
+  for (int m = 0; m < 4; ++m) {
+    hi[m] = float(q[m]);
+    lo[m] = float(q[m] - double(hi[m]));
+    p64[4 * (T.ivl + k) + m] = q[m];
+  }
+  ivl[2 * (T.ivl + k)] = make_float4(hi[0], hi[1], hi[2], hi[3]);
+  ivl[2 * (T.ivl + k) + 1] = make_float4(lo[0], lo[1], lo[2], lo[3]);
+}
+
+// every table of one parameter set, by the threads of one block: slopes and
+// band per knot, then the interval forms (n < 4: piecewise linear, no slopes)
+__device__ void tables_block(int ntab, const TableDesc *tabs, int nknots, const double *x,
+                             const double *y, double *s, double *S, float4 *ivl, double *p64) {
+  for (int e = threadIdx.x; e < nknots; e += blockDim.x) {
+    int t = 0;
+    while (t + 1 < ntab && tabs[t + 1].off <= e) ++t;
+    const TableDesc T = tabs[t];
+    const int k = e - T.off;
+    if (T.n >= 4) {
+      knot_slope(T.n, k, x + T.off, y + T.off, s[e], S + 5 * size_t(e));
+    } else {
+      s[e] = 0.0;
+      for (int q = 0; q < 5; ++q) S[5 * size_t(e) + q] = 0.0;
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < nknots; e += blockDim.x) {
+    int t = 0;
+    while (t + 1 < ntab && tabs[t + 1].off <= e) ++t;
+    const TableDesc T = tabs[t];
+    const int k = e - T.off;
+    if (k + 1 < T.n) interval_form(T, k, x + T.off, y + T.off, s + T.off, ivl, p64);
+  }
+}
+
+__global__ void k_slopes(int ntab, int nknots, const TableDesc *__restrict__ tabs,
+                         const double *__restrict__ x, const double *__restrict__ y,
+                         double *__restrict__ s, double *__restrict__ S, float4 *__restrict__ ivl,
                          double *__restrict__ p64) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ntab) return;
-  const TableDesc T = tabs[t];
-  table_slopes(T, x + T.off, y + T.off, s + T.off, S + 5 * T.off, ivl, p64);
+  tables_block(ntab, tabs, nknots, x, y, s, S, ivl, p64);
 }
 
 __global__ void k_scatter(int D, const double *__restrict__ xv, const int32_t *__restrict__ knot_of,
@@ -199,13 +180,18 @@ template <int LPA, class T> __device__ inline T sub_sum(T v) {
   return v;
 }
 
-__device__ inline void stage_tables(const EvalArgs &a, TableDesc *sT, float4 *sI) {
+// descriptors, FP32 hi/lo interval forms and FP64 interval forms into LDS;
+// returns the LDS copy of the FP64 forms (placed after the FP32 ones)
+__device__ inline const double *stage_tables(const EvalArgs &a, TableDesc *sT, float4 *sI) {
   const float4 *src = reinterpret_cast<const float4 *>(a.tabs);
   float4 *dst = reinterpret_cast<float4 *>(sT);
   const int nq = a.ntab * int(sizeof(TableDesc) / sizeof(float4));
   for (int k = threadIdx.x; k < nq; k += blockDim.x) dst[k] = src[k];
   for (int k = threadIdx.x; k < 2 * a.nivl; k += blockDim.x) sI[k] = a.ivl32[k];
+  double *sP = reinterpret_cast<double *>(sI + 2 * a.nivl);
+  for (int k = threadIdx.x; k < 4 * a.nivl; k += blockDim.x) sP[k] = a.ivl64[k];
   __syncthreads();
+  return sP;
 }
 
 // table lookups -------------------------------------------------------------
@@ -332,8 +318,8 @@ __global__ __launch_bounds__(256) void k_density(EvalArgs a) {
   extern __shared__ float4 smem[];
   TableDesc *sT = reinterpret_cast<TableDesc *>(smem);
   float4 *sI = smem + a.ntab * int(sizeof(TableDesc) / sizeof(float4));
-  stage_tables(a, sT, sI);
-  const Tabs tb{sT, sI, a.tx, a.ty, a.ts, a.ivl64};
+  const double *sP = stage_tables(a, sT, sI);
+  const Tabs tb{sT, sI, a.tx, a.ty, a.ts, sP};
   const int sub = threadIdx.x / LPA, lane = threadIdx.x % LPA;
   for (int tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
     if (sub >= a.tile_na[tile]) continue;
@@ -530,8 +516,8 @@ __global__ __launch_bounds__(256) void k_force(EvalArgs a) {
   TableDesc *sT = reinterpret_cast<TableDesc *>(smem);
   float4 *sI = smem + a.ntab * int(sizeof(TableDesc) / sizeof(float4));
   __shared__ double sRed[APB][kPart];
-  stage_tables(a, sT, sI);
-  const Tabs tb{sT, sI, a.tx, a.ty, a.ts, a.ivl64};
+  const double *sP = stage_tables(a, sT, sI);
+  const Tabs tb{sT, sI, a.tx, a.ty, a.ts, sP};
   const int sub = threadIdx.x / LPA, lane = threadIdx.x % LPA;
   for (int tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
     const int na = a.tile_na[tile];
@@ -612,12 +598,24 @@ __global__ void k_finalize(int C, const int32_t *__restrict__ cfg_tile0,
                            const double *__restrict__ ref_stress, double ew, double sw,
                            double *__restrict__ residual, double *__restrict__ E,
                            double *__restrict__ S, double *__restrict__ L) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  // one 64-thread block per configuration: thread t sums tiles t, t + 64, ...
+  // then a fixed-shape tree combines the 64 partial sums (deterministic)
+  const int c = blockIdx.x;
   if (c >= C) return;
+  __shared__ double sred[64][kPart];
   double acc[kPart];
   for (int k = 0; k < kPart; ++k) acc[k] = 0.0;
-  for (int t = cfg_tile0[c]; t < cfg_tile0[c + 1]; ++t)
+  for (int t = cfg_tile0[c] + int(threadIdx.x); t < cfg_tile0[c + 1]; t += 64)
     for (int k = 0; k < kPart; ++k) acc[k] += part[int64_t(t) * kPart + k];
+  for (int k = 0; k < kPart; ++k) sred[threadIdx.x][k] = acc[k];
+  __syncthreads();
+  for (int w = 32; w > 0; w >>= 1) {
+    if (int(threadIdx.x) < w)
+      for (int k = 0; k < kPart; ++k) sred[threadIdx.x][k] += sred[threadIdx.x + w][k];
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  for (int k = 0; k < kPart; ++k) acc[k] = sred[0][k];
   const double V = vol[c];
   double sig[9];
   for (int k = 0; k < 9; ++k) sig[k] = acc[2 + k] / V;
@@ -699,12 +697,8 @@ __global__ void k_pop_tables(int ntab, int D, int nknots, const TableDesc *__res
   __syncthreads();
   for (int c = threadIdx.x; c < D; c += blockDim.x) ym[knot_of[c]] = X[size_t(m) * D + c];
   __syncthreads();
-  for (int t = threadIdx.x; t < ntab; t += blockDim.x) {
-    const TableDesc T = tabs[t];
-    table_slopes(T, x + T.off, ym + T.off, s + size_t(m) * nknots + T.off,
-                 S + size_t(m) * 5 * nknots + 5 * T.off, ivl + size_t(m) * 2 * nivl,
-                 p64 + size_t(m) * 4 * nivl);
-  }
+  tables_block(ntab, tabs, nknots, x, ym, s + size_t(m) * nknots, S + size_t(m) * 5 * nknots,
+               ivl + size_t(m) * 2 * nivl, p64 + size_t(m) * 4 * nivl);
 }
 
 template <int LPA>
@@ -974,7 +968,7 @@ void launch_scatter_params(Context &c, const double *d_x) {
 
 void launch_slopes(Context &c) {
   if (c.kind == FS_SW || c.ntab == 0) return;
-  hipLaunchKernelGGL(k_slopes, dim3((c.ntab + 63) / 64), dim3(64), 0, c.stream, c.ntab, c.tabs.p,
+  hipLaunchKernelGGL(k_slopes, dim3(1), dim3(256), 0, c.stream, c.ntab, c.nknots, c.tabs.p,
                      c.tab_x.p, c.tab_y.p, c.tab_s.p, c.tab_S.p, c.ivl32.p, c.ivl64.p);
   FS_HIP(hipGetLastError());
 }
@@ -984,7 +978,8 @@ void launch_eval(Context &c, bool) {
     launch_sw_eval(c);
   } else {
     const EvalArgs a = make_args(c);
-    const size_t lds = size_t(c.ntab) * sizeof(TableDesc) + 2 * size_t(c.nivl) * sizeof(float4);
+    const size_t lds = size_t(c.ntab) * sizeof(TableDesc) + 2 * size_t(c.nivl) * sizeof(float4) +
+                       4 * size_t(c.nivl) * sizeof(double);
     const int grid = std::max(1, std::min(c.ntiles, c.blocks_per_cu * c.cu_count));
     if (c.ntiles > 0) {
       const bool f64 = c.precision == FS_FP64;
@@ -997,7 +992,7 @@ void launch_eval(Context &c, bool) {
       }
     }
   }
-  hipLaunchKernelGGL(k_finalize, dim3((c.C + 127) / 128), dim3(128), 0, c.stream, c.C,
+  hipLaunchKernelGGL(k_finalize, dim3(std::max(c.C, 1)), dim3(64), 0, c.stream, c.C,
                      c.cfg_tile0.p, c.tile_part.p, c.vol.p, c.row_off.p, c.atom_off.p,
                      c.ref_energy.p, c.ref_stress.p, c.energy_weight, c.stress_weight,
                      c.residual.p, c.cfg_E.p, c.cfg_S.p, c.cfg_L.p);
