@@ -126,6 +126,15 @@ def read_configs(src, name: str = ""):
     return ds, species
 
 
+def config_names(src) -> list:
+    """The optional "name" of every configuration ("" when absent), as
+    config_reader.cpp:119 reads it."""
+    j = _load(src)
+    if not isinstance(j, list):
+        raise ValueError("top-level JSON must be an array of configurations")
+    return [str(o.get("name", "")) if isinstance(o, dict) else "" for o in j]
+
+
 def write_configs(ds: Dataset, species, path=None) -> str:
     """The inverse of read_configs (weights written as 1)."""
     out = []

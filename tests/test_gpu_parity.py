@@ -218,3 +218,23 @@ def test_exact_edge_pairs_follow_linear_sites():
     Fo = o.eval_forces()[0]
     Fg = g.eval_forces()[0]
     assert np.abs(Fg - Fo).max() <= 1e-5
+
+
+@pytest.mark.parametrize("name", ["eam3", "adp2", "sw"])
+def test_evaluate_report_passes_the_force_gate(workloads, name):
+    # the report built from the device evaluation against the oracle's, through
+    # the reference harness's forces-only gate (rtol = atol = 1e-5); its
+    # total_sumsq is the device cost |F|^2
+    from forcesmith_amd import report as rp
+    w = workloads[name]
+    ew, sw = w.model.energy_weight, w.model.stress_weight
+    o = pair_oracle(w)
+    g = fa.GpuCalculator(w.dataset, w.model)
+    g.prepare()
+    rg = rp.evaluate_report(g, ew, sw)
+    ro = rp.build_report(w.dataset, *o.eval_forces(), ew, sw)
+    d = rp.diff_reports(rg, ro)
+    _print(f"[report {name}] n={d['n']} max|dF|={d['max_abs']:.3e} rel={d['rel']:.3e}")
+    assert d["ok"], d
+    assert rg["total_sumsq"] == pytest.approx(g.cost(), rel=1e-6)
+    assert rg["total_sumsq"] == pytest.approx(ro["total_sumsq"], rel=1e-6)
