@@ -33,11 +33,11 @@ STATUS = {0: "FS_OK", -1: "FS_ERR_ARG", -2: "FS_ERR_STATE", -3: "FS_ERR_HIP", -4
 # every symbol include/forcesmith_amd.h declares
 EXPORTS = (
     "fs_abi_version", "fs_ctx_create", "fs_ctx_retain", "fs_ctx_release", "fs_last_error",
-    "fs_upload_dataset", "fs_set_tabulated_model", "fs_set_sw_model", "fs_set_weights",
+    "fs_upload_dataset", "fs_set_tabulated_model", "fs_set_sw_model", "fs_set_weights", "fs_set_smooth_weight",
     "fs_param_count", "fs_gather_params", "fs_scatter_params", "fs_gather_bounds", "fs_max_cutoff",
     "fs_residual_count", "fs_row_offsets", "fs_prepare", "fs_neighbor_count", "fs_get_neighbor_list",
     "fs_eval_forces", "fs_eval_config", "fs_eval_residual", "fs_eval_jacobian", "fs_normal_equations",
-    "fs_lm_minimize", "fs_population_cost", "fs_de_minimize", "fs_device_buffer", "fs_synchronize",
+    "fs_lm_minimize", "fs_lm_solve", "fs_population_cost", "fs_de_minimize", "fs_device_buffer", "fs_synchronize",
     "fs_set_launch", "fs_autotune", "fs_time_step", "fs_mlp_forward", "fs_mlp_loss_grad",
 )
 
@@ -50,6 +50,11 @@ class FsError(RuntimeError):
 
 class fs_opts(ct.Structure):
     _fields_ = [("device", ct.c_int32), ("precision", ct.c_int32), ("reserved", ct.c_int32 * 6)]
+
+
+class fs_lm_opts(ct.Structure):
+    _fields_ = [("max_iter", ct.c_int32), ("maxfev", ct.c_int32), ("xtol", ct.c_double),
+                ("ftol", ct.c_double), ("gtol", ct.c_double), ("factor", ct.c_double)]
 
 
 class fs_de_opts(ct.Structure):
@@ -88,6 +93,7 @@ def load_library(path: str = LIB_PATH):
                                               _u8p]),
         "fs_set_sw_model": (ct.c_int, [vp, ct.c_int32, _dp, _u8p, _dp, _u8p]),
         "fs_set_weights": (ct.c_int, [vp, ct.c_double, ct.c_double]),
+        "fs_set_smooth_weight": (ct.c_int, [vp, ct.c_double]),
         "fs_param_count": (ct.c_int, [vp, _i32p]),
         "fs_gather_params": (ct.c_int, [vp, _dp]),
         "fs_scatter_params": (ct.c_int, [vp, _dp]),
@@ -104,6 +110,7 @@ def load_library(path: str = LIB_PATH):
         "fs_eval_jacobian": (ct.c_int, [vp, _fp]),
         "fs_normal_equations": (ct.c_int, [vp, _dp, _dp, _dp]),
         "fs_lm_minimize": (ct.c_int, [vp, _dp, ct.c_int32, _dp, _i32p]),
+        "fs_lm_solve": (ct.c_int, [vp, ct.POINTER(fs_lm_opts), _dp, _i32p, _i32p, _dp]),
         "fs_population_cost": (ct.c_int, [vp, _dp, ct.c_int32, _dp]),
         "fs_de_minimize": (ct.c_int, [vp, ct.POINTER(fs_de_opts), _dp, _dp, _dp, _dp, _dp, _i32p]),
         "fs_device_buffer": (ct.c_int, [vp, ct.c_int32, ct.POINTER(vp), _i64p]),
@@ -130,7 +137,7 @@ class GpuCalculator:
     """A dataset plus a model resident on one HIP device (one fs_ctx)."""
 
     def __init__(self, dataset: Dataset, model, precision: str = "mixed", device: int = 0,
-                 energy_weight=None, stress_weight=None):
+                 energy_weight=None, stress_weight=None, smooth_weight: float = 0.0):
         lib = load_library()
         self._lib = lib
         opts = fs_opts(device, FS_FP64 if precision == "fp64" else FS_MIXED)
@@ -154,6 +161,8 @@ class GpuCalculator:
         ew = model.energy_weight if energy_weight is None else energy_weight
         sw = model.stress_weight if stress_weight is None else stress_weight
         self._check(lib.fs_set_weights(h, float(ew), float(sw)))
+        if smooth_weight:
+            self.set_smooth_weight(smooth_weight)
         self.n_atoms = dataset.n_atoms
         self.n_configs = dataset.n_configs
 
@@ -177,6 +186,10 @@ class GpuCalculator:
             fx = np.concatenate([t.fixed for t in model.tables]).astype(np.uint8)
             self._check(lib.fs_set_tabulated_model(h, kind, model.ntypes, len(model.tables), _p(n, _i32p),
                                                    _p(rmin, _dp), _p(rmax, _dp), _p(y, _dp), _p(fx, _u8p)))
+
+    def set_smooth_weight(self, w: float):
+        """Curvature rows w (y[k-1] - 2 y[k] + y[k+1]) after the config rows."""
+        self._check(self._lib.fs_set_smooth_weight(self._h, float(w)))
 
     def close(self):
         if getattr(self, "_h", None):
@@ -298,6 +311,18 @@ class GpuCalculator:
         acc = ct.c_int32()
         self._check(self._lib.fs_lm_minimize(self._h, _p(x, _dp), int(max_iter), _p(hist, _dp), ct.byref(acc)))
         return x, hist, int(acc.value)
+
+    def lm_solve(self, x, max_iter=500, maxfev=0, xtol=1e-7, ftol=1e-7, gtol=0.0, factor=100.0):
+        """EigenLMSolver semantics: (x, status, (nfev, njev, accepted), cost
+        history per call of one step)."""
+        o = fs_lm_opts(int(max_iter), int(maxfev), float(xtol), float(ftol), float(gtol), float(factor))
+        x = np.ascontiguousarray(x, dtype=np.float64).copy()
+        st = ct.c_int32()
+        cnt = np.zeros(3, dtype=np.int32)
+        hist = np.zeros(max(int(max_iter), 1))
+        self._check(self._lib.fs_lm_solve(self._h, ct.byref(o), _p(x, _dp), ct.byref(st), _p(cnt, _i32p),
+                                          _p(hist, _dp)))
+        return x, int(st.value), tuple(int(v) for v in cnt), hist
 
     def population_cost(self, X):
         X = np.ascontiguousarray(X, dtype=np.float64)

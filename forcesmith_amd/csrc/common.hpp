@@ -121,6 +121,8 @@ void launch_slopes(Context &c);
 void launch_scatter_params(Context &c, const double *d_x);
 void launch_eval(Context &c, bool want_forces);
 void launch_cost(Context &c, double *d_out);
+void launch_smooth_residual(Context &c);
+void launch_smooth_jacobian(Context &c);
 void launch_jacobian(Context &c);
 void launch_normal_equations(Context &c);
 int lm_minimize_device(Context &c, double *x, int max_iter, double *hist);
@@ -184,7 +186,13 @@ struct Context {
   DBuf<double> x_dev;
   bool params_dirty = true;
   double energy_weight = 1.0, stress_weight = 0.0;
-  int64_t R = 0;
+  int64_t R = 0; // residual rows: configuration rows, then curvature rows
+  // curvature (smoothness) rows w (y[k-1] - 2 y[k] + y[k+1]) after the
+  // configuration rows: {first knot, free column of knots k-1, k, k+1 or -1}
+  double smooth_weight = 0.0;
+  int64_t R_cfg = 0;
+  int n_smooth = 0;
+  DBuf<int4> smooth_rows;
   std::vector<int64_t> h_row_off;
 
   // ---- neighbour list ------------------------------------------------------

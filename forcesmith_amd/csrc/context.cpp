@@ -86,16 +86,39 @@ void build_tiles(Context &c) {
   c.tile_part.alloc(size_t(std::max(c.ntiles, 1)) * 11);
 }
 
+// curvature rows of the tabulated model: n - 2 per table with a free knot and
+// n >= 3, tables in declaration order (SplinePotential::curvature_count,
+// spline.hpp:104-106; model_smoothness_count, smoothness.hpp:17-23), only when
+// the weight is positive (forcesmith_functor.cpp:202-207); SW has none
+void smooth_layout(Context &c) {
+  std::vector<int4> rows;
+  if (c.smooth_weight > 0.0 && c.have_model && c.kind != FS_SW) {
+    for (int t = 0; t < c.ntab; ++t) {
+      const TableDesc &T = c.h_tabs[t];
+      bool any_free = false;
+      for (int k = 0; k < T.n; ++k) any_free |= c.h_col_of[T.off + k] >= 0;
+      if (!any_free || T.n < 3) continue;
+      for (int k = 1; k + 1 < T.n; ++k)
+        rows.push_back(int4{T.off + k - 1, c.h_col_of[T.off + k - 1], c.h_col_of[T.off + k],
+                            c.h_col_of[T.off + k + 1]});
+    }
+  }
+  c.n_smooth = int(rows.size());
+  if (c.n_smooth > 0) upload(c, c.smooth_rows, rows);
+  c.R = c.R_cfg + c.n_smooth;
+  c.residual.alloc(size_t(std::max<int64_t>(c.R, 1)));
+  c.forces_valid = false;
+}
+
 void compute_rows(Context &c) {
   c.h_row_off.assign(c.C + 1, 0);
   for (int k = 0; k < c.C; ++k) {
     const int64_t na = c.h_atom_off[k + 1] - c.h_atom_off[k];
     c.h_row_off[k + 1] = c.h_row_off[k] + 3 * na + 2 + (c.stress_weight > 0.0 ? 6 : 0);
   }
-  c.R = c.h_row_off[c.C];
+  c.R_cfg = c.h_row_off[c.C];
   upload(c, c.row_off, c.h_row_off);
-  c.residual.alloc(size_t(std::max<int64_t>(c.R, 1)));
-  c.forces_valid = false;
+  smooth_layout(c);
 }
 
 double max_cutoff(const Context &c) {
@@ -211,6 +234,8 @@ template <class F> int guarded(fs_ctx *ctx, F &&f) {
 int fs_jacobian_impl(Context &c, float *J);
 int fs_normal_impl(Context &c, double *JtJ, double *Jtr, double *cost);
 int fs_lm_impl(Context &c, double *x, int max_iter, double *hist, int *accepted);
+int fs_lm_solve_impl(Context &c, const fs_lm_opts &o, double *x, int *status, int *counts,
+                     double *hist);
 int fs_population_impl(Context &c, const double *X, int np, double *cost);
 int fs_de_impl(Context &c, const fs_de_opts *o, const double *lo, const double *hi, double *x,
                double *best, double *hist, int *gens);
@@ -417,6 +442,7 @@ int fs_set_tabulated_model(fs_ctx *ctx, int32_t kind, int32_t ntypes, int32_t nt
     upload(c, c.knot_of, c.h_knot_of);
     c.have_model = true;
     c.params_dirty = true;
+    smooth_layout(c);
     if (c.have_nl && c.C > 0) {
       if (max_cutoff(c) == c.nl_rc) launch_pair_flags(c);
       else c.have_nl = false;
@@ -446,7 +472,9 @@ int fs_set_sw_model(fs_ctx *ctx, int32_t ntypes, const double *pair_params,
     c.have_model = true;
     c.params_dirty = true;
     c.have_nl = false;
+    smooth_layout(c);
     c.forces_valid = false;
+    FS_HIP(hipStreamSynchronize(c.stream));
   });
 }
 
@@ -457,6 +485,16 @@ int fs_set_weights(fs_ctx *ctx, double energy_weight, double stress_weight) {
     c.energy_weight = energy_weight;
     c.stress_weight = stress_weight;
     if (c.C > 0) compute_rows(c);
+    FS_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int fs_set_smooth_weight(fs_ctx *ctx, double weight) {
+  return guarded(ctx, [&](Context &c) {
+    FS_REQUIRE(std::isfinite(weight) && weight >= 0.0, FS_ERR_ARG,
+               "smooth weight must be finite and >= 0");
+    c.smooth_weight = weight;
+    smooth_layout(c);
     FS_HIP(hipStreamSynchronize(c.stream));
   });
 }
@@ -662,6 +700,24 @@ int fs_normal_equations(fs_ctx *ctx, double *JtJ, double *Jtr, double *cost) {
   int rc = FS_OK;
   const int g = guarded(ctx, [&](Context &c) { rc = fs_normal_impl(c, JtJ, Jtr, cost); });
   return g != FS_OK ? g : rc;
+}
+
+int fs_lm_solve(fs_ctx *ctx, const fs_lm_opts *opts, double *x, int32_t *status,
+                int32_t *counts, double *cost_hist) {
+  return guarded(ctx, [&](Context &c) {
+    FS_REQUIRE(x && status, FS_ERR_ARG, "null LM argument");
+    c.fail_if_no_dataset();
+    c.fail_if_no_model();
+    fs_lm_opts o{500, 0, 1e-7, 1e-7, 0.0, 100.0};
+    if (opts) o = *opts;
+    FS_REQUIRE(o.max_iter >= 0, FS_ERR_ARG, "max_iter must be >= 0");
+    ensure_ready(c);
+    int st = 0, cnt[3] = {0, 0, 0};
+    fs_lm_solve_impl(c, o, x, &st, cnt, cost_hist);
+    *status = st;
+    if (counts)
+      for (int k = 0; k < 3; ++k) counts[k] = cnt[k];
+  });
 }
 
 int fs_lm_minimize(fs_ctx *ctx, double *x, int32_t max_iter, double *cost_hist,

@@ -632,6 +632,18 @@ __global__ void k_finalize(int C, const int32_t *__restrict__ cfg_tile0,
 }
 
 // --- |F|^2, deterministic two-level sum ---------------------------------------
+// curvature rows w (y[k-1] - 2 y[k] + y[k+1]) (SplinePotential::
+// write_curvature, src/potentials/spline.cpp:205-213)
+__device__ inline double curvature_row(const int4 rw, const double *y, double w) {
+  return w * (y[rw.x] - 2.0 * y[rw.x + 1] + y[rw.x + 2]);
+}
+
+__global__ void k_smooth_res(int n, const int4 *__restrict__ rows, const double *__restrict__ y,
+                             double w, double *__restrict__ out) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) out[r] = curvature_row(rows[r], y, w);
+}
+
 __global__ void k_sumsq(int64_t n, const double *__restrict__ v, double *__restrict__ part) {
   __shared__ double s[256];
   double acc = 0.0;
@@ -891,11 +903,19 @@ __global__ void k_pop_finalize(int nb, int C, const int32_t *__restrict__ grp_t0
   cost_mc[id] = cost;
 }
 
-__global__ void k_pop_sum(int nb, int C, const double *__restrict__ cost_mc, double *__restrict__ cost) {
+// member cost: its configurations' costs, then its curvature rows squared
+__global__ void k_pop_sum(int nb, int C, const double *__restrict__ cost_mc, int n_smooth,
+                          const int4 *__restrict__ rows, double w, const double *__restrict__ y,
+                          int nknots, double *__restrict__ cost) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= nb) return;
   double s = 0.0;
   for (int c = 0; c < C; ++c) s += cost_mc[size_t(m) * C + c];
+  const double *ym = y + size_t(m) * nknots;
+  for (int r = 0; r < n_smooth; ++r) {
+    const double v = curvature_row(rows[r], ym, w);
+    s += v * v;
+  }
   cost[m] = s;
 }
 
@@ -997,6 +1017,15 @@ void launch_eval(Context &c, bool) {
                      c.ref_energy.p, c.ref_stress.p, c.energy_weight, c.stress_weight,
                      c.residual.p, c.cfg_E.p, c.cfg_S.p, c.cfg_L.p);
   FS_HIP(hipGetLastError());
+  launch_smooth_residual(c);
+}
+
+void launch_smooth_residual(Context &c) {
+  if (c.n_smooth == 0) return;
+  hipLaunchKernelGGL(k_smooth_res, dim3(unsigned((c.n_smooth + 127) / 128)), dim3(128), 0, c.stream,
+                     c.n_smooth, c.smooth_rows.p, c.tab_y.p, c.smooth_weight,
+                     c.residual.p + c.R_cfg);
+  FS_HIP(hipGetLastError());
 }
 
 bool population_supported(const Context &c) {
@@ -1090,7 +1119,8 @@ void population_launch(Context &c, const double *d_X, int np, double *d_cost) {
                        c.pop_cmc.p);
     FS_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_pop_sum, dim3(unsigned((nb + 127) / 128)), dim3(128), 0, c.stream, nb, c.C,
-                       c.pop_cmc.p, d_cost + m0);
+                       c.pop_cmc.p, c.n_smooth, c.smooth_rows.p, c.smooth_weight, c.pop_y.p,
+                       c.nknots, d_cost + m0);
     FS_HIP(hipGetLastError());
   }
 }

@@ -1338,7 +1338,28 @@ __global__ __launch_bounds__(64) void k_sw_jac_tiles(SwJArgs a) {
 
 inline unsigned nb(int64_t n, int b) { return unsigned((n + b - 1) / b); }
 
+// curvature rows: constant w (1, -2, 1) on the free knots k-1, k, k+1, zero
+// elsewhere; one block per row
+__global__ void k_smooth_jac(int D, const int4 *__restrict__ rows, float w, float *__restrict__ J) {
+  const int4 rw = rows[blockIdx.x];
+  float *row = J + int64_t(blockIdx.x) * D;
+  for (int col = threadIdx.x; col < D; col += blockDim.x) {
+    float v = 0.0f;
+    if (col == rw.y) v = w;
+    if (col == rw.z) v = -2.0f * w;
+    if (col == rw.w) v = w;
+    row[col] = v;
+  }
+}
+
 } // namespace
+
+void launch_smooth_jacobian(Context &c) {
+  if (c.n_smooth == 0 || c.D == 0) return;
+  hipLaunchKernelGGL(k_smooth_jac, dim3(unsigned(c.n_smooth)), dim3(256), 0, c.stream, c.D,
+                     c.smooth_rows.p, float(c.smooth_weight), c.J.p + size_t(c.R_cfg) * c.D);
+  FS_HIP(hipGetLastError());
+}
 
 void launch_jacobian(Context &c) {
   FS_REQUIRE(c.D <= 64 * kCPL, FS_ERR_UNSUPPORTED, "more than 256 free parameters");
@@ -1574,6 +1595,7 @@ void launch_jacobian(Context &c) {
   }
   hipLaunchKernelGGL(k_jac_rows, dim3(c.C), dim3(256), 0, c.stream, a);
   FS_HIP(hipGetLastError());
+  launch_smooth_jacobian(c);
 }
 
 } // namespace fsa

@@ -4,10 +4,12 @@
 // autotuner. Each step's arithmetic runs on the GPU; the host only carries
 // scalars (costs, the LM damping, DE selection) between launches.
 #include "common.hpp"
+#include "lm_tr.hpp"
 
 #include <algorithm>
 #include <cmath>
 #include <limits>
+#include <vector>
 
 namespace fsa {
 
@@ -148,6 +150,36 @@ int fs_lm_impl(Context &c, double *x, int max_iter, double *hist, int *accepted)
       launch_diag_scale(c);
     }
   }
+  scatter_host(c, x);
+  evaluate_for_solver(c);
+  FS_HIP(hipStreamSynchronize(c.stream));
+  return FS_OK;
+}
+
+// Levenberg-Marquardt with the reference's stopping semantics (lm_tr.hpp):
+// the residual norm and the FP64 normal equations come from the GPU, the
+// D x D trust-region algebra runs on the host
+int fs_lm_solve_impl(Context &c, const fs_lm_opts &o, double *x, int *status, int *counts,
+                     double *hist) {
+  const lmtr::Opts opts{o.max_iter, o.maxfev, o.xtol, o.ftol, o.gtol, o.factor};
+  const int n = c.D;
+  auto cost = [&](const double *xv) {
+    scatter_host(c, xv);
+    return cost_now(c);
+  };
+  auto normal = [&](double *A, double *g) {
+    launch_jacobian(c);
+    launch_normal_equations(c);
+    FS_HIP(hipMemcpyAsync(A, c.JtJ.p, sizeof(double) * size_t(n) * n, hipMemcpyDeviceToHost,
+                          c.stream));
+    FS_HIP(hipMemcpyAsync(g, c.Jtr.p, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+    FS_HIP(hipStreamSynchronize(c.stream));
+  };
+  // lmtr::minimize calls normal() right after the cost of the accepted x
+  // (the initial point or the trial it just accepted), so the device
+  // residual and per-atom state are at x
+  *status = lmtr::minimize(n, c.R, opts, x, cost, normal, counts, hist);
+  // leave the device state at the returned parameters
   scatter_host(c, x);
   evaluate_for_solver(c);
   FS_HIP(hipStreamSynchronize(c.stream));

@@ -92,6 +92,14 @@ int fs_set_sw_model(fs_ctx *ctx, int32_t ntypes, const double *pair_params,
                     const uint8_t *lambda_fixed);
 /* OptimizerOptions energy_weight / stress_weight (optimizer.hpp:11-15). */
 int fs_set_weights(fs_ctx *ctx, double energy_weight, double stress_weight);
+/* OptimizerOptions smooth_weight (optimizer.hpp:14): when > 0, the residual
+ * gains the curvature rows w (y[k-1] - 2 y[k] + y[k+1]), k = 1..n-2, of every
+ * table with a free knot and n >= 3, tables in declaration order, after the
+ * configuration rows (forcesmith_functor.cpp:53-55,198-207; spline.cpp:
+ * 205-213); their Jacobian rows are the constants w (1, -2, 1) on the free
+ * knots. fs_residual_count includes them; fs_row_offsets[C] is their first
+ * row. Stillinger-Weber models have none. */
+int fs_set_smooth_weight(fs_ctx *ctx, double weight);
 
 /* CFittable surface (fit_params.hpp:24-30; TableParams
  * force_calculator_concept.hpp:171-208). */
@@ -137,6 +145,31 @@ int fs_normal_equations(fs_ctx *ctx, double *JtJ, double *Jtr, double *cost);
  * x is read and written; cost_hist[max_iter] (may be NULL). */
 int fs_lm_minimize(fs_ctx *ctx, double *x, int32_t max_iter,
                    double *cost_hist, int32_t *accepted);
+
+/* Levenberg-Marquardt with the reference's stopping semantics: replaces
+ * Solver::minimize with EigenLMSolver (solver.cpp:67-85; solver.hpp:65-73),
+ * i.e. Eigen's LevenbergMarquardt (MINPACK lmder, More 1978): trust region
+ * on |D p|, the lmpar damping search, termination on xtol / ftol / gtol /
+ * maxfev. J, J^T J and J^T F are formed on the GPU (FP64 products of the FP32
+ * analytic Jacobian); the D x D trust-region algebra runs on the host from
+ * the normal equations. */
+typedef struct fs_lm_opts {
+  int32_t max_iter; /* calls of one step (EigenLMSolver::max_iter; 500) */
+  int32_t maxfev;   /* <= 0: max(max_iter, 1) * 100 + 100 (solver.cpp:75) */
+  double xtol;      /* 1e-7 */
+  double ftol;      /* 1e-7 */
+  double gtol;      /* 0 (Eigen's default) */
+  double factor;    /* initial step bound factor, 100 (Eigen's default) */
+} fs_lm_opts;
+/* opts NULL: the defaults above. x is read and written (the last accepted
+ * point). status receives Eigen's LevenbergMarquardtSpace code as
+ * EigenLMSolver::minimize returns it: 0 improper input, 1 relative reduction,
+ * 2 relative error (also: still running after max_iter calls, solver.cpp:84),
+ * 3 both, 4 gradient cosine, 5 too many evaluations, 6-8 tolerances too small.
+ * counts[3] (may be NULL): residual evaluations, Jacobian evaluations,
+ * accepted steps. cost_hist[max_iter] (may be NULL): |F|^2 after each call. */
+int fs_lm_solve(fs_ctx *ctx, const fs_lm_opts *opts, double *x, int32_t *status,
+                int32_t *counts, double *cost_hist);
 
 /* Exact cost |F(x_m)|^2 of np parameter vectors X[np*D] (row-major), the
  * per-member cost of BoostDESolver (solver.cpp:143-146), batched. */

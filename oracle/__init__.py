@@ -29,6 +29,19 @@ _i32p = ct.POINTER(ct.c_int32)
 _i64p = ct.POINTER(ct.c_int64)
 _u8p = ct.POINTER(ct.c_uint8)
 
+# or_lmder callbacks: f(user, x, out) -> int
+FCN = ct.CFUNCTYPE(ct.c_int, ct.c_void_p, _dp, _dp)
+
+
+class LmOpts(ct.Structure):
+    """or_lm_opts; defaults are EigenLMSolver's (solver.hpp:65-68) and Eigen's
+    factor 100, gtol 0; maxfev 0 = max(max_iter, 1) * 100 + 100."""
+    _fields_ = [("max_iter", ct.c_int32), ("maxfev", ct.c_int32), ("xtol", ct.c_double),
+                ("ftol", ct.c_double), ("gtol", ct.c_double), ("factor", ct.c_double)]
+
+    def __init__(self, max_iter=500, maxfev=0, xtol=1e-7, ftol=1e-7, gtol=0.0, factor=100.0):
+        super().__init__(int(max_iter), int(maxfev), float(xtol), float(ftol), float(gtol), float(factor))
+
 
 def build() -> str:
     """Compile the oracle with its committed Makefile (gcc, -ffp-contract=off)."""
@@ -56,6 +69,8 @@ def _load():
         "or_max_cutoff": (ct.c_double, [ct.c_void_p]),
         "or_set_weights": (None, [ct.c_void_p, ct.c_double, ct.c_double]),
         "or_residual_count": (ct.c_int64, [ct.c_void_p]),
+        "or_set_smooth_weight": (None, [ct.c_void_p, ct.c_double]),
+        "or_smooth_count": (ct.c_int64, [ct.c_void_p]),
         "or_row_offsets": (None, [ct.c_void_p, _i64p]),
         "or_prepare": (ct.c_int64, [ct.c_void_p]),
         "or_build_nl": (ct.c_int64, [ct.c_void_p, ct.c_double]),
@@ -73,6 +88,11 @@ def _load():
         "or_spline_param_grad": (None, [ct.c_int, _dp, _dp, ct.c_double, _dp, _dp, _dp]),
         "or_knot_grid": (None, [ct.c_int, ct.c_double, ct.c_double, _dp]),
         "or_lm": (ct.c_int, [ct.c_void_p, _dp, ct.c_int, ct.c_int, _dp]),
+        "or_lmder": (ct.c_int, [ct.c_int64, ct.c_int, FCN, FCN, ct.c_void_p, _dp, ct.POINTER(LmOpts), _i32p,
+                                _dp]),
+        "or_lm_minpack": (ct.c_int, [ct.c_void_p, _dp, ct.POINTER(LmOpts), _i32p, _dp]),
+        "or_qr_pivot": (ct.c_int, [ct.c_int64, ct.c_int, _dp, _dp, _i32p, _dp, _dp]),
+        "or_lmpar": (ct.c_int, [ct.c_int, _dp, _i32p, ct.c_int, _dp, _dp, ct.c_double, _dp, _dp]),
         "or_population_cost": (None, [ct.c_void_p, _dp, ct.c_int, _dp]),
         "or_rng_u64": (ct.c_uint64, [ct.c_uint64, ct.c_uint64, ct.c_uint64]),
         "or_rng_unit": (ct.c_double, [ct.c_uint64, ct.c_uint64, ct.c_uint64]),
@@ -257,6 +277,14 @@ class Oracle:
     def residual_count(self) -> int:
         return int(self._lib.or_residual_count(self._h))
 
+    def set_smooth_weight(self, w: float):
+        """Curvature rows w (y[k-1] - 2 y[k] + y[k+1]) after the config rows."""
+        self._lib.or_set_smooth_weight(self._h, float(w))
+
+    @property
+    def smooth_count(self) -> int:
+        return int(self._lib.or_smooth_count(self._h))
+
     def row_offsets(self):
         r = np.zeros(self.n_configs + 1, dtype=np.int64)
         self._lib.or_row_offsets(self._h, _p(r, _i64p))
@@ -333,6 +361,16 @@ class Oracle:
         acc = self._lib.or_lm(self._h, _p(x, _dp), int(max_iter), int(use_fd), _p(hist, _dp))
         return x, hist, acc
 
+    def lm_minpack(self, x, **opts):
+        """LM with the reference's stopping semantics; returns (x, status,
+        (nfev, njev, accepted), cost history per call)."""
+        o = LmOpts(**opts)
+        x = _f64(x).copy()
+        cnt = np.zeros(3, dtype=np.int32)
+        hist = np.zeros(max(o.max_iter, 1))
+        st = self._lib.or_lm_minpack(self._h, _p(x, _dp), ct.byref(o), _p(cnt, _i32p), _p(hist, _dp))
+        return x, int(st), tuple(int(v) for v in cnt), hist
+
     def population_cost(self, X):
         X = _f64(X)
         c = np.zeros(X.shape[0])
@@ -355,3 +393,53 @@ class Oracle:
         x = _f64(x)
         F = np.zeros(self.residual_count)
         return float(self._lib.or_time_residual(self._h, _p(x, _dp), int(reps), _p(F, _dp)))
+
+
+def lmder(fun, jac, x0, m, **opts):
+    """or_lmder on Python callables fun(x) -> f[m], jac(x) -> J[m, n]:
+    returns (x, status, (nfev, njev, accepted), cost history)."""
+    lib = _load()
+    x = _f64(x0).copy()
+    n = x.size
+
+    def f_cb(_u, xp, out):
+        xv = np.ctypeslib.as_array(xp, shape=(n,)).copy()
+        np.ctypeslib.as_array(out, shape=(m,))[:] = fun(xv)
+        return 0
+
+    def j_cb(_u, xp, out):
+        xv = np.ctypeslib.as_array(xp, shape=(n,)).copy()
+        np.ctypeslib.as_array(out, shape=(m * n,))[:] = np.asarray(jac(xv), dtype=np.float64).reshape(-1)
+        return 0
+
+    fcb, jcb = FCN(f_cb), FCN(j_cb)
+    o = LmOpts(**opts)
+    cnt = np.zeros(3, dtype=np.int32)
+    hist = np.zeros(max(o.max_iter, 1))
+    st = lib.or_lmder(int(m), int(n), fcb, jcb, None, _p(x, _dp), ct.byref(o), _p(cnt, _i32p), _p(hist, _dp))
+    return x, int(st), tuple(int(v) for v in cnt), hist
+
+
+def qr_pivot(J, f):
+    """Pivoted QR of J (m x n): (rank, perm, R (n x n upper), Q^T f [n])."""
+    lib = _load()
+    J = _f64(J)
+    m, n = J.shape
+    perm = np.zeros(n, dtype=np.int32)
+    Rc = np.zeros(n * n)
+    qtf = np.zeros(n)
+    rank = lib.or_qr_pivot(m, n, _p(J, _dp), _p(_f64(f), _dp), _p(perm, _i32p), _p(Rc, _dp), _p(qtf, _dp))
+    return int(rank), perm, Rc.reshape(n, n).T.copy(), qtf
+
+
+def lmpar(R, perm, rank, diag, qtf, delta, par=0.0):
+    """lmpar on a pivoted QR: (par, x, damped solves)."""
+    lib = _load()
+    R = _f64(R)
+    n = R.shape[0]
+    Rc = np.ascontiguousarray(R.T)
+    p = ct.c_double(par)
+    x = np.zeros(n)
+    k = lib.or_lmpar(n, _p(Rc, _dp), _p(np.ascontiguousarray(perm, dtype=np.int32), _i32p), int(rank),
+                     _p(_f64(diag), _dp), _p(_f64(qtf), _dp), float(delta), ct.byref(p), _p(x, _dp))
+    return p.value, x, int(k)

@@ -510,6 +510,7 @@ struct or_ctx {
   int nlam;
   /* weights */
   double ew, sw_w;
+  double smooth_w; /* curvature-row weight, 0 = no rows */
   int64_t *row_off;
   /* neighbour list */
   double nl_rc;
@@ -782,7 +783,23 @@ void or_set_weights(or_ctx *c, double ew, double sw) {
     c->row_off[k + 1] = c->row_off[k] + 3 * na + 2 + (sw > 0.0 ? 6 : 0);
   }
 }
-int64_t or_residual_count(const or_ctx *c) { return c->row_off[c->C]; }
+/* curvature rows: SplinePotential::curvature_count (spline.hpp:104-106) is
+ * n - 2 for a table with at least one free knot and n >= 3, summed over the
+ * tables in declaration order (smoothness.hpp:17-23); analytic SW slots have
+ * none. The functor adds them only when smooth_weight > 0
+ * (forcesmith_functor.cpp:202-207). */
+int64_t or_smooth_count(const or_ctx *c) {
+  if (!(c->smooth_w > 0.0) || c->kind == OR_SW) return 0;
+  int64_t n = 0;
+  for (int t = 0; t < c->ntab; ++t) {
+    int nfree = 0;
+    for (int k = 0; k < c->tab[t].n; ++k) nfree += !c->tab[t].fixed[k];
+    if (nfree > 0 && c->tab[t].n >= 3) n += c->tab[t].n - 2;
+  }
+  return n;
+}
+void or_set_smooth_weight(or_ctx *c, double w) { c->smooth_w = w; }
+int64_t or_residual_count(const or_ctx *c) { return c->row_off[c->C] + or_smooth_count(c); }
 void or_row_offsets(const or_ctx *c, int64_t *r) {
   memcpy(r, c->row_off, sizeof(int64_t) * (size_t)(c->C + 1));
 }
@@ -1311,12 +1328,30 @@ static void residual_task(void *arg, int64_t k) {
   write_rows(rj->c, (int)k, rj->F);
 }
 
+/* rows w (y[k-1] - 2 y[k] + y[k+1]), k = 1 .. n-2, of every table that
+ * has curvature rows, after the configuration rows
+ * (SplinePotential::write_curvature, spline.cpp:205-213;
+ * forcesmith_functor.cpp:53-55) */
+static void write_smooth_rows(const or_ctx *c, double *F) {
+  if (or_smooth_count(c) == 0) return;
+  int64_t row = c->row_off[c->C];
+  for (int t = 0; t < c->ntab; ++t) {
+    const or_spline *sp = &c->tab[t];
+    int nfree = 0;
+    for (int k = 0; k < sp->n; ++k) nfree += !sp->fixed[k];
+    if (nfree == 0 || sp->n < 3) continue;
+    for (int k = 1; k + 1 < sp->n; ++k)
+      F[row++] = c->smooth_w * (sp->y[k - 1] - 2.0 * sp->y[k] + sp->y[k + 1]);
+  }
+}
+
 /* ForcesmithFunctor eval_into (forcesmith_functor.cpp:35-57) */
 void or_residual(or_ctx *c, const double *x, double *F) {
   or_scatter_params(c, x);
   ensure_nl(c);
   res_job rj = {c, F};
   parallel_for(c->C, residual_task, &rj);
+  write_smooth_rows(c, F);
 }
 
 /* opt::fd_jacobian (fd_jacobian.hpp:37-49), h = 1e-5 */
@@ -1912,6 +1947,23 @@ int or_jacobian_analytic(or_ctx *c, const double *x, double *J) {
   jj.J = J;
   jj.D = D;
   parallel_for(c->C, spline_jac_config, &jj);
+  /* curvature rows: constant w (1, -2, 1) on the free knots k-1, k, k+1 */
+  if (or_smooth_count(c) > 0) {
+    int64_t row = c->row_off[c->C];
+    for (int t = 0; t < c->ntab; ++t) {
+      const or_spline *sp = &c->tab[t];
+      int nfree = 0;
+      for (int k = 0; k < sp->n; ++k) nfree += !sp->fixed[k];
+      if (nfree == 0 || sp->n < 3) continue;
+      for (int k = 1; k + 1 < sp->n; ++k, ++row) {
+        const double w3[3] = {1.0, -2.0, 1.0};
+        for (int d = 0; d < 3; ++d) {
+          const int col = jj.cm.col_of[jj.cm.col[t] + k - 1 + d];
+          if (col >= 0) J[row * D + col] = c->smooth_w * w3[d];
+        }
+      }
+    }
+  }
   free_colmap(&jj.cm);
   return 0;
 }
@@ -2047,6 +2099,384 @@ int or_lm(or_ctx *c, double *x, int max_iter, int use_fd, double *cost_hist) {
   or_scatter_params(c, x);
   free(F); free(Fn); free(J); free(A); free(M); free(g); free(dx); free(xn); free(dg);
   return accepted;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Levenberg-Marquardt with the reference's stopping semantics               */
+/* ------------------------------------------------------------------------ */
+/* EigenLMSolver::minimize (src/optimization/solver.cpp:67-85) runs Eigen's   */
+/* LevenbergMarquardt of the NonLinearOptimization module (Eigen 3.4,         */
+/* cmake/Dependencies.cmake:1; the dependency is not vendored). That class    */
+/* is the MINPACK lmder algorithm published by More (1978), "The Levenberg-   */
+/* Marquardt algorithm: implementation and theory", restated here: QR with    */
+/* column pivoting of J, the lmpar search for the damping parameter that puts */
+/* the scaled step on the trust-region boundary, the actual/predicted         */
+/* reduction ratio with its step-bound update, and the termination tests      */
+/* with Eigen's LevenbergMarquardtSpace status codes (1 relative reduction,   */
+/* 2 relative error, 3 both, 4 cosine, 5 too many evaluations, 6-8 tolerances */
+/* too small; 0 improper input). solver.cpp:73-84 sets maxfev =               */
+/* max(max_iter, 1) * 100 + 100, xtol, ftol, calls minimizeOneStep at most    */
+/* max_iter times and reports 2 when the solver is still running.             */
+/* ------------------------------------------------------------------------ */
+#define OR_EPS 2.220446049250313e-16
+#define OR_DWARF 2.2250738585072014e-308
+
+static double nrm2(const double *a, int64_t n) { return sqrt(dot(a, a, n)); }
+
+/* Householder QR of the m x n column-major a with column pivoting (largest
+ * remaining column norm, first on ties); a is overwritten. R (n x n,
+ * column-major, upper) and qtf = (Q^T f)[0..n) on return; perm[k] = original
+ * column at position k. Rank: |r_kk| > eps n max|r_jj| (ColPivHouseholderQR's
+ * default threshold). */
+static int qr_pivot(int64_t m, int n, double *a, const double *f, int *perm,
+                    double *R, double *qtf) {
+  double *fw = (double *)malloc(sizeof(double) * (size_t)m);
+  memcpy(fw, f, sizeof(double) * (size_t)m);
+  for (int j = 0; j < n; ++j) perm[j] = j;
+  double maxpiv = 0.0;
+  for (int k = 0; k < n; ++k) {
+    int p = k;
+    double best = -1.0;
+    for (int j = k; j < n; ++j) {
+      const double s = dot(a + (size_t)j * m + k, a + (size_t)j * m + k, m - k);
+      if (s > best) {
+        best = s;
+        p = j;
+      }
+    }
+    if (p != k) {
+      for (int64_t i = 0; i < m; ++i) {
+        const double t = a[(size_t)k * m + i];
+        a[(size_t)k * m + i] = a[(size_t)p * m + i];
+        a[(size_t)p * m + i] = t;
+      }
+      const int t = perm[k];
+      perm[k] = perm[p];
+      perm[p] = t;
+    }
+    double *v = a + (size_t)k * m + k; /* the column below the diagonal */
+    const double norm = sqrt(best);
+    if (norm > 0.0) {
+      const double alpha = v[0] >= 0.0 ? -norm : norm;
+      v[0] -= alpha; /* v = x - alpha e1 */
+      const double vtv = dot(v, v, m - k);
+      if (vtv > 0.0) {
+        for (int j = k + 1; j < n; ++j) {
+          double *aj = a + (size_t)j * m + k;
+          const double s = 2.0 * dot(v, aj, m - k) / vtv;
+          for (int64_t i = 0; i < m - k; ++i) aj[i] -= s * v[i];
+        }
+        const double s = 2.0 * dot(v, fw + k, m - k) / vtv;
+        for (int64_t i = 0; i < m - k; ++i) fw[k + i] -= s * v[i];
+      }
+      v[0] = alpha;
+      for (int64_t i = 1; i < m - k; ++i) v[i] = 0.0;
+    }
+    if (fabs(v[0]) > maxpiv) maxpiv = fabs(v[0]);
+  }
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) R[(size_t)j * n + i] = i <= j ? a[(size_t)j * m + i] : 0.0;
+  for (int i = 0; i < n; ++i) qtf[i] = fw[i];
+  free(fw);
+  int rank = 0;
+  for (int k = 0; k < n; ++k) rank += fabs(R[(size_t)k * n + k]) > OR_EPS * n * maxpiv;
+  return rank;
+}
+
+/* least-squares solution of (R P^T; sqrt(par) D) x = (qtf; 0) by a Householder
+ * QR of the stacked 2n x n system (no pivoting): S (n x n, column-major upper,
+ * S^T S = P^T (J^T J + par D^2) P) and z = P^T x on return */
+static void damped_solve(int n, const double *R, const int *perm, const double *diag,
+                         double par, const double *qtf, double *S, double *z) {
+  const int m = 2 * n;
+  double *a = (double *)calloc((size_t)m * n, sizeof(double));
+  double *b = (double *)calloc((size_t)m, sizeof(double));
+  const double sp = sqrt(par);
+  for (int j = 0; j < n; ++j) {
+    for (int i = 0; i <= j; ++i) a[(size_t)j * m + i] = R[(size_t)j * n + i];
+    a[(size_t)j * m + n + j] = sp * diag[perm[j]];
+    b[j] = qtf[j];
+  }
+  for (int k = 0; k < n; ++k) {
+    double *v = a + (size_t)k * m + k;
+    const double norm = nrm2(v, m - k);
+    if (norm > 0.0) {
+      const double alpha = v[0] >= 0.0 ? -norm : norm;
+      v[0] -= alpha;
+      const double vtv = dot(v, v, m - k);
+      if (vtv > 0.0) {
+        for (int j = k + 1; j < n; ++j) {
+          double *aj = a + (size_t)j * m + k;
+          const double s = 2.0 * dot(v, aj, m - k) / vtv;
+          for (int i = 0; i < m - k; ++i) aj[i] -= s * v[i];
+        }
+        const double s = 2.0 * dot(v, b + k, m - k) / vtv;
+        for (int i = 0; i < m - k; ++i) b[k + i] -= s * v[i];
+      }
+      v[0] = alpha;
+    }
+  }
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) S[(size_t)j * n + i] = i <= j ? a[(size_t)j * m + i] : 0.0;
+  for (int i = n - 1; i >= 0; --i) {
+    double t = b[i];
+    for (int j = i + 1; j < n; ++j) t -= S[(size_t)j * n + i] * z[j];
+    z[i] = t / S[(size_t)i * n + i];
+  }
+  free(a);
+  free(b);
+}
+
+/* w := U^-T w for an upper-triangular column-major U (forward substitution) */
+static void solve_upper_t(int n, const double *U, double *w) {
+  for (int j = 0; j < n; ++j) {
+    double t = w[j];
+    for (int i = 0; i < j; ++i) t -= U[(size_t)j * n + i] * w[i];
+    w[j] = t / U[(size_t)j * n + j];
+  }
+}
+
+/* lmpar (More 1978, sections 5-7): par >= 0 and x with (J^T J + par D^2) x =
+ * J^T f such that | |D x| - delta | <= 0.1 delta, or par = 0 when the
+ * Gauss-Newton step (minimum-norm over the rank of R) is within 1.1 delta.
+ * Returns the number of damped solves. */
+static int lmpar_qr(int n, const double *R, const int *perm, int rank, const double *diag,
+                    const double *qtf, double delta, double *par, double *x) {
+  double *z = (double *)calloc((size_t)n, sizeof(double));
+  double *w = (double *)malloc(sizeof(double) * (size_t)n);
+  double *S = (double *)malloc(sizeof(double) * (size_t)n * n);
+  /* Gauss-Newton direction over the leading rank columns */
+  for (int i = rank - 1; i >= 0; --i) {
+    double t = qtf[i];
+    for (int j = i + 1; j < rank; ++j) t -= R[(size_t)j * n + i] * z[j];
+    z[i] = t / R[(size_t)i * n + i];
+  }
+  for (int j = 0; j < n; ++j) x[perm[j]] = z[j];
+  double dxnorm = 0.0;
+  for (int j = 0; j < n; ++j) dxnorm += diag[j] * x[j] * diag[j] * x[j];
+  dxnorm = sqrt(dxnorm);
+  double fp = dxnorm - delta;
+  int solves = 0;
+  if (fp <= 0.1 * delta) {
+    *par = 0.0;
+    goto done;
+  }
+  {
+    /* lower bound from the Newton step at par = 0 (full rank only) */
+    double parl = 0.0;
+    if (rank == n) {
+      for (int j = 0; j < n; ++j) w[j] = diag[perm[j]] * diag[perm[j]] * z[j] / dxnorm;
+      solve_upper_t(n, R, w);
+      const double t = nrm2(w, n);
+      parl = fp / delta / t / t;
+    }
+    /* upper bound |D^-1 J^T f| / delta */
+    for (int j = 0; j < n; ++j) {
+      double s = 0.0;
+      for (int i = 0; i <= j; ++i) s += R[(size_t)j * n + i] * qtf[i];
+      w[j] = s / diag[perm[j]];
+    }
+    const double gnorm = nrm2(w, n);
+    double paru = gnorm / delta;
+    if (paru == 0.0) paru = OR_DWARF / (delta < 0.1 ? delta : 0.1);
+    double p = *par;
+    if (p < parl) p = parl;
+    if (p > paru) p = paru;
+    if (p == 0.0) p = gnorm / dxnorm;
+    for (int it = 1;; ++it) {
+      if (p == 0.0) p = OR_DWARF > 0.001 * paru ? OR_DWARF : 0.001 * paru;
+      damped_solve(n, R, perm, diag, p, qtf, S, z);
+      ++solves;
+      for (int j = 0; j < n; ++j) x[perm[j]] = z[j];
+      dxnorm = 0.0;
+      for (int j = 0; j < n; ++j) dxnorm += diag[j] * x[j] * diag[j] * x[j];
+      dxnorm = sqrt(dxnorm);
+      const double prev = fp;
+      fp = dxnorm - delta;
+      if (fabs(fp) <= 0.1 * delta || (parl == 0.0 && fp <= prev && prev < 0.0) || it == 10) break;
+      /* Newton correction */
+      for (int j = 0; j < n; ++j) w[j] = diag[perm[j]] * diag[perm[j]] * z[j] / dxnorm;
+      solve_upper_t(n, S, w);
+      const double t = nrm2(w, n);
+      const double parc = fp / delta / t / t;
+      if (fp > 0.0 && p > parl) parl = p;
+      if (fp < 0.0 && p < paru) paru = p;
+      p = p + parc > parl ? p + parc : parl;
+    }
+    *par = p;
+  }
+done:
+  free(z);
+  free(w);
+  free(S);
+  return solves;
+}
+
+int or_lmpar(int n, const double *R, const int32_t *perm, int rank, const double *diag,
+             const double *qtf, double delta, double *par, double *x) {
+  int *pm = (int *)malloc(sizeof(int) * (size_t)n);
+  for (int j = 0; j < n; ++j) pm[j] = perm[j];
+  const int s = lmpar_qr(n, R, pm, rank, diag, qtf, delta, par, x);
+  free(pm);
+  return s;
+}
+
+int or_qr_pivot(int64_t m, int n, const double *J, const double *f, int32_t *perm,
+                double *R, double *qtf) {
+  double *a = (double *)malloc(sizeof(double) * (size_t)m * n);
+  for (int64_t i = 0; i < m; ++i)
+    for (int j = 0; j < n; ++j) a[(size_t)j * m + i] = J[i * n + j];
+  int *pm = (int *)malloc(sizeof(int) * (size_t)n);
+  const int rank = qr_pivot(m, n, a, f, pm, R, qtf);
+  for (int j = 0; j < n; ++j) perm[j] = pm[j];
+  free(pm);
+  free(a);
+  return rank;
+}
+
+int or_lmder(int64_t m, int n, or_fcn fcn, or_jac jac, void *user, double *x,
+             const or_lm_opts *o, int32_t *counts, double *cost_hist) {
+  const int maxfev = o->maxfev > 0 ? o->maxfev : (o->max_iter > 1 ? o->max_iter : 1) * 100 + 100;
+  if (counts) counts[0] = counts[1] = counts[2] = 0;
+  if (n <= 0 || m < n || o->ftol < 0.0 || o->xtol < 0.0 || o->gtol < 0.0 || maxfev <= 0 ||
+      o->factor <= 0.0)
+    return 0;
+  double *f = (double *)malloc(sizeof(double) * (size_t)m);
+  double *fn = (double *)malloc(sizeof(double) * (size_t)m);
+  double *J = (double *)malloc(sizeof(double) * (size_t)m * n);
+  double *a = (double *)malloc(sizeof(double) * (size_t)m * n);
+  double *R = (double *)malloc(sizeof(double) * (size_t)n * n);
+  double *qtf = (double *)malloc(sizeof(double) * (size_t)n);
+  double *diag = (double *)malloc(sizeof(double) * (size_t)n);
+  double *cn = (double *)malloc(sizeof(double) * (size_t)n);
+  double *step = (double *)malloc(sizeof(double) * (size_t)n);
+  double *xt = (double *)malloc(sizeof(double) * (size_t)n);
+  double *t3 = (double *)malloc(sizeof(double) * (size_t)n);
+  int *perm = (int *)malloc(sizeof(int) * (size_t)n);
+  fcn(user, x, f);
+  int nfev = 1, njev = 0, iter = 1, steps = 0;
+  double fnorm = nrm2(f, m), par = 0.0, delta = 0.0, xnorm = 0.0;
+  const int RUNNING = -1;
+  int status = RUNNING;
+  for (int call = 0; call < o->max_iter && status == RUNNING; ++call) {
+    /* minimizeOneStep */
+    jac(user, x, J);
+    ++njev;
+    for (int j = 0; j < n; ++j) {
+      double s = 0.0;
+      for (int64_t i = 0; i < m; ++i) {
+        a[(size_t)j * m + i] = J[i * n + j];
+        s += J[i * n + j] * J[i * n + j];
+      }
+      cn[j] = sqrt(s);
+    }
+    const int rank = qr_pivot(m, n, a, f, perm, R, qtf);
+    if (iter == 1) {
+      for (int j = 0; j < n; ++j) diag[j] = cn[j] != 0.0 ? cn[j] : 1.0;
+      xnorm = 0.0;
+      for (int j = 0; j < n; ++j) xnorm += diag[j] * x[j] * diag[j] * x[j];
+      xnorm = sqrt(xnorm);
+      delta = o->factor * xnorm;
+      if (delta == 0.0) delta = o->factor;
+    }
+    /* scaled gradient cosine */
+    double gnorm = 0.0;
+    if (fnorm != 0.0)
+      for (int j = 0; j < n; ++j) {
+        const int l = perm[j];
+        if (cn[l] == 0.0) continue;
+        double s = 0.0;
+        for (int i = 0; i <= j; ++i) s += R[(size_t)j * n + i] * (qtf[i] / fnorm);
+        const double v = fabs(s / cn[l]);
+        if (v > gnorm) gnorm = v;
+      }
+    if (gnorm <= o->gtol) {
+      status = 4;
+      break;
+    }
+    for (int j = 0; j < n; ++j)
+      if (cn[j] > diag[j]) diag[j] = cn[j];
+    double ratio = 0.0;
+    do {
+      lmpar_qr(n, R, perm, rank, diag, qtf, delta, &par, step);
+      for (int j = 0; j < n; ++j) {
+        step[j] = -step[j];
+        xt[j] = x[j] + step[j];
+      }
+      double pnorm = 0.0;
+      for (int j = 0; j < n; ++j) pnorm += diag[j] * step[j] * diag[j] * step[j];
+      pnorm = sqrt(pnorm);
+      if (iter == 1 && pnorm < delta) delta = pnorm;
+      fcn(user, xt, fn);
+      ++nfev;
+      const double fnorm1 = nrm2(fn, m);
+      double actred = -1.0;
+      if (0.1 * fnorm1 < fnorm) actred = 1.0 - (fnorm1 / fnorm) * (fnorm1 / fnorm);
+      /* |J p| = |R P^T p| */
+      for (int i = 0; i < n; ++i) {
+        double s = 0.0;
+        for (int j = i; j < n; ++j) s += R[(size_t)j * n + i] * step[perm[j]];
+        t3[i] = s;
+      }
+      const double temp1 = nrm2(t3, n) / fnorm, temp2 = sqrt(par) * pnorm / fnorm;
+      const double prered = temp1 * temp1 + temp2 * temp2 / 0.5;
+      const double dirder = -(temp1 * temp1 + temp2 * temp2);
+      ratio = prered != 0.0 ? actred / prered : 0.0;
+      if (ratio <= 0.25) {
+        double temp = actred >= 0.0 ? 0.5 : 0.5 * dirder / (dirder + 0.5 * actred);
+        if (0.1 * fnorm1 >= fnorm || temp < 0.1) temp = 0.1;
+        delta = temp * (delta < pnorm / 0.1 ? delta : pnorm / 0.1);
+        par /= temp;
+      } else if (par == 0.0 || ratio >= 0.75) {
+        delta = pnorm / 0.5;
+        par = 0.5 * par;
+      }
+      if (ratio >= 1e-4) {
+        memcpy(x, xt, sizeof(double) * (size_t)n);
+        memcpy(f, fn, sizeof(double) * (size_t)m);
+        xnorm = 0.0;
+        for (int j = 0; j < n; ++j) xnorm += diag[j] * x[j] * diag[j] * x[j];
+        xnorm = sqrt(xnorm);
+        fnorm = fnorm1;
+        ++iter;
+        ++steps;
+      }
+      const int ftol_ok = fabs(actred) <= o->ftol && prered <= o->ftol && 0.5 * ratio <= 1.0;
+      if (ftol_ok && delta <= o->xtol * xnorm) { status = 3; break; }
+      if (ftol_ok) { status = 1; break; }
+      if (delta <= o->xtol * xnorm) { status = 2; break; }
+      if (nfev >= maxfev) { status = 5; break; }
+      if (fabs(actred) <= OR_EPS && prered <= OR_EPS && 0.5 * ratio <= 1.0) { status = 6; break; }
+      if (delta <= OR_EPS * xnorm) { status = 7; break; }
+      if (gnorm <= OR_EPS) { status = 8; break; }
+    } while (ratio < 1e-4);
+    if (cost_hist) cost_hist[call] = fnorm * fnorm;
+  }
+  if (counts) {
+    counts[0] = nfev;
+    counts[1] = njev;
+    counts[2] = steps;
+  }
+  free(f); free(fn); free(J); free(a); free(R); free(qtf); free(diag); free(cn);
+  free(step); free(xt); free(t3); free(perm);
+  return status == RUNNING ? 2 : status;
+}
+
+/* force fitting through or_lmder: residual and analytic Jacobian of the
+ * context */
+static int ctx_fcn(void *u, const double *x, double *f) {
+  or_residual((or_ctx *)u, x, f);
+  return 0;
+}
+static int ctx_jac(void *u, const double *x, double *J) {
+  return or_jacobian_analytic((or_ctx *)u, x, J);
+}
+int or_lm_minpack(or_ctx *c, double *x, const or_lm_opts *o, int32_t *counts,
+                  double *cost_hist) {
+  const int rc = or_lmder(or_residual_count(c), or_param_count(c), ctx_fcn, ctx_jac, c, x, o,
+                          counts, cost_hist);
+  or_scatter_params(c, x);
+  return rc;
 }
 
 /* ------------------------------------------------------------------------ */

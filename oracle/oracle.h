@@ -65,6 +65,10 @@ void or_gather_params(const or_ctx *c, double *x);
 void or_scatter_params(or_ctx *c, const double *x);
 double or_max_cutoff(const or_ctx *c);
 void or_set_weights(or_ctx *c, double energy_weight, double stress_weight);
+/* curvature (smoothness) rows appended after the configuration rows when
+ * the weight is > 0 (0 rows for SW) */
+void or_set_smooth_weight(or_ctx *c, double weight);
+int64_t or_smooth_count(const or_ctx *c);
 int64_t or_residual_count(const or_ctx *c);
 void or_row_offsets(const or_ctx *c, int64_t *row_off);
 
@@ -130,6 +134,29 @@ double or_mlp_loss_grad(int n, int d, int h, const double *X, const double *t,
  * config-parallel on the current thread count. Returns seconds. */
 double or_time_fd_jacobian(or_ctx *c, const double *x, int ncols, double *J);
 double or_time_residual(or_ctx *c, const double *x, int reps, double *F);
+
+/* Levenberg-Marquardt with the reference's stopping semantics
+ * (EigenLMSolver, solver.cpp:67-85): MINPACK lmder (More 1978) with QR and
+ * column pivoting. Returns the Eigen status (0 improper input, 1..8, and 2
+ * when still running after max_iter calls of one step); counts (may be NULL)
+ * receives {nfev, njev, accepted steps}; cost_hist[max_iter] (may be NULL)
+ * the cost |f|^2 after each call. */
+typedef int (*or_fcn)(void *user, const double *x, double *f);
+typedef int (*or_jac)(void *user, const double *x, double *J); /* m x n row-major */
+typedef struct {
+  int32_t max_iter, maxfev; /* maxfev <= 0: max(max_iter, 1) * 100 + 100 */
+  double xtol, ftol, gtol, factor;
+} or_lm_opts;
+int or_lmder(int64_t m, int n, or_fcn fcn, or_jac jac, void *user, double *x,
+             const or_lm_opts *o, int32_t *counts, double *cost_hist);
+int or_lm_minpack(or_ctx *c, double *x, const or_lm_opts *o, int32_t *counts,
+                  double *cost_hist);
+/* the pieces, for tests: pivoted QR (returns the rank; R column-major n x n)
+ * and lmpar (returns the number of damped solves) */
+int or_qr_pivot(int64_t m, int n, const double *J, const double *f, int32_t *perm,
+                double *R, double *qtf);
+int or_lmpar(int n, const double *R, const int32_t *perm, int rank, const double *diag,
+             const double *qtf, double delta, double *par, double *x);
 
 #ifdef __cplusplus
 }

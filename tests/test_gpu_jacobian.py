@@ -154,3 +154,119 @@ def test_lm_adp(wl):
     print(f"ADP LM: oracle {ho[0]:.6e} -> {ho[-1]:.6e}; gpu {hg[0]:.6e} -> {hg[-1]:.6e}", flush=True)
     assert acc > 0 and np.all(np.diff(hg) <= 0)
     assert abs(hg[-1] - ho[-1]) <= 1e-3 * ho[-1]
+
+
+@pytest.mark.parametrize("name", ["eam3s", "adp2all"])
+def test_smoothness_rows(wl, name):
+    """Curvature rows (smooth_weight > 0): residual rows, constant Jacobian
+    rows, the batched population cost and LM, each against the oracle."""
+    w = wl[name]
+    sw = 0.05
+    o = oracle.Oracle(w.dataset, w.model)
+    o.prepare()
+    o.set_smooth_weight(sw)
+    g = fa.GpuCalculator(w.dataset, w.model, smooth_weight=sw)
+    g.prepare()
+    print(f"smooth {name}: rows {o.smooth_count}", flush=True)
+    assert g.residual_count == o.residual_count
+    R0 = int(g.row_offsets()[-1])
+    assert o.smooth_count > 0 and g.residual_count - R0 == o.smooth_count
+    x = o.gather()
+    rng = np.random.default_rng(5)
+    x1 = x * (1.0 + 0.01 * rng.standard_normal(x.shape))
+    Fo, (Fg, cg) = o.residual(x1), g.residual(x1, with_cost=True)
+    # curvature rows are FP64 on both sides, same operation order
+    assert np.allclose(Fg[R0:], Fo[R0:], rtol=1e-13, atol=1e-15)
+    assert abs(cg - Fo @ Fo) <= 1e-6 * (Fo @ Fo)
+    Jo, Jg = o.jacobian(x1), g.jacobian(x1).astype(np.float64)
+    assert np.array_equal(Jg[R0:], Jo[R0:].astype(np.float32).astype(np.float64))
+    assert metric(Jg, Jo) <= 2e-4
+    if name.startswith("eam"):
+        X = np.stack([x, x1, x * 0.99])
+        co, cp = o.population_cost(X), g.population_cost(X)
+        print(f"population {cp} oracle {co}", flush=True)
+        assert np.allclose(cp, co, rtol=1e-6)
+    xo, ho, _ = o.lm(x, 4)
+    xg, hg, acc = g.lm(x, 4)
+    print(f"smooth LM: oracle {ho} gpu {hg}", flush=True)
+    assert acc > 0 and np.all(np.diff(hg) <= 0)
+    assert abs(hg[-1] - ho[-1]) <= 1e-3 * ho[-1]
+
+
+def test_lm_solve_reference_semantics():
+    """fs_lm_solve (EigenLMSolver semantics on the GPU normal equations)
+    against the oracle's MINPACK restatement (pivoted QR of the FP64 J) on
+    the single-element workload, FP64 residuals. With the pair knots free
+    (a well-conditioned fit): same status, evaluation counts within a few,
+    final cost within 1e-6. With every table free the EAM gauge freedom makes
+    J^T J nearly singular, and the FP32-J and FP64-J runs stop at different
+    points of the flat valley: both must still end at a comparable cost."""
+    w = dsm.config1(n_configs=3)
+    o = oracle.Oracle(w.dataset, w.model)
+    o.prepare()
+    x0 = o.gather()
+    xo, so, co, ho = o.lm_minpack(x0, max_iter=100)
+    g = fa.GpuCalculator(w.dataset, w.model, precision="fp64")
+    g.prepare()
+    xg, sg, cg, hg = g.lm_solve(x0, max_iter=100)
+    print(f"all tables: oracle status {so} counts {co} cost {o.cost(xo):.10e}; "
+          f"gpu status {sg} counts {cg} cost {g.cost(xg):.10e}", flush=True)
+    assert 1 <= sg <= 3 and 1 <= so <= 3
+    assert g.cost(xg) <= 1.02 * o.cost(xo)
+    for t in w.model.tables[1:]:
+        t.fixed[:] = True
+    o = oracle.Oracle(w.dataset, w.model)
+    o.prepare()
+    x0 = o.gather()
+    xo, so, co, ho = o.lm_minpack(x0, max_iter=100)
+    g = fa.GpuCalculator(w.dataset, w.model, precision="fp64")
+    g.prepare()
+    xg, sg, cg, hg = g.lm_solve(x0, max_iter=100)
+    print(f"oracle status {so} counts {co} cost {ho[co[1] - 1]:.10e}; "
+          f"gpu status {sg} counts {cg} cost {hg[cg[1] - 1]:.10e}", flush=True)
+    assert sg == so and 1 <= sg <= 3
+    assert abs(cg[0] - co[0]) <= 3 and abs(cg[1] - co[1]) <= 3
+    assert abs(g.cost(xg) - o.cost(xo)) <= 1e-6 * o.cost(xo)
+    assert abs(g.cost(xg) - hg[cg[1] - 1]) <= 1e-9 * g.cost(xg)
+
+
+def test_lm_solve_status_rules():
+    # pair knots free; the first calls follow the oracle's costs to the
+    # FP32-Jacobian level (the host algebra alone matches the oracle's QR
+    # form to rounding with FP64 Jacobians, tests/test_lm_host.py)
+    w = dsm.config1(n_configs=3)
+    for t in w.model.tables[1:]:
+        t.fixed[:] = True
+    o = oracle.Oracle(w.dataset, w.model)
+    o.prepare()
+    x0 = o.gather()
+    g = fa.GpuCalculator(w.dataset, w.model, precision="fp64")
+    g.prepare()
+    # still running after max_iter calls: 2 (solver.cpp:84); the first calls
+    # follow the oracle's costs
+    xo, so, co, ho = o.lm_minpack(x0, max_iter=3)
+    xg, sg, cg, hg = g.lm_solve(x0, max_iter=3)
+    print(f"3 calls: oracle {so} {co} {ho}; gpu {sg} {cg} {hg}", flush=True)
+    assert so == sg == 2 and cg[1] == co[1] == 3
+    assert np.allclose(hg, ho, rtol=1e-4)
+    # improper input: 0, nothing evaluated, x untouched
+    for bad in (dict(xtol=-1.0), dict(ftol=-1.0), dict(gtol=-1.0), dict(factor=0.0)):
+        xb, sb, cb, _ = g.lm_solve(x0, **bad)
+        assert sb == 0 and cb == (0, 0, 0) and np.array_equal(xb, x0)
+    # an evaluation budget: 5
+    xg, sg, cg, _ = g.lm_solve(x0, max_iter=50, maxfev=3)
+    assert sg == 5 and cg[0] == 3
+
+
+def test_lm_solve_stillinger_weber(wl):
+    w = wl["sw"]
+    o = oracle.Oracle(w.dataset, w.model)
+    o.prepare()
+    x0 = o.gather() * 1.02
+    xo, so, co, ho = o.lm_minpack(x0, max_iter=30)
+    g = fa.GpuCalculator(w.dataset, w.model)
+    g.prepare()
+    xg, sg, cg, hg = g.lm_solve(x0, max_iter=30)
+    print(f"SW: oracle {so} {co} {o.cost(xo):.8e}; gpu {sg} {cg} {g.cost(xg):.8e}", flush=True)
+    assert 1 <= sg <= 3 or sg == so
+    assert g.cost(xg) <= o.cost(xo) * (1 + 1e-3)

@@ -1,0 +1,249 @@
+"""The oracle's Levenberg-Marquardt with the reference's stopping semantics
+(EigenLMSolver, src/optimization/solver.cpp:67-85: MINPACK lmder, More 1978).
+
+Pinned without the (unvendored) Eigen dependency by:
+  * the pieces' defining properties: the pivoted QR reproduces J P and Q^T f,
+    its diagonal is non-increasing and its rank drops on a repeated column;
+    lmpar returns a step on the trust-region boundary (within 10 %) solving
+    (J^T J + par D^2) x = J^T f, or the Gauss-Newton step when that is inside;
+  * the published minima of the More-Garbow-Hillstrom test problems (ACM TOMS
+    7 (1981) 17-41) from their standard starting points, including rank-
+    deficient linear problems whose minimum is known in closed form;
+  * the reference's own LM unit case (tests/unit/test_optimizer.cpp:76-104): a
+    status >= 1 and a small final residual on a zero-residual fit;
+  * the input checks (status 0) and the still-running rule (status 2 after
+    max_iter calls, solver.cpp:84).
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+
+# ---------------------------------------------------------------------------
+# QR and lmpar
+# ---------------------------------------------------------------------------
+def test_pivoted_qr_properties():
+    rng = np.random.default_rng(0)
+    J = rng.standard_normal((40, 7)) * np.array([1, 10, 0.1, 3, 5, 0.5, 2])
+    f = rng.standard_normal(40)
+    rank, perm, R, qtf = oracle.qr_pivot(J, f)
+    assert rank == 7 and sorted(perm) == list(range(7))
+    JP = J[:, perm]
+    assert np.allclose(R.T @ R, JP.T @ JP, rtol=1e-12, atol=1e-10)
+    assert np.allclose(R.T @ qtf, JP.T @ f, rtol=1e-12, atol=1e-10)
+    d = np.abs(np.diag(R))
+    assert np.all(d[:-1] >= d[1:] * (1 - 1e-12))
+    # first pivot: the largest column
+    assert perm[0] == int(np.argmax(np.linalg.norm(J, axis=0)))
+    # a repeated column loses one rank; a zero column another
+    J2 = np.column_stack([J, J[:, 2], np.zeros(40)])
+    rank2, _, R2, _ = oracle.qr_pivot(J2, f)
+    assert rank2 == 7
+
+
+def test_lmpar_contract():
+    rng = np.random.default_rng(1)
+    for trial in range(30):
+        m, n = 30, 6
+        J = rng.standard_normal((m, n)) * rng.uniform(0.1, 10, n)
+        f = rng.standard_normal(m)
+        diag = rng.uniform(0.5, 5.0, n)
+        rank, perm, R, qtf = oracle.qr_pivot(J, f)
+        gn = np.linalg.lstsq(J, f, rcond=None)[0]
+        dgn = np.linalg.norm(diag * gn)
+        delta = dgn * rng.choice([0.01, 0.3, 0.95, 2.0])
+        par, x, solves = oracle.lmpar(R, perm, rank, diag, qtf, delta)
+        A, g = J.T @ J, J.T @ f
+        if par == 0.0:
+            assert dgn <= 1.1 * delta
+            assert np.allclose(x, gn, rtol=1e-9, atol=1e-12)
+        else:
+            assert abs(np.linalg.norm(diag * x) - delta) <= 0.1 * delta or solves == 10
+            lhs = (A + par * np.diag(diag ** 2)) @ x
+            assert np.allclose(lhs, g, rtol=1e-9, atol=1e-9 * np.abs(g).max())
+
+
+# ---------------------------------------------------------------------------
+# More-Garbow-Hillstrom problems
+# ---------------------------------------------------------------------------
+def fd_jac(fun, x, h=1e-7):
+    f0 = fun(x)
+    J = np.zeros((f0.size, x.size))
+    for j in range(x.size):
+        e = np.zeros_like(x)
+        e[j] = h * max(1.0, abs(x[j]))
+        J[:, j] = (fun(x + e) - fun(x - e)) / (2 * e[j])
+    return J
+
+
+def rosenbrock(x):
+    return np.array([10 * (x[1] - x[0] ** 2), 1 - x[0]])
+
+
+def freudenstein_roth(x):
+    return np.array([-13 + x[0] + ((5 - x[1]) * x[1] - 2) * x[1],
+                     -29 + x[0] + ((x[1] + 1) * x[1] - 14) * x[1]])
+
+
+def powell_singular(x):
+    return np.array([x[0] + 10 * x[1], np.sqrt(5) * (x[2] - x[3]), (x[1] - 2 * x[2]) ** 2,
+                     np.sqrt(10) * (x[0] - x[3]) ** 2])
+
+
+def helical(x):
+    th = np.arctan(x[1] / x[0]) / (2 * np.pi) if x[0] != 0 else 0.25 * np.sign(x[1])
+    if x[0] < 0:
+        th += 0.5
+    return np.array([10 * (x[2] - 10 * th), 10 * (np.hypot(x[0], x[1]) - 1), x[2]])
+
+
+def linear_full_rank(x, m=10):
+    s = 2.0 / m * x.sum()
+    f = np.full(m, -s - 1.0)
+    f[:x.size] += x
+    return f
+
+
+def linear_rank1(x, m=10):
+    j = np.arange(1, x.size + 1)
+    return np.arange(1, m + 1) * (j @ x) - 1.0
+
+
+def linear_rank1_zero(x, m=10):
+    n = x.size
+    j = np.arange(2, n)
+    s = j @ x[1:n - 1]
+    f = (np.arange(1, m + 1) - 1) * s - 1.0
+    f[0] = -1.0
+    f[-1] = -1.0
+    return f
+
+
+BARD_Y = np.array([0.14, 0.18, 0.22, 0.25, 0.29, 0.32, 0.35, 0.39, 0.37, 0.58, 0.73, 0.96, 1.34, 2.10, 4.39])
+
+
+def bard(x):
+    u = np.arange(1, 16.0)
+    v = 16 - u
+    w = np.minimum(u, v)
+    return BARD_Y - (x[0] + u / (v * x[1] + w * x[2]))
+
+
+KO_Y = np.array([0.1957, 0.1947, 0.1735, 0.1600, 0.0844, 0.0627, 0.0456, 0.0342, 0.0323, 0.0235, 0.0246])
+KO_U = np.array([4.0, 2.0, 1.0, 0.5, 0.25, 0.167, 0.125, 0.1, 0.0833, 0.0714, 0.0625])
+
+
+def kowalik_osborne(x):
+    u = KO_U
+    return KO_Y - x[0] * (u * u + u * x[1]) / (u * u + u * x[2] + x[3])
+
+
+def box3d(x):
+    t = 0.1 * np.arange(1, 11)
+    return np.exp(-t * x[0]) - np.exp(-t * x[1]) - x[2] * (np.exp(-t) - np.exp(-10 * t))
+
+
+def jennrich_sampson(x):
+    i = np.arange(1, 11.0)
+    return 2 + 2 * i - (np.exp(i * x[0]) + np.exp(i * x[1]))
+
+
+def brown_dennis(x):
+    t = np.arange(1, 21) / 5.0
+    return (x[0] + t * x[1] - np.exp(t)) ** 2 + (x[2] + x[3] * np.sin(t) - np.cos(t)) ** 2
+
+
+OS1_Y = np.array([0.844, 0.908, 0.932, 0.936, 0.925, 0.908, 0.881, 0.850, 0.818, 0.784, 0.751, 0.718,
+                  0.685, 0.658, 0.628, 0.603, 0.580, 0.558, 0.538, 0.522, 0.506, 0.490, 0.478, 0.467,
+                  0.457, 0.448, 0.438, 0.431, 0.424, 0.420, 0.414, 0.411, 0.406])
+
+
+def osborne1(x):
+    t = 10.0 * np.arange(33)
+    return OS1_Y - (x[0] + x[1] * np.exp(-t * x[3]) + x[2] * np.exp(-t * x[4]))
+
+
+# (name, f, m, x0, published minimum of |f|^2)
+MGH = [
+    ("rosenbrock", rosenbrock, 2, [-1.2, 1.0], 0.0),
+    ("freudenstein_roth", freudenstein_roth, 2, [0.5, -2.0], 48.9842536792400),
+    ("powell_singular", powell_singular, 4, [3.0, -1.0, 0.0, 1.0], 0.0),
+    ("helical_valley", helical, 3, [-1.0, 0.0, 0.0], 0.0),
+    ("linear_full_rank", linear_full_rank, 10, [1.0] * 5, 5.0),
+    ("linear_rank1", linear_rank1, 10, [1.0] * 5, 10 * 9 / (2 * 21)),
+    ("linear_rank1_zero", linear_rank1_zero, 10, [1.0] * 5, (100 + 30 - 6) / (2 * 17)),
+    ("bard", bard, 15, [1.0, 1.0, 1.0], 8.21487730657e-3),
+    ("kowalik_osborne", kowalik_osborne, 11, [0.25, 0.39, 0.415, 0.39], 3.07505603849e-4),
+    ("box3d", box3d, 10, [0.0, 10.0, 20.0], 0.0),
+    ("jennrich_sampson", jennrich_sampson, 10, [0.3, 0.4], 124.362182),
+    ("brown_dennis", brown_dennis, 20, [25.0, 5.0, -5.0, -1.0], 85822.2016),
+    ("osborne1", osborne1, 33, [0.5, 1.5, -1.0, 0.01, 0.02], 5.46489469e-5),
+]
+
+
+@pytest.mark.parametrize("name,fun,m,x0,fmin", MGH, ids=[p[0] for p in MGH])
+def test_mgh_minima(name, fun, m, x0, fmin):
+    x, st, (nfev, njev, acc), hist = oracle.lmder(fun, lambda v: fd_jac(fun, v), np.array(x0), m,
+                                                 max_iter=400, xtol=1e-10, ftol=1e-10)
+    cost = float(fun(x) @ fun(x))
+    print(f"{name}: status {st} nfev {nfev} njev {njev} cost {cost:.10e}")
+    assert 1 <= st <= 8
+    if fmin == 0.0:
+        assert cost <= 1e-10
+    else:
+        assert cost == pytest.approx(fmin, rel=2e-6)
+    # the history is the accepted cost after every call of one step
+    called = np.count_nonzero(hist) if fmin > 0 else njev
+    assert np.all(np.diff(hist[:called]) <= 1e-15 * max(hist[0], 1.0))
+
+
+def test_linear_problem_converges_in_one_gauss_newton_step():
+    # linear, full rank, from far away: delta = factor |D x0| admits the GN step
+    x, st, (nfev, njev, acc), hist = oracle.lmder(linear_full_rank, lambda v: fd_jac(linear_full_rank, v),
+                                                 np.full(5, 10.0), 10, max_iter=50)
+    assert np.allclose(x, -1.0, atol=1e-8)
+    assert acc >= 1 and hist[0] == pytest.approx(5.0, rel=1e-10)
+
+
+def test_status_rules():
+    f = rosenbrock
+    J = lambda v: fd_jac(f, v)
+    x0 = np.array([-1.2, 1.0])
+    # improper input: status 0, x untouched, nothing evaluated
+    for bad in (dict(xtol=-1.0), dict(ftol=-1.0), dict(gtol=-1.0), dict(factor=0.0)):
+        x, st, cnt, _ = oracle.lmder(f, J, x0, 2, **bad)
+        assert st == 0 and np.array_equal(x, x0) and cnt == (0, 0, 0)
+    x, st, cnt, _ = oracle.lmder(f, J, np.zeros(3), 2)  # m < n
+    assert st == 0
+    # still running after max_iter calls: 2 (solver.cpp:84)
+    x, st, (nfev, njev, acc), hist = oracle.lmder(f, J, x0, 2, max_iter=3)
+    assert st == 2 and njev == 3
+    # zero residual at the start: the gradient cosine test stops at once (4)
+    x, st, (nfev, njev, acc), _ = oracle.lmder(f, J, np.array([1.0, 1.0]), 2)
+    assert st == 4 and nfev == 1 and njev == 1
+    # an evaluation budget: 5
+    x, st, (nfev, njev, acc), _ = oracle.lmder(f, J, x0, 2, max_iter=100, maxfev=4)
+    assert st == 5 and nfev == 4
+
+
+def test_reference_unit_case_lj_dimer_shape():
+    """tests/unit/test_optimizer.cpp:76-104 fits (eps, sigma) of a LJ dimer from
+    eps = 0.5 with EigenLMSolver{2000, 1e-10, 1e-10} and expects a status >= 1
+    and a residual below 1e-6: the same fit through or_lmder."""
+    sigma_t, eps_t, r = 2.0, 1.0, 5.0
+
+    def lj(e, s, rr):
+        sr6 = (s / rr) ** 6
+        return 4 * e * (sr6 * sr6 - sr6), 24 * e * (2 * sr6 * sr6 - sr6) / rr
+
+    E_ref, F_ref = lj(eps_t, sigma_t, r)
+
+    def fun(p):
+        E, F = lj(p[0], p[1], r)
+        return np.array([F - F_ref, -(F - F_ref), E - E_ref])
+
+    x, st, cnt, _ = oracle.lmder(fun, lambda v: fd_jac(fun, v), np.array([0.5, sigma_t]), 3, max_iter=2000,
+                                 xtol=1e-10, ftol=1e-10)
+    assert st >= 1
+    assert float(fun(x) @ fun(x)) < 1e-6
