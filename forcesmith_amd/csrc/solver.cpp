@@ -200,9 +200,22 @@ int fs_population_impl(Context &c, const double *X, int np, double *cost) {
   return FS_OK;
 }
 
+// BoostDESolver's box (src/optimization/solver.cpp:118-125): a finite bound
+// is kept, an infinite one becomes v -+ max(2 |v|, 5e-4) around the initial
+// value v; NP <= 0 means NP_factor * D = 15 D (solver.hpp:78-81)
 int fs_de_impl(Context &c, const fs_de_opts *o, const double *lo, const double *hi, double *x,
                double *best, double *hist, int *gens) {
-  return de_run(c, o, lo, hi, x, best, hist, gens);
+  const int D = c.D;
+  std::vector<double> lb(D), ub(D);
+  for (int k = 0; k < D; ++k) {
+    const double v = x[k], half = std::max(2.0 * std::fabs(v), 5e-4);
+    lb[k] = std::isfinite(lo[k]) ? lo[k] : v - half;
+    ub[k] = std::isfinite(hi[k]) ? hi[k] : v + half;
+    FS_REQUIRE(lb[k] <= ub[k], FS_ERR_ARG, "DE box has lower > upper");
+  }
+  fs_de_opts oo = *o;
+  if (oo.np <= 0) oo.np = 15 * D;
+  return de_run(c, &oo, lb.data(), ub.data(), x, best, hist, gens);
 }
 
 // what: 0 = residual (density + force + finalize), 1 = Jacobian (all its

@@ -193,8 +193,12 @@ typedef struct fs_de_opts {
   int32_t reserved[3];
 } fs_de_opts;
 /* Differential evolution with the whole population evaluated per batched
- * launch; replaces BoostDESolver::minimize (solver.cpp:113-155). lo/hi: box;
- * x: initial guess in, best member out; hist[generations] best cost. */
+ * launch; replaces BoostDESolver::minimize (solver.cpp:113-155). lo/hi: the
+ * bounds; an infinite bound becomes x -+ max(2 |x|, 5e-4) around the initial
+ * value, as BoostDESolver derives its box (:118-125; fs_gather_bounds gives
+ * +-inf for knots). np <= 0: 15 * D (NP_factor, solver.hpp:78-81). x: initial
+ * guess in (clamped into the box as member 0, :138-141), best member out;
+ * hist[generations] best cost. */
 int fs_de_minimize(fs_ctx *ctx, const fs_de_opts *opts, const double *lo,
                    const double *hi, double *x, double *best_cost,
                    double *hist, int32_t *generations_run);

@@ -153,3 +153,28 @@ def test_de_target_stops_early(wl):
     _, bt, ht = g.de(x0, lo, hi, 40, 40, seed=9, target_cost=target)
     assert len(ht) == int(np.argmax(ha <= target)) + 1
     assert bt <= target
+
+
+def test_de_reference_box_and_population_size(wl):
+    """Infinite bounds take BoostDESolver's box v -+ max(2|v|, 5e-4)
+    (solver.cpp:118-125) and np <= 0 takes 15 D (solver.hpp:78-81): the same
+    run as with that box and size given explicitly."""
+    w = wl["eam1"]
+    g = fa.GpuCalculator(w.dataset, w.model)
+    g.prepare()
+    x0 = g.gather_params()
+    lo, hi = g.gather_bounds()
+    assert np.all(np.isinf(lo)) and np.all(np.isinf(hi))
+    xa, ba, ha = g.de(x0, lo, hi, 0, 3, seed=5)
+    blo, bhi = dsm.de_box(x0)
+    xb, bb, hb = g.de(x0, blo, bhi, 15 * x0.size, 3, seed=5)
+    _print(f"reference box DE best {ba:.10e}")
+    assert np.array_equal(xa, xb) and ba == bb and np.array_equal(ha, hb)
+    # a finite bound is kept as given
+    lo2 = lo.copy()
+    lo2[0] = x0[0] - 1e-3
+    xc, bc, hc = g.de(x0, lo2, hi, 0, 3, seed=5)
+    blo2 = blo.copy()
+    blo2[0] = x0[0] - 1e-3
+    xd, bd, hd = g.de(x0, blo2, bhi, 15 * x0.size, 3, seed=5)
+    assert np.array_equal(xc, xd) and bc == bd
