@@ -264,9 +264,14 @@ __device__ double emb_window(const JacArgs &a, const TableDesc *sT, int64_t i) {
 
 // ---------------------------------------------------------------------------
 // H_i = d rho_i / d y over the density columns, from staged band coefficients
+// band coefficients are read as [m - k + 2][q]: lanes of one slot often read
+// the same pair q at different offsets d, so rows are 65 floats apart to put
+// them in different LDS banks
+constexpr int kBandPitch = 65;
+
 struct DensStage {
   int32_t k[64];
-  float c[6][64];
+  float c[6][kBandPitch];
 };
 
 __global__ __launch_bounds__(256) void k_jac_dens(JacArgs a, int nknots) {
@@ -357,19 +362,40 @@ __global__ __launch_bounds__(256) void k_jac_dens(JacArgs a, int nknots) {
 // terms of pair (i,j) equal atom j's per-atom terms through
 // Gs_j = sum_i g'_ti(r) r u u. Those pair loops therefore accumulate forces
 // only, and the matching per-atom / F'_i terms carry their virial twice.
-struct JStage {
-  float r[64], u0[64], u1[64], u2[64], dgi[64];
-  int32_t jg[64], kphi[64], kgj[64], kgi[64], kfj[64];
-  float cphi[6][64], ephi[6][64], cgj[6][64], cgi[6][64], cemb[6][64];
+// Two passes over the same tiles, each with the lane slots of its own columns
+// and only the pair staging those columns read, which halves the registers
+// and LDS a wave holds (and so doubles the waves a CU keeps in flight):
+//   JP_PAIR: pair-table columns (the phi band of every pair);
+//   JP_DE:   density and embedding columns (the g_tj, g_ti and F_tj bands,
+//            the density-chain and per-atom terms).
+enum { JP_PAIR = 1, JP_DE = 2 };
+
+struct JStagePair {
+  float r[64], u0[64], u1[64], u2[64];
+  int32_t kphi[64];
+  float cphi[6][kBandPitch], ephi[6][kBandPitch];
 };
+
+struct JStageDE {
+  float r[64], u0[64], u1[64], u2[64], dgi[64];
+  int32_t jg[64], kgj[64], kgi[64], kfj[64];
+  float cgj[6][kBandPitch], cgi[6][kBandPitch], cemb[6][kBandPitch];
+};
+
+template <int PASS> struct JStageOf;
+template <> struct JStageOf<JP_PAIR> { using T = JStagePair; };
+template <> struct JStageOf<JP_DE> { using T = JStageDE; };
 
 struct SlotDesc {
   int32_t tab, m, kind, dc; // as ColDesc; dc = density column or pair-table types
   int32_t col, pad0, pad1, pad2;
 };
 
+// per-atom force accumulator of one column; the column's energy, virial and
+// limit sums go straight into the tile partial t[kJP] (e, xx, yy, zz, xy, xz,
+// yz, limit), which keeps the register count down
 struct ColAcc {
-  float f[3], s[6], e;
+  float f[3];
 };
 
 __device__ inline void add_force(ColAcc &A, float c, float u0, float u1, float u2) {
@@ -378,41 +404,35 @@ __device__ inline void add_force(ColAcc &A, float c, float u0, float u1, float u
   A.f[2] += c * u2;
 }
 
-__device__ inline void add_virial(ColAcc &A, float c, float r, float u0, float u1, float u2) {
+__device__ inline void add_virial(float *t, float c, float r, float u0, float u1, float u2) {
   const float cr = c * r;
-  A.s[0] += cr * u0 * u0;
-  A.s[1] += cr * u1 * u1;
-  A.s[2] += cr * u2 * u2;
-  A.s[3] += cr * u0 * u1;
-  A.s[4] += cr * u0 * u2;
-  A.s[5] += cr * u1 * u2;
+  t[1] += cr * u0 * u0;
+  t[2] += cr * u1 * u1;
+  t[3] += cr * u2 * u2;
+  t[4] += cr * u0 * u1;
+  t[5] += cr * u0 * u2;
+  t[6] += cr * u1 * u2;
 }
 
-__device__ inline void add_pair(ColAcc &A, float c, float r, float u0, float u1, float u2) {
-  A.f[0] += c * u0;
-  A.f[1] += c * u1;
-  A.f[2] += c * u2;
-  const float cr = c * r;
-  A.s[0] += cr * u0 * u0;
-  A.s[1] += cr * u1 * u1;
-  A.s[2] += cr * u2 * u2;
-  A.s[3] += cr * u0 * u1;
-  A.s[4] += cr * u0 * u2;
-  A.s[5] += cr * u1 * u2;
+__device__ inline void add_pair(ColAcc &A, float *t, float c, float r, float u0, float u1, float u2) {
+  add_force(A, c, u0, u1, u2);
+  add_virial(t, c, r, u0, u1, u2);
 }
 
-template <int NS>
+template <int PASS, int NS>
 __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots, const SlotDesc *slots) {
+  using Stage = typename JStageOf<PASS>::T;
+  constexpr bool kPairPass = PASS == JP_PAIR;
   extern __shared__ float4 smem[];
   TableDesc *sT = reinterpret_cast<TableDesc *>(smem);
   float *sS = reinterpret_cast<float *>(smem + a.ntab * int(sizeof(TableDesc) / sizeof(float4)));
-  JStage *sAll = reinterpret_cast<JStage *>(sS + ((nknots * 5 + 3) & ~3));
+  Stage *sAll = reinterpret_cast<Stage *>(sS + ((nknots * 5 + 3) & ~3));
   // the tile-end reduction reuses the staging area of the four waves
-  static_assert(4 * sizeof(JStage) >= sizeof(double) * kJP * 256, "reduction buffer");
+  static_assert(4 * sizeof(Stage) >= sizeof(double) * kJP * 256, "reduction buffer");
   double(*sPart)[256] = reinterpret_cast<double(*)[256]>(sAll);
   stage(a, sT, sS, nknots);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  JStage &P = sAll[wave];
+  Stage &P = sAll[wave];
   int ccol[NS], ctab[NS], cm[NS], ckind[NS], cdc[NS];
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
@@ -443,14 +463,11 @@ __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots, const S
     for (int li = wave; li < na; li += 4) {
       const int64_t i = int64_t(a.tile_a0[tile]) + li;
       const int ti = a.types[i];
-      const float Fpi = a.Fp32[i];
       const int64_t p0 = a.pair_off[i], p1 = a.pair_off[i + 1];
       ColAcc A[NS];
 #pragma unroll
       for (int s = 0; s < NS; ++s) {
         for (int q = 0; q < 3; ++q) A[s].f[q] = 0.0f;
-        for (int q = 0; q < 6; ++q) A[s].s[q] = 0.0f;
-        A[s].e = 0.0f;
       }
       float G[3] = {0, 0, 0}, Gs[6] = {0, 0, 0, 0, 0, 0};
       // partner type of each pair column for this centre type (-1: not ti's table)
@@ -458,7 +475,7 @@ __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots, const S
 #pragma unroll
       for (int s = 0; s < NS; ++s) {
         ptt[s] = -1;
-        if (ckind[s] == CK_PAIR) {
+        if (kPairPass && ckind[s] == CK_PAIR) {
           const int ta = cdc[s] & 0xff, tb = (cdc[s] >> 8) & 0xff;
           ptt[s] = ti == ta ? tb : (ti == tb ? ta : -1);
         }
@@ -473,18 +490,16 @@ __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots, const S
           const bool tiny = (b & kFlagTiny) != 0;
           const float inv_r = tiny ? 0.0f : 1.0f / rr;
           const int tj = pj_type(b);
-          const int32_t jg = int32_t(cbase + pj_local(b));
           ty = tj;
           const float u0 = q.x * inv_r, u1 = q.y * inv_r, u2 = q.z * inv_r;
           P.r[lane] = rr;
           P.u0[lane] = u0;
           P.u1[lane] = u1;
           P.u2[lane] = u2;
-          P.jg[lane] = jg;
-          const float fpj = a.Fp32[jg];
-          float bv[4], bd[4], cc[6], ce[6];
-          // pair table phi_(ti,tj)
-          {
+          float bv[4], bd[4], cc[6];
+          if constexpr (kPairPass) {
+            // pair table phi_(ti,tj)
+            float ce[6];
             const TableDesc T = sT[ord_pair(ti, tj, a.ntypes)];
             const int k = pair_basis(T, rr, bv, bd);
             const float w = ((b & kFlagPhi) && !tiny) ? 1.0f : 0.0f;
@@ -496,62 +511,67 @@ __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots, const S
               P.cphi[d][lane] = cc[d];
               P.ephi[d][lane] = ce[d];
             }
-          }
-          // density of j, weight F'_i; also g'_tj(r) for G_i
-          float dgj = 0.0f;
-          {
-            const TableDesc T = sT[a.npair + tj];
-            const int k = pair_basis(T, rr, bv, bd);
-            const bool on = (b & kFlagGj) && !tiny;
-            band6(sS, T, k, bd, on ? Fpi : 0.0f, cc);
-            P.kgj[lane] = k;
+          } else {
+            const int32_t jg = int32_t(cbase + pj_local(b));
+            P.jg[lane] = jg;
+            const float Fpi = a.Fp32[i];
+            const float fpj = a.Fp32[jg];
+            // density of j, weight F'_i; also g'_tj(r) for G_i
+            float dgj = 0.0f;
+            {
+              const TableDesc T = sT[a.npair + tj];
+              const int k = pair_basis(T, rr, bv, bd);
+              const bool on = (b & kFlagGj) && !tiny;
+              band6(sS, T, k, bd, on ? Fpi : 0.0f, cc);
+              P.kgj[lane] = k;
 #pragma unroll
-            for (int d = 0; d < 6; ++d) P.cgj[d][lane] = cc[d];
-            if (on) {
-              float tt, v;
-              const int kk = spl_interval32(T, rr, tt);
-              spl_eval32(T, a.ivl32, kk, tt, v, dgj);
+              for (int d = 0; d < 6; ++d) P.cgj[d][lane] = cc[d];
+              if (on) {
+                float tt, v;
+                const int kk = spl_interval32(T, rr, tt);
+                spl_eval32(T, a.ivl32, kk, tt, v, dgj);
+              }
             }
-          }
-          // density of i, weight F'_j; g'_ti(r) weights the embedding / H terms
-          float dgi = 0.0f;
-          {
-            const TableDesc T = sT[a.npair + ti];
-            const int k = pair_basis(T, rr, bv, bd);
-            const bool on = (b & kFlagGi) && !tiny;
-            band6(sS, T, k, bd, on ? fpj : 0.0f, cc);
-            P.kgi[lane] = k;
+            // density of i, weight F'_j; g'_ti(r) weights the embedding / H terms
+            float dgi = 0.0f;
+            {
+              const TableDesc T = sT[a.npair + ti];
+              const int k = pair_basis(T, rr, bv, bd);
+              const bool on = (b & kFlagGi) && !tiny;
+              band6(sS, T, k, bd, on ? fpj : 0.0f, cc);
+              P.kgi[lane] = k;
 #pragma unroll
-            for (int d = 0; d < 6; ++d) P.cgi[d][lane] = cc[d];
-            if (on) {
-              float tt, v;
-              const int kk = spl_interval32(T, rr, tt);
-              spl_eval32(T, a.ivl32, kk, tt, v, dgi);
+              for (int d = 0; d < 6; ++d) P.cgi[d][lane] = cc[d];
+              if (on) {
+                float tt, v;
+                const int kk = spl_interval32(T, rr, tt);
+                spl_eval32(T, a.ivl32, kk, tt, v, dgi);
+              }
             }
-          }
-          P.dgi[lane] = dgi;
-          // embedding of j at its density, weight g'_ti(r)
-          {
-            const TableDesc T = sT[a.npair + a.ntypes + tj];
-            const int kf = a.emb_k[jg];
-            const float4 e = reinterpret_cast<const float4 *>(a.emb_d)[jg];
-            const float eb[4] = {e.x, e.y, e.z, e.w};
-            band6(sS, T, kf, eb, dgi, cc);
-            P.kfj[lane] = kf;
+            P.dgi[lane] = dgi;
+            // embedding of j at its density, weight g'_ti(r)
+            {
+              const TableDesc T = sT[a.npair + a.ntypes + tj];
+              const int kf = a.emb_k[jg];
+              const float4 e = reinterpret_cast<const float4 *>(a.emb_d)[jg];
+              const float eb[4] = {e.x, e.y, e.z, e.w};
+              band6(sS, T, kf, eb, dgi, cc);
+              P.kfj[lane] = kf;
 #pragma unroll
-            for (int d = 0; d < 6; ++d) P.cemb[d][lane] = cc[d];
+              for (int d = 0; d < 6; ++d) P.cemb[d][lane] = cc[d];
+            }
+            // G_i = sum_j g'_tj(r) u, virial counterpart sum_j g'_tj r u u
+            G[0] += dgj * u0;
+            G[1] += dgj * u1;
+            G[2] += dgj * u2;
+            const float gr = dgj * rr;
+            Gs[0] += gr * u0 * u0;
+            Gs[1] += gr * u1 * u1;
+            Gs[2] += gr * u2 * u2;
+            Gs[3] += gr * u0 * u1;
+            Gs[4] += gr * u0 * u2;
+            Gs[5] += gr * u1 * u2;
           }
-          // G_i = sum_j g'_tj(r) u, virial counterpart sum_j g'_tj r u u
-          G[0] += dgj * u0;
-          G[1] += dgj * u1;
-          G[2] += dgj * u2;
-          const float gr = dgj * rr;
-          Gs[0] += gr * u0 * u0;
-          Gs[1] += gr * u1 * u1;
-          Gs[2] += gr * u2 * u2;
-          Gs[3] += gr * u0 * u1;
-          Gs[4] += gr * u0 * u2;
-          Gs[5] += gr * u1 * u2;
         }
         int seg[kMaxTypes + 1];
         chunk_segments(ty, a.ntypes, seg);
@@ -562,15 +582,15 @@ __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots, const S
           const int kind = ckind[s];
           if (kind == CK_NONE) continue;
           const int m = cm[s];
-          if (kind == CK_PAIR) {
+          if constexpr (kPairPass) {
             if (ptt[s] < 0) continue;
             const int s1 = seg[ptt[s] + 1];
             const int lo = first_ge(P.kphi, seg[ptt[s]], s1, m - 3);
             const int hi = first_ge(P.kphi, lo, s1, m + 3);
             for (int q = lo; q < hi; ++q) {
               const int d = m - P.kphi[q] + 2;
-              add_pair(A[s], P.cphi[d][q], P.r[q], P.u0[q], P.u1[q], P.u2[q]);
-              A[s].e += P.ephi[d][q];
+              add_pair(A[s], tp[s], P.cphi[d][q], P.r[q], P.u0[q], P.u1[q], P.u2[q]);
+              tp[s][0] += P.ephi[d][q];
             }
           } else if (kind == CK_DENS) {
             const int t = ctab[s] - a.npair;
@@ -582,7 +602,7 @@ __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots, const S
               for (int q = lo; q < hi; ++q) {
                 const float cq = P.cgj[m - P.kgj[q] + 2][q];
                 add_force(A[s], cq, P.u0[q], P.u1[q], P.u2[q]);
-                add_virial(A[s], 2.0f * cq, P.r[q], P.u0[q], P.u1[q], P.u2[q]);
+                add_virial(tp[s], 2.0f * cq, P.r[q], P.u0[q], P.u1[q], P.u2[q]);
               }
             }
             // F'_j g'_ti(r), ti == t, every partner type
@@ -620,44 +640,53 @@ __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots, const S
         }
         wave_sync();
       }
-      for (int q = 0; q < 3; ++q) G[q] = wave_sum(G[q]);
-      for (int q = 0; q < 6; ++q) Gs[q] = wave_sum(Gs[q]);
-      // per-atom terms
-      const bool clamped = a.over[i] != 0.0;
-      const double Fpi64 = a.Fp[i], over = a.over[i];
-      const int kfi = a.emb_k[i];
-      float ebi[4], evi[4];
-      for (int q = 0; q < 4; ++q) {
-        ebi[q] = a.emb_d[4 * i + q];
-        evi[q] = a.emb_v[4 * i + q];
-      }
-      const int temb_i = a.npair + a.ntypes + ti;
       const int64_t row = a.row_off[c] + 3 * (i - cbase);
+      if constexpr (kPairPass) {
 #pragma unroll
-      for (int s = 0; s < NS; ++s) {
-        if (ckind[s] == CK_NONE) continue;
-        float g = 0.0f, e_extra = 0.0f, l_extra = 0.0f;
-        if (ckind[s] == CK_DENS) {
-          g = a.Hs[i * a.Dd + cdc[s]];
-          const double h = a.H[i * a.Dd + cdc[s]];
-          if (clamped) l_extra = float(over * h);
-          else e_extra = float(Fpi64 * h);
-        } else if (ckind[s] == CK_EMB && ctab[s] == temb_i) {
-          const TableDesc D = sT[ctab[s]];
-          g = band(sS, D, kfi, cm[s], ebi);
-          e_extra = band(sS, D, kfi, cm[s], evi);
+        for (int s = 0; s < NS; ++s) {
+          if (ckind[s] == CK_NONE) continue;
+          const int col = ccol[s];
+          for (int q = 0; q < 3; ++q) a.J[(row + q) * a.D + col] = A[s].f[q];
         }
-        for (int q = 0; q < 3; ++q) A[s].f[q] += g * G[q];
-        // twice: the virial of the mirrored pair terms (see the pair loop)
-        for (int q = 0; q < 6; ++q) A[s].s[q] += 2.0f * g * Gs[q];
-        const int col = ccol[s];
-        for (int q = 0; q < 3; ++q) a.J[(row + q) * a.D + col] = A[s].f[q];
-        tp[s][0] += A[s].e + e_extra;
-        for (int q = 0; q < 6; ++q) tp[s][1 + q] += A[s].s[q];
-        tp[s][7] += l_extra;
+      } else {
+        for (int q = 0; q < 3; ++q) G[q] = wave_sum(G[q]);
+        for (int q = 0; q < 6; ++q) Gs[q] = wave_sum(Gs[q]);
+        // per-atom terms
+        const bool clamped = a.over[i] != 0.0;
+        const double Fpi64 = a.Fp[i], over = a.over[i];
+        const int kfi = a.emb_k[i];
+        float ebi[4], evi[4];
+        for (int q = 0; q < 4; ++q) {
+          ebi[q] = a.emb_d[4 * i + q];
+          evi[q] = a.emb_v[4 * i + q];
+        }
+        const int temb_i = a.npair + a.ntypes + ti;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+          if (ckind[s] == CK_NONE) continue;
+          float g = 0.0f, e_extra = 0.0f, l_extra = 0.0f;
+          if (ckind[s] == CK_DENS) {
+            g = a.Hs[i * a.Dd + cdc[s]];
+            const double h = a.H[i * a.Dd + cdc[s]];
+            if (clamped) l_extra = float(over * h);
+            else e_extra = float(Fpi64 * h);
+          } else if (ckind[s] == CK_EMB && ctab[s] == temb_i) {
+            const TableDesc D = sT[ctab[s]];
+            g = band(sS, D, kfi, cm[s], ebi);
+            e_extra = band(sS, D, kfi, cm[s], evi);
+          }
+          for (int q = 0; q < 3; ++q) A[s].f[q] += g * G[q];
+          // twice: the virial of the mirrored pair terms (see the pair loop)
+          for (int q = 0; q < 6; ++q) tp[s][1 + q] += 2.0f * g * Gs[q];
+          const int col = ccol[s];
+          for (int q = 0; q < 3; ++q) a.J[(row + q) * a.D + col] = A[s].f[q];
+          tp[s][0] += e_extra;
+          tp[s][7] += l_extra;
+        }
       }
     }
-    // reduce the four waves' partials in wave order (FP64), write the tile partial
+    // reduce the four waves' partials in wave order (FP64), write the tile
+    // partial of this pass's columns
     __syncthreads(); // every wave is done with its staging area
     for (int w = 0; w < 4; ++w) {
       if (wave == w) {
@@ -671,9 +700,13 @@ __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots, const S
       }
       __syncthreads();
     }
-    for (int k = threadIdx.x; k < kJP * a.D; k += blockDim.x) {
-      const int q = k / a.D, col = k % a.D;
-      a.part[(int64_t(tile) * kJP + q) * a.D + col] = sPart[q][col];
+    if (wave == 0) {
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        if (ckind[s] == CK_NONE) continue;
+        const int col = ccol[s];
+        for (int q = 0; q < kJP; ++q) a.part[(int64_t(tile) * kJP + q) * a.D + col] = sPart[q][col];
+      }
     }
     __syncthreads();
   }
@@ -1520,27 +1553,51 @@ void launch_jacobian(Context &c) {
     a.Fpp = c.Fpp.p;
     const size_t base = size_t(c.ntab) * sizeof(TableDesc) + ((size_t(c.nknots) * 5 + 3) & ~size_t(3)) * 4;
     const size_t lds_dens = base + 4 * sizeof(DensStage);
-    const size_t lds_main = base + 4 * sizeof(JStage);
-    FS_REQUIRE(lds_main <= 64 * 1024, FS_ERR_UNSUPPORTED, "Jacobian LDS budget exceeded");
-    // lane slots: pair, density and embedding columns each start a new slot
-    std::vector<SlotDesc> slots;
-    for (int kind : {int(CK_PAIR), int(CK_DENS), int(CK_EMB), int(CK_DIP), int(CK_QUAD)}) {
-      size_t n = 0;
-      for (int col = 0; col < c.D; ++col) {
-        if (cols[col].kind != kind) continue;
-        if (n % 64 == 0) slots.resize(slots.size() + 64, SlotDesc{-1, 0, CK_NONE, -1, 0, 0, 0, 0});
-        const ColDesc &cd = cols[col];
-        slots[slots.size() - 64 + n % 64] = SlotDesc{cd.tab, cd.m, cd.kind, cd.dc, col, 0, 0, 0};
-        ++n;
-      }
-    }
-    const int ns = int(slots.size() / 64);
     const bool adp = c.kind == FS_ADP;
-    FS_REQUIRE(ns <= (adp ? 8 : 6), FS_ERR_UNSUPPORTED, "Jacobian columns need too many lane slots");
-    const int NS = adp ? (ns <= 4 ? 4 : 8) : (ns <= 3 ? 3 : (ns <= 4 ? 4 : 6));
-    slots.resize(size_t(NS) * 64, SlotDesc{-1, 0, CK_NONE, -1, 0, 0, 0, 0});
+    // lane slots: pair, density and embedding columns each start a new slot.
+    // ADP: one map for its tangent kernels. EAM: the pair-table columns and
+    // the density/embedding columns form the maps of the two k_jac_main passes.
+    auto build_slots = [&](std::initializer_list<int> kinds) {
+      std::vector<SlotDesc> sl;
+      for (int kind : kinds) {
+        size_t n = 0;
+        for (int col = 0; col < c.D; ++col) {
+          if (cols[col].kind != kind) continue;
+          if (n % 64 == 0) sl.resize(sl.size() + 64, SlotDesc{-1, 0, CK_NONE, -1, 0, 0, 0, 0});
+          const ColDesc &cd = cols[col];
+          sl[sl.size() - 64 + n % 64] = SlotDesc{cd.tab, cd.m, cd.kind, cd.dc, col, 0, 0, 0};
+          ++n;
+        }
+      }
+      return sl;
+    };
+    auto pad_to = [](std::vector<SlotDesc> &sl, int ns) {
+      sl.resize(size_t(ns) * 64, SlotDesc{-1, 0, CK_NONE, -1, 0, 0, 0, 0});
+    };
+    std::vector<SlotDesc> slots, sde;
+    int NS = 0, NSP = 0, NSD = 0;
+    if (adp) {
+      slots = build_slots({CK_PAIR, CK_DENS, CK_EMB, CK_DIP, CK_QUAD});
+      const int ns = int(slots.size() / 64);
+      FS_REQUIRE(ns <= 8, FS_ERR_UNSUPPORTED, "Jacobian columns need too many lane slots");
+      NS = ns <= 4 ? 4 : 8;
+      pad_to(slots, NS);
+    } else {
+      slots = build_slots({CK_PAIR});
+      sde = build_slots({CK_DENS, CK_EMB});
+      const int nsp = int(slots.size() / 64), nsd = int(sde.size() / 64);
+      FS_REQUIRE(nsp <= 4 && nsd <= 6, FS_ERR_UNSUPPORTED, "Jacobian columns need too many lane slots");
+      NSP = nsp == 0 ? 0 : (nsp <= 2 ? 2 : 4);
+      NSD = nsd == 0 ? 0 : (nsd <= 2 ? 2 : (nsd <= 4 ? 4 : 6));
+      pad_to(slots, NSP);
+      pad_to(sde, NSD);
+      slots.insert(slots.end(), sde.begin(), sde.end());
+    }
+    const size_t lds_pair = base + 4 * sizeof(JStagePair), lds_de = base + 4 * sizeof(JStageDE);
+    FS_REQUIRE(lds_de <= 64 * 1024 && lds_pair <= 64 * 1024, FS_ERR_UNSUPPORTED,
+               "Jacobian LDS budget exceeded");
     static_assert(sizeof(SlotDesc) == 2 * sizeof(int4), "slot descriptor layout");
-    c.jslots.alloc(slots.size() * 2);
+    c.jslots.alloc(std::max<size_t>(slots.size(), 1) * 2);
     FS_HIP(hipMemcpyAsync(c.jslots.p, slots.data(), sizeof(SlotDesc) * slots.size(),
                           hipMemcpyHostToDevice, c.stream));
     FS_HIP(hipStreamSynchronize(c.stream));
@@ -1582,13 +1639,21 @@ void launch_jacobian(Context &c) {
       hipLaunchKernelGGL(k_jac_dens, dim3(g1), dim3(256), lds_dens, c.stream, a, c.nknots);
       FS_HIP(hipGetLastError());
       const unsigned g2 = unsigned(std::max(1, std::min(nt, c.cu_count * 8)));
+      const SlotDesc *d_de = d_slots + size_t(NSP) * 64;
       c.time_main_begin();
-      if (NS == 3)
-        hipLaunchKernelGGL(k_jac_main<3>, dim3(g2), dim3(256), lds_main, c.stream, a, c.nknots, d_slots);
-      else if (NS == 4)
-        hipLaunchKernelGGL(k_jac_main<4>, dim3(g2), dim3(256), lds_main, c.stream, a, c.nknots, d_slots);
-      else
-        hipLaunchKernelGGL(k_jac_main<6>, dim3(g2), dim3(256), lds_main, c.stream, a, c.nknots, d_slots);
+      if (NSP == 2)
+        hipLaunchKernelGGL((k_jac_main<JP_PAIR, 2>), dim3(g2), dim3(256), lds_pair, c.stream, a, c.nknots,
+                           d_slots);
+      else if (NSP == 4)
+        hipLaunchKernelGGL((k_jac_main<JP_PAIR, 4>), dim3(g2), dim3(256), lds_pair, c.stream, a, c.nknots,
+                           d_slots);
+      FS_HIP(hipGetLastError());
+      if (NSD == 2)
+        hipLaunchKernelGGL((k_jac_main<JP_DE, 2>), dim3(g2), dim3(256), lds_de, c.stream, a, c.nknots, d_de);
+      else if (NSD == 4)
+        hipLaunchKernelGGL((k_jac_main<JP_DE, 4>), dim3(g2), dim3(256), lds_de, c.stream, a, c.nknots, d_de);
+      else if (NSD == 6)
+        hipLaunchKernelGGL((k_jac_main<JP_DE, 6>), dim3(g2), dim3(256), lds_de, c.stream, a, c.nknots, d_de);
       FS_HIP(hipGetLastError());
       c.time_main_end();
     }
