@@ -369,6 +369,8 @@ __global__ __launch_bounds__(256) void k_jac_dens(JacArgs a, int nknots) {
 //   JP_DE:   density and embedding columns (the g_tj, g_ti and F_tj bands,
 //            the density-chain and per-atom terms).
 enum { JP_PAIR = 1, JP_DE = 2 };
+constexpr int kHBatch = 16; // Hs rows gathered per batch by the density-chain term
+constexpr int kWavesPair = 5, kWavesDE = 4; // waves per SIMD asked of the two passes
 
 struct JStagePair {
   float r[64], u0[64], u1[64], u2[64];
@@ -377,8 +379,9 @@ struct JStagePair {
 };
 
 struct JStageDE {
-  float r[64], u0[64], u1[64], u2[64], dgi[64];
-  int32_t jg[64], kgj[64], kgi[64], kfj[64];
+  float r[64], u0[64], u1[64], u2[64];
+  float hw0[64], hw1[64], hw2[64]; // g'_ti(r) u: the density-chain weights
+  int32_t jrow[64], kgj[64], kgi[64], kfj[64]; // jrow: j * Dd (row of Hs)
   float cgj[6][kBandPitch], cgi[6][kBandPitch], cemb[6][kBandPitch];
 };
 
@@ -420,7 +423,7 @@ __device__ inline void add_pair(ColAcc &A, float *t, float c, float r, float u0,
 }
 
 template <int PASS, int NS>
-__global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots, const SlotDesc *slots) {
+__device__ __forceinline__ void jac_main_body(const JacArgs &a, int nknots, const SlotDesc *slots) {
   using Stage = typename JStageOf<PASS>::T;
   constexpr bool kPairPass = PASS == JP_PAIR;
   extern __shared__ float4 smem[];
@@ -513,7 +516,7 @@ __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots, const S
             }
           } else {
             const int32_t jg = int32_t(cbase + pj_local(b));
-            P.jg[lane] = jg;
+            P.jrow[lane] = jg * a.Dd;
             const float Fpi = a.Fp32[i];
             const float fpj = a.Fp32[jg];
             // density of j, weight F'_i; also g'_tj(r) for G_i
@@ -548,7 +551,9 @@ __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots, const S
                 spl_eval32(T, a.ivl32, kk, tt, v, dgi);
               }
             }
-            P.dgi[lane] = dgi;
+            P.hw0[lane] = dgi * u0;
+            P.hw1[lane] = dgi * u1;
+            P.hw2[lane] = dgi * u2;
             // embedding of j at its density, weight g'_ti(r)
             {
               const TableDesc T = sT[a.npair + a.ntypes + tj];
@@ -617,15 +622,14 @@ __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots, const S
             }
             // F''_j H_j g'_ti(r): every pair, rows of Hs gathered eight at a time
             const float *Hs = a.Hs + cdc[s];
-            for (int q0 = 0; q0 < np; q0 += 8) {
-              float h[8];
+            for (int q0 = 0; q0 < np; q0 += kHBatch) {
+              float h[kHBatch];
 #pragma unroll
-              for (int v = 0; v < 8; ++v)
-                h[v] = q0 + v < np ? Hs[int64_t(P.jg[q0 + v]) * a.Dd] : 0.0f;
+              for (int v = 0; v < kHBatch; ++v) h[v] = q0 + v < np ? Hs[P.jrow[q0 + v]] : 0.0f;
 #pragma unroll
-              for (int v = 0; v < 8; ++v) {
+              for (int v = 0; v < kHBatch; ++v) {
                 const int q = q0 + v;
-                if (q < np) add_force(A[s], h[v] * P.dgi[q], P.u0[q], P.u1[q], P.u2[q]);
+                if (q < np) add_force(A[s], h[v], P.hw0[q], P.hw1[q], P.hw2[q]);
               }
             }
           } else if (kind == CK_EMB) {
@@ -710,6 +714,23 @@ __global__ __launch_bounds__(256) void k_jac_main(JacArgs a, int nknots, const S
     }
     __syncthreads();
   }
+}
+
+// The pass kernels. The window and gather loops are latency-bound, so each
+// pass asks for W waves per SIMD (the register budget follows; a few values
+// may live in scratch): 4 for the density/embedding pass, which its LDS
+// staging also allows (4 blocks of 4 waves per CU), and 5 for the lighter
+// pair pass.
+template <int NS, int W>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k_jac_pair(
+    JacArgs a, int nknots, const SlotDesc *slots) {
+  jac_main_body<JP_PAIR, NS>(a, nknots, slots);
+}
+
+template <int NS, int W>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k_jac_de(
+    JacArgs a, int nknots, const SlotDesc *slots) {
+  jac_main_body<JP_DE, NS>(a, nknots, slots);
 }
 
 // ---------------------------------------------------------------------------
@@ -1528,6 +1549,8 @@ void launch_jacobian(Context &c) {
                           c.stream));
     FS_HIP(hipStreamSynchronize(c.stream));
     a.Dd = Dd;
+    FS_REQUIRE(int64_t(c.N) * std::max(Dd, 1) < (int64_t(1) << 31), FS_ERR_UNSUPPORTED,
+               "density-column table exceeds 2^31 entries");
     a.dens_c0 = dens_c0;
     a.cols = reinterpret_cast<const ColDesc *>(c.jcols.p);
     a.tabs = c.tabs.p;
@@ -1641,19 +1664,22 @@ void launch_jacobian(Context &c) {
       const unsigned g2 = unsigned(std::max(1, std::min(nt, c.cu_count * 8)));
       const SlotDesc *d_de = d_slots + size_t(NSP) * 64;
       c.time_main_begin();
+      // wider maps (more than 128 columns of a pass) keep the compiler's
+      // occupancy rather than spill their larger slot state
       if (NSP == 2)
-        hipLaunchKernelGGL((k_jac_main<JP_PAIR, 2>), dim3(g2), dim3(256), lds_pair, c.stream, a, c.nknots,
-                           d_slots);
+        hipLaunchKernelGGL((k_jac_pair<2, kWavesPair>), dim3(g2), dim3(256), lds_pair, c.stream, a,
+                           c.nknots, d_slots);
       else if (NSP == 4)
-        hipLaunchKernelGGL((k_jac_main<JP_PAIR, 4>), dim3(g2), dim3(256), lds_pair, c.stream, a, c.nknots,
+        hipLaunchKernelGGL((k_jac_pair<4, 2>), dim3(g2), dim3(256), lds_pair, c.stream, a, c.nknots,
                            d_slots);
       FS_HIP(hipGetLastError());
       if (NSD == 2)
-        hipLaunchKernelGGL((k_jac_main<JP_DE, 2>), dim3(g2), dim3(256), lds_de, c.stream, a, c.nknots, d_de);
+        hipLaunchKernelGGL((k_jac_de<2, kWavesDE>), dim3(g2), dim3(256), lds_de, c.stream, a, c.nknots,
+                           d_de);
       else if (NSD == 4)
-        hipLaunchKernelGGL((k_jac_main<JP_DE, 4>), dim3(g2), dim3(256), lds_de, c.stream, a, c.nknots, d_de);
+        hipLaunchKernelGGL((k_jac_de<4, 2>), dim3(g2), dim3(256), lds_de, c.stream, a, c.nknots, d_de);
       else if (NSD == 6)
-        hipLaunchKernelGGL((k_jac_main<JP_DE, 6>), dim3(g2), dim3(256), lds_de, c.stream, a, c.nknots, d_de);
+        hipLaunchKernelGGL((k_jac_de<6, 1>), dim3(g2), dim3(256), lds_de, c.stream, a, c.nknots, d_de);
       FS_HIP(hipGetLastError());
       c.time_main_end();
     }
