@@ -7,7 +7,8 @@ parameter vector over the whole dataset (500 configurations, 32-2048 atoms,
 configs*params/s = C * D / (seconds per step).
 
 Prints one JSON line with the step throughput, the roofline figure of the
-dominant kernel (k_jac_main: FP32 Jacobian force rows), and a CPU baseline:
+dominant kernel (k_jac_de: the density/embedding-column pass of the Jacobian),
+and a CPU baseline:
 the FP64 oracle (a restatement of the reference's oneTBB calculator: central
 differences, two residual evaluations per column, configurations in parallel)
 timed on a bounded sample of columns on this host.
@@ -129,7 +130,7 @@ def de_measure(a):
     print(json.dumps(out), flush=True)
 
 
-def measured_traffic(kernel="k_jac_main"):
+def measured_traffic(kernel="k_jac_de"):
     """HBM bytes per launch of the kernel from its committed two-pass PMC
     summary (FETCH_SIZE and WRITE_SIZE, each scaled by a 1 GiB copy)."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
@@ -226,10 +227,18 @@ def main():
     t_res = g.time_step(0, 10)
     t_jac = g.time_step(1, 10)
     t_main = g.time_step(4, 10)
+    t_pair = g.time_step(5, 10)
     t_ne = g.time_step(2, 5)
-    # algorithmic bytes of k_jac_main per launch: pair records (16 B/pair) and
-    # the FP32 force rows it writes (3N x D x 4 B)
-    b_main = 16 * P + 4 * 3 * N * D
+    # algorithmic bytes of k_jac_de (the dominant kernel) per launch, each
+    # datum once: pair records (float4 + FP32 distance, 20 B/pair), the FP32
+    # force rows of the density/embedding columns it writes (3N x D_de x 4 B)
+    # and the per-atom density-column rows it reads (Hs FP32 + H FP64, 12 B
+    # per atom and density column)
+    m = w.model
+    n_pair_tab, n_dens = m.npair, m.npair + m.ntypes
+    D_dens = sum(int((~t.fixed).sum()) for t in m.tables[n_pair_tab:n_dens])
+    D_de = sum(int((~t.fixed).sum()) for t in m.tables[n_pair_tab:])
+    b_main = 20 * P + 12 * N * D_de + 12 * N * D_dens
     achieved = b_main / (t_main * 1e-3) / 1e9
     b_alg = 16 * P + 28 * N + 56 * C + 8 * R + 4 * R * D
     out = {
@@ -249,13 +258,14 @@ def main():
                                "12 tables x 20 knots (D=240), residual + full Jacobian",
                    "configs": C, "atoms": N, "pairs": P, "params": D, "rows": R,
                    "precision": a.precision, "lanes_per_atom": lpa},
-        "roofline": {"bound": "hbm", "kernel": "k_jac_main", "achieved": achieved, "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": "k_jac_de", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic() if C == 500 else None,
-                     "traffic_source": "profiles/k_jac_main_traffic_eam_ternary_500cfg.md (rocprofv3 "
+                     "traffic_source": "profiles/k_jac_de_traffic_eam_ternary_500cfg.md (rocprofv3 "
                                        "FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
                      "bytes_per_launch": b_main, "ms_per_launch": t_main},
-        "kernels_ms": {"residual": t_res, "jacobian": t_jac, "jac_main": t_main, "normal_equations": t_ne},
+        "kernels_ms": {"residual": t_res, "jacobian": t_jac, "jac_de": t_main, "jac_pair": t_pair,
+                       "normal_equations": t_ne},
         "step_bytes_alg": b_alg,
         "step_hbm_frac": b_alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
     }

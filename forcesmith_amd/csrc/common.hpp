@@ -248,18 +248,21 @@ struct Context {
   int pop_grp_tiles = -1;
   Mlp *mlp = nullptr;              // owned; see mlp.hpp
   SurrogateState *surr = nullptr;  // owned; see de.cpp
-  // per-kernel timing of the main Jacobian kernel (HIP events, context stream)
+  // per-kernel timing of one Jacobian stage (HIP events, context stream):
+  // stage 0 = the density/embedding pass (EAM) or the tangent kernels (ADP),
+  // stage 1 = the pair-column pass (EAM)
   hipEvent_t jev0 = nullptr, jev1 = nullptr;
   bool time_jac_main = false;
+  int time_which = 0;
   double jac_main_ms = 0.0;
   int jac_main_n = 0;
 
   ~Context();
-  void time_main_begin() {
-    if (time_jac_main) FS_HIP(hipEventRecord(jev0, stream));
+  void time_main_begin(int stage = 0) {
+    if (time_jac_main && stage == time_which) FS_HIP(hipEventRecord(jev0, stream));
   }
-  void time_main_end() {
-    if (!time_jac_main) return;
+  void time_main_end(int stage = 0) {
+    if (!time_jac_main || stage != time_which) return;
     FS_HIP(hipEventRecord(jev1, stream));
     FS_HIP(hipEventSynchronize(jev1));
     float ms = 0.f;

@@ -1663,7 +1663,7 @@ void launch_jacobian(Context &c) {
       FS_HIP(hipGetLastError());
       const unsigned g2 = unsigned(std::max(1, std::min(nt, c.cu_count * 8)));
       const SlotDesc *d_de = d_slots + size_t(NSP) * 64;
-      c.time_main_begin();
+      c.time_main_begin(1);
       // wider maps (more than 128 columns of a pass) keep the compiler's
       // occupancy rather than spill their larger slot state
       if (NSP == 2)
@@ -1673,6 +1673,8 @@ void launch_jacobian(Context &c) {
         hipLaunchKernelGGL((k_jac_pair<4, 2>), dim3(g2), dim3(256), lds_pair, c.stream, a, c.nknots,
                            d_slots);
       FS_HIP(hipGetLastError());
+      c.time_main_end(1);
+      c.time_main_begin(0);
       if (NSD == 2)
         hipLaunchKernelGGL((k_jac_de<2, kWavesDE>), dim3(g2), dim3(256), lds_de, c.stream, a, c.nknots,
                            d_de);
