@@ -274,6 +274,13 @@ struct DensStage {
   float c[6][kBandPitch];
 };
 
+// embedding window of every atom, one thread per atom (emb_k, emb_d, emb_v,
+// Fpp), before the passes that read them
+__global__ __launch_bounds__(256) void k_emb_sites(JacArgs a) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < a.N) (void)emb_window(a, a.tabs, i);
+}
+
 __global__ __launch_bounds__(256) void k_jac_dens(JacArgs a, int nknots) {
   extern __shared__ float4 smem[];
   TableDesc *sT = reinterpret_cast<TableDesc *>(smem);
@@ -330,12 +337,9 @@ __global__ __launch_bounds__(256) void k_jac_dens(JacArgs a, int nknots) {
       }
       wave_sync();
     }
-    // embedding at the clamped density (lane 0), broadcast F'' and the clamp
+    // F''(rho_i) from k_emb_sites; a clamped atom carries no density chain
     const bool clamped = a.over[i] != 0.0;
-    double F2 = 0.0;
-    if (lane == 0) F2 = emb_window(a, sT, i);
-    F2 = __shfl(F2, 0, 64);
-    const float hs = clamped ? 0.0f : float(F2);
+    const float hs = clamped ? 0.0f : float(a.Fpp[i]);
 #pragma unroll
     for (int s = 0; s < kCPL; ++s) {
       const int dc = lane + 64 * s;
@@ -1658,6 +1662,8 @@ void launch_jacobian(Context &c) {
         c.time_main_end();
       }
     } else if (c.N > 0) {
+      hipLaunchKernelGGL(k_emb_sites, dim3(nb(c.N, 256)), dim3(256), 0, c.stream, a);
+      FS_HIP(hipGetLastError());
       const unsigned g1 = unsigned(std::min<int64_t>((c.N + 3) / 4, int64_t(c.cu_count) * 16));
       hipLaunchKernelGGL(k_jac_dens, dim3(g1), dim3(256), lds_dens, c.stream, a, c.nknots);
       FS_HIP(hipGetLastError());
