@@ -37,7 +37,7 @@ EXPORTS = (
     "fs_param_count", "fs_gather_params", "fs_scatter_params", "fs_gather_bounds", "fs_max_cutoff",
     "fs_residual_count", "fs_row_offsets", "fs_prepare", "fs_neighbor_count", "fs_get_neighbor_list",
     "fs_eval_forces", "fs_eval_config", "fs_eval_residual", "fs_eval_jacobian", "fs_normal_equations",
-    "fs_lm_minimize", "fs_lm_solve", "fs_population_cost", "fs_de_minimize", "fs_device_buffer", "fs_synchronize",
+    "fs_lm_minimize", "fs_lm_solve", "fs_normal_equations_solve", "fs_population_cost", "fs_de_minimize", "fs_device_buffer", "fs_synchronize",
     "fs_set_launch", "fs_autotune", "fs_time_step", "fs_mlp_forward", "fs_mlp_loss_grad",
 )
 
@@ -111,6 +111,7 @@ def load_library(path: str = LIB_PATH):
         "fs_normal_equations": (ct.c_int, [vp, _dp, _dp, _dp]),
         "fs_lm_minimize": (ct.c_int, [vp, _dp, ct.c_int32, _dp, _i32p]),
         "fs_lm_solve": (ct.c_int, [vp, ct.POINTER(fs_lm_opts), _dp, _i32p, _i32p, _dp]),
+        "fs_normal_equations_solve": (ct.c_int, [vp, ct.c_double, _dp]),
         "fs_population_cost": (ct.c_int, [vp, _dp, ct.c_int32, _dp]),
         "fs_de_minimize": (ct.c_int, [vp, ct.POINTER(fs_de_opts), _dp, _dp, _dp, _dp, _dp, _i32p]),
         "fs_device_buffer": (ct.c_int, [vp, ct.c_int32, ct.POINTER(vp), _i64p]),
@@ -311,6 +312,12 @@ class GpuCalculator:
         acc = ct.c_int32()
         self._check(self._lib.fs_lm_minimize(self._h, _p(x, _dp), int(max_iter), _p(hist, _dp), ct.byref(acc)))
         return x, hist, int(acc.value)
+
+    def ne_solve(self, x, ridge=0.0):
+        """NormalEquationsSolver: one step x - (J^T J + lambda I)^-1 J^T F."""
+        x = np.ascontiguousarray(x, dtype=np.float64).copy()
+        self._check(self._lib.fs_normal_equations_solve(self._h, float(ridge), _p(x, _dp)))
+        return x
 
     def lm_solve(self, x, max_iter=500, maxfev=0, xtol=1e-7, ftol=1e-7, gtol=0.0, factor=100.0):
         """EigenLMSolver semantics: (x, status, (nfev, njev, accepted), cost

@@ -236,6 +236,7 @@ int fs_normal_impl(Context &c, double *JtJ, double *Jtr, double *cost);
 int fs_lm_impl(Context &c, double *x, int max_iter, double *hist, int *accepted);
 int fs_lm_solve_impl(Context &c, const fs_lm_opts &o, double *x, int *status, int *counts,
                      double *hist);
+int fs_ne_solve_impl(Context &c, double ridge, double *x);
 int fs_population_impl(Context &c, const double *X, int np, double *cost);
 int fs_de_impl(Context &c, const fs_de_opts *o, const double *lo, const double *hi, double *x,
                double *best, double *hist, int *gens);
@@ -717,6 +718,17 @@ int fs_lm_solve(fs_ctx *ctx, const fs_lm_opts *opts, double *x, int32_t *status,
     *status = st;
     if (counts)
       for (int k = 0; k < 3; ++k) counts[k] = cnt[k];
+  });
+}
+
+int fs_normal_equations_solve(fs_ctx *ctx, double ridge, double *x) {
+  return guarded(ctx, [&](Context &c) {
+    FS_REQUIRE(x || c.D == 0, FS_ERR_ARG, "null parameter vector");
+    FS_REQUIRE(std::isfinite(ridge), FS_ERR_ARG, "ridge must be finite");
+    c.fail_if_no_dataset();
+    c.fail_if_no_model();
+    ensure_ready(c);
+    fs_ne_solve_impl(c, ridge, x);
   });
 }
 

@@ -186,6 +186,41 @@ int fs_lm_solve_impl(Context &c, const fs_lm_opts &o, double *x, int *status, in
   return FS_OK;
 }
 
+// NormalEquationsSolver::minimize (src/optimization/solver.cpp:242-266): one
+// ridge-regularised Gauss-Newton step x -= (J^T J + lambda I)^-1 J^T F(x),
+// lambda = ridge if > 0, else 1e-8 max(mean diag(J^T J), 1). J^T J and J^T F
+// are formed on the GPU; the D x D solve (Cholesky of the SPD matrix, which
+// the reference's LDLT equals in exact arithmetic) runs on the host.
+int fs_ne_solve_impl(Context &c, double ridge, double *x) {
+  const int n = c.D;
+  if (n == 0 || c.R == 0) return FS_OK;
+  scatter_host(c, x);
+  evaluate_for_solver(c);
+  launch_jacobian(c);
+  launch_normal_equations(c);
+  std::vector<double> A(size_t(n) * n), g(n), dx(n);
+  FS_HIP(hipMemcpyAsync(A.data(), c.JtJ.p, sizeof(double) * A.size(), hipMemcpyDeviceToHost,
+                        c.stream));
+  FS_HIP(hipMemcpyAsync(g.data(), c.Jtr.p, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+  FS_HIP(hipStreamSynchronize(c.stream));
+  double lambda = ridge;
+  if (lambda <= 0.0) {
+    double mean = 0.0;
+    for (int k = 0; k < n; ++k) mean += A[size_t(k) * n + k];
+    lambda = 1e-8 * std::max(mean / n, 1.0);
+  }
+  std::vector<double> add(n, lambda);
+  lmtr::PivChol ch;
+  ch.factor(A.data(), n, add.data(), 0.0);
+  FS_REQUIRE(ch.rank == n, FS_ERR_NUMERIC, "normal equations are not positive definite");
+  ch.solve(g.data(), dx.data());
+  for (int k = 0; k < n; ++k) x[k] -= dx[k];
+  scatter_host(c, x);
+  evaluate_for_solver(c);
+  FS_HIP(hipStreamSynchronize(c.stream));
+  return FS_OK;
+}
+
 int fs_population_impl(Context &c, const double *X, int np, double *cost) {
   const int D = c.D;
   ensure_ready(c);

@@ -140,6 +140,12 @@ int fs_eval_jacobian(fs_ctx *ctx, float *J);
 /* FP64 normal equations of the last residual/Jacobian: JtJ[D*D], Jtr[D]. */
 int fs_normal_equations(fs_ctx *ctx, double *JtJ, double *Jtr, double *cost);
 
+/* One ridge-regularised normal-equation step, NormalEquationsSolver::minimize
+ * (solver.cpp:242-266; solver.hpp:138-145): x -= (J^T J + lambda I)^-1 J^T F(x)
+ * with lambda = ridge if > 0, else 1e-8 * max(mean diag(J^T J), 1). x is read
+ * and written; the device state is left at the new x. */
+int fs_normal_equations_solve(fs_ctx *ctx, double ridge, double *x);
+
 /* Levenberg-Marquardt on the GPU (FP64 normal equations, damped Cholesky);
  * replaces Solver::minimize with EigenLMSolver (solver.cpp:67-85).
  * x is read and written; cost_hist[max_iter] (may be NULL). */

@@ -91,6 +91,7 @@ def _load():
         "or_lmder": (ct.c_int, [ct.c_int64, ct.c_int, FCN, FCN, ct.c_void_p, _dp, ct.POINTER(LmOpts), _i32p,
                                 _dp]),
         "or_lm_minpack": (ct.c_int, [ct.c_void_p, _dp, ct.POINTER(LmOpts), _i32p, _dp]),
+        "or_normal_equations_solve": (ct.c_int, [ct.c_void_p, ct.c_double, _dp]),
         "or_qr_pivot": (ct.c_int, [ct.c_int64, ct.c_int, _dp, _dp, _i32p, _dp, _dp]),
         "or_lmpar": (ct.c_int, [ct.c_int, _dp, _i32p, ct.c_int, _dp, _dp, ct.c_double, _dp, _dp]),
         "or_population_cost": (None, [ct.c_void_p, _dp, ct.c_int, _dp]),
@@ -360,6 +361,14 @@ class Oracle:
         hist = np.zeros(max_iter)
         acc = self._lib.or_lm(self._h, _p(x, _dp), int(max_iter), int(use_fd), _p(hist, _dp))
         return x, hist, acc
+
+    def ne_solve(self, x, ridge=0.0):
+        """NormalEquationsSolver: one ridge step; returns the new x."""
+        x = _f64(x).copy()
+        rc = self._lib.or_normal_equations_solve(self._h, float(ridge), _p(x, _dp))
+        if rc != 0:
+            raise ValueError("normal equations not positive definite")
+        return x
 
     def lm_minpack(self, x, **opts):
         """LM with the reference's stopping semantics; returns (x, status,

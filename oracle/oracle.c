@@ -2479,6 +2479,41 @@ int or_lm_minpack(or_ctx *c, double *x, const or_lm_opts *o, int32_t *counts,
   return rc;
 }
 
+/* NormalEquationsSolver::minimize (src/optimization/solver.cpp:242-266): one
+ * step x -= (J^T J + lambda I)^-1 J^T F(x), lambda = ridge if > 0, else
+ * 1e-8 max(mean diag(J^T J), 1); the reference's LDLT equals the Cholesky
+ * solve of the SPD matrix. Returns -1 when the matrix is not positive
+ * definite. */
+int or_normal_equations_solve(or_ctx *c, double ridge, double *x) {
+  const int D = or_param_count(c);
+  const int64_t R = or_residual_count(c);
+  if (D == 0 || R == 0) return 0;
+  double *F = (double *)malloc(sizeof(double) * (size_t)R);
+  double *J = (double *)malloc(sizeof(double) * (size_t)R * D);
+  double *A = (double *)malloc(sizeof(double) * (size_t)D * D);
+  double *g = (double *)malloc(sizeof(double) * (size_t)D);
+  or_residual(c, x, F);
+  or_jacobian_analytic(c, x, J);
+  normal_eq(J, F, R, D, A, g);
+  double lambda = ridge;
+  if (lambda <= 0.0) {
+    double mean = 0.0;
+    for (int k = 0; k < D; ++k) mean += A[(size_t)k * D + k];
+    mean /= D;
+    lambda = 1e-8 * (mean > 1.0 ? mean : 1.0);
+  }
+  for (int k = 0; k < D; ++k) A[(size_t)k * D + k] += lambda;
+  const int rc = cholesky_solve(D, A, g);
+  if (rc == 0)
+    for (int k = 0; k < D; ++k) x[k] -= g[k];
+  or_scatter_params(c, x);
+  free(F);
+  free(J);
+  free(A);
+  free(g);
+  return rc;
+}
+
 /* ------------------------------------------------------------------------ */
 /* population cost and differential evolution                                */
 /* ------------------------------------------------------------------------ */

@@ -151,6 +151,8 @@ int or_lmder(int64_t m, int n, or_fcn fcn, or_jac jac, void *user, double *x,
              const or_lm_opts *o, int32_t *counts, double *cost_hist);
 int or_lm_minpack(or_ctx *c, double *x, const or_lm_opts *o, int32_t *counts,
                   double *cost_hist);
+/* NormalEquationsSolver (solver.cpp:242-266): one ridge step, x in/out */
+int or_normal_equations_solve(or_ctx *c, double ridge, double *x);
 /* the pieces, for tests: pivoted QR (returns the rank; R column-major n x n)
  * and lmpar (returns the number of damped solves) */
 int or_qr_pivot(int64_t m, int n, const double *J, const double *f, int32_t *perm,

@@ -270,3 +270,26 @@ def test_lm_solve_stillinger_weber(wl):
     print(f"SW: oracle {so} {co} {o.cost(xo):.8e}; gpu {sg} {cg} {g.cost(xg):.8e}", flush=True)
     assert 1 <= sg <= 3 or sg == so
     assert g.cost(xg) <= o.cost(xo) * (1 + 1e-3)
+
+
+def test_normal_equations_solve():
+    """NormalEquationsSolver step (fs_normal_equations_solve) against the
+    oracle's, pair knots free: the new parameters within the FP32-Jacobian
+    level and the same cost after the step; with the automatic ridge and with
+    a large one (a scaled gradient step)."""
+    w = dsm.config1(n_configs=3)
+    for t in w.model.tables[1:]:
+        t.fixed[:] = True
+    o = oracle.Oracle(w.dataset, w.model)
+    o.prepare()
+    x0 = o.gather()
+    g = fa.GpuCalculator(w.dataset, w.model, precision="fp64")
+    g.prepare()
+    for ridge in (0.0, 1e3):
+        xo = o.ne_solve(x0, ridge)
+        xg = g.ne_solve(x0, ridge)
+        d = np.linalg.norm(xg - xo) / np.linalg.norm(xo - x0)
+        co, cg = o.cost(xo), g.cost(xg)
+        print(f"ridge {ridge}: |dx| rel {d:.3e}, cost oracle {co:.10e} gpu {cg:.10e}", flush=True)
+        assert d <= 1e-3
+        assert abs(cg - co) <= 1e-5 * co
