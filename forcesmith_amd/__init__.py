@@ -33,11 +33,11 @@ STATUS = {0: "FS_OK", -1: "FS_ERR_ARG", -2: "FS_ERR_STATE", -3: "FS_ERR_HIP", -4
 # every symbol include/forcesmith_amd.h declares
 EXPORTS = (
     "fs_abi_version", "fs_ctx_create", "fs_ctx_retain", "fs_ctx_release", "fs_last_error",
-    "fs_upload_dataset", "fs_set_tabulated_model", "fs_set_sw_model", "fs_set_weights", "fs_set_smooth_weight",
+    "fs_upload_dataset", "fs_set_tabulated_model", "fs_set_sw_model", "fs_set_weights", "fs_set_smooth_weight", "fs_smoothness_count", "fs_write_smoothness",
     "fs_param_count", "fs_gather_params", "fs_scatter_params", "fs_gather_bounds", "fs_max_cutoff",
     "fs_residual_count", "fs_row_offsets", "fs_prepare", "fs_neighbor_count", "fs_get_neighbor_list",
     "fs_eval_forces", "fs_eval_config", "fs_eval_residual", "fs_eval_jacobian", "fs_normal_equations",
-    "fs_lm_minimize", "fs_lm_solve", "fs_normal_equations_solve", "fs_population_cost", "fs_de_minimize", "fs_device_buffer", "fs_synchronize",
+    "fs_lm_minimize", "fs_lm_solve", "fs_normal_equations_solve", "fs_lm_step", "fs_population_cost", "fs_de_minimize", "fs_device_buffer", "fs_synchronize",
     "fs_set_launch", "fs_autotune", "fs_time_step", "fs_mlp_forward", "fs_mlp_loss_grad",
 )
 
@@ -94,6 +94,8 @@ def load_library(path: str = LIB_PATH):
         "fs_set_sw_model": (ct.c_int, [vp, ct.c_int32, _dp, _u8p, _dp, _u8p]),
         "fs_set_weights": (ct.c_int, [vp, ct.c_double, ct.c_double]),
         "fs_set_smooth_weight": (ct.c_int, [vp, ct.c_double]),
+        "fs_smoothness_count": (ct.c_int, [vp, _i64p]),
+        "fs_write_smoothness": (ct.c_int, [vp, ct.c_double, _dp]),
         "fs_param_count": (ct.c_int, [vp, _i32p]),
         "fs_gather_params": (ct.c_int, [vp, _dp]),
         "fs_scatter_params": (ct.c_int, [vp, _dp]),
@@ -112,6 +114,7 @@ def load_library(path: str = LIB_PATH):
         "fs_lm_minimize": (ct.c_int, [vp, _dp, ct.c_int32, _dp, _i32p]),
         "fs_lm_solve": (ct.c_int, [vp, ct.POINTER(fs_lm_opts), _dp, _i32p, _i32p, _dp]),
         "fs_normal_equations_solve": (ct.c_int, [vp, ct.c_double, _dp]),
+        "fs_lm_step": (ct.c_int, [vp, ct.c_double, _dp]),
         "fs_population_cost": (ct.c_int, [vp, _dp, ct.c_int32, _dp]),
         "fs_de_minimize": (ct.c_int, [vp, ct.POINTER(fs_de_opts), _dp, _dp, _dp, _dp, _dp, _i32p]),
         "fs_device_buffer": (ct.c_int, [vp, ct.c_int32, ct.POINTER(vp), _i64p]),
@@ -187,6 +190,24 @@ class GpuCalculator:
             fx = np.concatenate([t.fixed for t in model.tables]).astype(np.uint8)
             self._check(lib.fs_set_tabulated_model(h, kind, model.ntypes, len(model.tables), _p(n, _i32p),
                                                    _p(rmin, _dp), _p(rmax, _dp), _p(y, _dp), _p(fx, _u8p)))
+
+    def smoothness_count(self) -> int:
+        v = ct.c_int64()
+        self._check(self._lib.fs_smoothness_count(self._h, ct.byref(v)))
+        return int(v.value)
+
+    def write_smoothness(self, weight: float):
+        out = np.zeros(max(self.smoothness_count(), 1))
+        self._check(self._lib.fs_write_smoothness(self._h, float(weight), _p(out, _dp)))
+        return out[:self.smoothness_count()]
+
+    def lm_step(self, lam: float, x=None):
+        """dx = -(J^T J + lam I)^-1 J^T F at the current (or given) parameters."""
+        if x is not None:
+            self.scatter_params(x)
+        dx = np.zeros(max(self.param_count, 1))
+        self._check(self._lib.fs_lm_step(self._h, float(lam), _p(dx, _dp)))
+        return dx[:self.param_count]
 
     def set_smooth_weight(self, w: float):
         """Curvature rows w (y[k-1] - 2 y[k] + y[k+1]) after the config rows."""

@@ -237,6 +237,7 @@ int fs_lm_impl(Context &c, double *x, int max_iter, double *hist, int *accepted)
 int fs_lm_solve_impl(Context &c, const fs_lm_opts &o, double *x, int *status, int *counts,
                      double *hist);
 int fs_ne_solve_impl(Context &c, double ridge, double *x);
+int fs_lm_step_impl(Context &c, double lambda, double *dx);
 int fs_population_impl(Context &c, const double *X, int np, double *cost);
 int fs_de_impl(Context &c, const fs_de_opts *o, const double *lo, const double *hi, double *x,
                double *best, double *hist, int *gens);
@@ -490,6 +491,43 @@ int fs_set_weights(fs_ctx *ctx, double energy_weight, double stress_weight) {
   });
 }
 
+// curvature rows of every table with a free knot and n >= 3, whatever the
+// weight (model_smoothness_count, smoothness.hpp:17-23)
+int fs_smoothness_count(fs_ctx *ctx, int64_t *count) {
+  return guarded(ctx, [&](Context &c) {
+    c.fail_if_no_model();
+    FS_REQUIRE(count, FS_ERR_ARG, "null output");
+    int64_t n = 0;
+    if (c.kind != FS_SW)
+      for (int t = 0; t < c.ntab; ++t) {
+        const TableDesc &T = c.h_tabs[t];
+        bool any_free = false;
+        for (int k = 0; k < T.n; ++k) any_free |= c.h_col_of[T.off + k] >= 0;
+        if (any_free && T.n >= 3) n += T.n - 2;
+      }
+    *count = n;
+  });
+}
+
+// model_write_smoothness (smoothness.hpp:26-31; spline.cpp:205-213) at the
+// current parameters
+int fs_write_smoothness(fs_ctx *ctx, double weight, double *dst) {
+  return guarded(ctx, [&](Context &c) {
+    c.fail_if_no_model();
+    if (c.kind == FS_SW) return;
+    int64_t row = 0;
+    for (int t = 0; t < c.ntab; ++t) {
+      const TableDesc &T = c.h_tabs[t];
+      bool any_free = false;
+      for (int k = 0; k < T.n; ++k) any_free |= c.h_col_of[T.off + k] >= 0;
+      if (!any_free || T.n < 3) continue;
+      FS_REQUIRE(dst, FS_ERR_ARG, "null output");
+      const double *y = c.h_y.data() + T.off;
+      for (int k = 1; k + 1 < T.n; ++k) dst[row++] = weight * (y[k - 1] - 2.0 * y[k] + y[k + 1]);
+    }
+  });
+}
+
 int fs_set_smooth_weight(fs_ctx *ctx, double weight) {
   return guarded(ctx, [&](Context &c) {
     FS_REQUIRE(std::isfinite(weight) && weight >= 0.0, FS_ERR_ARG,
@@ -718,6 +756,17 @@ int fs_lm_solve(fs_ctx *ctx, const fs_lm_opts *opts, double *x, int32_t *status,
     *status = st;
     if (counts)
       for (int k = 0; k < 3; ++k) counts[k] = cnt[k];
+  });
+}
+
+int fs_lm_step(fs_ctx *ctx, double lambda, double *dx) {
+  return guarded(ctx, [&](Context &c) {
+    FS_REQUIRE(dx || c.D == 0, FS_ERR_ARG, "null step");
+    FS_REQUIRE(std::isfinite(lambda) && lambda >= 0.0, FS_ERR_ARG, "lambda must be finite and >= 0");
+    c.fail_if_no_dataset();
+    c.fail_if_no_model();
+    ensure_ready(c);
+    fs_lm_step_impl(c, lambda, dx);
   });
 }
 

@@ -221,6 +221,28 @@ int fs_ne_solve_impl(Context &c, double ridge, double *x) {
   return FS_OK;
 }
 
+// one damped Gauss-Newton step at the current parameters: dx = -(J^T J +
+// lambda I)^-1 J^T F, parameters unchanged
+int fs_lm_step_impl(Context &c, double lambda, double *dx) {
+  const int n = c.D;
+  if (n == 0) return FS_OK;
+  evaluate_for_solver(c);
+  launch_jacobian(c);
+  launch_normal_equations(c);
+  std::vector<double> A(size_t(n) * n), g(n);
+  FS_HIP(hipMemcpyAsync(A.data(), c.JtJ.p, sizeof(double) * A.size(), hipMemcpyDeviceToHost,
+                        c.stream));
+  FS_HIP(hipMemcpyAsync(g.data(), c.Jtr.p, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+  FS_HIP(hipStreamSynchronize(c.stream));
+  std::vector<double> add(n, lambda);
+  lmtr::PivChol ch;
+  ch.factor(A.data(), n, add.data(), 0.0);
+  FS_REQUIRE(ch.rank == n, FS_ERR_NUMERIC, "damped normal equations are not positive definite");
+  ch.solve(g.data(), dx);
+  for (int k = 0; k < n; ++k) dx[k] = -dx[k];
+  return FS_OK;
+}
+
 int fs_population_impl(Context &c, const double *X, int np, double *cost) {
   const int D = c.D;
   ensure_ready(c);

@@ -100,6 +100,12 @@ int fs_set_weights(fs_ctx *ctx, double energy_weight, double stress_weight);
  * knots. fs_residual_count includes them; fs_row_offsets[C] is their first
  * row. Stillinger-Weber models have none. */
 int fs_set_smooth_weight(fs_ctx *ctx, double weight);
+/* model_smoothness_count / model_write_smoothness (smoothness.hpp:17-31): the
+ * number of curvature rows whatever the weight, and the rows themselves,
+ * weight * (y[k-1] - 2 y[k] + y[k+1]), at the current parameters (dst[count];
+ * nothing written for Stillinger-Weber). */
+int fs_smoothness_count(fs_ctx *ctx, int64_t *count);
+int fs_write_smoothness(fs_ctx *ctx, double weight, double *dst);
 
 /* CFittable surface (fit_params.hpp:24-30; TableParams
  * force_calculator_concept.hpp:171-208). */
@@ -139,6 +145,12 @@ int fs_eval_residual(fs_ctx *ctx, double *F, double *cost);
 int fs_eval_jacobian(fs_ctx *ctx, float *J);
 /* FP64 normal equations of the last residual/Jacobian: JtJ[D*D], Jtr[D]. */
 int fs_normal_equations(fs_ctx *ctx, double *JtJ, double *Jtr, double *cost);
+
+/* One damped Gauss-Newton step at the current parameters, for callers that
+ * drive their own LM loop: dx = -(J^T J + lambda I)^-1 J^T F (parameters
+ * unchanged; lambda >= 0). No reference counterpart: the reference's LM keeps
+ * its steps inside Eigen. */
+int fs_lm_step(fs_ctx *ctx, double lambda, double *dx);
 
 /* One ridge-regularised normal-equation step, NormalEquationsSolver::minimize
  * (solver.cpp:242-266; solver.hpp:138-145): x -= (J^T J + lambda I)^-1 J^T F(x)

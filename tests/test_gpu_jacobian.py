@@ -177,6 +177,9 @@ def test_smoothness_rows(wl, name):
     Fo, (Fg, cg) = o.residual(x1), g.residual(x1, with_cost=True)
     # curvature rows are FP64 on both sides, same operation order
     assert np.allclose(Fg[R0:], Fo[R0:], rtol=1e-13, atol=1e-15)
+    # the model-level hooks (model_smoothness_count / model_write_smoothness)
+    assert g.smoothness_count() == o.smooth_count
+    assert np.allclose(g.write_smoothness(sw), Fo[R0:], rtol=1e-13, atol=1e-15)
     assert abs(cg - Fo @ Fo) <= 1e-6 * (Fo @ Fo)
     Jo, Jg = o.jacobian(x1), g.jacobian(x1).astype(np.float64)
     assert np.array_equal(Jg[R0:], Jo[R0:].astype(np.float32).astype(np.float64))
@@ -293,3 +296,25 @@ def test_normal_equations_solve():
         print(f"ridge {ridge}: |dx| rel {d:.3e}, cost oracle {co:.10e} gpu {cg:.10e}", flush=True)
         assert d <= 1e-3
         assert abs(cg - co) <= 1e-5 * co
+
+
+def test_lm_step(wl):
+    """fs_lm_step: dx = -(J^T J + lambda I)^-1 J^T F against the oracle's FP64
+    normal equations, pair knots free (well conditioned)."""
+    w = dsm.config1(n_configs=3)
+    for t in w.model.tables[1:]:
+        t.fixed[:] = True
+    o = oracle.Oracle(w.dataset, w.model)
+    o.prepare()
+    x0 = o.gather()
+    J, F = o.jacobian(x0), o.residual(x0)
+    A, gr = J.T @ J, J.T @ F
+    g = fa.GpuCalculator(w.dataset, w.model, precision="fp64")
+    g.prepare()
+    for lam in (1e-3 * np.mean(np.diag(A)), 10.0 * np.mean(np.diag(A))):
+        ref = -np.linalg.solve(A + lam * np.eye(len(x0)), gr)
+        dx = g.lm_step(lam, x0)
+        rel = np.linalg.norm(dx - ref) / np.linalg.norm(ref)
+        print(f"lambda {lam:.3e}: |dx - ref| / |ref| = {rel:.3e}", flush=True)
+        assert rel <= 1e-4
+    assert np.array_equal(g.gather_params(), x0)  # parameters unchanged
