@@ -47,6 +47,10 @@ def parse():
     ap.add_argument("--de-plain-seconds", type=float, default=0.0,
                     help="surrogate run: the plain run's wall time")
     ap.add_argument("--de-lanes", type=int, default=4)
+    ap.add_argument("--de-plain-precision", default="fp64", choices=["mixed", "fp64"],
+                    help="precision of the plain DE run's batched costs (the baseline: FP64)")
+    ap.add_argument("--de-surrogate-precision", default="mixed", choices=["mixed", "fp64"],
+                    help="precision of the screened run's exact costs")
     ap.add_argument("--pmc-driver", action="store_true",
                     help="counter-collection driver: a 1 GiB copy (calibration) then three "
                          "residual + Jacobian evaluations; prints no JSON")
@@ -92,18 +96,26 @@ def de_measure(a):
 
     log(f"generating the ternary EAM workload ({a.configs} configs, seed 2)")
     w = datasets.config2(n_configs=a.configs)
-    g = fa.GpuCalculator(w.dataset, w.model)
-    g.prepare()
-    g.set_launch(a.de_lanes)
+
+    def calculator(precision):
+        g = fa.GpuCalculator(w.dataset, w.model, precision=precision)
+        g.prepare()
+        g.set_launch(a.de_lanes)
+        return g
+
+    g = calculator(a.de_plain_precision if a.de in ("plain", "both") else a.de_surrogate_precision)
     x0 = g.gather_params()
     lo = x0 - a.de_box * np.abs(x0) - 1e-3
     hi = x0 + a.de_box * np.abs(x0) + 1e-3
     common = dict(F=0.65, CR=0.5, seed=42, progress_every=25)
     out = {"metric": "DE wall-time to target cost vs plain GPU", "unit": "x", "higher_is_better": False,
-           "n_gpus": 1, "data": "synthetic (seeded ternary analytic EAM truth + noise)", "dtype": "fp32",
+           "n_gpus": 1, "data": "synthetic (seeded ternary analytic EAM truth + noise)",
+           "dtype": f"plain {a.de_plain_precision}, surrogate-run exact costs {a.de_surrogate_precision}",
            "config": {"workload": f"ternary tabulated EAM, {a.configs} configs, D={x0.size}",
                       "np": a.de_np, "generations": a.de_generations, "F": 0.65, "CR": 0.5, "seed": 42,
                       "box": f"x0 +- ({a.de_box}|x0| + 1e-3)", "lanes_per_atom": a.de_lanes,
+                      "plain_precision": a.de_plain_precision,
+                      "surrogate_precision": a.de_surrogate_precision,
                       "surrogate": {"hidden": 256, "oversample": 4, "retrain_every": 10}}}
     t_plain, c_plain = a.de_plain_seconds, a.de_target
     if a.de in ("plain", "both"):
@@ -116,6 +128,9 @@ def de_measure(a):
     if a.de in ("surrogate", "both"):
         if c_plain <= 0:
             raise SystemExit("--de surrogate needs --de-target (the plain run's final cost)")
+        if a.de == "both" and a.de_surrogate_precision != a.de_plain_precision:
+            g.close()
+            g = calculator(a.de_surrogate_precision)
         target = c_plain * (1 + 1e-6)
         t0 = time.perf_counter()
         _, c_s, hs = g.de(x0, lo, hi, a.de_np, a.de_generations, surrogate=True, oversample=4,

@@ -357,7 +357,7 @@ void surrogate_maybe_train(Context &c, const fs_de_opts *o, int gen) {
 
 
 // ---------------------------------------------------------------------------
-// Device-resident DE (EAM, mixed precision): population, trials, costs and
+// Device-resident DE (EAM, mixed or FP64): population, trials, costs and
 // selection never leave the GPU. A generation is
 //   k_de_trials -> population_launch -> k_de_select -> k_de_best
 // with the generation index in device memory, so the plain-mode generation

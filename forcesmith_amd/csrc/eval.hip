@@ -780,8 +780,11 @@ __global__ __launch_bounds__(256) void k_pop_density(EvalArgs a, PopArgs p) {
     p.partE[((size_t(m) * p.C + run_c) * kPopG + g) * kPopE + threadIdx.x] = run;
 }
 
-template <int LPA>
+// F64: every pair through the FP64 path (FP64 accumulation); otherwise FP32
+// pairs with the FP64 path only for edge pairs, as the residual kernels
+template <int LPA, bool F64>
 __global__ __launch_bounds__(256) void k_pop_force(EvalArgs a, PopArgs p) {
+  using AT = typename std::conditional<F64, double, float>::type;
   constexpr int APB = 256 / LPA;
   extern __shared__ float4 smem[];
   TableDesc *sT = reinterpret_cast<TableDesc *>(smem);
@@ -816,22 +819,22 @@ __global__ __launch_bounds__(256) void k_pop_force(EvalArgs a, PopArgs p) {
       const int ti = a.types[i];
       AtomState<float> si;
       si.Fp = am.Fp32[i];
-      ForceAcc<float> acc;
+      ForceAcc<AT> acc;
       for (int64_t q = a.pair_off[i] + lane; q < a.pair_off[i + 1]; q += LPA) {
         const float4 rec = a.rec[q];
         const uint32_t b = __float_as_uint(rec.w);
-        if (b & kFlagEdge) {
+        if (F64 || (b & kFlagEdge)) {
           const Pair<double> e = load_pair64<true>(a, q);
           AtomState<double> sd;
           sd.Fp = am.Fp[i];
           ForceAcc<double> t;
           force_pair<FS_EAM, true, double>(tb, am, ti, cbase, sd, e.b, e.d0, e.d1, e.d2, e.r, t);
-          acc.f0 += float(t.f0);
-          acc.f1 += float(t.f1);
-          acc.f2 += float(t.f2);
+          acc.f0 += AT(t.f0);
+          acc.f1 += AT(t.f1);
+          acc.f2 += AT(t.f2);
           acc.ep += t.ep;
           for (int k = 0; k < 9; ++k) acc.sg[k] += t.sg[k];
-        } else {
+        } else if constexpr (!F64) {
           const float r = sqrtf(rec.x * rec.x + rec.y * rec.y + rec.z * rec.z);
           force_pair<FS_EAM, false, float>(tb, am, ti, cbase, si, b, rec.x, rec.y, rec.z, r, acc,
                                            a.r64[q]);
@@ -1029,7 +1032,7 @@ void launch_smooth_residual(Context &c) {
 }
 
 bool population_supported(const Context &c) {
-  return c.kind == FS_EAM && c.precision == FS_MIXED && c.ntiles > 0 && c.D > 0;
+  return c.kind == FS_EAM && c.ntiles > 0 && c.D > 0;
 }
 
 // members per batch: per member 12 N bytes of embedding derivatives, the
@@ -1101,7 +1104,10 @@ void population_launch(Context &c, const double *d_X, int np, double *d_cost) {
   case L:                                                                                           \
     hipLaunchKernelGGL(k_pop_density<L>, grid, dim3(256), lds_d, c.stream, a, p);                   \
     FS_HIP(hipGetLastError());                                                                      \
-    hipLaunchKernelGGL(k_pop_force<L>, grid, dim3(256), lds_f, c.stream, a, p);                     \
+    if (c.precision == FS_FP64)                                                                     \
+      hipLaunchKernelGGL((k_pop_force<L, true>), grid, dim3(256), lds_f, c.stream, a, p);           \
+    else                                                                                            \
+      hipLaunchKernelGGL((k_pop_force<L, false>), grid, dim3(256), lds_f, c.stream, a, p);          \
     break;
       FS_POP(4)
       FS_POP(8)

@@ -178,3 +178,27 @@ def test_de_reference_box_and_population_size(wl):
     blo2[0] = x0[0] - 1e-3
     xd, bd, hd = g.de(x0, blo2, bhi, 15 * x0.size, 3, seed=5)
     assert np.array_equal(xc, xd) and bc == bd
+
+
+@pytest.mark.parametrize("name", ["eam1", "eam3"])
+def test_batched_population_fp64_matches_oracle(wl, name):
+    """FP64 batched population costs (every pair through the FP64 path) within
+    the FP64-mode bar of 1e-11, including a clamped member."""
+    w = wl[name]
+    o = oracle.Oracle(w.dataset, w.model)
+    o.prepare()
+    g = fa.GpuCalculator(w.dataset, w.model, precision="fp64")
+    g.prepare()
+    x0 = g.gather_params()
+    xc = x0.copy()
+    off = 0
+    for t, tab in enumerate(w.model.tables):
+        nfree = tab.free_count
+        if w.model.npair <= t < w.model.npair + w.model.ntypes:
+            xc[off:off + nfree] *= 3.0
+        off += nfree
+    X = members(x0, 9, 5, xc)
+    cg, co = g.population_cost(X), o.population_cost(X)
+    rel = np.abs(cg - co) / co
+    _print(f"[{name} fp64] members {len(X)} max rel vs oracle {rel.max():.3e}")
+    assert rel.max() <= 1e-11
