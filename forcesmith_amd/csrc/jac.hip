@@ -1662,24 +1662,33 @@ void launch_jacobian(Context &c) {
         c.time_main_end();
       }
     } else if (c.N > 0) {
+      // fork: the pair-column pass needs only the sorted pair records and the
+      // tables, so it runs on the second stream beside the embedding-site and
+      // density kernels; k_jac_rows joins it
+      if (!c.stream2) FS_HIP(hipStreamCreateWithFlags(&c.stream2, hipStreamNonBlocking));
+      if (!c.ev_fork) FS_HIP(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
+      if (!c.ev_join) FS_HIP(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
+      FS_HIP(hipEventRecord(c.ev_fork, c.stream));
+      FS_HIP(hipStreamWaitEvent(c.stream2, c.ev_fork, 0));
+      const unsigned g2 = unsigned(std::max(1, std::min(nt, c.cu_count * 8)));
+      const SlotDesc *d_de = d_slots + size_t(NSP) * 64;
+      c.time_main_begin(1, c.stream2);
+      // wider maps (more than 128 columns of a pass) keep the compiler's
+      // occupancy rather than spill their larger slot state
+      if (NSP == 2)
+        hipLaunchKernelGGL((k_jac_pair<2, kWavesPair>), dim3(g2), dim3(256), lds_pair, c.stream2, a,
+                           c.nknots, d_slots);
+      else if (NSP == 4)
+        hipLaunchKernelGGL((k_jac_pair<4, 2>), dim3(g2), dim3(256), lds_pair, c.stream2, a, c.nknots,
+                           d_slots);
+      FS_HIP(hipGetLastError());
+      c.time_main_end(1, c.stream2);
+      FS_HIP(hipEventRecord(c.ev_join, c.stream2));
       hipLaunchKernelGGL(k_emb_sites, dim3(nb(c.N, 256)), dim3(256), 0, c.stream, a);
       FS_HIP(hipGetLastError());
       const unsigned g1 = unsigned(std::min<int64_t>((c.N + 3) / 4, int64_t(c.cu_count) * 16));
       hipLaunchKernelGGL(k_jac_dens, dim3(g1), dim3(256), lds_dens, c.stream, a, c.nknots);
       FS_HIP(hipGetLastError());
-      const unsigned g2 = unsigned(std::max(1, std::min(nt, c.cu_count * 8)));
-      const SlotDesc *d_de = d_slots + size_t(NSP) * 64;
-      c.time_main_begin(1);
-      // wider maps (more than 128 columns of a pass) keep the compiler's
-      // occupancy rather than spill their larger slot state
-      if (NSP == 2)
-        hipLaunchKernelGGL((k_jac_pair<2, kWavesPair>), dim3(g2), dim3(256), lds_pair, c.stream, a,
-                           c.nknots, d_slots);
-      else if (NSP == 4)
-        hipLaunchKernelGGL((k_jac_pair<4, 2>), dim3(g2), dim3(256), lds_pair, c.stream, a, c.nknots,
-                           d_slots);
-      FS_HIP(hipGetLastError());
-      c.time_main_end(1);
       c.time_main_begin(0);
       if (NSD == 2)
         hipLaunchKernelGGL((k_jac_de<2, kWavesDE>), dim3(g2), dim3(256), lds_de, c.stream, a, c.nknots,
@@ -1690,6 +1699,7 @@ void launch_jacobian(Context &c) {
         hipLaunchKernelGGL((k_jac_de<6, 1>), dim3(g2), dim3(256), lds_de, c.stream, a, c.nknots, d_de);
       FS_HIP(hipGetLastError());
       c.time_main_end();
+      FS_HIP(hipStreamWaitEvent(c.stream, c.ev_join, 0));
     }
   }
   hipLaunchKernelGGL(k_jac_rows, dim3(c.C), dim3(256), 0, c.stream, a);
