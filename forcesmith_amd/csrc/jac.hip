@@ -108,6 +108,7 @@ struct JacArgs {
   int ntypes, npair, ntab;
   int ntiles;
   const int32_t *tile_cfg, *tile_a0, *tile_na, *cfg_tile0;
+  int gi_nf; // free knots per density table when all are equal and fit one slot, else 0
   const int64_t *atom_off, *pair_off, *row_off;
   const int32_t *types;
   const float4 *rec;
@@ -614,8 +615,36 @@ __device__ __forceinline__ void jac_main_body(const JacArgs &a, int nknots, cons
                 add_virial(tp[s], 2.0f * cq, P.r[q], P.u0[q], P.u1[q], P.u2[q]);
               }
             }
-            // F'_j g'_ti(r), ti == t, every partner type
-            if (t == ti) {
+            // F'_j g'_ti(r) of column (ti, j), every partner type. With equal
+            // free-knot counts nf per density table (lane = t nf + j), lane
+            // (t, j) takes partner segment t of column (ti, j) and the owner
+            // lane (ti, j) collects the ntypes partials by shuffles
+            if (a.gi_nf > 0) {
+              const int nf = a.gi_nf;
+              const int j = lane - t * nf;
+              const int mg = __shfl(m, ti * nf + j, 64);
+              float g0 = 0.0f, g1 = 0.0f, g2 = 0.0f;
+              {
+                const int s1 = seg[t + 1];
+                const int lo = first_ge(P.kgi, seg[t], s1, mg - 3);
+                const int hi = first_ge(P.kgi, lo, s1, mg + 3);
+                for (int q = lo; q < hi; ++q) {
+                  const float cq = P.cgi[mg - P.kgi[q] + 2][q];
+                  g0 += cq * P.u0[q];
+                  g1 += cq * P.u1[q];
+                  g2 += cq * P.u2[q];
+                }
+              }
+              for (int tt = 0; tt < a.ntypes; ++tt) {
+                const int src = tt * nf + j;
+                const float v0 = __shfl(g0, src, 64), v1 = __shfl(g1, src, 64), v2 = __shfl(g2, src, 64);
+                if (t == ti) {
+                  A[s].f[0] += v0;
+                  A[s].f[1] += v1;
+                  A[s].f[2] += v2;
+                }
+              }
+            } else if (t == ti) {
               for (int tt = 0; tt < a.ntypes; ++tt) {
                 const int s1 = seg[tt + 1];
                 const int lo = first_ge(P.kgi, seg[tt], s1, m - 3);
@@ -1553,6 +1582,16 @@ void launch_jacobian(Context &c) {
                           c.stream));
     FS_HIP(hipStreamSynchronize(c.stream));
     a.Dd = Dd;
+    // the balanced g_ti windows need every density table to contribute the same
+    // number of free columns, laid out table by table in one lane slot
+    {
+      std::vector<int> nfree(c.ntypes, 0);
+      for (int col = 0; col < c.D; ++col)
+        if (cols[col].kind == CK_DENS) ++nfree[cols[col].tab - c.npair];
+      bool equal = c.ntypes > 0;
+      for (int t = 1; t < c.ntypes; ++t) equal = equal && nfree[t] == nfree[0];
+      a.gi_nf = (equal && nfree[0] > 0 && nfree[0] * c.ntypes <= 64) ? nfree[0] : 0;
+    }
     FS_REQUIRE(int64_t(c.N) * std::max(Dd, 1) < (int64_t(1) << 31), FS_ERR_UNSUPPORTED,
                "density-column table exceeds 2^31 entries");
     a.dens_c0 = dens_c0;
