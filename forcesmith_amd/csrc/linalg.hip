@@ -197,68 +197,140 @@ __global__ void k_diag_scale(int D, const double *__restrict__ A, double *__rest
   if (threadIdx.x == 0) scal[0] = mx;
 }
 
-// (A + mu diag(dg)) dx = -g; scal[1] = info (0 ok), scal[2] = dx.(mu dg dx - g)
-__global__ __launch_bounds__(256) void k_chol_solve(int D, const double *__restrict__ A,
-                                                    const double *__restrict__ g,
-                                                    const double *__restrict__ dg, double mu,
-                                                    double *__restrict__ M, double *__restrict__ dx,
-                                                    double *__restrict__ scal) {
+// (A + mu diag(dg)) dx = -g by a blocked Cholesky in one workgroup (D <= 256);
+// scal[1] = info (0 ok), scal[2] = dx.(mu dg dx - g). The lower triangle of M
+// (global, L2-resident) holds the factor. Per 16-column block: the panel
+// (rows j0..D) is staged in LDS and factored column by column; the trailing
+// lower triangle takes one rank-16 update, each thread keeping one panel row
+// in registers; the triangular solves run block by block.
+constexpr int kCB = 16;
+
+__global__ __launch_bounds__(1024) void k_chol_solve(int D, const double *__restrict__ A,
+                                                     const double *__restrict__ g,
+                                                     const double *__restrict__ dg, double mu,
+                                                     double *__restrict__ M, double *__restrict__ dx,
+                                                     double *__restrict__ scal) {
+  __shared__ double sP[256][kCB + 1];
+  __shared__ double red[1024 / 64];
   __shared__ int bad;
-  __shared__ double red[256];
-  const int t = threadIdx.x;
+  const int t = threadIdx.x, nt = blockDim.x;
   if (t == 0) bad = 0;
-  for (int64_t e = t; e < int64_t(D) * D; e += blockDim.x) M[e] = A[e];
+  for (int e = t; e < D * D; e += nt) M[e] = A[e];
   __syncthreads();
-  for (int k = t; k < D; k += blockDim.x) {
+  for (int k = t; k < D; k += nt) {
     M[int64_t(k) * D + k] += mu * dg[k];
     dx[k] = -g[k];
   }
   __syncthreads();
-  for (int j = 0; j < D; ++j) {
-    if (t == 0) {
-      const double v = M[int64_t(j) * D + j];
-      if (!(v > 0.0) || !isfinite(v)) bad = 1;
-      M[int64_t(j) * D + j] = v > 0.0 ? sqrt(v) : 1.0;
+  for (int j0 = 0; j0 < D; j0 += kCB) {
+    const int w = min(kCB, D - j0), n2 = D - j0;
+    for (int e = t; e < n2 * w; e += nt) {
+      const int r = e / w, cc = e % w;
+      sP[r][cc] = M[int64_t(j0 + r) * D + j0 + cc];
     }
     __syncthreads();
-    const double ljj = M[int64_t(j) * D + j];
-    for (int i = j + 1 + t; i < D; i += blockDim.x) M[int64_t(i) * D + j] /= ljj;
-    __syncthreads();
-    // trailing update of the lower triangle
-    const int n = D - j - 1;
-    for (int64_t e = t; e < int64_t(n) * n; e += blockDim.x) {
-      const int ii = j + 1 + int(e / n), kk = j + 1 + int(e % n);
-      if (kk <= ii) M[int64_t(ii) * D + kk] -= M[int64_t(ii) * D + j] * M[int64_t(kk) * D + j];
+    // factor the panel (rows 0..n2 of it, columns 0..w)
+    for (int jj = 0; jj < w; ++jj) {
+      if (t == 0) {
+        const double d = sP[jj][jj];
+        if (!(d > 0.0) || !isfinite(d)) bad = 1;
+        sP[jj][jj] = d > 0.0 ? sqrt(d) : 1.0;
+      }
+      __syncthreads();
+      const double l = sP[jj][jj];
+      for (int r = jj + 1 + t; r < n2; r += nt) sP[r][jj] /= l;
+      __syncthreads();
+      const int nc = w - jj - 1;
+      if (nc > 0)
+        for (int e = t; e < (n2 - jj - 1) * nc; e += nt) {
+          const int r = jj + 1 + e / nc, cc = jj + 1 + e % nc;
+          if (cc <= r) sP[r][cc] -= sP[r][jj] * sP[cc][jj];
+        }
+      __syncthreads();
+    }
+    for (int e = t; e < n2 * w; e += nt) {
+      const int r = e / w, cc = e % w;
+      if (cc <= r) M[int64_t(j0 + r) * D + j0 + cc] = sP[r][cc];
+    }
+    // trailing update: M[i][k] -= sum_p P[i][p] P[k][p] for w <= k <= i < n2
+    const int m2 = n2 - w;
+    if (m2 > 0) {
+      const int G = nt / m2; // threads per trailing row (>= 4 for D <= 256)
+      const int ri = t / G, sub = t % G;
+      if (ri < m2) {
+        double li[kCB];
+#pragma unroll
+        for (int p = 0; p < kCB; ++p) li[p] = p < w ? sP[w + ri][p] : 0.0;
+        double *Mi = M + int64_t(j0 + w + ri) * D + j0 + w;
+        for (int rk = sub; rk <= ri; rk += G) {
+          double s = 0.0;
+#pragma unroll
+          for (int p = 0; p < kCB; ++p)
+            if (p < w) s += li[p] * sP[w + rk][p];
+          Mi[rk] -= s;
+        }
+      }
     }
     __syncthreads();
   }
-  // L y = b (column sweep)
-  for (int j = 0; j < D; ++j) {
-    if (t == 0) dx[j] /= M[int64_t(j) * D + j];
-    __syncthreads();
-    const double yj = dx[j];
-    for (int i = j + 1 + t; i < D; i += blockDim.x) dx[i] -= M[int64_t(i) * D + j] * yj;
-    __syncthreads();
-  }
-  // L^T x = y (row sweep from the bottom)
-  for (int j = D - 1; j >= 0; --j) {
-    if (t == 0) dx[j] /= M[int64_t(j) * D + j];
-    __syncthreads();
-    const double xj = dx[j];
-    for (int i = t; i < j; i += blockDim.x) dx[i] -= M[int64_t(j) * D + i] * xj;
-    __syncthreads();
-  }
-  double s = 0.0;
-  for (int k = t; k < D; k += blockDim.x) s += dx[k] * (mu * dg[k] * dx[k] - g[k]);
-  red[t] = s;
+  // L y = b, then L^T x = y, block by block with the right-hand side and each
+  // diagonal block in LDS (the serial part is a 16 x 16 solve on thread 0)
+  __shared__ double sx[256];
+  for (int k = t; k < D; k += nt) sx[k] = dx[k];
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (t < w) red[t] += red[t + w];
+  for (int j0 = 0; j0 < D; j0 += kCB) {
+    const int w = min(kCB, D - j0);
+    for (int e = t; e < w * w; e += nt) {
+      const int r = e / w, cc = e % w;
+      sP[r][cc] = cc <= r ? M[int64_t(j0 + r) * D + j0 + cc] : 0.0;
+    }
+    __syncthreads();
+    if (t == 0)
+      for (int k = 0; k < w; ++k) {
+        double v = sx[j0 + k];
+        for (int p = 0; p < k; ++p) v -= sP[k][p] * sx[j0 + p];
+        sx[j0 + k] = v / sP[k][k];
+      }
+    __syncthreads();
+    for (int i = j0 + w + t; i < D; i += nt) {
+      double v = 0.0;
+      for (int p = 0; p < w; ++p) v += M[int64_t(i) * D + j0 + p] * sx[j0 + p];
+      sx[i] -= v;
+    }
     __syncthreads();
   }
+  for (int j1 = D; j1 > 0; j1 -= kCB) {
+    const int j0 = max(0, j1 - kCB), w = j1 - j0;
+    for (int e = t; e < w * w; e += nt) {
+      const int r = e / w, cc = e % w;
+      sP[r][cc] = cc <= r ? M[int64_t(j0 + r) * D + j0 + cc] : 0.0;
+    }
+    __syncthreads();
+    if (t == 0)
+      for (int k = w - 1; k >= 0; --k) {
+        double v = sx[j0 + k];
+        for (int i = k + 1; i < w; ++i) v -= sP[i][k] * sx[j0 + i];
+        sx[j0 + k] = v / sP[k][k];
+      }
+    __syncthreads();
+    for (int k = t; k < j0; k += nt) {
+      double v = 0.0;
+      for (int i = j0; i < j1; ++i) v += M[int64_t(i) * D + k] * sx[i];
+      sx[k] -= v;
+    }
+    __syncthreads();
+  }
+  for (int k = t; k < D; k += nt) dx[k] = sx[k];
+  double s = 0.0;
+  for (int k = t; k < D; k += nt) s += sx[k] * (mu * dg[k] * sx[k] - g[k]);
+  for (int m = 32; m > 0; m >>= 1) s += __shfl_xor(s, m, 64);
+  if ((t & 63) == 0) red[t >> 6] = s;
+  __syncthreads();
   if (t == 0) {
+    double tot = 0.0;
+    for (int w = 0; w < nt / 64; ++w) tot += red[w];
     scal[1] = bad ? 1.0 : 0.0;
-    scal[2] = red[0];
+    scal[2] = tot;
   }
 }
 
@@ -301,7 +373,8 @@ void launch_diag_scale(Context &c) {
 
 void launch_damped_solve(Context &c, double mu) {
   c.lm_M.alloc(size_t(std::max(c.D, 1)) * std::max(c.D, 1));
-  hipLaunchKernelGGL(k_chol_solve, dim3(1), dim3(256), 0, c.stream, c.D, c.JtJ.p, c.Jtr.p,
+  FS_REQUIRE(c.D <= 256, FS_ERR_UNSUPPORTED, "the damped solve supports at most 256 parameters");
+  hipLaunchKernelGGL(k_chol_solve, dim3(1), dim3(1024), 0, c.stream, c.D, c.JtJ.p, c.Jtr.p,
                      c.lm_dx.p + c.D, mu, c.lm_M.p, c.lm_dx.p, c.lm_scal.p);
   FS_HIP(hipGetLastError());
 }
