@@ -235,6 +235,12 @@ struct Context {
   DBuf<int32_t> jt_cfg, jt_a0, jt_na, jt_ct0;
   int njt = 0;
   DBuf<int4> jcols, jslots;
+  // column descriptors / lane-slot maps derived from the model (jcols, jslots
+  // hold them on the device); rebuilt after every model change
+  struct JacLayout {
+    bool valid = false;
+    int Dd = 0, dens_c0 = 0, gi_nf = 0, NS = 0, NSP = 0, NSD = 0;
+  } jl;
   DBuf<float4> jrec; // pair records sorted by (partner type, r) per atom
   DBuf<float> jr;    // their FP32 distances
   bool jsort_valid = false;

@@ -447,6 +447,7 @@ int fs_set_tabulated_model(fs_ctx *ctx, int32_t kind, int32_t ntypes, int32_t nt
     upload(c, c.knot_of, c.h_knot_of);
     c.have_model = true;
     c.params_dirty = true;
+    c.jl.valid = false;
     smooth_layout(c);
     if (c.have_nl && c.C > 0) {
       if (max_cutoff(c) == c.nl_rc) launch_pair_flags(c);
@@ -477,6 +478,7 @@ int fs_set_sw_model(fs_ctx *ctx, int32_t ntypes, const double *pair_params,
     c.have_model = true;
     c.params_dirty = true;
     c.have_nl = false;
+    c.jl.valid = false;
     smooth_layout(c);
     c.forces_valid = false;
     FS_HIP(hipStreamSynchronize(c.stream));

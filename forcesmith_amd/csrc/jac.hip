@@ -39,7 +39,7 @@ namespace fsa {
 namespace {
 
 constexpr int kCPL = 4;        // columns per lane (D <= 256)
-constexpr int kJacTile = 64;   // atoms per Jacobian tile
+constexpr int kJacTile = 64;   // most atoms per Jacobian tile
 constexpr int kJP = 8;         // partial rows: energy, 6 virial, limit
 constexpr int kXcds = 8;       // MI355X accelerator complex dies (one L2 each)
 enum { CK_PAIR = 0, CK_DENS = 1, CK_EMB = 2, CK_DIP = 3, CK_QUAD = 4, CK_NONE = 7 };
@@ -1454,13 +1454,17 @@ void launch_jacobian(Context &c) {
   c.J.alloc(size_t(std::max<int64_t>(R * c.D, 1)));
   // Jacobian tiles: <= 64 consecutive atoms of one configuration
   if (c.njt == 0) {
+    // tile size: enough tiles for about 16 waves per CU (the ADP and SW tile
+    // kernels run one wave per tile), between 4 and 64 atoms, a multiple of 4
+    const int64_t want = c.N / std::max<int64_t>(1, int64_t(c.cu_count) * 16);
+    const int jt = int(std::max<int64_t>(4, std::min<int64_t>(kJacTile, (want + 3) & ~int64_t(3))));
     std::vector<int32_t> tcfg, ta0, tna, ct0(c.C + 1, 0);
     for (int k = 0; k < c.C; ++k) {
       ct0[k] = int32_t(tcfg.size());
-      for (int64_t a = c.h_atom_off[k]; a < c.h_atom_off[k + 1]; a += kJacTile) {
+      for (int64_t a = c.h_atom_off[k]; a < c.h_atom_off[k + 1]; a += jt) {
         tcfg.push_back(k);
         ta0.push_back(int32_t(a));
-        tna.push_back(int32_t(std::min<int64_t>(kJacTile, c.h_atom_off[k + 1] - a)));
+        tna.push_back(int32_t(std::min<int64_t>(jt, c.h_atom_off[k + 1] - a)));
       }
     }
     ct0[c.C] = int32_t(tcfg.size());
@@ -1550,48 +1554,106 @@ void launch_jacobian(Context &c) {
       FS_HIP(hipGetLastError());
     }
   } else {
-    // column descriptors
-    std::vector<ColDesc> cols(std::max(c.D, 1));
-    std::vector<std::pair<int, int>> pair_types(std::max(c.npair, 1));
-    for (int ta = 0; ta < c.ntypes; ++ta)
-      for (int tb = ta; tb < c.ntypes; ++tb) pair_types[ord_pair(ta, tb, c.ntypes)] = {ta, tb};
-    int dens_c0 = -1, Dd = 0;
-    for (int t = 0; t < c.ntab; ++t) {
-      const TableDesc &T = c.h_tabs[t];
-      int kind = CK_PAIR;
-      if (t >= c.npair && t < c.npair + c.ntypes) kind = CK_DENS;
-      else if (t >= c.npair + c.ntypes && t < c.npair + 2 * c.ntypes) kind = CK_EMB;
-      else if (t >= c.npair + 2 * c.ntypes && t < 2 * c.npair + 2 * c.ntypes) kind = CK_DIP;
-      else if (t >= 2 * c.npair + 2 * c.ntypes) kind = CK_QUAD;
-      for (int m = 0; m < T.n; ++m) {
-        const int col = c.h_col_of[T.off + m];
-        if (col < 0) continue;
-        ColDesc cd{t, m, kind, -1};
-        if (kind == CK_PAIR) cd.dc = pair_types[t].first | (pair_types[t].second << 8);
-        if (kind == CK_DENS) {
-          if (dens_c0 < 0) dens_c0 = col;
-          cd.dc = Dd++;
+    // column descriptors and lane-slot maps: built and uploaded once per model
+    const bool adp = c.kind == FS_ADP;
+    int Dd = 0, dens_c0 = 0, gi_nf = 0, NS = 0, NSP = 0, NSD = 0;
+    if (!c.jl.valid) {
+      // column descriptors
+      std::vector<ColDesc> cols(std::max(c.D, 1));
+      std::vector<std::pair<int, int>> pair_types(std::max(c.npair, 1));
+      for (int ta = 0; ta < c.ntypes; ++ta)
+        for (int tb = ta; tb < c.ntypes; ++tb) pair_types[ord_pair(ta, tb, c.ntypes)] = {ta, tb};
+      dens_c0 = -1;
+      Dd = 0;
+      for (int t = 0; t < c.ntab; ++t) {
+        const TableDesc &T = c.h_tabs[t];
+        int kind = CK_PAIR;
+        if (t >= c.npair && t < c.npair + c.ntypes) kind = CK_DENS;
+        else if (t >= c.npair + c.ntypes && t < c.npair + 2 * c.ntypes) kind = CK_EMB;
+        else if (t >= c.npair + 2 * c.ntypes && t < 2 * c.npair + 2 * c.ntypes) kind = CK_DIP;
+        else if (t >= 2 * c.npair + 2 * c.ntypes) kind = CK_QUAD;
+        for (int m = 0; m < T.n; ++m) {
+          const int col = c.h_col_of[T.off + m];
+          if (col < 0) continue;
+          ColDesc cd{t, m, kind, -1};
+          if (kind == CK_PAIR) cd.dc = pair_types[t].first | (pair_types[t].second << 8);
+          if (kind == CK_DENS) {
+            if (dens_c0 < 0) dens_c0 = col;
+            cd.dc = Dd++;
+          }
+          cols[col] = cd;
         }
-        cols[col] = cd;
       }
+      if (dens_c0 < 0) dens_c0 = 0;
+      static_assert(sizeof(ColDesc) == sizeof(int4), "column descriptor layout");
+      c.jcols.alloc(cols.size());
+      FS_HIP(hipMemcpyAsync(c.jcols.p, cols.data(), sizeof(ColDesc) * cols.size(), hipMemcpyHostToDevice,
+                            c.stream));
+      FS_HIP(hipStreamSynchronize(c.stream));
+      // the balanced g_ti windows need every density table to contribute the same
+      // number of free columns, laid out table by table in one lane slot
+      {
+        std::vector<int> nfree(c.ntypes, 0);
+        for (int col = 0; col < c.D; ++col)
+          if (cols[col].kind == CK_DENS) ++nfree[cols[col].tab - c.npair];
+        bool equal = c.ntypes > 0;
+        for (int t = 1; t < c.ntypes; ++t) equal = equal && nfree[t] == nfree[0];
+        gi_nf = (equal && nfree[0] > 0 && nfree[0] * c.ntypes <= 64) ? nfree[0] : 0;
+      }
+      // lane slots: pair, density and embedding columns each start a new slot.
+      // ADP: one map for its tangent kernels. EAM: the pair-table columns and
+      // the density/embedding columns form the maps of the two k_jac_main passes.
+      auto build_slots = [&](std::initializer_list<int> kinds) {
+        std::vector<SlotDesc> sl;
+        for (int kind : kinds) {
+          size_t n = 0;
+          for (int col = 0; col < c.D; ++col) {
+            if (cols[col].kind != kind) continue;
+            if (n % 64 == 0) sl.resize(sl.size() + 64, SlotDesc{-1, 0, CK_NONE, -1, 0, 0, 0, 0});
+            const ColDesc &cd = cols[col];
+            sl[sl.size() - 64 + n % 64] = SlotDesc{cd.tab, cd.m, cd.kind, cd.dc, col, 0, 0, 0};
+            ++n;
+          }
+        }
+        return sl;
+      };
+      auto pad_to = [](std::vector<SlotDesc> &sl, int ns) {
+        sl.resize(size_t(ns) * 64, SlotDesc{-1, 0, CK_NONE, -1, 0, 0, 0, 0});
+      };
+      std::vector<SlotDesc> slots, sde;
+      if (adp) {
+        slots = build_slots({CK_PAIR, CK_DENS, CK_EMB, CK_DIP, CK_QUAD});
+        const int ns = int(slots.size() / 64);
+        FS_REQUIRE(ns <= 8, FS_ERR_UNSUPPORTED, "Jacobian columns need too many lane slots");
+        NS = ns <= 4 ? 4 : 8;
+        pad_to(slots, NS);
+      } else {
+        slots = build_slots({CK_PAIR});
+        sde = build_slots({CK_DENS, CK_EMB});
+        const int nsp = int(slots.size() / 64), nsd = int(sde.size() / 64);
+        FS_REQUIRE(nsp <= 4 && nsd <= 6, FS_ERR_UNSUPPORTED, "Jacobian columns need too many lane slots");
+        NSP = nsp == 0 ? 0 : (nsp <= 2 ? 2 : 4);
+        NSD = nsd == 0 ? 0 : (nsd <= 2 ? 2 : (nsd <= 4 ? 4 : 6));
+        pad_to(slots, NSP);
+        pad_to(sde, NSD);
+        slots.insert(slots.end(), sde.begin(), sde.end());
+      }
+      static_assert(sizeof(SlotDesc) == 2 * sizeof(int4), "slot descriptor layout");
+      c.jslots.alloc(std::max<size_t>(slots.size(), 1) * 2);
+      FS_HIP(hipMemcpyAsync(c.jslots.p, slots.data(), sizeof(SlotDesc) * slots.size(),
+                            hipMemcpyHostToDevice, c.stream));
+      FS_HIP(hipStreamSynchronize(c.stream));
+
+      c.jl = Context::JacLayout{true, Dd, dens_c0, gi_nf, NS, NSP, NSD};
     }
-    if (dens_c0 < 0) dens_c0 = 0;
-    static_assert(sizeof(ColDesc) == sizeof(int4), "column descriptor layout");
-    c.jcols.alloc(cols.size());
-    FS_HIP(hipMemcpyAsync(c.jcols.p, cols.data(), sizeof(ColDesc) * cols.size(), hipMemcpyHostToDevice,
-                          c.stream));
-    FS_HIP(hipStreamSynchronize(c.stream));
+    Dd = c.jl.Dd;
+    dens_c0 = c.jl.dens_c0;
+    gi_nf = c.jl.gi_nf;
+    NS = c.jl.NS;
+    NSP = c.jl.NSP;
+    NSD = c.jl.NSD;
     a.Dd = Dd;
-    // the balanced g_ti windows need every density table to contribute the same
-    // number of free columns, laid out table by table in one lane slot
-    {
-      std::vector<int> nfree(c.ntypes, 0);
-      for (int col = 0; col < c.D; ++col)
-        if (cols[col].kind == CK_DENS) ++nfree[cols[col].tab - c.npair];
-      bool equal = c.ntypes > 0;
-      for (int t = 1; t < c.ntypes; ++t) equal = equal && nfree[t] == nfree[0];
-      a.gi_nf = (equal && nfree[0] > 0 && nfree[0] * c.ntypes <= 64) ? nfree[0] : 0;
-    }
+    a.gi_nf = gi_nf;
     FS_REQUIRE(int64_t(c.N) * std::max(Dd, 1) < (int64_t(1) << 31), FS_ERR_UNSUPPORTED,
                "density-column table exceeds 2^31 entries");
     a.dens_c0 = dens_c0;
@@ -1619,54 +1681,9 @@ void launch_jacobian(Context &c) {
     a.Fpp = c.Fpp.p;
     const size_t base = size_t(c.ntab) * sizeof(TableDesc) + ((size_t(c.nknots) * 5 + 3) & ~size_t(3)) * 4;
     const size_t lds_dens = base + 4 * sizeof(DensStage);
-    const bool adp = c.kind == FS_ADP;
-    // lane slots: pair, density and embedding columns each start a new slot.
-    // ADP: one map for its tangent kernels. EAM: the pair-table columns and
-    // the density/embedding columns form the maps of the two k_jac_main passes.
-    auto build_slots = [&](std::initializer_list<int> kinds) {
-      std::vector<SlotDesc> sl;
-      for (int kind : kinds) {
-        size_t n = 0;
-        for (int col = 0; col < c.D; ++col) {
-          if (cols[col].kind != kind) continue;
-          if (n % 64 == 0) sl.resize(sl.size() + 64, SlotDesc{-1, 0, CK_NONE, -1, 0, 0, 0, 0});
-          const ColDesc &cd = cols[col];
-          sl[sl.size() - 64 + n % 64] = SlotDesc{cd.tab, cd.m, cd.kind, cd.dc, col, 0, 0, 0};
-          ++n;
-        }
-      }
-      return sl;
-    };
-    auto pad_to = [](std::vector<SlotDesc> &sl, int ns) {
-      sl.resize(size_t(ns) * 64, SlotDesc{-1, 0, CK_NONE, -1, 0, 0, 0, 0});
-    };
-    std::vector<SlotDesc> slots, sde;
-    int NS = 0, NSP = 0, NSD = 0;
-    if (adp) {
-      slots = build_slots({CK_PAIR, CK_DENS, CK_EMB, CK_DIP, CK_QUAD});
-      const int ns = int(slots.size() / 64);
-      FS_REQUIRE(ns <= 8, FS_ERR_UNSUPPORTED, "Jacobian columns need too many lane slots");
-      NS = ns <= 4 ? 4 : 8;
-      pad_to(slots, NS);
-    } else {
-      slots = build_slots({CK_PAIR});
-      sde = build_slots({CK_DENS, CK_EMB});
-      const int nsp = int(slots.size() / 64), nsd = int(sde.size() / 64);
-      FS_REQUIRE(nsp <= 4 && nsd <= 6, FS_ERR_UNSUPPORTED, "Jacobian columns need too many lane slots");
-      NSP = nsp == 0 ? 0 : (nsp <= 2 ? 2 : 4);
-      NSD = nsd == 0 ? 0 : (nsd <= 2 ? 2 : (nsd <= 4 ? 4 : 6));
-      pad_to(slots, NSP);
-      pad_to(sde, NSD);
-      slots.insert(slots.end(), sde.begin(), sde.end());
-    }
     const size_t lds_pair = base + 4 * sizeof(JStagePair), lds_de = base + 4 * sizeof(JStageDE);
     FS_REQUIRE(lds_de <= 64 * 1024 && lds_pair <= 64 * 1024, FS_ERR_UNSUPPORTED,
                "Jacobian LDS budget exceeded");
-    static_assert(sizeof(SlotDesc) == 2 * sizeof(int4), "slot descriptor layout");
-    c.jslots.alloc(std::max<size_t>(slots.size(), 1) * 2);
-    FS_HIP(hipMemcpyAsync(c.jslots.p, slots.data(), sizeof(SlotDesc) * slots.size(),
-                          hipMemcpyHostToDevice, c.stream));
-    FS_HIP(hipStreamSynchronize(c.stream));
     const SlotDesc *d_slots = reinterpret_cast<const SlotDesc *>(c.jslots.p);
     if (!c.jsort_valid && c.N > 0) {
       c.jrec.alloc(size_t(std::max<int64_t>(c.P, 1)));
