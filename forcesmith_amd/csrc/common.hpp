@@ -142,10 +142,6 @@ struct Context {
   int device = 0;
   int precision = FS_MIXED;
   hipStream_t stream = nullptr;
-  // second stream and fork/join events: the Jacobian's pair-column pass runs
-  // beside the embedding-site and density kernels (default priority)
-  hipStream_t stream2 = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int cu_count = 256;
   int lanes_per_atom = 16;
   int blocks_per_cu = 8;
@@ -268,12 +264,12 @@ struct Context {
   int jac_main_n = 0;
 
   ~Context();
-  void time_main_begin(int stage = 0, hipStream_t s = nullptr) {
-    if (time_jac_main && stage == time_which) FS_HIP(hipEventRecord(jev0, s ? s : stream));
+  void time_main_begin(int stage = 0) {
+    if (time_jac_main && stage == time_which) FS_HIP(hipEventRecord(jev0, stream));
   }
-  void time_main_end(int stage = 0, hipStream_t s = nullptr) {
+  void time_main_end(int stage = 0) {
     if (!time_jac_main || stage != time_which) return;
-    FS_HIP(hipEventRecord(jev1, s ? s : stream));
+    FS_HIP(hipEventRecord(jev1, stream));
     FS_HIP(hipEventSynchronize(jev1));
     float ms = 0.f;
     FS_HIP(hipEventElapsedTime(&ms, jev0, jev1));

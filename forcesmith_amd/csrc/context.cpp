@@ -19,9 +19,6 @@ Context::~Context() {
   destroy_surrogate(surr);
   if (jev0) (void)hipEventDestroy(jev0);
   if (jev1) (void)hipEventDestroy(jev1);
-  if (ev_fork) (void)hipEventDestroy(ev_fork);
-  if (ev_join) (void)hipEventDestroy(ev_join);
-  if (stream2) (void)hipStreamDestroy(stream2);
   if (stream) (void)hipStreamDestroy(stream);
 }
 

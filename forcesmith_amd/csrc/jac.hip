@@ -1454,10 +1454,13 @@ void launch_jacobian(Context &c) {
   c.J.alloc(size_t(std::max<int64_t>(R * c.D, 1)));
   // Jacobian tiles: <= 64 consecutive atoms of one configuration
   if (c.njt == 0) {
-    // tile size: enough tiles for about 16 waves per CU (the ADP and SW tile
-    // kernels run one wave per tile), between 4 and 64 atoms, a multiple of 4
+    // tile size: 64 atoms when there are at least 32 per wave slot of the chip
+    // (16 waves per CU), else the power of two (>= 4) that gives about 16
+    // waves per CU (the ADP and SW tile kernels run one wave per tile)
     const int64_t want = c.N / std::max<int64_t>(1, int64_t(c.cu_count) * 16);
-    const int jt = int(std::max<int64_t>(4, std::min<int64_t>(kJacTile, (want + 3) & ~int64_t(3))));
+    int jt = 4;
+    while (jt < kJacTile && jt < want) jt *= 2;
+    if (want >= kJacTile / 2) jt = kJacTile;
     std::vector<int32_t> tcfg, ta0, tna, ct0(c.C + 1, 0);
     for (int k = 0; k < c.C; ++k) {
       ct0[k] = int32_t(tcfg.size());
@@ -1718,28 +1721,21 @@ void launch_jacobian(Context &c) {
         c.time_main_end();
       }
     } else if (c.N > 0) {
-      // fork: the pair-column pass needs only the sorted pair records and the
-      // tables, so it runs on the second stream beside the embedding-site and
-      // density kernels; k_jac_rows joins it
-      if (!c.stream2) FS_HIP(hipStreamCreateWithFlags(&c.stream2, hipStreamNonBlocking));
-      if (!c.ev_fork) FS_HIP(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
-      if (!c.ev_join) FS_HIP(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
-      FS_HIP(hipEventRecord(c.ev_fork, c.stream));
-      FS_HIP(hipStreamWaitEvent(c.stream2, c.ev_fork, 0));
+      // (the pair pass depends on neither kernel below; run beside them on a
+      // second stream it measured no faster and stretched both)
       const unsigned g2 = unsigned(std::max(1, std::min(nt, c.cu_count * 8)));
       const SlotDesc *d_de = d_slots + size_t(NSP) * 64;
-      c.time_main_begin(1, c.stream2);
+      c.time_main_begin(1);
       // wider maps (more than 128 columns of a pass) keep the compiler's
       // occupancy rather than spill their larger slot state
       if (NSP == 2)
-        hipLaunchKernelGGL((k_jac_pair<2, kWavesPair>), dim3(g2), dim3(256), lds_pair, c.stream2, a,
+        hipLaunchKernelGGL((k_jac_pair<2, kWavesPair>), dim3(g2), dim3(256), lds_pair, c.stream, a,
                            c.nknots, d_slots);
       else if (NSP == 4)
-        hipLaunchKernelGGL((k_jac_pair<4, 2>), dim3(g2), dim3(256), lds_pair, c.stream2, a, c.nknots,
+        hipLaunchKernelGGL((k_jac_pair<4, 2>), dim3(g2), dim3(256), lds_pair, c.stream, a, c.nknots,
                            d_slots);
       FS_HIP(hipGetLastError());
-      c.time_main_end(1, c.stream2);
-      FS_HIP(hipEventRecord(c.ev_join, c.stream2));
+      c.time_main_end(1);
       hipLaunchKernelGGL(k_emb_sites, dim3(nb(c.N, 256)), dim3(256), 0, c.stream, a);
       FS_HIP(hipGetLastError());
       const unsigned g1 = unsigned(std::min<int64_t>((c.N + 3) / 4, int64_t(c.cu_count) * 16));
@@ -1755,7 +1751,6 @@ void launch_jacobian(Context &c) {
         hipLaunchKernelGGL((k_jac_de<6, 1>), dim3(g2), dim3(256), lds_de, c.stream, a, c.nknots, d_de);
       FS_HIP(hipGetLastError());
       c.time_main_end();
-      FS_HIP(hipStreamWaitEvent(c.stream, c.ev_join, 0));
     }
   }
   hipLaunchKernelGGL(k_jac_rows, dim3(c.C), dim3(256), 0, c.stream, a);
