@@ -235,7 +235,7 @@ struct Context {
   // hold them on the device); rebuilt after every model change
   struct JacLayout {
     bool valid = false;
-    int Dd = 0, dens_c0 = 0, gi_nf = 0, NS = 0, NSP = 0, NSD = 0;
+    int Dd = 0, dens_c0 = 0, gi_nf = 0, NSP = 0, NSD = 0, NSU = 0, NSW = 0, Ddip = 0, Dquad = 0;
   } jl;
   DBuf<float4> jrec; // pair records sorted by (partner type, r) per atom
   DBuf<float> jr;    // their FP32 distances

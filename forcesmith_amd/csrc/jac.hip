@@ -19,13 +19,16 @@
 //   k_jac_dens   one wave per atom: H_i[c] = d rho_i / d y_c over the density
 //                columns, Hs_i = F''(rho_i) H_i (0 when the density is clamped),
 //                and the embedding window of atom i.
-//   k_jac_main   one wave per atom, lanes own columns (grouped by kind into
-//                slots): each 64-pair chunk stages per-pair band coefficients
-//                of every role in LDS, then each column sums its window; FP32
+//   k_jac_pair,  one wave per atom, lanes own columns (grouped by kind into
+//   k_jac_de     slots; pair-table columns, then density and embedding
+//                columns): each 64-pair chunk stages per-pair band
+//                coefficients in LDS, then each column sums its window; FP32
 //                force rows are written, energy / virial / limit partials are
 //                reduced per tile in FP64.
 //   k_jac_rows   per configuration: energy, stress and limit rows from the
 //                tile partials, in tile order (deterministic, no atomics).
+// ADP: the pair-table, density and embedding columns run through the same
+// passes; k_adp_mom and k_adp_dq add the dipole and quadrupole columns.
 // Stillinger-Weber: forward-mode tangents of the reference's pair and triplet
 // formulas (stiweb_force.cpp:24-54, kernels.hpp:40-68), one column per grid
 // row, with the triplet j/k forces gathered through the reverse pair index.
@@ -126,7 +129,8 @@ struct JacArgs {
   float *emb_d, *emb_v;   // [N x 4]
   double *Fpp;            // [N]
   const double *mu, *lamb; // ADP per-atom dipole (3) and quadrupole (6)
-  float *tan;             // ADP tangents [N][10][D]
+  int Ddip, Dquad;        // ADP dipole / quadrupole free columns
+  float *dmu, *dlam;      // ADP moment tangents [N][Ddip][4], [N][Dquad][6]
   float *J;               // [R x D]
   double *part;           // [ntiles x kJP x D]
   const double *vol;
@@ -353,7 +357,7 @@ __global__ __launch_bounds__(256) void k_jac_dens(JacArgs a, int nknots) {
 }
 
 // ---------------------------------------------------------------------------
-// k_jac_main: one wave per atom, lanes own columns, columns grouped by kind
+// The Jacobian passes: one wave per atom, lanes own columns, grouped by kind
 // into slots (one slot holds pair, density or embedding columns only, so a
 // slot's loop does not diverge). Per 64-pair chunk every lane first stages
 // its pair: unit vector, distance, and for each role of the pair (pair table,
@@ -396,7 +400,9 @@ template <> struct JStageOf<JP_DE> { using T = JStageDE; };
 
 struct SlotDesc {
   int32_t tab, m, kind, dc; // as ColDesc; dc = density column or pair-table types
-  int32_t col, pad0, pad1, pad2;
+  int32_t col;              // Jacobian column
+  int32_t qc;               // ADP dipole / quadrupole column: index among its kind
+  int32_t pad0, pad1;
 };
 
 // per-atom force accumulator of one column; the column's energy, virial and
@@ -767,118 +773,154 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
 }
 
 // ---------------------------------------------------------------------------
-// ADP Jacobian: forward-mode tangents per column, one wave per atom, lanes
-// own columns (kind-homogeneous slots). The chain is the one of the
-// reference's ADP parameter Jacobian (ADPForceCalculator::write_param_jacobian,
-// src/force/adp_force.cpp:141-385) with spline band coefficients:
-//   pass 1 (k_adp_tan1): d rho_i, d mu_i (3), d lambda_i (6) per column;
-//   pass 2 (k_adp_tan2): d F'_i = F''_i d rho_i (0 when clamped) + the
-//   embedding band, the pair-table, density, embedding, dipole and quadrupole
-//   terms of every pair's force, the energy (F' d rho + mu.d mu +
-//   lambda:d lambda - tr lambda tr d lambda / 3 + half d phi), virial and
-//   limit (over d rho) tangents.
+// ADP Jacobian. The pair-table, density and embedding columns are those of the
+// EAM passes unchanged (the dipole and quadrupole terms do not depend on the
+// phi, rho and F tables). The dipole (u) and quadrupole (w) columns follow the
+// chain of the reference's ADP parameter Jacobian
+// (ADPForceCalculator::write_param_jacobian, src/force/adp_force.cpp:141-385)
+// with spline band coefficients, in two kernels per kind:
+//   k_adp_mom: per atom and column, d mu_i = sum_j du(r) d (dipole) or
+//     d lambda_i = sum_j dw(r) d d (quadrupole), over the column's window of
+//     the sorted pairs;
+//   k_adp_dq: per atom and column, the force tangent of every pair term of
+//     f_ij = (u'/r)(dmu . d) d + u dmu + (w'/r) nu(dLambda, d) d
+//            + 2 w xi(dLambda, d)
+//     (dmu = mu_i - mu_j, Lambda = lambda_i + lambda_j; adp_force.cpp:73-135):
+//     the band terms of the column's own table over its window, and the
+//     moment-tangent terms over every pair, with the neighbours' d mu_j /
+//     d lambda_j gathered in batches; energy (mu_i . d mu_i and
+//     lambda_i : d lambda_i - tr lambda_i tr d lambda_i / 3) and virial.
 // ---------------------------------------------------------------------------
-constexpr int kTan = 10; // d rho, d mu (3), d lambda (xx, yy, zz, xy, xz, yz)
+enum { JP_DIP = 3, JP_QUAD = 4 };
+constexpr int kQBatch = 4; // neighbour moment tangents gathered per batch
+// moment-tangent rows: [atom][column][comp], dipole padded to a float4,
+// quadrupole as three float2, so that one lane reads its column's tangent
+// with one or three loads and a wave's gather is one contiguous span
 
-struct AStage1 {
+template <bool DIP> __device__ inline void store_moment(float *base, int64_t row, const float *v) {
+  if constexpr (DIP) {
+    reinterpret_cast<float4 *>(base)[row] = make_float4(v[0], v[1], v[2], 0.0f);
+  } else {
+    float2 *d = reinterpret_cast<float2 *>(base) + 3 * row;
+    d[0] = make_float2(v[0], v[1]);
+    d[1] = make_float2(v[2], v[3]);
+    d[2] = make_float2(v[4], v[5]);
+  }
+}
+
+template <bool DIP> __device__ inline void load_moment(const float *base, int64_t row, float *v) {
+  if constexpr (DIP) {
+    const float4 q = reinterpret_cast<const float4 *>(base)[row];
+    v[0] = q.x;
+    v[1] = q.y;
+    v[2] = q.z;
+  } else {
+    const float2 *d = reinterpret_cast<const float2 *>(base) + 3 * row;
+    const float2 a = d[0], b = d[1], c = d[2];
+    v[0] = a.x;
+    v[1] = a.y;
+    v[2] = b.x;
+    v[3] = b.y;
+    v[4] = c.x;
+    v[5] = c.y;
+  }
+}
+
+struct MomStage {
   float d0[64], d1[64], d2[64];
-  int32_t tg[64], tu[64], tw[64], kg[64], ku[64], kw[64];
-  float cg[6][64], cu[6][64], cw[6][64];
+  int32_t k[64];
+  float c[6][kBandPitch];
 };
 
-template <int NS>
-__global__ __launch_bounds__(64) void k_adp_tan1(JacArgs a, int nknots, const SlotDesc *slots) {
+// table and pair flag of the dipole (PASS == JP_DIP) or quadrupole pass
+template <int PASS> __device__ inline int dq_tab0(const JacArgs &a) {
+  return PASS == JP_DIP ? a.npair + 2 * a.ntypes : 2 * a.npair + 2 * a.ntypes;
+}
+template <int PASS> constexpr uint32_t dq_flag() { return PASS == JP_DIP ? kFlagDip : kFlagQuad; }
+
+// partner type of a (ta, tb) pair-table column for centre type ti (-1: not ti's table)
+__device__ inline int partner_type(int pt, int ti) {
+  const int ta = pt & 0xff, tb = (pt >> 8) & 0xff;
+  return ti == ta ? tb : (ti == tb ? ta : -1);
+}
+
+template <int PASS, int NS>
+__global__ __launch_bounds__(256) void k_adp_mom(JacArgs a, int nknots, const SlotDesc *slots) {
+  constexpr bool kDip = PASS == JP_DIP;
+  constexpr int NC = kDip ? 3 : 6;
   extern __shared__ float4 smem[];
   TableDesc *sT = reinterpret_cast<TableDesc *>(smem);
   float *sS = reinterpret_cast<float *>(smem + a.ntab * int(sizeof(TableDesc) / sizeof(float4)));
-  AStage1 &P = *reinterpret_cast<AStage1 *>(sS + ((nknots * 5 + 3) & ~3));
+  MomStage *sAll = reinterpret_cast<MomStage *>(sS + ((nknots * 5 + 3) & ~3));
   stage(a, sT, sS, nknots);
-  const int lane = threadIdx.x;
-  const int dip0 = a.npair + 2 * a.ntypes, quad0 = 2 * a.npair + 2 * a.ntypes;
-  int ccol[NS], ctab[NS], cm[NS], ckind[NS];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  MomStage &P = sAll[wave];
+  int cm[NS], cqc[NS], cpt[NS];
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
     const SlotDesc sd = slots[s * 64 + lane];
-    ccol[s] = sd.col;
-    ctab[s] = sd.tab;
     cm[s] = sd.m;
-    ckind[s] = sd.kind;
+    cqc[s] = sd.kind == CK_NONE ? -1 : sd.qc;
+    cpt[s] = sd.dc;
   }
-  for (int64_t i = blockIdx.x; i < a.N; i += gridDim.x) {
+  const int tab0 = dq_tab0<PASS>(a);
+  float *out = kDip ? a.dmu : a.dlam;
+  const int Dq = kDip ? a.Ddip : a.Dquad;
+  const int64_t gw = int64_t(blockIdx.x) * 4 + wave;
+  const int64_t nw = int64_t(gridDim.x) * 4;
+  for (int64_t i = gw; i < a.N; i += nw) {
     const int ti = a.types[i];
     const int64_t p0 = a.pair_off[i], p1 = a.pair_off[i + 1];
-    float T[NS][kTan];
+    int ptt[NS];
+    float T[NS][NC];
 #pragma unroll
-    for (int s = 0; s < NS; ++s)
+    for (int s = 0; s < NS; ++s) {
+      ptt[s] = cqc[s] < 0 ? -1 : partner_type(cpt[s], ti);
 #pragma unroll
-      for (int q = 0; q < kTan; ++q) T[s][q] = 0.0f;
+      for (int q = 0; q < NC; ++q) T[s][q] = 0.0f;
+    }
     for (int64_t pc = p0; pc < p1; pc += 64) {
       const int64_t p = pc + lane;
+      int ty = kMaxTypes;
       if (p < p1) {
-        const float4 q = a.rec[p];
+        const float4 q = a.jrec[p];
         const uint32_t b = __float_as_uint(q.w);
-        const bool tiny = (b & kFlagTiny) != 0;
-        const float r = sqrtf(q.x * q.x + q.y * q.y + q.z * q.z);
-        const int tj = pj_type(b), o = ord_pair(ti, tj, a.ntypes);
+        const float rr = a.jr[p];
+        const int tj = pj_type(b);
+        ty = tj;
+        const TableDesc Tq = sT[tab0 + ord_pair(ti, tj, a.ntypes)];
+        float bv[4], bd[4], c[6];
+        const int k = pair_basis(Tq, rr, bv, bd);
+        band6(sS, Tq, k, bv, ((b & dq_flag<PASS>()) && !(b & kFlagTiny)) ? 1.0f : 0.0f, c);
+        P.k[lane] = k;
         P.d0[lane] = q.x;
         P.d1[lane] = q.y;
         P.d2[lane] = q.z;
-        float bv[4], bd[4], c[6];
-        const int tg = (!tiny && (b & kFlagGj)) ? a.npair + tj : -1;
-        const int tu = (!tiny && (b & kFlagDip)) ? dip0 + o : -1;
-        const int tw = (!tiny && (b & kFlagQuad)) ? quad0 + o : -1;
-        P.tg[lane] = tg;
-        P.tu[lane] = tu;
-        P.tw[lane] = tw;
-        if (tg >= 0) {
-          P.kg[lane] = pair_basis(sT[tg], r, bv, bd);
-          band6(sS, sT[tg], P.kg[lane], bv, 1.0f, c);
-          for (int d = 0; d < 6; ++d) P.cg[d][lane] = c[d];
-        }
-        if (tu >= 0) {
-          P.ku[lane] = pair_basis(sT[tu], r, bv, bd);
-          band6(sS, sT[tu], P.ku[lane], bv, 1.0f, c);
-          for (int d = 0; d < 6; ++d) P.cu[d][lane] = c[d];
-        }
-        if (tw >= 0) {
-          P.kw[lane] = pair_basis(sT[tw], r, bv, bd);
-          band6(sS, sT[tw], P.kw[lane], bv, 1.0f, c);
-          for (int d = 0; d < 6; ++d) P.cw[d][lane] = c[d];
-        }
+#pragma unroll
+        for (int d = 0; d < 6; ++d) P.c[d][lane] = c[d];
       }
+      int seg[kMaxTypes + 1];
+      chunk_segments(ty, a.ntypes, seg);
       wave_sync();
-      const int np = int(min<int64_t>(64, p1 - pc));
 #pragma unroll
       for (int s = 0; s < NS; ++s) {
-        const int kind = ckind[s], m = cm[s], tab = ctab[s];
-        if (kind == CK_DENS) {
-          for (int q = 0; q < np; ++q) {
-            if (P.tg[q] != tab) continue;
-            const int d = m - P.kg[q] + 2;
-            if (d >= 0 && d < 6) T[s][0] += P.cg[d][q];
-          }
-        } else if (kind == CK_DIP) {
-          for (int q = 0; q < np; ++q) {
-            if (P.tu[q] != tab) continue;
-            const int d = m - P.ku[q] + 2;
-            if (d < 0 || d > 5) continue;
-            const float v = P.cu[d][q];
-            T[s][1] += v * P.d0[q];
-            T[s][2] += v * P.d1[q];
-            T[s][3] += v * P.d2[q];
-          }
-        } else if (kind == CK_QUAD) {
-          for (int q = 0; q < np; ++q) {
-            if (P.tw[q] != tab) continue;
-            const int d = m - P.kw[q] + 2;
-            if (d < 0 || d > 5) continue;
-            const float v = P.cw[d][q], x = P.d0[q], y = P.d1[q], z = P.d2[q];
-            T[s][4] += v * (x * x);
-            T[s][5] += v * (y * y);
-            T[s][6] += v * (z * z);
-            T[s][7] += v * (x * y);
-            T[s][8] += v * (x * z);
-            T[s][9] += v * (y * z);
+        if (ptt[s] < 0) continue;
+        const int m = cm[s], s1 = seg[ptt[s] + 1];
+        const int lo = first_ge(P.k, seg[ptt[s]], s1, m - 3);
+        const int hi = first_ge(P.k, lo, s1, m + 3);
+        for (int q = lo; q < hi; ++q) {
+          const float v = P.c[m - P.k[q] + 2][q], x = P.d0[q], y = P.d1[q], z = P.d2[q];
+          if constexpr (kDip) {
+            T[s][0] += v * x;
+            T[s][1] += v * y;
+            T[s][2] += v * z;
+          } else {
+            T[s][0] += v * (x * x);
+            T[s][1] += v * (y * y);
+            T[s][2] += v * (z * z);
+            T[s][3] += v * (x * y);
+            T[s][4] += v * (x * z);
+            T[s][5] += v * (y * z);
           }
         }
       }
@@ -886,22 +928,19 @@ __global__ __launch_bounds__(64) void k_adp_tan1(JacArgs a, int nknots, const Sl
     }
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-      if (ckind[s] == CK_NONE) continue;
-      for (int q = 0; q < kTan; ++q) a.tan[(i * kTan + q) * a.D + ccol[s]] = T[s][q];
+      if (cqc[s] < 0) continue;
+      store_moment<kDip>(out, i * Dq + cqc[s], T[s]);
     }
-    if (lane == 0) (void)emb_window(a, sT, i);
   }
 }
 
-struct AStage2 {
-  float r[64], d0[64], d1[64], d2[64];
-  float dgj[64], dgi[64], u[64], du[64], w[64], dw[64];
-  float fpj[64], fppj[64], muj[3][64], nu[64], xi[3][64];
-  int32_t jg[64], tj[64], kfj[64], tp[64], tgj[64], tgi[64], tu[64], tw[64];
-  int32_t kp[64], kgj[64], kgi[64], ku[64], kw[64];
-  float ebj[4][64];
-  float cpd[6][64], cpv[6][64], cgj[6][64], cgi[6][64], cud[6][64], cuv[6][64], cwd[6][64],
-      cwv[6][64];
+struct DQStage {
+  float x[64], y[64], z[64];
+  float g0[64], g1[64], g2[64]; // dipole: mu_i - mu_j; quadrupole: xi(Lambda, d)
+  float sg[64];                 // dipole: (mu_i - mu_j) . d / r; quadrupole: nu(Lambda, d) / r
+  float v[64], dv[64];          // u (w) and u'/r (w'/r); 0 when the role is off
+  int32_t jg[64], k[64];
+  float cd[6][kBandPitch], cv[6][kBandPitch]; // derivative and value bands
 };
 
 // nu(L, d) and xi(L, d) of a symmetric L stored as (xx, yy, zz, xy, xz, yz)
@@ -917,247 +956,236 @@ __device__ inline float qnu(const float *L, float x, float y, float z, float xi[
   return (x * Mx + y * My + z * Mz) - (x * x + y * y + z * z) * tr3;
 }
 
-template <int NS>
-__global__ __launch_bounds__(64) void k_adp_tan2(JacArgs a, int nknots, const SlotDesc *slots) {
+__device__ inline void add_dvirial(float *t, float x, float y, float z, float f0, float f1, float f2) {
+  t[1] += x * f0;
+  t[2] += y * f1;
+  t[3] += z * f2;
+  t[4] += x * f1;
+  t[5] += x * f2;
+  t[6] += y * f2;
+}
+
+template <int PASS, int NS>
+__global__ __launch_bounds__(256) void k_adp_dq(JacArgs a, int nknots, const SlotDesc *slots) {
+  constexpr bool kDip = PASS == JP_DIP;
+  constexpr int NC = kDip ? 3 : 6;
   extern __shared__ float4 smem[];
   TableDesc *sT = reinterpret_cast<TableDesc *>(smem);
   float *sS = reinterpret_cast<float *>(smem + a.ntab * int(sizeof(TableDesc) / sizeof(float4)));
-  AStage2 &P = *reinterpret_cast<AStage2 *>(sS + ((nknots * 5 + 3) & ~3));
+  DQStage *sAll = reinterpret_cast<DQStage *>(sS + ((nknots * 5 + 3) & ~3));
+  static_assert(4 * sizeof(DQStage) >= sizeof(double) * kJP * 256, "reduction buffer");
+  double(*sPart)[256] = reinterpret_cast<double(*)[256]>(sAll);
   stage(a, sT, sS, nknots);
-  const int lane = threadIdx.x;
-  const int dip0 = a.npair + 2 * a.ntypes, quad0 = 2 * a.npair + 2 * a.ntypes;
-  const int emb0 = a.npair + a.ntypes;
-  int ccol[NS], ctab[NS], cm[NS], ckind[NS];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  DQStage &P = sAll[wave];
+  int ccol[NS], cm[NS], cqc[NS], cpt[NS];
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
     const SlotDesc sd = slots[s * 64 + lane];
     ccol[s] = sd.col;
-    ctab[s] = sd.tab;
     cm[s] = sd.m;
-    ckind[s] = sd.kind;
+    cqc[s] = sd.kind == CK_NONE ? -1 : sd.qc;
+    cpt[s] = sd.dc;
   }
-  const int tile = blockIdx.x;
-  const int na = a.tile_na[tile];
-  const int c = a.tile_cfg[tile];
-  const int64_t cbase = a.atom_off[c];
-  float tp[NS][kJP];
+  const int tab0 = dq_tab0<PASS>(a);
+  const float *mom = kDip ? a.dmu : a.dlam;
+  const int Dq = kDip ? a.Ddip : a.Dquad;
+  // XCD-aware tiles (as in the EAM passes): the neighbours' moment tangents
+  // stay in the XCD's L2
+  const int ng = min(kXcds, int(gridDim.x));
+  const int xcd = blockIdx.x % ng, bslot = blockIdx.x / ng;
+  const int nslot = (int(gridDim.x) - xcd + ng - 1) / ng;
+  const int t_end = int(int64_t(xcd + 1) * a.ntiles / ng);
+  for (int tile = int(int64_t(xcd) * a.ntiles / ng) + bslot; tile < t_end; tile += nslot) {
+    const int na = a.tile_na[tile];
+    const int c = a.tile_cfg[tile];
+    const int64_t cbase = a.atom_off[c];
+    float tp[NS][kJP];
 #pragma unroll
-  for (int s = 0; s < NS; ++s)
+    for (int s = 0; s < NS; ++s)
 #pragma unroll
-    for (int q = 0; q < kJP; ++q) tp[s][q] = 0.0f;
-  for (int li = 0; li < na; ++li) {
-    const int64_t i = int64_t(a.tile_a0[tile]) + li;
-    const int ti = a.types[i];
-    const bool clamped = a.over[i] != 0.0;
-    const float Fpi = a.Fp32[i], Fppi = clamped ? 0.0f : float(a.Fpp[i]);
-    const float mui[3] = {float(a.mu[3 * i]), float(a.mu[3 * i + 1]), float(a.mu[3 * i + 2])};
-    float Li[6];
-    for (int q = 0; q < 6; ++q) Li[q] = float(a.lamb[6 * i + q]);
-    const int kfi = a.emb_k[i];
-    float ebi[4], evi[4];
-    for (int q = 0; q < 4; ++q) {
-      ebi[q] = a.emb_d[4 * i + q];
-      evi[q] = a.emb_v[4 * i + q];
-    }
-    // own tangents and the per-atom parts
-    float Ti[NS][kTan], dFi[NS], F[NS][4], V[NS][6];
+      for (int q = 0; q < kJP; ++q) tp[s][q] = 0.0f;
+    for (int li = wave; li < na; li += 4) {
+      const int64_t i = int64_t(a.tile_a0[tile]) + li;
+      const int ti = a.types[i];
+      const int64_t p0 = a.pair_off[i], p1 = a.pair_off[i + 1];
+      // the atom's own moment (mu_i or lambda_i) and moment tangents
+      float Mi[NC];
 #pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      for (int q = 0; q < kTan; ++q)
-        Ti[s][q] = ckind[s] == CK_NONE ? 0.0f : a.tan[(i * kTan + q) * a.D + ccol[s]];
-      float e = 0.0f, lim = 0.0f;
-      dFi[s] = Fppi * Ti[s][0];
-      if (ckind[s] == CK_DENS) {
-        if (clamped) lim = float(a.over[i]) * Ti[s][0];
-        else e = float(a.Fp[i]) * Ti[s][0];
-      } else if (ckind[s] == CK_EMB && ctab[s] == emb0 + ti) {
-        dFi[s] += band(sS, sT[ctab[s]], kfi, cm[s], ebi);
-        e = band(sS, sT[ctab[s]], kfi, cm[s], evi);
-      } else if (ckind[s] == CK_DIP) {
-        e = mui[0] * Ti[s][1] + mui[1] * Ti[s][2] + mui[2] * Ti[s][3];
-      } else if (ckind[s] == CK_QUAD) {
-        const float *dl = Ti[s] + 4;
-        const float dot = Li[0] * dl[0] + Li[1] * dl[1] + Li[2] * dl[2] +
-                          2.0f * (Li[3] * dl[3] + Li[4] * dl[4] + Li[5] * dl[5]);
-        e = dot - (Li[0] + Li[1] + Li[2]) * (dl[0] + dl[1] + dl[2]) / 3.0f;
-      }
-      for (int q = 0; q < 3; ++q) F[s][q] = 0.0f;
-      F[s][3] = e;
-      for (int q = 0; q < 6; ++q) V[s][q] = 0.0f;
-      tp[s][7] += lim;
-    }
-    const int64_t p0 = a.pair_off[i], p1 = a.pair_off[i + 1];
-    for (int64_t pc = p0; pc < p1; pc += 64) {
-      const int64_t p = pc + lane;
-      if (p < p1) {
-        const float4 q4 = a.rec[p];
-        const uint32_t b = __float_as_uint(q4.w);
-        const bool tiny = (b & kFlagTiny) != 0;
-        const float r = sqrtf(q4.x * q4.x + q4.y * q4.y + q4.z * q4.z);
-        const int tj = pj_type(b), o = ord_pair(ti, tj, a.ntypes);
-        const int64_t jg = cbase + pj_local(b);
-        P.r[lane] = tiny ? 1.0f : r;
-        P.d0[lane] = q4.x;
-        P.d1[lane] = q4.y;
-        P.d2[lane] = q4.z;
-        P.jg[lane] = int32_t(jg);
-        P.tj[lane] = tj;
-        const bool jcl = a.over[jg] != 0.0;
-        P.fpj[lane] = a.Fp32[jg];
-        P.fppj[lane] = jcl ? 0.0f : float(a.Fpp[jg]);
-        P.kfj[lane] = a.emb_k[jg];
-        for (int q = 0; q < 4; ++q) P.ebj[q][lane] = a.emb_d[4 * jg + q];
-        const float muj[3] = {float(a.mu[3 * jg]), float(a.mu[3 * jg + 1]), float(a.mu[3 * jg + 2])};
-        float Lj[6];
-        for (int q = 0; q < 6; ++q) Lj[q] = float(a.lamb[6 * jg + q]);
-        for (int q = 0; q < 3; ++q) P.muj[q][lane] = muj[q];
-        float xa[3], xb[3];
-        const float nu = qnu(Li, q4.x, q4.y, q4.z, xa) + qnu(Lj, q4.x, q4.y, q4.z, xb);
-        P.nu[lane] = nu;
-        for (int q = 0; q < 3; ++q) P.xi[q][lane] = xa[q] + xb[q];
-        float bv[4], bd[4], cc[6], cv[6];
-        auto role = [&](bool on, int tab, int32_t &tslot, int32_t &kslot, float (*cd)[64],
-                        float (*cvv)[64], float &val, float &der) {
-          tslot = on ? tab : -1;
-          val = der = 0.0f;
-          if (!on) return;
-          const TableDesc T = sT[tab];
-          kslot = pair_basis(T, r, bv, bd);
-          band6(sS, T, kslot, bd, 1.0f, cc);
-          for (int d = 0; d < 6; ++d) cd[d][lane] = cc[d];
-          if (cvv) {
-            band6(sS, T, kslot, bv, 1.0f, cv);
-            for (int d = 0; d < 6; ++d) cvv[d][lane] = cv[d];
-          }
-          float tt;
-          const int kk = spl_interval32(T, r, tt);
-          spl_eval32(T, a.ivl32, kk, tt, val, der);
-        };
-        float v0, d0;
-        role(!tiny && (b & kFlagPhi), o, P.tp[lane], P.kp[lane], P.cpd, P.cpv, v0, d0);
-        role(!tiny && (b & kFlagGj), a.npair + tj, P.tgj[lane], P.kgj[lane], P.cgj, nullptr, v0, d0);
-        P.dgj[lane] = d0;
-        role(!tiny && (b & kFlagGi), a.npair + ti, P.tgi[lane], P.kgi[lane], P.cgi, nullptr, v0, d0);
-        P.dgi[lane] = d0;
-        role(!tiny && (b & kFlagDip), dip0 + o, P.tu[lane], P.ku[lane], P.cud, P.cuv, v0, d0);
-        P.u[lane] = v0;
-        P.du[lane] = d0;
-        role(!tiny && (b & kFlagQuad), quad0 + o, P.tw[lane], P.kw[lane], P.cwd, P.cwv, v0, d0);
-        P.w[lane] = v0;
-        P.dw[lane] = d0;
-        if (tiny) P.jg[lane] = -1;
-      }
-      wave_sync();
-      const int np = int(min<int64_t>(64, p1 - pc));
+      for (int q = 0; q < NC; ++q) Mi[q] = float(kDip ? a.mu[3 * i + q] : a.lamb[6 * i + q]);
+      int ptt[NS];
+      float own[NS][NC], F[NS][3];
 #pragma unroll
       for (int s = 0; s < NS; ++s) {
-        const int kind = ckind[s];
-        if (kind == CK_NONE) continue;
-        const int tab = ctab[s], m = cm[s], col = ccol[s];
-        for (int q = 0; q < np; ++q) {
-          const int jg = P.jg[q];
-          if (jg < 0) continue;
-          const float x = P.d0[q], y = P.d1[q], z = P.d2[q], ir = 1.0f / P.r[q];
-          float fs = 0.0f, e = 0.0f, f[3] = {0.0f, 0.0f, 0.0f};
-          if (kind == CK_PAIR) {
-            if (P.tp[q] != tab) continue;
-            const int d = m - P.kp[q] + 2;
-            if (d < 0 || d > 5) continue;
-            fs = P.cpd[d][q];
-            e = 0.5f * P.cpv[d][q];
-          } else if (kind == CK_DENS || kind == CK_EMB) {
-            float dFj = 0.0f;
-            if (kind == CK_DENS) {
-              if (P.tgj[q] == tab) {
-                const int d = m - P.kgj[q] + 2;
-                if (d >= 0 && d < 6) fs += Fpi * P.cgj[d][q];
-              }
-              if (P.tgi[q] == tab) {
-                const int d = m - P.kgi[q] + 2;
-                if (d >= 0 && d < 6) fs += P.fpj[q] * P.cgi[d][q];
-              }
-              dFj = P.fppj[q] * a.tan[(int64_t(jg) * kTan) * a.D + col];
-            } else if (P.tj[q] == tab - emb0) {
-              const float eb[4] = {P.ebj[0][q], P.ebj[1][q], P.ebj[2][q], P.ebj[3][q]};
-              dFj = band(sS, sT[tab], P.kfj[q], m, eb);
-            }
-            fs += dFi[s] * P.dgj[q] + dFj * P.dgi[q];
-          } else if (kind == CK_DIP) {
-            const float dmx = mui[0] - P.muj[0][q], dmy = mui[1] - P.muj[1][q], dmz = mui[2] - P.muj[2][q];
-            if (P.tu[q] == tab) {
-              const int d = m - P.ku[q] + 2;
-              if (d >= 0 && d < 6) {
-                const float sc = P.cud[d][q] * ir * (dmx * x + dmy * y + dmz * z);
-                const float v = P.cuv[d][q];
-                f[0] += sc * x + v * dmx;
-                f[1] += sc * y + v * dmy;
-                f[2] += sc * z + v * dmz;
-              }
-            }
-            if (P.tu[q] >= 0) {
-              const float *tj = a.tan + (int64_t(jg) * kTan) * a.D + col;
-              const float gx = Ti[s][1] - tj[1 * a.D], gy = Ti[s][2] - tj[2 * a.D],
-                          gz = Ti[s][3] - tj[3 * a.D];
-              const float sc = P.du[q] * ir * (gx * x + gy * y + gz * z);
-              const float u = P.u[q];
-              f[0] += sc * x + u * gx;
-              f[1] += sc * y + u * gy;
-              f[2] += sc * z + u * gz;
-            }
-          } else if (kind == CK_QUAD) {
-            if (P.tw[q] == tab) {
-              const int d = m - P.kw[q] + 2;
-              if (d >= 0 && d < 6) {
-                const float sc = P.cwd[d][q] * ir * P.nu[q];
-                const float v2 = 2.0f * P.cwv[d][q];
-                f[0] += sc * x + v2 * P.xi[0][q];
-                f[1] += sc * y + v2 * P.xi[1][q];
-                f[2] += sc * z + v2 * P.xi[2][q];
-              }
-            }
-            if (P.tw[q] >= 0) {
-              const float *tj = a.tan + (int64_t(jg) * kTan) * a.D + col;
-              float dl[6];
-              for (int k = 0; k < 6; ++k) dl[k] = Ti[s][4 + k] + tj[(4 + k) * a.D];
-              float xq[3];
-              const float sc = P.dw[q] * ir * qnu(dl, x, y, z, xq);
-              const float w2 = 2.0f * P.w[q];
-              f[0] += sc * x + w2 * xq[0];
-              f[1] += sc * y + w2 * xq[1];
-              f[2] += sc * z + w2 * xq[2];
+        ptt[s] = cqc[s] < 0 ? -1 : partner_type(cpt[s], ti);
+        if (cqc[s] >= 0) load_moment<kDip>(mom, i * Dq + cqc[s], own[s]);
+        else
+#pragma unroll
+          for (int q = 0; q < NC; ++q) own[s][q] = 0.0f;
+        F[s][0] = F[s][1] = F[s][2] = 0.0f;
+        // energy tangent: mu_i . d mu_i, or lambda_i : d lambda_i - tr tr / 3
+        float e;
+        if constexpr (kDip) {
+          e = Mi[0] * own[s][0] + Mi[1] * own[s][1] + Mi[2] * own[s][2];
+        } else {
+          const float *dl = own[s];
+          const float dot = Mi[0] * dl[0] + Mi[1] * dl[1] + Mi[2] * dl[2] +
+                            2.0f * (Mi[3] * dl[3] + Mi[4] * dl[4] + Mi[5] * dl[5]);
+          e = dot - (Mi[0] + Mi[1] + Mi[2]) * (dl[0] + dl[1] + dl[2]) / 3.0f;
+        }
+        tp[s][0] += e;
+      }
+      for (int64_t pc = p0; pc < p1; pc += 64) {
+        const int64_t p = pc + lane;
+        int ty = kMaxTypes;
+        if (p < p1) {
+          const float4 q = a.jrec[p];
+          const uint32_t b = __float_as_uint(q.w);
+          const float rr = a.jr[p];
+          const bool on = (b & dq_flag<PASS>()) && !(b & kFlagTiny);
+          const int tj = pj_type(b);
+          ty = tj;
+          const int32_t jg = int32_t(cbase + pj_local(b));
+          const TableDesc Tq = sT[tab0 + ord_pair(ti, tj, a.ntypes)];
+          float bv[4], bd[4], cc[6], ce[6];
+          const int k = pair_basis(Tq, rr, bv, bd);
+          band6(sS, Tq, k, bd, on ? 1.0f : 0.0f, cc);
+          band6(sS, Tq, k, bv, on ? 1.0f : 0.0f, ce);
+          float v = 0.0f, dv = 0.0f;
+          if (on) {
+            float tt;
+            const int kk = spl_interval32(Tq, rr, tt);
+            spl_eval32(Tq, a.ivl32, kk, tt, v, dv);
+          }
+          const float ir = on ? 1.0f / rr : 0.0f;
+          P.x[lane] = q.x;
+          P.y[lane] = q.y;
+          P.z[lane] = q.z;
+          P.jg[lane] = jg;
+          P.k[lane] = k;
+          P.v[lane] = v;
+          P.dv[lane] = dv * ir;
+#pragma unroll
+          for (int d = 0; d < 6; ++d) {
+            P.cd[d][lane] = cc[d];
+            P.cv[d][lane] = ce[d];
+          }
+          if constexpr (kDip) {
+            const float g0 = Mi[0] - float(a.mu[3 * int64_t(jg)]);
+            const float g1 = Mi[1] - float(a.mu[3 * int64_t(jg) + 1]);
+            const float g2 = Mi[2] - float(a.mu[3 * int64_t(jg) + 2]);
+            P.g0[lane] = g0;
+            P.g1[lane] = g1;
+            P.g2[lane] = g2;
+            P.sg[lane] = (g0 * q.x + g1 * q.y + g2 * q.z) * ir;
+          } else {
+            float Lj[6], xa[3], xb[3];
+#pragma unroll
+            for (int t = 0; t < 6; ++t) Lj[t] = float(a.lamb[6 * int64_t(jg) + t]);
+            const float nu = qnu(Mi, q.x, q.y, q.z, xa) + qnu(Lj, q.x, q.y, q.z, xb);
+            P.g0[lane] = xa[0] + xb[0];
+            P.g1[lane] = xa[1] + xb[1];
+            P.g2[lane] = xa[2] + xb[2];
+            P.sg[lane] = nu * ir;
+          }
+        }
+        int seg[kMaxTypes + 1];
+        chunk_segments(ty, a.ntypes, seg);
+        wave_sync();
+        const int np = int(min<int64_t>(64, p1 - pc));
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+          if (cqc[s] < 0) continue;
+          // band terms of the column's own table: its window of the pairs
+          // with the matching partner type
+          if (ptt[s] >= 0) {
+            const int m = cm[s], s1 = seg[ptt[s] + 1];
+            const int lo = first_ge(P.k, seg[ptt[s]], s1, m - 3);
+            const int hi = first_ge(P.k, lo, s1, m + 3);
+            for (int q = lo; q < hi; ++q) {
+              const int d = m - P.k[q] + 2;
+              const float sc = P.cd[d][q] * P.sg[q];
+              const float vv = (kDip ? 1.0f : 2.0f) * P.cv[d][q];
+              const float x = P.x[q], y = P.y[q], z = P.z[q];
+              const float f0 = sc * x + vv * P.g0[q], f1 = sc * y + vv * P.g1[q],
+                          f2 = sc * z + vv * P.g2[q];
+              F[s][0] += f0;
+              F[s][1] += f1;
+              F[s][2] += f2;
+              add_dvirial(tp[s], x, y, z, f0, f1, f2);
             }
           }
-          const float sr = fs * ir;
-          f[0] += sr * x;
-          f[1] += sr * y;
-          f[2] += sr * z;
-          F[s][0] += f[0];
-          F[s][1] += f[1];
-          F[s][2] += f[2];
-          F[s][3] += e;
-          V[s][0] += x * f[0];
-          V[s][1] += y * f[1];
-          V[s][2] += z * f[2];
-          V[s][3] += x * f[1];
-          V[s][4] += x * f[2];
-          V[s][5] += y * f[2];
+          // moment-tangent terms: every pair, the neighbours' tangents
+          // gathered kQBatch pairs at a time
+          for (int q0 = 0; q0 < np; q0 += kQBatch) {
+            float h[kQBatch][NC];
+#pragma unroll
+            for (int u = 0; u < kQBatch; ++u) {
+              if (q0 + u < np) load_moment<kDip>(mom, int64_t(P.jg[q0 + u]) * Dq + cqc[s], h[u]);
+              else
+#pragma unroll
+                for (int t = 0; t < NC; ++t) h[u][t] = 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < kQBatch; ++u) {
+              const int q = q0 + u;
+              if (q >= np) break;
+              const float x = P.x[q], y = P.y[q], z = P.z[q];
+              float f0, f1, f2;
+              if constexpr (kDip) {
+                const float gx = own[s][0] - h[u][0], gy = own[s][1] - h[u][1], gz = own[s][2] - h[u][2];
+                const float sc = P.dv[q] * (gx * x + gy * y + gz * z), vv = P.v[q];
+                f0 = sc * x + vv * gx;
+                f1 = sc * y + vv * gy;
+                f2 = sc * z + vv * gz;
+              } else {
+                float dl[6], xq[3];
+#pragma unroll
+                for (int t = 0; t < 6; ++t) dl[t] = own[s][t] + h[u][t];
+                const float sc = P.dv[q] * qnu(dl, x, y, z, xq), w2 = 2.0f * P.v[q];
+                f0 = sc * x + w2 * xq[0];
+                f1 = sc * y + w2 * xq[1];
+                f2 = sc * z + w2 * xq[2];
+              }
+              F[s][0] += f0;
+              F[s][1] += f1;
+              F[s][2] += f2;
+              add_dvirial(tp[s], x, y, z, f0, f1, f2);
+            }
+          }
+        }
+        wave_sync();
+      }
+      const int64_t row = a.row_off[c] + 3 * (i - cbase);
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        if (cqc[s] < 0) continue;
+        for (int q = 0; q < 3; ++q) a.J[(row + q) * a.D + ccol[s]] = F[s][q];
+      }
+    }
+    // reduce the four waves' partials in wave order (FP64), write the tile
+    // partial of this pass's columns
+    __syncthreads();
+    for (int w = 0; w < 4; ++w) {
+      if (wave == w) {
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+          if (cqc[s] < 0) continue;
+          const int col = ccol[s];
+          for (int q = 0; q < kJP; ++q) sPart[q][col] = (w == 0 ? 0.0 : sPart[q][col]) + double(tp[s][q]);
         }
       }
-      wave_sync();
+      __syncthreads();
     }
-    const int64_t row = a.row_off[c] + 3 * (i - cbase);
+    if (wave == 0) {
 #pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      if (ckind[s] == CK_NONE) continue;
-      for (int q = 0; q < 3; ++q) a.J[(row + q) * a.D + ccol[s]] = F[s][q];
-      tp[s][0] += F[s][3];
-      for (int q = 0; q < 6; ++q) tp[s][1 + q] += V[s][q];
+      for (int s = 0; s < NS; ++s) {
+        if (cqc[s] < 0) continue;
+        const int col = ccol[s];
+        for (int q = 0; q < kJP; ++q) a.part[(int64_t(tile) * kJP + q) * a.D + col] = sPart[q][col];
+      }
     }
-  }
-#pragma unroll
-  for (int s = 0; s < NS; ++s) {
-    if (ckind[s] == CK_NONE) continue;
-    for (int q = 0; q < kJP; ++q) a.part[(int64_t(tile) * kJP + q) * a.D + ccol[s]] = double(tp[s][q]);
+    __syncthreads();
   }
 }
 
@@ -1559,7 +1587,7 @@ void launch_jacobian(Context &c) {
   } else {
     // column descriptors and lane-slot maps: built and uploaded once per model
     const bool adp = c.kind == FS_ADP;
-    int Dd = 0, dens_c0 = 0, gi_nf = 0, NS = 0, NSP = 0, NSD = 0;
+    int Dd = 0, dens_c0 = 0, gi_nf = 0, NSP = 0, NSD = 0, NSU = 0, NSW = 0, Ddip = 0, Dquad = 0;
     if (!c.jl.valid) {
       // column descriptors
       std::vector<ColDesc> cols(std::max(c.D, 1));
@@ -1579,7 +1607,12 @@ void launch_jacobian(Context &c) {
           const int col = c.h_col_of[T.off + m];
           if (col < 0) continue;
           ColDesc cd{t, m, kind, -1};
-          if (kind == CK_PAIR) cd.dc = pair_types[t].first | (pair_types[t].second << 8);
+          const int po = kind == CK_DIP ? t - c.npair - 2 * c.ntypes
+                                        : (kind == CK_QUAD ? t - 2 * c.npair - 2 * c.ntypes : t);
+          if (kind == CK_PAIR || kind == CK_DIP || kind == CK_QUAD)
+            cd.dc = pair_types[po].first | (pair_types[po].second << 8);
+          if (kind == CK_DIP) ++Ddip;
+          if (kind == CK_QUAD) ++Dquad;
           if (kind == CK_DENS) {
             if (dens_c0 < 0) dens_c0 = col;
             cd.dc = Dd++;
@@ -1604,17 +1637,18 @@ void launch_jacobian(Context &c) {
         gi_nf = (equal && nfree[0] > 0 && nfree[0] * c.ntypes <= 64) ? nfree[0] : 0;
       }
       // lane slots: pair, density and embedding columns each start a new slot.
-      // ADP: one map for its tangent kernels. EAM: the pair-table columns and
-      // the density/embedding columns form the maps of the two k_jac_main passes.
+      // Maps in order: pair-table columns (k_jac_pair), density and embedding
+      // columns (k_jac_de), then for ADP the dipole and the quadrupole columns
+      // (k_adp_mom / k_adp_dq, one map each); qc numbers a kind's columns.
       auto build_slots = [&](std::initializer_list<int> kinds) {
         std::vector<SlotDesc> sl;
         for (int kind : kinds) {
-          size_t n = 0;
+          int n = 0;
           for (int col = 0; col < c.D; ++col) {
             if (cols[col].kind != kind) continue;
             if (n % 64 == 0) sl.resize(sl.size() + 64, SlotDesc{-1, 0, CK_NONE, -1, 0, 0, 0, 0});
             const ColDesc &cd = cols[col];
-            sl[sl.size() - 64 + n % 64] = SlotDesc{cd.tab, cd.m, cd.kind, cd.dc, col, 0, 0, 0};
+            sl[sl.size() - 64 + n % 64] = SlotDesc{cd.tab, cd.m, cd.kind, cd.dc, col, n, 0, 0};
             ++n;
           }
         }
@@ -1623,23 +1657,23 @@ void launch_jacobian(Context &c) {
       auto pad_to = [](std::vector<SlotDesc> &sl, int ns) {
         sl.resize(size_t(ns) * 64, SlotDesc{-1, 0, CK_NONE, -1, 0, 0, 0, 0});
       };
-      std::vector<SlotDesc> slots, sde;
+      std::vector<SlotDesc> slots = build_slots({CK_PAIR}), sde = build_slots({CK_DENS, CK_EMB});
+      const int nsp = int(slots.size() / 64), nsd = int(sde.size() / 64);
+      FS_REQUIRE(nsp <= 4 && nsd <= 6, FS_ERR_UNSUPPORTED, "Jacobian columns need too many lane slots");
+      NSP = nsp == 0 ? 0 : (nsp <= 2 ? 2 : 4);
+      NSD = nsd == 0 ? 0 : (nsd <= 2 ? 2 : (nsd <= 4 ? 4 : 6));
+      pad_to(slots, NSP);
+      pad_to(sde, NSD);
+      slots.insert(slots.end(), sde.begin(), sde.end());
       if (adp) {
-        slots = build_slots({CK_PAIR, CK_DENS, CK_EMB, CK_DIP, CK_QUAD});
-        const int ns = int(slots.size() / 64);
-        FS_REQUIRE(ns <= 8, FS_ERR_UNSUPPORTED, "Jacobian columns need too many lane slots");
-        NS = ns <= 4 ? 4 : 8;
-        pad_to(slots, NS);
-      } else {
-        slots = build_slots({CK_PAIR});
-        sde = build_slots({CK_DENS, CK_EMB});
-        const int nsp = int(slots.size() / 64), nsd = int(sde.size() / 64);
-        FS_REQUIRE(nsp <= 4 && nsd <= 6, FS_ERR_UNSUPPORTED, "Jacobian columns need too many lane slots");
-        NSP = nsp == 0 ? 0 : (nsp <= 2 ? 2 : 4);
-        NSD = nsd == 0 ? 0 : (nsd <= 2 ? 2 : (nsd <= 4 ? 4 : 6));
-        pad_to(slots, NSP);
-        pad_to(sde, NSD);
-        slots.insert(slots.end(), sde.begin(), sde.end());
+        std::vector<SlotDesc> su = build_slots({CK_DIP}), sw = build_slots({CK_QUAD});
+        const int nsu = int(su.size() / 64), nsw = int(sw.size() / 64);
+        NSU = nsu <= 1 ? nsu : (nsu <= 2 ? 2 : 4);
+        NSW = nsw <= 1 ? nsw : (nsw <= 2 ? 2 : 4);
+        pad_to(su, NSU);
+        pad_to(sw, NSW);
+        slots.insert(slots.end(), su.begin(), su.end());
+        slots.insert(slots.end(), sw.begin(), sw.end());
       }
       static_assert(sizeof(SlotDesc) == 2 * sizeof(int4), "slot descriptor layout");
       c.jslots.alloc(std::max<size_t>(slots.size(), 1) * 2);
@@ -1647,14 +1681,17 @@ void launch_jacobian(Context &c) {
                             hipMemcpyHostToDevice, c.stream));
       FS_HIP(hipStreamSynchronize(c.stream));
 
-      c.jl = Context::JacLayout{true, Dd, dens_c0, gi_nf, NS, NSP, NSD};
+      c.jl = Context::JacLayout{true, Dd, dens_c0, gi_nf, NSP, NSD, NSU, NSW, Ddip, Dquad};
     }
     Dd = c.jl.Dd;
     dens_c0 = c.jl.dens_c0;
     gi_nf = c.jl.gi_nf;
-    NS = c.jl.NS;
     NSP = c.jl.NSP;
     NSD = c.jl.NSD;
+    NSU = c.jl.NSU;
+    NSW = c.jl.NSW;
+    Ddip = c.jl.Ddip;
+    Dquad = c.jl.Dquad;
     a.Dd = Dd;
     a.gi_nf = gi_nf;
     FS_REQUIRE(int64_t(c.N) * std::max(Dd, 1) < (int64_t(1) << 31), FS_ERR_UNSUPPORTED,
@@ -1698,29 +1735,7 @@ void launch_jacobian(Context &c) {
     }
     a.jrec = c.jrec.p;
     a.jr = c.jr.p;
-    if (adp) {
-      c.adp_tan.alloc(std::max<size_t>(size_t(c.N) * kTan * c.D, 1));
-      a.tan = c.adp_tan.p;
-      a.mu = c.mu.p;
-      a.lamb = c.lamb.p;
-      const size_t lds1 = base + sizeof(AStage1), lds2 = base + sizeof(AStage2);
-      FS_REQUIRE(lds2 <= 64 * 1024, FS_ERR_UNSUPPORTED, "ADP Jacobian LDS budget exceeded");
-      if (c.N > 0) {
-        const unsigned g1 = unsigned(std::min<int64_t>(c.N, int64_t(c.cu_count) * 32));
-        c.time_main_begin();
-        if (NS == 4) {
-          hipLaunchKernelGGL(k_adp_tan1<4>, dim3(g1), dim3(64), lds1, c.stream, a, c.nknots, d_slots);
-          FS_HIP(hipGetLastError());
-          hipLaunchKernelGGL(k_adp_tan2<4>, dim3(nt), dim3(64), lds2, c.stream, a, c.nknots, d_slots);
-        } else {
-          hipLaunchKernelGGL(k_adp_tan1<8>, dim3(g1), dim3(64), lds1, c.stream, a, c.nknots, d_slots);
-          FS_HIP(hipGetLastError());
-          hipLaunchKernelGGL(k_adp_tan2<8>, dim3(nt), dim3(64), lds2, c.stream, a, c.nknots, d_slots);
-        }
-        FS_HIP(hipGetLastError());
-        c.time_main_end();
-      }
-    } else if (c.N > 0) {
+    if (c.N > 0) {
       // (the pair pass depends on neither kernel below; run beside them on a
       // second stream it measured no faster and stretched both)
       const unsigned g2 = unsigned(std::max(1, std::min(nt, c.cu_count * 8)));
@@ -1739,8 +1754,10 @@ void launch_jacobian(Context &c) {
       hipLaunchKernelGGL(k_emb_sites, dim3(nb(c.N, 256)), dim3(256), 0, c.stream, a);
       FS_HIP(hipGetLastError());
       const unsigned g1 = unsigned(std::min<int64_t>((c.N + 3) / 4, int64_t(c.cu_count) * 16));
-      hipLaunchKernelGGL(k_jac_dens, dim3(g1), dim3(256), lds_dens, c.stream, a, c.nknots);
-      FS_HIP(hipGetLastError());
+      if (Dd > 0) {
+        hipLaunchKernelGGL(k_jac_dens, dim3(g1), dim3(256), lds_dens, c.stream, a, c.nknots);
+        FS_HIP(hipGetLastError());
+      }
       c.time_main_begin(0);
       if (NSD == 2)
         hipLaunchKernelGGL((k_jac_de<2, kWavesDE>), dim3(g2), dim3(256), lds_de, c.stream, a, c.nknots,
@@ -1750,6 +1767,34 @@ void launch_jacobian(Context &c) {
       else if (NSD == 6)
         hipLaunchKernelGGL((k_jac_de<6, 1>), dim3(g2), dim3(256), lds_de, c.stream, a, c.nknots, d_de);
       FS_HIP(hipGetLastError());
+      if (adp) {
+        // dipole and quadrupole columns: moment tangents, then the force pass
+        const size_t nmu = size_t(c.N) * 4 * Ddip, nla = size_t(c.N) * 6 * Dquad;
+        c.adp_tan.alloc(std::max<size_t>(nmu + nla, 1));
+        a.dmu = c.adp_tan.p;
+        a.dlam = c.adp_tan.p + nmu;
+        a.Ddip = Ddip;
+        a.Dquad = Dquad;
+        a.mu = c.mu.p;
+        a.lamb = c.lamb.p;
+        const size_t lds_mom = base + 4 * sizeof(MomStage), lds_dq = base + 4 * sizeof(DQStage);
+        FS_REQUIRE(lds_dq <= 64 * 1024, FS_ERR_UNSUPPORTED, "ADP Jacobian LDS budget exceeded");
+        const SlotDesc *d_u = d_de + size_t(NSD) * 64, *d_w = d_u + size_t(NSU) * 64;
+        auto pass = [&](auto kind, int ns, const SlotDesc *sl) {
+          constexpr int K = decltype(kind)::value;
+          if (ns == 0) return;
+          if (ns == 1) hipLaunchKernelGGL((k_adp_mom<K, 1>), dim3(g1), dim3(256), lds_mom, c.stream, a, c.nknots, sl);
+          else if (ns == 2) hipLaunchKernelGGL((k_adp_mom<K, 2>), dim3(g1), dim3(256), lds_mom, c.stream, a, c.nknots, sl);
+          else hipLaunchKernelGGL((k_adp_mom<K, 4>), dim3(g1), dim3(256), lds_mom, c.stream, a, c.nknots, sl);
+          FS_HIP(hipGetLastError());
+          if (ns == 1) hipLaunchKernelGGL((k_adp_dq<K, 1>), dim3(g2), dim3(256), lds_dq, c.stream, a, c.nknots, sl);
+          else if (ns == 2) hipLaunchKernelGGL((k_adp_dq<K, 2>), dim3(g2), dim3(256), lds_dq, c.stream, a, c.nknots, sl);
+          else hipLaunchKernelGGL((k_adp_dq<K, 4>), dim3(g2), dim3(256), lds_dq, c.stream, a, c.nknots, sl);
+          FS_HIP(hipGetLastError());
+        };
+        pass(std::integral_constant<int, JP_DIP>{}, NSU, d_u);
+        pass(std::integral_constant<int, JP_QUAD>{}, NSW, d_w);
+      }
       c.time_main_end();
     }
   }

@@ -277,7 +277,7 @@ int fs_de_impl(Context &c, const fs_de_opts *o, const double *lo, const double *
 
 // what: 0 = residual (density + force + finalize), 1 = Jacobian (all its
 // kernels), 2 = normal equations, 3 = residual + Jacobian, 4 = the
-// density/embedding pass of the EAM Jacobian (ADP: its tangent kernels),
+// density/embedding pass of the Jacobian (ADP: with the dipole/quadrupole passes),
 // 5 = the pair-column pass (events around that launch only)
 double time_step_impl(Context &c, int what, int reps) {
   ensure_ready(c);

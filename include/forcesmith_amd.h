@@ -238,8 +238,8 @@ int fs_autotune(fs_ctx *ctx, int32_t *lanes_per_atom);
 /* Average time (ms) of one stage over reps launches, measured with HIP events
  * on the context stream. what: 0 residual, 1 Jacobian (all its kernels),
  * 2 normal equations, 3 residual + Jacobian, 4 the density/embedding-column
- * pass of the EAM Jacobian (ADP: its tangent kernels), 5 the pair-column
- * pass. No reference counterpart (measurement only). */
+ * pass of the Jacobian (ADP: with the dipole/quadrupole-column kernels that
+ * follow it), 5 the pair-column pass. No reference counterpart (measurement only). */
 int fs_time_step(fs_ctx *ctx, int32_t what, int32_t reps, double *ms_per_rep);
 
 /* MLP surrogate kernels on host buffers (no reference counterpart; the
