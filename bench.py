@@ -12,6 +12,10 @@ and a CPU baseline:
 the FP64 oracle (a restatement of the reference's oneTBB calculator: central
 differences, two residual evaluations per column, configurations in parallel)
 timed on a bounded sample of columns on this host.
+
+--workload runs the same step on the other workloads (single-element EAM,
+binary ADP with the roofline of its quadrupole-column pass, Si
+Stillinger-Weber); the default line is the ternary EAM one.
 """
 from __future__ import annotations
 
@@ -24,12 +28,31 @@ import time
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 
 
+# workload name -> (dataset generator, description); the generators live in
+# forcesmith_amd.datasets
+WORKLOADS = {
+    "eam-ternary": ("config2", "ternary tabulated EAM, {C} configs of 32-2048 atoms at 0.06/A^3, "
+                               "12 tables x 20 knots (D={D}), residual + full Jacobian"),
+    "eam-single": ("config1", "single-element tabulated EAM, {C} FCC configs of 108 atoms, "
+                              "15/15/12 knots (D={D}), residual + full Jacobian"),
+    "adp-binary": ("config3", "binary tabulated ADP, {C} configs of 250-1024 atoms, 13 tables x 20 "
+                              "knots (D={D} free: dipole, quadrupole, embedding), residual + full "
+                              "Jacobian"),
+    "sw-si": ("config4", "Si Stillinger-Weber, {C} configs of 64-512 atoms (D={D}), residual + "
+                         "full Jacobian"),
+}
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--configs", type=int, default=500)
+    ap.add_argument("--configs", type=int, default=500,
+                    help="configurations of the ternary EAM workload")
+    ap.add_argument("--workload", default="eam-ternary", choices=sorted(WORKLOADS),
+                    help="dataset the step runs on; the headline figure is eam-ternary, the "
+                         "others are the single-element EAM, binary ADP and Si SW workloads")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target CPU time of the baseline sample")
@@ -208,9 +231,11 @@ def main():
     import forcesmith_amd as fa
     from forcesmith_amd import datasets
 
-    log("generating the ternary EAM workload (500 configs, seed 2)")
-    w = datasets.config2(n_configs=a.configs)
+    gen, desc = WORKLOADS[a.workload]
+    log(f"generating the {a.workload} workload")
+    w = getattr(datasets, gen)(n_configs=a.configs) if gen == "config2" else getattr(datasets, gen)()
     ds = w.dataset
+    kind = w.model.kind
     g = fa.GpuCalculator(ds, w.model, precision=a.precision)
     t0 = time.time()
     P = g.prepare()
@@ -241,20 +266,49 @@ def main():
     # kernel times with HIP events on the context stream
     t_res = g.time_step(0, 10)
     t_jac = g.time_step(1, 10)
-    t_main = g.time_step(4, 10)
-    t_pair = g.time_step(5, 10)
     t_ne = g.time_step(2, 5)
-    # algorithmic bytes of k_jac_de (the dominant kernel) per launch, each
-    # datum once: pair records (float4 + FP32 distance, 20 B/pair), the FP32
-    # force rows of the density/embedding columns it writes (3N x D_de x 4 B)
-    # and the per-atom density-column rows it reads (Hs FP32 + H FP64, 12 B
-    # per atom and density column)
     m = w.model
-    n_pair_tab, n_dens = m.npair, m.npair + m.ntypes
-    D_dens = sum(int((~t.fixed).sum()) for t in m.tables[n_pair_tab:n_dens])
-    D_de = sum(int((~t.fixed).sum()) for t in m.tables[n_pair_tab:])
-    b_main = 20 * P + 12 * N * D_de + 12 * N * D_dens
-    achieved = b_main / (t_main * 1e-3) / 1e9
+    free = [int((~t.fixed).sum()) for t in m.tables] if kind != "sw" else []
+    roofline = None
+    if kind == "eam":
+        t_main = g.time_step(4, 10)
+        t_pair = g.time_step(5, 10)
+        kernels = {"residual": t_res, "jacobian": t_jac, "jac_de": t_main, "jac_pair": t_pair,
+                   "normal_equations": t_ne}
+        # algorithmic bytes of k_jac_de (the dominant kernel) per launch, each
+        # datum once: pair records (float4 + FP32 distance, 20 B/pair), the FP32
+        # force rows of the density/embedding columns it writes (3N x D_de x 4 B)
+        # and the per-atom density-column rows it reads (Hs FP32 + H FP64, 12 B
+        # per atom and density column)
+        D_dens = sum(free[m.npair:m.npair + m.ntypes])
+        D_de = sum(free[m.npair:])
+        b_main = 20 * P + 12 * N * D_de + 12 * N * D_dens
+        achieved = b_main / (t_main * 1e-3) / 1e9
+        ternary500 = a.workload == "eam-ternary" and C == 500
+        roofline = {"bound": "hbm", "kernel": "k_jac_de", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                    "traffic": measured_traffic() if ternary500 else None,
+                    "traffic_source": ("profiles/k_jac_de_traffic_eam_ternary_500cfg.md (rocprofv3 "
+                                       "FETCH_SIZE + WRITE_SIZE, copy-calibrated)") if ternary500 else None,
+                    "bytes_per_launch": b_main, "ms_per_launch": t_main}
+    elif kind == "adp":
+        t_main = g.time_step(4, 10)
+        t_quad = g.time_step(6, 10)
+        kernels = {"residual": t_res, "jacobian": t_jac, "jac_de_and_moment_passes": t_main,
+                   "adp_quadrupole_pass": t_quad, "normal_equations": t_ne}
+        # algorithmic bytes of the quadrupole-column force pass (the dominant
+        # kernel) per launch: pair records (20 B/pair), the lambda rows of
+        # the atoms (48 B), their d lambda rows over the quadrupole columns
+        # (24 B per atom and column) and the FP32 force rows it writes
+        # (3N x D_w x 4 B)
+        D_w = sum(free[2 * m.npair + 2 * m.ntypes:])
+        b_main = 20 * P + 48 * N + 24 * N * D_w + 12 * N * D_w
+        achieved = b_main / (t_quad * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "kernel": "k_adp_dq (quadrupole columns)", "achieved": achieved,
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                    "traffic": None, "bytes_per_launch": b_main, "ms_per_launch": t_quad}
+    else:
+        kernels = {"residual": t_res, "jacobian": t_jac, "normal_equations": t_ne}
     b_alg = 16 * P + 28 * N + 56 * C + 8 * R + 4 * R * D
     out = {
         "metric": "residual+Jacobian configs*params/s",
@@ -268,19 +322,12 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32" if a.precision == "mixed" else "f64",
-        "data": "synthetic (seeded ternary analytic EAM truth + noise)",
-        "config": {"workload": "ternary tabulated EAM, 500 configs of 32-2048 atoms at 0.06/A^3, "
-                               "12 tables x 20 knots (D=240), residual + full Jacobian",
+        "data": f"synthetic (seeded analytic {kind.upper()} truth + noise)",
+        "config": {"workload": desc.format(C=C, D=D),
                    "configs": C, "atoms": N, "pairs": P, "params": D, "rows": R,
                    "precision": a.precision, "lanes_per_atom": lpa},
-        "roofline": {"bound": "hbm", "kernel": "k_jac_de", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic() if C == 500 else None,
-                     "traffic_source": "profiles/k_jac_de_traffic_eam_ternary_500cfg.md (rocprofv3 "
-                                       "FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
-                     "bytes_per_launch": b_main, "ms_per_launch": t_main},
-        "kernels_ms": {"residual": t_res, "jacobian": t_jac, "jac_de": t_main, "jac_pair": t_pair,
-                       "normal_equations": t_ne},
+        "roofline": roofline,
+        "kernels_ms": kernels,
         "step_bytes_alg": b_alg,
         "step_hbm_frac": b_alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
     }

@@ -1787,10 +1787,12 @@ void launch_jacobian(Context &c) {
           else if (ns == 2) hipLaunchKernelGGL((k_adp_mom<K, 2>), dim3(g1), dim3(256), lds_mom, c.stream, a, c.nknots, sl);
           else hipLaunchKernelGGL((k_adp_mom<K, 4>), dim3(g1), dim3(256), lds_mom, c.stream, a, c.nknots, sl);
           FS_HIP(hipGetLastError());
+          if (K == JP_QUAD) c.time_main_begin(2);
           if (ns == 1) hipLaunchKernelGGL((k_adp_dq<K, 1>), dim3(g2), dim3(256), lds_dq, c.stream, a, c.nknots, sl);
           else if (ns == 2) hipLaunchKernelGGL((k_adp_dq<K, 2>), dim3(g2), dim3(256), lds_dq, c.stream, a, c.nknots, sl);
           else hipLaunchKernelGGL((k_adp_dq<K, 4>), dim3(g2), dim3(256), lds_dq, c.stream, a, c.nknots, sl);
           FS_HIP(hipGetLastError());
+          if (K == JP_QUAD) c.time_main_end(2);
         };
         pass(std::integral_constant<int, JP_DIP>{}, NSU, d_u);
         pass(std::integral_constant<int, JP_QUAD>{}, NSW, d_w);

@@ -278,12 +278,14 @@ int fs_de_impl(Context &c, const fs_de_opts *o, const double *lo, const double *
 // what: 0 = residual (density + force + finalize), 1 = Jacobian (all its
 // kernels), 2 = normal equations, 3 = residual + Jacobian, 4 = the
 // density/embedding pass of the Jacobian (ADP: with the dipole/quadrupole passes),
-// 5 = the pair-column pass (events around that launch only)
+// 5 = the pair-column pass, 6 = the ADP quadrupole-column force pass (events
+// around those launches only)
 double time_step_impl(Context &c, int what, int reps) {
+  FS_REQUIRE(what >= 0 && what <= 6, FS_ERR_ARG, "unknown timing stage");
   ensure_ready(c);
   evaluate_for_solver(c);
-  if (what == 4 || what == 5) {
-    c.time_which = what == 4 ? 0 : 1;
+  if (what >= 4) {
+    c.time_which = what - 4;
     if (!c.jev0) FS_HIP(hipEventCreate(&c.jev0));
     if (!c.jev1) FS_HIP(hipEventCreate(&c.jev1));
     launch_jacobian(c); // warm-up

@@ -239,7 +239,8 @@ int fs_autotune(fs_ctx *ctx, int32_t *lanes_per_atom);
  * on the context stream. what: 0 residual, 1 Jacobian (all its kernels),
  * 2 normal equations, 3 residual + Jacobian, 4 the density/embedding-column
  * pass of the Jacobian (ADP: with the dipole/quadrupole-column kernels that
- * follow it), 5 the pair-column pass. No reference counterpart (measurement only). */
+ * follow it), 5 the pair-column pass, 6 the ADP quadrupole-column force
+ * pass. No reference counterpart (measurement only). */
 int fs_time_step(fs_ctx *ctx, int32_t what, int32_t reps, double *ms_per_rep);
 
 /* MLP surrogate kernels on host buffers (no reference counterpart; the
