@@ -318,3 +318,53 @@ def test_lm_step(wl):
         print(f"lambda {lam:.3e}: |dx - ref| / |ref| = {rel:.3e}", flush=True)
         assert rel <= 1e-4
     assert np.array_equal(g.gather_params(), x0)  # parameters unchanged
+
+
+def test_jacobian_edge_cases_clamp_empty_and_single_atom():
+    # clamped densities (limit rows and their density chain), a one-atom
+    # configuration, an empty configuration and a small cluster; every knot free
+    from tests.helpers import box, dimer, eam_model, make_dataset, random_cluster
+    cfgs = [dimer(2.5), ([[1.0, 1.0, 1.0]], box(20.0)), (np.zeros((0, 3)), box(10.0)),
+            (random_cluster(np.random.default_rng(3), 6, 5.0), box(30.0))]
+    ds = make_dataset(cfgs)
+    m = eam_model(lambda x: 3.0 * np.exp(-2.0 * x), lambda x: 2.0 - 0.3 * x, lambda x: 1.5 - 2.0 * x,
+                  emb_span=(1.0, 5.0))
+    print("jac edge cases", flush=True)
+    o = oracle.Oracle(ds, m, 1.0, 0.3)
+    o.prepare()
+    x = o.gather()
+    Jo = o.jacobian(x)
+    g = fa.GpuCalculator(ds, m, energy_weight=1.0, stress_weight=0.3)
+    g.prepare()
+    Jg = g.jacobian(x).astype(np.float64)
+    assert Jg.shape == Jo.shape
+    assert metric(Jg, Jo) <= 2e-4
+    # the limit rows carry the clamp chain (the lone atom's rho = 0 is clamped)
+    ro = o.row_offsets()
+    lim = [ro[c + 1] - 1 for c in range(len(ro) - 1)]
+    assert np.any(Jo[lim] != 0.0)
+    assert metric(Jg[lim], Jo[lim]) <= 2e-4
+
+
+def test_adp_two_lane_slots_per_kind():
+    # 30-knot dipole and quadrupole tables, every knot free: 90 columns per
+    # kind, two lane slots each in the moment and force passes
+    w = dsm.config3(n_configs=2)
+    m = w.model
+    first = m.npair + 2 * m.ntypes  # first dipole table
+    for k in range(first, len(m.tables)):
+        t = m.tables[k]
+        xn = np.linspace(t.rmin, t.rmax, 30)
+        m.tables[k] = type(t)(30, t.rmin, t.rmax, np.interp(xn, t.x, t.y), np.zeros(30, dtype=bool))
+    print("jac adp wide maps", flush=True)
+    o = oracle.Oracle(w.dataset, m)
+    o.prepare()
+    x = o.gather()
+    assert x.size == 6 * 30 + sum(t.free_count for t in m.tables[m.npair + m.ntypes:first])
+    Jo = o.jacobian(x)
+    g = fa.GpuCalculator(w.dataset, m)
+    g.prepare()
+    Jg = g.jacobian(x).astype(np.float64)
+    mt = metric(Jg, Jo)
+    print(f"[adp wide] J metric {mt:.3e} shape {Jg.shape}", flush=True)
+    assert mt <= 2e-4
