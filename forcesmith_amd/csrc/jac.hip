@@ -60,6 +60,19 @@ __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Wave-uniform values the compiler cannot prove uniform (the wave index, and
+// loads through pointers it must assume alias the kernel's own stores): taken
+// from lane 0 they live in scalar registers, so loops and bounds on them
+// branch on the scalar unit instead of masking lanes, and the loads and LDS
+// reads of a batch issue back to back.
+__device__ inline int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline int64_t uni64(int64_t v) {
+  const uint64_t u = uint64_t(v);
+  const uint32_t lo = uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(u))));
+  const uint32_t hi = uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(u >> 32))));
+  return int64_t((uint64_t(hi) << 32) | lo);
+}
+
 template <class T> __device__ inline T wave_sum(T v) {
 #pragma unroll
   for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
@@ -292,7 +305,7 @@ __global__ __launch_bounds__(256) void k_jac_dens(JacArgs a, int nknots) {
   float *sS = reinterpret_cast<float *>(smem + a.ntab * int(sizeof(TableDesc) / sizeof(float4)));
   DensStage *sAll = reinterpret_cast<DensStage *>(sS + ((nknots * 5 + 3) & ~3));
   stage(a, sT, sS, nknots);
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = uni(threadIdx.x >> 6), lane = threadIdx.x & 63;
   DensStage &P = sAll[wave];
   const int64_t gw = int64_t(blockIdx.x) * 4 + wave;
   const int64_t nw = int64_t(gridDim.x) * 4;
@@ -310,7 +323,7 @@ __global__ __launch_bounds__(256) void k_jac_dens(JacArgs a, int nknots) {
     }
   }
   for (int64_t i = gw; i < a.N; i += nw) {
-    const int64_t p0 = a.pair_off[i], p1 = a.pair_off[i + 1];
+    const int64_t p0 = uni64(a.pair_off[i]), p1 = uni64(a.pair_off[i + 1]);
     float H[kCPL] = {0, 0, 0, 0};
     for (int64_t pc = p0; pc < p1; pc += 64) {
       const int64_t p = pc + lane;
@@ -445,7 +458,7 @@ __device__ __forceinline__ void jac_main_body(const JacArgs &a, int nknots, cons
   static_assert(4 * sizeof(Stage) >= sizeof(double) * kJP * 256, "reduction buffer");
   double(*sPart)[256] = reinterpret_cast<double(*)[256]>(sAll);
   stage(a, sT, sS, nknots);
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = uni(threadIdx.x >> 6), lane = threadIdx.x & 63;
   Stage &P = sAll[wave];
   int ccol[NS], ctab[NS], cm[NS], ckind[NS], cdc[NS];
 #pragma unroll
@@ -466,18 +479,18 @@ __device__ __forceinline__ void jac_main_body(const JacArgs &a, int nknots, cons
   const int nslot = (int(gridDim.x) - xcd + ng - 1) / ng;
   const int t_end = int(int64_t(xcd + 1) * a.ntiles / ng);
   for (int tile = int(int64_t(xcd) * a.ntiles / ng) + slot; tile < t_end; tile += nslot) {
-    const int na = a.tile_na[tile];
-    const int c = a.tile_cfg[tile];
-    const int64_t cbase = a.atom_off[c];
+    const int na = uni(a.tile_na[tile]);
+    const int c = uni(a.tile_cfg[tile]);
+    const int64_t cbase = uni64(a.atom_off[c]);
     float tp[NS][kJP];
 #pragma unroll
     for (int s = 0; s < NS; ++s)
 #pragma unroll
       for (int q = 0; q < kJP; ++q) tp[s][q] = 0.0f;
     for (int li = wave; li < na; li += 4) {
-      const int64_t i = int64_t(a.tile_a0[tile]) + li;
-      const int ti = a.types[i];
-      const int64_t p0 = a.pair_off[i], p1 = a.pair_off[i + 1];
+      const int64_t i = int64_t(uni(a.tile_a0[tile])) + li;
+      const int ti = uni(a.types[i]);
+      const int64_t p0 = uni64(a.pair_off[i]), p1 = uni64(a.pair_off[i + 1]);
       ColAcc A[NS];
 #pragma unroll
       for (int s = 0; s < NS; ++s) {
@@ -660,17 +673,23 @@ __device__ __forceinline__ void jac_main_body(const JacArgs &a, int nknots, cons
               }
             }
             // F''_j H_j g'_ti(r): every pair, rows of Hs gathered eight at a time
-            const float *Hs = a.Hs + cdc[s];
-            for (int q0 = 0; q0 < np; q0 += kHBatch) {
+            // 32-bit byte offsets from the uniform table base (the table is
+            // < 4 GiB): one VALU op per address
+            const char *Hb = reinterpret_cast<const char *>(a.Hs);
+            const uint32_t colb = uint32_t(cdc[s]) << 2;
+            auto hs = [&](int q) {
+              return *reinterpret_cast<const float *>(Hb + ((uint32_t(P.jrow[q]) << 2) + colb));
+            };
+            int q0 = 0;
+            for (; q0 + kHBatch <= np; q0 += kHBatch) {
               float h[kHBatch];
 #pragma unroll
-              for (int v = 0; v < kHBatch; ++v) h[v] = q0 + v < np ? Hs[P.jrow[q0 + v]] : 0.0f;
+              for (int v = 0; v < kHBatch; ++v) h[v] = hs(q0 + v);
 #pragma unroll
-              for (int v = 0; v < kHBatch; ++v) {
-                const int q = q0 + v;
-                if (q < np) add_force(A[s], h[v], P.hw0[q], P.hw1[q], P.hw2[q]);
-              }
+              for (int v = 0; v < kHBatch; ++v)
+                add_force(A[s], h[v], P.hw0[q0 + v], P.hw1[q0 + v], P.hw2[q0 + v]);
             }
+            for (; q0 < np; ++q0) add_force(A[s], hs(q0), P.hw0[q0], P.hw1[q0], P.hw2[q0]);
           } else if (kind == CK_EMB) {
             // F'_j through the embedding window of j, tj == t
             const int t = ctab[s] - a.npair - a.ntypes;
@@ -808,6 +827,30 @@ template <bool DIP> __device__ inline void store_moment(float *base, int64_t row
   }
 }
 
+// bytes per moment-tangent entry (atom, column)
+template <bool DIP> constexpr uint32_t moment_bytes() { return DIP ? 16u : 24u; }
+
+// the same load from a 32-bit byte offset off the uniform table base (the
+// table is < 4 GiB): one VALU op per address in the gather loops
+template <bool DIP> __device__ inline void load_moment_at(const float *base, uint32_t off, float *v) {
+  const char *b = reinterpret_cast<const char *>(base) + off;
+  if constexpr (DIP) {
+    const float4 q = *reinterpret_cast<const float4 *>(b);
+    v[0] = q.x;
+    v[1] = q.y;
+    v[2] = q.z;
+  } else {
+    const float2 *d = reinterpret_cast<const float2 *>(b);
+    const float2 a = d[0], e = d[1], c = d[2];
+    v[0] = a.x;
+    v[1] = a.y;
+    v[2] = e.x;
+    v[3] = e.y;
+    v[4] = c.x;
+    v[5] = c.y;
+  }
+}
+
 template <bool DIP> __device__ inline void load_moment(const float *base, int64_t row, float *v) {
   if constexpr (DIP) {
     const float4 q = reinterpret_cast<const float4 *>(base)[row];
@@ -853,7 +896,7 @@ __global__ __launch_bounds__(256) void k_adp_mom(JacArgs a, int nknots, const Sl
   float *sS = reinterpret_cast<float *>(smem + a.ntab * int(sizeof(TableDesc) / sizeof(float4)));
   MomStage *sAll = reinterpret_cast<MomStage *>(sS + ((nknots * 5 + 3) & ~3));
   stage(a, sT, sS, nknots);
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = uni(threadIdx.x >> 6), lane = threadIdx.x & 63;
   MomStage &P = sAll[wave];
   int cm[NS], cqc[NS], cpt[NS];
 #pragma unroll
@@ -869,8 +912,8 @@ __global__ __launch_bounds__(256) void k_adp_mom(JacArgs a, int nknots, const Sl
   const int64_t gw = int64_t(blockIdx.x) * 4 + wave;
   const int64_t nw = int64_t(gridDim.x) * 4;
   for (int64_t i = gw; i < a.N; i += nw) {
-    const int ti = a.types[i];
-    const int64_t p0 = a.pair_off[i], p1 = a.pair_off[i + 1];
+    const int ti = uni(a.types[i]);
+    const int64_t p0 = uni64(a.pair_off[i]), p1 = uni64(a.pair_off[i + 1]);
     int ptt[NS];
     float T[NS][NC];
 #pragma unroll
@@ -939,7 +982,8 @@ struct DQStage {
   float g0[64], g1[64], g2[64]; // dipole: mu_i - mu_j; quadrupole: xi(Lambda, d)
   float sg[64];                 // dipole: (mu_i - mu_j) . d / r; quadrupole: nu(Lambda, d) / r
   float v[64], dv[64];          // u (w) and u'/r (w'/r); 0 when the role is off
-  int32_t jg[64], k[64];
+  uint32_t jb[64];              // byte offset of atom j's moment-tangent row
+  int32_t k[64];
   float cd[6][kBandPitch], cv[6][kBandPitch]; // derivative and value bands
 };
 
@@ -976,7 +1020,7 @@ __global__ __launch_bounds__(256) void k_adp_dq(JacArgs a, int nknots, const Slo
   static_assert(4 * sizeof(DQStage) >= sizeof(double) * kJP * 256, "reduction buffer");
   double(*sPart)[256] = reinterpret_cast<double(*)[256]>(sAll);
   stage(a, sT, sS, nknots);
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = uni(threadIdx.x >> 6), lane = threadIdx.x & 63;
   DQStage &P = sAll[wave];
   int ccol[NS], cm[NS], cqc[NS], cpt[NS];
 #pragma unroll
@@ -997,18 +1041,18 @@ __global__ __launch_bounds__(256) void k_adp_dq(JacArgs a, int nknots, const Slo
   const int nslot = (int(gridDim.x) - xcd + ng - 1) / ng;
   const int t_end = int(int64_t(xcd + 1) * a.ntiles / ng);
   for (int tile = int(int64_t(xcd) * a.ntiles / ng) + bslot; tile < t_end; tile += nslot) {
-    const int na = a.tile_na[tile];
-    const int c = a.tile_cfg[tile];
-    const int64_t cbase = a.atom_off[c];
+    const int na = uni(a.tile_na[tile]);
+    const int c = uni(a.tile_cfg[tile]);
+    const int64_t cbase = uni64(a.atom_off[c]);
     float tp[NS][kJP];
 #pragma unroll
     for (int s = 0; s < NS; ++s)
 #pragma unroll
       for (int q = 0; q < kJP; ++q) tp[s][q] = 0.0f;
     for (int li = wave; li < na; li += 4) {
-      const int64_t i = int64_t(a.tile_a0[tile]) + li;
-      const int ti = a.types[i];
-      const int64_t p0 = a.pair_off[i], p1 = a.pair_off[i + 1];
+      const int64_t i = int64_t(uni(a.tile_a0[tile])) + li;
+      const int ti = uni(a.types[i]);
+      const int64_t p0 = uni64(a.pair_off[i]), p1 = uni64(a.pair_off[i + 1]);
       // the atom's own moment (mu_i or lambda_i) and moment tangents
       float Mi[NC];
 #pragma unroll
@@ -1061,7 +1105,7 @@ __global__ __launch_bounds__(256) void k_adp_dq(JacArgs a, int nknots, const Slo
           P.x[lane] = q.x;
           P.y[lane] = q.y;
           P.z[lane] = q.z;
-          P.jg[lane] = jg;
+          P.jb[lane] = uint32_t(jg) * uint32_t(Dq) * moment_bytes<kDip>();
           P.k[lane] = k;
           P.v[lane] = v;
           P.dv[lane] = dv * ir;
@@ -1117,41 +1161,42 @@ __global__ __launch_bounds__(256) void k_adp_dq(JacArgs a, int nknots, const Slo
           }
           // moment-tangent terms: every pair, the neighbours' tangents
           // gathered kQBatch pairs at a time
-          for (int q0 = 0; q0 < np; q0 += kQBatch) {
+          auto pair_term = [&](int q, const float *hq) {
+            const float x = P.x[q], y = P.y[q], z = P.z[q];
+            float f0, f1, f2;
+            if constexpr (kDip) {
+              const float gx = own[s][0] - hq[0], gy = own[s][1] - hq[1], gz = own[s][2] - hq[2];
+              const float sc = P.dv[q] * (gx * x + gy * y + gz * z), vv = P.v[q];
+              f0 = sc * x + vv * gx;
+              f1 = sc * y + vv * gy;
+              f2 = sc * z + vv * gz;
+            } else {
+              float dl[6], xq[3];
+#pragma unroll
+              for (int t = 0; t < 6; ++t) dl[t] = own[s][t] + hq[t];
+              const float sc = P.dv[q] * qnu(dl, x, y, z, xq), w2 = 2.0f * P.v[q];
+              f0 = sc * x + w2 * xq[0];
+              f1 = sc * y + w2 * xq[1];
+              f2 = sc * z + w2 * xq[2];
+            }
+            F[s][0] += f0;
+            F[s][1] += f1;
+            F[s][2] += f2;
+            add_dvirial(tp[s], x, y, z, f0, f1, f2);
+          };
+          const uint32_t colb = uint32_t(cqc[s]) * moment_bytes<kDip>();
+          int q0 = 0;
+          for (; q0 + kQBatch <= np; q0 += kQBatch) {
             float h[kQBatch][NC];
 #pragma unroll
-            for (int u = 0; u < kQBatch; ++u) {
-              if (q0 + u < np) load_moment<kDip>(mom, int64_t(P.jg[q0 + u]) * Dq + cqc[s], h[u]);
-              else
+            for (int u = 0; u < kQBatch; ++u) load_moment_at<kDip>(mom, P.jb[q0 + u] + colb, h[u]);
 #pragma unroll
-                for (int t = 0; t < NC; ++t) h[u][t] = 0.0f;
-            }
-#pragma unroll
-            for (int u = 0; u < kQBatch; ++u) {
-              const int q = q0 + u;
-              if (q >= np) break;
-              const float x = P.x[q], y = P.y[q], z = P.z[q];
-              float f0, f1, f2;
-              if constexpr (kDip) {
-                const float gx = own[s][0] - h[u][0], gy = own[s][1] - h[u][1], gz = own[s][2] - h[u][2];
-                const float sc = P.dv[q] * (gx * x + gy * y + gz * z), vv = P.v[q];
-                f0 = sc * x + vv * gx;
-                f1 = sc * y + vv * gy;
-                f2 = sc * z + vv * gz;
-              } else {
-                float dl[6], xq[3];
-#pragma unroll
-                for (int t = 0; t < 6; ++t) dl[t] = own[s][t] + h[u][t];
-                const float sc = P.dv[q] * qnu(dl, x, y, z, xq), w2 = 2.0f * P.v[q];
-                f0 = sc * x + w2 * xq[0];
-                f1 = sc * y + w2 * xq[1];
-                f2 = sc * z + w2 * xq[2];
-              }
-              F[s][0] += f0;
-              F[s][1] += f1;
-              F[s][2] += f2;
-              add_dvirial(tp[s], x, y, z, f0, f1, f2);
-            }
+            for (int u = 0; u < kQBatch; ++u) pair_term(q0 + u, h[u]);
+          }
+          for (; q0 < np; ++q0) {
+            float h[NC];
+            load_moment_at<kDip>(mom, P.jb[q0] + colb, h);
+            pair_term(q0, h);
           }
         }
         wave_sync();
@@ -1694,8 +1739,8 @@ void launch_jacobian(Context &c) {
     Dquad = c.jl.Dquad;
     a.Dd = Dd;
     a.gi_nf = gi_nf;
-    FS_REQUIRE(int64_t(c.N) * std::max(Dd, 1) < (int64_t(1) << 31), FS_ERR_UNSUPPORTED,
-               "density-column table exceeds 2^31 entries");
+    FS_REQUIRE(int64_t(c.N) * std::max(Dd, 1) < (int64_t(1) << 30), FS_ERR_UNSUPPORTED,
+               "density-column table exceeds 2^30 entries");
     a.dens_c0 = dens_c0;
     a.cols = reinterpret_cast<const ColDesc *>(c.jcols.p);
     a.tabs = c.tabs.p;
@@ -1770,6 +1815,8 @@ void launch_jacobian(Context &c) {
       if (adp) {
         // dipole and quadrupole columns: moment tangents, then the force pass
         const size_t nmu = size_t(c.N) * 4 * Ddip, nla = size_t(c.N) * 6 * Dquad;
+        FS_REQUIRE(4 * nmu < (size_t(1) << 32) && 4 * nla < (size_t(1) << 32), FS_ERR_UNSUPPORTED,
+                   "ADP moment-tangent table exceeds 4 GiB");
         c.adp_tan.alloc(std::max<size_t>(nmu + nla, 1));
         a.dmu = c.adp_tan.p;
         a.dlam = c.adp_tan.p + nmu;
