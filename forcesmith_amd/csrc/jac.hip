@@ -73,9 +73,22 @@ __device__ inline int64_t uni64(int64_t v) {
   return int64_t((uint64_t(hi) << 32) | lo);
 }
 
-template <class T> __device__ inline T wave_sum(T v) {
-#pragma unroll
-  for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
+// one DPP move of a float across lanes (bound_ctrl off: every source lane
+// is valid for the patterns used here)
+template <int CTRL> __device__ inline float dpp_mov(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+// sum over the wave, in every lane: within rows of 16 lanes by DPP (swap
+// neighbours, swap pairs, mirror the half-rows, mirror the row), then across
+// the four rows by two lane permutes
+__device__ inline float wave_sum(float v) {
+  v += dpp_mov<0xB1>(v);  // quad_perm [1, 0, 3, 2]
+  v += dpp_mov<0x4E>(v);  // quad_perm [2, 3, 0, 1]
+  v += dpp_mov<0x141>(v); // row_half_mirror
+  v += dpp_mov<0x140>(v); // row_mirror
+  v += __shfl_xor(v, 16, 64);
+  v += __shfl_xor(v, 32, 64);
   return v;
 }
 
@@ -215,12 +228,19 @@ __device__ inline int first_ge(const int32_t *sk, int lo, int hi, int kmin) {
   return lo;
 }
 
-// segment starts of the partner types in the staged chunk (invalid lanes
-// carry type kMaxTypes and sort last)
-__device__ inline void chunk_segments(int ty, int ntypes, int seg[kMaxTypes + 1]) {
-#pragma unroll
-  for (int t = 0; t <= kMaxTypes; ++t)
-    seg[t] = t <= ntypes ? int(__popcll(__ballot(ty < t))) : seg[ntypes];
+// segment starts of the partner types in the staged chunk, written to the
+// wave's LDS: seg[t] = staged pairs of type < t, t = 0..ntypes (invalid lanes
+// carry type kMaxTypes and sort last). Lanes then read their own column's
+// bounds with one LDS access instead of selecting among registers.
+constexpr int kSegPad = 16; // seg entries per wave (>= kMaxTypes + 1; keeps 64-B alignment)
+static_assert(kSegPad >= kMaxTypes + 1, "segment table");
+__device__ inline void stage_segments(int ty, int ntypes, int lane, int32_t *seg) {
+  int v = 0;
+  for (int u = 1; u <= ntypes; ++u) {
+    const int cnt = int(__popcll(__ballot(ty < u)));
+    v = lane == u ? cnt : v;
+  }
+  if (lane <= ntypes) seg[lane] = v;
 }
 
 // Embedding window of atom i at its clamped density: interval, FP32 value and
@@ -288,6 +308,7 @@ __device__ double emb_window(const JacArgs &a, const TableDesc *sT, int64_t i) {
 constexpr int kBandPitch = 65;
 
 struct DensStage {
+  int32_t seg[kSegPad]; // partner-type segment starts of the chunk
   int32_t k[64];
   float c[6][kBandPitch];
 };
@@ -341,8 +362,8 @@ __global__ __launch_bounds__(256) void k_jac_dens(JacArgs a, int nknots) {
 #pragma unroll
         for (int d = 0; d < 6; ++d) P.c[d][lane] = c[d];
       }
-      int seg[kMaxTypes + 1];
-      chunk_segments(ty, a.ntypes, seg);
+      stage_segments(ty, a.ntypes, lane, P.seg);
+      const int32_t *seg = P.seg;
       wave_sync();
 #pragma unroll
       for (int s = 0; s < kCPL; ++s) {
@@ -395,12 +416,14 @@ constexpr int kHBatch = 16; // Hs rows gathered per batch by the density-chain t
 constexpr int kWavesPair = 5, kWavesDE = 4; // waves per SIMD asked of the two passes
 
 struct JStagePair {
+  int32_t seg[kSegPad]; // partner-type segment starts of the chunk
   float r[64], u0[64], u1[64], u2[64];
   int32_t kphi[64];
   float cphi[6][kBandPitch], ephi[6][kBandPitch];
 };
 
 struct JStageDE {
+  int32_t seg[kSegPad]; // partner-type segment starts of the chunk
   float r[64], u0[64], u1[64], u2[64];
   float hw0[64], hw1[64], hw2[64]; // g'_ti(r) u: the density-chain weights
   int32_t jrow[64], kgj[64], kgi[64], kfj[64]; // jrow: j * Dd (row of Hs)
@@ -602,8 +625,8 @@ __device__ __forceinline__ void jac_main_body(const JacArgs &a, int nknots, cons
             Gs[5] += gr * u1 * u2;
           }
         }
-        int seg[kMaxTypes + 1];
-        chunk_segments(ty, a.ntypes, seg);
+        stage_segments(ty, a.ntypes, lane, P.seg);
+        const int32_t *seg = P.seg;
         wave_sync();
         const int np = int(min<int64_t>(64, p1 - pc));
 #pragma unroll
@@ -672,7 +695,7 @@ __device__ __forceinline__ void jac_main_body(const JacArgs &a, int nknots, cons
                   add_force(A[s], P.cgi[m - P.kgi[q] + 2][q], P.u0[q], P.u1[q], P.u2[q]);
               }
             }
-            // F''_j H_j g'_ti(r): every pair, rows of Hs gathered eight at a time
+            // F''_j H_j g'_ti(r): every pair, rows of Hs gathered kHBatch at a time
             // 32-bit byte offsets from the uniform table base (the table is
             // < 4 GiB): one VALU op per address
             const char *Hb = reinterpret_cast<const char *>(a.Hs);
@@ -870,6 +893,7 @@ template <bool DIP> __device__ inline void load_moment(const float *base, int64_
 }
 
 struct MomStage {
+  int32_t seg[kSegPad]; // partner-type segment starts of the chunk
   float d0[64], d1[64], d2[64];
   int32_t k[64];
   float c[6][kBandPitch];
@@ -942,8 +966,8 @@ __global__ __launch_bounds__(256) void k_adp_mom(JacArgs a, int nknots, const Sl
 #pragma unroll
         for (int d = 0; d < 6; ++d) P.c[d][lane] = c[d];
       }
-      int seg[kMaxTypes + 1];
-      chunk_segments(ty, a.ntypes, seg);
+      stage_segments(ty, a.ntypes, lane, P.seg);
+      const int32_t *seg = P.seg;
       wave_sync();
 #pragma unroll
       for (int s = 0; s < NS; ++s) {
@@ -978,6 +1002,7 @@ __global__ __launch_bounds__(256) void k_adp_mom(JacArgs a, int nknots, const Sl
 }
 
 struct DQStage {
+  int32_t seg[kSegPad]; // partner-type segment starts of the chunk
   float x[64], y[64], z[64];
   float g0[64], g1[64], g2[64]; // dipole: mu_i - mu_j; quadrupole: xi(Lambda, d)
   float sg[64];                 // dipole: (mu_i - mu_j) . d / r; quadrupole: nu(Lambda, d) / r
@@ -1133,8 +1158,8 @@ __global__ __launch_bounds__(256) void k_adp_dq(JacArgs a, int nknots, const Slo
             P.sg[lane] = nu * ir;
           }
         }
-        int seg[kMaxTypes + 1];
-        chunk_segments(ty, a.ntypes, seg);
+        stage_segments(ty, a.ntypes, lane, P.seg);
+        const int32_t *seg = P.seg;
         wave_sync();
         const int np = int(min<int64_t>(64, p1 - pc));
 #pragma unroll
