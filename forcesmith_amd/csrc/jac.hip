@@ -412,7 +412,7 @@ __global__ __launch_bounds__(256) void k_jac_dens(JacArgs a, int nknots) {
 //   JP_DE:   density and embedding columns (the g_tj, g_ti and F_tj bands,
 //            the density-chain and per-atom terms).
 enum { JP_PAIR = 1, JP_DE = 2 };
-constexpr int kHBatch = 16; // Hs rows gathered per batch by the density-chain term
+constexpr int kHBatch = 8; // Hs rows gathered per batch by the density-chain term
 constexpr int kWavesPair = 5, kWavesDE = 4; // waves per SIMD asked of the two passes
 
 struct JStagePair {
