@@ -241,7 +241,8 @@ def main():
     P = g.prepare()
     log(f"neighbour list: {P} pairs in {time.time() - t0:.2f} s")
     lpa = g.autotune()
-    log(f"autotuned lanes per atom: {lpa}")
+    shape = g.launch_shape()
+    log(f"autotuned lanes per atom: {lpa}, Jacobian tile: {shape['jacobian_tile_atoms']} atoms")
     x = g.gather_params()
     C, D, N, R = ds.n_configs, g.param_count, ds.n_atoms, g.residual_count
     lib, h = g._lib, g._h
@@ -325,7 +326,8 @@ def main():
         "data": f"synthetic (seeded analytic {kind.upper()} truth + noise)",
         "config": {"workload": desc.format(C=C, D=D),
                    "configs": C, "atoms": N, "pairs": P, "params": D, "rows": R,
-                   "precision": a.precision, "lanes_per_atom": lpa},
+                   "precision": a.precision, "lanes_per_atom": lpa,
+                   "jacobian_tile_atoms": shape["jacobian_tile_atoms"]},
         "roofline": roofline,
         "kernels_ms": kernels,
         "step_bytes_alg": b_alg,

@@ -38,7 +38,7 @@ EXPORTS = (
     "fs_residual_count", "fs_row_offsets", "fs_prepare", "fs_neighbor_count", "fs_get_neighbor_list",
     "fs_eval_forces", "fs_eval_config", "fs_eval_residual", "fs_eval_jacobian", "fs_normal_equations",
     "fs_lm_minimize", "fs_lm_solve", "fs_normal_equations_solve", "fs_lm_step", "fs_population_cost", "fs_de_minimize", "fs_device_buffer", "fs_synchronize",
-    "fs_set_launch", "fs_autotune", "fs_time_step", "fs_mlp_forward", "fs_mlp_loss_grad",
+    "fs_set_launch", "fs_set_jacobian_tile", "fs_get_launch", "fs_autotune", "fs_time_step", "fs_mlp_forward", "fs_mlp_loss_grad",
 )
 
 
@@ -120,6 +120,8 @@ def load_library(path: str = LIB_PATH):
         "fs_device_buffer": (ct.c_int, [vp, ct.c_int32, ct.POINTER(vp), _i64p]),
         "fs_synchronize": (ct.c_int, [vp]),
         "fs_set_launch": (ct.c_int, [vp, ct.c_int32, ct.c_int32]),
+        "fs_set_jacobian_tile": (ct.c_int, [vp, ct.c_int32]),
+        "fs_get_launch": (ct.c_int, [vp, _i32p, _i32p, _i32p]),
         "fs_autotune": (ct.c_int, [vp, _i32p]),
         "fs_time_step": (ct.c_int, [vp, ct.c_int32, ct.c_int32, _dp]),
         "fs_mlp_forward": (ct.c_int, [vp, ct.c_int32, ct.c_int32, ct.c_int32, _fp, _fp, _fp]),
@@ -376,7 +378,22 @@ class GpuCalculator:
     def set_launch(self, lanes_per_atom: int, blocks_per_cu: int = 8):
         self._check(self._lib.fs_set_launch(self._h, int(lanes_per_atom), int(blocks_per_cu)))
 
+    def set_jacobian_tile(self, atoms: int = 0):
+        """Atoms per Jacobian tile (4, 8, 16, 32 or 64; 0 picks it from the
+        atom count)."""
+        self._check(self._lib.fs_set_jacobian_tile(self._h, int(atoms)))
+
+    def launch_shape(self) -> dict:
+        lpa, bpc, jt = ct.c_int32(), ct.c_int32(), ct.c_int32()
+        self._check(self._lib.fs_get_launch(self._h, ct.byref(lpa), ct.byref(bpc), ct.byref(jt)))
+        return {"lanes_per_atom": int(lpa.value), "blocks_per_cu": int(bpc.value),
+                "jacobian_tile_atoms": int(jt.value)}
+
     def autotune(self) -> int:
+        """Times the launch shapes on the resident dataset and keeps the
+        fastest: lanes per atom over the residual, then atoms per Jacobian
+        tile over the whole Jacobian. Returns the lanes per atom
+        (launch_shape() has both)."""
         v = ct.c_int32()
         self._check(self._lib.fs_autotune(self._h, ct.byref(v)))
         return int(v.value)

@@ -145,6 +145,9 @@ struct Context {
   int cu_count = 256;
   int lanes_per_atom = 16;
   int blocks_per_cu = 8;
+  // atoms per Jacobian tile: 0 picks it from the atom count, else a power of
+  // two in [4, 64] (fs_set_jacobian_tile, fs_autotune)
+  int jac_tile = 0;
 
   // ---- dataset (host mirror + device) ------------------------------------
   int C = 0;
@@ -230,6 +233,7 @@ struct Context {
   // density histograms and embedding windows, SW tangents
   DBuf<int32_t> jt_cfg, jt_a0, jt_na, jt_ct0;
   int njt = 0;
+  int jt_atoms = 0; // atoms per tile of the current Jacobian tiles
   DBuf<int4> jcols, jslots;
   // column descriptors / lane-slot maps derived from the model (jcols, jslots
   // hold them on the device); rebuilt after every model change

@@ -860,6 +860,24 @@ int fs_set_launch(fs_ctx *ctx, int32_t lanes_per_atom, int32_t blocks_per_cu) {
   });
 }
 
+int fs_set_jacobian_tile(fs_ctx *ctx, int32_t atoms) {
+  return guarded(ctx, [&](Context &c) {
+    FS_REQUIRE(atoms == 0 || atoms == 4 || atoms == 8 || atoms == 16 || atoms == 32 || atoms == 64,
+               FS_ERR_ARG, "Jacobian tile must be 0 (from the atom count) or 4, 8, 16, 32 or 64 atoms");
+    c.jac_tile = atoms;
+    c.njt = 0;
+  });
+}
+
+int fs_get_launch(fs_ctx *ctx, int32_t *lanes_per_atom, int32_t *blocks_per_cu,
+                  int32_t *jacobian_tile) {
+  return guarded(ctx, [&](Context &c) {
+    if (lanes_per_atom) *lanes_per_atom = c.lanes_per_atom;
+    if (blocks_per_cu) *blocks_per_cu = c.blocks_per_cu;
+    if (jacobian_tile) *jacobian_tile = c.njt > 0 ? c.jt_atoms : c.jac_tile;
+  });
+}
+
 int fs_autotune(fs_ctx *ctx, int32_t *lanes_per_atom) {
   int rc = FS_OK;
   const int g = guarded(ctx, [&](Context &c) {

@@ -1559,6 +1559,7 @@ void launch_jacobian(Context &c) {
     int jt = 4;
     while (jt < kJacTile && jt < want) jt *= 2;
     if (want >= kJacTile / 2) jt = kJacTile;
+    if (c.jac_tile > 0) jt = c.jac_tile;
     std::vector<int32_t> tcfg, ta0, tna, ct0(c.C + 1, 0);
     for (int k = 0; k < c.C; ++k) {
       ct0[k] = int32_t(tcfg.size());
@@ -1580,6 +1581,7 @@ void launch_jacobian(Context &c) {
     FS_HIP(hipMemcpyAsync(c.jt_ct0.p, ct0.data(), 4 * (c.C + 1), hipMemcpyHostToDevice, c.stream));
     FS_HIP(hipStreamSynchronize(c.stream));
     c.njt = n;
+    c.jt_atoms = jt;
   }
   const int nt = c.njt;
   c.jac_part.alloc(size_t(std::max(nt, 1)) * kJP * std::max(c.D, 1));

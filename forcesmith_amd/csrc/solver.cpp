@@ -344,6 +344,27 @@ int autotune_impl(Context &c) {
   c.lanes_per_atom = best;
   rebuild_tiles(c);
   c.forces_valid = false;
+  // Jacobian tile: powers of two from 4 to 64 atoms, timed over the whole
+  // Jacobian; candidates that would cut the dataset into more than 64 tiles
+  // per CU are skipped (their per-tile partials and row reductions grow with
+  // the tile count, and the chip is full long before that)
+  if (c.N > 0 && c.D > 0) {
+    const int keep = c.jac_tile;
+    int best_jt = -1;
+    double best_jt_ms = std::numeric_limits<double>::infinity();
+    for (int jt = 4; jt <= 64; jt *= 2) {
+      if (jt < 64 && c.N / jt > int64_t(64) * c.cu_count) continue;
+      c.jac_tile = jt;
+      c.njt = 0;
+      const double ms = time_step_impl(c, 1, 3);
+      if (ms < best_jt_ms) {
+        best_jt_ms = ms;
+        best_jt = jt;
+      }
+    }
+    c.jac_tile = best_jt > 0 ? best_jt : keep;
+    c.njt = 0;
+  }
   return FS_OK;
 }
 

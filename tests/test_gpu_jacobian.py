@@ -55,6 +55,42 @@ def test_jacobian_matches_oracle(wl, name):
     assert m <= 2e-4
 
 
+@pytest.mark.parametrize("name", ["eam3s", "sw", "adp2"])
+def test_jacobian_tile_sizes_agree(wl, name):
+    # every atom's force rows come from one wave whatever the tile, so they
+    # are bit-identical across tile sizes; the energy, stress and limit rows
+    # sum per-tile FP64 partials in another grouping (FP64 rounding only)
+    w = wl[name]
+    print(f"jac tiles {name}", flush=True)
+    g = fa.GpuCalculator(w.dataset, w.model)
+    g.prepare()
+    x = g.gather_params()
+    nrow = 3 * np.asarray(w.dataset.config_sizes())
+    ro = g.row_offsets()
+    force = np.zeros(g.residual_count, bool)
+    for c in range(len(nrow)):
+        force[ro[c]:ro[c] + nrow[c]] = True
+    ref = None
+    for jt in (4, 8, 16, 32, 64):
+        g.set_jacobian_tile(jt)
+        J = g.jacobian(x).astype(np.float64)
+        assert g.launch_shape()["jacobian_tile_atoms"] == jt
+        if ref is None:
+            ref = J
+            continue
+        assert np.array_equal(J[force], ref[force]), jt
+        assert np.max(np.abs(J[~force] - ref[~force]) / (1.0 + np.abs(ref[~force]))) <= 1e-6, jt
+    g.set_jacobian_tile(0)
+    g.autotune()
+    shape = g.launch_shape()
+    assert shape["jacobian_tile_atoms"] in (4, 8, 16, 32, 64)
+    assert shape["lanes_per_atom"] in (4, 8, 16, 32, 64)
+    J = g.jacobian(x).astype(np.float64)
+    assert np.array_equal(J[force], ref[force])
+    with pytest.raises(fa.FsError):
+        g.set_jacobian_tile(12)
+
+
 def test_jacobian_rows_by_kind(wl):
     # force, energy, stress and limit rows are each exercised
     w = wl["eam3s"]
