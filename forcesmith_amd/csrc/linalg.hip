@@ -11,6 +11,7 @@
 #include "common.hpp"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace fsa {
 
@@ -24,112 +25,178 @@ constexpr int kMT = 16; // MFMA tile edge (v_mfma_f64_16x16x4f64)
 __host__ __device__ inline int tile_pair(int I, int J, int nt) { return I * nt - I * (I - 1) / 2 + (J - I); }
 
 // One pass over a chunk of Jacobian rows on the FP64 matrix cores. Wave w
-// owns tile rows I0 = w and I1 = nt - 1 - w of the upper triangle of J^T J
-// (nt + 1 tiles per wave, so the waves are balanced and two waves share a
-// SIMD within its 512 registers), four rows per step, plus the Jt r entries of
-// its columns. Operand layout of v_mfma_f64_16x16x4f64 (measured on gfx950):
-// lane l holds A[l % 16][l / 16] and B[l / 16][l % 16], and after the product
-// D[4 v + l / 16][l % 16] in element v. A is the transposed column tile I of
-// four rows and B the column tile J of the same rows: the FP32 entries are
-// widened to FP64 and every product and sum is FP64. Partials per chunk are
-// reduced in chunk order (deterministic).
-constexpr int kMaxWaveTiles = 17; // nt + 1 for nt <= 16
-constexpr int kNeRows = 64;       // Jacobian rows per LDS stage
+// owns tile rows I0 = w and I1 = NT - 1 - w of the upper triangle of J^T J
+// (NT + 1 tiles per wave, so the waves are balanced; for odd NT the middle
+// wave owns row I0 = I1 alone, (NT + 1) / 2 tiles), four rows per step, plus
+// the Jt r entries of its columns. Operand layout of v_mfma_f64_16x16x4f64
+// (measured on gfx950): lane l holds A[l % 16][l / 16] and B[l / 16][l % 16],
+// and after the product D[4 v + l / 16][l % 16] in element v. A is the
+// transposed column tile I of four rows and B the column tile J of the same
+// rows: the FP32 entries are widened to FP64 and every product and sum is
+// FP64. Partials per chunk are reduced in chunk order (deterministic).
+//
+// NT (tiles per row of the grid) is a template parameter, so a wave's tile
+// list is a fixed, fully unrolled sequence of independent accumulators: a
+// step reads all its B operands from LDS first and then issues its MFMAs back
+// to back. The 32-row slabs are double-buffered in LDS: the next slab's
+// global loads are in flight in registers while the current one is
+// multiplied, and one barrier per slab separates the buffers.
+// 32-row slabs: 16 floats per thread in flight. (64-row slabs, 32 floats in
+// flight, spill more registers and measured 3 % slower; 64-row slabs staged
+// without double buffering were 0.2 % slower on 240 columns and 12 % on 140.)
+constexpr int kNeRows = 32; // Jacobian rows per LDS stage
+constexpr int kNePre = 16;  // floats per thread of a slab in flight (see launch)
 
 // LDS row pitch: D rounded up to 32 floats plus 8, so the four rows one MFMA
 // step reads start 8 banks apart and every bank serves exactly two lanes
 __host__ __device__ inline int ne_pitch(int D) { return ((D + 31) / 32) * 32 + 8; }
 
+// The first K of the KA tile slots: slot j < n0 is tile (I0, I0 + j), slot
+// j >= n0 is tile (I1, I1 + j - n0); n0 is wave-uniform
+template <int K, int KA>
+__device__ __forceinline__ void ne_step(const float *row, double f, int c0, int c1, int cl,
+                                        int I0, int I1, int n0, d4v (&acc)[KA], double &g0,
+                                        double &g1) {
+  const double a0 = double(row[c0]);
+  const double a1 = double(row[c1]);
+  g0 += a0 * f;
+  g1 += a1 * f;
+  float b[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    const int jt = j < n0 ? I0 + j : I1 + (j - n0);
+    b[j] = row[kMT * jt + cl];
+  }
+#pragma unroll
+  for (int j = 0; j < K; ++j)
+    acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(j < n0 ? a0 : a1, double(b[j]), acc[j], 0, 0, 0);
+}
+
+template <int K, int KA>
+__device__ __forceinline__ void ne_store(const d4v (&acc)[KA], double *o, int I0, int I1, int n0,
+                                         int NT, int kr, int cl) {
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    const int I = j < n0 ? I0 : I1;
+    const int Jt = j < n0 ? I0 + j : I1 + (j - n0);
+    double *t = o + int64_t(tile_pair(I, Jt, NT)) * 256;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) t[(4 * v + kr) * kMT + cl] = acc[j][v];
+  }
+}
+
+template <int NT>
 __global__ __launch_bounds__(512) void k_ne_mfma(const float *__restrict__ J,
                                                  const double *__restrict__ F, int64_t R, int D,
-                                                 int nt, int nchunk, double *__restrict__ part) {
-  extern __shared__ float sJ[]; // [kNeRows][pitch]
-  __shared__ double sF[kNeRows];
+                                                 int nchunk, double *__restrict__ part) {
+  constexpr int KW = NT + 1, KM = (NT + 1) / 2; // tile slots: paired rows, middle row
+  extern __shared__ float sJ[];                 // [2][kNeRows][pitch]
+  __shared__ double sF[2][kNeRows];
   const int pitch = ne_pitch(D);
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  const int I0 = w, I1 = nt - 1 - w;
-  const bool live = I0 <= I1;               // waves past the middle row only load
-  const int n0 = live ? nt - I0 : 0;        // tiles of row I0
-  const int n1 = I1 > I0 ? nt - I1 : 0;     // tiles of row I1 (none when it is I0)
-  const int ntp = nt * (nt + 1) / 2;
+  const int w = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6)), l = threadIdx.x & 63;
+  const int I0 = w, I1 = NT - 1 - w;
+  const bool mid = I0 == I1; // odd NT: the middle wave owns one tile row
+  const int n0 = NT - I0;
+  const int ntp = NT * (NT + 1) / 2;
   const int64_t rows_per = (((R + nchunk - 1) / nchunk) + 3) & ~int64_t(3);
   const int64_t r0 = int64_t(blockIdx.x) * rows_per;
   const int64_t r1 = r0 + rows_per < R ? r0 + rows_per : R;
   const int kr = l >> 4, cl = l & 15;
   const int c0 = kMT * I0 + cl, c1 = kMT * I1 + cl;
-  d4v acc[kMaxWaveTiles];
+  const int nthr = blockDim.x;
+  d4v acc[KW];
 #pragma unroll
-  for (int j = 0; j < kMaxWaveTiles; ++j) acc[j] = d4v{0.0, 0.0, 0.0, 0.0};
+  for (int j = 0; j < KW; ++j) acc[j] = d4v{0.0, 0.0, 0.0, 0.0};
   double g0 = 0.0, g1 = 0.0;
-  // pad columns D .. pitch stay zero for the whole pass
-  for (int e = threadIdx.x; e < kNeRows * (pitch - D); e += blockDim.x)
+  // pad columns D .. pitch of both buffers stay zero for the whole pass
+  for (int e = threadIdx.x; e < 2 * kNeRows * (pitch - D); e += nthr)
     sJ[(e / (pitch - D)) * pitch + D + e % (pitch - D)] = 0.0f;
-  const bool vec = (D & 3) == 0; // rows start 16-byte aligned: float4 staging
-  for (int64_t sb = r0; sb < r1; sb += kNeRows) {
+  // a slab in flight: float4 elements (rows 16-byte aligned) or single floats
+  const bool vec = (D & 3) == 0;
+  const int q = vec ? D >> 2 : D; // elements per row
+  float pre[kNePre];
+  double fpre = 0.0;
+  auto load = [&](int64_t sb) {
     const int nr = int(r1 - sb < kNeRows ? r1 - sb : kNeRows);
-    // stage nr rows; a short last stage is zero-padded to kNeRows rows
+    const int ne = nr * q;
     if (vec) {
-      const int q4 = D >> 2;
       const float4 *src = reinterpret_cast<const float4 *>(J + sb * D);
-      for (int e = threadIdx.x; e < nr * q4; e += blockDim.x) {
-        const int rr = e / q4, cc = 4 * (e % q4);
-        const float4 v = src[e];
-        float *d = sJ + rr * pitch + cc;
-        d[0] = v.x;
-        d[1] = v.y;
-        d[2] = v.z;
-        d[3] = v.w;
+#pragma unroll
+      for (int k = 0; k < kNePre / 4; ++k) {
+        const int e = threadIdx.x + k * nthr;
+        const float4 v = e < ne ? src[e] : float4{0.0f, 0.0f, 0.0f, 0.0f};
+        pre[4 * k] = v.x;
+        pre[4 * k + 1] = v.y;
+        pre[4 * k + 2] = v.z;
+        pre[4 * k + 3] = v.w;
       }
     } else {
-      for (int e = threadIdx.x; e < nr * D; e += blockDim.x) {
-        const int rr = e / D, cc = e % D;
-        sJ[rr * pitch + cc] = J[(sb + rr) * D + cc];
+      const float *src = J + sb * D;
+#pragma unroll
+      for (int k = 0; k < kNePre; ++k) {
+        const int e = threadIdx.x + k * nthr;
+        pre[k] = e < ne ? src[e] : 0.0f;
       }
     }
-    for (int e = threadIdx.x; e < (kNeRows - nr) * D; e += blockDim.x)
-      sJ[(nr + e / D) * pitch + e % D] = 0.0f;
-    for (int e = threadIdx.x; e < kNeRows; e += blockDim.x) sF[e] = e < nr ? F[sb + e] : 0.0;
-    __syncthreads();
-    if (live) {
-      for (int rb = 0; rb < nr; rb += 4) {
-        const float *row = sJ + (rb + kr) * pitch;
-        const double f = sF[rb + kr];
-        const double a0 = double(row[c0]);
-        const double a1 = n1 > 0 ? double(row[c1]) : 0.0;
-        g0 += a0 * f;
-        g1 += a1 * f;
+    fpre = int(threadIdx.x) < nr ? F[sb + threadIdx.x] : 0.0;
+  };
+  // rows past the slab's end are written as zeros (a short last slab)
+  auto store = [&](int buf) {
+    float *dst = sJ + buf * kNeRows * pitch;
+    const int ne = kNeRows * q;
+    if (vec) {
 #pragma unroll
-        for (int j = 0; j < kMaxWaveTiles; ++j) {
-          if (j < n0 + n1) { // uniform across the wave
-            const int Jt = j < n0 ? I0 + j : I1 + (j - n0);
-            const double b = double(row[kMT * Jt + cl]);
-            acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(j < n0 ? a0 : a1, b, acc[j], 0, 0, 0);
-          }
+      for (int k = 0; k < kNePre / 4; ++k) {
+        const int e = threadIdx.x + k * nthr;
+        if (e < ne) {
+          float *d = dst + (e / q) * pitch + 4 * (e % q);
+          d[0] = pre[4 * k];
+          d[1] = pre[4 * k + 1];
+          d[2] = pre[4 * k + 2];
+          d[3] = pre[4 * k + 3];
         }
       }
+    } else {
+#pragma unroll
+      for (int k = 0; k < kNePre; ++k) {
+        const int e = threadIdx.x + k * nthr;
+        if (e < ne) dst[(e / q) * pitch + e % q] = pre[k];
+      }
     }
+    if (int(threadIdx.x) < kNeRows) sF[buf][threadIdx.x] = fpre;
+  };
+  if (r0 < r1) {
+    load(r0);
+    store(0);
+  }
+  __syncthreads();
+  int buf = 0;
+  for (int64_t sb = r0; sb < r1; sb += kNeRows, buf ^= 1) {
+    const int nr = int(r1 - sb < kNeRows ? r1 - sb : kNeRows);
+    const bool more = sb + kNeRows < r1;
+    if (more) load(sb + kNeRows); // in flight during the products below
+    const float *base = sJ + buf * kNeRows * pitch;
+    if (mid) {
+      for (int rb = 0; rb < nr; rb += 4)
+        ne_step<KM>(base + (rb + kr) * pitch, sF[buf][rb + kr], c0, c0, cl, I0, I0, n0, acc, g0, g1);
+    } else {
+      for (int rb = 0; rb < nr; rb += 4)
+        ne_step<KW>(base + (rb + kr) * pitch, sF[buf][rb + kr], c0, c1, cl, I0, I1, n0, acc, g0, g1);
+    }
+    if (more) store(buf ^ 1);
     __syncthreads();
   }
-  if (!live) return;
   // Jt r: the four lanes of one column hold its four row-phase sums
   g0 += __shfl_xor(g0, 16, 64);
   g0 += __shfl_xor(g0, 32, 64);
   g1 += __shfl_xor(g1, 16, 64);
   g1 += __shfl_xor(g1, 32, 64);
   double *o = part + int64_t(blockIdx.x) * (int64_t(ntp) * 256 + D);
-#pragma unroll
-  for (int j = 0; j < kMaxWaveTiles; ++j) {
-    if (j < n0 + n1) {
-      const int I = j < n0 ? I0 : I1;
-      const int Jt = j < n0 ? I0 + j : I1 + (j - n0);
-      double *t = o + int64_t(tile_pair(I, Jt, nt)) * 256;
-#pragma unroll
-      for (int v = 0; v < 4; ++v) t[(4 * v + kr) * kMT + cl] = acc[j][v];
-    }
-  }
+  if (mid) ne_store<KM>(acc, o, I0, I0, n0, NT, kr, cl);
+  else ne_store<KW>(acc, o, I0, I1, n0, NT, kr, cl);
   if (kr == 0) {
     if (c0 < D) o[int64_t(ntp) * 256 + c0] = g0;
-    if (n1 > 0 && c1 < D) o[int64_t(ntp) * 256 + c1] = g1;
+    if (!mid && c1 < D) o[int64_t(ntp) * 256 + c1] = g1;
   }
 }
 
@@ -350,9 +417,35 @@ void launch_normal_equations(Context &c) {
   const int64_t per = int64_t(ntp) * 256 + D;
   c.jtj_part.alloc(size_t(nchunk + kNeGroups) * per);
   double *part2 = c.jtj_part.p + size_t(nchunk) * per;
-  const size_t lds = size_t(kNeRows) * ne_pitch(D) * sizeof(float);
-  hipLaunchKernelGGL(k_ne_mfma, dim3(nchunk), dim3(64 * ((nt + 1) / 2)), lds, c.stream, c.J.p,
-                     c.residual.p, R, D, nt, nchunk, c.jtj_part.p);
+  // two slab buffers; a thread holds at most kNePre floats of a slab in
+  // flight (kNeRows rows of D floats over 64 (nt + 1) / 2 threads: D <= 16 nt)
+  const size_t lds = 2 * size_t(kNeRows) * ne_pitch(D) * sizeof(float);
+  const int nthr = 64 * ((nt + 1) / 2);
+  FS_REQUIRE((kNeRows * D + nthr - 1) / nthr <= kNePre, FS_ERR_UNSUPPORTED,
+             "normal-equation slab does not fit the per-thread staging");
+  auto go = [&](auto ntc) {
+    constexpr int NT = decltype(ntc)::value;
+    hipLaunchKernelGGL((k_ne_mfma<NT>), dim3(nchunk), dim3(nthr), lds, c.stream, c.J.p,
+                       c.residual.p, R, D, nchunk, c.jtj_part.p);
+  };
+  switch (nt) {
+  case 1: go(std::integral_constant<int, 1>{}); break;
+  case 2: go(std::integral_constant<int, 2>{}); break;
+  case 3: go(std::integral_constant<int, 3>{}); break;
+  case 4: go(std::integral_constant<int, 4>{}); break;
+  case 5: go(std::integral_constant<int, 5>{}); break;
+  case 6: go(std::integral_constant<int, 6>{}); break;
+  case 7: go(std::integral_constant<int, 7>{}); break;
+  case 8: go(std::integral_constant<int, 8>{}); break;
+  case 9: go(std::integral_constant<int, 9>{}); break;
+  case 10: go(std::integral_constant<int, 10>{}); break;
+  case 11: go(std::integral_constant<int, 11>{}); break;
+  case 12: go(std::integral_constant<int, 12>{}); break;
+  case 13: go(std::integral_constant<int, 13>{}); break;
+  case 14: go(std::integral_constant<int, 14>{}); break;
+  case 15: go(std::integral_constant<int, 15>{}); break;
+  default: go(std::integral_constant<int, 16>{}); break;
+  }
   FS_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_ne_reduce1, dim3(unsigned((per + 255) / 256), kNeGroups), dim3(256), 0,
                      c.stream, per, nchunk, c.jtj_part.p, part2);
