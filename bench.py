@@ -74,6 +74,8 @@ def parse():
                     help="precision of the plain DE run's batched costs (the baseline: FP64)")
     ap.add_argument("--de-surrogate-precision", default="mixed", choices=["mixed", "fp64"],
                     help="precision of the screened run's exact costs")
+    ap.add_argument("--jacobian-tile", type=int, default=0,
+                    help="atoms per Jacobian tile (4-64); 0 lets the autotuner time the sizes")
     ap.add_argument("--pmc-driver", action="store_true",
                     help="counter-collection driver: a 1 GiB copy (calibration) then three "
                          "residual + Jacobian evaluations; prints no JSON")
@@ -202,7 +204,10 @@ def pmc_driver(a):
     w = datasets.config2(n_configs=a.configs)
     g = fa.GpuCalculator(w.dataset, w.model)
     g.prepare()
-    g.set_launch(8)
+    if a.jacobian_tile:
+        g.set_jacobian_tile(a.jacobian_tile)
+    g.autotune()
+    log(f"launch shape {g.launch_shape()}")
     lib, h = g._lib, g._h
     for _ in range(3):
         g._check(lib.fs_eval_residual(h, None, None))
@@ -240,6 +245,8 @@ def main():
     t0 = time.time()
     P = g.prepare()
     log(f"neighbour list: {P} pairs in {time.time() - t0:.2f} s")
+    if a.jacobian_tile:
+        g.set_jacobian_tile(a.jacobian_tile)
     lpa = g.autotune()
     shape = g.launch_shape()
     log(f"autotuned lanes per atom: {lpa}, Jacobian tile: {shape['jacobian_tile_atoms']} atoms")

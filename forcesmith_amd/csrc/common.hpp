@@ -148,6 +148,7 @@ struct Context {
   // atoms per Jacobian tile: 0 picks it from the atom count, else a power of
   // two in [4, 64] (fs_set_jacobian_tile, fs_autotune)
   int jac_tile = 0;
+  bool jac_tile_fixed = false; // set by hand: fs_autotune keeps it
 
   // ---- dataset (host mirror + device) ------------------------------------
   int C = 0;

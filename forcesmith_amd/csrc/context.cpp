@@ -865,6 +865,7 @@ int fs_set_jacobian_tile(fs_ctx *ctx, int32_t atoms) {
     FS_REQUIRE(atoms == 0 || atoms == 4 || atoms == 8 || atoms == 16 || atoms == 32 || atoms == 64,
                FS_ERR_ARG, "Jacobian tile must be 0 (from the atom count) or 4, 8, 16, 32 or 64 atoms");
     c.jac_tile = atoms;
+    c.jac_tile_fixed = atoms != 0;
     c.njt = 0;
   });
 }

@@ -347,8 +347,9 @@ int autotune_impl(Context &c) {
   // Jacobian tile: powers of two from 4 to 64 atoms, timed over the whole
   // Jacobian; candidates that would cut the dataset into more than 64 tiles
   // per CU are skipped (their per-tile partials and row reductions grow with
-  // the tile count, and the chip is full long before that)
-  if (c.N > 0 && c.D > 0) {
+  // the tile count, and the chip is full long before that); a tile fixed by
+  // fs_set_jacobian_tile is kept
+  if (!c.jac_tile_fixed && c.N > 0 && c.D > 0) {
     const int keep = c.jac_tile;
     int best_jt = -1;
     double best_jt_ms = std::numeric_limits<double>::infinity();

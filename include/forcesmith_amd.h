@@ -235,8 +235,9 @@ int fs_synchronize(fs_ctx *ctx);
  * fastest (lanes per atom over the residual, then the Jacobian tile size). */
 int fs_set_launch(fs_ctx *ctx, int32_t lanes_per_atom, int32_t blocks_per_cu);
 /* Atoms per Jacobian tile: 0 (the default) picks it from the atom count, else
- * 4, 8, 16, 32 or 64. fs_autotune also times the tile sizes over the whole
- * Jacobian and keeps the fastest. fs_get_launch reports the current shape
+ * 4, 8, 16, 32 or 64. Unless a size is fixed here (non-zero), fs_autotune
+ * also times the tile sizes over the whole Jacobian and keeps the fastest.
+ * fs_get_launch reports the current shape
  * (jacobian_tile: the tile size of the last Jacobian, or the requested one
  * before the first). No reference counterpart (launch tuning). */
 int fs_set_jacobian_tile(fs_ctx *ctx, int32_t atoms);

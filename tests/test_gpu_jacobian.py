@@ -87,6 +87,11 @@ def test_jacobian_tile_sizes_agree(wl, name):
     assert shape["lanes_per_atom"] in (4, 8, 16, 32, 64)
     J = g.jacobian(x).astype(np.float64)
     assert np.array_equal(J[force], ref[force])
+    # a tile fixed by hand survives the autotuner
+    g.set_jacobian_tile(16)
+    g.autotune()
+    g.jacobian(x)
+    assert g.launch_shape()["jacobian_tile_atoms"] == 16
     with pytest.raises(fa.FsError):
         g.set_jacobian_tile(12)
 
