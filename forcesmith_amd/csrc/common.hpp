@@ -134,6 +134,8 @@ void population_prepare(Context &c, int np);
 void population_launch(Context &c, const double *d_X, int np, double *d_cost);
 void launch_sw_rev(Context &c);
 void launch_sw_eval(Context &c);
+// per-pair SW scalars {v2, v2', h, h', dh/dgamma, dh'/dgamma, dh/da2, dh'/da2}
+void launch_sw_pair_terms(Context &c);
 
 struct Context {
   std::mutex lock;
@@ -224,6 +226,7 @@ struct Context {
   DBuf<double> forces, tile_part, cfg_E, cfg_S, cfg_L;
   DBuf<double> residual, cost, cost_part;
   DBuf<double> pairbuf; // SW triplet forces per pair
+  DBuf<double> sw_pv;   // SW per-pair scalars, 8 per pair
   bool forces_valid = false;
   // Jacobian
   DBuf<float> J;

@@ -1266,16 +1266,37 @@ __global__ void k_jac_rows(JacArgs a) {
   const int64_t e_row = a.row_off[c] + 3 * na;
   const bool ws = a.sw > 0.0;
   const double iv = 1.0 / a.vol[c];
-  for (int col = threadIdx.x; col < a.D; col += blockDim.x) {
-    double acc[kJP];
-    for (int q = 0; q < kJP; ++q) acc[q] = 0.0;
-    for (int t = a.cfg_tile0[c]; t < a.cfg_tile0[c + 1]; ++t)
-      for (int q = 0; q < kJP; ++q) acc[q] += a.part[(int64_t(t) * kJP + q) * a.D + col];
+  // one thread per (partial row q, column): the tile partials of a
+  // configuration are contiguous in (tile, q, column), so a wave reads
+  // consecutive doubles; four tiles' loads are in flight before their sums,
+  // which are added in tile order
+  const int t0 = a.cfg_tile0[c], t1 = a.cfg_tile0[c + 1];
+  const int nqc = kJP * a.D;
+  for (int e = threadIdx.x; e < nqc; e += blockDim.x) {
+    const int q = e / a.D, col = e - q * a.D;
+    if (!ws && q >= 1 && q <= 6) continue;
+    const double *p = a.part + int64_t(t0) * nqc + e;
+    double acc = 0.0;
+    int t = t0;
+    for (; t + 4 <= t1; t += 4, p += 4 * int64_t(nqc)) {
+      const double v0 = p[0], v1 = p[nqc], v2 = p[2 * int64_t(nqc)], v3 = p[3 * int64_t(nqc)];
+      acc += v0;
+      acc += v1;
+      acc += v2;
+      acc += v3;
+    }
+    for (; t < t1; ++t, p += nqc) acc += p[0];
+    float v;
     int64_t row = e_row;
-    a.J[row++ * a.D + col] = float(a.ew * acc[0]);
-    if (ws)
-      for (int v = 0; v < 6; ++v) a.J[row++ * a.D + col] = float(-0.5 * a.sw * acc[1 + v] * iv);
-    a.J[row * a.D + col] = float(acc[7]);
+    if (q == 0) v = float(a.ew * acc);
+    else if (q == 7) {
+      v = float(acc);
+      row += ws ? 7 : 1;
+    } else {
+      v = float(-0.5 * a.sw * acc * iv);
+      row += q;
+    }
+    a.J[row * a.D + col] = v;
   }
 }
 
@@ -1320,41 +1341,6 @@ __device__ inline void v2_tan(double r, const SwT &p, int f, double &v2, double 
   tdv = dP * e + dpoly * E + P * e * g + poly * E * g + poly * e * Gt;
 }
 
-__device__ inline void h_tan(double r, const SwT &p, int f, double &h, double &dh, double &th,
-                             double &tdh) {
-  h = dh = th = tdh = 0.0;
-  if (r >= p.a2) return;
-  const double dd = r - p.a2;
-  const double e = exp(p.gamma / dd);
-  if (e == 0.0) return;
-  const double g = -p.gamma / (dd * dd);
-  h = e;
-  dh = e * g;
-  double H = 0.0, Gt = 0.0;
-  if (f == 6) {
-    H = e / dd;
-    Gt = -1.0 / (dd * dd);
-  } else if (f == 7) {
-    H = e * p.gamma / (dd * dd);
-    Gt = -2.0 * p.gamma / (dd * dd * dd);
-  }
-  th = H;
-  tdh = H * g + e * Gt;
-}
-
-__device__ inline void tb(const double *d1, const double *d2, double i1, double i2, double c,
-                          double f1, double df1, double f2, double df2, double g, double dg,
-                          double *fi, double *fj, double *fk) {
-  const double i12 = i1 * i2;
-  const double A = c * i1 * i1 - i12, B = c * i2 * i2 - i12;
-  const double s1 = df1 * i1 * f2 * g, s2 = f1 * df2 * i2 * g, s3 = f1 * f2 * dg;
-  for (int x = 0; x < 3; ++x) {
-    fi[x] = s1 * d1[x] + s2 * d2[x] - s3 * (A * d1[x] + B * d2[x]);
-    fj[x] = -s1 * d1[x] - s3 * (i12 * d2[x] - c * i1 * i1 * d1[x]);
-    fk[x] = -s2 * d2[x] - s3 * (i12 * d1[x] - c * i2 * i2 * d2[x]);
-  }
-}
-
 struct SwJArgs {
   int64_t N;
   int D, ntypes, npair;
@@ -1364,6 +1350,7 @@ struct SwJArgs {
   const uint32_t *bits;
   const int32_t *rev;
   const double *sw, *lam;
+  const double *pv;       // 8 per pair: v2, v2', h, h' and the gamma / a2 tangents of h, h'
   const int32_t *col_sw;  // npair*8 -> column or -1
   const int32_t *col_lam; // nlam -> column or -1
   double *own;            // [D][N][3]
@@ -1385,107 +1372,125 @@ __device__ inline double pr64(const double *pd, int64_t p) {
   return sqrt(r2);
 }
 
-__global__ __launch_bounds__(128) void k_sw_jac_centre(SwJArgs a) {
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+// j force of the triplet (i; j, k) (three_body_forces, kernels.hpp:40-56),
+// multilinear in (f1, df1), (f2), (g, dg): its tangent is the sum of three
+// calls with one factor group replaced by its tangent
+__device__ inline void tb_j(const double *d1, const double *d2, double i1, double i2, double c,
+                            double f1, double df1, double f2, double g, double dg, double *fj) {
+  const double i12 = i1 * i2;
+  const double s1 = df1 * i1 * f2 * g, s3 = f1 * f2 * dg;
+  for (int x = 0; x < 3; ++x) fj[x] = -s1 * d1[x] - s3 * (i12 * d2[x] - c * i1 * i1 * d1[x]);
+}
+
+constexpr int kSwJLanes = 16; // lanes per centre atom (a divisor of 64)
+
+__device__ inline double sw_group_sum(double v) {
+#pragma unroll
+  for (int o = kSwJLanes / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Tangents of the SW forces, energy and virial of one centre atom w.r.t. one
+// column (blockIdx.y). A group of kSwJLanes lanes takes the centre; the lane of
+// neighbour j forms the tangent of the pair term of (i, j) and of the triplet
+// force on j summed over every other neighbour k, T_j, which it writes into
+// the per-pair slot of (i -> j). The centre's tangent force is the pair terms
+// minus sum_j T_j (a triplet's forces sum to zero for every parameter value,
+// so their tangents do too) and the triplet virial tangent sum_j d_j (x) T_j.
+__global__ __launch_bounds__(256) void k_sw_jac_centre(SwJArgs a) {
+  const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t i = g / kSwJLanes;
+  const int l = int(g % kSwJLanes);
   const int col = blockIdx.y;
-  if (i >= a.N) return;
-  const int ti = a.types[i];
-  const int64_t p0 = a.pair_off[i], p1 = a.pair_off[i + 1];
+  const bool live = i < a.N; // every lane takes part in the group sums
   double F[3] = {0, 0, 0}, E = 0.0, sg[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   double *pt = a.ptan + int64_t(col) * a.P * 3;
-  for (int64_t p = p0; p < p1; ++p) pt[3 * p] = pt[3 * p + 1] = pt[3 * p + 2] = 0.0;
-  for (int64_t p = p0; p < p1; ++p) {
-    const uint32_t b = a.bits[p];
-    if (b & kFlagTiny) continue;
-    const int slot = ord_pair(ti, pj_type(b), a.ntypes);
-    int f = -1;
-    for (int q = 0; q < 6; ++q)
-      if (a.col_sw[8 * slot + q] == col) f = q;
-    if (f < 0) continue;
-    const double r = pr64(a.pd, p);
-    double v2, dv2, tv, tdv;
-    v2_tan(r, lsw(a.sw, slot), f, v2, dv2, tv, tdv);
-    const double s = tdv / r;
-    const double *d = a.pd + 3 * p;
-    const double df[3] = {s * d[0], s * d[1], s * d[2]};
-    for (int x = 0; x < 3; ++x) F[x] += df[x];
-    E += 0.5 * tv;
-    for (int x = 0; x < 3; ++x)
-      for (int y = 0; y < 3; ++y) sg[3 * x + y] -= 0.5 * d[x] * df[y];
-  }
-  for (int64_t pj = p0; pj < p1; ++pj) {
-    const uint32_t bj = a.bits[pj];
-    if (bj & kFlagTiny) continue;
-    const int tj = pj_type(bj);
-    const int s1 = ord_pair(ti, tj, a.ntypes);
-    const double r1 = pr64(a.pd, pj);
-    double h1, dh1, th, tdh;
-    h_tan(r1, lsw(a.sw, s1), -1, h1, dh1, th, tdh);
-    if (h1 == 0.0 && dh1 == 0.0) continue;
-    const double i1 = 1.0 / r1;
-    const double *d1 = a.pd + 3 * pj;
-    for (int64_t pk = pj + 1; pk < p1; ++pk) {
-      const uint32_t bk = a.bits[pk];
-      if (bk & kFlagTiny) continue;
-      const int tk = pj_type(bk);
-      const int s2 = ord_pair(ti, tk, a.ntypes);
-      const double r2 = pr64(a.pd, pk);
-      double h2, dh2;
-      h_tan(r2, lsw(a.sw, s2), -1, h2, dh2, th, tdh);
-      if (h2 == 0.0 && dh2 == 0.0) continue;
-      const double i2 = 1.0 / r2;
-      const double *d2 = a.pd + 3 * pk;
-      const int li = ti * a.npair + ord_pair(tj, tk, a.ntypes);
-      const double lam = a.lam[li];
-      const double cs = (d1[0] * d2[0] + d1[1] * d2[1] + d1[2] * d2[2]) * i1 * i2;
-      const double cp = cs + 1.0 / 3.0;
-      const double w = lam * cp * cp, dw = 2.0 * lam * cp;
-      // tangents of (h1, dh1, h2, dh2, w, dw) w.r.t. this column
-      double t1 = 0, td1 = 0, t2 = 0, td2 = 0, tg = 0, tdg = 0;
-      bool any = false;
-      for (int f = 6; f < 8; ++f) {
-        if (a.col_sw[8 * s1 + f] == col) {
-          double hh, dd, a1_, b1_;
-          h_tan(r1, lsw(a.sw, s1), f, hh, dd, a1_, b1_);
-          t1 += a1_;
-          td1 += b1_;
-          any = true;
+  if (live) {
+    const int ti = a.types[i];
+    const int64_t p0 = a.pair_off[i], p1 = a.pair_off[i + 1];
+    for (int64_t pj = p0 + l; pj < p1; pj += kSwJLanes) {
+      double T[3] = {0, 0, 0};
+      const uint32_t bj = a.bits[pj];
+      if (!(bj & kFlagTiny)) {
+        const int tj = pj_type(bj);
+        const int s1 = ord_pair(ti, tj, a.ntypes);
+        const double *d1 = a.pd + 3 * pj;
+        const double r1 = pr64(a.pd, pj);
+        // pair term: the column's field among the slot's two-body fields
+        int f = -1;
+        for (int q = 0; q < 6; ++q)
+          if (a.col_sw[8 * s1 + q] == col) f = q;
+        if (f >= 0) {
+          double v2, dv2, tv, tdv;
+          v2_tan(r1, lsw(a.sw, s1), f, v2, dv2, tv, tdv);
+          const double s = tdv / r1;
+          const double df[3] = {s * d1[0], s * d1[1], s * d1[2]};
+          for (int x = 0; x < 3; ++x) F[x] += df[x];
+          E += 0.5 * tv;
+          for (int x = 0; x < 3; ++x)
+            for (int y = 0; y < 3; ++y) sg[3 * x + y] -= 0.5 * d1[x] * df[y];
         }
-        if (a.col_sw[8 * s2 + f] == col) {
-          double hh, dd, a2_, b2_;
-          h_tan(r2, lsw(a.sw, s2), f, hh, dd, a2_, b2_);
-          t2 += a2_;
-          td2 += b2_;
-          any = true;
+        const double *q1 = a.pv + 8 * pj;
+        const double h1 = q1[2], dh1 = q1[3];
+        if (h1 != 0.0 || dh1 != 0.0) {
+          // tangent of (h1, h1') when the column is gamma or a2 of slot s1
+          double t1 = 0.0, td1 = 0.0;
+          if (a.col_sw[8 * s1 + 6] == col) {
+            t1 += q1[4];
+            td1 += q1[5];
+          }
+          if (a.col_sw[8 * s1 + 7] == col) {
+            t1 += q1[6];
+            td1 += q1[7];
+          }
+          const double i1 = 1.0 / r1;
+          for (int64_t pk = p0; pk < p1; ++pk) {
+            if (pk == pj) continue;
+            const uint32_t bk = a.bits[pk];
+            if (bk & kFlagTiny) continue;
+            const double *q2 = a.pv + 8 * pk;
+            const double h2 = q2[2], dh2 = q2[3];
+            if (h2 == 0.0 && dh2 == 0.0) continue;
+            const int s2 = ord_pair(ti, pj_type(bk), a.ntypes);
+            double t2 = 0.0;
+            if (a.col_sw[8 * s2 + 6] == col) t2 += q2[4];
+            if (a.col_sw[8 * s2 + 7] == col) t2 += q2[6];
+            const int li = ti * a.npair + ord_pair(tj, pj_type(bk), a.ntypes);
+            const bool lam_col = a.col_lam[li] == col;
+            if (t1 == 0.0 && td1 == 0.0 && t2 == 0.0 && !lam_col) continue;
+            const double i2 = 1.0 / pr64(a.pd, pk);
+            const double *d2 = a.pd + 3 * pk;
+            const double lam = a.lam[li];
+            const double cs = (d1[0] * d2[0] + d1[1] * d2[1] + d1[2] * d2[2]) * i1 * i2;
+            const double cp = cs + 1.0 / 3.0;
+            const double w = lam * cp * cp, dw = 2.0 * lam * cp;
+            const double tg = lam_col ? cp * cp : 0.0, tdg = lam_col ? 2.0 * cp : 0.0;
+            double fa[3], fb[3], fc[3];
+            tb_j(d1, d2, i1, i2, cs, t1, td1, h2, w, dw, fa);
+            tb_j(d1, d2, i1, i2, cs, h1, dh1, t2, w, dw, fb);
+            tb_j(d1, d2, i1, i2, cs, h1, dh1, h2, tg, tdg, fc);
+            for (int x = 0; x < 3; ++x) T[x] += fa[x] + fb[x] + fc[x];
+            // each triplet is met from both of its bonds
+            E += 0.5 * (t1 * h2 * w + h1 * t2 * w + h1 * h2 * tg);
+          }
+          for (int x = 0; x < 3; ++x) F[x] -= T[x];
+          for (int x = 0; x < 3; ++x)
+            for (int y = 0; y < 3; ++y) sg[3 * x + y] += d1[x] * T[y];
         }
       }
-      if (a.col_lam[li] == col) {
-        tg = cp * cp;
-        tdg = 2.0 * cp;
-        any = true;
-      }
-      if (!any) continue;
-      double fi[3], fj[3], fk[3], xi[3], xj[3], xk[3];
-      tb(d1, d2, i1, i2, cs, t1, td1, h2, dh2, w, dw, fi, fj, fk);
-      tb(d1, d2, i1, i2, cs, h1, dh1, t2, td2, w, dw, xi, xj, xk);
-      for (int x = 0; x < 3; ++x) { fi[x] += xi[x]; fj[x] += xj[x]; fk[x] += xk[x]; }
-      tb(d1, d2, i1, i2, cs, h1, dh1, h2, dh2, tg, tdg, xi, xj, xk);
-      for (int x = 0; x < 3; ++x) { fi[x] += xi[x]; fj[x] += xj[x]; fk[x] += xk[x]; }
-      E += t1 * h2 * w + h1 * t2 * w + h1 * h2 * tg;
-      for (int x = 0; x < 3; ++x) {
-        F[x] += fi[x];
-        pt[3 * pj + x] += fj[x];
-        pt[3 * pk + x] += fk[x];
-      }
-      for (int x = 0; x < 3; ++x)
-        for (int y = 0; y < 3; ++y) sg[3 * x + y] += d1[x] * fj[y] + d2[x] * fk[y];
+      for (int x = 0; x < 3; ++x) pt[3 * pj + x] = T[x];
     }
   }
-  double *o = a.own + (int64_t(col) * a.N + i) * 3;
-  for (int x = 0; x < 3; ++x) o[x] = F[x];
-  double *e = a.etan + (int64_t(col) * a.N + i) * 10;
-  e[0] = E;
-  for (int k = 0; k < 9; ++k) e[1 + k] = sg[k];
+  for (int x = 0; x < 3; ++x) F[x] = sw_group_sum(F[x]);
+  E = sw_group_sum(E);
+  for (int k = 0; k < 9; ++k) sg[k] = sw_group_sum(sg[k]);
+  if (live && l == 0) {
+    double *o = a.own + (int64_t(col) * a.N + i) * 3;
+    for (int x = 0; x < 3; ++x) o[x] = F[x];
+    double *e = a.etan + (int64_t(col) * a.N + i) * 10;
+    e[0] = E;
+    for (int k = 0; k < 9; ++k) e[1 + k] = sg[k];
+  }
 }
 
 // one block per tile, x = atoms of the tile, y = column
@@ -1623,6 +1628,8 @@ void launch_jacobian(Context &c) {
     s.rev = c.rev.p;
     s.sw = c.sw.p;
     s.lam = c.lam.p;
+    launch_sw_pair_terms(c);
+    s.pv = c.sw_pv.p;
     std::vector<int32_t> csw(8 * c.npair), clam(c.h_lam.size());
     int o = 0;
     for (size_t k = 0; k < csw.size(); ++k) csw[k] = c.h_sw_fixed[k] ? -1 : o++;
@@ -1650,7 +1657,8 @@ void launch_jacobian(Context &c) {
     s.part = c.jac_part.p;
     if (c.N > 0 && c.D > 0) {
       c.time_main_begin();
-      hipLaunchKernelGGL(k_sw_jac_centre, dim3(nb(c.N, 128), c.D), dim3(128), 0, c.stream, s);
+      hipLaunchKernelGGL(k_sw_jac_centre, dim3(nb(c.N * kSwJLanes, 256), c.D), dim3(256), 0,
+                         c.stream, s);
       FS_HIP(hipGetLastError());
       c.time_main_end();
       hipLaunchKernelGGL(k_sw_jac_tiles, dim3(std::max(nt, 1), c.D), dim3(64), 0, c.stream, s);

@@ -1,9 +1,16 @@
 // Stillinger-Weber residual path: pair term plus triplets centred on each
 // atom, with the reference's scatter of the triplet forces onto j and k
 // (include/forcesmith/force/kernels.hpp:58-66) recast as a deterministic
-// gather: the centre's thread writes the j- and k-forces into per-pair slots
-// of its own neighbour segment, and every atom then gathers the slots of the
-// reverse pairs (k -> i for each of its pairs i -> k).
+// gather. k_sw_pair_terms evaluates the radial factors once per pair; in
+// k_sw_centre a group of kSwLanes lanes takes one centre atom i, and the lane
+// of neighbour j sums over every other neighbour k the triplet force on j,
+//   T_j = sum_k f_j(i; j, k),
+// and writes it into the per-pair slot of (i -> j). A triplet's three forces
+// sum to zero (the term depends on the two bond vectors only), so the centre
+// force is the pair forces minus sum_j T_j, and the triplet virial
+// sum (d_j (x) f_j + d_k (x) f_k) over triplets is sum_j d_j (x) T_j. Every
+// atom then gathers the slots of the reverse pairs (k -> i for each of its
+// pairs i -> k).
 //
 // Reference: StiwebForceCalculator::eval_forces (src/force/stiweb_force.cpp:
 // 241-271), v2_dv2 / h_dh (:24-54), three_body_forces (kernels.hpp:40-56).
@@ -50,20 +57,6 @@ __device__ inline void h_dh(double r, const SwP &p, double &h, double &dh) {
   dh = e * (-p.gamma / (d * d));
 }
 
-__device__ inline void three_body(const double *d1, const double *d2, double inv_r1, double inv_r2,
-                                  double c, double f1, double df1, double f2, double df2, double g,
-                                  double dg, double *fi, double *fj, double *fk) {
-  const double inv_r1r2 = inv_r1 * inv_r2;
-  const double A = c * inv_r1 * inv_r1 - inv_r1r2;
-  const double B = c * inv_r2 * inv_r2 - inv_r1r2;
-  const double s1 = df1 * inv_r1 * f2 * g, s2 = f1 * df2 * inv_r2 * g, s3 = f1 * f2 * dg;
-  for (int a = 0; a < 3; ++a) {
-    fi[a] = s1 * d1[a] + s2 * d2[a] - s3 * (A * d1[a] + B * d2[a]);
-    fj[a] = -s1 * d1[a] - s3 * (inv_r1r2 * d2[a] - c * inv_r1 * inv_r1 * d1[a]);
-    fk[a] = -s2 * d2[a] - s3 * (inv_r1r2 * d1[a] - c * inv_r2 * inv_r2 * d2[a]);
-  }
-}
-
 // reverse pair: for p = (i -> j, image n) the entry (j -> i, image -n)
 __global__ void k_sw_rev(int64_t N, const int64_t *__restrict__ pair_off,
                          const int32_t *__restrict__ cfg_of, const int64_t *__restrict__ atom_off,
@@ -88,6 +81,56 @@ __global__ void k_sw_rev(int64_t N, const int64_t *__restrict__ pair_off,
   }
 }
 
+constexpr int kSwLanes = 16; // lanes per centre atom (a divisor of 64)
+
+// sum over the kSwLanes lanes of one centre's group (all lanes of the wave
+// take part)
+__device__ inline double group_sum(double v) {
+#pragma unroll
+  for (int o = kSwLanes / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// per pair (i -> j): {v2, v2', h, h', dh/dgamma, dh'/dgamma, dh/da2, dh'/da2}
+// of the pair's parameter slot (ti, tj); zeros for an r < 1e-14 pair
+__global__ __launch_bounds__(256) void k_sw_pair_terms(int64_t N, const int64_t *__restrict__ pair_off,
+                                                       const int32_t *__restrict__ types,
+                                                       const double *__restrict__ pd,
+                                                       const uint32_t *__restrict__ bits,
+                                                       const double *__restrict__ sw, int ntypes,
+                                                       double *__restrict__ pv) {
+  const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t i = g / kSwLanes;
+  const int l = int(g % kSwLanes);
+  if (i >= N) return;
+  const int ti = types[i];
+  for (int64_t p = pair_off[i] + l; p < pair_off[i + 1]; p += kSwLanes) {
+    double o[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const uint32_t b = bits[p];
+    if (!(b & kFlagTiny)) {
+      const double d0 = pd[3 * p], d1 = pd[3 * p + 1], d2 = pd[3 * p + 2];
+      double r2 = d0 * d0 + d1 * d1;
+      r2 = r2 + d2 * d2;
+      const double r = sqrt(r2);
+      const SwP P = load_sw(sw, ord_pair(ti, pj_type(b), ntypes));
+      v2_dv2(r, P, o[0], o[1]);
+      h_dh(r, P, o[2], o[3]);
+      if (o[2] != 0.0 || o[3] != 0.0) {
+        // tangents of (h, h') w.r.t. gamma and a2 (h = exp(gamma / (r - a2)))
+        const double dd = r - P.a2, e = o[2], g1 = -P.gamma / (dd * dd);
+        const double Hg = e / dd, Gg = -1.0 / (dd * dd);
+        o[4] = Hg;
+        o[5] = Hg * g1 + e * Gg;
+        const double Ha = e * P.gamma / (dd * dd), Ga = -2.0 * P.gamma / (dd * dd * dd);
+        o[6] = Ha;
+        o[7] = Ha * g1 + e * Ga;
+      }
+    }
+    double *q = pv + 8 * p;
+    for (int k = 0; k < 8; ++k) q[k] = o[k];
+  }
+}
+
 struct SwArgs {
   int64_t N;
   const int64_t *pair_off;
@@ -97,6 +140,7 @@ struct SwArgs {
   const int32_t *rev;
   const double *sw, *lam;
   int ntypes, npair;
+  const double *pv; // 8 per pair (k_sw_pair_terms)
   double *pairbuf;  // 3 per pair: force on the neighbour from triplets centred here
   double *fcentre;  // 3 per atom
   double *e_atom;   // per atom energy
@@ -115,71 +159,80 @@ __device__ inline double pr(const double *pd, int64_t p) {
   return sqrt(r2);
 }
 
-__global__ __launch_bounds__(128) void k_sw_centre(SwArgs a) {
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= a.N) return;
-  const int ti = a.types[i];
-  const int64_t p0 = a.pair_off[i], p1 = a.pair_off[i + 1];
+// force on j of the triplet (i; j, k) with bond vectors d1 = d_ij, d2 = d_ik,
+// radial factors f1 = h(r1), f2 = h(r2) and angular factor g = w(cos) (the j
+// force of three_body_forces, kernels.hpp:40-56; by symmetry the same formula
+// with j and k exchanged is its k force)
+__device__ inline void three_body_j(const double *d1, const double *d2, double inv_r1, double inv_r2,
+                                    double c, double f1, double df1, double f2, double g, double dg,
+                                    double *fj) {
+  const double inv_r1r2 = inv_r1 * inv_r2;
+  const double s1 = df1 * inv_r1 * f2 * g, s3 = f1 * f2 * dg;
+  for (int x = 0; x < 3; ++x) fj[x] = -s1 * d1[x] - s3 * (inv_r1r2 * d2[x] - c * inv_r1 * inv_r1 * d1[x]);
+}
+
+__global__ __launch_bounds__(256) void k_sw_centre(SwArgs a) {
+  const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t i = g / kSwLanes;
+  const int l = int(g % kSwLanes);
+  const bool live = i < a.N; // every lane takes part in the group sums
   double F[3] = {0, 0, 0}, E = 0.0, sg[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (int64_t p = p0; p < p1; ++p) {
-    a.pairbuf[3 * p] = 0.0;
-    a.pairbuf[3 * p + 1] = 0.0;
-    a.pairbuf[3 * p + 2] = 0.0;
-  }
-  for (int64_t p = p0; p < p1; ++p) {
-    const uint32_t b = a.bits[p];
-    if (b & kFlagTiny) continue;
-    const double r = pr(a.pd, p);
-    const SwP P = load_sw(a.sw, ord_pair(ti, pj_type(b), a.ntypes));
-    double v2, dv2;
-    v2_dv2(r, P, v2, dv2);
-    const double s = dv2 / r;
-    const double *d = a.pd + 3 * p;
-    const double f[3] = {s * d[0], s * d[1], s * d[2]};
-    for (int k = 0; k < 3; ++k) F[k] += f[k];
-    E += 0.5 * v2;
-    for (int x = 0; x < 3; ++x)
-      for (int y = 0; y < 3; ++y) sg[3 * x + y] -= 0.5 * d[x] * f[y];
-  }
-  for (int64_t pj = p0; pj < p1; ++pj) {
-    const uint32_t bj = a.bits[pj];
-    if (bj & kFlagTiny) continue;
-    const int tj = pj_type(bj);
-    const double r1 = pr(a.pd, pj);
-    double h1, dh1;
-    h_dh(r1, load_sw(a.sw, ord_pair(ti, tj, a.ntypes)), h1, dh1);
-    if (h1 == 0.0 && dh1 == 0.0) continue;
-    const double inv_r1 = 1.0 / r1;
-    const double *d1 = a.pd + 3 * pj;
-    for (int64_t pk = pj + 1; pk < p1; ++pk) {
-      const uint32_t bk = a.bits[pk];
-      if (bk & kFlagTiny) continue;
-      const int tk = pj_type(bk);
-      const double r2 = pr(a.pd, pk);
-      double h2, dh2;
-      h_dh(r2, load_sw(a.sw, ord_pair(ti, tk, a.ntypes)), h2, dh2);
-      if (h2 == 0.0 && dh2 == 0.0) continue;
-      const double inv_r2 = 1.0 / r2;
-      const double *d2 = a.pd + 3 * pk;
-      const double lam = a.lam[ti * a.npair + ord_pair(tj, tk, a.ntypes)];
-      const double cs = (d1[0] * d2[0] + d1[1] * d2[1] + d1[2] * d2[2]) * inv_r1 * inv_r2;
-      const double cp13 = cs + 1.0 / 3.0;
-      const double w = lam * cp13 * cp13, dw = 2.0 * lam * cp13;
-      E += h1 * h2 * w;
-      double fi[3], fj[3], fk[3];
-      three_body(d1, d2, inv_r1, inv_r2, cs, h1, dh1, h2, dh2, w, dw, fi, fj, fk);
-      for (int k = 0; k < 3; ++k) {
-        F[k] += fi[k];
-        a.pairbuf[3 * pj + k] += fj[k];
-        a.pairbuf[3 * pk + k] += fk[k];
+  if (live) {
+    const int ti = a.types[i];
+    const int64_t p0 = a.pair_off[i], p1 = a.pair_off[i + 1];
+    for (int64_t pj = p0 + l; pj < p1; pj += kSwLanes) {
+      double T[3] = {0, 0, 0};
+      const uint32_t bj = a.bits[pj];
+      if (!(bj & kFlagTiny)) {
+        const double *d1 = a.pd + 3 * pj;
+        const double r1 = pr(a.pd, pj);
+        const double *q1 = a.pv + 8 * pj;
+        const double v2 = q1[0], dv2 = q1[1], h1 = q1[2], dh1 = q1[3];
+        // pair term
+        const double s = dv2 / r1;
+        const double f[3] = {s * d1[0], s * d1[1], s * d1[2]};
+        for (int k = 0; k < 3; ++k) F[k] += f[k];
+        E += 0.5 * v2;
+        for (int x = 0; x < 3; ++x)
+          for (int y = 0; y < 3; ++y) sg[3 * x + y] -= 0.5 * d1[x] * f[y];
+        // triplets (i; j, k) over every other neighbour k
+        if (h1 != 0.0 || dh1 != 0.0) {
+          const int tj = pj_type(bj);
+          const double inv_r1 = 1.0 / r1;
+          for (int64_t pk = p0; pk < p1; ++pk) {
+            if (pk == pj) continue;
+            const uint32_t bk = a.bits[pk];
+            if (bk & kFlagTiny) continue;
+            const double *q2 = a.pv + 8 * pk;
+            const double h2 = q2[2], dh2 = q2[3];
+            if (h2 == 0.0 && dh2 == 0.0) continue;
+            const double inv_r2 = 1.0 / pr(a.pd, pk);
+            const double *d2 = a.pd + 3 * pk;
+            const double lam = a.lam[ti * a.npair + ord_pair(tj, pj_type(bk), a.ntypes)];
+            const double cs = (d1[0] * d2[0] + d1[1] * d2[1] + d1[2] * d2[2]) * inv_r1 * inv_r2;
+            const double cp13 = cs + 1.0 / 3.0;
+            const double w = lam * cp13 * cp13, dw = 2.0 * lam * cp13;
+            E += 0.5 * (h1 * h2 * w); // each triplet is met from both of its bonds
+            double fj[3];
+            three_body_j(d1, d2, inv_r1, inv_r2, cs, h1, dh1, h2, w, dw, fj);
+            for (int k = 0; k < 3; ++k) T[k] += fj[k];
+          }
+          for (int k = 0; k < 3; ++k) F[k] -= T[k];
+          for (int x = 0; x < 3; ++x)
+            for (int y = 0; y < 3; ++y) sg[3 * x + y] += d1[x] * T[y];
+        }
       }
-      for (int x = 0; x < 3; ++x)
-        for (int y = 0; y < 3; ++y) sg[3 * x + y] += d1[x] * fj[y] + d2[x] * fk[y];
+      for (int k = 0; k < 3; ++k) a.pairbuf[3 * pj + k] = T[k];
     }
   }
-  for (int k = 0; k < 3; ++k) a.fcentre[3 * i + k] = F[k];
-  a.e_atom[i] = E;
-  for (int k = 0; k < 9; ++k) a.sig_atom[9 * i + k] = sg[k];
+  for (int k = 0; k < 3; ++k) F[k] = group_sum(F[k]);
+  E = group_sum(E);
+  for (int k = 0; k < 9; ++k) sg[k] = group_sum(sg[k]);
+  if (live && l == 0) {
+    for (int k = 0; k < 3; ++k) a.fcentre[3 * i + k] = F[k];
+    a.e_atom[i] = E;
+    for (int k = 0; k < 9; ++k) a.sig_atom[9 * i + k] = sg[k];
+  }
 }
 
 // one block per tile: gather the reverse-pair slots, write force rows, reduce
@@ -240,6 +293,7 @@ SwArgs make_sw_args(Context &c) {
   a.lam = c.lam.p;
   a.ntypes = c.ntypes;
   a.npair = c.npair;
+  a.pv = c.sw_pv.p;
   a.pairbuf = c.pairbuf.p;
   a.fcentre = c.Fpp.p;
   a.e_atom = c.e_atom.p;
@@ -258,11 +312,23 @@ SwArgs make_sw_args(Context &c) {
   return a;
 }
 
+void launch_sw_pair_terms(Context &c) {
+  c.sw_pv.alloc(8 * size_t(std::max<int64_t>(c.P, 1)));
+  if (c.N == 0) return;
+  const int64_t nthr = c.N * kSwLanes;
+  hipLaunchKernelGGL(k_sw_pair_terms, dim3(unsigned((nthr + 255) / 256)), dim3(256), 0, c.stream,
+                     c.N, c.pair_off.p, c.types.p, c.pair_d.p, c.pair_bits.p, c.sw.p, c.ntypes,
+                     c.sw_pv.p);
+  FS_HIP(hipGetLastError());
+}
+
 void launch_sw_eval(Context &c) {
   FS_REQUIRE(c.tile_atoms <= 64, FS_ERR_UNSUPPORTED, "SW tiles hold at most 64 atoms");
+  launch_sw_pair_terms(c);
   const SwArgs a = make_sw_args(c);
   if (c.N > 0)
-    hipLaunchKernelGGL(k_sw_centre, dim3(unsigned((c.N + 127) / 128)), dim3(128), 0, c.stream, a);
+    hipLaunchKernelGGL(k_sw_centre, dim3(unsigned((c.N * kSwLanes + 255) / 256)), dim3(256), 0,
+                       c.stream, a);
   FS_HIP(hipGetLastError());
   if (c.ntiles > 0)
     hipLaunchKernelGGL(k_sw_tiles, dim3(c.ntiles), dim3(64), 0, c.stream, a);
