@@ -227,6 +227,7 @@ struct Context {
   DBuf<double> residual, cost, cost_part;
   DBuf<double> pairbuf; // SW triplet forces per pair
   DBuf<double> sw_pv;   // SW per-pair scalars, 8 per pair
+  DBuf<double> sw_pt2;  // SW per-pair two-body tangents, 12 per pair
   bool forces_valid = false;
   // Jacobian
   DBuf<float> J;

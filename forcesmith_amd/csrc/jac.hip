@@ -1311,10 +1311,12 @@ __device__ inline SwT lsw(const double *sw, int s) {
   return SwT{q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7]};
 }
 
-// values and tangents w.r.t. field f (0..5) of (v2, dv2)
-__device__ inline void v2_tan(double r, const SwT &p, int f, double &v2, double &dv2, double &tv,
-                              double &tdv) {
-  v2 = dv2 = tv = tdv = 0.0;
+// tangents (tv, tdv) of (v2, v2') w.r.t. each two-body field f = 0..5 (A, B,
+// p, q, delta, a1), the parameter derivatives of v2_dv2
+// (stiweb_force.cpp:24-41), with the powers, the logarithm and the
+// exponential evaluated once for all fields
+__device__ inline void v2_tan_all(double r, const SwT &p, double *t) {
+  for (int k = 0; k < 12; ++k) t[k] = 0.0;
   if (r >= p.a1) return;
   const double dd = r - p.a1;
   const double e = exp(p.delta / dd);
@@ -1325,20 +1327,19 @@ __device__ inline void v2_tan(double r, const SwT &p, int f, double &v2, double 
   const double poly = p.A * rp - p.B * rq;
   const double dpoly = -p.A * p.p * rp1 + p.B * p.q * rq1;
   const double g = -p.delta / (dd * dd);
-  v2 = poly * e;
-  dv2 = dpoly * e + poly * e * g;
-  double P = 0.0, dP = 0.0, E = 0.0, Gt = 0.0;
-  switch (f) {
-  case 0: P = rp; dP = -p.p * rp1; break;
-  case 1: P = -rq; dP = p.q * rq1; break;
-  case 2: P = -p.A * lr * rp; dP = p.A * rp1 * (p.p * lr - 1.0); break;
-  case 3: P = p.B * lr * rq; dP = p.B * rq1 * (1.0 - p.q * lr); break;
-  case 4: E = e / dd; Gt = -1.0 / (dd * dd); break;
-  case 5: E = e * p.delta / (dd * dd); Gt = -2.0 * p.delta / (dd * dd * dd); break;
-  default: break;
+  for (int f = 0; f < 6; ++f) {
+    double P = 0.0, dP = 0.0, E = 0.0, Gt = 0.0;
+    switch (f) {
+    case 0: P = rp; dP = -p.p * rp1; break;
+    case 1: P = -rq; dP = p.q * rq1; break;
+    case 2: P = -p.A * lr * rp; dP = p.A * rp1 * (p.p * lr - 1.0); break;
+    case 3: P = p.B * lr * rq; dP = p.B * rq1 * (1.0 - p.q * lr); break;
+    case 4: E = e / dd; Gt = -1.0 / (dd * dd); break;
+    default: E = e * p.delta / (dd * dd); Gt = -2.0 * p.delta / (dd * dd * dd); break;
+    }
+    t[2 * f] = P * e + poly * E;
+    t[2 * f + 1] = dP * e + dpoly * E + P * e * g + poly * E * g + poly * e * Gt;
   }
-  tv = P * e + poly * E;
-  tdv = dP * e + dpoly * E + P * e * g + poly * E * g + poly * e * Gt;
 }
 
 struct SwJArgs {
@@ -1351,6 +1352,7 @@ struct SwJArgs {
   const int32_t *rev;
   const double *sw, *lam;
   const double *pv;       // 8 per pair: v2, v2', h, h' and the gamma / a2 tangents of h, h'
+  double *pt2;            // 12 per pair: (tv, tdv) of v2 w.r.t. the two-body fields (k_sw_pair_tan)
   const int32_t *col_sw;  // npair*8 -> column or -1
   const int32_t *col_lam; // nlam -> column or -1
   double *own;            // [D][N][3]
@@ -1384,6 +1386,27 @@ __device__ inline void tb_j(const double *d1, const double *d2, double i1, doubl
 
 constexpr int kSwJLanes = 16; // lanes per centre atom (a divisor of 64)
 
+// per pair (i -> j): the two-body tangents of its slot (ti, tj), every field
+// (zeros for an r < 1e-14 pair)
+__global__ __launch_bounds__(256) void k_sw_pair_tan(SwJArgs a) {
+  const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t i = g / kSwJLanes;
+  const int l = int(g % kSwJLanes);
+  if (i >= a.N) return;
+  const int ti = a.types[i];
+  for (int64_t p = a.pair_off[i] + l; p < a.pair_off[i + 1]; p += kSwJLanes) {
+    double t[12];
+    const uint32_t b = a.bits[p];
+    if (b & kFlagTiny) {
+      for (int k = 0; k < 12; ++k) t[k] = 0.0;
+    } else {
+      v2_tan_all(pr64(a.pd, p), lsw(a.sw, ord_pair(ti, pj_type(b), a.ntypes)), t);
+    }
+    double *o = a.pt2 + 12 * p;
+    for (int k = 0; k < 12; ++k) o[k] = t[k];
+  }
+}
+
 __device__ inline double sw_group_sum(double v) {
 #pragma unroll
   for (int o = kSwJLanes / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -1405,6 +1428,11 @@ __global__ __launch_bounds__(256) void k_sw_jac_centre(SwJArgs a) {
   const bool live = i < a.N; // every lane takes part in the group sums
   double F[3] = {0, 0, 0}, E = 0.0, sg[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   double *pt = a.ptan + int64_t(col) * a.P * 3;
+  // does the column reach the triplets at all (gamma or a2 of a slot, or a
+  // lambda)? The two-body columns skip the triplet loop
+  bool tri = false;
+  for (int q = 0; q < a.npair; ++q) tri = tri || a.col_sw[8 * q + 6] == col || a.col_sw[8 * q + 7] == col;
+  for (int q = 0; q < a.ntypes * a.npair; ++q) tri = tri || a.col_lam[q] == col;
   if (live) {
     const int ti = a.types[i];
     const int64_t p0 = a.pair_off[i], p1 = a.pair_off[i + 1];
@@ -1421,8 +1449,7 @@ __global__ __launch_bounds__(256) void k_sw_jac_centre(SwJArgs a) {
         for (int q = 0; q < 6; ++q)
           if (a.col_sw[8 * s1 + q] == col) f = q;
         if (f >= 0) {
-          double v2, dv2, tv, tdv;
-          v2_tan(r1, lsw(a.sw, s1), f, v2, dv2, tv, tdv);
+          const double tv = a.pt2[12 * pj + 2 * f], tdv = a.pt2[12 * pj + 2 * f + 1];
           const double s = tdv / r1;
           const double df[3] = {s * d1[0], s * d1[1], s * d1[2]};
           for (int x = 0; x < 3; ++x) F[x] += df[x];
@@ -1432,7 +1459,7 @@ __global__ __launch_bounds__(256) void k_sw_jac_centre(SwJArgs a) {
         }
         const double *q1 = a.pv + 8 * pj;
         const double h1 = q1[2], dh1 = q1[3];
-        if (h1 != 0.0 || dh1 != 0.0) {
+        if (tri && (h1 != 0.0 || dh1 != 0.0)) {
           // tangent of (h1, h1') when the column is gamma or a2 of slot s1
           double t1 = 0.0, td1 = 0.0;
           if (a.col_sw[8 * s1 + 6] == col) {
@@ -1498,25 +1525,40 @@ __global__ __launch_bounds__(64) void k_sw_jac_tiles(SwJArgs a) {
   __shared__ double red[64][kJP];
   const int tile = blockIdx.x, col = blockIdx.y, t = threadIdx.x;
   const int na = a.tile_na[tile];
-  if (t < na) {
-    const int64_t i = int64_t(a.tile_a0[tile]) + t;
-    const int c = a.tile_cfg[tile];
-    const double *o = a.own + (int64_t(col) * a.N + i) * 3;
-    double F[3] = {o[0], o[1], o[2]};
-    const double *pt = a.ptan + int64_t(col) * a.P * 3;
-    for (int64_t p = a.pair_off[i]; p < a.pair_off[i + 1]; ++p) {
-      const int32_t q = a.rev[p];
-      if (q < 0) continue;
-      for (int x = 0; x < 3; ++x) F[x] += pt[3 * int64_t(q) + x];
+  // kLpt lanes per atom (the same for every tile size, so an atom's sum does
+  // not depend on the tiling): lane sub of atom li gathers every kLpt-th
+  // reverse-pair slot, and the lanes' sums are added by shuffles within the
+  // aligned group; 64 / kLpt atoms per round
+  constexpr int kLpt = 4;
+  const int sub = t % kLpt;
+  const double *pt = a.ptan + int64_t(col) * a.P * 3;
+  for (int l0 = 0; l0 < na; l0 += 64 / kLpt) { // uniform trip count: every lane shuffles
+    const int li = l0 + t / kLpt;
+    double F[3] = {0, 0, 0};
+    int64_t i = 0;
+    if (li < na) {
+      i = int64_t(a.tile_a0[tile]) + li;
+      for (int64_t p = a.pair_off[i] + sub; p < a.pair_off[i + 1]; p += kLpt) {
+        const int32_t q = a.rev[p];
+        if (q < 0) continue;
+        for (int k = 0; k < 3; ++k) F[k] += pt[3 * int64_t(q) + k];
+      }
     }
-    const int64_t row = a.row_off[c] + 3 * (i - a.atom_off[c]);
-    for (int x = 0; x < 3; ++x) a.J[(row + x) * a.D + col] = float(F[x]);
-    const double *e = a.etan + (int64_t(col) * a.N + i) * 10;
-    red[t][0] = e[0];
-    // stored as -2 x virial so the row kernel's -w_S/(2V) scaling yields w_S * virial / V
-    const int vo[6] = {0, 4, 8, 1, 2, 5};
-    for (int v = 0; v < 6; ++v) red[t][1 + v] = -2.0 * e[1 + vo[v]];
-    red[t][7] = 0.0;
+    for (int o = kLpt / 2; o > 0; o >>= 1)
+      for (int k = 0; k < 3; ++k) F[k] += __shfl_xor(F[k], o, 64);
+    if (li < na && sub == 0) {
+      const int c = a.tile_cfg[tile];
+      const double *o = a.own + (int64_t(col) * a.N + i) * 3;
+      for (int x = 0; x < 3; ++x) F[x] += o[x];
+      const int64_t row = a.row_off[c] + 3 * (i - a.atom_off[c]);
+      for (int x = 0; x < 3; ++x) a.J[(row + x) * a.D + col] = float(F[x]);
+      const double *e = a.etan + (int64_t(col) * a.N + i) * 10;
+      red[li][0] = e[0];
+      // stored as -2 x virial so the row kernel's -w_S/(2V) scaling yields w_S * virial / V
+      const int vo[6] = {0, 4, 8, 1, 2, 5};
+      for (int v = 0; v < 6; ++v) red[li][1 + v] = -2.0 * e[1 + vo[v]];
+      red[li][7] = 0.0;
+    }
   }
   __syncthreads();
   if (t < kJP) {
@@ -1630,6 +1672,8 @@ void launch_jacobian(Context &c) {
     s.lam = c.lam.p;
     launch_sw_pair_terms(c);
     s.pv = c.sw_pv.p;
+    c.sw_pt2.alloc(12 * size_t(std::max<int64_t>(c.P, 1)));
+    s.pt2 = c.sw_pt2.p;
     std::vector<int32_t> csw(8 * c.npair), clam(c.h_lam.size());
     int o = 0;
     for (size_t k = 0; k < csw.size(); ++k) csw[k] = c.h_sw_fixed[k] ? -1 : o++;
@@ -1656,6 +1700,8 @@ void launch_jacobian(Context &c) {
     s.J = c.J.p;
     s.part = c.jac_part.p;
     if (c.N > 0 && c.D > 0) {
+      hipLaunchKernelGGL(k_sw_pair_tan, dim3(nb(c.N * kSwJLanes, 256)), dim3(256), 0, c.stream, s);
+      FS_HIP(hipGetLastError());
       c.time_main_begin();
       hipLaunchKernelGGL(k_sw_jac_centre, dim3(nb(c.N * kSwJLanes, 256), c.D), dim3(256), 0,
                          c.stream, s);
