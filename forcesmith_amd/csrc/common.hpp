@@ -227,6 +227,9 @@ struct Context {
   DBuf<double> residual, cost, cost_part;
   DBuf<double> pairbuf; // SW triplet forces per pair
   DBuf<double> sw_pv;   // SW per-pair scalars, 8 per pair
+  // SW parameter / pair-list generation (bumped on every upload of the SW
+  // parameters and every neighbour build) and the one sw_pv was computed for
+  uint64_t sw_gen = 1, sw_pv_gen = 0;
   DBuf<double> sw_pt2;  // SW per-pair two-body tangents, 12 per pair
   bool forces_valid = false;
   // Jacobian

@@ -153,6 +153,7 @@ void ensure_nl(Context &c) {
     c.nl_rc = rc;
   }
   if (rebuild) {
+    ++c.sw_gen;
     launch_neighbor_build(c);
     launch_pair_flags(c);
     if (c.kind == FS_SW) launch_sw_rev(c);
@@ -166,6 +167,7 @@ void refresh_params(Context &c) {
     c.sw.alloc(c.h_sw.size());
     FS_HIP(hipMemcpyAsync(c.sw.p, c.h_sw.data(), sizeof(double) * c.h_sw.size(),
                           hipMemcpyHostToDevice, c.stream));
+    ++c.sw_gen;
     c.lam.alloc(std::max<size_t>(c.h_lam.size(), 1));
     FS_HIP(hipMemcpyAsync(c.lam.p, c.h_lam.data(), sizeof(double) * c.h_lam.size(),
                           hipMemcpyHostToDevice, c.stream));

@@ -315,6 +315,9 @@ SwArgs make_sw_args(Context &c) {
 void launch_sw_pair_terms(Context &c) {
   c.sw_pv.alloc(8 * size_t(std::max<int64_t>(c.P, 1)));
   if (c.N == 0) return;
+  // the residual and the Jacobian at one parameter vector share the factors
+  if (c.sw_pv_gen == c.sw_gen) return;
+  c.sw_pv_gen = c.sw_gen;
   const int64_t nthr = c.N * kSwLanes;
   hipLaunchKernelGGL(k_sw_pair_terms, dim3(unsigned((nthr + 255) / 256)), dim3(256), 0, c.stream,
                      c.N, c.pair_off.p, c.types.p, c.pair_d.p, c.pair_bits.p, c.sw.p, c.ntypes,
