@@ -223,6 +223,9 @@ struct Context {
   DBuf<double> rho, Fp, Fpp, e_atom, lim_atom, clamp_over;
   DBuf<float> Fp32;
   DBuf<double> mu, lamb; // ADP per-atom dipole (3) and quadrupole (6)
+  // their FP32 copies for the mixed force pass: mu32[N] (x, y, z, 0),
+  // lamb32[2N] (xx, yy, zz, xy | xz, yz, 0, 0)
+  DBuf<float4> mu32, lamb32;
   DBuf<double> forces, tile_part, cfg_E, cfg_S, cfg_L;
   DBuf<double> residual, cost, cost_part;
   DBuf<double> pairbuf; // SW triplet forces per pair

@@ -191,6 +191,8 @@ void alloc_eval(Context &c) {
   c.lim_atom.alloc(N);
   c.mu.alloc(3 * N);
   c.lamb.alloc(6 * N);
+  c.mu32.alloc(N);
+  c.lamb32.alloc(2 * N);
   c.forces.alloc(3 * N);
   c.cfg_E.alloc(std::max(c.C, 1));
   c.cfg_S.alloc(9 * size_t(std::max(c.C, 1)));

@@ -171,6 +171,7 @@ struct EvalArgs {
   double *rho, *over, *Fp;
   float *Fp32;
   double *e_atom, *lim, *mu, *lamb;
+  float4 *mu32, *lamb32; // FP32 copies of mu, lamb (mixed ADP force pass)
   double *forces, *residual, *tile_part;
 };
 
@@ -281,9 +282,7 @@ __device__ inline void density_pair(const Tabs &tb, const EvalArgs &a, int ti, u
         tb.eval64(a.npair + 2 * a.ntypes + o, double(r), vv, dd);
         u = R(vv);
       } else {
-        float vv, dd;
-        tb.eval32(a.npair + 2 * a.ntypes + o, double(r), vv, dd);
-        u = R(vv);
+        u = R(tb.value64(a.npair + 2 * a.ntypes + o, double(r)));
       }
       acc.m0 += u * d0;
       acc.m1 += u * d1;
@@ -296,9 +295,7 @@ __device__ inline void density_pair(const Tabs &tb, const EvalArgs &a, int ti, u
         tb.eval64(2 * a.npair + 2 * a.ntypes + o, double(r), vv, dd);
         w = R(vv);
       } else {
-        float vv, dd;
-        tb.eval32(2 * a.npair + 2 * a.ntypes + o, double(r), vv, dd);
-        w = R(vv);
+        w = R(tb.value64(2 * a.npair + 2 * a.ntypes + o, double(r)));
       }
       acc.l[0] += w * (d0 * d0);
       acc.l[1] += w * (d1 * d1);
@@ -329,8 +326,10 @@ __global__ __launch_bounds__(256) void k_density(EvalArgs a) {
     DensAcc<double> acc;
     for (int64_t p = p0 + lane; p < p1; p += LPA) {
       if constexpr (KIND == FS_ADP) {
+        // FP64 mode: the cached-site form; mixed: the FP64 interval power
+        // form of the same interval (values only), as for the EAM densities
         const Pair<double> q = load_pair64<true>(a, p);
-        density_pair<KIND, true, double>(tb, a, ti, q.b, q.d0, q.d1, q.d2, q.r, acc);
+        density_pair<KIND, F64, double>(tb, a, ti, q.b, q.d0, q.d1, q.d2, q.r, acc);
       } else {
         const uint32_t b = a.bits[p];
         if (b & kFlagGj) {
@@ -381,6 +380,9 @@ __global__ __launch_bounds__(256) void k_density(EvalArgs a) {
         a.mu[3 * i + 1] = m1;
         a.mu[3 * i + 2] = m2;
         for (int k = 0; k < 6; ++k) a.lamb[6 * i + k] = double(acc.l[k]);
+        a.mu32[i] = float4{float(m0), float(m1), float(m2), 0.0f};
+        a.lamb32[2 * i] = float4{float(L0), float(L1), float(L2), float(L3)};
+        a.lamb32[2 * i + 1] = float4{float(L4), float(L5), 0.0f, 0.0f};
       }
       a.rho[i] = rh;
       a.over[i] = over;
@@ -469,8 +471,22 @@ __device__ inline void force_pair(const Tabs &tb, const EvalArgs &a, int ti, int
   R f0 = fs * d0, f1 = fs * d1, f2 = fs * d2;
   if (KIND == FS_ADP) {
     R muj[3], lj[6];
-    for (int k = 0; k < 3; ++k) muj[k] = R(a.mu[3 * jg + k]);
-    for (int k = 0; k < 6; ++k) lj[k] = R(a.lamb[6 * jg + k]);
+    if constexpr (std::is_same<R, float>::value) {
+      // the FP32 copies: the same roundings as R(double), in three loads
+      const float4 m = a.mu32[jg], l0 = a.lamb32[2 * jg], l1 = a.lamb32[2 * jg + 1];
+      muj[0] = m.x;
+      muj[1] = m.y;
+      muj[2] = m.z;
+      lj[0] = l0.x;
+      lj[1] = l0.y;
+      lj[2] = l0.z;
+      lj[3] = l0.w;
+      lj[4] = l1.x;
+      lj[5] = l1.y;
+    } else {
+      for (int k = 0; k < 3; ++k) muj[k] = R(a.mu[3 * jg + k]);
+      for (int k = 0; k < 6; ++k) lj[k] = R(a.lamb[6 * jg + k]);
+    }
     if (b & kFlagDip) {
       R u, du;
       ev(a.npair + 2 * a.ntypes + o, u, du);
@@ -956,6 +972,8 @@ EvalArgs make_args(Context &c) {
   a.lim = c.lim_atom.p;
   a.mu = c.mu.p;
   a.lamb = c.lamb.p;
+  a.mu32 = c.mu32.p;
+  a.lamb32 = c.lamb32.p;
   a.forces = c.forces.p;
   a.residual = c.residual.p;
   a.tile_part = c.tile_part.p;
