@@ -1266,15 +1266,15 @@ __global__ void k_jac_rows(JacArgs a) {
   const int64_t e_row = a.row_off[c] + 3 * na;
   const bool ws = a.sw > 0.0;
   const double iv = 1.0 / a.vol[c];
-  // one thread per (partial row q, column): the tile partials of a
-  // configuration are contiguous in (tile, q, column), so a wave reads
-  // consecutive doubles; four tiles' loads are in flight before their sums,
-  // which are added in tile order
+  // block (c, q) sums partial row q of configuration c, one thread per
+  // column: a wave reads consecutive doubles of each tile, four tiles' loads
+  // are in flight before their sums, which are added in tile order
   const int t0 = a.cfg_tile0[c], t1 = a.cfg_tile0[c + 1];
   const int nqc = kJP * a.D;
-  for (int e = threadIdx.x; e < nqc; e += blockDim.x) {
-    const int q = e / a.D, col = e - q * a.D;
-    if (!ws && q >= 1 && q <= 6) continue;
+  const int q = blockIdx.y;
+  if (!ws && q >= 1 && q <= 6) return;
+  for (int col = threadIdx.x; col < a.D; col += blockDim.x) {
+    const int e = q * a.D + col;
     const double *p = a.part + int64_t(t0) * nqc + e;
     double acc = 0.0;
     int t = t0;
@@ -1928,7 +1928,7 @@ void launch_jacobian(Context &c) {
       c.time_main_end();
     }
   }
-  hipLaunchKernelGGL(k_jac_rows, dim3(c.C), dim3(256), 0, c.stream, a);
+  hipLaunchKernelGGL(k_jac_rows, dim3(c.C, kJP), dim3(256), 0, c.stream, a);
   FS_HIP(hipGetLastError());
   launch_smooth_jacobian(c);
 }
