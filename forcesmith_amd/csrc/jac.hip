@@ -1492,11 +1492,20 @@ __global__ __launch_bounds__(256) void k_sw_jac_centre(SwJArgs a) {
             const double cp = cs + 1.0 / 3.0;
             const double w = lam * cp * cp, dw = 2.0 * lam * cp;
             const double tg = lam_col ? cp * cp : 0.0, tdg = lam_col ? 2.0 * cp : 0.0;
-            double fa[3], fb[3], fc[3];
-            tb_j(d1, d2, i1, i2, cs, t1, td1, h2, w, dw, fa);
-            tb_j(d1, d2, i1, i2, cs, h1, dh1, t2, w, dw, fb);
-            tb_j(d1, d2, i1, i2, cs, h1, dh1, h2, tg, tdg, fc);
-            for (int x = 0; x < 3; ++x) T[x] += fa[x] + fb[x] + fc[x];
+            // only the factor groups this column moves (the others add zeros)
+            double f[3];
+            if (t1 != 0.0 || td1 != 0.0) {
+              tb_j(d1, d2, i1, i2, cs, t1, td1, h2, w, dw, f);
+              for (int x = 0; x < 3; ++x) T[x] += f[x];
+            }
+            if (t2 != 0.0) {
+              tb_j(d1, d2, i1, i2, cs, h1, dh1, t2, w, dw, f);
+              for (int x = 0; x < 3; ++x) T[x] += f[x];
+            }
+            if (lam_col) {
+              tb_j(d1, d2, i1, i2, cs, h1, dh1, h2, tg, tdg, f);
+              for (int x = 0; x < 3; ++x) T[x] += f[x];
+            }
             // each triplet is met from both of its bonds
             E += 0.5 * (t1 * h2 * w + h1 * t2 * w + h1 * h2 * tg);
           }
