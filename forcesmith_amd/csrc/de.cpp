@@ -594,10 +594,15 @@ int de_run_device(Context &c, const fs_de_opts *o, const double *lo, const doubl
         stop = reached(done);
       }
     } catch (...) {
+      (void)hipStreamSynchronize(st);
       if (exec) (void)hipGraphExecDestroy(exec);
       if (graph) (void)hipGraphDestroy(graph);
       throw;
     }
+    // the replays are only queued (without a target cost nothing above waits
+    // for them): the executable graph and its kernel arguments must outlive
+    // every queued launch, so wait before destroying it
+    FS_HIP(hipStreamSynchronize(st));
     if (exec) FS_HIP(hipGraphExecDestroy(exec));
     if (graph) FS_HIP(hipGraphDestroy(graph));
   } else {

@@ -190,6 +190,8 @@ void mlp_init(Mlp &m, int D, int Hn) {
   m.step = 0;
 }
 
+// y_out: caller-owned outputs, or null for the workspace's own m.y (taken
+// after the workspace is sized for B)
 void mlp_forward(Mlp &m, hipStream_t s, int B, const float *X, float *y_out) {
   m.ensure(B);
   gemm<false, true, EPI_BIAS_GELU>(s, B, m.Hn, m.D, X, m.D, m.W1(), m.D, m.Z1.p, m.Hn, m.b1(),
@@ -204,7 +206,10 @@ void mlp_forward(Mlp &m, hipStream_t s, int B, const float *X, float *y_out) {
 // gradient of mean 0.5 (y - t)^2 into m.G; returns nothing (loss in m.loss)
 void mlp_backward(Mlp &m, hipStream_t s, int B, const float *X, const float *t) {
   const int Hn = m.Hn, D = m.D;
-  mlp_forward(m, s, B, X, m.y.p);
+  // outputs into m.y: passed as null, so that mlp_forward takes the pointer
+  // after m.ensure(B), which reallocates the workspace when B exceeds every
+  // earlier batch (a pointer read here would be the freed buffer)
+  mlp_forward(m, s, B, X, nullptr);
   hipLaunchKernelGGL(k_loss, dim3(1), dim3(256), 0, s, B, m.y.p, t, m.loss.p);
   // dZ2 in tmp
   hipLaunchKernelGGL(k_head_back, dim3(cdiv(int64_t(B) * Hn, 256)), dim3(256), 0, s, B, Hn, m.y.p, t,
