@@ -122,11 +122,17 @@ def de_measure(a):
     log(f"generating the ternary EAM workload ({a.configs} configs, seed 2)")
     w = datasets.config2(n_configs=a.configs)
 
+    pairs = [0]
+
     def calculator(precision):
         g = fa.GpuCalculator(w.dataset, w.model, precision=precision)
-        g.prepare()
+        pairs[0] = g.prepare()
         g.set_launch(a.de_lanes)
         return g
+
+    def pair_member_rate(seconds, gens):
+        # one generation evaluates NP members over every neighbour pair
+        return a.de_np * pairs[0] * gens / seconds if seconds > 0 else None
 
     g = calculator(a.de_plain_precision if a.de in ("plain", "both") else a.de_surrogate_precision)
     x0 = g.gather_params()
@@ -148,7 +154,8 @@ def de_measure(a):
         _, c_plain, hp = g.de(x0, lo, hi, a.de_np, a.de_generations, **common)
         t_plain = time.perf_counter() - t0
         out["plain"] = {"seconds": t_plain, "final_cost": c_plain, "generations": len(hp),
-                        "ms_per_generation": 1e3 * t_plain / max(len(hp), 1)}
+                        "ms_per_generation": 1e3 * t_plain / max(len(hp), 1),
+                        "pair_member_evals_per_s": pair_member_rate(t_plain, len(hp))}
         log(f"plain: {len(hp)} generations {t_plain:.1f} s final {c_plain:.9e}")
     if a.de in ("surrogate", "both"):
         if c_plain <= 0:
@@ -163,7 +170,8 @@ def de_measure(a):
         t_s = time.perf_counter() - t0
         out["surrogate"] = {"seconds": t_s, "final_cost": c_s, "generations": len(hs),
                             "reached": bool(c_s <= target),
-                            "ms_per_generation": 1e3 * t_s / max(len(hs), 1)}
+                            "ms_per_generation": 1e3 * t_s / max(len(hs), 1),
+                            "pair_member_evals_per_s": pair_member_rate(t_s, len(hs))}
         log(f"surrogate: {len(hs)} generations {t_s:.1f} s best {c_s:.9e} reached {c_s <= target}")
         if t_plain > 0:
             out["value"] = t_s / t_plain if c_s <= target else None
