@@ -96,6 +96,27 @@ def test_jacobian_tile_sizes_agree(wl, name):
         g.set_jacobian_tile(12)
 
 
+def test_sw_factors_follow_the_parameters(wl):
+    # the SW per-pair factors are shared by the residual and the Jacobian of
+    # one parameter vector; a residual at x1 followed by a Jacobian at x2 (and
+    # back) must give the Jacobian and residual of x2 and x1
+    w = wl["sw"]
+    print("sw factor cache", flush=True)
+    o = oracle.Oracle(w.dataset, w.model)
+    o.prepare()
+    g = fa.GpuCalculator(w.dataset, w.model)
+    g.prepare()
+    x1 = g.gather_params()
+    x2 = x1 * (1.0 + 0.02 * np.cos(np.arange(x1.size)))
+    g.residual(x1)
+    J2 = g.jacobian(x2).astype(np.float64)
+    assert metric(J2, o.jacobian(x2)) <= 2e-4
+    F1 = g.residual(x1)
+    assert np.max(np.abs(F1 - o.residual(x1))) <= 1e-5
+    J1 = g.jacobian().astype(np.float64)  # same parameters: the kept factors
+    assert metric(J1, o.jacobian(x1)) <= 2e-4
+
+
 def test_jacobian_rows_by_kind(wl):
     # force, energy, stress and limit rows are each exercised
     w = wl["eam3s"]
