@@ -413,7 +413,7 @@ __global__ __launch_bounds__(256) void k_jac_dens(JacArgs a, int nknots) {
 //            the density-chain and per-atom terms).
 enum { JP_PAIR = 1, JP_DE = 2 };
 constexpr int kHBatch = 8; // Hs rows gathered per batch by the density-chain term
-constexpr int kWavesPair = 5, kWavesDE = 4; // waves per SIMD asked of the two passes
+constexpr int kWavesPair = 5, kWavesDE = 3; // waves per SIMD asked of the two passes
 
 struct JStagePair {
   int32_t seg[kSegPad]; // partner-type segment starts of the chunk
@@ -798,10 +798,11 @@ __device__ __forceinline__ void jac_main_body(const JacArgs &a, int nknots, cons
 }
 
 // The pass kernels. The window and gather loops are latency-bound, so each
-// pass asks for W waves per SIMD (the register budget follows; a few values
-// may live in scratch): 4 for the density/embedding pass, which its LDS
-// staging also allows (4 blocks of 4 waves per CU), and 5 for the lighter
-// pair pass.
+// pass asks for W waves per SIMD (the register budget follows): 3 for the
+// density/embedding pass (measured 1 % faster than 4, whose tighter register
+// budget puts a few values in scratch; its LDS staging would allow 4 blocks
+// of 4 waves per CU), and 5 for the lighter pair pass (4 and 6 measured the
+// same).
 template <int NS, int W>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k_jac_pair(
     JacArgs a, int nknots, const SlotDesc *slots) {
