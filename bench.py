@@ -257,7 +257,8 @@ def main():
         g.set_jacobian_tile(a.jacobian_tile)
     lpa = g.autotune()
     shape = g.launch_shape()
-    log(f"autotuned lanes per atom: {lpa}, Jacobian tile: {shape['jacobian_tile_atoms']} atoms")
+    log(f"autotuned lanes per atom: {lpa}, blocks per CU: {shape['blocks_per_cu']}, "
+        f"Jacobian tile: {shape['jacobian_tile_atoms']} atoms")
     x = g.gather_params()
     C, D, N, R = ds.n_configs, g.param_count, ds.n_atoms, g.residual_count
     lib, h = g._lib, g._h
@@ -342,6 +343,7 @@ def main():
         "config": {"workload": desc.format(C=C, D=D),
                    "configs": C, "atoms": N, "pairs": P, "params": D, "rows": R,
                    "precision": a.precision, "lanes_per_atom": lpa,
+                   "blocks_per_cu": shape["blocks_per_cu"],
                    "jacobian_tile_atoms": shape["jacobian_tile_atoms"]},
         "roofline": roofline,
         "kernels_ms": kernels,

@@ -391,9 +391,9 @@ class GpuCalculator:
 
     def autotune(self) -> int:
         """Times the launch shapes on the resident dataset and keeps the
-        fastest: lanes per atom over the residual, then atoms per Jacobian
-        tile over the whole Jacobian. Returns the lanes per atom
-        (launch_shape() has both)."""
+        fastest: lanes per atom and blocks per CU over the residual, then
+        atoms per Jacobian tile over the whole Jacobian. Returns the lanes per
+        atom (launch_shape() has all three)."""
         v = ct.c_int32()
         self._check(self._lib.fs_autotune(self._h, ct.byref(v)))
         return int(v.value)

@@ -344,6 +344,23 @@ int autotune_impl(Context &c) {
   c.lanes_per_atom = best;
   rebuild_tiles(c);
   c.forces_valid = false;
+  // persistent-grid size of the residual kernels (blocks per CU; the grid
+  // strides over the tiles, so the result does not depend on it)
+  {
+    int best_b = c.blocks_per_cu;
+    double best_b_ms = std::numeric_limits<double>::infinity();
+    for (int bpc : {4, 8, 16, 32}) {
+      c.blocks_per_cu = bpc;
+      c.forces_valid = false;
+      const double ms = time_step_impl(c, 0, 5);
+      if (ms < best_b_ms) {
+        best_b_ms = ms;
+        best_b = bpc;
+      }
+    }
+    c.blocks_per_cu = best_b;
+    c.forces_valid = false;
+  }
   // Jacobian tile: powers of two from 4 to 64 atoms, timed over the whole
   // Jacobian; candidates that would cut the dataset into more than 64 tiles
   // per CU are skipped (their per-tile partials and row reductions grow with

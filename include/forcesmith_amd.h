@@ -232,7 +232,8 @@ int fs_device_buffer(fs_ctx *ctx, int32_t which, void **ptr, int64_t *bytes);
 int fs_synchronize(fs_ctx *ctx);
 /* Launch shape: lanes per atom (4..64), blocks per CU for persistent grids.
  * fs_autotune times the candidates on the resident dataset and keeps the
- * fastest (lanes per atom over the residual, then the Jacobian tile size). */
+ * fastest (lanes per atom and blocks per CU over the residual, then the
+ * Jacobian tile size). */
 int fs_set_launch(fs_ctx *ctx, int32_t lanes_per_atom, int32_t blocks_per_cu);
 /* Atoms per Jacobian tile: 0 (the default) picks it from the atom count, else
  * 4, 8, 16, 32 or 64. Unless a size is fixed here (non-zero), fs_autotune
