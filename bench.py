@@ -258,7 +258,8 @@ def main():
     lpa = g.autotune()
     shape = g.launch_shape()
     log(f"autotuned lanes per atom: {lpa}, blocks per CU: {shape['blocks_per_cu']}, "
-        f"Jacobian tile: {shape['jacobian_tile_atoms']} atoms")
+        f"Jacobian tile: {shape['jacobian_tile_atoms']} atoms, Jacobian blocks per CU: "
+        f"{shape['jacobian_blocks_per_cu']}")
     x = g.gather_params()
     C, D, N, R = ds.n_configs, g.param_count, ds.n_atoms, g.residual_count
     lib, h = g._lib, g._h
@@ -344,7 +345,8 @@ def main():
                    "configs": C, "atoms": N, "pairs": P, "params": D, "rows": R,
                    "precision": a.precision, "lanes_per_atom": lpa,
                    "blocks_per_cu": shape["blocks_per_cu"],
-                   "jacobian_tile_atoms": shape["jacobian_tile_atoms"]},
+                   "jacobian_tile_atoms": shape["jacobian_tile_atoms"],
+                   "jacobian_blocks_per_cu": shape["jacobian_blocks_per_cu"]},
         "roofline": roofline,
         "kernels_ms": kernels,
         "step_bytes_alg": b_alg,

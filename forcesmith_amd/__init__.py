@@ -121,7 +121,7 @@ def load_library(path: str = LIB_PATH):
         "fs_synchronize": (ct.c_int, [vp]),
         "fs_set_launch": (ct.c_int, [vp, ct.c_int32, ct.c_int32]),
         "fs_set_jacobian_tile": (ct.c_int, [vp, ct.c_int32]),
-        "fs_get_launch": (ct.c_int, [vp, _i32p, _i32p, _i32p]),
+        "fs_get_launch": (ct.c_int, [vp, _i32p, _i32p, _i32p, _i32p]),
         "fs_autotune": (ct.c_int, [vp, _i32p]),
         "fs_time_step": (ct.c_int, [vp, ct.c_int32, ct.c_int32, _dp]),
         "fs_mlp_forward": (ct.c_int, [vp, ct.c_int32, ct.c_int32, ct.c_int32, _fp, _fp, _fp]),
@@ -384,16 +384,16 @@ class GpuCalculator:
         self._check(self._lib.fs_set_jacobian_tile(self._h, int(atoms)))
 
     def launch_shape(self) -> dict:
-        lpa, bpc, jt = ct.c_int32(), ct.c_int32(), ct.c_int32()
-        self._check(self._lib.fs_get_launch(self._h, ct.byref(lpa), ct.byref(bpc), ct.byref(jt)))
+        lpa, bpc, jt, jb = ct.c_int32(), ct.c_int32(), ct.c_int32(), ct.c_int32()
+        self._check(self._lib.fs_get_launch(self._h, ct.byref(lpa), ct.byref(bpc), ct.byref(jt), ct.byref(jb)))
         return {"lanes_per_atom": int(lpa.value), "blocks_per_cu": int(bpc.value),
-                "jacobian_tile_atoms": int(jt.value)}
+                "jacobian_tile_atoms": int(jt.value), "jacobian_blocks_per_cu": int(jb.value)}
 
     def autotune(self) -> int:
         """Times the launch shapes on the resident dataset and keeps the
         fastest: lanes per atom and blocks per CU over the residual, then
         atoms per Jacobian tile over the whole Jacobian. Returns the lanes per
-        atom (launch_shape() has all three)."""
+        atom (launch_shape() has the whole shape)."""
         v = ct.c_int32()
         self._check(self._lib.fs_autotune(self._h, ct.byref(v)))
         return int(v.value)

@@ -151,6 +151,9 @@ struct Context {
   // two in [4, 64] (fs_set_jacobian_tile, fs_autotune)
   int jac_tile = 0;
   bool jac_tile_fixed = false; // set by hand: fs_autotune keeps it
+  // persistent-grid size of the Jacobian passes (blocks per CU; each tile is
+  // one block's work whatever the grid, so the result does not depend on it)
+  int jac_blocks_per_cu = 8;
 
   // ---- dataset (host mirror + device) ------------------------------------
   int C = 0;

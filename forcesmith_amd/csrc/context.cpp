@@ -875,11 +875,12 @@ int fs_set_jacobian_tile(fs_ctx *ctx, int32_t atoms) {
 }
 
 int fs_get_launch(fs_ctx *ctx, int32_t *lanes_per_atom, int32_t *blocks_per_cu,
-                  int32_t *jacobian_tile) {
+                  int32_t *jacobian_tile, int32_t *jacobian_blocks_per_cu) {
   return guarded(ctx, [&](Context &c) {
     if (lanes_per_atom) *lanes_per_atom = c.lanes_per_atom;
     if (blocks_per_cu) *blocks_per_cu = c.blocks_per_cu;
     if (jacobian_tile) *jacobian_tile = c.njt > 0 ? c.jt_atoms : c.jac_tile;
+    if (jacobian_blocks_per_cu) *jacobian_blocks_per_cu = c.jac_blocks_per_cu;
   });
 }
 

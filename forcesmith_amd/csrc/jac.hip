@@ -1874,7 +1874,7 @@ void launch_jacobian(Context &c) {
     if (c.N > 0) {
       // (the pair pass depends on neither kernel below; run beside them on a
       // second stream it measured no faster and stretched both)
-      const unsigned g2 = unsigned(std::max(1, std::min(nt, c.cu_count * 8)));
+      const unsigned g2 = unsigned(std::max(1, std::min(nt, c.cu_count * c.jac_blocks_per_cu)));
       const SlotDesc *d_de = d_slots + size_t(NSP) * 64;
       c.time_main_begin(1);
       // wider maps (more than 128 columns of a pass) keep the compiler's

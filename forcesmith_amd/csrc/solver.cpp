@@ -383,6 +383,20 @@ int autotune_impl(Context &c) {
     c.jac_tile = best_jt > 0 ? best_jt : keep;
     c.njt = 0;
   }
+  // persistent-grid size of the Jacobian passes
+  if (c.N > 0 && c.D > 0) {
+    int best_b = c.jac_blocks_per_cu;
+    double best_b_ms = std::numeric_limits<double>::infinity();
+    for (int bpc : {4, 8, 16, 32}) {
+      c.jac_blocks_per_cu = bpc;
+      const double ms = time_step_impl(c, 1, 3);
+      if (ms < best_b_ms) {
+        best_b_ms = ms;
+        best_b = bpc;
+      }
+    }
+    c.jac_blocks_per_cu = best_b;
+  }
   return FS_OK;
 }
 

@@ -233,17 +233,17 @@ int fs_synchronize(fs_ctx *ctx);
 /* Launch shape: lanes per atom (4..64), blocks per CU for persistent grids.
  * fs_autotune times the candidates on the resident dataset and keeps the
  * fastest (lanes per atom and blocks per CU over the residual, then the
- * Jacobian tile size). */
+ * Jacobian tile size and the Jacobian passes' blocks per CU). */
 int fs_set_launch(fs_ctx *ctx, int32_t lanes_per_atom, int32_t blocks_per_cu);
 /* Atoms per Jacobian tile: 0 (the default) picks it from the atom count, else
  * 4, 8, 16, 32 or 64. Unless a size is fixed here (non-zero), fs_autotune
  * also times the tile sizes over the whole Jacobian and keeps the fastest.
  * fs_get_launch reports the current shape
  * (jacobian_tile: the tile size of the last Jacobian, or the requested one
- * before the first). No reference counterpart (launch tuning). */
+ * before the first; jacobian_blocks_per_cu: the Jacobian passes' grid). No reference counterpart (launch tuning). */
 int fs_set_jacobian_tile(fs_ctx *ctx, int32_t atoms);
 int fs_get_launch(fs_ctx *ctx, int32_t *lanes_per_atom, int32_t *blocks_per_cu,
-                  int32_t *jacobian_tile);
+                  int32_t *jacobian_tile, int32_t *jacobian_blocks_per_cu);
 int fs_autotune(fs_ctx *ctx, int32_t *lanes_per_atom);
 /* Average time (ms) of one stage over reps launches, measured with HIP events
  * on the context stream. what: 0 residual, 1 Jacobian (all its kernels),
