@@ -374,7 +374,7 @@ int autotune_impl(Context &c) {
       if (jt < 64 && c.N / jt > int64_t(64) * c.cu_count) continue;
       c.jac_tile = jt;
       c.njt = 0;
-      const double ms = time_step_impl(c, 1, 3);
+      const double ms = time_step_impl(c, 1, 5);
       if (ms < best_jt_ms) {
         best_jt_ms = ms;
         best_jt = jt;
@@ -389,7 +389,7 @@ int autotune_impl(Context &c) {
     double best_b_ms = std::numeric_limits<double>::infinity();
     for (int bpc : {4, 8, 16, 32}) {
       c.jac_blocks_per_cu = bpc;
-      const double ms = time_step_impl(c, 1, 3);
+      const double ms = time_step_impl(c, 1, 5);
       if (ms < best_b_ms) {
         best_b_ms = ms;
         best_b = bpc;
