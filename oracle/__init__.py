@@ -31,6 +31,7 @@ _u8p = ct.POINTER(ct.c_uint8)
 
 # or_lmder callbacks: f(user, x, out) -> int
 FCN = ct.CFUNCTYPE(ct.c_int, ct.c_void_p, _dp, _dp)
+POPCOST = ct.CFUNCTYPE(None, ct.c_void_p, _dp, ct.c_int, _dp)
 
 
 class LmOpts(ct.Structure):
@@ -99,6 +100,12 @@ def _load():
         "or_rng_unit": (ct.c_double, [ct.c_uint64, ct.c_uint64, ct.c_uint64]),
         "or_de": (ct.c_double, [ct.c_void_p, _dp, _dp, _dp, ct.c_int, ct.c_int, ct.c_double, ct.c_double,
                                 ct.c_uint64, _dp]),
+        "or_de_init": (None, [_dp, _dp, _dp, ct.c_int, ct.c_int, ct.c_uint64, _dp]),
+        "or_de_trial": (None, [_dp, ct.c_int, ct.c_int, _dp, _dp, ct.c_double, ct.c_double, ct.c_uint64,
+                               ct.c_int, ct.c_int, _dp, _i32p]),
+        "or_de_select": (ct.c_int, [ct.c_int, ct.c_int, _dp, _dp, _dp, _dp, _u8p]),
+        "or_de_fn": (ct.c_double, [POPCOST, ct.c_void_p, ct.c_int, _dp, _dp, _dp, ct.c_int, ct.c_int,
+                                   ct.c_double, ct.c_double, ct.c_uint64, _dp]),
         "or_mlp_forward": (None, [ct.c_int, ct.c_int, ct.c_int, _dp, _dp, _dp, _dp, _dp, _dp, ct.c_double, _dp]),
         "or_mlp_loss_grad": (ct.c_double, [ct.c_int, ct.c_int, ct.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                                            ct.c_double, _dp, _dp, _dp, _dp, _dp, _dp]),
@@ -402,6 +409,56 @@ class Oracle:
         x = _f64(x)
         F = np.zeros(self.residual_count)
         return float(self._lib.or_time_residual(self._h, _p(x, _dp), int(reps), _p(F, _dp)))
+
+
+def de_init(x, lo, hi, np_, seed):
+    """or_de_init: the initial population (np_ x D)."""
+    lib = _load()
+    x = _f64(x)
+    pop = np.zeros((int(np_), x.size))
+    lib.or_de_init(_p(x, _dp), _p(_f64(lo), _dp), _p(_f64(hi), _dp), int(np_), x.size, int(seed), _p(pop, _dp))
+    return pop
+
+
+def de_trial(pop, lo, hi, F, CR, seed, g, m):
+    """or_de_trial: (trial of member m in generation g, (r1, r2, r3, jrand))."""
+    lib = _load()
+    pop = _f64(pop)
+    np_, D = pop.shape
+    out = np.zeros(D)
+    picks = np.zeros(4, dtype=np.int32)
+    lib.or_de_trial(_p(pop, _dp), np_, D, _p(_f64(lo), _dp), _p(_f64(hi), _dp), float(F), float(CR),
+                    int(seed), int(g), int(m), _p(out, _dp), _p(picks, _i32p))
+    return out, tuple(int(v) for v in picks)
+
+
+def de_select(pop, cost, trial, tcost):
+    """or_de_select: (new population, new costs, kept flags, best index)."""
+    lib = _load()
+    pop = _f64(pop).copy()
+    cost = _f64(cost).copy()
+    np_, D = pop.shape
+    kept = np.zeros(np_, dtype=np.uint8)
+    bm = lib.or_de_select(np_, D, _p(pop, _dp), _p(cost, _dp), _p(_f64(trial), _dp), _p(_f64(tcost), _dp),
+                          _p(kept, _u8p))
+    return pop, cost, kept.astype(bool), int(bm)
+
+
+def de_fn(fun, x, lo, hi, np_, generations, F=0.65, CR=0.5, seed=42):
+    """or_de_fn on a Python cost fun(X[np, D]) -> cost[np]: (x, best, hist)."""
+    lib = _load()
+    x = _f64(x).copy()
+    D = x.size
+
+    def cb(_u, Xp, n, out):
+        X = np.ctypeslib.as_array(Xp, shape=(n * D,)).reshape(n, D).copy()
+        np.ctypeslib.as_array(out, shape=(n,))[:] = fun(X)
+
+    fcb = POPCOST(cb)
+    hist = np.zeros(max(int(generations), 1))
+    best = lib.or_de_fn(fcb, None, D, _p(x, _dp), _p(_f64(lo), _dp), _p(_f64(hi), _dp), int(np_),
+                        int(generations), float(F), float(CR), int(seed), _p(hist, _dp))
+    return x, float(best), hist[:int(generations)]
 
 
 def lmder(fun, jac, x0, m, **opts):

@@ -2545,13 +2545,17 @@ double or_rng_unit(uint64_t seed, uint64_t stream, uint64_t index) {
   return (double)(or_rng_u64(seed, stream, index) >> 11) * 0x1.0p-53;
 }
 
-/* DE/rand/1/bin, synchronous generations (the BoostDESolver semantics of
- * src/optimization/solver.cpp:113-155 with a documented counter RNG):
- *   member 0 = clamp(x), others uniform in [lo, hi] (stream 0, index m*D+k);
- *   generation g: picks r1,r2,r3 distinct and != m by rejection
- *   (stream 2g, index m*1024 + t), jrand and crossover draws
- *   (stream 2g+1, index m*(D+1) + k, jrand at k = D);
- *   mutant clamped to the box; trial replaces the member if cost <= cost_m. */
+/* DE/rand/1/bin (Storn & Price, J. Global Optim. 11 (1997) 341-359) with
+ * synchronous generations: the BoostDESolver semantics of
+ * src/optimization/solver.cpp:113-155 with a documented counter RNG.
+ *   init:   member 0 = clamp(x) (:136-140), member m > 0 component k =
+ *           lo + u (hi - lo), u = unit(seed, 0, m*D + k)
+ *   trial:  r1, r2, r3 distinct and != m by rejection (stream 2g, index
+ *           m*1024 + t); jrand and crossover draws (stream 2g+1, index
+ *           m*(D+1) + k, jrand at k = D); component k takes the mutant
+ *           clamp(x_r1 + F (x_r2 - x_r3)) when its draw is < CR or k = jrand,
+ *           else the parent's value
+ *   select: the trial replaces the member when its cost is <= the member's */
 static int de_pick(uint64_t seed, uint64_t stream, int m, int np, int *t,
                    int e0, int e1, int e2) {
   for (;;) {
@@ -2561,61 +2565,93 @@ static int de_pick(uint64_t seed, uint64_t stream, int m, int np, int *t,
   }
 }
 
-double or_de(or_ctx *c, double *x, const double *lo, const double *hi, int np,
-             int generations, double Fm, double CR, uint64_t seed,
-             double *best_hist) {
-  const int D = or_param_count(c);
+static double clampd(double v, double lo, double hi) {
+  if (v < lo) v = lo;
+  if (v > hi) v = hi;
+  return v;
+}
+
+void or_de_init(const double *x, const double *lo, const double *hi, int np, int D,
+                uint64_t seed, double *pop) {
+  for (int k = 0; k < D; ++k) pop[k] = clampd(x[k], lo[k], hi[k]);
+  for (int m = 1; m < np; ++m)
+    for (int k = 0; k < D; ++k)
+      pop[(size_t)m * D + k] = lo[k] + or_rng_unit(seed, 0, (uint64_t)m * D + k) * (hi[k] - lo[k]);
+}
+
+void or_de_trial(const double *pop, int np, int D, const double *lo, const double *hi,
+                 double Fm, double CR, uint64_t seed, int g, int m, double *out,
+                 int *picks) {
+  int t = 0;
+  const int r1 = de_pick(seed, 2u * (uint64_t)g, m, np, &t, m, -1, -1);
+  const int r2 = de_pick(seed, 2u * (uint64_t)g, m, np, &t, m, r1, -1);
+  const int r3 = de_pick(seed, 2u * (uint64_t)g, m, np, &t, m, r1, r2);
+  const uint64_t cs = 2u * (uint64_t)g + 1u;
+  const int jr = (int)(or_rng_unit(seed, cs, (uint64_t)m * (D + 1) + D) * D);
+  for (int k = 0; k < D; ++k) {
+    if (or_rng_unit(seed, cs, (uint64_t)m * (D + 1) + k) < CR || k == jr)
+      out[k] = clampd(pop[(size_t)r1 * D + k] + Fm * (pop[(size_t)r2 * D + k] - pop[(size_t)r3 * D + k]),
+                      lo[k], hi[k]);
+    else
+      out[k] = pop[(size_t)m * D + k];
+  }
+  if (picks) {
+    picks[0] = r1;
+    picks[1] = r2;
+    picks[2] = r3;
+    picks[3] = jr;
+  }
+}
+
+int or_de_select(int np, int D, double *pop, double *cost, const double *trial,
+                 const double *tcost, unsigned char *kept) {
+  int bm = 0;
+  for (int m = 0; m < np; ++m) {
+    const int take = tcost[m] <= cost[m];
+    if (take) {
+      cost[m] = tcost[m];
+      memcpy(pop + (size_t)m * D, trial + (size_t)m * D, sizeof(double) * (size_t)D);
+    }
+    if (kept) kept[m] = (unsigned char)take;
+    if (cost[m] < cost[bm]) bm = m;
+  }
+  return bm;
+}
+
+double or_de_fn(or_popcost fn, void *user, int D, double *x, const double *lo,
+                const double *hi, int np, int generations, double Fm, double CR,
+                uint64_t seed, double *best_hist) {
   double *pop = (double *)malloc(sizeof(double) * (size_t)np * D);
   double *trial = (double *)malloc(sizeof(double) * (size_t)np * D);
   double *cost = (double *)malloc(sizeof(double) * (size_t)np);
   double *tcost = (double *)malloc(sizeof(double) * (size_t)np);
-  for (int k = 0; k < D; ++k) {
-    double v = x[k];
-    if (v < lo[k]) v = lo[k];
-    if (v > hi[k]) v = hi[k];
-    pop[k] = v;
-  }
-  for (int m = 1; m < np; ++m)
-    for (int k = 0; k < D; ++k)
-      pop[(size_t)m * D + k] = lo[k] + or_rng_unit(seed, 0, (uint64_t)m * D + k) * (hi[k] - lo[k]);
-  or_population_cost(c, pop, np, cost);
-  for (int g = 1; g <= generations; ++g) {
-    for (int m = 0; m < np; ++m) {
-      int t = 0;
-      const int r1 = de_pick(seed, 2u * g, m, np, &t, m, -1, -1);
-      const int r2 = de_pick(seed, 2u * g, m, np, &t, m, r1, -1);
-      const int r3 = de_pick(seed, 2u * g, m, np, &t, m, r1, r2);
-      const uint64_t cs = 2u * (uint64_t)g + 1u;
-      const int jr = (int)(or_rng_unit(seed, cs, (uint64_t)m * (D + 1) + D) * D);
-      for (int k = 0; k < D; ++k) {
-        double v;
-        if (or_rng_unit(seed, cs, (uint64_t)m * (D + 1) + k) < CR || k == jr) {
-          v = pop[(size_t)r1 * D + k] + Fm * (pop[(size_t)r2 * D + k] - pop[(size_t)r3 * D + k]);
-          if (v < lo[k]) v = lo[k];
-          if (v > hi[k]) v = hi[k];
-        } else {
-          v = pop[(size_t)m * D + k];
-        }
-        trial[(size_t)m * D + k] = v;
-      }
-    }
-    or_population_cost(c, trial, np, tcost);
-    double best = INFINITY;
-    for (int m = 0; m < np; ++m) {
-      if (tcost[m] <= cost[m]) {
-        cost[m] = tcost[m];
-        memcpy(pop + (size_t)m * D, trial + (size_t)m * D, sizeof(double) * (size_t)D);
-      }
-      if (cost[m] < best) best = cost[m];
-    }
-    if (best_hist) best_hist[g - 1] = best;
-  }
+  or_de_init(x, lo, hi, np, D, seed, pop);
+  fn(user, pop, np, cost);
   int bm = 0;
   for (int m = 1; m < np; ++m) if (cost[m] < cost[bm]) bm = m;
+  for (int g = 1; g <= generations; ++g) {
+    for (int m = 0; m < np; ++m)
+      or_de_trial(pop, np, D, lo, hi, Fm, CR, seed, g, m, trial + (size_t)m * D, NULL);
+    fn(user, trial, np, tcost);
+    bm = or_de_select(np, D, pop, cost, trial, tcost, NULL);
+    if (best_hist) best_hist[g - 1] = cost[bm];
+  }
   memcpy(x, pop + (size_t)bm * D, sizeof(double) * (size_t)D);
   const double b = cost[bm];
-  or_scatter_params(c, x);
   free(pop); free(trial); free(cost); free(tcost);
+  return b;
+}
+
+static void de_ctx_cost(void *user, const double *X, int np, double *cost) {
+  or_population_cost((or_ctx *)user, X, np, cost);
+}
+
+double or_de(or_ctx *c, double *x, const double *lo, const double *hi, int np,
+             int generations, double Fm, double CR, uint64_t seed,
+             double *best_hist) {
+  const double b = or_de_fn(de_ctx_cost, c, or_param_count(c), x, lo, hi, np, generations, Fm,
+                            CR, seed, best_hist);
+  or_scatter_params(c, x);
   return b;
 }
 

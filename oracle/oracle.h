@@ -19,9 +19,17 @@
  *                     law, ADP(u=w=0)==EAM) and F = -dE/dx finite differences.
  *   analytic J      - pinned against the reference's own central-difference
  *                     Jacobian (h = 1e-5) with its test metric.
- *   LM / DE / MLP   - parity unpinned against the reference (third-party
- *                     Eigen MINPACK / Boost.Math DE trajectories); the oracle
- *                     defines the algorithm the GPU path must reproduce.
+ *   LM (MINPACK)    - pinned by the published minima of the More-Garbow-
+ *                     Hillstrom test problems and the defining properties of
+ *                     the pivoted QR and lmpar (Eigen's LevenbergMarquardt is
+ *                     not vendored by the reference).
+ *   DE              - pinned by the invariants of Storn-Price rand/1/bin
+ *                     (distinct picks, forced jrand component, CR = 0 / 1
+ *                     limits, box clamping, <= selection, crossover rate) and
+ *                     by the known minima of the sphere and Rosenbrock
+ *                     functions; the RNG stream is this build's own (Boost.Math
+ *                     and mt19937_64 trajectories are not reproducible here).
+ *   MLP             - gradients pinned by central differences of the loss.
  */
 #ifndef FS_ORACLE_H
 #define FS_ORACLE_H
@@ -118,6 +126,25 @@ double or_rng_unit(uint64_t seed, uint64_t stream, uint64_t index);
 double or_de(or_ctx *c, double *x, const double *lo, const double *hi, int np,
              int generations, double F, double CR, uint64_t seed,
              double *best_hist);
+/* The pieces of or_de, exposed so tests can check each one:
+ * or_de_init: member 0 = clamp(x), the others uniform in the box;
+ * or_de_trial: trial of member m in generation g; picks (may be NULL)
+ *   receives r1, r2, r3 and jrand;
+ * or_de_select: greedy selection (trial kept when its cost is <= the
+ *   member's); kept[m] (may be NULL) flags the replaced members; returns the
+ *   index of the first member of lowest cost;
+ * or_de_fn: the whole driver on any population cost function. */
+typedef void (*or_popcost)(void *user, const double *X, int np, double *cost);
+void or_de_init(const double *x, const double *lo, const double *hi, int np, int D,
+                uint64_t seed, double *pop);
+void or_de_trial(const double *pop, int np, int D, const double *lo, const double *hi,
+                 double F, double CR, uint64_t seed, int g, int m, double *out,
+                 int *picks);
+int or_de_select(int np, int D, double *pop, double *cost, const double *trial,
+                 const double *tcost, unsigned char *kept);
+double or_de_fn(or_popcost fn, void *user, int D, double *x, const double *lo,
+                const double *hi, int np, int generations, double F, double CR,
+                uint64_t seed, double *best_hist);
 
 /* FP64 MLP (D -> H -> H -> 1, GELU tanh form) forward and backward, the CPU
  * check for the BF16-MFMA surrogate kernels. W1[H x D], W2[H x H], W3[H]. */
