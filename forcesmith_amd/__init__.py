@@ -37,7 +37,7 @@ EXPORTS = (
     "fs_param_count", "fs_gather_params", "fs_scatter_params", "fs_gather_bounds", "fs_max_cutoff",
     "fs_residual_count", "fs_row_offsets", "fs_prepare", "fs_neighbor_count", "fs_get_neighbor_list",
     "fs_eval_forces", "fs_eval_config", "fs_eval_residual", "fs_eval_jacobian", "fs_normal_equations",
-    "fs_lm_minimize", "fs_lm_solve", "fs_normal_equations_solve", "fs_lm_step", "fs_population_cost", "fs_de_minimize", "fs_device_buffer", "fs_synchronize",
+    "fs_lm_minimize", "fs_lm_solve", "fs_normal_equations_solve", "fs_lm_step", "fs_population_cost", "fs_de_minimize", "fs_de_last_stats", "fs_device_buffer", "fs_synchronize",
     "fs_set_launch", "fs_set_jacobian_tile", "fs_get_launch", "fs_autotune", "fs_time_step", "fs_mlp_forward", "fs_mlp_loss_grad",
 )
 
@@ -62,6 +62,12 @@ class fs_de_opts(ct.Structure):
                 ("seed", ct.c_uint64), ("surrogate", ct.c_int32), ("oversample", ct.c_int32),
                 ("retrain_every", ct.c_int32), ("hidden", ct.c_int32), ("target_cost", ct.c_double),
                 ("rank_tau_min", ct.c_double), ("progress_every", ct.c_int32), ("reserved", ct.c_int32 * 3)]
+
+
+class fs_de_stats(ct.Structure):
+    _fields_ = [("generations", ct.c_int32), ("retrains", ct.c_int32), ("screened", ct.c_int32),
+                ("fallbacks", ct.c_int32), ("first_loss", ct.c_float), ("last_loss", ct.c_float),
+                ("reserved", ct.c_int32 * 2)]
 
 
 _lib = None
@@ -117,6 +123,7 @@ def load_library(path: str = LIB_PATH):
         "fs_lm_step": (ct.c_int, [vp, ct.c_double, _dp]),
         "fs_population_cost": (ct.c_int, [vp, _dp, ct.c_int32, _dp]),
         "fs_de_minimize": (ct.c_int, [vp, ct.POINTER(fs_de_opts), _dp, _dp, _dp, _dp, _dp, _i32p]),
+        "fs_de_last_stats": (ct.c_int, [vp, ct.POINTER(fs_de_stats)]),
         "fs_device_buffer": (ct.c_int, [vp, ct.c_int32, ct.POINTER(vp), _i64p]),
         "fs_synchronize": (ct.c_int, [vp]),
         "fs_set_launch": (ct.c_int, [vp, ct.c_int32, ct.c_int32]),
@@ -373,6 +380,14 @@ class GpuCalculator:
                                              _p(np.ascontiguousarray(hi, np.float64), _dp), _p(x, _dp),
                                              ct.byref(best), _p(hist, _dp), ct.byref(gens)))
         return x, best.value, hist[:gens.value]
+
+    def de_stats(self) -> dict:
+        """Statistics of the last de() call: generations, retrains, screened
+        generations, rank-error fallbacks, first and last training loss."""
+        st = fs_de_stats()
+        self._check(self._lib.fs_de_last_stats(self._h, ct.byref(st)))
+        return {"generations": st.generations, "retrains": st.retrains, "screened": st.screened,
+                "fallbacks": st.fallbacks, "first_loss": st.first_loss, "last_loss": st.last_loss}
 
     # ---- launch shape / timing ----
     def set_launch(self, lanes_per_atom: int, blocks_per_cu: int = 8):

@@ -270,6 +270,7 @@ struct Context {
   DBuf<int32_t> pop_grp;
   std::vector<int32_t> h_pop_grp;
   int pop_grp_tiles = -1;
+  fs_de_stats de_stats{};          // of the last fs_de_minimize call
   Mlp *mlp = nullptr;              // owned; see mlp.hpp
   SurrogateState *surr = nullptr;  // owned; see de.cpp
   // per-kernel timing of one Jacobian stage (HIP events, context stream):

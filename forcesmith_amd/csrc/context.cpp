@@ -820,6 +820,13 @@ int fs_de_minimize(fs_ctx *ctx, const fs_de_opts *opts, const double *lo, const 
   return g != FS_OK ? g : rc;
 }
 
+int fs_de_last_stats(fs_ctx *ctx, fs_de_stats *out) {
+  return guarded(ctx, [&](Context &c) {
+    FS_REQUIRE(out, FS_ERR_ARG, "null output");
+    *out = c.de_stats;
+  });
+}
+
 int fs_device_buffer(fs_ctx *ctx, int32_t which, void **ptr, int64_t *bytes) {
   return guarded(ctx, [&](Context &c) {
     FS_REQUIRE(ptr && bytes, FS_ERR_ARG, "null output");

@@ -82,6 +82,7 @@ void de_trial(uint64_t seed, int g, int m, int np, int D, double F, double CR, c
 
 int de_run_device(Context &c, const fs_de_opts *o, const double *lo, const double *hi, double *x,
                   double *best, double *hist, int *gens);
+void record_de_stats(Context &c, int gens, bool screened_run);
 
 int de_run(Context &c, const fs_de_opts *o, const double *lo, const double *hi, double *x,
            double *best, double *hist, int *gens) {
@@ -126,6 +127,7 @@ int de_run(Context &c, const fs_de_opts *o, const double *lo, const double *hi, 
     }
   }
   *gens = g - 1;
+  record_de_stats(c, *gens, o->surrogate != 0);
   int bm = 0;
   for (int m = 1; m < NP; ++m)
     if (cost[m] < cost[bm]) bm = m;
@@ -186,9 +188,25 @@ struct SurrogateState {
   std::vector<float> chosen_pred;
   uint64_t rng_ctr = 0;
   int retrains = 0, fallbacks = 0, screened = 0;
+  float first_loss = 0.0f, last_loss = 0.0f;
 };
 
 void destroy_surrogate(SurrogateState *s) { delete s; }
+
+// statistics of the run that just ended (the surrogate state of a plain run
+// is that of an earlier call, so it is read only when this run screened)
+void record_de_stats(Context &c, int gens, bool screened_run) {
+  fs_de_stats st{};
+  st.generations = gens;
+  if (screened_run && c.surr) {
+    st.retrains = c.surr->retrains;
+    st.screened = c.surr->screened;
+    st.fallbacks = c.surr->fallbacks;
+    st.first_loss = c.surr->first_loss;
+    st.last_loss = c.surr->last_loss;
+  }
+  c.de_stats = st;
+}
 
 namespace {
 
@@ -257,8 +275,12 @@ void train(Context &c, SurrogateState &S, int D, int hidden, uint64_t seed) {
     FS_HIP(hipGetLastError());
     S.rng_ctr += uint64_t(B);
     mlp_backward(S.mlp, c.stream, B, S.bX.p, S.bt.p);
+    if (st == 0 && S.retrains == 0)
+      FS_HIP(hipMemcpyAsync(&S.first_loss, S.mlp.loss.p, sizeof(float), hipMemcpyDeviceToHost,
+                            c.stream));
     mlp_adam(S.mlp, c.stream, 1e-3f);
   }
+  FS_HIP(hipMemcpyAsync(&S.last_loss, S.mlp.loss.p, sizeof(float), hipMemcpyDeviceToHost, c.stream));
   FS_HIP(hipStreamSynchronize(c.stream));
   S.ready = true;
   S.fallback = false;
@@ -671,6 +693,7 @@ int de_run_device(Context &c, const fs_de_opts *o, const double *lo, const doubl
   }
   const int ran = stop > 0 ? stop : done;
   *gens = ran;
+  record_de_stats(c, ran, o->surrogate != 0);
   // best member
   std::vector<double> costv(NP);
   FS_HIP(hipMemcpyAsync(costv.data(), d.cost.p, 8 * NP, hipMemcpyDeviceToHost, st));

@@ -22,7 +22,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 namespace {
 
 constexpr int kBM = 64, kBN = 64, kBK = 32;
-enum { EPI_STORE = 0, EPI_BIAS_GELU = 1 };
+constexpr int EPI_STORE = MLP_EPI_STORE, EPI_BIAS_GELU = MLP_EPI_BIAS_GELU;
 
 __device__ inline float gelu_f(float u) {
   const float k = 0.7978845608028654f;
@@ -166,82 +166,65 @@ __global__ void k_adam(int64_t n, float *__restrict__ p, const float *__restrict
 
 inline unsigned cdiv(int64_t a, int b) { return unsigned((a + b - 1) / b); }
 
-template <bool TA, bool TB, int EPI>
-void gemm(hipStream_t s, int M, int N, int K, const float *A, int lda, const float *B, int ldb,
-          float *C, int ldc, const float *bias = nullptr, float *H = nullptr) {
-  if (M <= 0 || N <= 0) return;
-  dim3 grid(cdiv(N, kBN), cdiv(M, kBM));
-  hipLaunchKernelGGL((k_gemm_bf16<TA, TB, EPI>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C,
-                     ldc, bias, H);
-  FS_HIP(hipGetLastError());
-}
+// the kernels behind the step sequence of mlp_seq.hpp, on one stream
+struct GpuOps {
+  hipStream_t s;
+  template <bool TA, bool TB, int EPI>
+  void gemm(int M, int N, int K, const float *A, int lda, const float *B, int ldb, float *C, int ldc,
+            const float *bias, float *H) {
+    if (M <= 0 || N <= 0) return;
+    dim3 grid(cdiv(N, kBN), cdiv(M, kBM));
+    hipLaunchKernelGGL((k_gemm_bf16<TA, TB, EPI>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C,
+                       ldc, bias, H);
+    FS_HIP(hipGetLastError());
+  }
+  void head(int B, int Hn, const float *H2, const float *W3, const float *b3, float *y) {
+    hipLaunchKernelGGL(k_head, dim3(cdiv(B, 4)), dim3(256), 0, s, B, Hn, H2, W3, b3, y);
+    FS_HIP(hipGetLastError());
+  }
+  void loss(int B, const float *y, const float *t, float *out) {
+    hipLaunchKernelGGL(k_loss, dim3(1), dim3(256), 0, s, B, y, t, out);
+    FS_HIP(hipGetLastError());
+  }
+  void head_back(int B, int Hn, const float *y, const float *t, const float *W3, const float *Z2,
+                 float *gy, float *dZ2) {
+    hipLaunchKernelGGL(k_head_back, dim3(cdiv(int64_t(B) * Hn, 256)), dim3(256), 0, s, B, Hn, y, t, W3,
+                       Z2, gy, dZ2);
+    FS_HIP(hipGetLastError());
+  }
+  void colsum(int B, int Ncol, const float *A, const float *w, float *out) {
+    hipLaunchKernelGGL(k_colsum, dim3(cdiv(Ncol, 256)), dim3(256), 0, s, B, Ncol, A, w, out);
+    FS_HIP(hipGetLastError());
+  }
+  void gelu_back(int64_t n, const float *Z, float *dH) {
+    hipLaunchKernelGGL(k_gelu_back, dim3(cdiv(n, 256)), dim3(256), 0, s, n, Z, dH);
+    FS_HIP(hipGetLastError());
+  }
+  void adam(int64_t n, float *p, const float *g, float *m, float *v, float lr, float b1, float b2,
+            float c1, float c2, float eps) {
+    hipLaunchKernelGGL(k_adam, dim3(cdiv(n, 256)), dim3(256), 0, s, n, p, g, m, v, lr, b1, b2, c1, c2,
+                       eps);
+    FS_HIP(hipGetLastError());
+  }
+};
 
 } // namespace
 
-void mlp_init(Mlp &m, int D, int Hn) {
-  m.D = D;
-  m.Hn = Hn;
-  m.cap = 0;
-  const size_t np = m.n_params();
-  m.P.alloc(np);
-  m.G.alloc(np);
-  m.Mo.alloc(np);
-  m.Vo.alloc(np);
-  m.step = 0;
-}
+void mlp_init(Mlp &m, int D, int Hn) { mlp_state_init(m, D, Hn); }
 
-// y_out: caller-owned outputs, or null for the workspace's own m.y (taken
-// after the workspace is sized for B)
 void mlp_forward(Mlp &m, hipStream_t s, int B, const float *X, float *y_out) {
-  m.ensure(B);
-  gemm<false, true, EPI_BIAS_GELU>(s, B, m.Hn, m.D, X, m.D, m.W1(), m.D, m.Z1.p, m.Hn, m.b1(),
-                                   m.H1.p);
-  gemm<false, true, EPI_BIAS_GELU>(s, B, m.Hn, m.Hn, m.H1.p, m.Hn, m.W2(), m.Hn, m.Z2.p, m.Hn,
-                                   m.b2(), m.H2.p);
-  hipLaunchKernelGGL(k_head, dim3(cdiv(B, 4)), dim3(256), 0, s, B, m.Hn, m.H2.p, m.W3(), m.b3(),
-                     y_out ? y_out : m.y.p);
-  FS_HIP(hipGetLastError());
+  GpuOps ops{s};
+  mlp_forward_seq(ops, m, B, X, y_out);
 }
 
-// gradient of mean 0.5 (y - t)^2 into m.G; returns nothing (loss in m.loss)
 void mlp_backward(Mlp &m, hipStream_t s, int B, const float *X, const float *t) {
-  const int Hn = m.Hn, D = m.D;
-  // outputs into m.y: passed as null, so that mlp_forward takes the pointer
-  // after m.ensure(B), which reallocates the workspace when B exceeds every
-  // earlier batch (a pointer read here would be the freed buffer)
-  mlp_forward(m, s, B, X, nullptr);
-  hipLaunchKernelGGL(k_loss, dim3(1), dim3(256), 0, s, B, m.y.p, t, m.loss.p);
-  // dZ2 in tmp
-  hipLaunchKernelGGL(k_head_back, dim3(cdiv(int64_t(B) * Hn, 256)), dim3(256), 0, s, B, Hn, m.y.p, t,
-                     m.W3(), m.Z2.p, m.gy.p, m.tmp.p);
-  float *gW1 = m.G.p, *gb1 = gW1 + size_t(Hn) * D, *gW2 = gb1 + Hn, *gb2 = gW2 + size_t(Hn) * Hn,
-        *gW3 = gb2 + Hn, *gb3 = gW3 + Hn;
-  // gW3 = sum_b gy_b H2_b ; gb3 = sum_b gy_b
-  hipLaunchKernelGGL(k_colsum, dim3(cdiv(Hn, 256)), dim3(256), 0, s, B, Hn, m.H2.p, m.gy.p, gW3);
-  hipLaunchKernelGGL(k_colsum, dim3(1), dim3(256), 0, s, B, 1, m.gy.p, (const float *)nullptr, gb3);
-  // gW2 = dZ2^T H1 ; gb2 = colsum dZ2
-  gemm<true, false, EPI_STORE>(s, Hn, Hn, B, m.tmp.p, Hn, m.H1.p, Hn, gW2, Hn);
-  hipLaunchKernelGGL(k_colsum, dim3(cdiv(Hn, 256)), dim3(256), 0, s, B, Hn, m.tmp.p,
-                     (const float *)nullptr, gb2);
-  // dH1 = dZ2 W2 -> Z2 buffer reused; dZ1 = dH1 * gelu'(Z1)
-  gemm<false, false, EPI_STORE>(s, B, Hn, Hn, m.tmp.p, Hn, m.W2(), Hn, m.H2.p, Hn);
-  hipLaunchKernelGGL(k_gelu_back, dim3(cdiv(int64_t(B) * Hn, 256)), dim3(256), 0, s,
-                     int64_t(B) * Hn, m.Z1.p, m.H2.p);
-  // gW1 = dZ1^T X ; gb1 = colsum dZ1
-  gemm<true, false, EPI_STORE>(s, Hn, D, B, m.H2.p, Hn, X, D, gW1, D);
-  hipLaunchKernelGGL(k_colsum, dim3(cdiv(Hn, 256)), dim3(256), 0, s, B, Hn, m.H2.p,
-                     (const float *)nullptr, gb1);
-  FS_HIP(hipGetLastError());
+  GpuOps ops{s};
+  mlp_backward_seq(ops, m, B, X, t);
 }
 
 void mlp_adam(Mlp &m, hipStream_t s, float lr) {
-  ++m.step;
-  const float b1 = 0.9f, b2 = 0.999f;
-  const float c1 = 1.f - std::pow(b1, float(m.step)), c2 = 1.f - std::pow(b2, float(m.step));
-  const int64_t n = int64_t(m.n_params());
-  hipLaunchKernelGGL(k_adam, dim3(cdiv(n, 256)), dim3(256), 0, s, n, m.P.p, m.G.p, m.Mo.p, m.Vo.p, lr,
-                     b1, b2, c1, c2, 1e-8f);
-  FS_HIP(hipGetLastError());
+  GpuOps ops{s};
+  mlp_adam_seq(ops, m, lr);
 }
 
 } // namespace fsa

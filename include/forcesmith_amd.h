@@ -221,6 +221,26 @@ int fs_de_minimize(fs_ctx *ctx, const fs_de_opts *opts, const double *lo,
                    const double *hi, double *x, double *best_cost,
                    double *hist, int32_t *generations_run);
 
+/* Statistics of the last fs_de_minimize call on this context (no reference
+ * counterpart: the surrogate is new functionality). generations: generations
+ * run; retrains: surrogate trainings; screened: generations whose trials were
+ * picked by the surrogate; fallbacks: screened generations after which the
+ * Kendall tau of predicted against exact costs fell below rank_tau_min (the
+ * driver then proposes plain trials until the next retraining); first_loss /
+ * last_loss: the training loss (mean 0.5 (y - t)^2 of standardised log costs)
+ * at the first step of the first retraining and the last step of the last
+ * (0 when nothing was trained). */
+typedef struct fs_de_stats {
+  int32_t generations;
+  int32_t retrains;
+  int32_t screened;
+  int32_t fallbacks;
+  float first_loss;
+  float last_loss;
+  int32_t reserved[2];
+} fs_de_stats;
+int fs_de_last_stats(fs_ctx *ctx, fs_de_stats *out);
+
 /* Device views and tuning (no reference counterpart: measurement hooks). */
 enum fs_buffer {
   FS_BUF_RESIDUAL = 0, /* double[R] */

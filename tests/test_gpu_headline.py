@@ -1,0 +1,126 @@
+"""The throughput workload itself against the oracle on the device: the
+ternary tabulated EAM over 500 configurations of 32-2048 atoms (236,289
+atoms, 9.67M directed pairs, D = 240), plus the cost at parameters an LM run
+has converged, where residuals approach the noise floor.
+
+Bars (the same as tests/test_gpu_parity.py):
+  mixed precision: |dF| <= 1e-5 eV/A; |dE| <= 1e-6 |E|; stress max|ds| <=
+      1e-6 max|s| per configuration; cost |dc| <= 1e-6 c
+  FP64 mode: 1e-11 relative on every quantity
+  Jacobian: max |J - J_oracle| / (1 + |J_oracle|) <= 2e-4 (the reference FD
+      metric, test_param_jacobian.cpp:181-193, at FP32 rounding) and
+      |J - J_oracle|_F <= 1e-5 |J_oracle|_F over the compared rows
+The Jacobian of the full dataset is compared on the rows of its largest and
+smallest configurations (the oracle evaluates those configurations alone:
+configurations are independent, so their rows are the same).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import forcesmith_amd as fa
+import oracle
+from forcesmith_amd import datasets as dsm
+
+pytestmark = pytest.mark.gpu
+
+
+def _print(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def c2():
+    oracle.set_threads(max(1, min(16, os.cpu_count() or 1)))
+    _print("generating the 500-configuration ternary EAM workload")
+    w = dsm.config2()
+    o = oracle.Oracle(w.dataset, w.model)
+    o.prepare()
+    return w, o
+
+
+def _compare(tag, o, g, mixed):
+    Fo, Eo, So, Lo = o.eval_forces()
+    Fg, Eg, Sg, Lg = g.eval_forces()
+    co, cg = o.cost(o.gather()), g.cost()
+    fmax = np.abs(Fg - Fo).max()
+    erel = np.max(np.abs(Eg - Eo) / np.abs(Eo))
+    sscale = np.abs(So).reshape(len(So), -1).max(1)[:, None, None] + 1e-12
+    srel = np.max(np.abs(Sg - So) / sscale)
+    crel = abs(cg - co) / co
+    _print(f"[{tag}] |dF|max={fmax:.3e} dE/E={erel:.3e} ds/s={srel:.3e} dcost/cost={crel:.3e}")
+    if mixed:
+        assert fmax <= 1e-5 and erel <= 1e-6 and srel <= 1e-6 and crel <= 1e-6
+        assert np.max(np.abs(Lg - Lo)) <= 1e-6 * max(1.0, np.max(np.abs(Lo)))
+    else:
+        assert np.max(np.abs(Fg - Fo) / (1 + np.abs(Fo))) <= 1e-11
+        assert erel <= 1e-11 and srel <= 1e-11 and crel <= 1e-11
+        assert np.max(np.abs(Lg - Lo)) <= 1e-11 * max(1.0, np.max(np.abs(Lo)))
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_full_workload_forces_energy_stress_cost(c2, precision):
+    w, o = c2
+    g = fa.GpuCalculator(w.dataset, w.model, precision=precision)
+    assert g.prepare() == 9671480
+    _compare(f"500 configs/{precision}", o, g, precision == "mixed")
+
+
+@pytest.mark.timeout(900)
+def test_full_workload_jacobian_on_largest_and_smallest_configs(c2):
+    w, _ = c2
+    sizes = w.dataset.config_sizes()
+    order = np.argsort(sizes, kind="stable")
+    pick = sorted(set(order[-3:].tolist() + order[:2].tolist()))
+    _print(f"Jacobian rows of configurations {pick} (atoms {sizes[pick].tolist()})")
+    g = fa.GpuCalculator(w.dataset, w.model)
+    g.prepare()
+    g.autotune()
+    x = g.gather_params()
+    Jg = g.jacobian(x)
+    ro = g.row_offsets()
+    rows = np.concatenate([np.arange(ro[c], ro[c + 1]) for c in pick])
+    Jg = Jg[rows].astype(np.float64)
+    sub = w.dataset.subset(pick)
+    o = oracle.Oracle(sub, w.model)
+    o.prepare()
+    Jo = o.jacobian(x)
+    assert Jo.shape == Jg.shape
+    m = float(np.max(np.abs(Jg - Jo) / (1.0 + np.abs(Jo))))
+    fro = float(np.linalg.norm(Jg - Jo) / np.linalg.norm(Jo))
+    _print(f"[500 configs] J metric {m:.3e}, relative Frobenius {fro:.3e} over {len(rows)} rows")
+    assert m <= 2e-4
+    assert fro <= 1e-5
+
+
+def _converged_cost(w, iters, tag):
+    o = oracle.Oracle(w.dataset, w.model)
+    o.prepare()
+    xs, hist, acc = o.lm(o.gather(), iters)
+    _print(f"[{tag}] oracle LM {iters} iterations: cost {hist[0]:.6e} -> {hist[-1]:.6e} ({acc} accepted)")
+    co = o.cost(xs)
+    Fo = o.eval_forces()[0]
+    g = fa.GpuCalculator(w.dataset, w.model)
+    g.prepare()
+    cg = g.cost(xs)
+    Fg = g.eval_forces()[0]
+    crel = abs(cg - co) / co
+    _print(f"[{tag}] converged cost oracle {co:.9e} gpu {cg:.9e} rel {crel:.3e}; "
+           f"|dF|max {np.abs(Fg - Fo).max():.3e}")
+    assert crel <= 1e-6
+    assert np.abs(Fg - Fo).max() <= 1e-5
+
+
+@pytest.mark.timeout(600)
+def test_cost_at_converged_parameters_single_element():
+    # one residual + Jacobian, then 50 LM iterations (the single-element FCC run)
+    oracle.set_threads(max(1, min(16, os.cpu_count() or 1)))
+    _converged_cost(dsm.config1(), 50, "FCC 20 configs")
+
+
+@pytest.mark.timeout(600)
+def test_cost_at_converged_parameters_ternary():
+    oracle.set_threads(max(1, min(16, os.cpu_count() or 1)))
+    _converged_cost(dsm.config2(n_configs=40), 15, "ternary 40 configs")
