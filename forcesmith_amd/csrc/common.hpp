@@ -256,7 +256,7 @@ struct Context {
     int Dd = 0, dens_c0 = 0, gi_nf = 0, NSP = 0, NSD = 0, NSU = 0, NSW = 0, Ddip = 0, Dquad = 0;
   } jl;
   DBuf<float4> jrec; // pair records sorted by (partner type, r) per atom
-  DBuf<float> jr;    // their FP32 distances
+  DBuf<double> jr;   // their FP64 distances
   bool jsort_valid = false;
   DBuf<float> Hs, emb_d, emb_v;
   DBuf<int32_t> emb_k, sw_csw, sw_clam;

@@ -105,11 +105,23 @@ __device__ inline float band(const float *sS, const TableDesc &T, int k, int m, 
   return v;
 }
 
+// interval of a pair table at the FP64 distance and the FP32 offset in it:
+// an FP32 distance and 1/h would place every pair of a fine grid (knots a few
+// hundredths of an angstrom apart) off by up to u * 6e-8 intervals, a shift
+// the derivative basis (slope ~6/h in t) turns into a systematic error that
+// survives the cancelling sum over neighbours
+__device__ inline int pair_interval(const TableDesc &T, double r, float &t) {
+  double td;
+  const int k = spl_interval64(T, r, td);
+  t = float(td);
+  return k;
+}
+
 // interval + value / derivative basis of a pair table at r (cached-site
 // semantics: linear site only at the span ends or for n < 4 tables)
-__device__ inline int pair_basis(const TableDesc &T, float r, float bv[4], float bd[4]) {
+__device__ inline int pair_basis(const TableDesc &T, double r, float bv[4], float bd[4]) {
   float t;
-  const int k = spl_interval32(T, r, t);
+  const int k = pair_interval(T, r, t);
   if (T.linear) {
     bv[0] = 1.0f - t;
     bv[1] = t;
@@ -143,7 +155,7 @@ struct JacArgs {
   const float4 *rec;
   const uint32_t *bits;
   const float4 *jrec; // pairs sorted by (partner type, r) per atom
-  const float *jr;    // their FP32 distances
+  const double *jr;   // their FP64 distances (the sort key)
   const TableDesc *tabs;
   const double *tx, *ty, *ts, *tS;
   const float4 *ivl32;
@@ -177,18 +189,18 @@ __device__ inline void stage(const JacArgs &a, TableDesc *sT, float *sS, int nk)
 // Jacobian pair order. Every column's pairs are those whose interval k on the
 // column's table lies in [m - 3, m + 2]; with an atom's pairs sorted by
 // (partner type, r) they form one contiguous run per partner type, found by
-// binary search on the FP32 interval index (monotone in r). The FP32
-// distance stored here is the one every Jacobian kernel uses, so the search
-// and the evaluation agree exactly.
+// binary search on the interval index (monotone in r). The FP64 distance
+// stored here is the one every Jacobian kernel locates intervals from, so the
+// search and the evaluation agree exactly.
 __global__ void k_jac_sort(int64_t N, const int64_t *__restrict__ pair_off,
-                           const float4 *__restrict__ rec, float4 *__restrict__ jrec,
-                           float *__restrict__ jr) {
+                           const float4 *__restrict__ rec, const double *__restrict__ r64,
+                           float4 *__restrict__ jrec, double *__restrict__ jr) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= N) return;
   const int64_t p0 = pair_off[i], p1 = pair_off[i + 1];
   for (int64_t p = p0; p < p1; ++p) {
     const float4 q = rec[p];
-    const float r = sqrtf(q.x * q.x + q.y * q.y + q.z * q.z);
+    const double r = r64[p];
     const int t = pj_type(__float_as_uint(q.w));
     int64_t s = p;
     while (s > p0) {
@@ -351,7 +363,7 @@ __global__ __launch_bounds__(256) void k_jac_dens(JacArgs a, int nknots) {
       int ty = kMaxTypes;
       if (p < p1) {
         const uint32_t b = __float_as_uint(a.jrec[p].w);
-        const float r = a.jr[p];
+        const double r = a.jr[p];
         ty = pj_type(b);
         const TableDesc T = sT[a.npair + ty];
         float bv[4], bd[4];
@@ -536,7 +548,8 @@ __device__ __forceinline__ void jac_main_body(const JacArgs &a, int nknots, cons
         if (p < p1) {
           const float4 q = a.jrec[p];
           const uint32_t b = __float_as_uint(q.w);
-          const float rr = a.jr[p];
+          const double r64 = a.jr[p];
+          const float rr = float(r64);
           const bool tiny = (b & kFlagTiny) != 0;
           const float inv_r = tiny ? 0.0f : 1.0f / rr;
           const int tj = pj_type(b);
@@ -551,7 +564,7 @@ __device__ __forceinline__ void jac_main_body(const JacArgs &a, int nknots, cons
             // pair table phi_(ti,tj)
             float ce[6];
             const TableDesc T = sT[ord_pair(ti, tj, a.ntypes)];
-            const int k = pair_basis(T, rr, bv, bd);
+            const int k = pair_basis(T, r64, bv, bd);
             const float w = ((b & kFlagPhi) && !tiny) ? 1.0f : 0.0f;
             band6(sS, T, k, bd, w, cc);
             band6(sS, T, k, bv, 0.5f * w, ce);
@@ -570,7 +583,7 @@ __device__ __forceinline__ void jac_main_body(const JacArgs &a, int nknots, cons
             float dgj = 0.0f;
             {
               const TableDesc T = sT[a.npair + tj];
-              const int k = pair_basis(T, rr, bv, bd);
+              const int k = pair_basis(T, r64, bv, bd);
               const bool on = (b & kFlagGj) && !tiny;
               band6(sS, T, k, bd, on ? Fpi : 0.0f, cc);
               P.kgj[lane] = k;
@@ -578,7 +591,7 @@ __device__ __forceinline__ void jac_main_body(const JacArgs &a, int nknots, cons
               for (int d = 0; d < 6; ++d) P.cgj[d][lane] = cc[d];
               if (on) {
                 float tt, v;
-                const int kk = spl_interval32(T, rr, tt);
+                const int kk = pair_interval(T, r64, tt);
                 spl_eval32(T, a.ivl32, kk, tt, v, dgj);
               }
             }
@@ -586,7 +599,7 @@ __device__ __forceinline__ void jac_main_body(const JacArgs &a, int nknots, cons
             float dgi = 0.0f;
             {
               const TableDesc T = sT[a.npair + ti];
-              const int k = pair_basis(T, rr, bv, bd);
+              const int k = pair_basis(T, r64, bv, bd);
               const bool on = (b & kFlagGi) && !tiny;
               band6(sS, T, k, bd, on ? fpj : 0.0f, cc);
               P.kgi[lane] = k;
@@ -594,7 +607,7 @@ __device__ __forceinline__ void jac_main_body(const JacArgs &a, int nknots, cons
               for (int d = 0; d < 6; ++d) P.cgi[d][lane] = cc[d];
               if (on) {
                 float tt, v;
-                const int kk = spl_interval32(T, rr, tt);
+                const int kk = pair_interval(T, r64, tt);
                 spl_eval32(T, a.ivl32, kk, tt, v, dgi);
               }
             }
@@ -953,12 +966,12 @@ __global__ __launch_bounds__(256) void k_adp_mom(JacArgs a, int nknots, const Sl
       if (p < p1) {
         const float4 q = a.jrec[p];
         const uint32_t b = __float_as_uint(q.w);
-        const float rr = a.jr[p];
+        const double r64 = a.jr[p];
         const int tj = pj_type(b);
         ty = tj;
         const TableDesc Tq = sT[tab0 + ord_pair(ti, tj, a.ntypes)];
         float bv[4], bd[4], c[6];
-        const int k = pair_basis(Tq, rr, bv, bd);
+        const int k = pair_basis(Tq, r64, bv, bd);
         band6(sS, Tq, k, bv, ((b & dq_flag<PASS>()) && !(b & kFlagTiny)) ? 1.0f : 0.0f, c);
         P.k[lane] = k;
         P.d0[lane] = q.x;
@@ -1111,20 +1124,21 @@ __global__ __launch_bounds__(256) void k_adp_dq(JacArgs a, int nknots, const Slo
         if (p < p1) {
           const float4 q = a.jrec[p];
           const uint32_t b = __float_as_uint(q.w);
-          const float rr = a.jr[p];
+          const double r64 = a.jr[p];
+          const float rr = float(r64);
           const bool on = (b & dq_flag<PASS>()) && !(b & kFlagTiny);
           const int tj = pj_type(b);
           ty = tj;
           const int32_t jg = int32_t(cbase + pj_local(b));
           const TableDesc Tq = sT[tab0 + ord_pair(ti, tj, a.ntypes)];
           float bv[4], bd[4], cc[6], ce[6];
-          const int k = pair_basis(Tq, rr, bv, bd);
+          const int k = pair_basis(Tq, r64, bv, bd);
           band6(sS, Tq, k, bd, on ? 1.0f : 0.0f, cc);
           band6(sS, Tq, k, bv, on ? 1.0f : 0.0f, ce);
           float v = 0.0f, dv = 0.0f;
           if (on) {
             float tt;
-            const int kk = spl_interval32(Tq, rr, tt);
+            const int kk = pair_interval(Tq, r64, tt);
             spl_eval32(Tq, a.ivl32, kk, tt, v, dv);
           }
           const float ir = on ? 1.0f / rr : 0.0f;
@@ -1865,7 +1879,7 @@ void launch_jacobian(Context &c) {
       c.jrec.alloc(size_t(std::max<int64_t>(c.P, 1)));
       c.jr.alloc(size_t(std::max<int64_t>(c.P, 1)));
       hipLaunchKernelGGL(k_jac_sort, dim3(nb(c.N, 128)), dim3(128), 0, c.stream, c.N, c.pair_off.p,
-                         c.pair_rec.p, c.jrec.p, c.jr.p);
+                         c.pair_rec.p, c.pair_r64.p, c.jrec.p, c.jr.p);
       FS_HIP(hipGetLastError());
       c.jsort_valid = true;
     }

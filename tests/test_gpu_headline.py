@@ -7,9 +7,9 @@ Bars (the same as tests/test_gpu_parity.py):
   mixed precision: |dF| <= 1e-5 eV/A; |dE| <= 1e-6 |E|; stress max|ds| <=
       1e-6 max|s| per configuration; cost |dc| <= 1e-6 c
   FP64 mode: 1e-11 relative on every quantity
-  Jacobian: max |J - J_oracle| / (1 + |J_oracle|) <= 2e-4 (the reference FD
+  Jacobian: max |J - J_oracle| / (1 + |J_oracle|) <= 5e-5 (the reference FD
       metric, test_param_jacobian.cpp:181-193, at FP32 rounding) and
-      |J - J_oracle|_F <= 1e-5 |J_oracle|_F over the compared rows
+      |J - J_oracle|_F <= 1e-6 |J_oracle|_F over the compared rows
 The Jacobian of the full dataset is compared on the rows of its largest and
 smallest configurations (the oracle evaluates those configurations alone:
 configurations are independent, so their rows are the same).
@@ -91,8 +91,8 @@ def test_full_workload_jacobian_on_largest_and_smallest_configs(c2):
     m = float(np.max(np.abs(Jg - Jo) / (1.0 + np.abs(Jo))))
     fro = float(np.linalg.norm(Jg - Jo) / np.linalg.norm(Jo))
     _print(f"[500 configs] J metric {m:.3e}, relative Frobenius {fro:.3e} over {len(rows)} rows")
-    assert m <= 2e-4
-    assert fro <= 1e-5
+    assert m <= 5e-5
+    assert fro <= 1e-6
 
 
 def _converged_cost(w, iters, tag):

@@ -1,13 +1,15 @@
 """GPU analytic Jacobian, FP64 normal equations and LM against the oracle.
 
-Bars: the FP32 Jacobian within max |J - J_oracle| / (1 + |J_oracle|) <= 2e-4
-(the reference FD metric, test_param_jacobian.cpp:181-193, at FP32 rounding);
+Bars: the FP32 Jacobian within max |J - J_oracle| / (1 + |J_oracle|) <= 5e-5
+(the reference FD metric, test_param_jacobian.cpp:181-193, at FP32 rounding:
+every interval is located from the FP64 distance, measured <= 2.4e-5);
 J^T J and J^T F within 1e-4 of the oracle's FP64 products relative to their
 largest entry; LM cost history non-increasing and its final cost within 1e-3
 of the oracle LM after the same number of iterations.
 """
 import numpy as np
 import pytest
+from scipy.interpolate import CubicSpline
 
 import forcesmith_amd as fa
 import oracle
@@ -52,7 +54,7 @@ def test_jacobian_matches_oracle(wl, name):
     m = metric(Jg, Jo)
     print(f"[{name}] J metric {m:.3e} shape {Jg.shape}", flush=True)
     assert Jg.shape == Jo.shape
-    assert m <= 2e-4
+    assert m <= 5e-5
 
 
 @pytest.mark.parametrize("name", ["eam3s", "sw", "adp2"])
@@ -110,11 +112,11 @@ def test_sw_factors_follow_the_parameters(wl):
     x2 = x1 * (1.0 + 0.02 * np.cos(np.arange(x1.size)))
     g.residual(x1)
     J2 = g.jacobian(x2).astype(np.float64)
-    assert metric(J2, o.jacobian(x2)) <= 2e-4
+    assert metric(J2, o.jacobian(x2)) <= 5e-5
     F1 = g.residual(x1)
     assert np.max(np.abs(F1 - o.residual(x1))) <= 1e-5
     J1 = g.jacobian().astype(np.float64)  # same parameters: the kept factors
-    assert metric(J1, o.jacobian(x1)) <= 2e-4
+    assert metric(J1, o.jacobian(x1)) <= 5e-5
 
 
 def test_jacobian_rows_by_kind(wl):
@@ -132,7 +134,7 @@ def test_jacobian_rows_by_kind(wl):
     for c in range(len(sizes)):
         e = ro[c] + 3 * sizes[c]
         for r in range(e, ro[c + 1]):
-            assert metric(Jg[r], Jo[r]) <= 2e-4, (c, r - e)
+            assert metric(Jg[r], Jo[r]) <= 5e-5, (c, r - e)
 
 
 def test_normal_equations(wl):
@@ -245,7 +247,7 @@ def test_smoothness_rows(wl, name):
     assert abs(cg - Fo @ Fo) <= 1e-6 * (Fo @ Fo)
     Jo, Jg = o.jacobian(x1), g.jacobian(x1).astype(np.float64)
     assert np.array_equal(Jg[R0:], Jo[R0:].astype(np.float32).astype(np.float64))
-    assert metric(Jg, Jo) <= 2e-4
+    assert metric(Jg, Jo) <= 5e-5
     if name.startswith("eam"):
         X = np.stack([x, x1, x * 0.99])
         co, cp = o.population_cost(X), g.population_cost(X)
@@ -400,12 +402,12 @@ def test_jacobian_edge_cases_clamp_empty_and_single_atom():
     g.prepare()
     Jg = g.jacobian(x).astype(np.float64)
     assert Jg.shape == Jo.shape
-    assert metric(Jg, Jo) <= 2e-4
+    assert metric(Jg, Jo) <= 5e-5
     # the limit rows carry the clamp chain (the lone atom's rho = 0 is clamped)
     ro = o.row_offsets()
     lim = [ro[c + 1] - 1 for c in range(len(ro) - 1)]
     assert np.any(Jo[lim] != 0.0)
-    assert metric(Jg[lim], Jo[lim]) <= 2e-4
+    assert metric(Jg[lim], Jo[lim]) <= 5e-5
 
 
 def test_adp_two_lane_slots_per_kind():
@@ -417,7 +419,7 @@ def test_adp_two_lane_slots_per_kind():
     for k in range(first, len(m.tables)):
         t = m.tables[k]
         xn = np.linspace(t.rmin, t.rmax, 30)
-        m.tables[k] = type(t)(30, t.rmin, t.rmax, np.interp(xn, t.x, t.y), np.zeros(30, dtype=bool))
+        m.tables[k] = type(t)(30, t.rmin, t.rmax, CubicSpline(t.x, t.y)(xn), np.zeros(30, dtype=bool))
     print("jac adp wide maps", flush=True)
     o = oracle.Oracle(w.dataset, m)
     o.prepare()
@@ -429,4 +431,4 @@ def test_adp_two_lane_slots_per_kind():
     Jg = g.jacobian(x).astype(np.float64)
     mt = metric(Jg, Jo)
     print(f"[adp wide] J metric {mt:.3e} shape {Jg.shape}", flush=True)
-    assert mt <= 2e-4
+    assert mt <= 5e-5

@@ -12,11 +12,12 @@ other branch:
   - density + embedding columns in four slots, and in six;
   - density tables with unequal free-knot counts (one table partly fixed).
 
-Bars: max |J - J_oracle| / (1 + |J_oracle|) <= 2e-4 and
-|J - J_oracle|_F <= 1e-5 |J_oracle|_F.
+Bars: max |J - J_oracle| / (1 + |J_oracle|) <= 5e-5 and
+|J - J_oracle|_F <= 1e-6 |J_oracle|_F.
 """
 import numpy as np
 import pytest
+from scipy.interpolate import CubicSpline
 
 import forcesmith_amd as fa
 import oracle
@@ -27,8 +28,11 @@ pytestmark = pytest.mark.gpu
 
 
 def resample(t: Table, n: int, free: bool) -> Table:
+    # through a smooth cubic: linear interpolation would put runs of knots on
+    # one line, where the makima weights |d_k+1 - d_k| sit at their kink and
+    # the Jacobian is undefined
     x = np.linspace(t.rmin, t.rmax, n)
-    return Table(n, t.rmin, t.rmax, np.interp(x, t.x, t.y), np.full(n, not free))
+    return Table(n, t.rmin, t.rmax, CubicSpline(t.x, t.y)(x), np.full(n, not free))
 
 
 def relayout(model: TabulatedModel, spec):
@@ -56,8 +60,8 @@ def check(ds, model, tag):
     m = float(np.max(np.abs(Jg - Jo) / (1.0 + np.abs(Jo))))
     fro = float(np.linalg.norm(Jg - Jo) / np.linalg.norm(Jo))
     print(f"[{tag}] D={Jg.shape[1]} J metric {m:.3e} relative Frobenius {fro:.3e}", flush=True)
-    assert m <= 2e-4
-    assert fro <= 1e-5
+    assert m <= 5e-5
+    assert fro <= 1e-6
     Fo = o.residual(x)
     Fg = g.residual(x)
     assert np.max(np.abs(Fg - Fo) / (1 + np.abs(Fo))) <= 1e-5
