@@ -25,6 +25,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 
 
@@ -86,8 +88,48 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(w, x, seconds, threads):
-    """FD Jacobian of the oracle on a bounded sample of columns."""
+def host_cpu():
+    """CPU model, logical CPUs and the CPUs this process may run on."""
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = os.cpu_count() or 1
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity": aff,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
+def baseline_threads(host):
+    """Threads of the CPU baseline: every CPU this process may use, within the
+    thread budget the host sets for it (OMP_NUM_THREADS; a GPU box allots 16
+    CPUs to each GPU)."""
+    n = host["affinity"]
+    try:
+        n = min(n, int(host["omp_num_threads"]))
+    except (TypeError, ValueError):
+        pass
+    return max(1, n)
+
+
+def cpu_baseline(w, x, seconds, threads, host):
+    """The oracle (an FP64 restatement of the reference's calculator) on this
+    host, following the baseline protocol: untimed setup, then per figure one
+    warm-up run and the median of three timed runs.
+      value: the reference algorithm, central differences (h = 1e-5) with the
+        configurations in parallel: 2D + 1 residual evaluations per unit,
+        timed on a bounded sample of k columns and scaled to 2D + 1;
+      single_thread: the same on one thread (one run, one column);
+      analytic_jacobian: labelled second baseline, the oracle's analytic
+        per-knot Jacobian plus one residual (not the reference's algorithm)."""
+    import statistics
+
     import oracle
     oracle.set_threads(threads)
     o = oracle.Oracle(w.dataset, w.model)
@@ -95,18 +137,46 @@ def cpu_baseline(w, x, seconds, threads):
     o.prepare()
     t_prep = time.time() - t0
     D = o.param_count
-    t_res = o.time_residual(x, 1)
-    k = int(max(1, min(D, (seconds / max(t_res, 1e-6) - 1) / 2)))
-    t_sample = o.time_fd_jacobian(x, k)  # 1 residual + 2k residuals
-    t_per_res = t_sample / (2 * k + 1)
-    t_unit = t_per_res * (2 * D + 1)
     C = w.dataset.n_configs
-    return {"value": C * D / t_unit, "unit": "configs*params/s", "cores": threads, "kind": "port",
-            "sample": f"oracle FP64 central-difference Jacobian (reference algorithm, h=1e-5): "
-                      f"1 residual + {k} of {D} columns over all {C} configs ({2 * k + 1} residual "
-                      f"evaluations, {t_sample:.1f} s); full residual+Jacobian extrapolated to "
-                      f"{2 * D + 1} evaluations = {t_unit:.1f} s; neighbour list + site priming "
-                      f"{t_prep:.1f} s untimed"}
+    t_res = o.time_residual(x, 1)
+    # one run of 2k + 1 residuals takes about a quarter of the budget
+    k = int(max(1, min(D, (seconds / 4 / max(t_res, 1e-6) - 1) / 2)))
+    o.time_fd_jacobian(x, k)  # warm-up
+    runs = [o.time_fd_jacobian(x, k) for _ in range(3)]
+    t_sample = statistics.median(runs)
+    t_unit = t_sample / (2 * k + 1) * (2 * D + 1)
+    out = {"value": C * D / t_unit, "unit": "configs*params/s", "cores": threads, "kind": "port",
+           "sample": f"oracle FP64 central-difference Jacobian (reference algorithm, h=1e-5) on "
+                     f"{threads} threads: 1 residual + {k} of {D} columns over all {C} configs "
+                     f"({2 * k + 1} residual evaluations), 1 warm-up + median of 3 runs "
+                     f"({', '.join(f'{r:.2f}' for r in runs)} s); scaled to {2 * D + 1} evaluations "
+                     f"= {t_unit:.1f} s per residual+Jacobian; neighbour list + site priming "
+                     f"{t_prep:.1f} s untimed",
+           "runs_s": runs, "host": host}
+    oracle.set_threads(1)
+    t1 = o.time_fd_jacobian(x, 1)
+    oracle.set_threads(threads)
+    t_unit1 = t1 / 3 * (2 * D + 1)
+    out["single_thread"] = {"value": C * D / t_unit1, "cores": 1,
+                            "sample": f"1 residual + 1 column (3 residual evaluations, {t1:.2f} s) "
+                                      f"scaled to {2 * D + 1}"}
+    J = np.zeros((o.residual_count, D))
+
+    def analytic():
+        t = time.perf_counter()
+        o.residual(x)
+        o._lib.or_jacobian_analytic(o._h, oracle._p(np.ascontiguousarray(x, dtype=np.float64), oracle._dp),
+                                    oracle._p(J, oracle._dp))
+        return time.perf_counter() - t
+
+    analytic()  # warm-up
+    ar = [analytic() for _ in range(3)]
+    ta = statistics.median(ar)
+    out["analytic_jacobian"] = {"value": C * D / ta, "cores": threads, "runs_s": ar,
+                                "label": "oracle FP64 residual + analytic per-knot Jacobian (a second "
+                                         "baseline; the reference computes these columns by central "
+                                         "differences)"}
+    return out
 
 
 def de_measure(a):
@@ -114,8 +184,6 @@ def de_measure(a):
     (D = 240, NP = 15 D, F = 0.65, CR = 0.5, seed 42): wall time of the
     screened run until its best cost is <= (1 + 1e-6) x the plain run's final
     cost, over the plain run's wall time."""
-    import numpy as np
-
     import forcesmith_amd as fa
     from forcesmith_amd import datasets
 
@@ -192,6 +260,20 @@ def measured_traffic(kernel="k_jac_de"):
     return None
 
 
+def measured_limiter(kernel="k_jac_de"):
+    """What the counters say bounds the kernel, from its committed summary
+    (the line 'limiter: ...'), or None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"{kernel}_traffic_eam_ternary_500cfg.md")
+    try:
+        for line in open(path):
+            if line.startswith("limiter:"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def pmc_driver(a):
     """Workload for rocprofv3 --pmc passes: the copy kernel of known size that
     scales FETCH_SIZE / WRITE_SIZE, then the Jacobian kernels."""
@@ -239,8 +321,6 @@ def main():
         print(f"bench.py measures one GPU: --gpus {a.gpus} is not supported; this build runs the "
               "evaluator on a single device", file=sys.stderr)
         sys.exit(2)
-    import numpy as np
-
     import forcesmith_amd as fa
     from forcesmith_amd import datasets
 
@@ -293,14 +373,16 @@ def main():
         t_pair = g.time_step(5, 10)
         kernels = {"residual": t_res, "jacobian": t_jac, "jac_de": t_main, "jac_pair": t_pair,
                    "normal_equations": t_ne}
-        # algorithmic bytes of k_jac_de (the dominant kernel) per launch, each
-        # datum once: pair records (float4 + FP32 distance, 20 B/pair), the FP32
-        # force rows of the density/embedding columns it writes (3N x D_de x 4 B)
-        # and the per-atom density-column rows it reads (Hs FP32 + H FP64, 12 B
-        # per atom and density column)
+        # compulsory bytes of k_jac_de (the dominant kernel) per launch: the
+        # pair records (int32 partner + 3 x FP32 displacement, 16 B/pair) and the
+        # FP32 force rows of the density/embedding columns it writes
+        # (3N x D_de x 4 B). Intermediates it also reads, reported apart: the
+        # FP64 sorted distances (8 B/pair) and the per-atom density-column rows
+        # (Hs FP32 + H FP64, 12 B per atom and density column)
         D_dens = sum(free[m.npair:m.npair + m.ntypes])
         D_de = sum(free[m.npair:])
-        b_main = 20 * P + 12 * N * D_de + 12 * N * D_dens
+        b_main = 16 * P + 12 * N * D_de
+        b_inter = 8 * P + 12 * N * D_dens
         achieved = b_main / (t_main * 1e-3) / 1e9
         ternary500 = a.workload == "eam-ternary" and C == 500
         roofline = {"bound": "hbm", "kernel": "k_jac_de", "achieved": achieved, "peak": HBM_PEAK_GBS,
@@ -308,23 +390,26 @@ def main():
                     "traffic": measured_traffic() if ternary500 else None,
                     "traffic_source": ("profiles/k_jac_de_traffic_eam_ternary_500cfg.md (rocprofv3 "
                                        "FETCH_SIZE + WRITE_SIZE, copy-calibrated)") if ternary500 else None,
-                    "bytes_per_launch": b_main, "ms_per_launch": t_main}
+                    "bytes_per_launch": b_main, "intermediate_bytes_per_launch": b_inter,
+                    "ms_per_launch": t_main,
+                    "limiter": measured_limiter() if ternary500 else None}
     elif kind == "adp":
         t_main = g.time_step(4, 10)
         t_quad = g.time_step(6, 10)
         kernels = {"residual": t_res, "jacobian": t_jac, "jac_de_and_moment_passes": t_main,
                    "adp_quadrupole_pass": t_quad, "normal_equations": t_ne}
-        # algorithmic bytes of the quadrupole-column force pass (the dominant
-        # kernel) per launch: pair records (20 B/pair), the lambda rows of
-        # the atoms (48 B), their d lambda rows over the quadrupole columns
-        # (24 B per atom and column) and the FP32 force rows it writes
-        # (3N x D_w x 4 B)
+        # compulsory bytes of the quadrupole-column force pass (the dominant
+        # kernel) per launch: pair records (16 B/pair) and the FP32 force rows
+        # it writes (3N x D_w x 4 B); the lambda rows (48 B per atom) and the
+        # d lambda rows (24 B per atom and column) it gathers are intermediates
         D_w = sum(free[2 * m.npair + 2 * m.ntypes:])
-        b_main = 20 * P + 48 * N + 24 * N * D_w + 12 * N * D_w
+        b_main = 16 * P + 12 * N * D_w
+        b_inter = 8 * P + 48 * N + 24 * N * D_w
         achieved = b_main / (t_quad * 1e-3) / 1e9
         roofline = {"bound": "hbm", "kernel": "k_adp_dq (quadrupole columns)", "achieved": achieved,
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                    "traffic": None, "bytes_per_launch": b_main, "ms_per_launch": t_quad}
+                    "traffic": None, "bytes_per_launch": b_main, "intermediate_bytes_per_launch": b_inter,
+                    "ms_per_launch": t_quad}
     else:
         kernels = {"residual": t_res, "jacobian": t_jac, "normal_equations": t_ne}
     b_alg = 16 * P + 28 * N + 56 * C + 8 * R + 4 * R * D
@@ -353,10 +438,14 @@ def main():
         "step_hbm_frac": b_alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
     }
     if not a.no_cpu_baseline:
-        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
-        log(f"CPU baseline on {threads} threads")
-        out["cpu_baseline"] = cpu_baseline(w, x, a.cpu_seconds, threads)
-        out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
+        host = host_cpu()
+        threads = a.cpu_threads or baseline_threads(host)
+        log(f"CPU baseline on {threads} threads ({host['cpu_model']}, nproc {host['nproc']}, "
+            f"affinity {host['affinity']})")
+        cb = cpu_baseline(w, x, a.cpu_seconds, threads, host)
+        out["cpu_baseline"] = cb
+        out["speedup_vs_cpu"] = value / cb["value"]
+        out["speedup_vs_cpu_analytic"] = value / cb["analytic_jacobian"]["value"]
     print(json.dumps(out), flush=True)
 
 
