@@ -121,6 +121,13 @@ void launch_slopes(Context &c);
 void launch_scatter_params(Context &c, const double *d_x);
 void launch_eval(Context &c, bool want_forces);
 void launch_cost(Context &c, double *d_out);
+// sum of squares of v[n] into d_out[0] through part[kSumsqBlocks] block
+// partials (every partial written, fixed order)
+constexpr int kSumsqBlocks = 512;
+void launch_sumsq(hipStream_t st, int64_t n, const double *v, double *part, double *d_out);
+// the no-allocation scratch of the normal equations: per-device address of
+// the static device array, looked up once per device by fs_ctx_create
+void noalloc_scratch_init(int device);
 void launch_smooth_residual(Context &c);
 void launch_smooth_jacobian(Context &c);
 void launch_jacobian(Context &c);

@@ -1162,13 +1162,16 @@ bool population_cost_batched(Context &c, const double *X, int np, double *cost) 
   return true;
 }
 
+void launch_sumsq(hipStream_t st, int64_t n, const double *v, double *part, double *d_out) {
+  hipLaunchKernelGGL(k_sumsq, dim3(kSumsqBlocks), dim3(256), 0, st, n, v, part);
+  FS_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, st, kSumsqBlocks, part, d_out);
+  FS_HIP(hipGetLastError());
+}
+
 void launch_cost(Context &c, double *d_out) {
-  const int nb = 512;
-  c.cost_part.alloc(nb);
-  hipLaunchKernelGGL(k_sumsq, dim3(nb), dim3(256), 0, c.stream, c.R, c.residual.p, c.cost_part.p);
-  FS_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, c.stream, nb, c.cost_part.p, d_out);
-  FS_HIP(hipGetLastError());
+  c.cost_part.alloc(kSumsqBlocks);
+  launch_sumsq(c.stream, c.R, c.residual.p, c.cost_part.p, d_out);
 }
 
 } // namespace fsa

@@ -403,22 +403,36 @@ __global__ __launch_bounds__(1024) void k_chol_solve(int D, const double *__rest
 
 } // namespace
 
-void launch_normal_equations(Context &c) {
-  const int D = c.D;
-  const int64_t R = c.R;
-  c.JtJ.alloc(size_t(std::max(D, 1)) * std::max(D, 1));
-  c.Jtr.alloc(std::max(D, 1));
-  if (D == 0) return;
+// the partial sums of one normal-equation evaluation: nchunk chunk partials
+// and kNeGroups group partials of per = ntp 256 + D doubles each
+struct NeShape {
+  int nt, nchunk;
+  int64_t per;
+};
+
+// one chunk of rows per CU (at least 64 rows per chunk): one 8-wave block
+// fits a CU at a time; 1, 2, 3 and 4 chunks per CU measured 1.15, 1.18, 1.21
+// and 1.23 ms at 240 columns. max_chunks caps the count (a fixed-size
+// scratch); the chunking fixes the summation order, so equal chunk counts
+// give bit-identical products.
+NeShape ne_shape(int cu_count, int64_t R, int D, int max_chunks) {
   FS_REQUIRE(D <= 16 * kMT, FS_ERR_UNSUPPORTED, "normal equations support at most 256 parameters");
-  const int nt = (D + kMT - 1) / kMT;
-  const int ntp = nt * (nt + 1) / 2;
-  // one chunk of rows per CU (at least 64 rows per chunk): one 8-wave block
-  // fits a CU at a time; 1, 2, 3 and 4 chunks per CU measured 1.15, 1.18,
-  // 1.21 and 1.23 ms at 240 columns
-  const int nchunk = int(std::max<int64_t>(1, std::min<int64_t>(c.cu_count, (R + 63) / 64)));
-  const int64_t per = int64_t(ntp) * 256 + D;
-  c.jtj_part.alloc(size_t(nchunk + kNeGroups) * per);
-  double *part2 = c.jtj_part.p + size_t(nchunk) * per;
+  NeShape sh;
+  sh.nt = (D + kMT - 1) / kMT;
+  sh.per = int64_t(sh.nt * (sh.nt + 1) / 2) * 256 + D;
+  sh.nchunk = int(std::max<int64_t>(1, std::min<int64_t>({int64_t(cu_count), (R + 63) / 64,
+                                                          int64_t(max_chunks)})));
+  return sh;
+}
+
+// JtJ[D x D] = J^T J, Jtr[D] = J^T F of J (FP32, R x D row-major) and F
+// (FP64) through part[(nchunk + kNeGroups) * per]; every partial is written
+// before it is read, so nothing needs clearing
+void ne_launch(hipStream_t st, const NeShape &sh, const float *J, const double *F, int64_t R, int D,
+               double *part, double *JtJ, double *Jtr) {
+  const int nt = sh.nt, nchunk = sh.nchunk;
+  const int64_t per = sh.per;
+  double *part2 = part + size_t(nchunk) * per;
   // two slab buffers; a thread holds at most kNePre floats of a slab in
   // flight (kNeRows rows of D floats over 64 (nt + 1) / 2 threads: D <= 16 nt)
   const size_t lds = 2 * size_t(kNeRows) * ne_pitch(D) * sizeof(float);
@@ -427,8 +441,7 @@ void launch_normal_equations(Context &c) {
              "normal-equation slab does not fit the per-thread staging");
   auto go = [&](auto ntc) {
     constexpr int NT = decltype(ntc)::value;
-    hipLaunchKernelGGL((k_ne_mfma<NT>), dim3(nchunk), dim3(nthr), lds, c.stream, c.J.p,
-                       c.residual.p, R, D, nchunk, c.jtj_part.p);
+    hipLaunchKernelGGL((k_ne_mfma<NT>), dim3(nchunk), dim3(nthr), lds, st, J, F, R, D, nchunk, part);
   };
   switch (nt) {
   case 1: go(std::integral_constant<int, 1>{}); break;
@@ -449,12 +462,22 @@ void launch_normal_equations(Context &c) {
   default: go(std::integral_constant<int, 16>{}); break;
   }
   FS_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_ne_reduce1, dim3(unsigned((per + 255) / 256), kNeGroups), dim3(256), 0,
-                     c.stream, per, nchunk, c.jtj_part.p, part2);
+  hipLaunchKernelGGL(k_ne_reduce1, dim3(unsigned((per + 255) / 256), kNeGroups), dim3(256), 0, st, per,
+                     nchunk, part, part2);
   FS_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_ne_reduce, dim3(unsigned((per + 255) / 256)), dim3(256), 0, c.stream, D, nt,
-                     kNeGroups, part2, c.JtJ.p, c.Jtr.p);
+  hipLaunchKernelGGL(k_ne_reduce, dim3(unsigned((per + 255) / 256)), dim3(256), 0, st, D, nt,
+                     kNeGroups, part2, JtJ, Jtr);
   FS_HIP(hipGetLastError());
+}
+
+void launch_normal_equations(Context &c) {
+  const int D = c.D;
+  c.JtJ.alloc(size_t(std::max(D, 1)) * std::max(D, 1));
+  c.Jtr.alloc(std::max(D, 1));
+  if (D == 0) return;
+  const NeShape sh = ne_shape(c.cu_count, c.R, D, 1 << 30);
+  c.jtj_part.alloc(size_t(sh.nchunk + kNeGroups) * sh.per);
+  ne_launch(c.stream, sh, c.J.p, c.residual.p, c.R, D, c.jtj_part.p, c.JtJ.p, c.Jtr.p);
 }
 
 // scal layout: [0] max diag, [1] info, [2] predicted reduction
