@@ -243,6 +243,8 @@ int fs_lm_solve_impl(Context &c, const fs_lm_opts &o, double *x, int *status, in
 int fs_ne_solve_impl(Context &c, double ridge, double *x);
 int fs_lm_step_impl(Context &c, double lambda, double *dx);
 int fs_population_impl(Context &c, const double *X, int np, double *cost);
+int fs_ne_noalloc_impl(Context &c, const float *J, const double *F, int64_t R, int D, double *JtJ,
+                       double *Jtr, double *cost);
 int fs_de_impl(Context &c, const fs_de_opts *o, const double *lo, const double *hi, double *x,
                double *best, double *hist, int *gens);
 double time_step_impl(Context &c, int what, int reps);
@@ -288,6 +290,7 @@ int fs_ctx_create(const fs_opts *opts, fs_ctx **out) {
     int cus = 0;
     FS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
     c->cu_count = cus > 0 ? cus : 256;
+    noalloc_scratch_init(c->device);
     *out = reinterpret_cast<fs_ctx *>(c);
     return FS_OK;
   } catch (const Error &e) {
@@ -744,6 +747,14 @@ int fs_eval_jacobian(fs_ctx *ctx, float *J) {
 int fs_normal_equations(fs_ctx *ctx, double *JtJ, double *Jtr, double *cost) {
   int rc = FS_OK;
   const int g = guarded(ctx, [&](Context &c) { rc = fs_normal_impl(c, JtJ, Jtr, cost); });
+  return g != FS_OK ? g : rc;
+}
+
+int fs_normal_equations_noalloc(fs_ctx *ctx, const float *J, const double *F, int64_t R,
+                                int32_t D, double *JtJ, double *Jtr, double *cost) {
+  int rc = FS_OK;
+  const int g =
+      guarded(ctx, [&](Context &c) { rc = fs_ne_noalloc_impl(c, J, F, R, D, JtJ, Jtr, cost); });
   return g != FS_OK ? g : rc;
 }
 

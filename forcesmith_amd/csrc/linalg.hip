@@ -11,6 +11,7 @@
 #include "common.hpp"
 
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 
 namespace fsa {
@@ -468,6 +469,72 @@ void ne_launch(hipStream_t st, const NeShape &sh, const float *J, const double *
   hipLaunchKernelGGL(k_ne_reduce, dim3(unsigned((per + 255) / 256)), dim3(256), 0, st, D, nt,
                      kNeGroups, part2, JtJ, Jtr);
   FS_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Normal equations that allocate nothing (fs_normal_equations_noalloc): the
+// chunk and group partials and the cost partials live in one file-scope
+// device array. Every device holds its own copy of it at its own address,
+// looked up once per device, with that device current, by fs_ctx_create
+// (noalloc_scratch_init); host code uses only those addresses, never the
+// array's name (in host code that name is a host-side stand-in).
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20; // 64 MiB per device
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[kNoAllocScratchBytes / sizeof(double)];
+
+namespace {
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+double *scratch_on(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return device >= 0 && device < kMaxDevices ? g_scratch_ptr[device] : nullptr;
+}
+
+// a device pointer of this device (not host memory, not another device's)
+void require_device_ptr(const void *p, int device, const char *what) {
+  hipPointerAttribute_t at{};
+  const hipError_t e = hipPointerGetAttributes(&at, p);
+  if (e != hipSuccess) (void)hipGetLastError(); // host memory unknown to HIP
+  FS_REQUIRE(e == hipSuccess && at.device == device &&
+                 (at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeManaged),
+             FS_ERR_ARG, std::string(what) + " must be device memory of the context's device");
+}
+} // namespace
+
+void noalloc_scratch_init(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  FS_REQUIRE(device >= 0 && device < kMaxDevices, FS_ERR_UNSUPPORTED, "device ordinal above 63");
+  if (g_scratch_ptr[device]) return;
+  void *p = nullptr;
+  FS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  g_scratch_ptr[device] = static_cast<double *>(p);
+}
+
+int fs_ne_noalloc_impl(Context &c, const float *J, const double *F, int64_t R, int D, double *JtJ,
+                       double *Jtr, double *cost) {
+  FS_REQUIRE(J && F && JtJ && Jtr, FS_ERR_ARG, "null normal-equation argument");
+  FS_REQUIRE(R >= 1 && D >= 1, FS_ERR_ARG, "R and D must be positive");
+  FS_REQUIRE(D <= 16 * kMT, FS_ERR_UNSUPPORTED,
+             "normal equations without allocation support at most 256 parameters");
+  double *base = scratch_on(c.device);
+  FS_REQUIRE(base, FS_ERR_STATE, "no-allocation scratch not set up on this device");
+  require_device_ptr(J, c.device, "J");
+  require_device_ptr(F, c.device, "F");
+  require_device_ptr(JtJ, c.device, "JtJ");
+  require_device_ptr(Jtr, c.device, "Jtr");
+  if (cost) require_device_ptr(cost, c.device, "cost");
+  // cost partials first, then the chunk and group partials: as many chunks as
+  // the array holds, up to the allocating entry point's one per CU
+  const int64_t avail = int64_t(kNoAllocScratchBytes / sizeof(double)) - kSumsqBlocks;
+  const int nt = (D + kMT - 1) / kMT;
+  const int64_t per = int64_t(nt * (nt + 1) / 2) * 256 + D;
+  const int64_t max_chunks = avail / per - kNeGroups;
+  FS_REQUIRE(max_chunks >= 1, FS_ERR_UNSUPPORTED, "normal-equation partials exceed the scratch");
+  const NeShape sh = ne_shape(c.cu_count, R, D, int(std::min<int64_t>(max_chunks, 1 << 30)));
+  ne_launch(c.stream, sh, J, F, R, D, base + kSumsqBlocks, JtJ, Jtr);
+  if (cost) launch_sumsq(c.stream, R, F, base, cost);
+  return FS_OK;
 }
 
 void launch_normal_equations(Context &c) {

@@ -146,6 +146,22 @@ int fs_eval_jacobian(fs_ctx *ctx, float *J);
 /* FP64 normal equations of the last residual/Jacobian: JtJ[D*D], Jtr[D]. */
 int fs_normal_equations(fs_ctx *ctx, double *JtJ, double *Jtr, double *cost);
 
+/* FP64 normal equations of caller-supplied device arrays, allocating
+ * nothing. No counterpart in the reference (its NormalEquationsSolver forms
+ * J^T J on the CPU, solver.cpp:242-266); it is for callers that cannot
+ * allocate device memory: code under HIP graph capture, tight serving loops.
+ * J: float[R * D] row-major, F: double[R]; out: JtJ double[D * D], Jtr
+ * double[D], cost double[1] (may be NULL: F . F); every pointer is device
+ * memory of the context's device. The work is queued on the context's stream
+ * (fs_synchronize waits for it); the products equal fs_normal_equations'
+ * for the same J and F. Partial sums live in one static device array of
+ * 64 MiB per device, set up by fs_ctx_create: D <= 256 (else
+ * FS_ERR_UNSUPPORTED). Calls on one device share that array, so calls that
+ * overlap on one device (other contexts or host threads) race. */
+int fs_normal_equations_noalloc(fs_ctx *ctx, const float *J, const double *F,
+                                int64_t R, int32_t D, double *JtJ, double *Jtr,
+                                double *cost);
+
 /* One damped Gauss-Newton step at the current parameters, for callers that
  * drive their own LM loop: dx = -(J^T J + lambda I)^-1 J^T F (parameters
  * unchanged; lambda >= 0). No reference counterpart: the reference's LM keeps
