@@ -356,10 +356,24 @@ def main():
     for _ in range(a.steps):
         step()
     g.synchronize()
+    ms_eager = 1e3 * (time.perf_counter() - t0) / a.steps
+    log(f"eager step {ms_eager:.3f} ms")
+
+    # the same step replayed from its captured hipGraph (fs_eval_step): the
+    # parameter vector is copied to a device buffer the graph reads, then
+    # scatter, slopes, residual, Jacobian and cost run as one graph launch
+    g.step(x)  # eager warm-up and capture
+    for _ in range(a.warmup):
+        g.step(x, want_cost=False)
+    g.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        g.step(x, want_cost=False)
+    g.synchronize()
     dt = time.perf_counter() - t0
     ms = 1e3 * dt / a.steps
     value = C * D * a.steps / dt
-    log(f"step {ms:.3f} ms -> {value:.4g} configs*params/s")
+    log(f"captured step {ms:.3f} ms -> {value:.4g} configs*params/s")
 
     # kernel times with HIP events on the context stream
     t_res = g.time_step(0, 10)
@@ -434,6 +448,9 @@ def main():
                    "jacobian_blocks_per_cu": shape["jacobian_blocks_per_cu"]},
         "roofline": roofline,
         "kernels_ms": kernels,
+        "eager_ms_per_step": ms_eager,
+        "step": "fs_eval_step: parameter copy + one hipGraph launch (scatter, slopes, residual, "
+                "Jacobian, cost)",
         "step_bytes_alg": b_alg,
         "step_hbm_frac": b_alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
     }

@@ -36,7 +36,7 @@ EXPORTS = (
     "fs_upload_dataset", "fs_set_tabulated_model", "fs_set_sw_model", "fs_set_weights", "fs_set_smooth_weight", "fs_smoothness_count", "fs_write_smoothness",
     "fs_param_count", "fs_gather_params", "fs_scatter_params", "fs_gather_bounds", "fs_max_cutoff",
     "fs_residual_count", "fs_row_offsets", "fs_prepare", "fs_neighbor_count", "fs_get_neighbor_list",
-    "fs_eval_forces", "fs_eval_config", "fs_eval_residual", "fs_eval_jacobian", "fs_normal_equations", "fs_normal_equations_noalloc",
+    "fs_eval_forces", "fs_eval_config", "fs_eval_residual", "fs_eval_jacobian", "fs_eval_step", "fs_normal_equations", "fs_normal_equations_noalloc",
     "fs_lm_minimize", "fs_lm_solve", "fs_normal_equations_solve", "fs_lm_step", "fs_population_cost", "fs_de_minimize", "fs_de_last_stats", "fs_device_buffer", "fs_synchronize",
     "fs_set_launch", "fs_set_jacobian_tile", "fs_get_launch", "fs_autotune", "fs_time_step", "fs_mlp_forward", "fs_mlp_loss_grad",
 )
@@ -116,6 +116,7 @@ def load_library(path: str = LIB_PATH):
         "fs_eval_config": (ct.c_int, [vp, ct.c_int32, _dp, _dp, _dp, _dp]),
         "fs_eval_residual": (ct.c_int, [vp, _dp, _dp]),
         "fs_eval_jacobian": (ct.c_int, [vp, _fp]),
+        "fs_eval_step": (ct.c_int, [vp, _dp, _dp, _fp, _dp]),
         "fs_normal_equations": (ct.c_int, [vp, _dp, _dp, _dp]),
         "fs_normal_equations_noalloc": (ct.c_int, [vp, vp, vp, ct.c_int64, ct.c_int32, vp, vp, vp]),
         "fs_lm_minimize": (ct.c_int, [vp, _dp, ct.c_int32, _dp, _i32p]),
@@ -330,6 +331,18 @@ class GpuCalculator:
         J = np.zeros((self.residual_count, self.param_count), dtype=np.float32)
         self._check(self._lib.fs_eval_jacobian(self._h, _p(J, _fp)))
         return J
+
+    def step(self, x, want_residual=False, want_jacobian=False, want_cost=True):
+        """One residual + Jacobian step at x, replayed from a captured graph
+        after the first call: cost (None with want_cost=False: the step is
+        then only queued), and F / J when asked for (else None)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        F = np.zeros(self.residual_count) if want_residual else None
+        J = np.zeros((self.residual_count, self.param_count), np.float32) if want_jacobian else None
+        c = ct.c_double()
+        self._check(self._lib.fs_eval_step(self._h, _p(x, _dp), _p(F, _dp), _p(J, _fp),
+                                           ct.byref(c) if want_cost else None))
+        return (c.value if want_cost else None), F, J
 
     def normal_equations(self):
         D = self.param_count

@@ -26,6 +26,7 @@ COMMON = [
 PER_FILE = {
     "context.cpp": ["-ffp-contract=off", "-x", "hip"],
     "solver.cpp": ["-x", "hip"],
+    "step.cpp": ["-x", "hip"],
     "de.cpp": ["-ffp-contract=off", "-x", "hip"],
     "nl.hip": ["-ffp-contract=off"],
 }

@@ -118,6 +118,8 @@ struct SurrogateState;
 void launch_neighbor_build(Context &c);
 void launch_pair_flags(Context &c);
 void launch_slopes(Context &c);
+// the free knots from the device vector d_x, then the slopes (one launch)
+void launch_scatter_slopes(Context &c, const double *d_x);
 void launch_scatter_params(Context &c, const double *d_x);
 void launch_eval(Context &c, bool want_forces);
 void launch_cost(Context &c, double *d_out);
@@ -278,6 +280,12 @@ struct Context {
   std::vector<int32_t> h_pop_grp;
   int pop_grp_tiles = -1;
   fs_de_stats de_stats{};          // of the last fs_de_minimize call
+  // the residual + Jacobian step captured as a graph (step.cpp): parameters
+  // from step_x, keyed on the buffer addresses and launch shapes it holds
+  hipGraph_t step_graph = nullptr;
+  hipGraphExec_t step_exec = nullptr;
+  std::vector<uintptr_t> step_key;
+  DBuf<double> step_x;
   Mlp *mlp = nullptr;              // owned; see mlp.hpp
   SurrogateState *surr = nullptr;  // owned; see de.cpp
   // per-kernel timing of one Jacobian stage (HIP events, context stream):

@@ -13,8 +13,10 @@
 namespace fsa {
 
 void destroy_surrogate(SurrogateState *s);
+void destroy_step_graph(Context &c);
 
 Context::~Context() {
+  destroy_step_graph(*this);
   delete mlp;
   destroy_surrogate(surr);
   if (jev0) (void)hipEventDestroy(jev0);
@@ -243,6 +245,7 @@ int fs_lm_solve_impl(Context &c, const fs_lm_opts &o, double *x, int *status, in
 int fs_ne_solve_impl(Context &c, double ridge, double *x);
 int fs_lm_step_impl(Context &c, double lambda, double *dx);
 int fs_population_impl(Context &c, const double *X, int np, double *cost);
+int fs_step_impl(Context &c, const double *x, double *F, float *J, double *cost);
 int fs_ne_noalloc_impl(Context &c, const float *J, const double *F, int64_t R, int D, double *JtJ,
                        double *Jtr, double *cost);
 int fs_de_impl(Context &c, const fs_de_opts *o, const double *lo, const double *hi, double *x,
@@ -741,6 +744,12 @@ int fs_eval_residual(fs_ctx *ctx, double *F, double *cost) {
 int fs_eval_jacobian(fs_ctx *ctx, float *J) {
   int rc = FS_OK;
   const int g = guarded(ctx, [&](Context &c) { rc = fs_jacobian_impl(c, J); });
+  return g != FS_OK ? g : rc;
+}
+
+int fs_eval_step(fs_ctx *ctx, const double *x, double *F, float *J, double *cost) {
+  int rc = FS_OK;
+  const int g = guarded(ctx, [&](Context &c) { rc = fs_step_impl(c, x, F, J, cost); });
   return g != FS_OK ? g : rc;
 }
 

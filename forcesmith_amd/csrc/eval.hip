@@ -142,6 +142,21 @@ __global__ void k_slopes(int ntab, int nknots, const TableDesc *__restrict__ tab
   tables_block(ntab, tabs, nknots, x, y, s, S, ivl, p64);
 }
 
+// the free knots from a device parameter vector, then every table's slopes
+// and interval forms (the parameter scatter of the captured step folded into
+// the slopes launch; one block, so the knot writes are visible after the
+// barrier)
+__global__ void k_scatter_slopes(int D, const double *__restrict__ xv,
+                                 const int32_t *__restrict__ knot_of, int ntab, int nknots,
+                                 const TableDesc *__restrict__ tabs, const double *__restrict__ x,
+                                 double *__restrict__ y, double *__restrict__ s,
+                                 double *__restrict__ S, float4 *__restrict__ ivl,
+                                 double *__restrict__ p64) {
+  for (int c = threadIdx.x; c < D; c += blockDim.x) y[knot_of[c]] = xv[c];
+  __syncthreads();
+  tables_block(ntab, tabs, nknots, x, y, s, S, ivl, p64);
+}
+
 __global__ void k_scatter(int D, const double *__restrict__ xv, const int32_t *__restrict__ knot_of,
                           double *__restrict__ y) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1004,6 +1019,14 @@ void launch_scatter_params(Context &c, const double *d_x) {
   if (c.D > 0)
     hipLaunchKernelGGL(k_scatter, dim3((c.D + 255) / 256), dim3(256), 0, c.stream, c.D, d_x,
                        c.knot_of.p, c.tab_y.p);
+  FS_HIP(hipGetLastError());
+}
+
+void launch_scatter_slopes(Context &c, const double *d_x) {
+  if (c.kind == FS_SW || c.ntab == 0) return;
+  hipLaunchKernelGGL(k_scatter_slopes, dim3(1), dim3(256), 0, c.stream, c.D, d_x, c.knot_of.p, c.ntab,
+                     c.nknots, c.tabs.p, c.tab_x.p, c.tab_y.p, c.tab_s.p, c.tab_S.p, c.ivl32.p,
+                     c.ivl64.p);
   FS_HIP(hipGetLastError());
 }
 

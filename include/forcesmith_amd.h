@@ -143,6 +143,17 @@ int fs_eval_residual(fs_ctx *ctx, double *F, double *cost);
  * 37-49) and eval_cached_jacobian (force_calculator.hpp:77-78). J may be
  * NULL (device only). */
 int fs_eval_jacobian(fs_ctx *ctx, float *J);
+/* One residual + Jacobian step at the parameter vector x[D]: the
+ * functor's operator() and df at one x (forcesmith_functor.cpp:221-292)
+ * in one call. The first call at a layout runs eagerly and captures the step
+ * (parameter scatter, slopes, residual, Jacobian, cost) as a hipGraph that
+ * reads x from a device buffer; later calls copy x there and replay the
+ * graph (Stillinger-Weber models run eagerly). F[R], J[R*D] (FP32) and cost
+ * may each be NULL (device only); the device buffers hold the step's results
+ * as after fs_eval_residual and fs_eval_jacobian. With all three NULL the
+ * step stays queued on the context stream (fs_synchronize waits for it). */
+int fs_eval_step(fs_ctx *ctx, const double *x, double *F, float *J, double *cost);
+
 /* FP64 normal equations of the last residual/Jacobian: JtJ[D*D], Jtr[D]. */
 int fs_normal_equations(fs_ctx *ctx, double *JtJ, double *Jtr, double *cost);
 
