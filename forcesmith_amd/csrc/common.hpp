@@ -110,6 +110,11 @@ struct CfgGrid {
 
 constexpr int kMaxTypes = 8;
 
+// Knot tables are staged in LDS while they fit this budget per block; larger
+// tables (hundreds of knots, as the reference's model writer resamples to)
+// are read from global memory, where they stay L2-resident.
+constexpr size_t kTableStageBytes = 48 * 1024;
+
 struct Context;
 struct Mlp;
 struct SurrogateState;
@@ -145,6 +150,10 @@ void launch_sw_rev(Context &c);
 void launch_sw_eval(Context &c);
 // per-pair SW scalars {v2, v2', h, h', dh/dgamma, dh'/dgamma, dh/da2, dh'/da2}
 void launch_sw_pair_terms(Context &c);
+
+struct Context;
+// the residual kernels' descriptors plus both interval forms fit the budget
+bool stage_intervals(const Context &c);
 
 struct Context {
   std::mutex lock;
@@ -193,6 +202,7 @@ struct Context {
   int D = 0, nknots = 0, nivl = 0;
   DBuf<TableDesc> tabs;
   DBuf<double> tab_x, tab_y, tab_s, tab_S; // S: n*5 band of ds/dy per table
+  DBuf<float> tab_S32; // FP32 copy of tab_S for Jacobian passes that do not stage it
   DBuf<float4> ivl32; // per interval k: power form {a, b, c, d} as FP32 hi at 2k, FP32 lo at 2k+1
   DBuf<double> ivl64; // per interval k: the same power form in FP64 at 4k .. 4k+3
   DBuf<int32_t> col_of, knot_of;
@@ -263,6 +273,8 @@ struct Context {
   struct JacLayout {
     bool valid = false;
     int Dd = 0, dens_c0 = 0, gi_nf = 0, NSP = 0, NSD = 0, NSU = 0, NSW = 0, Ddip = 0, Dquad = 0;
+    // slots of each kind over all launches (a multiple of the per-launch width)
+    int TSP = 0, TSD = 0, TSU = 0, TSW = 0;
   } jl;
   DBuf<float4> jrec; // pair records sorted by (partner type, r) per atom
   DBuf<double> jr;   // their FP64 distances
@@ -272,6 +284,8 @@ struct Context {
   DBuf<double> sw_own, sw_ptan, sw_etan;
   DBuf<float> adp_tan; // ADP Jacobian tangents [N][10][D]
   DBuf<double> lm_M, lm_dx, lm_scal;
+  DBuf<float> ne_W;   // normal equations of > 256 columns: two column groups of J
+  DBuf<double> ne_M;  // and their product
   // batched population cost: member tables, embedding derivatives, partials
   DBuf<double> pop_X, pop_y, pop_s, pop_S, pop_i64, pop_Fp, pop_partE, pop_partF, pop_cmc, pop_cost;
   DBuf<float4> pop_i32;

@@ -88,6 +88,16 @@ void build_tiles(Context &c) {
   c.tile_part.alloc(size_t(std::max(c.ntiles, 1)) * 11);
 }
 
+} // namespace
+
+bool stage_intervals(const Context &c) {
+  return size_t(c.ntab) * sizeof(TableDesc) + 2 * size_t(c.nivl) * sizeof(float4) +
+             4 * size_t(c.nivl) * sizeof(double) <=
+         kTableStageBytes;
+}
+
+namespace {
+
 // curvature rows of the tabulated model: n - 2 per table with a free knot and
 // n >= 3, tables in declaration order (SplinePotential::curvature_count,
 // spline.hpp:104-106; model_smoothness_count, smoothness.hpp:17-23), only when
@@ -432,9 +442,6 @@ int fs_set_tabulated_model(fs_ctx *ctx, int32_t kind, int32_t ntypes, int32_t nt
     }
     c.nknots = off;
     c.nivl = ivl;
-    FS_REQUIRE(size_t(ntab) * sizeof(TableDesc) + 2 * size_t(ivl) * sizeof(float4) +
-                       4 * size_t(ivl) * sizeof(double) <= 48 * 1024,
-               FS_ERR_UNSUPPORTED, "knot tables exceed the 48 KiB LDS staging budget");
     c.h_col_of.assign(off, -1);
     c.h_knot_of.clear();
     for (int k = 0; k < off; ++k)

@@ -178,6 +178,7 @@ struct EvalArgs {
   const double *r64;
   const TableDesc *tabs;
   int ntab, nivl;
+  int stage_ivl; // interval forms staged in LDS (else read from global memory)
   const float4 *ivl32;
   const double *ivl64;
   const double *tx, *ty, *ts;
@@ -196,18 +197,26 @@ template <int LPA, class T> __device__ inline T sub_sum(T v) {
   return v;
 }
 
-// descriptors, FP32 hi/lo interval forms and FP64 interval forms into LDS;
-// returns the LDS copy of the FP64 forms (placed after the FP32 ones)
-__device__ inline const double *stage_tables(const EvalArgs &a, TableDesc *sT, float4 *sI) {
+// descriptors into LDS, and the FP32 hi/lo and FP64 interval forms after
+// them when they fit the staging budget (a.stage_ivl); otherwise the lookups
+// read the interval forms from global memory (a few KB per table, L2-resident)
+__device__ inline void stage_tables(const EvalArgs &a, TableDesc *sT, float4 *smem_ivl,
+                                    const float4 *&sI, const double *&sP) {
   const float4 *src = reinterpret_cast<const float4 *>(a.tabs);
   float4 *dst = reinterpret_cast<float4 *>(sT);
   const int nq = a.ntab * int(sizeof(TableDesc) / sizeof(float4));
   for (int k = threadIdx.x; k < nq; k += blockDim.x) dst[k] = src[k];
-  for (int k = threadIdx.x; k < 2 * a.nivl; k += blockDim.x) sI[k] = a.ivl32[k];
-  double *sP = reinterpret_cast<double *>(sI + 2 * a.nivl);
-  for (int k = threadIdx.x; k < 4 * a.nivl; k += blockDim.x) sP[k] = a.ivl64[k];
+  if (a.stage_ivl) {
+    for (int k = threadIdx.x; k < 2 * a.nivl; k += blockDim.x) smem_ivl[k] = a.ivl32[k];
+    double *p64 = reinterpret_cast<double *>(smem_ivl + 2 * a.nivl);
+    for (int k = threadIdx.x; k < 4 * a.nivl; k += blockDim.x) p64[k] = a.ivl64[k];
+    sI = smem_ivl;
+    sP = p64;
+  } else {
+    sI = a.ivl32;
+    sP = a.ivl64;
+  }
   __syncthreads();
-  return sP;
 }
 
 // table lookups -------------------------------------------------------------
@@ -329,8 +338,9 @@ __global__ __launch_bounds__(256) void k_density(EvalArgs a) {
   // evaluated in FP64 in both precision modes, from the FP64 distance.
   extern __shared__ float4 smem[];
   TableDesc *sT = reinterpret_cast<TableDesc *>(smem);
-  float4 *sI = smem + a.ntab * int(sizeof(TableDesc) / sizeof(float4));
-  const double *sP = stage_tables(a, sT, sI);
+  const float4 *sI;
+  const double *sP;
+  stage_tables(a, sT, smem + a.ntab * int(sizeof(TableDesc) / sizeof(float4)), sI, sP);
   const Tabs tb{sT, sI, a.tx, a.ty, a.ts, sP};
   const int sub = threadIdx.x / LPA, lane = threadIdx.x % LPA;
   for (int tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
@@ -545,9 +555,10 @@ __global__ __launch_bounds__(256) void k_force(EvalArgs a) {
   constexpr int APB = 256 / LPA;
   extern __shared__ float4 smem[];
   TableDesc *sT = reinterpret_cast<TableDesc *>(smem);
-  float4 *sI = smem + a.ntab * int(sizeof(TableDesc) / sizeof(float4));
   __shared__ double sRed[APB][kPart];
-  const double *sP = stage_tables(a, sT, sI);
+  const float4 *sI;
+  const double *sP;
+  stage_tables(a, sT, smem + a.ntab * int(sizeof(TableDesc) / sizeof(float4)), sI, sP);
   const Tabs tb{sT, sI, a.tx, a.ty, a.ts, sP};
   const int sub = threadIdx.x / LPA, lane = threadIdx.x % LPA;
   for (int tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
@@ -717,6 +728,7 @@ constexpr int kPopF = 11; // force pass: |F - F_ref|^2, pair energy, 9 virial
 struct PopArgs {
   int nb;                            // members in this batch
   int nknots, nivl;
+  int stage_ivl;                     // member interval forms staged in LDS (else global)
   const double *y, *s;               // [nb][nknots] member knots and slopes
   const float4 *ivl32;               // [nb][2 nivl] FP32 hi/lo interval forms
   const double *ivl64;               // [nb][4 nivl] FP64 interval forms
@@ -749,11 +761,15 @@ __global__ __launch_bounds__(256) void k_pop_density(EvalArgs a, PopArgs p) {
   constexpr int APB = 256 / LPA;
   extern __shared__ float4 smem[];
   TableDesc *sT = reinterpret_cast<TableDesc *>(smem);
-  double *P64 = reinterpret_cast<double *>(smem + a.ntab * int(sizeof(TableDesc) / sizeof(float4)));
   __shared__ double sRed[APB][kPopE];
   const int g = blockIdx.x % kPopG, m = blockIdx.x / kPopG;
   for (int k = threadIdx.x; k < a.ntab; k += blockDim.x) sT[k] = a.tabs[k];
-  for (int k = threadIdx.x; k < 4 * p.nivl; k += blockDim.x) P64[k] = p.ivl64[size_t(m) * 4 * p.nivl + k];
+  const double *P64 = p.ivl64 + size_t(m) * 4 * p.nivl;
+  if (p.stage_ivl) {
+    double *l64 = reinterpret_cast<double *>(smem + a.ntab * int(sizeof(TableDesc) / sizeof(float4)));
+    for (int k = threadIdx.x; k < 4 * p.nivl; k += blockDim.x) l64[k] = P64[k];
+    P64 = l64;
+  }
   __syncthreads();
   const double *ym = p.y + size_t(m) * p.nknots, *sm = p.s + size_t(m) * p.nknots;
   const int sub = threadIdx.x / LPA, lane = threadIdx.x % LPA;
@@ -819,22 +835,26 @@ __global__ __launch_bounds__(256) void k_pop_force(EvalArgs a, PopArgs p) {
   constexpr int APB = 256 / LPA;
   extern __shared__ float4 smem[];
   TableDesc *sT = reinterpret_cast<TableDesc *>(smem);
-  float4 *sI = smem + a.ntab * int(sizeof(TableDesc) / sizeof(float4));
   __shared__ double sRed[APB][kPopF];
   const int g = blockIdx.x % kPopG, m = blockIdx.x / kPopG;
+  const float4 *sI = p.ivl32 + size_t(m) * 2 * p.nivl;
+  const double *sP = p.ivl64 + size_t(m) * 4 * p.nivl;
   {
     const float4 *src = reinterpret_cast<const float4 *>(a.tabs);
     float4 *dst = reinterpret_cast<float4 *>(sT);
     const int nq = a.ntab * int(sizeof(TableDesc) / sizeof(float4));
     for (int k = threadIdx.x; k < nq; k += blockDim.x) dst[k] = src[k];
-    const float4 *mi = p.ivl32 + size_t(m) * 2 * p.nivl;
-    for (int k = threadIdx.x; k < 2 * p.nivl; k += blockDim.x) sI[k] = mi[k];
-    double *sP = reinterpret_cast<double *>(sI + 2 * p.nivl);
-    for (int k = threadIdx.x; k < 4 * p.nivl; k += blockDim.x) sP[k] = p.ivl64[size_t(m) * 4 * p.nivl + k];
+    if (p.stage_ivl) {
+      float4 *l32 = smem + nq;
+      for (int k = threadIdx.x; k < 2 * p.nivl; k += blockDim.x) l32[k] = sI[k];
+      double *l64 = reinterpret_cast<double *>(l32 + 2 * p.nivl);
+      for (int k = threadIdx.x; k < 4 * p.nivl; k += blockDim.x) l64[k] = sP[k];
+      sI = l32;
+      sP = l64;
+    }
     __syncthreads();
   }
-  const Tabs tb{sT, sI, a.tx, p.y + size_t(m) * p.nknots, p.s + size_t(m) * p.nknots,
-                reinterpret_cast<const double *>(sI + 2 * p.nivl)};
+  const Tabs tb{sT, sI, a.tx, p.y + size_t(m) * p.nknots, p.s + size_t(m) * p.nknots, sP};
   EvalArgs am = a; // the member's embedding derivatives
   am.Fp = p.Fp + size_t(m) * a.N;
   am.Fp32 = p.Fp32 + size_t(m) * a.N;
@@ -971,6 +991,7 @@ EvalArgs make_args(Context &c) {
   a.tabs = c.tabs.p;
   a.ntab = c.ntab;
   a.nivl = c.nivl;
+  a.stage_ivl = stage_intervals(c) ? 1 : 0;
   a.ivl32 = c.ivl32.p;
   a.ivl64 = c.ivl64.p;
   a.tx = c.tab_x.p;
@@ -1042,8 +1063,10 @@ void launch_eval(Context &c, bool) {
     launch_sw_eval(c);
   } else {
     const EvalArgs a = make_args(c);
-    const size_t lds = size_t(c.ntab) * sizeof(TableDesc) + 2 * size_t(c.nivl) * sizeof(float4) +
-                       4 * size_t(c.nivl) * sizeof(double);
+    const size_t lds = size_t(c.ntab) * sizeof(TableDesc) +
+                       (a.stage_ivl ? 2 * size_t(c.nivl) * sizeof(float4) +
+                                          4 * size_t(c.nivl) * sizeof(double)
+                                    : 0);
     const int grid = std::max(1, std::min(c.ntiles, c.blocks_per_cu * c.cu_count));
     if (c.ntiles > 0) {
       const bool f64 = c.precision == FS_FP64;
@@ -1116,9 +1139,11 @@ void population_launch(Context &c, const double *d_X, int np, double *d_cost) {
   const int G = kPopG;
   const int cap = population_batch_cap(c, np);
   EvalArgs a = make_args(c);
+  const bool stage = stage_intervals(c);
   const size_t lds_t = size_t(c.ntab) * sizeof(TableDesc);
-  const size_t lds_d = lds_t + 4 * size_t(c.nivl) * sizeof(double);
-  const size_t lds_f = lds_t + 2 * size_t(c.nivl) * sizeof(float4) + 4 * size_t(c.nivl) * sizeof(double);
+  const size_t lds_d = lds_t + (stage ? 4 * size_t(c.nivl) * sizeof(double) : 0);
+  const size_t lds_f =
+      lds_t + (stage ? 2 * size_t(c.nivl) * sizeof(float4) + 4 * size_t(c.nivl) * sizeof(double) : 0);
   for (int m0 = 0; m0 < np; m0 += cap) {
     const int nb = std::min(cap, np - m0);
     hipLaunchKernelGGL(k_pop_tables, dim3(nb), dim3(64), 0, c.stream, c.ntab, c.D, c.nknots,
@@ -1129,6 +1154,7 @@ void population_launch(Context &c, const double *d_X, int np, double *d_cost) {
     p.nb = nb;
     p.nknots = c.nknots;
     p.nivl = c.nivl;
+    p.stage_ivl = stage ? 1 : 0;
     p.y = c.pop_y.p;
     p.s = c.pop_s.p;
     p.ivl32 = c.pop_i32.p;

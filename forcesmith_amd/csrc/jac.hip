@@ -158,6 +158,8 @@ struct JacArgs {
   const double *jr;   // their FP64 distances (the sort key)
   const TableDesc *tabs;
   const double *tx, *ty, *ts, *tS;
+  const float *tS32;      // FP32 copy of tS, read in place when it is not staged
+  int stage_S;            // FP32 slope band staged in LDS
   const float4 *ivl32;
   const ColDesc *cols;
   const double *rho, *over, *Fp;
@@ -175,14 +177,25 @@ struct JacArgs {
   double ew, sw;
 };
 
-// LDS staging of descriptors and float S bands
-__device__ inline void stage(const JacArgs &a, TableDesc *sT, float *sS, int nk) {
+// LDS staging of the descriptors and, when it fits the staging budget, the
+// FP32 slope bands (else sS points at their FP32 copy in global memory);
+// returns the start of the per-wave staging area that follows
+__device__ inline void *stage(const JacArgs &a, float4 *smem, TableDesc *&sT, const float *&sS,
+                              int nk) {
+  sT = reinterpret_cast<TableDesc *>(smem);
   const float4 *src = reinterpret_cast<const float4 *>(a.tabs);
-  float4 *dst = reinterpret_cast<float4 *>(sT);
   const int nq = a.ntab * int(sizeof(TableDesc) / sizeof(float4));
-  for (int k = threadIdx.x; k < nq; k += blockDim.x) dst[k] = src[k];
-  for (int k = threadIdx.x; k < nk * 5; k += blockDim.x) sS[k] = float(a.tS[k]);
+  for (int k = threadIdx.x; k < nq; k += blockDim.x) smem[k] = src[k];
+  float *after = reinterpret_cast<float *>(smem + nq);
+  if (a.stage_S) {
+    for (int k = threadIdx.x; k < nk * 5; k += blockDim.x) after[k] = float(a.tS[k]);
+    sS = after;
+    after += (nk * 5 + 3) & ~3;
+  } else {
+    sS = a.tS32;
+  }
   __syncthreads();
+  return after;
 }
 
 // ---------------------------------------------------------------------------
@@ -332,12 +345,12 @@ __global__ __launch_bounds__(256) void k_emb_sites(JacArgs a) {
   if (i < a.N) (void)emb_window(a, a.tabs, i);
 }
 
-__global__ __launch_bounds__(256) void k_jac_dens(JacArgs a, int nknots) {
+// density columns [dc0, dc0 + 256) of H and Hs, lane + 64 s per lane
+__global__ __launch_bounds__(256) void k_jac_dens(JacArgs a, int nknots, int dc0) {
   extern __shared__ float4 smem[];
-  TableDesc *sT = reinterpret_cast<TableDesc *>(smem);
-  float *sS = reinterpret_cast<float *>(smem + a.ntab * int(sizeof(TableDesc) / sizeof(float4)));
-  DensStage *sAll = reinterpret_cast<DensStage *>(sS + ((nknots * 5 + 3) & ~3));
-  stage(a, sT, sS, nknots);
+  TableDesc *sT;
+  const float *sS;
+  DensStage *sAll = reinterpret_cast<DensStage *>(stage(a, smem, sT, sS, nknots));
   const int wave = uni(threadIdx.x >> 6), lane = threadIdx.x & 63;
   DensStage &P = sAll[wave];
   const int64_t gw = int64_t(blockIdx.x) * 4 + wave;
@@ -345,7 +358,7 @@ __global__ __launch_bounds__(256) void k_jac_dens(JacArgs a, int nknots) {
   // my density columns
   int ctab[kCPL], cm[kCPL];
   for (int s = 0; s < kCPL; ++s) {
-    const int dc = lane + 64 * s;
+    const int dc = dc0 + lane + 64 * s;
     if (dc < a.Dd) {
       const ColDesc cd = a.cols[a.dens_c0 + dc];
       ctab[s] = cd.tab;
@@ -393,7 +406,7 @@ __global__ __launch_bounds__(256) void k_jac_dens(JacArgs a, int nknots) {
     const float hs = clamped ? 0.0f : float(a.Fpp[i]);
 #pragma unroll
     for (int s = 0; s < kCPL; ++s) {
-      const int dc = lane + 64 * s;
+      const int dc = dc0 + lane + 64 * s;
       if (dc < a.Dd) {
         a.H[i * a.Dd + dc] = H[s];
         a.Hs[i * a.Dd + dc] = hs * H[s];
@@ -486,13 +499,13 @@ __device__ __forceinline__ void jac_main_body(const JacArgs &a, int nknots, cons
   using Stage = typename JStageOf<PASS>::T;
   constexpr bool kPairPass = PASS == JP_PAIR;
   extern __shared__ float4 smem[];
-  TableDesc *sT = reinterpret_cast<TableDesc *>(smem);
-  float *sS = reinterpret_cast<float *>(smem + a.ntab * int(sizeof(TableDesc) / sizeof(float4)));
-  Stage *sAll = reinterpret_cast<Stage *>(sS + ((nknots * 5 + 3) & ~3));
-  // the tile-end reduction reuses the staging area of the four waves
-  static_assert(4 * sizeof(Stage) >= sizeof(double) * kJP * 256, "reduction buffer");
-  double(*sPart)[256] = reinterpret_cast<double(*)[256]>(sAll);
-  stage(a, sT, sS, nknots);
+  TableDesc *sT;
+  const float *sS;
+  Stage *sAll = reinterpret_cast<Stage *>(stage(a, smem, sT, sS, nknots));
+  // the tile-end reduction reuses the staging area of the four waves: one
+  // entry per partial row and slot column (s, lane)
+  static_assert(4 * sizeof(Stage) >= sizeof(double) * kJP * 64 * NS, "reduction buffer");
+  double(*sPart)[64 * NS] = reinterpret_cast<double(*)[64 * NS]>(sAll);
   const int wave = uni(threadIdx.x >> 6), lane = threadIdx.x & 63;
   Stage &P = sAll[wave];
   int ccol[NS], ctab[NS], cm[NS], ckind[NS], cdc[NS];
@@ -791,9 +804,9 @@ __device__ __forceinline__ void jac_main_body(const JacArgs &a, int nknots, cons
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
           if (ckind[s] == CK_NONE) continue;
-          const int col = ccol[s];
+          const int lc = 64 * s + lane;
           for (int q = 0; q < kJP; ++q)
-            sPart[q][col] = (w == 0 ? 0.0 : sPart[q][col]) + double(tp[s][q]);
+            sPart[q][lc] = (w == 0 ? 0.0 : sPart[q][lc]) + double(tp[s][q]);
         }
       }
       __syncthreads();
@@ -802,8 +815,8 @@ __device__ __forceinline__ void jac_main_body(const JacArgs &a, int nknots, cons
 #pragma unroll
       for (int s = 0; s < NS; ++s) {
         if (ckind[s] == CK_NONE) continue;
-        const int col = ccol[s];
-        for (int q = 0; q < kJP; ++q) a.part[(int64_t(tile) * kJP + q) * a.D + col] = sPart[q][col];
+        const int col = ccol[s], lc = 64 * s + lane;
+        for (int q = 0; q < kJP; ++q) a.part[(int64_t(tile) * kJP + q) * a.D + col] = sPart[q][lc];
       }
     }
     __syncthreads();
@@ -930,10 +943,9 @@ __global__ __launch_bounds__(256) void k_adp_mom(JacArgs a, int nknots, const Sl
   constexpr bool kDip = PASS == JP_DIP;
   constexpr int NC = kDip ? 3 : 6;
   extern __shared__ float4 smem[];
-  TableDesc *sT = reinterpret_cast<TableDesc *>(smem);
-  float *sS = reinterpret_cast<float *>(smem + a.ntab * int(sizeof(TableDesc) / sizeof(float4)));
-  MomStage *sAll = reinterpret_cast<MomStage *>(sS + ((nknots * 5 + 3) & ~3));
-  stage(a, sT, sS, nknots);
+  TableDesc *sT;
+  const float *sS;
+  MomStage *sAll = reinterpret_cast<MomStage *>(stage(a, smem, sT, sS, nknots));
   const int wave = uni(threadIdx.x >> 6), lane = threadIdx.x & 63;
   MomStage &P = sAll[wave];
   int cm[NS], cqc[NS], cpt[NS];
@@ -1053,12 +1065,11 @@ __global__ __launch_bounds__(256) void k_adp_dq(JacArgs a, int nknots, const Slo
   constexpr bool kDip = PASS == JP_DIP;
   constexpr int NC = kDip ? 3 : 6;
   extern __shared__ float4 smem[];
-  TableDesc *sT = reinterpret_cast<TableDesc *>(smem);
-  float *sS = reinterpret_cast<float *>(smem + a.ntab * int(sizeof(TableDesc) / sizeof(float4)));
-  DQStage *sAll = reinterpret_cast<DQStage *>(sS + ((nknots * 5 + 3) & ~3));
-  static_assert(4 * sizeof(DQStage) >= sizeof(double) * kJP * 256, "reduction buffer");
-  double(*sPart)[256] = reinterpret_cast<double(*)[256]>(sAll);
-  stage(a, sT, sS, nknots);
+  TableDesc *sT;
+  const float *sS;
+  DQStage *sAll = reinterpret_cast<DQStage *>(stage(a, smem, sT, sS, nknots));
+  static_assert(4 * sizeof(DQStage) >= sizeof(double) * kJP * 64 * NS, "reduction buffer");
+  double(*sPart)[64 * NS] = reinterpret_cast<double(*)[64 * NS]>(sAll);
   const int wave = uni(threadIdx.x >> 6), lane = threadIdx.x & 63;
   DQStage &P = sAll[wave];
   int ccol[NS], cm[NS], cqc[NS], cpt[NS];
@@ -1256,8 +1267,8 @@ __global__ __launch_bounds__(256) void k_adp_dq(JacArgs a, int nknots, const Slo
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
           if (cqc[s] < 0) continue;
-          const int col = ccol[s];
-          for (int q = 0; q < kJP; ++q) sPart[q][col] = (w == 0 ? 0.0 : sPart[q][col]) + double(tp[s][q]);
+          const int lc = 64 * s + lane;
+          for (int q = 0; q < kJP; ++q) sPart[q][lc] = (w == 0 ? 0.0 : sPart[q][lc]) + double(tp[s][q]);
         }
       }
       __syncthreads();
@@ -1266,8 +1277,8 @@ __global__ __launch_bounds__(256) void k_adp_dq(JacArgs a, int nknots, const Slo
 #pragma unroll
       for (int s = 0; s < NS; ++s) {
         if (cqc[s] < 0) continue;
-        const int col = ccol[s];
-        for (int q = 0; q < kJP; ++q) a.part[(int64_t(tile) * kJP + q) * a.D + col] = sPart[q][col];
+        const int col = ccol[s], lc = 64 * s + lane;
+        for (int q = 0; q < kJP; ++q) a.part[(int64_t(tile) * kJP + q) * a.D + col] = sPart[q][lc];
       }
     }
     __syncthreads();
@@ -1617,8 +1628,16 @@ void launch_smooth_jacobian(Context &c) {
   FS_HIP(hipGetLastError());
 }
 
+// FP32 copy of the slope bands for passes that read them in place
+__global__ void k_band_f32(int64_t n, const double *__restrict__ S, float *__restrict__ S32) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e < n) S32[e] = float(S[e]);
+}
+
+// slots padded to whole launches of w slots
+inline int round_slots(int n, int w) { return w == 0 ? 0 : (n + w - 1) / w * w; }
+
 void launch_jacobian(Context &c) {
-  FS_REQUIRE(c.D <= 64 * kCPL, FS_ERR_UNSUPPORTED, "more than 256 free parameters");
   const int64_t R = c.R;
   c.J.alloc(size_t(std::max<int64_t>(R * c.D, 1)));
   // Jacobian tiles: <= 64 consecutive atoms of one configuration
@@ -1807,21 +1826,27 @@ void launch_jacobian(Context &c) {
       auto pad_to = [](std::vector<SlotDesc> &sl, int ns) {
         sl.resize(size_t(ns) * 64, SlotDesc{-1, 0, CK_NONE, -1, 0, 0, 0, 0});
       };
+      // slots per launch: 2 or 4 pair slots, 2, 4 or 6 density/embedding
+      // slots, 1, 2 or 4 dipole/quadrupole slots; wider column sets run as
+      // several launches of the widest form over consecutive slot groups
       std::vector<SlotDesc> slots = build_slots({CK_PAIR}), sde = build_slots({CK_DENS, CK_EMB});
       const int nsp = int(slots.size() / 64), nsd = int(sde.size() / 64);
-      FS_REQUIRE(nsp <= 4 && nsd <= 6, FS_ERR_UNSUPPORTED, "Jacobian columns need too many lane slots");
       NSP = nsp == 0 ? 0 : (nsp <= 2 ? 2 : 4);
       NSD = nsd == 0 ? 0 : (nsd <= 2 ? 2 : (nsd <= 4 ? 4 : 6));
-      pad_to(slots, NSP);
-      pad_to(sde, NSD);
+      const int TSP = round_slots(nsp, NSP), TSD = round_slots(nsd, NSD);
+      pad_to(slots, TSP);
+      pad_to(sde, TSD);
       slots.insert(slots.end(), sde.begin(), sde.end());
+      int TSU = 0, TSW = 0;
       if (adp) {
         std::vector<SlotDesc> su = build_slots({CK_DIP}), sw = build_slots({CK_QUAD});
         const int nsu = int(su.size() / 64), nsw = int(sw.size() / 64);
         NSU = nsu <= 1 ? nsu : (nsu <= 2 ? 2 : 4);
         NSW = nsw <= 1 ? nsw : (nsw <= 2 ? 2 : 4);
-        pad_to(su, NSU);
-        pad_to(sw, NSW);
+        TSU = round_slots(nsu, NSU);
+        TSW = round_slots(nsw, NSW);
+        pad_to(su, TSU);
+        pad_to(sw, TSW);
         slots.insert(slots.end(), su.begin(), su.end());
         slots.insert(slots.end(), sw.begin(), sw.end());
       }
@@ -1831,7 +1856,8 @@ void launch_jacobian(Context &c) {
                             hipMemcpyHostToDevice, c.stream));
       FS_HIP(hipStreamSynchronize(c.stream));
 
-      c.jl = Context::JacLayout{true, Dd, dens_c0, gi_nf, NSP, NSD, NSU, NSW, Ddip, Dquad};
+      c.jl = Context::JacLayout{true, Dd, dens_c0, gi_nf, NSP, NSD, NSU, NSW, Ddip, Dquad,
+                                TSP, TSD, TSU, TSW};
     }
     Dd = c.jl.Dd;
     dens_c0 = c.jl.dens_c0;
@@ -1842,6 +1868,7 @@ void launch_jacobian(Context &c) {
     NSW = c.jl.NSW;
     Ddip = c.jl.Ddip;
     Dquad = c.jl.Dquad;
+    const int TSP = c.jl.TSP, TSD = c.jl.TSD, TSU = c.jl.TSU, TSW = c.jl.TSW;
     a.Dd = Dd;
     a.gi_nf = gi_nf;
     FS_REQUIRE(int64_t(c.N) * std::max(Dd, 1) < (int64_t(1) << 30), FS_ERR_UNSUPPORTED,
@@ -1869,7 +1896,22 @@ void launch_jacobian(Context &c) {
     a.emb_v = c.emb_v.p;
     a.emb_k = c.emb_k.p;
     a.Fpp = c.Fpp.p;
-    const size_t base = size_t(c.ntab) * sizeof(TableDesc) + ((size_t(c.nknots) * 5 + 3) & ~size_t(3)) * 4;
+    // the slope bands are staged in LDS while every pass stays within the
+    // staging budget; larger tables are read from their FP32 copy in place
+    const size_t desc = size_t(c.ntab) * sizeof(TableDesc);
+    const size_t band = ((size_t(c.nknots) * 5 + 3) & ~size_t(3)) * 4;
+    const size_t widest = std::max({sizeof(DensStage), sizeof(JStagePair), sizeof(JStageDE),
+                                    sizeof(MomStage), sizeof(DQStage)});
+    a.stage_S = desc + band + 4 * widest <= kTableStageBytes ? 1 : 0;
+    a.tS32 = nullptr;
+    if (!a.stage_S) {
+      c.tab_S32.alloc(size_t(c.nknots) * 5);
+      hipLaunchKernelGGL(k_band_f32, dim3(nb(int64_t(c.nknots) * 5, 256)), dim3(256), 0, c.stream,
+                         int64_t(c.nknots) * 5, c.tab_S.p, c.tab_S32.p);
+      FS_HIP(hipGetLastError());
+      a.tS32 = c.tab_S32.p;
+    }
+    const size_t base = desc + (a.stage_S ? band : 0);
     const size_t lds_dens = base + 4 * sizeof(DensStage);
     const size_t lds_pair = base + 4 * sizeof(JStagePair), lds_de = base + 4 * sizeof(JStageDE);
     FS_REQUIRE(lds_de <= 64 * 1024 && lds_pair <= 64 * 1024, FS_ERR_UNSUPPORTED,
@@ -1889,34 +1931,39 @@ void launch_jacobian(Context &c) {
       // (the pair pass depends on neither kernel below; run beside them on a
       // second stream it measured no faster and stretched both)
       const unsigned g2 = unsigned(std::max(1, std::min(nt, c.cu_count * c.jac_blocks_per_cu)));
-      const SlotDesc *d_de = d_slots + size_t(NSP) * 64;
+      const SlotDesc *d_de = d_slots + size_t(TSP) * 64;
       c.time_main_begin(1);
       // wider maps (more than 128 columns of a pass) keep the compiler's
       // occupancy rather than spill their larger slot state
-      if (NSP == 2)
-        hipLaunchKernelGGL((k_jac_pair<2, kWavesPair>), dim3(g2), dim3(256), lds_pair, c.stream, a,
-                           c.nknots, d_slots);
-      else if (NSP == 4)
-        hipLaunchKernelGGL((k_jac_pair<4, 2>), dim3(g2), dim3(256), lds_pair, c.stream, a, c.nknots,
-                           d_slots);
-      FS_HIP(hipGetLastError());
+      for (int g = 0; g < TSP; g += NSP) {
+        if (NSP == 2)
+          hipLaunchKernelGGL((k_jac_pair<2, kWavesPair>), dim3(g2), dim3(256), lds_pair, c.stream, a,
+                             c.nknots, d_slots + size_t(g) * 64);
+        else
+          hipLaunchKernelGGL((k_jac_pair<4, 2>), dim3(g2), dim3(256), lds_pair, c.stream, a, c.nknots,
+                             d_slots + size_t(g) * 64);
+        FS_HIP(hipGetLastError());
+      }
       c.time_main_end(1);
       hipLaunchKernelGGL(k_emb_sites, dim3(nb(c.N, 256)), dim3(256), 0, c.stream, a);
       FS_HIP(hipGetLastError());
       const unsigned g1 = unsigned(std::min<int64_t>((c.N + 3) / 4, int64_t(c.cu_count) * 16));
-      if (Dd > 0) {
-        hipLaunchKernelGGL(k_jac_dens, dim3(g1), dim3(256), lds_dens, c.stream, a, c.nknots);
+      for (int dc0 = 0; dc0 < Dd; dc0 += 64 * kCPL) {
+        hipLaunchKernelGGL(k_jac_dens, dim3(g1), dim3(256), lds_dens, c.stream, a, c.nknots, dc0);
         FS_HIP(hipGetLastError());
       }
       c.time_main_begin(0);
-      if (NSD == 2)
-        hipLaunchKernelGGL((k_jac_de<2, kWavesDE>), dim3(g2), dim3(256), lds_de, c.stream, a, c.nknots,
-                           d_de);
-      else if (NSD == 4)
-        hipLaunchKernelGGL((k_jac_de<4, 2>), dim3(g2), dim3(256), lds_de, c.stream, a, c.nknots, d_de);
-      else if (NSD == 6)
-        hipLaunchKernelGGL((k_jac_de<6, 1>), dim3(g2), dim3(256), lds_de, c.stream, a, c.nknots, d_de);
-      FS_HIP(hipGetLastError());
+      for (int g = 0; g < TSD; g += NSD) {
+        const SlotDesc *sl = d_de + size_t(g) * 64;
+        if (NSD == 2)
+          hipLaunchKernelGGL((k_jac_de<2, kWavesDE>), dim3(g2), dim3(256), lds_de, c.stream, a,
+                             c.nknots, sl);
+        else if (NSD == 4)
+          hipLaunchKernelGGL((k_jac_de<4, 2>), dim3(g2), dim3(256), lds_de, c.stream, a, c.nknots, sl);
+        else
+          hipLaunchKernelGGL((k_jac_de<6, 1>), dim3(g2), dim3(256), lds_de, c.stream, a, c.nknots, sl);
+        FS_HIP(hipGetLastError());
+      }
       if (adp) {
         // dipole and quadrupole columns: moment tangents, then the force pass
         const size_t nmu = size_t(c.N) * 4 * Ddip, nla = size_t(c.N) * 6 * Dquad;
@@ -1931,23 +1978,30 @@ void launch_jacobian(Context &c) {
         a.lamb = c.lamb.p;
         const size_t lds_mom = base + 4 * sizeof(MomStage), lds_dq = base + 4 * sizeof(DQStage);
         FS_REQUIRE(lds_dq <= 64 * 1024, FS_ERR_UNSUPPORTED, "ADP Jacobian LDS budget exceeded");
-        const SlotDesc *d_u = d_de + size_t(NSD) * 64, *d_w = d_u + size_t(NSU) * 64;
-        auto pass = [&](auto kind, int ns, const SlotDesc *sl) {
+        const SlotDesc *d_u = d_de + size_t(TSD) * 64, *d_w = d_u + size_t(TSU) * 64;
+        // every moment-tangent group before any force pass reads the tangents
+        auto pass = [&](auto kind, int ns, int total, const SlotDesc *slots0) {
           constexpr int K = decltype(kind)::value;
           if (ns == 0) return;
-          if (ns == 1) hipLaunchKernelGGL((k_adp_mom<K, 1>), dim3(g1), dim3(256), lds_mom, c.stream, a, c.nknots, sl);
-          else if (ns == 2) hipLaunchKernelGGL((k_adp_mom<K, 2>), dim3(g1), dim3(256), lds_mom, c.stream, a, c.nknots, sl);
-          else hipLaunchKernelGGL((k_adp_mom<K, 4>), dim3(g1), dim3(256), lds_mom, c.stream, a, c.nknots, sl);
-          FS_HIP(hipGetLastError());
+          for (int g = 0; g < total; g += ns) {
+            const SlotDesc *sl = slots0 + size_t(g) * 64;
+            if (ns == 1) hipLaunchKernelGGL((k_adp_mom<K, 1>), dim3(g1), dim3(256), lds_mom, c.stream, a, c.nknots, sl);
+            else if (ns == 2) hipLaunchKernelGGL((k_adp_mom<K, 2>), dim3(g1), dim3(256), lds_mom, c.stream, a, c.nknots, sl);
+            else hipLaunchKernelGGL((k_adp_mom<K, 4>), dim3(g1), dim3(256), lds_mom, c.stream, a, c.nknots, sl);
+            FS_HIP(hipGetLastError());
+          }
           if (K == JP_QUAD) c.time_main_begin(2);
-          if (ns == 1) hipLaunchKernelGGL((k_adp_dq<K, 1>), dim3(g2), dim3(256), lds_dq, c.stream, a, c.nknots, sl);
-          else if (ns == 2) hipLaunchKernelGGL((k_adp_dq<K, 2>), dim3(g2), dim3(256), lds_dq, c.stream, a, c.nknots, sl);
-          else hipLaunchKernelGGL((k_adp_dq<K, 4>), dim3(g2), dim3(256), lds_dq, c.stream, a, c.nknots, sl);
-          FS_HIP(hipGetLastError());
+          for (int g = 0; g < total; g += ns) {
+            const SlotDesc *sl = slots0 + size_t(g) * 64;
+            if (ns == 1) hipLaunchKernelGGL((k_adp_dq<K, 1>), dim3(g2), dim3(256), lds_dq, c.stream, a, c.nknots, sl);
+            else if (ns == 2) hipLaunchKernelGGL((k_adp_dq<K, 2>), dim3(g2), dim3(256), lds_dq, c.stream, a, c.nknots, sl);
+            else hipLaunchKernelGGL((k_adp_dq<K, 4>), dim3(g2), dim3(256), lds_dq, c.stream, a, c.nknots, sl);
+            FS_HIP(hipGetLastError());
+          }
           if (K == JP_QUAD) c.time_main_end(2);
         };
-        pass(std::integral_constant<int, JP_DIP>{}, NSU, d_u);
-        pass(std::integral_constant<int, JP_QUAD>{}, NSW, d_w);
+        pass(std::integral_constant<int, JP_DIP>{}, NSU, TSU, d_u);
+        pass(std::integral_constant<int, JP_QUAD>{}, NSW, TSW, d_w);
       }
       c.time_main_end();
     }

@@ -11,6 +11,7 @@
 #include "common.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <mutex>
 #include <type_traits>
 
@@ -537,11 +538,74 @@ int fs_ne_noalloc_impl(Context &c, const float *J, const double *F, int64_t R, i
   return FS_OK;
 }
 
+// columns [a0, a0 + na) and [b0, b0 + nb) of J (R x D) side by side into W
+// (R x (na + nb))
+__global__ void k_gather_cols(int64_t R, int D, const float *__restrict__ J, int a0, int na, int b0,
+                              int nb, float *__restrict__ W) {
+  const int w = na + nb;
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= R * w) return;
+  const int64_t r = e / w;
+  const int k = int(e % w);
+  W[e] = J[r * D + (k < na ? a0 + k : b0 + k - na)];
+}
+
+// the blocks of M = W^T W (w = na + nb) into J^T J: the diagonal block of
+// group a when put_a, of group b when put_b, and the off-diagonal (a, b)
+// block with its transpose; J^T F of the groups likewise
+__global__ void k_put_blocks(int D, int a0, int na, int b0, int nb, int put_a, int put_b,
+                             const double *__restrict__ M, const double *__restrict__ gw,
+                             double *__restrict__ A, double *__restrict__ g) {
+  const int w = na + nb;
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= int64_t(w) * w) return;
+  const int i = int(e / w), j = int(e % w);
+  const bool ia = i < na, ja = j < na;
+  const int gi = ia ? a0 + i : b0 + i - na, gj = ja ? a0 + j : b0 + j - na;
+  if ((ia && ja && put_a) || (!ia && !ja && put_b) || (ia != ja))
+    A[int64_t(gi) * D + gj] = M[e];
+  if (j == 0 && ((ia && put_a) || (!ia && put_b))) g[gi] = gw[i];
+}
+
+// Normal equations of more than 256 columns: the columns in groups of 128;
+// for every pair of groups the two side by side are one <= 256-column product
+// of the kernels above, whose off-diagonal block is that pair's block of J^T J
+// (the diagonal blocks are taken from the first pair that holds them)
+void normal_equations_wide(Context &c) {
+  const int D = c.D;
+  constexpr int kGc = 128;
+  const int G = (D + kGc - 1) / kGc;
+  const int64_t R = c.R;
+  c.ne_W.alloc(size_t(std::max<int64_t>(R, 1)) * 2 * kGc);
+  c.ne_M.alloc(size_t(2 * kGc) * 2 * kGc + 2 * kGc);
+  const NeShape big = ne_shape(c.cu_count, R, 2 * kGc, 1 << 30);
+  c.jtj_part.alloc(size_t(big.nchunk + kNeGroups) * big.per);
+  for (int ga = 0; ga < G; ++ga)
+    for (int gb = ga + 1; gb < G; ++gb) {
+      const int a0 = ga * kGc, na = std::min(kGc, D - a0), b0 = gb * kGc, nb = std::min(kGc, D - b0);
+      const int w = na + nb;
+      hipLaunchKernelGGL(k_gather_cols, dim3(unsigned((R * w + 255) / 256)), dim3(256), 0, c.stream, R,
+                         D, c.J.p, a0, na, b0, nb, c.ne_W.p);
+      FS_HIP(hipGetLastError());
+      const NeShape sh = ne_shape(c.cu_count, R, w, 1 << 30);
+      double *M = c.ne_M.p, *gw = c.ne_M.p + size_t(w) * w;
+      ne_launch(c.stream, sh, c.ne_W.p, c.residual.p, R, w, c.jtj_part.p, M, gw);
+      const int put_a = gb == ga + 1, put_b = gb == G - 1 && ga == G - 2;
+      hipLaunchKernelGGL(k_put_blocks, dim3(unsigned((int64_t(w) * w + 255) / 256)), dim3(256), 0,
+                         c.stream, D, a0, na, b0, nb, put_a, put_b, M, gw, c.JtJ.p, c.Jtr.p);
+      FS_HIP(hipGetLastError());
+    }
+}
+
 void launch_normal_equations(Context &c) {
   const int D = c.D;
   c.JtJ.alloc(size_t(std::max(D, 1)) * std::max(D, 1));
   c.Jtr.alloc(std::max(D, 1));
   if (D == 0) return;
+  if (D > 16 * kMT) {
+    normal_equations_wide(c);
+    return;
+  }
   const NeShape sh = ne_shape(c.cu_count, c.R, D, 1 << 30);
   c.jtj_part.alloc(size_t(sh.nchunk + kNeGroups) * sh.per);
   ne_launch(c.stream, sh, c.J.p, c.residual.p, c.R, D, c.jtj_part.p, c.JtJ.p, c.Jtr.p);
@@ -556,9 +620,59 @@ void launch_diag_scale(Context &c) {
   FS_HIP(hipGetLastError());
 }
 
+// the damped solve of k_chol_solve on the host, for more than 256 columns
+// (one workgroup's LDS panels hold 256): the same outputs in lm_dx / lm_scal
+void damped_solve_host(Context &c, double mu) {
+  const int D = c.D;
+  std::vector<double> M(size_t(D) * D), g(D), dg(D), dx(D, 0.0);
+  FS_HIP(hipMemcpyAsync(M.data(), c.JtJ.p, sizeof(double) * M.size(), hipMemcpyDeviceToHost, c.stream));
+  FS_HIP(hipMemcpyAsync(g.data(), c.Jtr.p, sizeof(double) * D, hipMemcpyDeviceToHost, c.stream));
+  FS_HIP(hipMemcpyAsync(dg.data(), c.lm_dx.p + D, sizeof(double) * D, hipMemcpyDeviceToHost, c.stream));
+  FS_HIP(hipStreamSynchronize(c.stream));
+  for (int k = 0; k < D; ++k) M[size_t(k) * D + k] += mu * dg[k];
+  // lower Cholesky factor in place, row by row
+  double info = 0.0;
+  for (int i = 0; i < D && info == 0.0; ++i) {
+    double *Ri = &M[size_t(i) * D];
+    for (int j = 0; j <= i; ++j) {
+      const double *Rj = &M[size_t(j) * D];
+      double v = Ri[j];
+      for (int k = 0; k < j; ++k) v -= Ri[k] * Rj[k];
+      if (j < i) {
+        Ri[j] = v / Rj[j];
+      } else if (v > 0.0 && std::isfinite(v)) {
+        Ri[i] = std::sqrt(v);
+      } else {
+        info = 1.0;
+      }
+    }
+  }
+  double pred = 0.0;
+  if (info == 0.0) {
+    for (int i = 0; i < D; ++i) { // L y = -g
+      double v = -g[i];
+      for (int k = 0; k < i; ++k) v -= M[size_t(i) * D + k] * dx[k];
+      dx[i] = v / M[size_t(i) * D + i];
+    }
+    for (int i = D - 1; i >= 0; --i) { // L^T dx = y
+      double v = dx[i];
+      for (int k = i + 1; k < D; ++k) v -= M[size_t(k) * D + i] * dx[k];
+      dx[i] = v / M[size_t(i) * D + i];
+    }
+    for (int k = 0; k < D; ++k) pred += dx[k] * (mu * dg[k] * dx[k] - g[k]);
+  }
+  const double sc[2] = {info, pred};
+  FS_HIP(hipMemcpyAsync(c.lm_dx.p, dx.data(), sizeof(double) * D, hipMemcpyHostToDevice, c.stream));
+  FS_HIP(hipMemcpyAsync(c.lm_scal.p + 1, sc, sizeof(sc), hipMemcpyHostToDevice, c.stream));
+  FS_HIP(hipStreamSynchronize(c.stream)); // the host vectors go out of scope
+}
+
 void launch_damped_solve(Context &c, double mu) {
+  if (c.D > 256) {
+    damped_solve_host(c, mu);
+    return;
+  }
   c.lm_M.alloc(size_t(std::max(c.D, 1)) * std::max(c.D, 1));
-  FS_REQUIRE(c.D <= 256, FS_ERR_UNSUPPORTED, "the damped solve supports at most 256 parameters");
   hipLaunchKernelGGL(k_chol_solve, dim3(1), dim3(1024), 0, c.stream, c.D, c.JtJ.p, c.Jtr.p,
                      c.lm_dx.p + c.D, mu, c.lm_M.p, c.lm_dx.p, c.lm_scal.p);
   FS_HIP(hipGetLastError());

@@ -233,8 +233,20 @@ def read_model(src, energy_weight: float = 1.0, stress_weight: float = 0.0):
     raise ValueError(f"model '{kind}' is not evaluated by this library (eam, adp, stiweb)")
 
 
-def write_model(model, path=None) -> str:
-    """The model file of a TabulatedModel or SWModel (current parameters)."""
+def write_model(model, path=None, nknots=None) -> str:
+    """The model file of a TabulatedModel or SWModel (current parameters).
+    nknots: resample every table to that many knots over its span, as the
+    reference's writer does (io::write_native, src/io/output_writer.cpp:23-43,
+    500 knots by default there); the tables are then written free."""
+    if isinstance(model, TabulatedModel) and nknots is not None:
+        from .spline import resample
+        if int(nknots) < 2:
+            raise ValueError("nknots must be >= 2")
+        tabs = []
+        for t in model.tables:
+            lo, hi, vals = resample(t.x, t.y, int(nknots))
+            tabs.append(Table(int(nknots), lo, hi, vals))
+        model = TabulatedModel(model.kind, model.ntypes, tabs, model.energy_weight, model.stress_weight)
     if isinstance(model, TabulatedModel):
         P, T = model.npair, model.ntypes
         keys = [("pair", P), ("density", T), ("embedding", T)]

@@ -129,3 +129,48 @@ def test_stiweb_parameter_objects():
     assert m.param_count == 7
     with pytest.raises(ValueError, match="lambda"):
         fio.read_model(json.dumps(dict(j, **{"lambda": []})))
+
+
+def test_host_spline_matches_the_oracle():
+    # the package's host spline (model-file resampling) against the oracle's
+    # restatement of spline.cpp: slopes and uncached values, in and outside
+    # the span, linear tables (n < 4) included
+    from forcesmith_amd import spline as hs
+    rng = np.random.default_rng(3)
+    for n in (3, 4, 5, 9, 20):
+        x = knot_grid(n, 1.2, 5.0)
+        y = np.cumsum(rng.normal(size=n))
+        so = oracle.spline_slopes(x, y)
+        sh = hs.makima_slopes(x, y)
+        if n < 4:
+            assert sh is None and so is None
+        else:
+            assert np.array_equal(sh, so)
+        for r in np.concatenate([rng.uniform(0.5, 5.7, 50), x]):
+            vo, _ = oracle.spline_eval(x, y, r)
+            assert hs.evaluate(x, y, sh, r) == pytest.approx(vo, rel=1e-14, abs=1e-14)
+    # equal secants: both weights vanish and the slope is 0, as the reference
+    x = knot_grid(6, 0.0, 5.0)
+    assert np.array_equal(hs.makima_slopes(x, 2.0 * x), oracle.spline_slopes(x, 2.0 * x))
+
+
+def test_written_model_resampled_to_500_knots():
+    # output_writer.cpp:23-43: every table sampled at n knots over its span,
+    # written free; reading it back gives the resampled spline
+    from forcesmith_amd import spline as hs
+    w = dsm.config1(n_configs=1)
+    m = fio.read_model(fio.write_model(w.model, nknots=500))
+    assert [t.n for t in m.tables] == [500] * len(w.model.tables)
+    assert m.param_count == 500 * len(w.model.tables)
+    for t0, t1 in zip(w.model.tables, m.tables):
+        assert t1.rmin == t0.x[0] and t1.rmax == t0.x[-1]
+        s0 = hs.makima_slopes(t0.x, t0.y)
+        step = (t1.rmax - t1.rmin) / 499
+        for k in (0, 1, 250, 498, 499):
+            assert t1.y[k] == hs.evaluate(t0.x, t0.y, s0, t1.rmin + k * step)
+        # the resampled spline reproduces the source one between its knots
+        s1 = hs.makima_slopes(t1.x, t1.y)
+        r = np.linspace(t0.x[0], t0.x[-1], 37)
+        v0 = np.array([hs.evaluate(t0.x, t0.y, s0, q) for q in r])
+        v1 = np.array([hs.evaluate(t1.x, t1.y, s1, q) for q in r])
+        assert np.max(np.abs(v1 - v0)) <= 1e-4 * (1 + np.max(np.abs(v0)))
