@@ -146,6 +146,7 @@ bool population_cost_batched(Context &c, const double *X, int np, double *cost);
 bool population_supported(const Context &c);
 void population_prepare(Context &c, int np);
 void population_launch(Context &c, const double *d_X, int np, double *d_cost);
+double population_tune_launch(Context &c, const double *d_X, int np, double *d_cost);
 void launch_sw_rev(Context &c);
 void launch_sw_eval(Context &c);
 // per-pair SW scalars {v2, v2', h, h', dh/dgamma, dh'/dgamma, dh/da2, dh'/da2}
@@ -293,6 +294,9 @@ struct Context {
   DBuf<int32_t> pop_grp;
   std::vector<int32_t> h_pop_grp;
   int pop_grp_tiles = -1;
+  // members per block of the population passes (1, 2 or 4); 0 until the DE
+  // driver or fs_autotune has timed them (1 meanwhile)
+  int pop_members = 0;
   fs_de_stats de_stats{};          // of the last fs_de_minimize call
   // the residual + Jacobian step captured as a graph (step.cpp): parameters
   // from step_x, keyed on the buffer addresses and launch shapes it holds

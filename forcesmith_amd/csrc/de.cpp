@@ -550,7 +550,7 @@ int de_run_device(Context &c, const fs_de_opts *o, const double *lo, const doubl
   hipLaunchKernelGGL(k_de_init, dim3(nblk(int64_t(NP) * D, 256)), dim3(256), 0, st, NP, D, o->seed,
                      d.x0.p, d.lo.p, d.hi.p, d.pop.p);
   FS_HIP(hipGetLastError());
-  population_launch(c, d.pop.p, NP, d.cost.p);
+  population_tune_launch(c, d.pop.p, NP, d.cost.p);
   FS_HIP(hipStreamSynchronize(st));
 
   auto generation = [&](bool screened) {

@@ -714,16 +714,16 @@ __global__ void k_sum_final(int n, const double *__restrict__ part, double *__re
 }
 
 // ---------------------------------------------------------------------------
-// Batched population cost (EAM, mixed precision): |F(x_m)|^2 for a batch of
-// parameter vectors in three launches instead of one residual per member.
-// Block b works on member b / G over tile group b % G; the G = 8 groups are
+// Batched population cost (EAM, mixed or FP64): |F(x_m)|^2 for a batch of
+// parameter vectors in a few launches instead of one residual per member.
+// Block b works on M members over tile group b % G; the G = 8 groups are
 // contiguous tile ranges, one per XCD under the round-robin block dispatch,
 // so the blocks of the members in flight on an XCD share that group's pair
-// records in its L2. Each block keeps its member's FP32 interval table in LDS
-// for its whole tile range.
+// records in its L2. Each block keeps its members' coefficients in LDS for
+// its whole tile range.
 constexpr int kPopG = 8;
 constexpr int kPopE = 2;  // density pass: embedding energy, limit
-constexpr int kPopF = 11; // force pass: |F - F_ref|^2, pair energy, 9 virial
+constexpr int kPopF = 8;  // force pass: |F - F_ref|^2, pair energy, 6 virial
 
 struct PopArgs {
   int nb;                            // members in this batch
@@ -756,171 +756,366 @@ __global__ void k_pop_tables(int ntab, int D, int nknots, const TableDesc *__res
                ivl + size_t(m) * 2 * nivl, p64 + size_t(m) * 4 * nivl);
 }
 
-template <int LPA>
+// The population passes take M members per block (block b: tile group
+// b % kPopG, members M (b / kPopG) .. + M). Per pair the record, the distance
+// and the interval location are formed once and applied to every member's
+// coefficients staged in LDS: in FP32 mode the intervals' power forms located
+// from the FP64 distance (the residual kernels' location), in FP64 mode the
+// cached-site interval and Hermite basis of spl_site64. Per member the sums
+// keep the single-member order (per atom over its lanes, per tile in atom
+// order, per configuration in tile order).
+
+// cached-site lookup split into its member-independent part (interval,
+// linear-site flag and Hermite basis: spl_site64 with the same operations)
+// and the member's values
+struct Site64 {
+  int i;
+  bool lin;
+  double inv_dx, off, bv[4], bd[4];
+};
+
+__device__ inline Site64 site64(const TableDesc &T, const double *x, double r) {
+  Site64 s;
+  const int n = T.n;
+  if (r <= x[0]) {
+    s.i = 0;
+    s.lin = true;
+  } else if (r >= x[n - 1]) {
+    s.i = n - 2;
+    s.lin = true;
+  } else {
+    s.i = spl_locate(x, n, r);
+    s.lin = T.linear != 0;
+  }
+  const double dx = x[s.i + 1] - x[s.i];
+  if (s.lin) {
+    s.inv_dx = 1.0 / dx;
+    s.off = r - x[s.i];
+  } else {
+    herm_basis64(x[s.i], dx, r, s.bv, s.bd);
+  }
+  return s;
+}
+
+__device__ inline void site64_apply(const Site64 &s, const double *y, const double *sl, double &v,
+                                    double &d) {
+  const int i = s.i;
+  if (s.lin) {
+    v = y[i] + s.off * s.inv_dx * (y[i + 1] - y[i]);
+    d = (y[i + 1] - y[i]) * s.inv_dx;
+    return;
+  }
+  const double yi = y[i], yi1 = y[i + 1], si = sl[i], si1 = sl[i + 1];
+  v = s.bv[0] * yi + s.bv[1] * yi1 + s.bv[2] * si + s.bv[3] * si1;
+  d = s.bd[0] * yi + s.bd[1] * yi1 + s.bd[2] * si + s.bd[3] * si1;
+}
+
+// FP64 value of interval k at offset u from its FP32 hi/lo power form (the
+// hi + lo coefficients carry the FP64 form to ~1e-15)
+__device__ inline double hilo_value64(const float4 &q, const float4 &e, double u) {
+  const double c0 = double(q.x) + double(e.x), c1 = double(q.y) + double(e.y);
+  const double c2 = double(q.z) + double(e.z), c3 = double(q.w) + double(e.w);
+  return c0 + u * (c1 + u * (c2 + u * c3));
+}
+
+__device__ inline float hilo_deriv32(const TableDesc &T, const float4 &q, const float4 &e, float t) {
+  const float dt = (q.y + t * (2.0f * q.z + 3.0f * t * q.w)) + (e.y + t * (2.0f * e.z + 3.0f * t * e.w));
+  return spl_dscale(T, dt);
+}
+
+template <int LPA, int M>
 __global__ __launch_bounds__(256) void k_pop_density(EvalArgs a, PopArgs p) {
   constexpr int APB = 256 / LPA;
   extern __shared__ float4 smem[];
   TableDesc *sT = reinterpret_cast<TableDesc *>(smem);
-  __shared__ double sRed[APB][kPopE];
-  const int g = blockIdx.x % kPopG, m = blockIdx.x / kPopG;
+  __shared__ double sRed[APB][M * kPopE];
+  const int g = blockIdx.x % kPopG, m0 = (blockIdx.x / kPopG) * M;
+  const int nm = min(M, p.nb - m0);
   for (int k = threadIdx.x; k < a.ntab; k += blockDim.x) sT[k] = a.tabs[k];
-  const double *P64 = p.ivl64 + size_t(m) * 4 * p.nivl;
+  const size_t per = size_t(4) * p.nivl; // FP64 interval forms of one member
+  const double *P64 = p.ivl64 + size_t(m0) * per;
   if (p.stage_ivl) {
     double *l64 = reinterpret_cast<double *>(smem + a.ntab * int(sizeof(TableDesc) / sizeof(float4)));
-    for (int k = threadIdx.x; k < 4 * p.nivl; k += blockDim.x) l64[k] = P64[k];
+    for (size_t k = threadIdx.x; k < size_t(nm) * per; k += blockDim.x) l64[k] = P64[k];
     P64 = l64;
   }
   __syncthreads();
-  const double *ym = p.y + size_t(m) * p.nknots, *sm = p.s + size_t(m) * p.nknots;
   const int sub = threadIdx.x / LPA, lane = threadIdx.x % LPA;
-  double run = 0.0; // component threadIdx.x (< kPopE) of the running config partial
+  // sRed[sub]: the running sums of atom slot sub over the tiles of the
+  // current configuration, kept by the slot's lane 0; the block reduces them
+  // when the configuration changes (the tiles of a configuration are
+  // consecutive)
+  if (lane == 0)
+    for (int k = 0; k < M * kPopE; ++k) sRed[sub][k] = 0.0;
   int run_c = -1;
+  auto flush = [&]() {
+    __syncthreads();
+    if (int(threadIdx.x) < nm * kPopE) {
+      const int m = threadIdx.x / kPopE, comp = threadIdx.x % kPopE;
+      double v = 0.0;
+      for (int q = 0; q < APB; ++q) v += sRed[q][threadIdx.x];
+      p.partE[((size_t(m0 + m) * p.C + run_c) * kPopG + g) * kPopE + comp] = v;
+    }
+    __syncthreads();
+    if (lane == 0)
+      for (int k = 0; k < M * kPopE; ++k) sRed[sub][k] = 0.0;
+  };
   for (int tile = p.grp_t0[g]; tile < p.grp_t0[g + 1]; ++tile) {
     const int na = a.tile_na[tile];
     const int c = a.tile_cfg[tile];
+    if (c != run_c) {
+      if (run_c >= 0) flush();
+      run_c = c;
+    }
     if (sub < na) {
       const int64_t i = int64_t(a.tile_a0[tile]) + sub;
       const int ti = a.types[i];
-      double rho = 0.0;
+      double rho[M];
+#pragma unroll
+      for (int m = 0; m < M; ++m) rho[m] = 0.0;
       for (int64_t q = a.pair_off[i] + lane; q < a.pair_off[i + 1]; q += LPA) {
         const uint32_t b = a.bits[q];
         if (b & kFlagGj) {
           const TableDesc D = sT[a.npair + pj_type(b)];
           double u;
           const int k = spl_interval64(D, a.r64[q], u);
-          rho += spl_value64(D, P64, k, u);
+#pragma unroll
+          for (int m = 0; m < M; ++m)
+            if (m < nm) rho[m] += spl_value64(D, P64 + m * per, k, u);
         }
       }
-      rho = sub_sum<LPA>(rho);
+#pragma unroll
+      for (int m = 0; m < M; ++m) rho[m] = sub_sum<LPA>(rho[m]);
       if (lane == 0) {
         const TableDesc Te = sT[a.npair + a.ntypes + ti];
-        double rh = rho, lim = 0.0;
-        if (rh > Te.xn) {
-          const double dd = rh - Te.xn;
-          lim = kPenalty * dd * dd;
-          rh = Te.xn;
-        } else if (rh < Te.x0) {
-          const double dd = Te.x0 - rh;
-          lim = kPenalty * dd * dd;
-          rh = Te.x0;
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+          if (m >= nm) break;
+          double rh = rho[m], lim = 0.0;
+          if (rh > Te.xn) {
+            const double dd = rh - Te.xn;
+            lim = kPenalty * dd * dd;
+            rh = Te.xn;
+          } else if (rh < Te.x0) {
+            const double dd = Te.x0 - rh;
+            lim = kPenalty * dd * dd;
+            rh = Te.x0;
+          }
+          const size_t mm = size_t(m0 + m);
+          double F, dF;
+          spl_uncached64(Te, a.tx + Te.off, p.y + mm * p.nknots + Te.off, p.s + mm * p.nknots + Te.off,
+                         rh, F, dF);
+          p.Fp[mm * a.N + i] = dF;
+          p.Fp32[mm * a.N + i] = float(dF);
+          sRed[sub][m * kPopE] += F;
+          sRed[sub][m * kPopE + 1] += lim;
         }
-        double F, dF;
-        spl_uncached64(Te, a.tx + Te.off, ym + Te.off, sm + Te.off, rh, F, dF);
-        p.Fp[size_t(m) * a.N + i] = dF;
-        p.Fp32[size_t(m) * a.N + i] = float(dF);
-        sRed[sub][0] = F;
-        sRed[sub][1] = lim;
       }
     }
-    __syncthreads();
-    if (threadIdx.x < kPopE) {
-      if (c != run_c) {
-        if (run_c >= 0) p.partE[((size_t(m) * p.C + run_c) * kPopG + g) * kPopE + threadIdx.x] = run;
-        run = 0.0;
-        run_c = c;
-      }
-      for (int q = 0; q < na && q < APB; ++q) run += sRed[q][threadIdx.x];
-    }
-    __syncthreads();
   }
-  if (threadIdx.x < kPopE && run_c >= 0)
-    p.partE[((size_t(m) * p.C + run_c) * kPopG + g) * kPopE + threadIdx.x] = run;
+  if (run_c >= 0) flush();
 }
 
-// F64: every pair through the FP64 path (FP64 accumulation); otherwise FP32
-// pairs with the FP64 path only for edge pairs, as the residual kernels
-template <int LPA, bool F64>
+// per member and atom: force (FP32 or FP64), pair energy, virial in the six
+// residual components xx, yy, zz, xy, xz, yz (-1/2 d_p f_q, p <= q)
+template <class AT> struct PopAcc {
+  AT f0 = 0, f1 = 0, f2 = 0;
+  double ep = 0.0;
+  double sg[6] = {0, 0, 0, 0, 0, 0};
+};
+
+template <class AT, class FT>
+__device__ inline void pop_add(PopAcc<AT> &A, FT f0, FT f1, FT f2, double ep, double d0, double d1,
+                               double d2) {
+  A.f0 += AT(f0);
+  A.f1 += AT(f1);
+  A.f2 += AT(f2);
+  A.ep += ep;
+  const double h0 = 0.5 * d0, h1 = 0.5 * d1, h2 = 0.5 * d2;
+  const double g0 = double(f0), g1 = double(f1), g2 = double(f2);
+  A.sg[0] -= h0 * g0;
+  A.sg[1] -= h1 * g1;
+  A.sg[2] -= h2 * g2;
+  A.sg[3] -= h0 * g1;
+  A.sg[4] -= h0 * g2;
+  A.sg[5] -= h1 * g2;
+}
+
+// F64: every pair through the FP64 cached-site path with FP64 sums; otherwise
+// FP32 pairs with the FP64 path only for pairs at an exact span edge, as the
+// residual kernels. LDS: the members' FP32 interval forms (FP32 mode) or knot
+// values and slopes (FP64 mode), when they fit the staging budget.
+template <int LPA, bool F64, int M>
 __global__ __launch_bounds__(256) void k_pop_force(EvalArgs a, PopArgs p) {
   using AT = typename std::conditional<F64, double, float>::type;
   constexpr int APB = 256 / LPA;
   extern __shared__ float4 smem[];
   TableDesc *sT = reinterpret_cast<TableDesc *>(smem);
-  __shared__ double sRed[APB][kPopF];
-  const int g = blockIdx.x % kPopG, m = blockIdx.x / kPopG;
-  const float4 *sI = p.ivl32 + size_t(m) * 2 * p.nivl;
-  const double *sP = p.ivl64 + size_t(m) * 4 * p.nivl;
-  {
-    const float4 *src = reinterpret_cast<const float4 *>(a.tabs);
-    float4 *dst = reinterpret_cast<float4 *>(sT);
-    const int nq = a.ntab * int(sizeof(TableDesc) / sizeof(float4));
-    for (int k = threadIdx.x; k < nq; k += blockDim.x) dst[k] = src[k];
-    if (p.stage_ivl) {
+  __shared__ double sRed[APB][M * kPopF];
+  const int g = blockIdx.x % kPopG, m0 = (blockIdx.x / kPopG) * M;
+  const int nm = min(M, p.nb - m0);
+  const int nq = a.ntab * int(sizeof(TableDesc) / sizeof(float4));
+  for (int k = threadIdx.x; k < nq; k += blockDim.x) smem[k] = reinterpret_cast<const float4 *>(a.tabs)[k];
+  const size_t per32 = size_t(2) * p.nivl; // float4s of one member's FP32 interval forms
+  const size_t perk = size_t(p.nknots);
+  const float4 *I32 = p.ivl32 + size_t(m0) * per32;
+  const double *Y = p.y + size_t(m0) * perk, *S = p.s + size_t(m0) * perk;
+  if (p.stage_ivl) {
+    if constexpr (F64) {
+      double *ly = reinterpret_cast<double *>(smem + nq);
+      for (size_t k = threadIdx.x; k < size_t(nm) * perk; k += blockDim.x) {
+        ly[k] = Y[k];
+        ly[size_t(M) * perk + k] = S[k];
+      }
+      Y = ly;
+      S = ly + size_t(M) * perk;
+    } else {
       float4 *l32 = smem + nq;
-      for (int k = threadIdx.x; k < 2 * p.nivl; k += blockDim.x) l32[k] = sI[k];
-      double *l64 = reinterpret_cast<double *>(l32 + 2 * p.nivl);
-      for (int k = threadIdx.x; k < 4 * p.nivl; k += blockDim.x) l64[k] = sP[k];
-      sI = l32;
-      sP = l64;
+      for (size_t k = threadIdx.x; k < size_t(nm) * per32; k += blockDim.x) l32[k] = I32[k];
+      I32 = l32;
+    }
+  }
+  __syncthreads();
+  const int sub = threadIdx.x / LPA, lane = threadIdx.x % LPA;
+  // sRed[sub]: running sums of atom slot sub over the tiles of the current
+  // configuration (its lane 0), reduced by the block when the configuration
+  // changes
+  if (lane == 0)
+    for (int k = 0; k < M * kPopF; ++k) sRed[sub][k] = 0.0;
+  int run_c = -1;
+  auto flush = [&]() {
+    __syncthreads();
+    if (int(threadIdx.x) < nm * kPopF) {
+      const int m = threadIdx.x / kPopF, comp = threadIdx.x % kPopF;
+      double v = 0.0;
+      for (int q = 0; q < APB; ++q) v += sRed[q][threadIdx.x];
+      p.partF[((size_t(m0 + m) * p.C + run_c) * kPopG + g) * kPopF + comp] = v;
     }
     __syncthreads();
-  }
-  const Tabs tb{sT, sI, a.tx, p.y + size_t(m) * p.nknots, p.s + size_t(m) * p.nknots, sP};
-  EvalArgs am = a; // the member's embedding derivatives
-  am.Fp = p.Fp + size_t(m) * a.N;
-  am.Fp32 = p.Fp32 + size_t(m) * a.N;
-  const int sub = threadIdx.x / LPA, lane = threadIdx.x % LPA;
-  double run = 0.0; // component threadIdx.x (< kPopF) of the running config partial
-  int run_c = -1;
+    if (lane == 0)
+      for (int k = 0; k < M * kPopF; ++k) sRed[sub][k] = 0.0;
+  };
   for (int tile = p.grp_t0[g]; tile < p.grp_t0[g + 1]; ++tile) {
     const int na = a.tile_na[tile];
     const int c = a.tile_cfg[tile];
+    if (c != run_c) {
+      if (run_c >= 0) flush();
+      run_c = c;
+    }
     if (sub < na) {
       const int64_t i = int64_t(a.tile_a0[tile]) + sub;
       const int64_t cbase = a.atom_off[c];
       const int ti = a.types[i];
-      AtomState<float> si;
-      si.Fp = am.Fp32[i];
-      ForceAcc<AT> acc;
+      AT Fpi[M];
+      PopAcc<AT> acc[M];
+#pragma unroll
+      for (int m = 0; m < M; ++m) {
+        const size_t mm = size_t(m0 + (m < nm ? m : 0)) * a.N + i;
+        Fpi[m] = F64 ? AT(p.Fp[mm]) : AT(p.Fp32[mm]);
+      }
+      // FP64 mode: every pair through the FP64 sites; FP32 mode: the FP32
+      // pairs first, then (their own loop, so their FP64 bases do not add to
+      // the main loop's registers) the few pairs at an exact span edge
+      for (int pass = F64 ? 1 : 0; pass < 2; ++pass)
       for (int64_t q = a.pair_off[i] + lane; q < a.pair_off[i + 1]; q += LPA) {
         const float4 rec = a.rec[q];
         const uint32_t b = __float_as_uint(rec.w);
-        if (F64 || (b & kFlagEdge)) {
+        if (b & kFlagTiny) continue;
+        const bool exact = F64 || (b & kFlagEdge);
+        if (exact != (pass == 1)) continue;
+        const int tj = pj_type(b);
+        const int64_t jg = cbase + pj_local(b);
+        const int o = ord_pair(ti, tj, a.ntypes);
+        if (exact) {
+          // FP64 cached sites: interval and basis once, the members' values
           const Pair<double> e = load_pair64<true>(a, q);
-          AtomState<double> sd;
-          sd.Fp = am.Fp[i];
-          ForceAcc<double> t;
-          force_pair<FS_EAM, true, double>(tb, am, ti, cbase, sd, e.b, e.d0, e.d1, e.d2, e.r, t);
-          acc.f0 += AT(t.f0);
-          acc.f1 += AT(t.f1);
-          acc.f2 += AT(t.f2);
-          acc.ep += t.ep;
-          for (int k = 0; k < 9; ++k) acc.sg[k] += t.sg[k];
+          const double inv_r = 1.0 / e.r;
+          // one table's site at a time, applied to every member's knots and
+          // slopes (staged in FP64 mode, else read in place)
+          double phi[M], dphi[M], dgj[M], dgi[M], vv;
+#pragma unroll
+          for (int m = 0; m < M; ++m) phi[m] = dphi[m] = dgj[m] = dgi[m] = 0.0;
+          auto each = [&](int t, double *v, double *d) {
+            const TableDesc &T = sT[t];
+            const Site64 st = site64(T, a.tx + T.off, e.r);
+#pragma unroll
+            for (int m = 0; m < M; ++m)
+              if (m < nm) site64_apply(st, Y + size_t(m) * perk + T.off, S + size_t(m) * perk + T.off,
+                                       v ? v[m] : vv, d[m]);
+          };
+          if (b & kFlagPhi) each(o, phi, dphi);
+          if (b & kFlagGj) each(a.npair + tj, nullptr, dgj);
+          if (b & kFlagGi) each(a.npair + ti, nullptr, dgi);
+#pragma unroll
+          for (int m = 0; m < M; ++m) {
+            if (m >= nm) break;
+            const double fpi = F64 ? double(Fpi[m]) : p.Fp[size_t(m0 + m) * a.N + i];
+            const double fpj = p.Fp[size_t(m0 + m) * a.N + jg];
+            const double fs = (dphi[m] + fpi * dgj[m] + fpj * dgi[m]) * inv_r;
+            pop_add(acc[m], fs * e.d0, fs * e.d1, fs * e.d2, 0.5 * phi[m], e.d0, e.d1, e.d2);
+          }
         } else if constexpr (!F64) {
           const float r = sqrtf(rec.x * rec.x + rec.y * rec.y + rec.z * rec.z);
-          force_pair<FS_EAM, false, float>(tb, am, ti, cbase, si, b, rec.x, rec.y, rec.z, r, acc,
-                                           a.r64[q]);
+          const float inv_r = 1.0f / r;
+          const double r64 = a.r64[q];
+          int kp = -1, kj = -1, ki = -1;
+          double up = 0.0, u;
+          float tp = 0.f, tjf = 0.f, tif = 0.f;
+          if (b & kFlagPhi) kp = spl_interval64(sT[o], r64, up), tp = float(up);
+          if (b & kFlagGj) kj = spl_interval64(sT[a.npair + tj], r64, u), tjf = float(u);
+          if (b & kFlagGi) ki = spl_interval64(sT[a.npair + ti], r64, u), tif = float(u);
+          const int ip = kp >= 0 ? 2 * (sT[o].ivl + kp) : 0;
+          const int ij = kj >= 0 ? 2 * (sT[a.npair + tj].ivl + kj) : 0;
+          const int ii = ki >= 0 ? 2 * (sT[a.npair + ti].ivl + ki) : 0;
+#pragma unroll
+          for (int m = 0; m < M; ++m) {
+            if (m >= nm) break;
+            const float4 *Im = I32 + size_t(m) * per32;
+            float dphi = 0.f, dgj = 0.f, dgi = 0.f;
+            double phi64 = 0.0;
+            if (kp >= 0) {
+              const float4 q4 = Im[ip], e4 = Im[ip + 1];
+              dphi = hilo_deriv32(sT[o], q4, e4, tp);
+              phi64 = hilo_value64(q4, e4, up);
+            }
+            if (kj >= 0) dgj = hilo_deriv32(sT[a.npair + tj], Im[ij], Im[ij + 1], tjf);
+            if (ki >= 0) dgi = hilo_deriv32(sT[a.npair + ti], Im[ii], Im[ii + 1], tif);
+            const float fpj = p.Fp32[size_t(m0 + m) * a.N + jg];
+            const float fs = (dphi + float(Fpi[m]) * dgj + fpj * dgi) * inv_r;
+            pop_add(acc[m], fs * rec.x, fs * rec.y, fs * rec.z, 0.5 * phi64, double(rec.x),
+                    double(rec.y), double(rec.z));
+          }
         }
       }
-      acc.f0 = sub_sum<LPA>(acc.f0);
-      acc.f1 = sub_sum<LPA>(acc.f1);
-      acc.f2 = sub_sum<LPA>(acc.f2);
-      acc.ep = sub_sum<LPA>(acc.ep);
-      for (int k = 0; k < 9; ++k) acc.sg[k] = sub_sum<LPA>(acc.sg[k]);
+#pragma unroll
+      for (int m = 0; m < M; ++m) {
+        acc[m].f0 = sub_sum<LPA>(acc[m].f0);
+        acc[m].f1 = sub_sum<LPA>(acc[m].f1);
+        acc[m].f2 = sub_sum<LPA>(acc[m].f2);
+        acc[m].ep = sub_sum<LPA>(acc[m].ep);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) acc[m].sg[k] = sub_sum<LPA>(acc[m].sg[k]);
+      }
       if (lane == 0) {
-        const double F[3] = {double(acc.f0), double(acc.f1), double(acc.f2)};
-        double r2 = 0.0;
-        for (int k = 0; k < 3; ++k) {
-          const double d = F[k] - a.ref_force[3 * i + k];
-          r2 += d * d;
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+          if (m >= nm) break;
+          const double F[3] = {double(acc[m].f0), double(acc[m].f1), double(acc[m].f2)};
+          double r2 = 0.0;
+          for (int k = 0; k < 3; ++k) {
+            const double d = F[k] - a.ref_force[3 * i + k];
+            r2 += d * d;
+          }
+          double *row = &sRed[sub][m * kPopF];
+          row[0] += r2;
+          row[1] += acc[m].ep;
+#pragma unroll
+          for (int k = 0; k < 6; ++k) row[2 + k] += acc[m].sg[k];
         }
-        sRed[sub][0] = r2;
-        sRed[sub][1] = acc.ep;
-        for (int k = 0; k < 9; ++k) sRed[sub][2 + k] = acc.sg[k];
       }
     }
-    __syncthreads();
-    if (threadIdx.x < kPopF) {
-      if (c != run_c) {
-        if (run_c >= 0) p.partF[((size_t(m) * p.C + run_c) * kPopG + g) * kPopF + threadIdx.x] = run;
-        run = 0.0;
-        run_c = c;
-      }
-      for (int q = 0; q < na && q < APB; ++q) run += sRed[q][threadIdx.x];
-    }
-    __syncthreads();
   }
-  if (threadIdx.x < kPopF && run_c >= 0)
-    p.partF[((size_t(m) * p.C + run_c) * kPopG + g) * kPopF + threadIdx.x] = run;
+  if (run_c >= 0) flush();
 }
 
 // cost of every (member, configuration): the residual rows of
@@ -934,7 +1129,7 @@ __global__ void k_pop_finalize(int nb, int C, const int32_t *__restrict__ grp_t0
   const int64_t id = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (id >= int64_t(nb) * C) return;
   const int m = int(id / C), c = int(id % C);
-  double e = 0.0, lim = 0.0, r2 = 0.0, vir[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  double e = 0.0, lim = 0.0, r2 = 0.0, vir[6] = {0, 0, 0, 0, 0, 0};
   const int t0 = cfg_tile0[c], t1 = cfg_tile0[c + 1];
   for (int g = 0; g < kPopG; ++g) {
     if (!(grp_t0[g] < t1 && grp_t0[g + 1] > t0)) continue; // group holds no tile of c
@@ -943,14 +1138,13 @@ __global__ void k_pop_finalize(int nb, int C, const int32_t *__restrict__ grp_t0
     e += pe[0] + pf[1];
     lim += pe[1];
     r2 += pf[0];
-    for (int k = 0; k < 9; ++k) vir[k] += pf[2 + k];
+    for (int k = 0; k < 6; ++k) vir[k] += pf[2 + k];
   }
   const double er = ew * (e - ref_energy[c]);
   double cost = r2 + er * er + lim * lim;
   if (sw > 0.0) {
-    const int vo[6] = {0, 4, 8, 1, 2, 5};
-    for (int v = 0; v < 6; ++v) {
-      const double sr = sw * (vir[vo[v]] / vol[c] - ref_stress[6 * c + v]);
+    for (int v = 0; v < 6; ++v) { // xx, yy, zz, xy, xz, yz
+      const double sr = sw * (vir[v] / vol[c] - ref_stress[6 * c + v]);
       cost += sr * sr;
     }
   }
@@ -1135,15 +1329,32 @@ void population_prepare(Context &c, int np) {
 // |F(x_m)|^2 of the np device vectors d_X[np][D] into d_cost[np]: launches
 // only (no allocation, copy or synchronisation, so a stream capture can
 // record it); population_prepare(c, np) must have run
+template <int M> void population_launch_m(Context &c, const double *d_X, int np, double *d_cost);
+
+// members per block: c.pop_members (1, 2 or 4; the DE driver times them on its
+// first population)
 void population_launch(Context &c, const double *d_X, int np, double *d_cost) {
+  switch (c.pop_members) {
+  case 2: population_launch_m<2>(c, d_X, np, d_cost); break;
+  case 4: population_launch_m<4>(c, d_X, np, d_cost); break;
+  default: population_launch_m<1>(c, d_X, np, d_cost); break;
+  }
+}
+
+template <int M> void population_launch_m(Context &c, const double *d_X, int np, double *d_cost) {
   const int G = kPopG;
   const int cap = population_batch_cap(c, np);
   EvalArgs a = make_args(c);
-  const bool stage = stage_intervals(c);
+  // the M members' coefficients of a block are staged while they fit the
+  // budget: FP64 interval forms (density pass), FP32 hi/lo interval forms or,
+  // in FP64 mode, knot values and slopes (force pass)
+  const bool f64 = c.precision == FS_FP64;
   const size_t lds_t = size_t(c.ntab) * sizeof(TableDesc);
-  const size_t lds_d = lds_t + (stage ? 4 * size_t(c.nivl) * sizeof(double) : 0);
-  const size_t lds_f =
-      lds_t + (stage ? 2 * size_t(c.nivl) * sizeof(float4) + 4 * size_t(c.nivl) * sizeof(double) : 0);
+  const size_t mem_d = size_t(M) * 4 * c.nivl * sizeof(double);
+  const size_t mem_f = f64 ? size_t(M) * 2 * c.nknots * sizeof(double)
+                           : size_t(M) * 2 * c.nivl * sizeof(float4);
+  const bool stage_d = lds_t + mem_d <= kTableStageBytes, stage_f = lds_t + mem_f <= kTableStageBytes;
+  const size_t lds_d = lds_t + (stage_d ? mem_d : 0), lds_f = lds_t + (stage_f ? mem_f : 0);
   for (int m0 = 0; m0 < np; m0 += cap) {
     const int nb = std::min(cap, np - m0);
     hipLaunchKernelGGL(k_pop_tables, dim3(nb), dim3(64), 0, c.stream, c.ntab, c.D, c.nknots,
@@ -1154,7 +1365,7 @@ void population_launch(Context &c, const double *d_X, int np, double *d_cost) {
     p.nb = nb;
     p.nknots = c.nknots;
     p.nivl = c.nivl;
-    p.stage_ivl = stage ? 1 : 0;
+    p.stage_ivl = stage_d ? 1 : 0;
     p.y = c.pop_y.p;
     p.s = c.pop_s.p;
     p.ivl32 = c.pop_i32.p;
@@ -1165,16 +1376,18 @@ void population_launch(Context &c, const double *d_X, int np, double *d_cost) {
     p.partE = c.pop_partE.p;
     p.partF = c.pop_partF.p;
     p.C = c.C;
-    const dim3 grid(unsigned(nb) * G);
+    PopArgs pf = p;
+    pf.stage_ivl = stage_f ? 1 : 0;
+    const dim3 grid(unsigned((nb + M - 1) / M) * G);
     switch (c.lanes_per_atom) {
 #define FS_POP(L)                                                                                   \
   case L:                                                                                           \
-    hipLaunchKernelGGL(k_pop_density<L>, grid, dim3(256), lds_d, c.stream, a, p);                   \
+    hipLaunchKernelGGL((k_pop_density<L, M>), grid, dim3(256), lds_d, c.stream, a, p);              \
     FS_HIP(hipGetLastError());                                                                      \
-    if (c.precision == FS_FP64)                                                                     \
-      hipLaunchKernelGGL((k_pop_force<L, true>), grid, dim3(256), lds_f, c.stream, a, p);           \
+    if (f64)                                                                                        \
+      hipLaunchKernelGGL((k_pop_force<L, true, M>), grid, dim3(256), lds_f, c.stream, a, pf);       \
     else                                                                                            \
-      hipLaunchKernelGGL((k_pop_force<L, false>), grid, dim3(256), lds_f, c.stream, a, p);          \
+      hipLaunchKernelGGL((k_pop_force<L, false, M>), grid, dim3(256), lds_f, c.stream, a, pf);      \
     break;
       FS_POP(4)
       FS_POP(8)
@@ -1198,6 +1411,43 @@ void population_launch(Context &c, const double *d_X, int np, double *d_cost) {
   }
 }
 
+// the population cost of np members with c.pop_members picked by timing 1, 2
+// and 4 members per block on this population (once per context; every
+// variant gives the same costs: a member's sums do not depend on the others
+// in its block); returns the time of the chosen variant in ms
+double population_tune_launch(Context &c, const double *d_X, int np, double *d_cost) {
+  if (c.pop_members != 0) {
+    population_launch(c, d_X, np, d_cost);
+    return 0.0;
+  }
+  hipEvent_t e0, e1;
+  FS_HIP(hipEventCreate(&e0));
+  FS_HIP(hipEventCreate(&e1));
+  double best = 1e300;
+  int pick = 1;
+  try {
+    for (int m : {1, 2, 4}) {
+      c.pop_members = m;
+      FS_HIP(hipEventRecord(e0, c.stream));
+      population_launch(c, d_X, np, d_cost);
+      FS_HIP(hipEventRecord(e1, c.stream));
+      FS_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      FS_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) best = ms, pick = m;
+    }
+  } catch (...) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c.pop_members = 0;
+    throw;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c.pop_members = pick;
+  return best;
+}
+
 bool population_cost_batched(Context &c, const double *X, int np, double *cost) {
   if (!population_supported(c) || np <= 0) return false;
   population_prepare(c, np);
@@ -1205,7 +1455,7 @@ bool population_cost_batched(Context &c, const double *X, int np, double *cost) 
   c.pop_cost.alloc(size_t(np));
   FS_HIP(hipMemcpyAsync(c.pop_X.p, X, sizeof(double) * size_t(np) * c.D, hipMemcpyHostToDevice,
                         c.stream));
-  population_launch(c, c.pop_X.p, np, c.pop_cost.p);
+  population_tune_launch(c, c.pop_X.p, np, c.pop_cost.p);
   FS_HIP(hipMemcpyAsync(cost, c.pop_cost.p, sizeof(double) * np, hipMemcpyDeviceToHost, c.stream));
   FS_HIP(hipStreamSynchronize(c.stream));
   return true;
