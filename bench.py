@@ -67,6 +67,9 @@ def parse():
     ap.add_argument("--de-generations", type=int, default=1000)
     ap.add_argument("--de-box", type=float, default=0.2,
                     help="box half-width as a fraction of |x0| (plus 1e-3)")
+    ap.add_argument("--de-reference-box", action="store_true",
+                    help="BoostDESolver's box instead: unbounded knots become x0 -+ max(2|x0|, 5e-4) "
+                         "(solver.cpp:118-125)")
     ap.add_argument("--de-target", type=float, default=0.0,
                     help="surrogate run: the plain run's final cost")
     ap.add_argument("--de-plain-seconds", type=float, default=0.0,
@@ -204,15 +207,20 @@ def de_measure(a):
 
     g = calculator(a.de_plain_precision if a.de in ("plain", "both") else a.de_surrogate_precision)
     x0 = g.gather_params()
-    lo = x0 - a.de_box * np.abs(x0) - 1e-3
-    hi = x0 + a.de_box * np.abs(x0) + 1e-3
+    if a.de_reference_box:
+        lo, hi = g.gather_bounds()  # +-inf: the driver derives the reference's box
+        box = "reference: x0 -+ max(2|x0|, 5e-4)"
+    else:
+        lo = x0 - a.de_box * np.abs(x0) - 1e-3
+        hi = x0 + a.de_box * np.abs(x0) + 1e-3
+        box = f"x0 +- ({a.de_box}|x0| + 1e-3)"
     common = dict(F=0.65, CR=0.5, seed=42, progress_every=25)
     out = {"metric": "DE wall-time to target cost vs plain GPU", "unit": "x", "higher_is_better": False,
            "n_gpus": 1, "data": "synthetic (seeded ternary analytic EAM truth + noise)",
            "dtype": f"plain {a.de_plain_precision}, surrogate-run exact costs {a.de_surrogate_precision}",
            "config": {"workload": f"ternary tabulated EAM, {a.configs} configs, D={x0.size}",
                       "np": a.de_np, "generations": a.de_generations, "F": 0.65, "CR": 0.5, "seed": 42,
-                      "box": f"x0 +- ({a.de_box}|x0| + 1e-3)", "lanes_per_atom": a.de_lanes,
+                      "box": box, "lanes_per_atom": a.de_lanes,
                       "plain_precision": a.de_plain_precision,
                       "surrogate_precision": a.de_surrogate_precision,
                       "surrogate": {"hidden": 256, "oversample": 4, "retrain_every": 10}}}
@@ -222,6 +230,7 @@ def de_measure(a):
         _, c_plain, hp = g.de(x0, lo, hi, a.de_np, a.de_generations, **common)
         t_plain = time.perf_counter() - t0
         out["plain"] = {"seconds": t_plain, "final_cost": c_plain, "generations": len(hp),
+                        "initial_best": float(hp[0]) if len(hp) else None,
                         "ms_per_generation": 1e3 * t_plain / max(len(hp), 1),
                         "pair_member_evals_per_s": pair_member_rate(t_plain, len(hp))}
         log(f"plain: {len(hp)} generations {t_plain:.1f} s final {c_plain:.9e}")
@@ -237,7 +246,7 @@ def de_measure(a):
                           retrain_every=10, hidden=256, target_cost=target, **common)
         t_s = time.perf_counter() - t0
         out["surrogate"] = {"seconds": t_s, "final_cost": c_s, "generations": len(hs),
-                            "reached": bool(c_s <= target),
+                            "reached": bool(c_s <= target), "stats": g.de_stats(),
                             "ms_per_generation": 1e3 * t_s / max(len(hs), 1),
                             "pair_member_evals_per_s": pair_member_rate(t_s, len(hs))}
         log(f"surrogate: {len(hs)} generations {t_s:.1f} s best {c_s:.9e} reached {c_s <= target}")
