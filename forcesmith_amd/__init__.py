@@ -38,6 +38,8 @@ EXPORTS = (
     "fs_residual_count", "fs_row_offsets", "fs_prepare", "fs_neighbor_count", "fs_get_neighbor_list",
     "fs_eval_forces", "fs_eval_config", "fs_eval_residual", "fs_eval_jacobian", "fs_eval_step", "fs_normal_equations", "fs_normal_equations_noalloc",
     "fs_lm_minimize", "fs_lm_solve", "fs_normal_equations_solve", "fs_lm_step", "fs_population_cost", "fs_de_minimize", "fs_de_last_stats", "fs_device_buffer", "fs_synchronize",
+    "fs_eam_set_gauge", "fs_eam_get_gauge", "fs_eam_rho_stats", "fs_eam_rescale_rho_axis",
+    "fs_eam_embed_shift", "fs_eam_rescale",
     "fs_set_launch", "fs_set_jacobian_tile", "fs_get_launch", "fs_autotune", "fs_time_step", "fs_mlp_forward", "fs_mlp_loss_grad",
 )
 
@@ -126,6 +128,12 @@ def load_library(path: str = LIB_PATH):
         "fs_population_cost": (ct.c_int, [vp, _dp, ct.c_int32, _dp]),
         "fs_de_minimize": (ct.c_int, [vp, ct.POINTER(fs_de_opts), _dp, _dp, _dp, _dp, _dp, _i32p]),
         "fs_de_last_stats": (ct.c_int, [vp, ct.POINTER(fs_de_stats)]),
+        "fs_eam_set_gauge": (ct.c_int, [vp, ct.c_double, _dp, _dp]),
+        "fs_eam_get_gauge": (ct.c_int, [vp, _dp, _dp, _dp]),
+        "fs_eam_rho_stats": (ct.c_int, [vp, _dp, _dp, _dp, _i64p]),
+        "fs_eam_rescale_rho_axis": (ct.c_int, [vp, _dp]),
+        "fs_eam_embed_shift": (ct.c_int, [vp, _dp]),
+        "fs_eam_rescale": (ct.c_int, [vp, _dp, _dp]),
         "fs_device_buffer": (ct.c_int, [vp, ct.c_int32, ct.POINTER(vp), _i64p]),
         "fs_synchronize": (ct.c_int, [vp]),
         "fs_set_launch": (ct.c_int, [vp, ct.c_int32, ct.c_int32]),
@@ -187,6 +195,7 @@ class GpuCalculator:
 
     def set_model(self, model):
         lib, h = self._lib, self._h
+        self.ntypes = int(model.ntypes)
         if model.kind == "sw":
             self._check(lib.fs_set_sw_model(h, model.ntypes, _p(np.ascontiguousarray(model.pair_params), _dp),
                                             _p(np.ascontiguousarray(model.pair_fixed, np.uint8), _u8p),
@@ -402,6 +411,54 @@ class GpuCalculator:
         self._check(self._lib.fs_de_last_stats(self._h, ct.byref(st)))
         return {"generations": st.generations, "retrains": st.retrains, "screened": st.screened,
                 "fallbacks": st.fallbacks, "first_loss": st.first_loss, "last_loss": st.last_loss}
+
+    # ---- EAM gauge (src/core/rescale.cpp, core/scale_potentials.hpp) ----
+    def set_gauge(self, scale=1.0, slope=None, intercept=None):
+        """Density scale A, per-type embedding slopes S and intercepts F0:
+        density A g, embedding F(rho / A) - S rho - F0, pairs compensated
+        by S_i A g_j + S_j A g_i."""
+        nt = self.ntypes
+        S = np.ascontiguousarray(np.zeros(nt) if slope is None else slope, dtype=np.float64)
+        F0 = np.ascontiguousarray(np.zeros(nt) if intercept is None else intercept, dtype=np.float64)
+        self._check(self._lib.fs_eam_set_gauge(self._h, float(scale), _p(S, _dp), _p(F0, _dp)))
+
+    def gauge(self):
+        """(A, S[ntypes], F0[ntypes])."""
+        A = np.zeros(1)
+        S, F0 = np.zeros(self.ntypes), np.zeros(self.ntypes)
+        self._check(self._lib.fs_eam_get_gauge(self._h, _p(A, _dp), _p(S, _dp), _p(F0, _dp)))
+        return float(A[0]), S, F0
+
+    def rho_stats(self):
+        """Per type: min, max and mean of the clamped densities (the gauge's
+        units) over every configuration, and the atom count."""
+        nt = self.ntypes
+        lo, hi, mean = np.zeros(nt), np.zeros(nt), np.zeros(nt)
+        cnt = np.zeros(nt, dtype=np.int64)
+        self._check(self._lib.fs_eam_rho_stats(self._h, _p(lo, _dp), _p(hi, _dp), _p(mean, _dp),
+                                               _p(cnt, _i64p)))
+        return lo, hi, mean, cnt
+
+    def compute_rho_ref(self):
+        """compute_rho_ref (rescale.cpp:11-33): mean clamped density per type."""
+        return self.rho_stats()[2]
+
+    def rescale_rho_axis(self) -> float:
+        """rescale_rho_axis (rescale.cpp:35-101); returns the scale applied."""
+        a = np.zeros(1)
+        self._check(self._lib.fs_eam_rescale_rho_axis(self._h, _p(a, _dp)))
+        return float(a[0])
+
+    def embed_shift(self, rho_ref):
+        """embed_shift (rescale.cpp:103-136) at the reference densities."""
+        r = np.ascontiguousarray(rho_ref, dtype=np.float64)
+        self._check(self._lib.fs_eam_embed_shift(self._h, _p(r, _dp)))
+
+    def rescale_eam(self):
+        """rescale_eam (rescale.cpp:138-158); returns (a, rho_ref)."""
+        a, ref = np.zeros(1), np.zeros(self.ntypes)
+        self._check(self._lib.fs_eam_rescale(self._h, _p(a, _dp), _p(ref, _dp)))
+        return float(a[0]), ref
 
     # ---- launch shape / timing ----
     def set_launch(self, lanes_per_atom: int, blocks_per_cu: int = 8):

@@ -27,6 +27,7 @@ PER_FILE = {
     "context.cpp": ["-ffp-contract=off", "-x", "hip"],
     "solver.cpp": ["-x", "hip"],
     "step.cpp": ["-x", "hip"],
+    "gauge.cpp": ["-x", "hip"],
     "de.cpp": ["-ffp-contract=off", "-x", "hip"],
     "nl.hip": ["-ffp-contract=off"],
 }

@@ -151,6 +151,16 @@ void launch_sw_rev(Context &c);
 void launch_sw_eval(Context &c);
 // per-pair SW scalars {v2, v2', h, h', dh/dgamma, dh'/dgamma, dh/da2, dh'/da2}
 void launch_sw_pair_terms(Context &c);
+// the gauge's device copy from gA / h_gS / h_gF0 (allocated on first use)
+constexpr int kGaugeLen = 1 + 2 * kMaxTypes;
+void upload_gauge(Context &c);
+void reset_gauge(Context &c);
+// rescale_eam and its steps (gauge.cpp); per-type arrays of ntypes entries
+void gauge_set(Context &c, double A, const double *S, const double *F0);
+void gauge_rho_stats(Context &c, double *rmin, double *rmax, double *rmean, int64_t *count);
+double gauge_rescale_rho_axis(Context &c);
+void gauge_embed_shift(Context &c, const double *rho_ref);
+double gauge_rescale_eam(Context &c, double *rho_ref);
 
 struct Context;
 // the residual kernels' descriptors plus both interval forms fit the budget
@@ -207,6 +217,14 @@ struct Context {
   DBuf<float4> ivl32; // per interval k: power form {a, b, c, d} as FP32 hi at 2k, FP32 lo at 2k+1
   DBuf<double> ivl64; // per interval k: the same power form in FP64 at 4k .. 4k+3
   DBuf<int32_t> col_of, knot_of;
+  // EAM gauge of rescale_eam (src/core/rescale.cpp:82-158): density scale A,
+  // per-type embedding slope S_t and intercept F0_t; the device copy is
+  // {A, S[kMaxTypes], F0[kMaxTypes]}, the identity {1, 0.., 0..} by default
+  // and for every other model kind (gauge.cpp)
+  double gA = 1.0;
+  double h_gS[kMaxTypes] = {}, h_gF0[kMaxTypes] = {};
+  double h_gauge_img[1 + 2 * kMaxTypes] = {}; // source of the last upload
+  DBuf<double> gauge;
   // SW
   std::vector<double> h_sw, h_lam;
   std::vector<uint8_t> h_sw_fixed, h_lam_fixed;

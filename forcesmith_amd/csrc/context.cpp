@@ -263,6 +263,7 @@ int fs_de_impl(Context &c, const fs_de_opts *o, const double *lo, const double *
 double time_step_impl(Context &c, int what, int reps);
 int autotune_impl(Context &c);
 void evaluate_for_solver(Context &c) { evaluate(c); }
+void params_to_device(Context &c) { refresh_params(c); }
 void rebuild_tiles(Context &c) { build_tiles(c); }
 void ensure_ready(Context &c) {
   ensure_nl(c);
@@ -459,6 +460,7 @@ int fs_set_tabulated_model(fs_ctx *ctx, int32_t kind, int32_t ntypes, int32_t nt
     c.ivl64.alloc(4 * size_t(std::max(ivl, 1)));
     upload(c, c.col_of, c.h_col_of);
     upload(c, c.knot_of, c.h_knot_of);
+    reset_gauge(c);
     c.have_model = true;
     c.params_dirty = true;
     c.jl.valid = false;
@@ -845,6 +847,52 @@ int fs_de_minimize(fs_ctx *ctx, const fs_de_opts *opts, const double *lo, const 
     if (generations_run) *generations_run = gens;
   });
   return g != FS_OK ? g : rc;
+}
+
+int fs_eam_set_gauge(fs_ctx *ctx, double scale, const double *slope, const double *intercept) {
+  return guarded(ctx, [&](Context &c) {
+    gauge_set(c, scale, slope, intercept);
+    FS_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int fs_eam_get_gauge(fs_ctx *ctx, double *scale, double *slope, double *intercept) {
+  return guarded(ctx, [&](Context &c) {
+    c.fail_if_no_model();
+    if (scale) *scale = c.gA;
+    for (int t = 0; t < c.ntypes; ++t) {
+      if (slope) slope[t] = c.h_gS[t];
+      if (intercept) intercept[t] = c.h_gF0[t];
+    }
+  });
+}
+
+int fs_eam_rho_stats(fs_ctx *ctx, double *rho_min, double *rho_max, double *rho_mean,
+                     int64_t *count) {
+  return guarded(ctx, [&](Context &c) { gauge_rho_stats(c, rho_min, rho_max, rho_mean, count); });
+}
+
+int fs_eam_rescale_rho_axis(fs_ctx *ctx, double *a) {
+  return guarded(ctx, [&](Context &c) {
+    const double v = gauge_rescale_rho_axis(c);
+    if (a) *a = v;
+    FS_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int fs_eam_embed_shift(fs_ctx *ctx, const double *rho_ref) {
+  return guarded(ctx, [&](Context &c) {
+    gauge_embed_shift(c, rho_ref);
+    FS_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int fs_eam_rescale(fs_ctx *ctx, double *a, double *rho_ref) {
+  return guarded(ctx, [&](Context &c) {
+    const double v = gauge_rescale_eam(c, rho_ref);
+    if (a) *a = v;
+    FS_HIP(hipStreamSynchronize(c.stream));
+  });
 }
 
 int fs_de_last_stats(fs_ctx *ctx, fs_de_stats *out) {

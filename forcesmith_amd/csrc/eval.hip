@@ -189,6 +189,7 @@ struct EvalArgs {
   double *e_atom, *lim, *mu, *lamb;
   float4 *mu32, *lamb32; // FP32 copies of mu, lamb (mixed ADP force pass)
   double *forces, *residual, *tile_part;
+  const double *gauge; // EAM gauge {A, S[kMaxTypes], F0[kMaxTypes]} (identity: 1, 0, 0)
 };
 
 template <int LPA, class T> __device__ inline T sub_sum(T v) {
@@ -217,6 +218,45 @@ __device__ inline void stage_tables(const EvalArgs &a, TableDesc *sT, float4 *sm
     sP = a.ivl64;
   }
   __syncthreads();
+}
+
+// The EAM gauge (rescale_eam, src/core/rescale.cpp:82-158) on the embedding
+// of one atom. sigma is the density sum of the unscaled tables; the gauge's
+// density is rho = A sigma (ScaledOutputPotential), clamped against the
+// embedding's span A [x0, xn] (ScaledArgPotential::span, clamp_embedding
+// kernels.hpp:19-31, the reference's branch order for either sign of A). The
+// embedding is evaluated at u = rho / A: sigma itself when no clamp applies,
+// (A end) / A at a clamped atom, the reference's argument. The returned
+// energy is F(u) - F0 + S (rho - clamped rho): the gauge's embedding
+// F(u) - S rho_c - F0 plus the slope compensation of the pairs,
+// sum_j S A g_j(r) = S rho, exact when the pair and density tables share one
+// span (the host accepts no other gauge). Forces need no change: F'(u) / A - S
+// times A g'(r) plus the compensation S A g'(r) is F'(u) g'(r). u is returned
+// for the embedding lookup; lim and over (d lim / d sigma) for the limit row.
+struct GaugedRho {
+  double u, lim, over, e_shift;
+};
+__device__ inline GaugedRho gauge_clamp(const double *gauge, int ti, const TableDesc &Te,
+                                        double sigma) {
+  const double A = gauge[0], S = gauge[1 + ti], F0 = gauge[1 + kMaxTypes + ti];
+  const double rho = A * sigma, begin = A * Te.x0, end = A * Te.xn;
+  GaugedRho g{sigma, 0.0, 0.0, 0.0};
+  double rc = rho;
+  if (rho > end) {
+    const double dd = rho - end;
+    g.lim = kPenalty * dd * dd;
+    g.over = 2.0 * kPenalty * dd * A;
+    rc = end;
+    g.u = rc / A;
+  } else if (rho < begin) {
+    const double dd = begin - rho;
+    g.lim = kPenalty * dd * dd;
+    g.over = -2.0 * kPenalty * dd * A;
+    rc = begin;
+    g.u = rc / A;
+  }
+  g.e_shift = S * (rho - rc) - F0;
+  return g;
 }
 
 // table lookups -------------------------------------------------------------
@@ -377,22 +417,14 @@ __global__ __launch_bounds__(256) void k_density(EvalArgs a) {
     }
     if (lane == 0) {
       // clamp_embedding + F(rho) in FP64 (kernels.hpp:19-31, eam_force.cpp:68-73)
+      // under the gauge (the identity for ADP: the operations reduce exactly
+      // to the plain clamp)
       const TableDesc Te = sT[a.npair + a.ntypes + ti];
-      double rh = double(acc.rho), lim = 0.0, over = 0.0;
-      if (rh > Te.xn) {
-        const double dd = rh - Te.xn;
-        lim += kPenalty * dd * dd;
-        over = 2.0 * kPenalty * dd;
-        rh = Te.xn;
-      } else if (rh < Te.x0) {
-        const double dd = Te.x0 - rh;
-        lim += kPenalty * dd * dd;
-        over = -2.0 * kPenalty * dd;
-        rh = Te.x0;
-      }
+      const GaugedRho gr = gauge_clamp(a.gauge, ti, Te, double(acc.rho));
+      const double rh = gr.u, lim = gr.lim, over = gr.over;
       double F, dF;
       spl_uncached64(Te, a.tx + Te.off, a.ty + Te.off, a.ts + Te.off, rh, F, dF);
-      double e = F;
+      double e = F + gr.e_shift;
       if (KIND == FS_ADP) {
         const double m0 = acc.m0, m1 = acc.m1, m2 = acc.m2;
         e += 0.5 * (m0 * m0 + m1 * m1 + m2 * m2);
@@ -891,24 +923,15 @@ __global__ __launch_bounds__(256) void k_pop_density(EvalArgs a, PopArgs p) {
 #pragma unroll
         for (int m = 0; m < M; ++m) {
           if (m >= nm) break;
-          double rh = rho[m], lim = 0.0;
-          if (rh > Te.xn) {
-            const double dd = rh - Te.xn;
-            lim = kPenalty * dd * dd;
-            rh = Te.xn;
-          } else if (rh < Te.x0) {
-            const double dd = Te.x0 - rh;
-            lim = kPenalty * dd * dd;
-            rh = Te.x0;
-          }
+          const GaugedRho gr = gauge_clamp(a.gauge, ti, Te, rho[m]);
           const size_t mm = size_t(m0 + m);
           double F, dF;
           spl_uncached64(Te, a.tx + Te.off, p.y + mm * p.nknots + Te.off, p.s + mm * p.nknots + Te.off,
-                         rh, F, dF);
+                         gr.u, F, dF);
           p.Fp[mm * a.N + i] = dF;
           p.Fp32[mm * a.N + i] = float(dF);
-          sRed[sub][m * kPopE] += F;
-          sRed[sub][m * kPopE + 1] += lim;
+          sRed[sub][m * kPopE] += F + gr.e_shift;
+          sRed[sub][m * kPopE + 1] += gr.lim;
         }
       }
     }
@@ -1207,6 +1230,7 @@ EvalArgs make_args(Context &c) {
   a.forces = c.forces.p;
   a.residual = c.residual.p;
   a.tile_part = c.tile_part.p;
+  a.gauge = c.gauge.p;
   return a;
 }
 

@@ -175,6 +175,7 @@ struct JacArgs {
   double *part;           // [ntiles x kJP x D]
   const double *vol;
   double ew, sw;
+  const double *gauge;    // EAM gauge {A, S[kMaxTypes], F0[kMaxTypes]}
 };
 
 // LDS staging of the descriptors and, when it fits the staging budget, the
@@ -763,8 +764,11 @@ __device__ __forceinline__ void jac_main_body(const JacArgs &a, int nknots, cons
         for (int q = 0; q < 3; ++q) G[q] = wave_sum(G[q]);
         for (int q = 0; q < 6; ++q) Gs[q] = wave_sum(Gs[q]);
         // per-atom terms
+        // a clamped atom's energy keeps S_t A (rho - clamped rho) under the
+        // gauge (the slope compensation of its pairs), d / d sigma = S_t A
         const bool clamped = a.over[i] != 0.0;
         const double Fpi64 = a.Fp[i], over = a.over[i];
+        const double SA = a.gauge[1 + ti] * a.gauge[0];
         const int kfi = a.emb_k[i];
         float ebi[4], evi[4];
         for (int q = 0; q < 4; ++q) {
@@ -779,8 +783,12 @@ __device__ __forceinline__ void jac_main_body(const JacArgs &a, int nknots, cons
           if (ckind[s] == CK_DENS) {
             g = a.Hs[i * a.Dd + cdc[s]];
             const double h = a.H[i * a.Dd + cdc[s]];
-            if (clamped) l_extra = float(over * h);
-            else e_extra = float(Fpi64 * h);
+            if (clamped) {
+              l_extra = float(over * h);
+              e_extra = float(SA * h);
+            } else {
+              e_extra = float(Fpi64 * h);
+            }
           } else if (ckind[s] == CK_EMB && ctab[s] == temb_i) {
             const TableDesc D = sT[ctab[s]];
             g = band(sS, D, kfi, cm[s], ebi);
@@ -1885,6 +1893,7 @@ void launch_jacobian(Context &c) {
     a.over = c.clamp_over.p;
     a.Fp = c.Fp.p;
     a.Fp32 = c.Fp32.p;
+    a.gauge = c.gauge.p;
     c.H.alloc(size_t(std::max<int64_t>(c.N * std::max(Dd, 1), 1)));
     c.Hs.alloc(size_t(std::max<int64_t>(c.N * std::max(Dd, 1), 1)));
     c.emb_d.alloc(4 * size_t(std::max<int64_t>(c.N, 1)));

@@ -16,7 +16,7 @@
 
 namespace fsa {
 
-__device__ inline int spl_locate(const double *x, int n, double r) {
+__host__ __device__ inline int spl_locate(const double *x, int n, double r) {
   int lo = 0, hi = n; // first index with x > r
   while (lo < hi) {
     const int mid = (lo + hi) >> 1;
@@ -83,8 +83,9 @@ __device__ inline void spl_site64(const TableDesc &T, const double *x, const dou
   d = bd[0] * yi + bd[1] * yi1 + bd[2] * si + bd[3] * si1;
 }
 
-// uncached evaluation with linear extrapolation (embedding F(rho))
-__device__ inline void spl_uncached64(const TableDesc &T, const double *x, const double *y,
+// uncached evaluation with linear extrapolation (embedding F(rho)); also
+// the host's evaluation of the embedding for the gauge
+__host__ __device__ inline void spl_uncached64(const TableDesc &T, const double *x, const double *y,
                                       const double *s, double r, double &v, double &d) {
   const int n = T.n;
   if (r <= x[0]) {

@@ -49,7 +49,7 @@ std::vector<uintptr_t> step_key(const Context &c) {
       ad(c.tile_part), ad(c.cfg_E), ad(c.cfg_S), ad(c.cfg_L), ad(c.residual), ad(c.cost),
       ad(c.cost_part), ad(c.J), ad(c.H), ad(c.jac_part), ad(c.jt_cfg), ad(c.jt_a0), ad(c.jt_na),
       ad(c.jt_ct0), ad(c.jcols), ad(c.jslots), ad(c.jrec), ad(c.jr), ad(c.Hs), ad(c.emb_d),
-      ad(c.emb_v), ad(c.emb_k), ad(c.adp_tan), ad(c.step_x)};
+      ad(c.emb_v), ad(c.emb_k), ad(c.adp_tan), ad(c.step_x), ad(c.gauge)};
   k.insert(k.end(), std::begin(ptrs), std::end(ptrs));
   return k;
 }

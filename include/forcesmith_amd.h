@@ -268,6 +268,38 @@ typedef struct fs_de_stats {
 } fs_de_stats;
 int fs_de_last_stats(fs_ctx *ctx, fs_de_stats *out);
 
+/* EAM gauge transformation (src/core/rescale.cpp, include/forcesmith/core/
+ * rescale.hpp:11-19): the reference wraps its tables
+ * (core/scale_potentials.hpp) as density A g_t (ScaledOutputPotential),
+ * embedding F_t(rho / A) - S_t rho - F0_t over the span A [lo, hi]
+ * (ScaledArgPotential under LinearAdjustedPotential) and pair
+ * phi_ij + S_i A g_j + S_j A g_i (CompensatedPairPotential). Here the gauge is
+ * that scale A and the per-type slopes S and intercepts F0 (ntypes entries),
+ * applied by every later evaluation, Jacobian, solver and population call;
+ * the knot values stay the parameters. A new model resets it to the
+ * identity (A = 1, S = F0 = 0). EAM only (FS_ERR_UNSUPPORTED otherwise); a
+ * slope that would compensate a pair table of another span than the density
+ * table it pairs with is refused with FS_ERR_UNSUPPORTED.
+ *   fs_eam_set_gauge / fs_eam_get_gauge: the gauge itself (NULL arrays: 0).
+ *   fs_eam_rho_stats: compute_rho_ref (rescale.cpp:11-33) and the extremes
+ *     of rescale_rho_axis (:39-52): per type the min, max and mean of the
+ *     clamped densities over every configuration, in the gauge's units, and
+ *     the atom count (min = +inf, max = -inf, mean = 0 for a type with no
+ *     atoms); any output may be NULL.
+ *   fs_eam_rescale_rho_axis: rescale_rho_axis (:35-101); *a receives the
+ *     scale applied (1 when it leaves the gauge unchanged).
+ *   fs_eam_embed_shift: embed_shift (:103-136) at the given reference
+ *     densities.
+ *   fs_eam_rescale: rescale_eam (:138-158); *a and rho_ref[ntypes] (either
+ *     may be NULL) receive the rho-axis scale and the reference densities. */
+int fs_eam_set_gauge(fs_ctx *ctx, double scale, const double *slope, const double *intercept);
+int fs_eam_get_gauge(fs_ctx *ctx, double *scale, double *slope, double *intercept);
+int fs_eam_rho_stats(fs_ctx *ctx, double *rho_min, double *rho_max, double *rho_mean,
+                     int64_t *count);
+int fs_eam_rescale_rho_axis(fs_ctx *ctx, double *a);
+int fs_eam_embed_shift(fs_ctx *ctx, const double *rho_ref);
+int fs_eam_rescale(fs_ctx *ctx, double *a, double *rho_ref);
+
 /* Device views and tuning (no reference counterpart: measurement hooks). */
 enum fs_buffer {
   FS_BUF_RESIDUAL = 0, /* double[R] */

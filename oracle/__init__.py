@@ -111,6 +111,14 @@ def _load():
                                            ct.c_double, _dp, _dp, _dp, _dp, _dp, _dp]),
         "or_time_fd_jacobian": (ct.c_double, [ct.c_void_p, _dp, ct.c_int, _dp]),
         "or_time_residual": (ct.c_double, [ct.c_void_p, _dp, ct.c_int, _dp]),
+        "or_set_gauge": (ct.c_int, [ct.c_void_p, ct.c_double, _dp, _dp]),
+        "or_get_gauge": (None, [ct.c_void_p, _dp, _dp, _dp]),
+        "or_gauge_spans_match": (ct.c_int, [ct.c_void_p]),
+        "or_emb_gauged": (None, [ct.c_void_p, ct.c_int, ct.c_double, _dp, _dp]),
+        "or_rho_stats": (None, [ct.c_void_p, _dp, _dp, _dp, _i64p]),
+        "or_rescale_rho_axis": (ct.c_double, [ct.c_void_p]),
+        "or_embed_shift": (None, [ct.c_void_p, _dp]),
+        "or_rescale_eam": (ct.c_double, [ct.c_void_p, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -237,6 +245,7 @@ class Oracle:
                                 _p(pos, _dp), _p(rf, _dp), _p(cell, _dp), _p(re, _dp), _p(rs, _dp))
         self.n_atoms = int(off[-1])
         self.n_configs = ds.n_configs
+        self.ntypes = int(model.ntypes)
         if model.kind == "sw":
             pp = _f64(model.pair_params)
             pf = np.ascontiguousarray(model.pair_fixed, dtype=np.uint8)
@@ -360,8 +369,58 @@ class Oracle:
     def jacobian(self, x):
         x = _f64(x)
         J = np.zeros((self.residual_count, self.param_count))
-        self._lib.or_jacobian_analytic(self._h, _p(x, _dp), _p(J, _dp))
+        rc = self._lib.or_jacobian_analytic(self._h, _p(x, _dp), _p(J, _dp))
+        if rc != 0:
+            raise ValueError(f"oracle analytic Jacobian refused the model (code {rc})")
         return J
+
+    # --- EAM gauge (src/core/rescale.cpp, core/scale_potentials.hpp) ---
+    def set_gauge(self, A=1.0, S=None, F0=None):
+        nt = self.ntypes
+        S = _f64(np.zeros(nt) if S is None else S)
+        F0 = _f64(np.zeros(nt) if F0 is None else F0)
+        rc = self._lib.or_set_gauge(self._h, float(A), _p(S, _dp), _p(F0, _dp))
+        if rc != 0:
+            raise ValueError(f"oracle rejected the gauge (code {rc})")
+
+    def gauge(self):
+        A = np.zeros(1)
+        S, F0 = np.zeros(self.ntypes), np.zeros(self.ntypes)
+        self._lib.or_get_gauge(self._h, _p(A, _dp), _p(S, _dp), _p(F0, _dp))
+        return float(A[0]), S, F0
+
+    def gauge_spans_match(self) -> bool:
+        return bool(self._lib.or_gauge_spans_match(self._h))
+
+    def embedding(self, t, rho):
+        """Value and derivative of the gauged embedding of type t at rho."""
+        F, dF = np.zeros(1), np.zeros(1)
+        self._lib.or_emb_gauged(self._h, int(t), float(rho), _p(F, _dp), _p(dF, _dp))
+        return float(F[0]), float(dF[0])
+
+    def rho_stats(self):
+        """Per type: min, max, mean of the clamped densities and atom count."""
+        nt = self.ntypes
+        lo, hi, mean = np.zeros(nt), np.zeros(nt), np.zeros(nt)
+        cnt = np.zeros(nt, dtype=np.int64)
+        self._lib.or_rho_stats(self._h, _p(lo, _dp), _p(hi, _dp), _p(mean, _dp), _p(cnt, _i64p))
+        return lo, hi, mean, cnt
+
+    def compute_rho_ref(self):
+        return self.rho_stats()[2]
+
+    def rescale_rho_axis(self) -> float:
+        return float(self._lib.or_rescale_rho_axis(self._h))
+
+    def embed_shift(self, rho_ref):
+        r = _f64(rho_ref)
+        self._lib.or_embed_shift(self._h, _p(r, _dp))
+
+    def rescale_eam(self):
+        """Returns (a, rho_ref)."""
+        ref = np.zeros(self.ntypes)
+        a = self._lib.or_rescale_eam(self._h, _p(ref, _dp))
+        return float(a), ref
 
     def lm(self, x, max_iter, use_fd=False):
         x = _f64(x).copy()

@@ -488,6 +488,8 @@ typedef struct {
   double A, B, p, q, delta, a1, gamma, a2;
 } or_sw;
 
+#define OR_MAXT 8
+
 struct or_ctx {
   int kind, ntypes, npair;
   int C;
@@ -524,6 +526,10 @@ struct or_ctx {
   /* outputs / scratch */
   double *force, *energy, *stress, *limit;
   double *rho, *gradF, *mu, *lam9;
+  /* EAM gauge: density scale, per-type embedding slope and intercept
+   * (identity: gA = 1, gS = gF0 = 0) */
+  int gauge;
+  double gA, gS[OR_MAXT], gF0[OR_MAXT];
 };
 
 or_ctx *or_create(int kind, int ntypes, int n_configs, const int64_t *atom_off,
@@ -564,6 +570,7 @@ or_ctx *or_create(int kind, int ntypes, int n_configs, const int64_t *atom_off,
   c->lam9 = (double *)calloc(9 * (N ? N : 1), sizeof(double));
   c->ew = 1.0;
   c->sw_w = 0.0;
+  c->gA = 1.0;
   c->nl_rc = -1.0;
   c->nrole = kind == OR_ADP ? 5 : (kind == OR_EAM ? 3 : 0);
   c->row_off = (int64_t *)calloc((size_t)n_configs + 1, sizeof(int64_t));
@@ -1048,6 +1055,273 @@ static void eam_config(or_ctx *c, int k) {
   c->limit[k] = lim;
 }
 
+/* ------------------------------------------------------------------------ */
+/* EAM gauge                                                                 */
+/* ------------------------------------------------------------------------ */
+/* The tables after rescale_eam (src/core/rescale.cpp:82-158) as the wrappers
+ * of include/forcesmith/core/scale_potentials.hpp compose them, held as one
+ * density scale A and per-type slope S_t and intercept F0_t:
+ *   density     A g_t(r)                         ScaledOutputPotential (:48-60)
+ *   embedding   F_t(rho / A) - S_t rho - F0_t,   ScaledArgPotential (:62-79)
+ *               derivative F'_t(rho / A) / A - S_t, span A [lo, hi]
+ *               under LinearAdjustedPotential (:34-46)
+ *   pair        phi_ij(r) + S_i A g_j(r) + S_j A g_i(r) over the span of
+ *               phi_ij, g evaluated ungated (CompensatedPairPotential :81-100)
+ * Wrapped potentials have no site cache (DelegatingPotential :11-15), so
+ * eval_gated takes its uncached branch: span check, then eval (eam_force.cpp:
+ * 19-43). The density is wrapped when A != 1, a pair when S_i or S_j != 0.
+ * Composition: a later rho-axis rescale by a maps (A, S) to (A a, S / a);
+ * a later embed_shift by s adds s to S; a later intercept adds to F0. */
+static int dens_wrapped(const or_ctx *c) { return c->gA != 1.0; }
+
+/* value of density t at r, ungated (spline eval extrapolates) */
+static void dens_value(const or_ctx *c, int t, double r, double *v, double *d) {
+  sp_eval_uncached(&c->tab[tab_dens(c, t)], r, v, d);
+  if (dens_wrapped(c)) {
+    *v = c->gA * *v;
+    *d = c->gA * *d;
+  }
+}
+
+/* eval_gated of density t for pair p in role GJ / GI */
+static void dens_gated(const or_ctx *c, int64_t p, int role, int t, double r, double *v,
+                       double *d) {
+  if (!dens_wrapped(c)) {
+    gated(c, p, role, tab_dens(c, t), r, v, d);
+  } else if (sp_in_range(&c->tab[tab_dens(c, t)], r)) {
+    dens_value(c, t, r, v, d);
+  } else {
+    *v = *d = 0.0;
+  }
+}
+
+/* eval_deriv_gated of pair (ti, tj) */
+static void pair_gated(const or_ctx *c, int64_t p, int ti, int tj, double r, double *v,
+                       double *d) {
+  const int tp = tab_pair(c, ti, tj);
+  if (c->gS[ti] == 0.0 && c->gS[tj] == 0.0) {
+    gated(c, p, ROLE_PHI, tp, r, v, d);
+    return;
+  }
+  if (!sp_in_range(&c->tab[tp], r)) {
+    *v = *d = 0.0;
+    return;
+  }
+  double pv, pd, bv, bd, av, ad;
+  sp_eval_uncached(&c->tab[tp], r, &pv, &pd);
+  dens_value(c, tj, r, &bv, &bd); /* g_beta: density of tj */
+  dens_value(c, ti, r, &av, &ad); /* g_alpha: density of ti */
+  *v = pv + c->gS[ti] * bv + c->gS[tj] * av;
+  *d = pd + c->gS[ti] * bd + c->gS[tj] * ad;
+}
+
+/* the gauged embedding of type t: value and derivative at rho */
+static void emb_gauged(const or_ctx *c, int t, double rho, double *F, double *dF) {
+  const or_spline *e = &c->tab[tab_emb(c, t)];
+  double v, d;
+  if (dens_wrapped(c)) {
+    sp_eval_uncached(e, rho / c->gA, &v, &d);
+    d = d / c->gA;
+  } else {
+    sp_eval_uncached(e, rho, &v, &d);
+  }
+  *F = v - c->gS[t] * rho - c->gF0[t];
+  *dF = d - c->gS[t];
+}
+
+/* span of the gauged embedding of type t (ScaledArgPotential::span) */
+static void emb_span(const or_ctx *c, int t, double *lo, double *hi) {
+  const or_spline *e = &c->tab[tab_emb(c, t)];
+  *lo = c->gA * e->x[0];
+  *hi = c->gA * e->x[e->n - 1];
+}
+
+/* eval_forces of the gauged tables (eam_force.cpp:45-102) */
+static void eam_config_gauge(or_ctx *c, int k) {
+  const int64_t a0 = c->atom_off[k], a1 = c->atom_off[k + 1];
+  double E = 0.0, lim = 0.0, sig[9] = {0};
+  for (int64_t i = a0; i < a1; ++i) {
+    c->force[3 * i] = c->force[3 * i + 1] = c->force[3 * i + 2] = 0.0;
+    c->rho[i] = 0.0;
+    c->gradF[i] = 0.0;
+  }
+  for (int64_t i = a0; i < a1; ++i)
+    for (int64_t p = c->pair_off[i]; p < c->pair_off[i + 1]; ++p) {
+      double v, d;
+      dens_gated(c, p, ROLE_GJ, c->types[c->nj[p]], pair_r(c, p), &v, &d);
+      c->rho[i] += v;
+    }
+  for (int64_t i = a0; i < a1; ++i) {
+    const int t = c->types[i];
+    double begin, end; /* clamp_embedding (kernels.hpp:19-31) on the span */
+    emb_span(c, t, &begin, &end);
+    if (c->rho[i] > end) {
+      const double dd = c->rho[i] - end;
+      lim += KDUMMY * 10.0 * dd * dd;
+      c->rho[i] = end;
+    } else if (c->rho[i] < begin) {
+      const double dd = begin - c->rho[i];
+      lim += KDUMMY * 10.0 * dd * dd;
+      c->rho[i] = begin;
+    }
+    double F, dF;
+    emb_gauged(c, t, c->rho[i], &F, &dF);
+    E += F;
+    c->gradF[i] = dF;
+  }
+  for (int64_t i = a0; i < a1; ++i) {
+    const int ti = c->types[i];
+    for (int64_t p = c->pair_off[i]; p < c->pair_off[i + 1]; ++p) {
+      const int64_t j = c->nj[p];
+      const int tj = c->types[j];
+      const double r = pair_r(c, p);
+      if (r < 1e-14) continue;
+      double phi, dphi, rj, drho_j, ri, drho_i;
+      pair_gated(c, p, ti, tj, r, &phi, &dphi);
+      dens_gated(c, p, ROLE_GJ, tj, r, &rj, &drho_j);
+      dens_gated(c, p, ROLE_GI, ti, r, &ri, &drho_i);
+      const double fscale = (dphi + c->gradF[i] * drho_j + c->gradF[j] * drho_i) * (1.0 / r);
+      const double *d = c->nd + 3 * p;
+      const double f[3] = {fscale * d[0], fscale * d[1], fscale * d[2]};
+      acc_pair(c->force + 3 * i, &E, sig, d, f, phi);
+    }
+  }
+  const double V = c->box[k].vol;
+  for (int q = 0; q < 9; ++q) c->stress[9 * k + q] = sig[q] / V;
+  c->energy[k] = E;
+  c->limit[k] = lim;
+}
+
+int or_set_gauge(or_ctx *c, double A, const double *S, const double *F0) {
+  if (c->kind != OR_EAM || c->ntypes > OR_MAXT) return -1;
+  if (!isfinite(A) || A == 0.0) return -2;
+  c->gA = A;
+  int id = A == 1.0;
+  for (int t = 0; t < c->ntypes; ++t) {
+    c->gS[t] = S ? S[t] : 0.0;
+    c->gF0[t] = F0 ? F0[t] : 0.0;
+    id = id && c->gS[t] == 0.0 && c->gF0[t] == 0.0;
+  }
+  c->gauge = !id;
+  return 0;
+}
+
+void or_get_gauge(const or_ctx *c, double *A, double *S, double *F0) {
+  if (A) *A = c->gA;
+  for (int t = 0; t < c->ntypes; ++t) {
+    if (S) S[t] = c->gS[t];
+    if (F0) F0[t] = c->gF0[t];
+  }
+}
+
+void or_emb_gauged(const or_ctx *c, int t, double rho, double *F, double *dF) {
+  emb_gauged(c, t, rho, F, dF);
+}
+
+/* compute_rho_ref (rescale.cpp:11-33) with the extremes rescale_rho_axis
+ * takes (:39-52): every configuration evaluated, the clamped densities per
+ * type; count 0 leaves min = +inf, max = -inf, mean = 0 */
+void or_rho_stats(or_ctx *c, double *rmin, double *rmax, double *rmean, int64_t *count) {
+  or_eval_forces(c, NULL, NULL, NULL, NULL);
+  for (int t = 0; t < c->ntypes; ++t) {
+    double lo = INFINITY, hi = -INFINITY, sum = 0.0;
+    int64_t n = 0;
+    for (int64_t i = 0; i < c->N; ++i) {
+      if (c->types[i] != t) continue;
+      sum += c->rho[i];
+      if (c->rho[i] > hi) hi = c->rho[i];
+      if (c->rho[i] < lo) lo = c->rho[i];
+      ++n;
+    }
+    if (rmin) rmin[t] = lo;
+    if (rmax) rmax[t] = hi;
+    if (rmean) rmean[t] = n > 0 ? sum / (double)n : 0.0;
+    if (count) count[t] = n;
+  }
+}
+
+/* rescale_rho_axis (rescale.cpp:35-101): the scale a that takes the
+ * dominant type's extreme density (plus 0.003 of its embedding span) to
+ * the end of that span; 1 (no change) without data, for |a| within
+ * [0.95, 1.05] while every type's densities lie inside its span, or for a
+ * non-finite / vanishing a. Applies a to the gauge and returns it. */
+double or_rescale_rho_axis(or_ctx *c) {
+  const int n = c->ntypes;
+  double mx[OR_MAXT], mn[OR_MAXT];
+  or_rho_stats(c, mn, mx, NULL, NULL);
+  int dom = 0, sign_pos = 1;
+  double best = -1.0;
+  for (int t = 0; t < n; ++t) {
+    if (!isfinite(mx[t])) continue;
+    const double ext = fmax(fabs(mx[t]), fabs(mn[t]));
+    if (ext > best) {
+      best = ext;
+      dom = t;
+      sign_pos = mx[t] >= -mn[t];
+    }
+  }
+  if (best < 0.0) return 1.0;
+  double elo, ehi;
+  emb_span(c, dom, &elo, &ehi);
+  const double pad = 0.003 * (ehi - elo);
+  const double upper = sign_pos ? ehi : elo;
+  const double right = sign_pos ? mx[dom] + pad : mn[dom] - pad;
+  if (fabs(right) < 1e-30) return 1.0;
+  const double a = upper / right;
+  int violation = 0;
+  for (int t = 0; t < n; ++t) {
+    if (!isfinite(mx[t])) continue;
+    double lo, hi;
+    emb_span(c, t, &lo, &hi);
+    if (mn[t] < lo || mx[t] > hi) violation = 1;
+  }
+  if (!isfinite(a) || fabs(a) < 1e-30 ||
+      (!violation && fabs(a) >= 0.95 && fabs(a) <= 1.05))
+    return 1.0;
+  double S[OR_MAXT];
+  for (int t = 0; t < n; ++t) S[t] = c->gS[t] / a;
+  or_set_gauge(c, c->gA * a, S, c->gF0);
+  return a;
+}
+
+/* embed_shift (rescale.cpp:103-136): per type the slope s_t = F'(rho_ref_t)
+ * of the current embedding (types with rho_ref_t > 0), subtracted from the
+ * embedding and compensated in the pairs. Slopes below 1e-14 are dropped
+ * (the reference skips them for the embedding; its pair compensation keeps
+ * such a slope while the partner's is larger, a term below 1e-14 g(r)). */
+void or_embed_shift(or_ctx *c, const double *rho_ref) {
+  double S[OR_MAXT];
+  for (int t = 0; t < c->ntypes; ++t) {
+    S[t] = c->gS[t];
+    if (!(rho_ref[t] > 0.0)) continue;
+    double F, dF;
+    emb_gauged(c, t, rho_ref[t], &F, &dF);
+    if (fabs(dF) >= 1e-14) S[t] += dF;
+  }
+  or_set_gauge(c, c->gA, S, c->gF0);
+}
+
+/* rescale_eam (rescale.cpp:138-158): rho axis, reference densities, slope
+ * shift, then the intercept F0_t = F(rho_ref_t) of the shifted embedding
+ * (types with rho_ref_t > 0 and |F0_t| >= 1e-14). Returns a. */
+double or_rescale_eam(or_ctx *c, double *rho_ref_out) {
+  const double a = or_rescale_rho_axis(c);
+  double ref[OR_MAXT], F0[OR_MAXT];
+  or_rho_stats(c, NULL, NULL, ref, NULL);
+  or_embed_shift(c, ref);
+  for (int t = 0; t < c->ntypes; ++t) {
+    F0[t] = c->gF0[t];
+    if (!(ref[t] > 0.0)) continue;
+    double F, dF;
+    emb_gauged(c, t, ref[t], &F, &dF);
+    if (fabs(F) >= 1e-14) F0[t] += F;
+  }
+  or_set_gauge(c, c->gA, c->gS, F0);
+  if (rho_ref_out)
+    for (int t = 0; t < c->ntypes; ++t) rho_ref_out[t] = ref[t];
+  return a;
+}
+
 /* ADPForceCalculator::eval_forces (adp_force.cpp:73-135); quad_nu/quad_xi
  * (adp_force.hpp:63-68) */
 static double quad_nu(const double *M, const double *d) {
@@ -1280,8 +1554,10 @@ static void ensure_nl(or_ctx *c) {
 
 static void eval_config_task(void *arg, int64_t k) {
   or_ctx *c = (or_ctx *)arg;
-  if (c->kind == OR_EAM) eam_config(c, (int)k);
-  else if (c->kind == OR_ADP) adp_config(c, (int)k);
+  if (c->kind == OR_EAM) {
+    if (c->gauge) eam_config_gauge(c, (int)k);
+    else eam_config(c, (int)k);
+  } else if (c->kind == OR_ADP) adp_config(c, (int)k);
   else sw_config(c, (int)k);
 }
 
@@ -1541,14 +1817,20 @@ static void spline_jac_config(void *arg, int64_t kk) {
     const int64_t i = a0 + li;
     const int te = tab_emb(c, c->types[i]);
     const or_spline *emb = &c->tab[te];
-    const double lo = emb->x[0], hi = emb->x[emb->n - 1];
+    /* the gauge (identity outside EAM): clamp on A rho against A [lo, hi];
+     * a clamped atom's energy keeps S (A rho - clamped) from the
+     * compensated pairs, whose sum over the neighbours is S A rho when the
+     * pair and density spans agree (the only case taken here) */
+    const double gA = c->gA, gS = c->gS[c->types[i]];
+    const double rr = gA * rho[li];
+    const double lo = gA * emb->x[0], hi = gA * emb->x[emb->n - 1];
     double rb = rho[li], over = 0.0;
-    if (rb > hi) {
-      over = 2.0 * KDUMMY * 10.0 * (rb - hi);
-      rb = hi;
-    } else if (rb < lo) {
-      over = -2.0 * KDUMMY * 10.0 * (lo - rb);
-      rb = lo;
+    if (rr > hi) {
+      over = 2.0 * KDUMMY * 10.0 * (rr - hi) * gA;
+      rb = hi / gA;
+    } else if (rr < lo) {
+      over = -2.0 * KDUMMY * 10.0 * (lo - rr) * gA;
+      rb = lo / gA;
     }
     const int clamped = over != 0.0;
     double Fv, Fd, F2;
@@ -1568,8 +1850,12 @@ static void spline_jac_config(void *arg, int64_t kk) {
         if (col < 0) continue;
         const double dr = drho[li * D + col];
         if (dr == 0.0) continue;
-        if (clamped) J[l_row * D + col] += over * dr;
-        else J[e_row * D + col] += c->ew * gradF[li] * dr;
+        if (clamped) {
+          J[l_row * D + col] += over * dr;
+          if (gS != 0.0) J[e_row * D + col] += c->ew * (gS * gA) * dr;
+        } else {
+          J[e_row * D + col] += c->ew * gradF[li] * dr;
+        }
       }
     if (clamped)
       for (int t = dens_t0; t < dens_t0 + c->ntypes; ++t)
@@ -1917,7 +2203,22 @@ static void sw_jac_config(void *arg, int64_t kk) {
   }
 }
 
+/* the analytic chain of a gauge takes pair and density spans that agree
+ * wherever a slope compensates a pair */
+static int gauge_spans_match(const or_ctx *c) {
+  for (int ti = 0; ti < c->ntypes; ++ti)
+    for (int tj = 0; tj < c->ntypes; ++tj) {
+      if (c->gS[ti] == 0.0) continue;
+      const or_spline *p = &c->tab[tab_pair(c, ti, tj)], *g = &c->tab[tab_dens(c, tj)];
+      if (p->x[0] != g->x[0] || p->x[p->n - 1] != g->x[g->n - 1]) return 0;
+    }
+  return 1;
+}
+
+int or_gauge_spans_match(const or_ctx *c) { return gauge_spans_match(c); }
+
 int or_jacobian_analytic(or_ctx *c, const double *x, double *J) {
+  if (c->gauge && !gauge_spans_match(c)) return -3;
   or_scatter_params(c, x);
   ensure_nl(c);
   const int D = or_param_count(c);

@@ -98,6 +98,20 @@ void or_jacobian_fd_cols(or_ctx *c, const double *x, int col0, int ncols,
                          double *Jcols);
 int or_jacobian_analytic(or_ctx *c, const double *x, double *J);
 
+/* EAM gauge (src/core/rescale.cpp, include/forcesmith/core/
+ * scale_potentials.hpp): density scale A, per-type embedding slope S and
+ * intercept F0 (ntypes each; NULL = zeros). or_set_gauge: -1 for a model
+ * other than EAM, -2 for A not finite or 0. or_jacobian_analytic returns -3
+ * under a gauge whose slopes compensate pairs of another span. */
+int or_set_gauge(or_ctx *c, double A, const double *S, const double *F0);
+void or_get_gauge(const or_ctx *c, double *A, double *S, double *F0);
+int or_gauge_spans_match(const or_ctx *c);
+void or_emb_gauged(const or_ctx *c, int t, double rho, double *F, double *dF);
+void or_rho_stats(or_ctx *c, double *rmin, double *rmax, double *rmean, int64_t *count);
+double or_rescale_rho_axis(or_ctx *c);
+void or_embed_shift(or_ctx *c, const double *rho_ref);
+double or_rescale_eam(or_ctx *c, double *rho_ref);
+
 /* Spline primitives (SplinePotential), exposed for unit tests. */
 int or_spline_slopes(int n, const double *x, const double *y, double *s);
 void or_spline_eval(int n, const double *x, const double *y, const double *s,
