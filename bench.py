@@ -75,6 +75,8 @@ def parse():
     ap.add_argument("--de-plain-seconds", type=float, default=0.0,
                     help="surrogate run: the plain run's wall time")
     ap.add_argument("--de-lanes", type=int, default=4)
+    ap.add_argument("--lm-iterations", type=int, default=6,
+                    help="fs_lm_solve iterations timed in the default line (0: none)")
     ap.add_argument("--de-sample-generations", type=int, default=10,
                     help="generations of the DE sample in the default line (0: none)")
     ap.add_argument("--de-plain-precision", default="fp64", choices=["mixed", "fp64"],
@@ -521,6 +523,18 @@ def main():
         "step_bytes_alg": b_alg,
         "step_hbm_frac": b_alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
     }
+    if a.lm_iterations > 0:
+        # EigenLMSolver's loop (fs_lm_solve) from x: device calls against the
+        # host's D x D trust-region algebra
+        _, status, counts, hist = g.lm_solve(x, max_iter=a.lm_iterations)
+        tm = g.lm_timing()
+        out["lm_solve"] = {"iterations": a.lm_iterations, "status": status,
+                           "nfev": counts[0], "njev": counts[1], "accepted": counts[2],
+                           "cost_first": float(hist[0]) if len(hist) else None,
+                           "cost_last": float(hist[len(hist) - 1]) if len(hist) else None,
+                           **tm, "host_algebra_share": tm["host_algebra_ms"] / tm["total_ms"]}
+        log(f"lm_solve {a.lm_iterations} iterations: {tm['total_ms']:.1f} ms, host algebra "
+            f"{tm['host_algebra_ms']:.2f} ms ({100 * tm['host_algebra_ms'] / tm['total_ms']:.1f} %)")
     if kind == "eam" and a.de_sample_generations > 0:
         out["de"] = de_sample(w, P, a.de_sample_generations, a.de_np, a.de_lanes)
     if not a.no_cpu_baseline:

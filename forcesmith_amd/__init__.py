@@ -39,7 +39,7 @@ EXPORTS = (
     "fs_eval_forces", "fs_eval_config", "fs_eval_residual", "fs_eval_jacobian", "fs_eval_step", "fs_normal_equations", "fs_normal_equations_noalloc",
     "fs_lm_minimize", "fs_lm_solve", "fs_normal_equations_solve", "fs_lm_step", "fs_population_cost", "fs_de_minimize", "fs_de_last_stats", "fs_device_buffer", "fs_synchronize",
     "fs_eam_set_gauge", "fs_eam_get_gauge", "fs_eam_rho_stats", "fs_eam_rescale_rho_axis",
-    "fs_eam_embed_shift", "fs_eam_rescale",
+    "fs_eam_embed_shift", "fs_eam_rescale", "fs_lm_last_timing",
     "fs_set_launch", "fs_set_jacobian_tile", "fs_get_launch", "fs_autotune", "fs_time_step", "fs_mlp_forward", "fs_mlp_loss_grad",
 )
 
@@ -134,6 +134,7 @@ def load_library(path: str = LIB_PATH):
         "fs_eam_rescale_rho_axis": (ct.c_int, [vp, _dp]),
         "fs_eam_embed_shift": (ct.c_int, [vp, _dp]),
         "fs_eam_rescale": (ct.c_int, [vp, _dp, _dp]),
+        "fs_lm_last_timing": (ct.c_int, [vp, _dp]),
         "fs_device_buffer": (ct.c_int, [vp, ct.c_int32, ct.POINTER(vp), _i64p]),
         "fs_synchronize": (ct.c_int, [vp]),
         "fs_set_launch": (ct.c_int, [vp, ct.c_int32, ct.c_int32]),
@@ -383,6 +384,12 @@ class GpuCalculator:
         self._check(self._lib.fs_lm_solve(self._h, ct.byref(o), _p(x, _dp), ct.byref(st), _p(cnt, _i32p),
                                           _p(hist, _dp)))
         return x, int(st.value), tuple(int(v) for v in cnt), hist
+
+    def lm_timing(self) -> dict:
+        """Wall time of the last lm_solve (ms): whole, device calls, host algebra."""
+        ms = np.zeros(3)
+        self._check(self._lib.fs_lm_last_timing(self._h, _p(ms, _dp)))
+        return {"total_ms": float(ms[0]), "device_ms": float(ms[1]), "host_algebra_ms": float(ms[2])}
 
     def population_cost(self, X):
         X = np.ascontiguousarray(X, dtype=np.float64)

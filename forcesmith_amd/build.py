@@ -25,7 +25,9 @@ COMMON = [
 # host geometry (box inverse) must match the oracle bit for bit
 PER_FILE = {
     "context.cpp": ["-ffp-contract=off", "-x", "hip"],
-    "solver.cpp": ["-x", "hip"],
+    # the LM trust-region algebra (lm_tr.hpp) runs on the host: AVX2 + FMA
+    # (x86-64-v3; every EPYC generation has it) for its factorizations
+    "solver.cpp": ["-x", "hip", "-Xarch_host", "-march=x86-64-v3"],
     "step.cpp": ["-x", "hip"],
     "gauge.cpp": ["-x", "hip"],
     "de.cpp": ["-ffp-contract=off", "-x", "hip"],

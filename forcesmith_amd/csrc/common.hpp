@@ -316,6 +316,8 @@ struct Context {
   // driver or fs_autotune has timed them (1 meanwhile)
   int pop_members = 0;
   fs_de_stats de_stats{};          // of the last fs_de_minimize call
+  // of the last fs_lm_solve: whole solve, device calls, host algebra (ms)
+  double lm_timing[3] = {0.0, 0.0, 0.0};
   // the residual + Jacobian step captured as a graph (step.cpp): parameters
   // from step_x, keyed on the buffer addresses and launch shapes it holds
   hipGraph_t step_graph = nullptr;

@@ -895,6 +895,13 @@ int fs_eam_rescale(fs_ctx *ctx, double *a, double *rho_ref) {
   });
 }
 
+int fs_lm_last_timing(fs_ctx *ctx, double *ms) {
+  return guarded(ctx, [&](Context &c) {
+    FS_REQUIRE(ms, FS_ERR_ARG, "null output");
+    for (int k = 0; k < 3; ++k) ms[k] = c.lm_timing[k];
+  });
+}
+
 int fs_de_last_stats(fs_ctx *ctx, fs_de_stats *out) {
   return guarded(ctx, [&](Context &c) {
     FS_REQUIRE(out, FS_ERR_ARG, "null output");

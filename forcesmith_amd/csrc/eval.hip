@@ -968,8 +968,13 @@ __device__ inline void pop_add(PopAcc<AT> &A, FT f0, FT f1, FT f2, double ep, do
 // FP32 pairs with the FP64 path only for pairs at an exact span edge, as the
 // residual kernels. LDS: the members' FP32 interval forms (FP32 mode) or knot
 // values and slopes (FP64 mode), when they fit the staging budget.
+// Mixed precision asks for 3 waves per SIMD: at 4 members per block the
+// compiler's own pick (191 VGPRs, 2 waves) made generations 7 % slower at
+// C5's shape; the FP64 form measured 13 % slower at 3 waves (more spills),
+// so it keeps the compiler's choice.
 template <int LPA, bool F64, int M>
-__global__ __launch_bounds__(256) void k_pop_force(EvalArgs a, PopArgs p) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(F64 ? 1 : 3))) void k_pop_force(
+    EvalArgs a, PopArgs p) {
   using AT = typename std::conditional<F64, double, float>::type;
   constexpr int APB = 256 / LPA;
   extern __shared__ float4 smem[];

@@ -4,6 +4,8 @@
 // autotuner. Each step's arithmetic runs on the GPU; the host only carries
 // scalars (costs, the LM damping, DE selection) between launches.
 #include "common.hpp"
+
+#include <chrono>
 #include "lm_tr.hpp"
 
 #include <algorithm>
@@ -163,22 +165,42 @@ int fs_lm_solve_impl(Context &c, const fs_lm_opts &o, double *x, int *status, in
                      double *hist) {
   const lmtr::Opts opts{o.max_iter, o.maxfev, o.xtol, o.ftol, o.gtol, o.factor};
   const int n = c.D;
+  // wall time of the device calls (each ends in a stream synchronisation)
+  // against the whole solve: the rest is the host's D x D algebra
+  using clk = std::chrono::steady_clock;
+  double dev_s = 0.0;
+  auto timed = [&](auto &&f) {
+    const auto t0 = clk::now();
+    f();
+    dev_s += std::chrono::duration<double>(clk::now() - t0).count();
+  };
   auto cost = [&](const double *xv) {
-    scatter_host(c, xv);
-    return cost_now(c);
+    double v = 0.0;
+    timed([&] {
+      scatter_host(c, xv);
+      v = cost_now(c);
+    });
+    return v;
   };
   auto normal = [&](double *A, double *g) {
-    launch_jacobian(c);
-    launch_normal_equations(c);
-    FS_HIP(hipMemcpyAsync(A, c.JtJ.p, sizeof(double) * size_t(n) * n, hipMemcpyDeviceToHost,
-                          c.stream));
-    FS_HIP(hipMemcpyAsync(g, c.Jtr.p, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
-    FS_HIP(hipStreamSynchronize(c.stream));
+    timed([&] {
+      launch_jacobian(c);
+      launch_normal_equations(c);
+      FS_HIP(hipMemcpyAsync(A, c.JtJ.p, sizeof(double) * size_t(n) * n, hipMemcpyDeviceToHost,
+                            c.stream));
+      FS_HIP(hipMemcpyAsync(g, c.Jtr.p, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+      FS_HIP(hipStreamSynchronize(c.stream));
+    });
   };
   // lmtr::minimize calls normal() right after the cost of the accepted x
   // (the initial point or the trial it just accepted), so the device
   // residual and per-atom state are at x
+  const auto t_all = clk::now();
   *status = lmtr::minimize(n, c.R, opts, x, cost, normal, counts, hist);
+  const double all_s = std::chrono::duration<double>(clk::now() - t_all).count();
+  c.lm_timing[0] = 1e3 * all_s;
+  c.lm_timing[1] = 1e3 * dev_s;
+  c.lm_timing[2] = 1e3 * (all_s - dev_s);
   // leave the device state at the returned parameters
   scatter_host(c, x);
   evaluate_for_solver(c);

@@ -314,6 +314,11 @@ int fs_synchronize(fs_ctx *ctx);
  * fastest (lanes per atom and blocks per CU over the residual, then the
  * Jacobian tile size and the Jacobian passes' blocks per CU). */
 int fs_set_launch(fs_ctx *ctx, int32_t lanes_per_atom, int32_t blocks_per_cu);
+/* Wall time of the last fs_lm_solve in ms: ms[0] the whole solve, ms[1] its
+ * device calls (residual norms, Jacobians, normal equations, each ending in
+ * a stream synchronisation, copies included), ms[2] the rest: the host's
+ * D x D trust-region algebra (QR-free lmpar on the normal equations). */
+int fs_lm_last_timing(fs_ctx *ctx, double *ms);
 /* Atoms per Jacobian tile: 0 (the default) picks it from the atom count, else
  * 4, 8, 16, 32 or 64. Unless a size is fixed here (non-zero), fs_autotune
  * also times the tile sizes over the whole Jacobian and keeps the fastest.
