@@ -271,6 +271,8 @@ struct Context {
   DBuf<double> residual, cost, cost_part;
   DBuf<double> pairbuf; // SW triplet forces per pair
   DBuf<double> sw_pv;   // SW per-pair scalars, 8 per pair
+  // SW triplet records, 8 per pair: unit vector, 1/r, h, h', dh/dgamma, dh/da2
+  DBuf<double> sw_tri;
   // SW parameter / pair-list generation (bumped on every upload of the SW
   // parameters and every neighbour build) and the one sw_pv was computed for
   uint64_t sw_gen = 1, sw_pv_gen = 0;

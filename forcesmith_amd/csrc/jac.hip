@@ -1444,6 +1444,7 @@ struct SwJArgs {
   const int32_t *rev;
   const double *sw, *lam;
   const double *pv;       // 8 per pair: v2, v2', h, h' and the gamma / a2 tangents of h, h'
+  const double *tri;      // 8 per pair: unit vector, 1/r, h, h', dh/dgamma, dh/da2
   double *pt2;            // 12 per pair: (tv, tdv) of v2 w.r.t. the two-body fields (k_sw_pair_tan)
   const int32_t *col_sw;  // npair*8 -> column or -1
   const int32_t *col_lam; // nlam -> column or -1
@@ -1466,17 +1467,8 @@ __device__ inline double pr64(const double *pd, int64_t p) {
   return sqrt(r2);
 }
 
-// j force of the triplet (i; j, k) (three_body_forces, kernels.hpp:40-56),
-// multilinear in (f1, df1), (f2), (g, dg): its tangent is the sum of three
-// calls with one factor group replaced by its tangent
-__device__ inline void tb_j(const double *d1, const double *d2, double i1, double i2, double c,
-                            double f1, double df1, double f2, double g, double dg, double *fj) {
-  const double i12 = i1 * i2;
-  const double s1 = df1 * i1 * f2 * g, s3 = f1 * f2 * dg;
-  for (int x = 0; x < 3; ++x) fj[x] = -s1 * d1[x] - s3 * (i12 * d2[x] - c * i1 * i1 * d1[x]);
-}
-
 constexpr int kSwJLanes = 16; // lanes per centre atom (a divisor of 64)
+constexpr int kSwStage = 32;  // neighbour records per centre staged in LDS
 
 // per pair (i -> j): the two-body tangents of its slot (ti, tj), every field
 // (zeros for an r < 1e-14 pair)
@@ -1512,22 +1504,75 @@ __device__ inline double sw_group_sum(double v) {
 // the per-pair slot of (i -> j). The centre's tangent force is the pair terms
 // minus sum_j T_j (a triplet's forces sum to zero for every parameter value,
 // so their tangents do too) and the triplet virial tangent sum_j d_j (x) T_j.
+//
+// The j force of a triplet (three_body_forces, kernels.hpp:40-56) in unit
+// vectors u1, u2 and cosine c is -f1' f2 g u1 - f1 f2 g' (u2 - c u1) / r_ij,
+// multilinear in the factor groups (f1, f1'), f2 and (g, g'): its tangent is
+// the sum of the three forms with one group replaced by its tangent, whose
+// u1 and (u2 - c u1) coefficients are summed over k as scalars (Ss, Sqc) and
+// one vector (V), giving T_j once per bond. FP64 throughout; the bond's unit
+// vector and 1/r come from the per-pair records, so the loop over k has no
+// square root or division.
 __global__ __launch_bounds__(256) void k_sw_jac_centre(SwJArgs a) {
+  constexpr int kGroups = 256 / kSwJLanes;
+  // per centre, the first kSwStage neighbours' records: unit vector and h,
+  // the column's tangent of h, and the neighbour's type (-1: no triplets);
+  // the lambda table and which lambda is the column
+  __shared__ double4 s_u[kGroups][kSwStage];
+  __shared__ double s_t[kGroups][kSwStage];
+  __shared__ int s_ty[kGroups][kSwStage];
+  __shared__ double s_lam[kMaxTypes * kMaxTypes * (kMaxTypes + 1) / 2];
+  __shared__ int s_lamc[kMaxTypes * kMaxTypes * (kMaxTypes + 1) / 2];
   const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t i = g / kSwJLanes;
   const int l = int(g % kSwJLanes);
+  const int grp = int(threadIdx.x) / kSwJLanes;
   const int col = blockIdx.y;
   const bool live = i < a.N; // every lane takes part in the group sums
   double F[3] = {0, 0, 0}, E = 0.0, sg[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   double *pt = a.ptan + int64_t(col) * a.P * 3;
   // does the column reach the triplets at all (gamma or a2 of a slot, or a
-  // lambda)? The two-body columns skip the triplet loop
+  // lambda)? The two-body columns skip the triplet loop; the test depends on
+  // the column only, so every thread of the block takes the same branch
   bool tri = false;
   for (int q = 0; q < a.npair; ++q) tri = tri || a.col_sw[8 * q + 6] == col || a.col_sw[8 * q + 7] == col;
   for (int q = 0; q < a.ntypes * a.npair; ++q) tri = tri || a.col_lam[q] == col;
+  const int ti = live ? a.types[i] : 0;
+  const int64_t p0 = live ? a.pair_off[i] : 0, p1 = live ? a.pair_off[i + 1] : 0;
+  // record of neighbour pk of this centre: unit vector, h, the column's
+  // tangent of h, type (-1 when the pair takes no part in triplets)
+  auto record = [&](int64_t pk, double *u, double &h, double &t, int &ty) {
+    const uint32_t bk = a.bits[pk];
+    const double *r = a.tri + 8 * pk;
+    u[0] = r[0];
+    u[1] = r[1];
+    u[2] = r[2];
+    h = r[4];
+    ty = (bk & kFlagTiny) || (r[4] == 0.0 && r[5] == 0.0) ? -1 : pj_type(bk);
+    t = 0.0;
+    if (ty >= 0) {
+      const int s2 = ord_pair(ti, ty, a.ntypes);
+      if (a.col_sw[8 * s2 + 6] == col) t += r[6];
+      if (a.col_sw[8 * s2 + 7] == col) t += r[7];
+    }
+  };
+  if (tri) {
+    for (int q = threadIdx.x; q < a.ntypes * a.npair; q += blockDim.x) {
+      s_lam[q] = a.lam[q];
+      s_lamc[q] = a.col_lam[q] == col;
+    }
+    const int ns = int(std::min<int64_t>(p1 - p0, kSwStage));
+    for (int k = l; k < ns; k += kSwJLanes) {
+      double u[3], h, t;
+      int ty;
+      record(p0 + k, u, h, t, ty);
+      s_u[grp][k] = make_double4(u[0], u[1], u[2], h);
+      s_t[grp][k] = t;
+      s_ty[grp][k] = ty;
+    }
+    __syncthreads();
+  }
   if (live) {
-    const int ti = a.types[i];
-    const int64_t p0 = a.pair_off[i], p1 = a.pair_off[i + 1];
     for (int64_t pj = p0 + l; pj < p1; pj += kSwJLanes) {
       double T[3] = {0, 0, 0};
       const uint32_t bj = a.bits[pj];
@@ -1535,14 +1580,14 @@ __global__ __launch_bounds__(256) void k_sw_jac_centre(SwJArgs a) {
         const int tj = pj_type(bj);
         const int s1 = ord_pair(ti, tj, a.ntypes);
         const double *d1 = a.pd + 3 * pj;
-        const double r1 = pr64(a.pd, pj);
+        const double *r1p = a.tri + 8 * pj; // unit vector, 1/r, h, h', ...
         // pair term: the column's field among the slot's two-body fields
         int f = -1;
         for (int q = 0; q < 6; ++q)
           if (a.col_sw[8 * s1 + q] == col) f = q;
         if (f >= 0) {
           const double tv = a.pt2[12 * pj + 2 * f], tdv = a.pt2[12 * pj + 2 * f + 1];
-          const double s = tdv / r1;
+          const double s = tdv * r1p[3];
           const double df[3] = {s * d1[0], s * d1[1], s * d1[2]};
           for (int x = 0; x < 3; ++x) F[x] += df[x];
           E += 0.5 * tv;
@@ -1562,45 +1607,44 @@ __global__ __launch_bounds__(256) void k_sw_jac_centre(SwJArgs a) {
             t1 += q1[6];
             td1 += q1[7];
           }
-          const double i1 = 1.0 / r1;
+          const double u1[3] = {r1p[0], r1p[1], r1p[2]}, i1 = r1p[3];
+          double Ss = 0.0, Sqc = 0.0, Se = 0.0, V[3] = {0.0, 0.0, 0.0};
           for (int64_t pk = p0; pk < p1; ++pk) {
             if (pk == pj) continue;
-            const uint32_t bk = a.bits[pk];
-            if (bk & kFlagTiny) continue;
-            const double *q2 = a.pv + 8 * pk;
-            const double h2 = q2[2], dh2 = q2[3];
-            if (h2 == 0.0 && dh2 == 0.0) continue;
-            const int s2 = ord_pair(ti, pj_type(bk), a.ntypes);
-            double t2 = 0.0;
-            if (a.col_sw[8 * s2 + 6] == col) t2 += q2[4];
-            if (a.col_sw[8 * s2 + 7] == col) t2 += q2[6];
-            const int li = ti * a.npair + ord_pair(tj, pj_type(bk), a.ntypes);
-            const bool lam_col = a.col_lam[li] == col;
+            const int k = int(pk - p0);
+            double u2[3], h2, t2;
+            int tk;
+            if (k < kSwStage) {
+              const double4 U = s_u[grp][k];
+              u2[0] = U.x;
+              u2[1] = U.y;
+              u2[2] = U.z;
+              h2 = U.w;
+              t2 = s_t[grp][k];
+              tk = s_ty[grp][k];
+            } else {
+              record(pk, u2, h2, t2, tk);
+            }
+            if (tk < 0) continue;
+            const int li = ti * a.npair + ord_pair(tj, tk, a.ntypes);
+            const bool lam_col = s_lamc[li] != 0;
             if (t1 == 0.0 && td1 == 0.0 && t2 == 0.0 && !lam_col) continue;
-            const double i2 = 1.0 / pr64(a.pd, pk);
-            const double *d2 = a.pd + 3 * pk;
-            const double lam = a.lam[li];
-            const double cs = (d1[0] * d2[0] + d1[1] * d2[1] + d1[2] * d2[2]) * i1 * i2;
+            const double lam = s_lam[li];
+            const double cs = u1[0] * u2[0] + u1[1] * u2[1] + u1[2] * u2[2];
             const double cp = cs + 1.0 / 3.0;
             const double w = lam * cp * cp, dw = 2.0 * lam * cp;
             const double tg = lam_col ? cp * cp : 0.0, tdg = lam_col ? 2.0 * cp : 0.0;
-            // only the factor groups this column moves (the others add zeros)
-            double f[3];
-            if (t1 != 0.0 || td1 != 0.0) {
-              tb_j(d1, d2, i1, i2, cs, t1, td1, h2, w, dw, f);
-              for (int x = 0; x < 3; ++x) T[x] += f[x];
-            }
-            if (t2 != 0.0) {
-              tb_j(d1, d2, i1, i2, cs, h1, dh1, t2, w, dw, f);
-              for (int x = 0; x < 3; ++x) T[x] += f[x];
-            }
-            if (lam_col) {
-              tb_j(d1, d2, i1, i2, cs, h1, dh1, h2, tg, tdg, f);
-              for (int x = 0; x < 3; ++x) T[x] += f[x];
-            }
-            // each triplet is met from both of its bonds
-            E += 0.5 * (t1 * h2 * w + h1 * t2 * w + h1 * h2 * tg);
+            // groups (t1, td1) for (f1, f1'), t2 for f2, (tg, tdg) for (g, g')
+            const double sk = td1 * h2 * w + dh1 * (t2 * w + h2 * tg);
+            const double qk = t1 * h2 * dw + h1 * (t2 * dw + h2 * tdg);
+            Ss += sk;
+            Sqc += qk * cs;
+            for (int x = 0; x < 3; ++x) V[x] += qk * u2[x];
+            Se += t1 * h2 * w + h1 * (t2 * w + h2 * tg);
           }
+          const double cu = Ss - i1 * Sqc;
+          for (int x = 0; x < 3; ++x) T[x] = -cu * u1[x] - i1 * V[x];
+          E += 0.5 * Se; // each triplet is met from both of its bonds
           for (int x = 0; x < 3; ++x) F[x] -= T[x];
           for (int x = 0; x < 3; ++x)
             for (int y = 0; y < 3; ++y) sg[3 * x + y] += d1[x] * T[y];
@@ -1781,6 +1825,7 @@ void launch_jacobian(Context &c) {
     s.lam = c.lam.p;
     launch_sw_pair_terms(c);
     s.pv = c.sw_pv.p;
+    s.tri = c.sw_tri.p;
     c.sw_pt2.alloc(12 * size_t(std::max<int64_t>(c.P, 1)));
     s.pt2 = c.sw_pt2.p;
     std::vector<int32_t> csw(8 * c.npair), clam(c.h_lam.size());

@@ -14,7 +14,12 @@
 //
 // Reference: StiwebForceCalculator::eval_forces (src/force/stiweb_force.cpp:
 // 241-271), v2_dv2 / h_dh (:24-54), three_body_forces (kernels.hpp:40-56).
-// The SW arithmetic (pow, exp) runs in FP64.
+// Everything runs in FP64, in both precision modes. The radial factors (pow,
+// exp), the pair terms, the bond's unit vector and 1/r are formed once per
+// pair, so the triplet loop has no square root or division: per triplet it
+// forms the cosine and the angular factor and adds four scalar sums, from
+// which the lane's T_j follows once per bond (the j-constant factors h(r_ij),
+// h'(r_ij), 1/r_ij are taken out of the sum over k).
 #include "common.hpp"
 #include "spline.hpp"
 
@@ -82,6 +87,7 @@ __global__ void k_sw_rev(int64_t N, const int64_t *__restrict__ pair_off,
 }
 
 constexpr int kSwLanes = 16; // lanes per centre atom (a divisor of 64)
+constexpr int kSwStage = 32; // neighbour records per centre staged in LDS
 
 // sum over the kSwLanes lanes of one centre's group (all lanes of the wave
 // take part)
@@ -98,7 +104,8 @@ __global__ __launch_bounds__(256) void k_sw_pair_terms(int64_t N, const int64_t 
                                                        const double *__restrict__ pd,
                                                        const uint32_t *__restrict__ bits,
                                                        const double *__restrict__ sw, int ntypes,
-                                                       double *__restrict__ pv) {
+                                                       double *__restrict__ pv,
+                                                       double *__restrict__ tri) {
   const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t i = g / kSwLanes;
   const int l = int(g % kSwLanes);
@@ -107,11 +114,16 @@ __global__ __launch_bounds__(256) void k_sw_pair_terms(int64_t N, const int64_t 
   for (int64_t p = pair_off[i] + l; p < pair_off[i + 1]; p += kSwLanes) {
     double o[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     const uint32_t b = bits[p];
+    double u[4] = {0, 0, 0, 0};
     if (!(b & kFlagTiny)) {
       const double d0 = pd[3 * p], d1 = pd[3 * p + 1], d2 = pd[3 * p + 2];
       double r2 = d0 * d0 + d1 * d1;
       r2 = r2 + d2 * d2;
       const double r = sqrt(r2);
+      u[3] = 1.0 / r;
+      u[0] = d0 * u[3];
+      u[1] = d1 * u[3];
+      u[2] = d2 * u[3];
       const SwP P = load_sw(sw, ord_pair(ti, pj_type(b), ntypes));
       v2_dv2(r, P, o[0], o[1]);
       h_dh(r, P, o[2], o[3]);
@@ -128,6 +140,12 @@ __global__ __launch_bounds__(256) void k_sw_pair_terms(int64_t N, const int64_t 
     }
     double *q = pv + 8 * p;
     for (int k = 0; k < 8; ++k) q[k] = o[k];
+    double *t = tri + 8 * p;
+    for (int k = 0; k < 4; ++k) t[k] = u[k];
+    t[4] = o[2];
+    t[5] = o[3];
+    t[6] = o[4];
+    t[7] = o[6];
   }
 }
 
@@ -141,6 +159,7 @@ struct SwArgs {
   const double *sw, *lam;
   int ntypes, npair;
   const double *pv; // 8 per pair (k_sw_pair_terms)
+  const double *tri; // 8 per pair: unit vector, 1/r, h, h', dh/dgamma, dh/da2
   double *pairbuf;  // 3 per pair: force on the neighbour from triplets centred here
   double *fcentre;  // 3 per atom
   double *e_atom;   // per atom energy
@@ -152,44 +171,50 @@ struct SwArgs {
   double *forces, *residual, *tile_part, *lim_atom;
 };
 
-__device__ inline double pr(const double *pd, int64_t p) {
-  const double d0 = pd[3 * p], d1 = pd[3 * p + 1], d2 = pd[3 * p + 2];
-  double r2 = d0 * d0 + d1 * d1;
-  r2 = r2 + d2 * d2;
-  return sqrt(r2);
-}
-
-// force on j of the triplet (i; j, k) with bond vectors d1 = d_ij, d2 = d_ik,
-// radial factors f1 = h(r1), f2 = h(r2) and angular factor g = w(cos) (the j
-// force of three_body_forces, kernels.hpp:40-56; by symmetry the same formula
-// with j and k exchanged is its k force)
-__device__ inline void three_body_j(const double *d1, const double *d2, double inv_r1, double inv_r2,
-                                    double c, double f1, double df1, double f2, double g, double dg,
-                                    double *fj) {
-  const double inv_r1r2 = inv_r1 * inv_r2;
-  const double s1 = df1 * inv_r1 * f2 * g, s3 = f1 * f2 * dg;
-  for (int x = 0; x < 3; ++x) fj[x] = -s1 * d1[x] - s3 * (inv_r1r2 * d2[x] - c * inv_r1 * inv_r1 * d1[x]);
-}
-
+// The force on j of the triplet (i; j, k) (three_body_forces,
+// kernels.hpp:40-56), with unit vectors u1, u2 of d_ij, d_ik, cosine c, radial
+// factors f1 = h(r_ij), f2 = h(r_ik) and angular factor g = w(c), is
+//   f_j = -f1' f2 g u1 - f1 f2 g' (u2 - c u1) / r_ij,
+// so T_j = sum_k f_j = -(h1' Sa - h1 Sbc / r_ij) u1 - h1 V / r_ij with
+// Sa = sum h2 w, Sbc = sum h2 w' c, V = sum h2 w' u2; the triplet energy seen
+// from bond j is h1 Sa / 2 (each triplet is met from both of its bonds).
+// The first kSwStage neighbours' records of each centre (unit vector, h and
+// type, -1 for a pair that takes no part in triplets) and the lambda table
+// are staged in LDS.
 __global__ __launch_bounds__(256) void k_sw_centre(SwArgs a) {
+  constexpr int kGroups = 256 / kSwLanes;
+  __shared__ double4 s_u[kGroups][kSwStage];
+  __shared__ int s_ty[kGroups][kSwStage];
+  __shared__ double s_lam[kMaxTypes * kMaxTypes * (kMaxTypes + 1) / 2];
   const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t i = g / kSwLanes;
   const int l = int(g % kSwLanes);
+  const int grp = int(threadIdx.x) / kSwLanes;
   const bool live = i < a.N; // every lane takes part in the group sums
   double F[3] = {0, 0, 0}, E = 0.0, sg[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  const int64_t p0 = live ? a.pair_off[i] : 0, p1 = live ? a.pair_off[i + 1] : 0;
+  auto record = [&](int64_t pk, double4 &u, int &ty) {
+    const uint32_t bk = a.bits[pk];
+    const double *r = a.tri + 8 * pk;
+    u = make_double4(r[0], r[1], r[2], r[4]);
+    ty = (bk & kFlagTiny) || (r[4] == 0.0 && r[5] == 0.0) ? -1 : pj_type(bk);
+  };
+  for (int q = threadIdx.x; q < a.ntypes * a.npair; q += blockDim.x) s_lam[q] = a.lam[q];
+  for (int k = l; k < int(std::min<int64_t>(p1 - p0, kSwStage)); k += kSwLanes)
+    record(p0 + k, s_u[grp][k], s_ty[grp][k]);
+  __syncthreads();
   if (live) {
     const int ti = a.types[i];
-    const int64_t p0 = a.pair_off[i], p1 = a.pair_off[i + 1];
     for (int64_t pj = p0 + l; pj < p1; pj += kSwLanes) {
       double T[3] = {0, 0, 0};
       const uint32_t bj = a.bits[pj];
       if (!(bj & kFlagTiny)) {
         const double *d1 = a.pd + 3 * pj;
-        const double r1 = pr(a.pd, pj);
         const double *q1 = a.pv + 8 * pj;
+        const double *t1 = a.tri + 8 * pj;
         const double v2 = q1[0], dv2 = q1[1], h1 = q1[2], dh1 = q1[3];
         // pair term
-        const double s = dv2 / r1;
+        const double s = dv2 * t1[3];
         const double f[3] = {s * d1[0], s * d1[1], s * d1[2]};
         for (int k = 0; k < 3; ++k) F[k] += f[k];
         E += 0.5 * v2;
@@ -198,25 +223,33 @@ __global__ __launch_bounds__(256) void k_sw_centre(SwArgs a) {
         // triplets (i; j, k) over every other neighbour k
         if (h1 != 0.0 || dh1 != 0.0) {
           const int tj = pj_type(bj);
-          const double inv_r1 = 1.0 / r1;
+          const double u1[3] = {t1[0], t1[1], t1[2]}, i1 = t1[3];
+          double Sa = 0.0, Sbc = 0.0, V[3] = {0.0, 0.0, 0.0};
           for (int64_t pk = p0; pk < p1; ++pk) {
             if (pk == pj) continue;
-            const uint32_t bk = a.bits[pk];
-            if (bk & kFlagTiny) continue;
-            const double *q2 = a.pv + 8 * pk;
-            const double h2 = q2[2], dh2 = q2[3];
-            if (h2 == 0.0 && dh2 == 0.0) continue;
-            const double inv_r2 = 1.0 / pr(a.pd, pk);
-            const double *d2 = a.pd + 3 * pk;
-            const double lam = a.lam[ti * a.npair + ord_pair(tj, pj_type(bk), a.ntypes)];
-            const double cs = (d1[0] * d2[0] + d1[1] * d2[1] + d1[2] * d2[2]) * inv_r1 * inv_r2;
+            const int k = int(pk - p0);
+            double4 u2;
+            int tk;
+            if (k < kSwStage) {
+              u2 = s_u[grp][k];
+              tk = s_ty[grp][k];
+            } else {
+              record(pk, u2, tk);
+            }
+            if (tk < 0) continue;
+            const double lam = s_lam[ti * a.npair + ord_pair(tj, tk, a.ntypes)];
+            const double cs = u1[0] * u2.x + u1[1] * u2.y + u1[2] * u2.z;
             const double cp13 = cs + 1.0 / 3.0;
-            const double w = lam * cp13 * cp13, dw = 2.0 * lam * cp13;
-            E += 0.5 * (h1 * h2 * w); // each triplet is met from both of its bonds
-            double fj[3];
-            three_body_j(d1, d2, inv_r1, inv_r2, cs, h1, dh1, h2, w, dw, fj);
-            for (int k = 0; k < 3; ++k) T[k] += fj[k];
+            const double hw = u2.w * (lam * cp13 * cp13), hdw = u2.w * (2.0 * lam * cp13);
+            Sa += hw;
+            Sbc += hdw * cs;
+            V[0] += hdw * u2.x;
+            V[1] += hdw * u2.y;
+            V[2] += hdw * u2.z;
           }
+          const double hi = h1 * i1, cu = dh1 * Sa - hi * Sbc;
+          for (int k = 0; k < 3; ++k) T[k] = -cu * u1[k] - hi * V[k];
+          E += 0.5 * (h1 * Sa);
           for (int k = 0; k < 3; ++k) F[k] -= T[k];
           for (int x = 0; x < 3; ++x)
             for (int y = 0; y < 3; ++y) sg[3 * x + y] += d1[x] * T[y];
@@ -294,6 +327,7 @@ SwArgs make_sw_args(Context &c) {
   a.ntypes = c.ntypes;
   a.npair = c.npair;
   a.pv = c.sw_pv.p;
+  a.tri = c.sw_tri.p;
   a.pairbuf = c.pairbuf.p;
   a.fcentre = c.Fpp.p;
   a.e_atom = c.e_atom.p;
@@ -314,6 +348,7 @@ SwArgs make_sw_args(Context &c) {
 
 void launch_sw_pair_terms(Context &c) {
   c.sw_pv.alloc(8 * size_t(std::max<int64_t>(c.P, 1)));
+  c.sw_tri.alloc(8 * size_t(std::max<int64_t>(c.P, 1)));
   if (c.N == 0) return;
   // the residual and the Jacobian at one parameter vector share the factors
   if (c.sw_pv_gen == c.sw_gen) return;
@@ -321,7 +356,7 @@ void launch_sw_pair_terms(Context &c) {
   const int64_t nthr = c.N * kSwLanes;
   hipLaunchKernelGGL(k_sw_pair_terms, dim3(unsigned((nthr + 255) / 256)), dim3(256), 0, c.stream,
                      c.N, c.pair_off.p, c.types.p, c.pair_d.p, c.pair_bits.p, c.sw.p, c.ntypes,
-                     c.sw_pv.p);
+                     c.sw_pv.p, c.sw_tri.p);
   FS_HIP(hipGetLastError());
 }
 
@@ -329,9 +364,10 @@ void launch_sw_eval(Context &c) {
   FS_REQUIRE(c.tile_atoms <= 64, FS_ERR_UNSUPPORTED, "SW tiles hold at most 64 atoms");
   launch_sw_pair_terms(c);
   const SwArgs a = make_sw_args(c);
-  if (c.N > 0)
-    hipLaunchKernelGGL(k_sw_centre, dim3(unsigned((c.N * kSwLanes + 255) / 256)), dim3(256), 0,
-                       c.stream, a);
+  if (c.N > 0) {
+    const dim3 grid(unsigned((c.N * kSwLanes + 255) / 256));
+    hipLaunchKernelGGL(k_sw_centre, grid, dim3(256), 0, c.stream, a);
+  }
   FS_HIP(hipGetLastError());
   if (c.ntiles > 0)
     hipLaunchKernelGGL(k_sw_tiles, dim3(c.ntiles), dim3(64), 0, c.stream, a);
