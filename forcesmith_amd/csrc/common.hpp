@@ -276,7 +276,7 @@ struct Context {
   // SW parameter / pair-list generation (bumped on every upload of the SW
   // parameters and every neighbour build) and the one sw_pv was computed for
   uint64_t sw_gen = 1, sw_pv_gen = 0;
-  DBuf<double> sw_pt2;  // SW per-pair two-body tangents, 12 per pair
+  DBuf<double2> sw_pt2; // SW two-body tangents, [6 fields][pairs] of (tv, tdv / r)
   bool forces_valid = false;
   // Jacobian
   DBuf<float> J;
@@ -301,7 +301,7 @@ struct Context {
   DBuf<double> jr;   // their FP64 distances
   bool jsort_valid = false;
   DBuf<float> Hs, emb_d, emb_v;
-  DBuf<int32_t> emb_k, sw_csw, sw_clam;
+  DBuf<int32_t> emb_k, sw_csw, sw_clam, sw_tmap;
   DBuf<double> sw_own, sw_ptan, sw_etan;
   DBuf<float> adp_tan; // ADP Jacobian tangents [N][10][D]
   DBuf<double> lm_M, lm_dx, lm_scal;

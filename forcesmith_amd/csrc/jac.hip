@@ -1445,12 +1445,15 @@ struct SwJArgs {
   const double *sw, *lam;
   const double *pv;       // 8 per pair: v2, v2', h, h' and the gamma / a2 tangents of h, h'
   const double *tri;      // 8 per pair: unit vector, 1/r, h, h', dh/dgamma, dh/da2
-  double *pt2;            // 12 per pair: (tv, tdv) of v2 w.r.t. the two-body fields (k_sw_pair_tan)
+  double2 *pt2;           // [6][P]: (tv, tdv / r) of v2 w.r.t. each two-body field (k_sw_pair_tan)
   const int32_t *col_sw;  // npair*8 -> column or -1
   const int32_t *col_lam; // nlam -> column or -1
-  double *own;            // [D][N][3]
-  double *ptan;           // [D][P][3]
-  double *etan;           // [D][N][10]: energy + 9 virial
+  const int32_t *tidx;    // [D]: the column's triplet pass, or -1 for a two-body column
+  const int32_t *tcol;    // [nTri]: the column of each triplet pass
+  const int32_t *tsf;     // [D]: 8 slot + field of a two-body column, -1 for a triplet column
+  double *own;            // [nTri][N][3]
+  double *ptan;           // [nTri][P][3]
+  double *etan;           // [nTri][N][10]: energy + 9 virial
   int64_t P;
   // tiles
   int ntiles;
@@ -1481,13 +1484,16 @@ __global__ __launch_bounds__(256) void k_sw_pair_tan(SwJArgs a) {
   for (int64_t p = a.pair_off[i] + l; p < a.pair_off[i + 1]; p += kSwJLanes) {
     double t[12];
     const uint32_t b = a.bits[p];
+    double ir = 0.0;
     if (b & kFlagTiny) {
       for (int k = 0; k < 12; ++k) t[k] = 0.0;
     } else {
-      v2_tan_all(pr64(a.pd, p), lsw(a.sw, ord_pair(ti, pj_type(b), a.ntypes)), t);
+      const double r = pr64(a.pd, p);
+      ir = 1.0 / r;
+      v2_tan_all(r, lsw(a.sw, ord_pair(ti, pj_type(b), a.ntypes)), t);
     }
-    double *o = a.pt2 + 12 * p;
-    for (int k = 0; k < 12; ++k) o[k] = t[k];
+    // field-major, so a column's pass reads 16 contiguous bytes per pair
+    for (int f = 0; f < 6; ++f) a.pt2[int64_t(f) * a.P + p] = make_double2(t[2 * f], t[2 * f + 1] * ir);
   }
 }
 
@@ -1527,16 +1533,10 @@ __global__ __launch_bounds__(256) void k_sw_jac_centre(SwJArgs a) {
   const int64_t i = g / kSwJLanes;
   const int l = int(g % kSwJLanes);
   const int grp = int(threadIdx.x) / kSwJLanes;
-  const int col = blockIdx.y;
+  const int tp = blockIdx.y, col = a.tcol[tp]; // a triplet column: gamma or a2 of a slot, or a lambda
   const bool live = i < a.N; // every lane takes part in the group sums
   double F[3] = {0, 0, 0}, E = 0.0, sg[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  double *pt = a.ptan + int64_t(col) * a.P * 3;
-  // does the column reach the triplets at all (gamma or a2 of a slot, or a
-  // lambda)? The two-body columns skip the triplet loop; the test depends on
-  // the column only, so every thread of the block takes the same branch
-  bool tri = false;
-  for (int q = 0; q < a.npair; ++q) tri = tri || a.col_sw[8 * q + 6] == col || a.col_sw[8 * q + 7] == col;
-  for (int q = 0; q < a.ntypes * a.npair; ++q) tri = tri || a.col_lam[q] == col;
+  double *pt = a.ptan + int64_t(tp) * a.P * 3;
   const int ti = live ? a.types[i] : 0;
   const int64_t p0 = live ? a.pair_off[i] : 0, p1 = live ? a.pair_off[i + 1] : 0;
   // record of neighbour pk of this centre: unit vector, h, the column's
@@ -1556,22 +1556,20 @@ __global__ __launch_bounds__(256) void k_sw_jac_centre(SwJArgs a) {
       if (a.col_sw[8 * s2 + 7] == col) t += r[7];
     }
   };
-  if (tri) {
-    for (int q = threadIdx.x; q < a.ntypes * a.npair; q += blockDim.x) {
-      s_lam[q] = a.lam[q];
-      s_lamc[q] = a.col_lam[q] == col;
-    }
-    const int ns = int(std::min<int64_t>(p1 - p0, kSwStage));
-    for (int k = l; k < ns; k += kSwJLanes) {
-      double u[3], h, t;
-      int ty;
-      record(p0 + k, u, h, t, ty);
-      s_u[grp][k] = make_double4(u[0], u[1], u[2], h);
-      s_t[grp][k] = t;
-      s_ty[grp][k] = ty;
-    }
-    __syncthreads();
+  for (int q = threadIdx.x; q < a.ntypes * a.npair; q += blockDim.x) {
+    s_lam[q] = a.lam[q];
+    s_lamc[q] = a.col_lam[q] == col;
   }
+  const int ns = int(std::min<int64_t>(p1 - p0, kSwStage));
+  for (int k = l; k < ns; k += kSwJLanes) {
+    double u[3], h, t;
+    int ty;
+    record(p0 + k, u, h, t, ty);
+    s_u[grp][k] = make_double4(u[0], u[1], u[2], h);
+    s_t[grp][k] = t;
+    s_ty[grp][k] = ty;
+  }
+  __syncthreads();
   if (live) {
     for (int64_t pj = p0 + l; pj < p1; pj += kSwJLanes) {
       double T[3] = {0, 0, 0};
@@ -1581,22 +1579,9 @@ __global__ __launch_bounds__(256) void k_sw_jac_centre(SwJArgs a) {
         const int s1 = ord_pair(ti, tj, a.ntypes);
         const double *d1 = a.pd + 3 * pj;
         const double *r1p = a.tri + 8 * pj; // unit vector, 1/r, h, h', ...
-        // pair term: the column's field among the slot's two-body fields
-        int f = -1;
-        for (int q = 0; q < 6; ++q)
-          if (a.col_sw[8 * s1 + q] == col) f = q;
-        if (f >= 0) {
-          const double tv = a.pt2[12 * pj + 2 * f], tdv = a.pt2[12 * pj + 2 * f + 1];
-          const double s = tdv * r1p[3];
-          const double df[3] = {s * d1[0], s * d1[1], s * d1[2]};
-          for (int x = 0; x < 3; ++x) F[x] += df[x];
-          E += 0.5 * tv;
-          for (int x = 0; x < 3; ++x)
-            for (int y = 0; y < 3; ++y) sg[3 * x + y] -= 0.5 * d1[x] * df[y];
-        }
         const double *q1 = a.pv + 8 * pj;
         const double h1 = q1[2], dh1 = q1[3];
-        if (tri && (h1 != 0.0 || dh1 != 0.0)) {
+        if (h1 != 0.0 || dh1 != 0.0) {
           // tangent of (h1, h1') when the column is gamma or a2 of slot s1
           double t1 = 0.0, td1 = 0.0;
           if (a.col_sw[8 * s1 + 6] == col) {
@@ -1657,51 +1642,81 @@ __global__ __launch_bounds__(256) void k_sw_jac_centre(SwJArgs a) {
   E = sw_group_sum(E);
   for (int k = 0; k < 9; ++k) sg[k] = sw_group_sum(sg[k]);
   if (live && l == 0) {
-    double *o = a.own + (int64_t(col) * a.N + i) * 3;
+    double *o = a.own + (int64_t(tp) * a.N + i) * 3;
     for (int x = 0; x < 3; ++x) o[x] = F[x];
-    double *e = a.etan + (int64_t(col) * a.N + i) * 10;
+    double *e = a.etan + (int64_t(tp) * a.N + i) * 10;
     e[0] = E;
     for (int k = 0; k < 9; ++k) e[1 + k] = sg[k];
   }
 }
 
 // one block per tile, x = atoms of the tile, y = column
+// one block per tile, x = atoms of the tile, y = column. A triplet column
+// gathers, per atom, its own tangent force and the triplet slots of its
+// reverse pairs, and takes the energy and virial tangents of the centre pass;
+// a two-body column (a field of v2, no triplet term) forms the pair terms of
+// the atom's own pairs here: force (v2' tangent / r) d, energy half the v2
+// tangent, virial -d (x) force / 2.
 __global__ __launch_bounds__(64) void k_sw_jac_tiles(SwJArgs a) {
   __shared__ double red[64][kJP];
   const int tile = blockIdx.x, col = blockIdx.y, t = threadIdx.x;
   const int na = a.tile_na[tile];
+  const int tp = a.tidx[col];
   // kLpt lanes per atom (the same for every tile size, so an atom's sum does
-  // not depend on the tiling): lane sub of atom li gathers every kLpt-th
-  // reverse-pair slot, and the lanes' sums are added by shuffles within the
-  // aligned group; 64 / kLpt atoms per round
+  // not depend on the tiling): lane sub of atom li takes every kLpt-th pair
+  // or reverse-pair slot, and the lanes' sums are added by shuffles within
+  // the aligned group; 64 / kLpt atoms per round
   constexpr int kLpt = 4;
   const int sub = t % kLpt;
-  const double *pt = a.ptan + int64_t(col) * a.P * 3;
+  const int vo[6] = {0, 4, 8, 1, 2, 5}; // xx yy zz xy xz yz
   for (int l0 = 0; l0 < na; l0 += 64 / kLpt) { // uniform trip count: every lane shuffles
     const int li = l0 + t / kLpt;
-    double F[3] = {0, 0, 0};
-    int64_t i = 0;
-    if (li < na) {
-      i = int64_t(a.tile_a0[tile]) + li;
+    const int64_t i = li < na ? int64_t(a.tile_a0[tile]) + li : 0;
+    double F[3] = {0, 0, 0}, E = 0.0, W[6] = {0, 0, 0, 0, 0, 0};
+    if (tp >= 0) {
+      const double *pt = a.ptan + int64_t(tp) * a.P * 3;
+      if (li < na) {
+        for (int64_t p = a.pair_off[i] + sub; p < a.pair_off[i + 1]; p += kLpt) {
+          const int32_t q = a.rev[p];
+          if (q < 0) continue;
+          for (int k = 0; k < 3; ++k) F[k] += pt[3 * int64_t(q) + k];
+        }
+      }
+    } else if (li < na) {
+      // the column is field f of slot s: the pairs of that slot carry it
+      const int sf = a.tsf[col], slot = sf >> 3;
+      const double2 *tf = a.pt2 + int64_t(sf & 7) * a.P;
+      const int ti = a.types[i];
       for (int64_t p = a.pair_off[i] + sub; p < a.pair_off[i + 1]; p += kLpt) {
-        const int32_t q = a.rev[p];
-        if (q < 0) continue;
-        for (int k = 0; k < 3; ++k) F[k] += pt[3 * int64_t(q) + k];
+        const uint32_t b = a.bits[p];
+        if ((b & kFlagTiny) || ord_pair(ti, pj_type(b), a.ntypes) != slot) continue;
+        const double2 tt = tf[p];
+        const double d[3] = {a.pd[3 * p], a.pd[3 * p + 1], a.pd[3 * p + 2]};
+        const double df[3] = {tt.y * d[0], tt.y * d[1], tt.y * d[2]};
+        for (int x = 0; x < 3; ++x) F[x] += df[x];
+        E += 0.5 * tt.x;
+        for (int v = 0; v < 6; ++v) W[v] += d[vo[v] / 3] * df[vo[v] % 3];
       }
     }
-    for (int o = kLpt / 2; o > 0; o >>= 1)
+    for (int o = kLpt / 2; o > 0; o >>= 1) {
       for (int k = 0; k < 3; ++k) F[k] += __shfl_xor(F[k], o, 64);
+      E += __shfl_xor(E, o, 64);
+      for (int v = 0; v < 6; ++v) W[v] += __shfl_xor(W[v], o, 64);
+    }
     if (li < na && sub == 0) {
       const int c = a.tile_cfg[tile];
-      const double *o = a.own + (int64_t(col) * a.N + i) * 3;
-      for (int x = 0; x < 3; ++x) F[x] += o[x];
+      if (tp >= 0) {
+        const double *o = a.own + (int64_t(tp) * a.N + i) * 3;
+        for (int x = 0; x < 3; ++x) F[x] += o[x];
+        const double *e = a.etan + (int64_t(tp) * a.N + i) * 10;
+        E = e[0];
+        // stored as -2 x virial so the row kernel's -w_S/(2V) scaling yields w_S * virial / V
+        for (int v = 0; v < 6; ++v) W[v] = -2.0 * e[1 + vo[v]];
+      }
       const int64_t row = a.row_off[c] + 3 * (i - a.atom_off[c]);
       for (int x = 0; x < 3; ++x) a.J[(row + x) * a.D + col] = float(F[x]);
-      const double *e = a.etan + (int64_t(col) * a.N + i) * 10;
-      red[li][0] = e[0];
-      // stored as -2 x virial so the row kernel's -w_S/(2V) scaling yields w_S * virial / V
-      const int vo[6] = {0, 4, 8, 1, 2, 5};
-      for (int v = 0; v < 6; ++v) red[li][1 + v] = -2.0 * e[1 + vo[v]];
+      red[li][0] = E;
+      for (int v = 0; v < 6; ++v) red[li][1 + v] = W[v];
       red[li][7] = 0.0;
     }
   }
@@ -1826,7 +1841,7 @@ void launch_jacobian(Context &c) {
     launch_sw_pair_terms(c);
     s.pv = c.sw_pv.p;
     s.tri = c.sw_tri.p;
-    c.sw_pt2.alloc(12 * size_t(std::max<int64_t>(c.P, 1)));
+    c.sw_pt2.alloc(6 * size_t(std::max<int64_t>(c.P, 1)));
     s.pt2 = c.sw_pt2.p;
     std::vector<int32_t> csw(8 * c.npair), clam(c.h_lam.size());
     int o = 0;
@@ -1838,10 +1853,35 @@ void launch_jacobian(Context &c) {
     FS_HIP(hipMemcpyAsync(c.sw_clam.p, clam.data(), 4 * clam.size(), hipMemcpyHostToDevice, c.stream));
     s.col_sw = c.sw_csw.p;
     s.col_lam = c.sw_clam.p;
+    // triplet columns (gamma or a2 of a slot, or a lambda) take the centre
+    // pass; the others are two-body fields, formed in the tile pass
+    // [D] triplet pass of each column, [D] column of each triplet pass,
+    // [D] 8 slot + field of each two-body column
+    const size_t Dm = size_t(std::max(c.D, 1));
+    std::vector<int32_t> tmap(3 * Dm, -1);
+    int ntri = 0;
+    for (int col = 0; col < c.D; ++col) {
+      bool t = false;
+      for (int q = 0; q < c.npair; ++q) t = t || csw[8 * q + 6] == col || csw[8 * q + 7] == col;
+      for (int32_t v : clam) t = t || v == col;
+      if (t) {
+        tmap[col] = ntri;
+        tmap[Dm + ntri] = col;
+        ++ntri;
+      }
+      for (int q = 0; q < c.npair; ++q)
+        for (int f = 0; f < 6; ++f)
+          if (csw[8 * q + f] == col) tmap[2 * Dm + col] = 8 * q + f;
+    }
+    c.sw_tmap.alloc(tmap.size());
+    FS_HIP(hipMemcpyAsync(c.sw_tmap.p, tmap.data(), 4 * tmap.size(), hipMemcpyHostToDevice, c.stream));
+    s.tidx = c.sw_tmap.p;
+    s.tcol = c.sw_tmap.p + Dm;
+    s.tsf = c.sw_tmap.p + 2 * Dm;
     s.P = c.P;
-    c.sw_own.alloc(size_t(c.D) * c.N * 3 + 1);
-    c.sw_ptan.alloc(size_t(c.D) * std::max<int64_t>(c.P, 1) * 3);
-    c.sw_etan.alloc(size_t(c.D) * c.N * 10 + 1);
+    c.sw_own.alloc(size_t(ntri) * c.N * 3 + 1);
+    c.sw_ptan.alloc(size_t(ntri) * std::max<int64_t>(c.P, 1) * 3);
+    c.sw_etan.alloc(size_t(ntri) * c.N * 10 + 1);
     s.own = c.sw_own.p;
     s.ptan = c.sw_ptan.p;
     s.etan = c.sw_etan.p;
@@ -1857,9 +1897,11 @@ void launch_jacobian(Context &c) {
       hipLaunchKernelGGL(k_sw_pair_tan, dim3(nb(c.N * kSwJLanes, 256)), dim3(256), 0, c.stream, s);
       FS_HIP(hipGetLastError());
       c.time_main_begin();
-      hipLaunchKernelGGL(k_sw_jac_centre, dim3(nb(c.N * kSwJLanes, 256), c.D), dim3(256), 0,
-                         c.stream, s);
-      FS_HIP(hipGetLastError());
+      if (ntri > 0) {
+        hipLaunchKernelGGL(k_sw_jac_centre, dim3(nb(c.N * kSwJLanes, 256), ntri), dim3(256), 0,
+                           c.stream, s);
+        FS_HIP(hipGetLastError());
+      }
       c.time_main_end();
       hipLaunchKernelGGL(k_sw_jac_tiles, dim3(std::max(nt, 1), c.D), dim3(64), 0, c.stream, s);
       FS_HIP(hipGetLastError());
