@@ -150,7 +150,7 @@ double population_tune_launch(Context &c, const double *d_X, int np, double *d_c
 void launch_sw_rev(Context &c);
 void launch_sw_eval(Context &c);
 // per-pair SW scalars {v2, v2', h, h', dh/dgamma, dh'/dgamma, dh/da2, dh'/da2}
-void launch_sw_pair_terms(Context &c);
+void launch_sw_pair_terms(Context &c, bool tangents);
 // the gauge's device copy from gA / h_gS / h_gF0 (allocated on first use)
 constexpr int kGaugeLen = 1 + 2 * kMaxTypes;
 void upload_gauge(Context &c);
@@ -275,7 +275,8 @@ struct Context {
   DBuf<double> sw_tri;
   // SW parameter / pair-list generation (bumped on every upload of the SW
   // parameters and every neighbour build) and the one sw_pv was computed for
-  uint64_t sw_gen = 1, sw_pv_gen = 0;
+  uint64_t sw_gen = 1, sw_pv_gen = 0, sw_pt2_gen = 0;
+  bool sw_tan_next = false; // the next SW pair-term pass also forms the two-body tangents
   DBuf<double2> sw_pt2; // SW two-body tangents, [6 fields][pairs] of (tv, tdv / r)
   bool forces_valid = false;
   // Jacobian

@@ -1395,45 +1395,6 @@ __global__ void k_jac_rows(JacArgs a) {
 // ---------------------------------------------------------------------------
 // Stillinger-Weber tangents
 // ---------------------------------------------------------------------------
-struct SwT {
-  double A, B, p, q, delta, a1, gamma, a2;
-};
-__device__ inline SwT lsw(const double *sw, int s) {
-  const double *q = sw + 8 * s;
-  return SwT{q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7]};
-}
-
-// tangents (tv, tdv) of (v2, v2') w.r.t. each two-body field f = 0..5 (A, B,
-// p, q, delta, a1), the parameter derivatives of v2_dv2
-// (stiweb_force.cpp:24-41), with the powers, the logarithm and the
-// exponential evaluated once for all fields
-__device__ inline void v2_tan_all(double r, const SwT &p, double *t) {
-  for (int k = 0; k < 12; ++k) t[k] = 0.0;
-  if (r >= p.a1) return;
-  const double dd = r - p.a1;
-  const double e = exp(p.delta / dd);
-  if (e == 0.0) return;
-  const double rp = pow(r, -p.p), rq = pow(r, -p.q);
-  const double rp1 = pow(r, -p.p - 1.0), rq1 = pow(r, -p.q - 1.0);
-  const double lr = log(r);
-  const double poly = p.A * rp - p.B * rq;
-  const double dpoly = -p.A * p.p * rp1 + p.B * p.q * rq1;
-  const double g = -p.delta / (dd * dd);
-  for (int f = 0; f < 6; ++f) {
-    double P = 0.0, dP = 0.0, E = 0.0, Gt = 0.0;
-    switch (f) {
-    case 0: P = rp; dP = -p.p * rp1; break;
-    case 1: P = -rq; dP = p.q * rq1; break;
-    case 2: P = -p.A * lr * rp; dP = p.A * rp1 * (p.p * lr - 1.0); break;
-    case 3: P = p.B * lr * rq; dP = p.B * rq1 * (1.0 - p.q * lr); break;
-    case 4: E = e / dd; Gt = -1.0 / (dd * dd); break;
-    default: E = e * p.delta / (dd * dd); Gt = -2.0 * p.delta / (dd * dd * dd); break;
-    }
-    t[2 * f] = P * e + poly * E;
-    t[2 * f + 1] = dP * e + dpoly * E + P * e * g + poly * E * g + poly * e * Gt;
-  }
-}
-
 struct SwJArgs {
   int64_t N;
   int D, ntypes, npair;
@@ -1445,7 +1406,7 @@ struct SwJArgs {
   const double *sw, *lam;
   const double *pv;       // 8 per pair: v2, v2', h, h' and the gamma / a2 tangents of h, h'
   const double *tri;      // 8 per pair: unit vector, 1/r, h, h', dh/dgamma, dh/da2
-  double2 *pt2;           // [6][P]: (tv, tdv / r) of v2 w.r.t. each two-body field (k_sw_pair_tan)
+  double2 *pt2;           // [6][P]: (tv, tdv / r) of v2 w.r.t. each two-body field (k_sw_pair_terms)
   const int32_t *col_sw;  // npair*8 -> column or -1
   const int32_t *col_lam; // nlam -> column or -1
   const int32_t *tidx;    // [D]: the column's triplet pass, or -1 for a two-body column
@@ -1463,39 +1424,8 @@ struct SwJArgs {
   double *part;
 };
 
-__device__ inline double pr64(const double *pd, int64_t p) {
-  const double d0 = pd[3 * p], d1 = pd[3 * p + 1], d2 = pd[3 * p + 2];
-  double r2 = d0 * d0 + d1 * d1;
-  r2 = r2 + d2 * d2;
-  return sqrt(r2);
-}
-
 constexpr int kSwJLanes = 16; // lanes per centre atom (a divisor of 64)
 constexpr int kSwStage = 32;  // neighbour records per centre staged in LDS
-
-// per pair (i -> j): the two-body tangents of its slot (ti, tj), every field
-// (zeros for an r < 1e-14 pair)
-__global__ __launch_bounds__(256) void k_sw_pair_tan(SwJArgs a) {
-  const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  const int64_t i = g / kSwJLanes;
-  const int l = int(g % kSwJLanes);
-  if (i >= a.N) return;
-  const int ti = a.types[i];
-  for (int64_t p = a.pair_off[i] + l; p < a.pair_off[i + 1]; p += kSwJLanes) {
-    double t[12];
-    const uint32_t b = a.bits[p];
-    double ir = 0.0;
-    if (b & kFlagTiny) {
-      for (int k = 0; k < 12; ++k) t[k] = 0.0;
-    } else {
-      const double r = pr64(a.pd, p);
-      ir = 1.0 / r;
-      v2_tan_all(r, lsw(a.sw, ord_pair(ti, pj_type(b), a.ntypes)), t);
-    }
-    // field-major, so a column's pass reads 16 contiguous bytes per pair
-    for (int f = 0; f < 6; ++f) a.pt2[int64_t(f) * a.P + p] = make_double2(t[2 * f], t[2 * f + 1] * ir);
-  }
-}
 
 __device__ inline double sw_group_sum(double v) {
 #pragma unroll
@@ -1838,10 +1768,9 @@ void launch_jacobian(Context &c) {
     s.rev = c.rev.p;
     s.sw = c.sw.p;
     s.lam = c.lam.p;
-    launch_sw_pair_terms(c);
+    launch_sw_pair_terms(c, true);
     s.pv = c.sw_pv.p;
     s.tri = c.sw_tri.p;
-    c.sw_pt2.alloc(6 * size_t(std::max<int64_t>(c.P, 1)));
     s.pt2 = c.sw_pt2.p;
     std::vector<int32_t> csw(8 * c.npair), clam(c.h_lam.size());
     int o = 0;
@@ -1894,8 +1823,6 @@ void launch_jacobian(Context &c) {
     s.J = c.J.p;
     s.part = c.jac_part.p;
     if (c.N > 0 && c.D > 0) {
-      hipLaunchKernelGGL(k_sw_pair_tan, dim3(nb(c.N * kSwJLanes, 256)), dim3(256), 0, c.stream, s);
-      FS_HIP(hipGetLastError());
       c.time_main_begin();
       if (ntri > 0) {
         hipLaunchKernelGGL(k_sw_jac_centre, dim3(nb(c.N * kSwJLanes, 256), ntri), dim3(256), 0,

@@ -90,7 +90,15 @@ int fs_step_impl(Context &c, const double *x, double *F, float *J, double *cost)
   c.params_dirty = true;
   c.forces_valid = false;
   if (c.kind == FS_SW || c.D == 0) {
-    evaluate_for_solver(c);
+    // the SW pair-term pass of the residual also forms the Jacobian's tangents
+    c.sw_tan_next = c.kind == FS_SW && c.D > 0;
+    try {
+      evaluate_for_solver(c);
+    } catch (...) {
+      c.sw_tan_next = false;
+      throw;
+    }
+    c.sw_tan_next = false;
     launch_jacobian(c);
     launch_cost(c, c.cost.p);
   } else {

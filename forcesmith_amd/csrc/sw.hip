@@ -36,30 +36,63 @@ __device__ inline SwP load_sw(const double *sw, int s) {
   return SwP{q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7]};
 }
 
-__device__ inline void v2_dv2(double r, const SwP &p, double &v2, double &dv2) {
-  v2 = 0.0;
-  dv2 = 0.0;
-  if (r >= p.a1) return;
-  const double d = r - p.a1;
-  const double e = exp(p.delta / d);
-  if (e == 0.0) return;
-  const double rp = pow(r, -p.p);
-  const double rq = pow(r, -p.q);
-  const double poly = p.A * rp - p.B * rq;
-  const double dpoly = -p.A * p.p * pow(r, -p.p - 1.0) + p.B * p.q * pow(r, -p.q - 1.0);
-  v2 = poly * e;
-  dv2 = dpoly * e + poly * e * (-p.delta / (d * d));
-}
-
-__device__ inline void h_dh(double r, const SwP &p, double &h, double &dh) {
-  h = 0.0;
-  dh = 0.0;
-  if (r >= p.a2) return;
-  const double d = r - p.a2;
-  const double e = exp(p.gamma / d);
-  if (e == 0.0) return;
-  h = e;
-  dh = e * (-p.gamma / (d * d));
+// Radial factors of one pair at distance r (ir = 1 / r), v2_dv2 and h_dh
+// (stiweb_force.cpp:24-54): o = {v2, v2', h, h', dh/dgamma, dh'/dgamma,
+// dh/da2, dh'/da2}; with TAN also the tangents (tv, tdv / r) of (v2, v2')
+// w.r.t. each two-body field f = 0..5 (A, B, p, q, delta, a1) into t. The
+// powers come from one logarithm, r^-p = exp(-p ln r) and r^(-p-1) =
+// r^-p / r, shared by the values and the tangents.
+template <bool TAN>
+__device__ inline void sw_radial(double r, double ir, const SwP &P, double *o, double2 *t) {
+  for (int k = 0; k < 8; ++k) o[k] = 0.0;
+  if (TAN)
+    for (int f = 0; f < 6; ++f) t[f] = make_double2(0.0, 0.0);
+  if (r < P.a1) {
+    const double dd = r - P.a1;
+    const double e = exp(P.delta / dd);
+    if (e != 0.0) {
+      const double lr = log(r);
+      const double rp = exp(-P.p * lr), rq = exp(-P.q * lr);
+      const double rp1 = rp * ir, rq1 = rq * ir;
+      const double poly = P.A * rp - P.B * rq;
+      const double dpoly = -P.A * P.p * rp1 + P.B * P.q * rq1;
+      const double g = -P.delta / (dd * dd);
+      o[0] = poly * e;
+      o[1] = dpoly * e + poly * e * g;
+      if (TAN) {
+        for (int f = 0; f < 6; ++f) {
+          double Pf = 0.0, dPf = 0.0, Ef = 0.0, Gt = 0.0;
+          switch (f) {
+          case 0: Pf = rp; dPf = -P.p * rp1; break;
+          case 1: Pf = -rq; dPf = P.q * rq1; break;
+          case 2: Pf = -P.A * lr * rp; dPf = P.A * rp1 * (P.p * lr - 1.0); break;
+          case 3: Pf = P.B * lr * rq; dPf = P.B * rq1 * (1.0 - P.q * lr); break;
+          case 4: Ef = e / dd; Gt = -1.0 / (dd * dd); break;
+          default: Ef = e * P.delta / (dd * dd); Gt = -2.0 * P.delta / (dd * dd * dd); break;
+          }
+          const double tv = Pf * e + poly * Ef;
+          const double tdv = dPf * e + dpoly * Ef + Pf * e * g + poly * Ef * g + poly * e * Gt;
+          t[f] = make_double2(tv, tdv * ir);
+        }
+      }
+    }
+  }
+  if (r < P.a2) {
+    const double dd = r - P.a2;
+    const double e = exp(P.gamma / dd);
+    if (e != 0.0) {
+      const double g1 = -P.gamma / (dd * dd);
+      o[2] = e;
+      o[3] = e * g1;
+      // tangents of (h, h') w.r.t. gamma and a2 (h = exp(gamma / (r - a2)))
+      const double Hg = e / dd, Gg = -1.0 / (dd * dd);
+      o[4] = Hg;
+      o[5] = Hg * g1 + e * Gg;
+      const double Ha = e * P.gamma / (dd * dd), Ga = -2.0 * P.gamma / (dd * dd * dd);
+      o[6] = Ha;
+      o[7] = Ha * g1 + e * Ga;
+    }
+  }
 }
 
 // reverse pair: for p = (i -> j, image n) the entry (j -> i, image -n)
@@ -97,24 +130,30 @@ __device__ inline double group_sum(double v) {
   return v;
 }
 
-// per pair (i -> j): {v2, v2', h, h', dh/dgamma, dh'/dgamma, dh/da2, dh'/da2}
-// of the pair's parameter slot (ti, tj); zeros for an r < 1e-14 pair
-__global__ __launch_bounds__(256) void k_sw_pair_terms(int64_t N, const int64_t *__restrict__ pair_off,
+// per pair (i -> j) of the pair's parameter slot (ti, tj), zeros for an
+// r < 1e-14 pair: pv = {v2, v2', h, h', dh/dgamma, dh'/dgamma, dh/da2,
+// dh'/da2}, the triplet record tri = {unit vector, 1/r, h, h', dh/dgamma,
+// dh/da2}, and with TAN the two-body tangents pt2, field-major ([6][P]) so a
+// Jacobian column's pass reads 16 contiguous bytes per pair
+template <bool TAN>
+__global__ __launch_bounds__(256) void k_sw_pair_terms(int64_t N, int64_t P,
+                                                       const int64_t *__restrict__ pair_off,
                                                        const int32_t *__restrict__ types,
                                                        const double *__restrict__ pd,
                                                        const uint32_t *__restrict__ bits,
                                                        const double *__restrict__ sw, int ntypes,
                                                        double *__restrict__ pv,
-                                                       double *__restrict__ tri) {
+                                                       double *__restrict__ tri,
+                                                       double2 *__restrict__ pt2) {
   const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t i = g / kSwLanes;
   const int l = int(g % kSwLanes);
   if (i >= N) return;
   const int ti = types[i];
   for (int64_t p = pair_off[i] + l; p < pair_off[i + 1]; p += kSwLanes) {
-    double o[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double o[8], u[4] = {0, 0, 0, 0};
+    double2 t[6];
     const uint32_t b = bits[p];
-    double u[4] = {0, 0, 0, 0};
     if (!(b & kFlagTiny)) {
       const double d0 = pd[3 * p], d1 = pd[3 * p + 1], d2 = pd[3 * p + 2];
       double r2 = d0 * d0 + d1 * d1;
@@ -124,28 +163,21 @@ __global__ __launch_bounds__(256) void k_sw_pair_terms(int64_t N, const int64_t 
       u[0] = d0 * u[3];
       u[1] = d1 * u[3];
       u[2] = d2 * u[3];
-      const SwP P = load_sw(sw, ord_pair(ti, pj_type(b), ntypes));
-      v2_dv2(r, P, o[0], o[1]);
-      h_dh(r, P, o[2], o[3]);
-      if (o[2] != 0.0 || o[3] != 0.0) {
-        // tangents of (h, h') w.r.t. gamma and a2 (h = exp(gamma / (r - a2)))
-        const double dd = r - P.a2, e = o[2], g1 = -P.gamma / (dd * dd);
-        const double Hg = e / dd, Gg = -1.0 / (dd * dd);
-        o[4] = Hg;
-        o[5] = Hg * g1 + e * Gg;
-        const double Ha = e * P.gamma / (dd * dd), Ga = -2.0 * P.gamma / (dd * dd * dd);
-        o[6] = Ha;
-        o[7] = Ha * g1 + e * Ga;
-      }
+      sw_radial<TAN>(r, u[3], load_sw(sw, ord_pair(ti, pj_type(b), ntypes)), o, t);
+    } else {
+      for (int k = 0; k < 8; ++k) o[k] = 0.0;
+      for (int f = 0; f < 6; ++f) t[f] = make_double2(0.0, 0.0);
     }
     double *q = pv + 8 * p;
     for (int k = 0; k < 8; ++k) q[k] = o[k];
-    double *t = tri + 8 * p;
-    for (int k = 0; k < 4; ++k) t[k] = u[k];
-    t[4] = o[2];
-    t[5] = o[3];
-    t[6] = o[4];
-    t[7] = o[6];
+    double *w = tri + 8 * p;
+    for (int k = 0; k < 4; ++k) w[k] = u[k];
+    w[4] = o[2];
+    w[5] = o[3];
+    w[6] = o[4];
+    w[7] = o[6];
+    if (TAN)
+      for (int f = 0; f < 6; ++f) pt2[int64_t(f) * P + p] = t[f];
   }
 }
 
@@ -346,23 +378,34 @@ SwArgs make_sw_args(Context &c) {
   return a;
 }
 
-void launch_sw_pair_terms(Context &c) {
-  c.sw_pv.alloc(8 * size_t(std::max<int64_t>(c.P, 1)));
-  c.sw_tri.alloc(8 * size_t(std::max<int64_t>(c.P, 1)));
+void launch_sw_pair_terms(Context &c, bool tangents) {
+  const size_t P1 = size_t(std::max<int64_t>(c.P, 1));
+  c.sw_pv.alloc(8 * P1);
+  c.sw_tri.alloc(8 * P1);
+  tangents = tangents || c.sw_tan_next;
+  if (tangents) c.sw_pt2.alloc(6 * P1);
   if (c.N == 0) return;
-  // the residual and the Jacobian at one parameter vector share the factors
-  if (c.sw_pv_gen == c.sw_gen) return;
+  // the residual and the Jacobian at one parameter vector share the factors;
+  // the Jacobian's two-body tangents come from the same pass when asked for
+  if (c.sw_pv_gen == c.sw_gen && (!tangents || c.sw_pt2_gen == c.sw_gen)) return;
   c.sw_pv_gen = c.sw_gen;
-  const int64_t nthr = c.N * kSwLanes;
-  hipLaunchKernelGGL(k_sw_pair_terms, dim3(unsigned((nthr + 255) / 256)), dim3(256), 0, c.stream,
-                     c.N, c.pair_off.p, c.types.p, c.pair_d.p, c.pair_bits.p, c.sw.p, c.ntypes,
-                     c.sw_pv.p, c.sw_tri.p);
+  const dim3 grid(unsigned((c.N * kSwLanes + 255) / 256));
+  if (tangents) {
+    c.sw_pt2_gen = c.sw_gen;
+    hipLaunchKernelGGL(k_sw_pair_terms<true>, grid, dim3(256), 0, c.stream, c.N, c.P, c.pair_off.p,
+                       c.types.p, c.pair_d.p, c.pair_bits.p, c.sw.p, c.ntypes, c.sw_pv.p, c.sw_tri.p,
+                       c.sw_pt2.p);
+  } else {
+    hipLaunchKernelGGL(k_sw_pair_terms<false>, grid, dim3(256), 0, c.stream, c.N, c.P, c.pair_off.p,
+                       c.types.p, c.pair_d.p, c.pair_bits.p, c.sw.p, c.ntypes, c.sw_pv.p, c.sw_tri.p,
+                       nullptr);
+  }
   FS_HIP(hipGetLastError());
 }
 
 void launch_sw_eval(Context &c) {
   FS_REQUIRE(c.tile_atoms <= 64, FS_ERR_UNSUPPORTED, "SW tiles hold at most 64 atoms");
-  launch_sw_pair_terms(c);
+  launch_sw_pair_terms(c, false);
   const SwArgs a = make_sw_args(c);
   if (c.N > 0) {
     const dim3 grid(unsigned((c.N * kSwLanes + 255) / 256));
