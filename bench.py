@@ -494,7 +494,23 @@ def main():
                     "traffic": None, "bytes_per_launch": b_main, "intermediate_bytes_per_launch": b_inter,
                     "ms_per_launch": t_quad}
     else:
-        kernels = {"residual": t_res, "jacobian": t_jac, "normal_equations": t_ne}
+        t_main = g.time_step(4, 10)
+        kernels = {"residual": t_res, "jacobian": t_jac, "jac_centre": t_main, "normal_equations": t_ne}
+        # compulsory bytes of the SW triplet-column centre pass (the dominant
+        # kernel) per launch: the pair records (16 B/pair) and the force
+        # tangents of its columns (3N x D_tri x 4 B). The per-pair records it
+        # reads (unit vector, 1/r, radial factors: 64 B/pair) and the triplet
+        # slots it hands to the tile pass (24 B per pair and column) are
+        # intermediates. The pass is FP64 arithmetic and latency bound
+        pf = np.asarray(m.pair_fixed, bool).reshape(-1, 8)
+        D_tri = int((~pf[:, 6:8]).sum() + (~np.asarray(m.lam_fixed, bool)).sum())  # gamma, a2, lambda
+        b_main = 16 * P + 12 * N * D_tri
+        b_inter = 64 * P + 24 * P * D_tri
+        achieved = b_main / (t_main * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "kernel": "k_sw_jac_centre (triplet columns)", "achieved": achieved,
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                    "traffic": None, "bytes_per_launch": b_main, "intermediate_bytes_per_launch": b_inter,
+                    "ms_per_launch": t_main}
     b_alg = 16 * P + 28 * N + 56 * C + 8 * R + 4 * R * D
     out = {
         "metric": "residual+Jacobian configs*params/s",
@@ -507,7 +523,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32" if a.precision == "mixed" else "f64",
+        # SW is FP64 arithmetic in both modes
+        "dtype": "fp32" if a.precision == "mixed" and kind != "sw" else "f64",
         "data": f"synthetic (seeded analytic {kind.upper()} truth + noise)",
         "config": {"workload": desc.format(C=C, D=D),
                    "configs": C, "atoms": N, "pairs": P, "params": D, "rows": R,
