@@ -180,6 +180,9 @@ struct Context {
   // two in [4, 64] (fs_set_jacobian_tile, fs_autotune)
   int jac_tile = 0;
   bool jac_tile_fixed = false; // set by hand: fs_autotune keeps it
+  // form of the density/embedding-column pass (fs_set_jacobian_form):
+  // 0 knot bins while they fit, else windows; 1 windows; 2 knot bins
+  int jac_form = 0;
   // persistent-grid size of the Jacobian passes (blocks per CU; each tile is
   // one block's work whatever the grid, so the result does not depend on it)
   int jac_blocks_per_cu = 8;
@@ -301,6 +304,14 @@ struct Context {
   DBuf<float4> jrec; // pair records sorted by (partner type, r) per atom
   DBuf<double> jr;   // their FP64 distances
   bool jsort_valid = false;
+  // knot sums of the geometry (jac.hip, k_knot_sums): per-atom pair-table
+  // force sums and density value sums, per-configuration pair-table energy
+  // and virial sums; built once per neighbour list and model grid, folded
+  // with the current slope band by every Jacobian
+  bool ks_valid = false;
+  int ks_lp = 0, ks_ld = 0, ks_kp = 0; // entries per atom (pair, density), pair-table knots
+  DBuf<float> ks_pair, ks_dens;
+  DBuf<double> ks_tile, ks_cfg;
   DBuf<float> Hs, emb_d, emb_v;
   DBuf<int32_t> emb_k, sw_csw, sw_clam, sw_tmap;
   DBuf<double> sw_own, sw_ptan, sw_etan;

@@ -963,6 +963,13 @@ int fs_set_jacobian_tile(fs_ctx *ctx, int32_t atoms) {
   });
 }
 
+int fs_set_jacobian_form(fs_ctx *ctx, int32_t form) {
+  return guarded(ctx, [&](Context &c) {
+    FS_REQUIRE(form >= 0 && form <= 2, FS_ERR_ARG, "Jacobian form must be 0, 1 or 2");
+    c.jac_form = form;
+  });
+}
+
 int fs_get_launch(fs_ctx *ctx, int32_t *lanes_per_atom, int32_t *blocks_per_cu,
                   int32_t *jacobian_tile, int32_t *jacobian_blocks_per_cu) {
   return guarded(ctx, [&](Context &c) {

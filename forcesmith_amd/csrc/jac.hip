@@ -908,6 +908,417 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
 }
 
 // ---------------------------------------------------------------------------
+// Knot sums. A pair-table column of the Jacobian, and a density column's rows
+// H_i, are linear in per-atom sums that depend on the geometry and the table
+// grids only, not on the knot values. For a table T, knot m and the atom's
+// pairs on T:
+//   Y[m] = sum of b0 w over the pairs in interval m, plus b1 w over those in
+//          interval m - 1,
+//   S[m] = the same with b2 and b3,
+// where (b0, b1, b2, b3) is the Hermite basis of the pair's place in its
+// interval (the derivative basis for forces and the virial, the value basis
+// for the energy and the density) and w the pair's weight (its direction u
+// for forces, r u u for the virial, 1/2 for the energy, 1 for the density).
+// The column of knot m is then
+//   Y[m] + sum_{k = m-2..m+2} S[k] ds_k/dy_m,
+// identity on the knot values and the makima slope band on the slopes:
+// band6's coefficients summed over the pairs, regrouped by knot.
+// k_knot_sums forms the sums once per neighbour list and model grid (lanes
+// own knots and walk the pairs of their knot's two intervals in the sorted
+// pair order); k_jac_pair_ks and k_jac_dens_ks fold them with the current
+// slope band at every Jacobian, which reads the sums and writes the columns.
+//   ks_pair  per atom: force sums of the tables phi(ti, t), t in type order,
+//            [Y: 3 lp floats][S: 3 lp floats]
+//   ks_dens  per atom: density value sums of the tables g_t, t in type order,
+//            [Y: ld floats][S: ld floats]
+//   ks_cfg   per configuration (FP64): pair-table energy and virial sums,
+//            [14][kp] as (Y_e, S_e, Y_w[6], S_w[6]) by flat pair-table knot
+// ---------------------------------------------------------------------------
+struct KsArgs {
+  float *pair, *dens;
+  double *tile, *cfg;
+  int lp, ld, kp;
+};
+
+// first entry of table phi(ti, t) among the pair sums of an atom of type ti
+__device__ inline int ks_pair_off(const TableDesc *sT, int ntypes, int ti, int t) {
+  int o = 0;
+  for (int u = 0; u < t; ++u) o += sT[ord_pair(ti, u, ntypes)].n;
+  return o;
+}
+
+struct KsStage {
+  int32_t seg[kSegPad]; // partner-type segment starts of the chunk
+  float r[64], u0[64], u1[64], u2[64];
+  int32_t kp[64], kg[64];                     // intervals on phi(ti, tj) and g_tj
+  float bdp[4][64], bvp[4][64], bvg[4][64];   // weighted bases
+};
+
+__host__ __device__ constexpr size_t ks_wave_bytes(int kp) {
+  return (sizeof(KsStage) + sizeof(float) * 14 * size_t(kp) + 15) & ~size_t(15);
+}
+
+// one wave per atom; lanes own knots: two slots of the atom's pair-table
+// knots (entry L = lane + 64 s of its pair sums), two of the density knots.
+// Energy and virial sums go into the wave's FP32 accumulators by flat
+// pair-table knot (distinct per lane for one atom), then per tile in FP64.
+__global__ __launch_bounds__(256) void k_knot_sums(JacArgs a, KsArgs k) {
+  extern __shared__ float4 smem[];
+  TableDesc *sT = reinterpret_cast<TableDesc *>(smem);
+  const int ndq = a.ntab * int(sizeof(TableDesc) / sizeof(float4));
+  for (int e = threadIdx.x; e < ndq; e += blockDim.x)
+    smem[e] = reinterpret_cast<const float4 *>(a.tabs)[e];
+  const int wave = uni(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const size_t wb = ks_wave_bytes(k.kp);
+  char *base = reinterpret_cast<char *>(smem + ndq);
+  KsStage &P = *reinterpret_cast<KsStage *>(base + wave * wb);
+  float *acc = reinterpret_cast<float *>(base + wave * wb + sizeof(KsStage)); // [14][kp]
+  __syncthreads();
+  const int nk = 14 * k.kp;
+  for (int tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
+    for (int e = lane; e < nk; e += 64) acc[e] = 0.0f;
+    wave_sync();
+    const int na = uni(a.tile_na[tile]);
+    for (int li = wave; li < na; li += 4) {
+      const int64_t i = int64_t(uni(a.tile_a0[tile])) + li;
+      const int ti = uni(a.types[i]);
+      const int64_t p0 = uni64(a.pair_off[i]), p1 = uni64(a.pair_off[i + 1]);
+      // this lane's knots: (partner type, knot) of a pair table, of a density table
+      int pt[2], pm[2], pK[2], dt[2], dm[2];
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        const int L = lane + 64 * s;
+        pt[s] = dt[s] = -1;
+        pm[s] = dm[s] = pK[s] = 0;
+        int o = 0;
+        for (int t = 0; t < a.ntypes; ++t) {
+          const TableDesc &T = sT[ord_pair(ti, t, a.ntypes)];
+          if (pt[s] < 0 && L >= o && L < o + T.n) {
+            pt[s] = t;
+            pm[s] = L - o;
+            pK[s] = T.off + L - o;
+          }
+          o += T.n;
+        }
+        o = 0;
+        for (int t = 0; t < a.ntypes; ++t) {
+          const int n = sT[a.npair + t].n;
+          if (dt[s] < 0 && L >= o && L < o + n) {
+            dt[s] = t;
+            dm[s] = L - o;
+          }
+          o += n;
+        }
+      }
+      float f[2][6], e[2][14], v[2][2];
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) f[s][q] = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 14; ++q) e[s][q] = 0.0f;
+        v[s][0] = v[s][1] = 0.0f;
+      }
+      for (int64_t pc = p0; pc < p1; pc += 64) {
+        const int64_t p = pc + lane;
+        int ty = kMaxTypes;
+        if (p < p1) {
+          const float4 q = a.jrec[p];
+          const uint32_t b = __float_as_uint(q.w);
+          const double r64 = a.jr[p];
+          const float rr = float(r64);
+          const bool tiny = (b & kFlagTiny) != 0;
+          const float inv_r = tiny ? 0.0f : 1.0f / rr;
+          const int tj = pj_type(b);
+          ty = tj;
+          P.r[lane] = rr;
+          P.u0[lane] = q.x * inv_r;
+          P.u1[lane] = q.y * inv_r;
+          P.u2[lane] = q.z * inv_r;
+          float bv[4], bd[4];
+          {
+            const TableDesc &T = sT[ord_pair(ti, tj, a.ntypes)];
+            P.kp[lane] = pair_basis(T, r64, bv, bd);
+            const float w = ((b & kFlagPhi) && !tiny) ? 1.0f : 0.0f;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+              P.bdp[c][lane] = w * bd[c];
+              P.bvp[c][lane] = w * bv[c];
+            }
+          }
+          {
+            const TableDesc &T = sT[a.npair + tj];
+            P.kg[lane] = pair_basis(T, r64, bv, bd);
+            const float w = ((b & kFlagGj) && !tiny) ? 1.0f : 0.0f;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) P.bvg[c][lane] = w * bv[c];
+          }
+        }
+        stage_segments(ty, a.ntypes, lane, P.seg);
+        wave_sync();
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+          if (pt[s] >= 0) {
+            const int m = pm[s], s1 = P.seg[pt[s] + 1];
+            const int lo = first_ge(P.kp, P.seg[pt[s]], s1, m - 1);
+            const int hi = first_ge(P.kp, lo, s1, m + 1);
+            for (int q = lo; q < hi; ++q) {
+              const bool at = P.kp[q] == m; // interval m (b0, b2) or m - 1 (b1, b3)
+              const float cy = at ? P.bdp[0][q] : P.bdp[1][q], cs = at ? P.bdp[2][q] : P.bdp[3][q];
+              const float ey = at ? P.bvp[0][q] : P.bvp[1][q], es = at ? P.bvp[2][q] : P.bvp[3][q];
+              const float u0 = P.u0[q], u1 = P.u1[q], u2 = P.u2[q], r = P.r[q];
+              f[s][0] += cy * u0;
+              f[s][1] += cy * u1;
+              f[s][2] += cy * u2;
+              f[s][3] += cs * u0;
+              f[s][4] += cs * u1;
+              f[s][5] += cs * u2;
+              e[s][0] += 0.5f * ey;
+              e[s][1] += 0.5f * es;
+              const float ruu[6] = {r * u0 * u0, r * u1 * u1, r * u2 * u2,
+                                    r * u0 * u1, r * u0 * u2, r * u1 * u2};
+#pragma unroll
+              for (int c = 0; c < 6; ++c) {
+                e[s][2 + c] += cy * ruu[c];
+                e[s][8 + c] += cs * ruu[c];
+              }
+            }
+          }
+          if (dt[s] >= 0) {
+            const int m = dm[s], s1 = P.seg[dt[s] + 1];
+            const int lo = first_ge(P.kg, P.seg[dt[s]], s1, m - 1);
+            const int hi = first_ge(P.kg, lo, s1, m + 1);
+            for (int q = lo; q < hi; ++q) {
+              const bool at = P.kg[q] == m;
+              v[s][0] += at ? P.bvg[0][q] : P.bvg[1][q];
+              v[s][1] += at ? P.bvg[2][q] : P.bvg[3][q];
+            }
+          }
+        }
+        wave_sync();
+      }
+      float *fp = k.pair + size_t(i) * 6 * k.lp;
+      float *fd = k.dens + size_t(i) * 2 * k.ld;
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        const int L = lane + 64 * s;
+        if (pt[s] >= 0) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            fp[3 * L + c] = f[s][c];
+            fp[3 * k.lp + 3 * L + c] = f[s][3 + c];
+          }
+#pragma unroll
+          for (int q = 0; q < 14; ++q) acc[q * k.kp + pK[s]] += e[s][q];
+        }
+        if (dt[s] >= 0) {
+          fd[L] = v[s][0];
+          fd[k.ld + L] = v[s][1];
+        }
+      }
+    }
+    // tile sums in FP64, waves in order
+    __syncthreads();
+    for (int e = threadIdx.x; e < nk; e += blockDim.x) {
+      double s = 0.0;
+      for (int w = 0; w < 4; ++w)
+        s += double(reinterpret_cast<const float *>(base + w * wb + sizeof(KsStage))[e]);
+      k.tile[size_t(tile) * nk + e] = s;
+    }
+    __syncthreads();
+  }
+}
+
+// per configuration: its tiles' energy and virial sums in tile order
+__global__ void k_knot_sums_cfg(JacArgs a, KsArgs k) {
+  const int c = blockIdx.x;
+  const int t0 = a.cfg_tile0[c], t1 = a.cfg_tile0[c + 1];
+  const int nk = 14 * k.kp;
+  for (int e = threadIdx.x; e < nk; e += blockDim.x) {
+    double s = 0.0;
+    for (int t = t0; t < t1; ++t) s += k.tile[size_t(t) * nk + e];
+    k.cfg[size_t(c) * nk + e] = s;
+  }
+}
+
+// slope weights of the column of knot m of table T: ds_k/dy_m for
+// k = m - 2 + q, q = 0..4 (zero outside the table)
+__device__ inline void ks_slope_weights(const JacArgs &a, const TableDesc &T, int m, float Sw[5]) {
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    const int kk = m - 2 + q;
+    Sw[q] = (kk >= 0 && kk < T.n) ? float(a.tS[size_t(T.off + kk) * 5 + (4 - q)]) : 0.0f;
+  }
+}
+
+// pair-table columns from the knot sums: the force rows of every atom, and
+// the energy and virial partials of a configuration carried by its first
+// tile (its other tiles carry zeros, so the per-configuration sums over tiles
+// are the configuration's)
+// (each wave copies its atom's sums into LDS with whole-wave float4 loads,
+// one pass over the block, and the columns fold them from there)
+template <int NS>
+__global__ __launch_bounds__(256) void k_jac_pair_ks(JacArgs a, KsArgs k, const SlotDesc *slots) {
+  extern __shared__ float4 smem[];
+  TableDesc *sT = reinterpret_cast<TableDesc *>(smem);
+  const int ndq = a.ntab * int(sizeof(TableDesc) / sizeof(float4));
+  for (int e = threadIdx.x; e < ndq; e += blockDim.x)
+    smem[e] = reinterpret_cast<const float4 *>(a.tabs)[e];
+  __syncthreads();
+  const int wave = uni(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  float *sK = reinterpret_cast<float *>(smem + ndq) + wave * 6 * k.lp; // the atom's sums
+  const int nq = (6 * k.lp) >> 2;
+  int ccol[NS], cm[NS], cn[NS], cta[NS], ctb[NS], coffa[NS], coffb[NS];
+  float Sw[NS][5];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const SlotDesc sd = slots[s * 64 + lane];
+    ccol[s] = sd.kind == CK_PAIR ? sd.col : -1;
+    cm[s] = sd.m;
+    cn[s] = 1;
+    cta[s] = ctb[s] = -1;
+    coffa[s] = coffb[s] = 0;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) Sw[s][q] = 0.0f;
+    if (sd.kind == CK_PAIR) {
+      const TableDesc &T = sT[sd.tab];
+      cn[s] = T.n;
+      cta[s] = sd.dc & 0xff;
+      ctb[s] = (sd.dc >> 8) & 0xff;
+      coffa[s] = ks_pair_off(sT, a.ntypes, cta[s], ctb[s]); // centre of type ta
+      coffb[s] = ks_pair_off(sT, a.ntypes, ctb[s], cta[s]); // centre of type tb
+      ks_slope_weights(a, T, sd.m, Sw[s]);
+    }
+  }
+  const int ng = min(kXcds, int(gridDim.x));
+  const int xcd = blockIdx.x % ng, slot = blockIdx.x / ng;
+  const int nslot = (int(gridDim.x) - xcd + ng - 1) / ng;
+  const int t_end = int(int64_t(xcd + 1) * a.ntiles / ng);
+  for (int tile = int(int64_t(xcd) * a.ntiles / ng) + slot; tile < t_end; tile += nslot) {
+    const int na = uni(a.tile_na[tile]);
+    const int c = uni(a.tile_cfg[tile]);
+    const int64_t cbase = uni64(a.atom_off[c]);
+    const int64_t row0 = a.row_off[c];
+    for (int li = wave; li < na; li += 4) {
+      const int64_t i = int64_t(uni(a.tile_a0[tile])) + li;
+      const int ti = uni(a.types[i]);
+      const float4 *src = reinterpret_cast<const float4 *>(k.pair + size_t(i) * 6 * k.lp);
+      for (int e = lane; e < nq; e += 64) reinterpret_cast<float4 *>(sK)[e] = src[e];
+      wave_sync();
+      const float *Yb = sK, *Sb = sK + 3 * k.lp;
+      const int64_t row = row0 + 3 * (i - cbase);
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        if (ccol[s] < 0) continue;
+        float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
+        const int off = ti == cta[s] ? coffa[s] : (ti == ctb[s] ? coffb[s] : -1);
+        if (off >= 0) {
+          const int m = cm[s], L = off + m;
+          f0 = Yb[3 * L];
+          f1 = Yb[3 * L + 1];
+          f2 = Yb[3 * L + 2];
+#pragma unroll
+          for (int q = 0; q < 5; ++q) {
+            const int kk = off + min(max(m - 2 + q, 0), cn[s] - 1);
+            f0 += Sw[s][q] * Sb[3 * kk];
+            f1 += Sw[s][q] * Sb[3 * kk + 1];
+            f2 += Sw[s][q] * Sb[3 * kk + 2];
+          }
+        }
+        float *Jr = a.J + row * a.D + ccol[s];
+        Jr[0] = f0;
+        Jr[a.D] = f1;
+        Jr[2 * a.D] = f2;
+      }
+      wave_sync(); // every lane has folded before the next atom's copy
+    }
+  }
+}
+
+// energy and virial partials of the pair-table columns: a configuration's
+// sums are carried by its first tile, its other tiles carry zeros (the rows
+// sum a configuration's tiles); one block per configuration, one thread per
+// lane-slot column
+__global__ __launch_bounds__(256) void k_jac_pair_ks_rows(JacArgs a, KsArgs k, const SlotDesc *slots,
+                                                          int nslot) {
+  const int c = blockIdx.x;
+  const int t0 = a.cfg_tile0[c], t1 = a.cfg_tile0[c + 1];
+  const double *E = k.cfg + size_t(c) * 14 * k.kp;
+  for (int e = threadIdx.x; e < 64 * nslot; e += blockDim.x) {
+    const SlotDesc sd = slots[e];
+    if (sd.kind != CK_PAIR) continue;
+    const TableDesc T = a.tabs[sd.tab];
+    float Sw[5];
+    ks_slope_weights(a, T, sd.m, Sw);
+    double val[kJP];
+    const int K = T.off + sd.m;
+    val[0] = E[K];
+    for (int q = 0; q < 6; ++q) val[1 + q] = E[(2 + q) * k.kp + K];
+    val[7] = 0.0;
+    for (int q = 0; q < 5; ++q) {
+      const int kk = T.off + min(max(sd.m - 2 + q, 0), T.n - 1);
+      const double w = Sw[q];
+      val[0] += w * E[k.kp + kk];
+      for (int v = 0; v < 6; ++v) val[1 + v] += w * E[(8 + v) * k.kp + kk];
+    }
+    for (int t = t0; t < t1; ++t)
+      for (int q = 0; q < kJP; ++q) a.part[(int64_t(t) * kJP + q) * a.D + sd.col] = t == t0 ? val[q] : 0.0;
+  }
+}
+
+// H_i and Hs_i = F''(rho_i) H_i (0 for a clamped density) over the density
+// columns [dc0, dc0 + 256), from the density value sums; one wave per atom
+__global__ __launch_bounds__(256) void k_jac_dens_ks(JacArgs a, KsArgs k, int dc0) {
+  extern __shared__ float4 smem[];
+  const int wave = uni(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  float *sK = reinterpret_cast<float *>(smem) + wave * 2 * k.ld; // the atom's sums
+  const int nq = (2 * k.ld) >> 2;
+  int cdc[kCPL], cm[kCPL], cn[kCPL], coff[kCPL];
+  float Sw[kCPL][5];
+#pragma unroll
+  for (int s = 0; s < kCPL; ++s) {
+    const int dc = dc0 + lane + 64 * s;
+    cdc[s] = -1;
+    cm[s] = coff[s] = 0;
+    cn[s] = 1;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) Sw[s][q] = 0.0f;
+    if (dc < a.Dd) {
+      const ColDesc cd = a.cols[a.dens_c0 + dc];
+      const TableDesc T = a.tabs[cd.tab];
+      cdc[s] = dc;
+      cm[s] = cd.m;
+      cn[s] = T.n;
+      int o = 0;
+      for (int t = 0; t < cd.tab - a.npair; ++t) o += a.tabs[a.npair + t].n;
+      coff[s] = o;
+      ks_slope_weights(a, T, cd.m, Sw[s]);
+    }
+  }
+  const int64_t gw = int64_t(blockIdx.x) * 4 + wave;
+  const int64_t nw = int64_t(gridDim.x) * 4;
+  for (int64_t i = uni64(gw); i < a.N; i += nw) {
+    const float4 *src = reinterpret_cast<const float4 *>(k.dens + size_t(i) * 2 * k.ld);
+    for (int e = lane; e < nq; e += 64) reinterpret_cast<float4 *>(sK)[e] = src[e];
+    wave_sync();
+    const float *Yb = sK, *Sb = sK + k.ld;
+    const bool clamped = a.over[i] != 0.0;
+    const float hs = clamped ? 0.0f : float(a.Fpp[i]);
+#pragma unroll
+    for (int s = 0; s < kCPL; ++s) {
+      if (cdc[s] < 0) continue;
+      const int m = cm[s], o = coff[s];
+      float h = Yb[o + m];
+#pragma unroll
+      for (int q = 0; q < 5; ++q) h += Sw[s][q] * Sb[o + min(max(m - 2 + q, 0), cn[s] - 1)];
+      a.H[i * a.Dd + cdc[s]] = h;
+      a.Hs[i * a.Dd + cdc[s]] = hs * h;
+    }
+    wave_sync();
+  }
+}
+
+// ---------------------------------------------------------------------------
 // ADP Jacobian. The pair-table, density and embedding columns are those of the
 // EAM passes unchanged (the dipole and quadrupole terms do not depend on the
 // phi, rho and F tables). The dipole (u) and quadrupole (w) columns follow the
@@ -1836,6 +2247,7 @@ void launch_jacobian(Context &c) {
   } else {
     // column descriptors and lane-slot maps: built and uploaded once per model
     const bool adp = c.kind == FS_ADP;
+    if (!c.jl.valid || !c.jsort_valid) c.ks_valid = false; // model or neighbour list changed
     int Dd = 0, dens_c0 = 0, gi_nf = 0, NSP = 0, NSD = 0, NSU = 0, NSW = 0, Ddip = 0, Dquad = 0;
     if (!c.jl.valid) {
       // column descriptors
@@ -2014,6 +2426,59 @@ void launch_jacobian(Context &c) {
     }
     a.jrec = c.jrec.p;
     a.jr = c.jr.p;
+    // knot sums: the pair-table columns and H_i fold per-atom sums of the
+    // geometry, built once per neighbour list and model grid, while they fit
+    // (at most 128 pair-table and 128 density knots per atom, the build's LDS
+    // accumulators, 4 GiB of sums); else the window passes
+    KsArgs ks{};
+    bool use_ks = false;
+    {
+      int lp = 0, ld = 0, kp = 0;
+      for (int ti = 0; ti < c.ntypes; ++ti) {
+        int o = 0;
+        for (int t = 0; t < c.ntypes; ++t) o += c.h_tabs[ord_pair(ti, t, c.ntypes)].n;
+        lp = std::max(lp, o);
+      }
+      for (int t = 0; t < c.ntypes; ++t) ld += c.h_tabs[c.npair + t].n;
+      for (int t = 0; t < c.npair; ++t) kp += c.h_tabs[t].n;
+      // even entry counts: an atom's block is a whole number of float4
+      lp = (lp + 1) & ~1;
+      ld = (ld + 1) & ~1;
+      const size_t lds_ks = desc + 4 * ks_wave_bytes(kp);
+      const size_t bytes = size_t(c.N) * (6 * size_t(lp) + 2 * size_t(ld)) * sizeof(float);
+      const bool fits = lp <= 128 && ld <= 128 && lds_ks <= 64 * 1024 && bytes <= (size_t(4) << 30) &&
+                        (TSP == 0 || NSP == 2 || NSP == 4);
+      FS_REQUIRE(c.jac_form != 2 || fits, FS_ERR_UNSUPPORTED,
+                 "the knot-sum form of the Jacobian does not fit this model");
+      use_ks = fits && c.jac_form != 1 && c.N > 0;
+      if (use_ks) {
+        const bool build = !c.ks_valid || c.ks_lp != lp || c.ks_ld != ld || c.ks_kp != kp;
+        if (build) {
+          c.ks_pair.alloc(size_t(c.N) * 6 * lp);
+          c.ks_dens.alloc(size_t(c.N) * 2 * ld);
+          c.ks_tile.alloc(size_t(std::max(nt, 1)) * 14 * kp);
+          c.ks_cfg.alloc(size_t(c.C) * 14 * kp);
+        }
+        ks.pair = c.ks_pair.p;
+        ks.dens = c.ks_dens.p;
+        ks.tile = c.ks_tile.p;
+        ks.cfg = c.ks_cfg.p;
+        ks.lp = lp;
+        ks.ld = ld;
+        ks.kp = kp;
+        if (build) {
+          const unsigned gk = unsigned(std::max(1, std::min(nt, c.cu_count * 4)));
+          hipLaunchKernelGGL(k_knot_sums, dim3(gk), dim3(256), lds_ks, c.stream, a, ks);
+          FS_HIP(hipGetLastError());
+          hipLaunchKernelGGL(k_knot_sums_cfg, dim3(c.C), dim3(256), 0, c.stream, a, ks);
+          FS_HIP(hipGetLastError());
+          c.ks_valid = true;
+          c.ks_lp = lp;
+          c.ks_ld = ld;
+          c.ks_kp = kp;
+        }
+      }
+    }
     if (c.N > 0) {
       // (the pair pass depends on neither kernel below; run beside them on a
       // second stream it measured no faster and stretched both)
@@ -2023,7 +2488,15 @@ void launch_jacobian(Context &c) {
       // wider maps (more than 128 columns of a pass) keep the compiler's
       // occupancy rather than spill their larger slot state
       for (int g = 0; g < TSP; g += NSP) {
-        if (NSP == 2)
+        if (use_ks) {
+          const size_t lds = desc + 4 * 6 * sizeof(float) * size_t(ks.lp);
+          if (NSP == 2)
+            hipLaunchKernelGGL((k_jac_pair_ks<2>), dim3(g2), dim3(256), lds, c.stream, a, ks,
+                               d_slots + size_t(g) * 64);
+          else
+            hipLaunchKernelGGL((k_jac_pair_ks<4>), dim3(g2), dim3(256), lds, c.stream, a, ks,
+                               d_slots + size_t(g) * 64);
+        } else if (NSP == 2)
           hipLaunchKernelGGL((k_jac_pair<2, kWavesPair>), dim3(g2), dim3(256), lds_pair, c.stream, a,
                              c.nknots, d_slots + size_t(g) * 64);
         else
@@ -2031,12 +2504,20 @@ void launch_jacobian(Context &c) {
                              d_slots + size_t(g) * 64);
         FS_HIP(hipGetLastError());
       }
+      if (use_ks && TSP > 0) {
+        hipLaunchKernelGGL(k_jac_pair_ks_rows, dim3(c.C), dim3(256), 0, c.stream, a, ks, d_slots, TSP);
+        FS_HIP(hipGetLastError());
+      }
       c.time_main_end(1);
       hipLaunchKernelGGL(k_emb_sites, dim3(nb(c.N, 256)), dim3(256), 0, c.stream, a);
       FS_HIP(hipGetLastError());
       const unsigned g1 = unsigned(std::min<int64_t>((c.N + 3) / 4, int64_t(c.cu_count) * 16));
       for (int dc0 = 0; dc0 < Dd; dc0 += 64 * kCPL) {
-        hipLaunchKernelGGL(k_jac_dens, dim3(g1), dim3(256), lds_dens, c.stream, a, c.nknots, dc0);
+        if (use_ks)
+          hipLaunchKernelGGL(k_jac_dens_ks, dim3(g1), dim3(256), 4 * 2 * sizeof(float) * size_t(ks.ld),
+                             c.stream, a, ks, dc0);
+        else
+          hipLaunchKernelGGL(k_jac_dens, dim3(g1), dim3(256), lds_dens, c.stream, a, c.nknots, dc0);
         FS_HIP(hipGetLastError());
       }
       c.time_main_begin(0);

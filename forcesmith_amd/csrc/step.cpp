@@ -32,7 +32,10 @@ std::vector<uintptr_t> step_key(const Context &c) {
       uintptr_t(c.R_cfg), uintptr_t(c.n_smooth), uintptr_t(c.ntab), uintptr_t(c.nknots),
       uintptr_t(c.nivl), uintptr_t(c.ntiles), uintptr_t(c.njt), uintptr_t(c.jt_atoms),
       uintptr_t(c.lanes_per_atom), uintptr_t(c.blocks_per_cu), uintptr_t(c.jac_blocks_per_cu),
-      uintptr_t(c.precision), uintptr_t(c.kind), uintptr_t(c.ntypes)};
+      uintptr_t(c.precision), uintptr_t(c.kind), uintptr_t(c.ntypes), uintptr_t(c.jac_form),
+      // cached layouts and sorted pairs the eager step rebuilds after a model
+      // upload or a neighbour build (the captured launches do not)
+      uintptr_t(c.jl.valid), uintptr_t(c.jsort_valid), uintptr_t(c.ks_valid)};
   for (double v : {c.energy_weight, c.stress_weight, c.smooth_weight}) {
     uint64_t u;
     std::memcpy(&u, &v, 8);
@@ -49,7 +52,8 @@ std::vector<uintptr_t> step_key(const Context &c) {
       ad(c.tile_part), ad(c.cfg_E), ad(c.cfg_S), ad(c.cfg_L), ad(c.residual), ad(c.cost),
       ad(c.cost_part), ad(c.J), ad(c.H), ad(c.jac_part), ad(c.jt_cfg), ad(c.jt_a0), ad(c.jt_na),
       ad(c.jt_ct0), ad(c.jcols), ad(c.jslots), ad(c.jrec), ad(c.jr), ad(c.Hs), ad(c.emb_d),
-      ad(c.emb_v), ad(c.emb_k), ad(c.adp_tan), ad(c.step_x), ad(c.gauge)};
+      ad(c.emb_v), ad(c.emb_k), ad(c.adp_tan), ad(c.step_x), ad(c.gauge), ad(c.ks_pair),
+      ad(c.ks_dens), ad(c.ks_cfg)};
   k.insert(k.end(), std::begin(ptrs), std::end(ptrs));
   return k;
 }

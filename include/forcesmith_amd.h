@@ -326,6 +326,15 @@ int fs_lm_last_timing(fs_ctx *ctx, double *ms);
  * (jacobian_tile: the tile size of the last Jacobian, or the requested one
  * before the first; jacobian_blocks_per_cu: the Jacobian passes' grid). No reference counterpart (launch tuning). */
 int fs_set_jacobian_tile(fs_ctx *ctx, int32_t atoms);
+/* Form of the Jacobian's pair-table columns and density rows: 0 (default)
+ * the knot-sum form while it fits (per-atom sums of the pair geometry, built
+ * once per neighbour list and model grid and folded with the current slope
+ * band at every Jacobian; at most 128 pair-table and 128 density knots per
+ * atom), else the window passes; 1 the window passes; 2 the knot-sum form
+ * (FS_ERR_UNSUPPORTED from the Jacobian when it does not fit). Both forms
+ * give the same columns to FP32 rounding. No reference counterpart (kernel
+ * choice). */
+int fs_set_jacobian_form(fs_ctx *ctx, int32_t form);
 int fs_get_launch(fs_ctx *ctx, int32_t *lanes_per_atom, int32_t *blocks_per_cu,
                   int32_t *jacobian_tile, int32_t *jacobian_blocks_per_cu);
 int fs_autotune(fs_ctx *ctx, int32_t *lanes_per_atom);

@@ -57,6 +57,27 @@ def test_jacobian_matches_oracle(wl, name):
     assert m <= 5e-5
 
 
+@pytest.mark.parametrize("name", ["eam1", "eam3s", "adp2", "adp2all"])
+def test_jacobian_forms_agree(wl, name):
+    # the knot-sum and window forms of the pair-table columns and density
+    # rows sum the same FP32 terms in another grouping; a repeat of either is
+    # bit-identical
+    w = wl[name]
+    print(f"jac forms {name}", flush=True)
+    g = fa.GpuCalculator(w.dataset, w.model)
+    g.prepare()
+    x = g.gather_params()
+    g.set_jacobian_form(1)
+    Jw = g.jacobian(x).astype(np.float64)
+    g.set_jacobian_form(2)
+    Jk = g.jacobian(x).astype(np.float64)
+    Jk2 = g.jacobian(x).astype(np.float64)
+    m = metric(Jk, Jw)
+    print(f"[{name}] knot sums vs windows {m:.3e}", flush=True)
+    assert m <= 2e-5
+    assert np.array_equal(Jk, Jk2)
+
+
 @pytest.mark.parametrize("name", ["eam3s", "sw", "adp2"])
 def test_jacobian_tile_sizes_agree(wl, name):
     # every atom's force rows come from one wave whatever the tile, so they

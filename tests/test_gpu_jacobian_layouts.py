@@ -11,6 +11,10 @@ other branch:
   - 75 density columns (more than one slot, so the unbalanced windows);
   - density + embedding columns in four slots, and in six;
   - density tables with unequal free-knot counts (one table partly fixed).
+Each case runs both forms of the pair-table columns and density rows (window
+walks and knot sums, fs_set_jacobian_form). A pair table of 150 knots is
+beyond the knot sums' width: there the knot-sum form is refused and the
+default falls back to the windows.
 
 Bars: max |J - J_oracle| / (1 + |J_oracle|) <= 5e-5 and
 |J - J_oracle|_F <= 1e-6 |J_oracle|_F.
@@ -48,23 +52,27 @@ def relayout(model: TabulatedModel, spec):
     return TabulatedModel(model.kind, model.ntypes, tabs, model.energy_weight, model.stress_weight)
 
 
-def check(ds, model, tag):
+def check(ds, model, tag, forms=(1, 2)):
     o = oracle.Oracle(ds, model)
     o.prepare()
     x = o.gather()
     Jo = o.jacobian(x)
     g = fa.GpuCalculator(ds, model)
     g.prepare()
-    Jg = g.jacobian(x).astype(np.float64)
-    assert Jg.shape == Jo.shape
-    m = float(np.max(np.abs(Jg - Jo) / (1.0 + np.abs(Jo))))
-    fro = float(np.linalg.norm(Jg - Jo) / np.linalg.norm(Jo))
-    print(f"[{tag}] D={Jg.shape[1]} J metric {m:.3e} relative Frobenius {fro:.3e}", flush=True)
-    assert m <= 5e-5
-    assert fro <= 1e-6
+    for form in forms:
+        g.set_jacobian_form(form)
+        Jg = g.jacobian(x).astype(np.float64)
+        assert Jg.shape == Jo.shape
+        m = float(np.max(np.abs(Jg - Jo) / (1.0 + np.abs(Jo))))
+        fro = float(np.linalg.norm(Jg - Jo) / np.linalg.norm(Jo))
+        print(f"[{tag}] form {form} D={Jg.shape[1]} J metric {m:.3e} relative Frobenius {fro:.3e}",
+              flush=True)
+        assert m <= 5e-5
+        assert fro <= 1e-6
     Fo = o.residual(x)
     Fg = g.residual(x)
     assert np.max(np.abs(Fg - Fo) / (1 + np.abs(Fo))) <= 1e-5
+    return g, x
 
 
 @pytest.fixture(scope="module")
@@ -97,3 +105,13 @@ def test_six_density_embedding_slots():
     m = relayout(w.model, {"pair": (None, False), "density": (100, True), "embedding": (150, True)})
     assert m.param_count == 250
     check(w.dataset, m, "single element density 100 + embedding 150")
+
+
+def test_knot_sums_refused_beyond_their_width():
+    w = dsm.config1(n_configs=2)
+    m = relayout(w.model, {"pair": (150, True), "density": (None, False), "embedding": (None, False)})
+    assert m.param_count == 150
+    g, x = check(w.dataset, m, "single element pair 150", forms=(1, 0))
+    g.set_jacobian_form(2)
+    with pytest.raises(fa.FsError):
+        g.jacobian(x)

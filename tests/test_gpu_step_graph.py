@@ -36,14 +36,30 @@ def test_replays_equal_eager_evaluations(name):
         assert np.array_equal(g.gather_params(), x)
 
 
+def _inner_ends_moved(model, dr):
+    # the same layout on other grids: the pair and density tables start dr
+    # further out (the cutoffs, and so the neighbour list, are unchanged)
+    from forcesmith_amd.model import Table, TabulatedModel
+    tabs = []
+    for (role, _), t in zip(model.table_roles(), model.tables):
+        lo = t.rmin + dr if role in ("pair", "density") else t.rmin
+        tabs.append(Table(t.n, lo, t.rmax, t.y.copy(), t.fixed.copy()))
+    return TabulatedModel(model.kind, model.ntypes, tabs, model.energy_weight, model.stress_weight)
+
+
 def test_recapture_after_layout_changes():
+    # weights, launch shape, Jacobian tile, the form of the pair-table
+    # columns, and a model upload on other grids (the knot sums and the
+    # sorted pair flags are rebuilt by an eager step before the recapture)
     w = dsm.config2(n_configs=6, nmin=32, nmax=300)
     g = fa.GpuCalculator(w.dataset, w.model)
     g.prepare()
     x = g.gather_params()
     g.step(x)
     for change in (lambda: g._check(g._lib.fs_set_weights(g._h, 0.7, 0.2)),
-                   lambda: g.set_launch(16, 4), lambda: g.set_jacobian_tile(8)):
+                   lambda: g.set_launch(16, 4), lambda: g.set_jacobian_tile(8),
+                   lambda: g.set_jacobian_form(1), lambda: g.set_jacobian_form(0),
+                   lambda: g.set_model(_inner_ends_moved(w.model, 0.05))):
         change()
         for xx in _xs(x, 2):
             cost, F, J = g.step(xx, want_residual=True, want_jacobian=True)
