@@ -117,11 +117,10 @@ __device__ inline int pair_interval(const TableDesc &T, double r, float &t) {
   return k;
 }
 
-// interval + value / derivative basis of a pair table at r (cached-site
-// semantics: linear site only at the span ends or for n < 4 tables)
-__device__ inline int pair_basis(const TableDesc &T, double r, float bv[4], float bd[4]) {
-  float t;
-  const int k = pair_interval(T, r, t);
+// value / derivative basis of a pair table at offset t of its interval
+// (cached-site semantics: linear site only at the span ends or for n < 4
+// tables)
+__device__ inline void basis_at(const TableDesc &T, float t, float bv[4], float bd[4]) {
   if (T.linear) {
     bv[0] = 1.0f - t;
     bv[1] = t;
@@ -129,7 +128,7 @@ __device__ inline int pair_basis(const TableDesc &T, double r, float bv[4], floa
     bd[0] = -T.inv_hf;
     bd[1] = T.inv_hf;
     bd[2] = bd[3] = 0.0f;
-    return k;
+    return;
   }
   const float h = T.hf, omt = 1.0f - t;
   bv[0] = omt * omt * (1.0f + 2.0f * t);
@@ -140,6 +139,13 @@ __device__ inline int pair_basis(const TableDesc &T, double r, float bv[4], floa
   bd[1] = 6.0f * t * omt * T.inv_hf;
   bd[2] = omt * (1.0f - 3.0f * t);
   bd[3] = t * (3.0f * t - 2.0f);
+}
+
+// interval + value / derivative basis of a pair table at r
+__device__ inline int pair_basis(const TableDesc &T, double r, float bv[4], float bd[4]) {
+  float t;
+  const int k = pair_interval(T, r, t);
+  basis_at(T, t, bv, bd);
   return k;
 }
 
@@ -627,32 +633,34 @@ __device__ __forceinline__ void jac_main_body(const JacArgs &a, int nknots, cons
             float dgj = 0.0f;
             {
               const TableDesc T = sT[a.npair + tj];
-              const int k = pair_basis(T, r64, bv, bd);
+              float tt;
+              const int k = pair_interval(T, r64, tt);
+              basis_at(T, tt, bv, bd);
               const bool on = (b & kFlagGj) && !tiny;
               band6(sS, T, k, bd, on ? Fpi : 0.0f, cc);
               P.kgj[lane] = k;
 #pragma unroll
               for (int d = 0; d < 6; ++d) P.cgj[d][lane] = cc[d];
               if (on) {
-                float tt, v;
-                const int kk = pair_interval(T, r64, tt);
-                spl_eval32(T, a.ivl32, kk, tt, v, dgj);
+                float v;
+                spl_eval32(T, a.ivl32, k, tt, v, dgj);
               }
             }
             // density of i, weight F'_j; g'_ti(r) weights the embedding / H terms
             float dgi = 0.0f;
             {
               const TableDesc T = sT[a.npair + ti];
-              const int k = pair_basis(T, r64, bv, bd);
+              float tt;
+              const int k = pair_interval(T, r64, tt);
+              basis_at(T, tt, bv, bd);
               const bool on = (b & kFlagGi) && !tiny;
               band6(sS, T, k, bd, on ? fpj : 0.0f, cc);
               P.kgi[lane] = k;
 #pragma unroll
               for (int d = 0; d < 6; ++d) P.cgi[d][lane] = cc[d];
               if (on) {
-                float tt, v;
-                const int kk = pair_interval(T, r64, tt);
-                spl_eval32(T, a.ivl32, kk, tt, v, dgi);
+                float v;
+                spl_eval32(T, a.ivl32, k, tt, v, dgi);
               }
             }
             P.hw0[lane] = dgi * u0;
