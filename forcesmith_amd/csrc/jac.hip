@@ -184,13 +184,16 @@ struct JacArgs {
   const double *gauge;    // EAM gauge {A, S[kMaxTypes], F0[kMaxTypes]}
   // knot stride of the pair pass's window tables (0: binary searches)
   int win_pair;
+  // the density tables' intervals [ivl_lo, ivl_lo + nivl_st), their FP32
+  // forms staged in LDS when stage_ivl
+  int ivl_lo, nivl_st, stage_ivl;
 };
 
 // LDS staging of the descriptors and, when it fits the staging budget, the
 // FP32 slope bands (else sS points at their FP32 copy in global memory);
 // returns the start of the per-wave staging area that follows
 __device__ inline void *stage(const JacArgs &a, float4 *smem, TableDesc *&sT, const float *&sS,
-                              int nk) {
+                              int nk, const float4 **sI = nullptr) {
   sT = reinterpret_cast<TableDesc *>(smem);
   const float4 *src = reinterpret_cast<const float4 *>(a.tabs);
   const int nq = a.ntab * int(sizeof(TableDesc) / sizeof(float4));
@@ -202,6 +205,17 @@ __device__ inline void *stage(const JacArgs &a, float4 *smem, TableDesc *&sT, co
     after += (nk * 5 + 3) & ~3;
   } else {
     sS = a.tS32;
+  }
+  // the FP32 interval forms of the density tables (the staging's spline
+  // derivatives), intervals [ivl_lo, ivl_lo + nivl_st), while they fit; *sI
+  // is indexed by interval like the global array
+  if (a.stage_ivl) {
+    float4 *d = reinterpret_cast<float4 *>(after);
+    for (int k = threadIdx.x; k < 2 * a.nivl_st; k += blockDim.x) d[k] = a.ivl32[2 * a.ivl_lo + k];
+    if (sI) *sI = d - 2 * a.ivl_lo;
+    after += 8 * a.nivl_st;
+  } else if (sI) {
+    *sI = a.ivl32;
   }
   __syncthreads();
   return after;
@@ -538,7 +552,8 @@ __device__ __forceinline__ void jac_main_body(const JacArgs &a, int nknots, cons
   extern __shared__ float4 smem[];
   TableDesc *sT;
   const float *sS;
-  Stage *sAll = reinterpret_cast<Stage *>(stage(a, smem, sT, sS, nknots));
+  const float4 *sI;
+  Stage *sAll = reinterpret_cast<Stage *>(stage(a, smem, sT, sS, nknots, &sI));
   // the tile-end reduction reuses the staging area of the four waves: one
   // entry per partial row and slot column (s, lane)
   static_assert(4 * sizeof(Stage) >= sizeof(double) * kJP * 64 * NS, "reduction buffer");
@@ -643,7 +658,7 @@ __device__ __forceinline__ void jac_main_body(const JacArgs &a, int nknots, cons
               for (int d = 0; d < 6; ++d) P.cgj[d][lane] = cc[d];
               if (on) {
                 float v;
-                spl_eval32(T, a.ivl32, k, tt, v, dgj);
+                spl_eval32(T, sI, k, tt, v, dgj);
               }
             }
             // density of i, weight F'_j; g'_ti(r) weights the embedding / H terms
@@ -660,7 +675,7 @@ __device__ __forceinline__ void jac_main_body(const JacArgs &a, int nknots, cons
               for (int d = 0; d < 6; ++d) P.cgi[d][lane] = cc[d];
               if (on) {
                 float v;
-                spl_eval32(T, a.ivl32, k, tt, v, dgi);
+                spl_eval32(T, sI, k, tt, v, dgi);
               }
             }
             P.hw0[lane] = dgi * u0;
@@ -689,6 +704,11 @@ __device__ __forceinline__ void jac_main_body(const JacArgs &a, int nknots, cons
             Gs[4] += gr * u0 * u2;
             Gs[5] += gr * u1 * u2;
           }
+        } else if constexpr (!kPairPass) {
+          // past the chunk's pairs: a zero-weight gather of row 0, so that the
+          // density-chain gathers run in whole batches
+          P.jrow[lane] = 0;
+          P.hw0[lane] = P.hw1[lane] = P.hw2[lane] = 0.0f;
         }
         stage_segments(ty, a.ntypes, lane, P.seg);
         const int32_t *seg = P.seg;
@@ -782,8 +802,10 @@ __device__ __forceinline__ void jac_main_body(const JacArgs &a, int nknots, cons
               return *reinterpret_cast<const float *>(Hb + ((uint32_t(P.jrow[q]) << 2) + colb));
             };
             // two batches in flight: batch b + 1's loads are issued before
-            // batch b is consumed (0.8 % faster than one batch at a time)
-            const int nbat = np / kHBatch;
+            // batch b is consumed (0.8 % faster than one batch at a time);
+            // the last batch is padded with the chunk's zero-weight lanes, so
+            // no pair waits alone for its row
+            const int nbat = (np + kHBatch - 1) / kHBatch;
             float hn[kHBatch];
             if (nbat > 0) {
 #pragma unroll
@@ -805,7 +827,6 @@ __device__ __forceinline__ void jac_main_body(const JacArgs &a, int nknots, cons
               for (int v = 0; v < kHBatch; ++v)
                 add_force(A[s], h[v], P.hw0[q0 + v], P.hw1[q0 + v], P.hw2[q0 + v]);
             }
-            for (int q = nbat * kHBatch; q < np; ++q) add_force(A[s], hs(q), P.hw0[q], P.hw1[q], P.hw2[q]);
           } else if (kind == CK_EMB) {
             // F'_j through the embedding window of j, tj == t
             const int t = ctab[s] - a.npair - a.ntypes;
@@ -2410,6 +2431,15 @@ void launch_jacobian(Context &c) {
     const size_t widest = std::max({sizeof(DensStage), sizeof(JStagePair), sizeof(JStageDE),
                                     sizeof(MomStage), sizeof(DQStage)});
     a.stage_S = desc + band + 4 * widest <= kTableStageBytes ? 1 : 0;
+    // the density tables' FP32 interval forms too, while everything still
+    // fits and the density/embedding pass keeps four blocks per CU (40 KiB)
+    a.ivl_lo = c.h_tabs[c.npair].ivl;
+    a.nivl_st = (c.npair + c.ntypes < c.ntab ? c.h_tabs[c.npair + c.ntypes].ivl : c.nivl) - a.ivl_lo;
+    const size_t ivl = size_t(a.nivl_st) * 2 * sizeof(float4);
+    a.stage_ivl = a.stage_S && desc + band + ivl + 4 * widest <= kTableStageBytes &&
+                          desc + band + ivl + 4 * sizeof(JStageDE) <= 40 * 1024
+                      ? 1
+                      : 0;
     a.tS32 = nullptr;
     if (!a.stage_S) {
       c.tab_S32.alloc(size_t(c.nknots) * 5);
@@ -2418,7 +2448,7 @@ void launch_jacobian(Context &c) {
       FS_HIP(hipGetLastError());
       a.tS32 = c.tab_S32.p;
     }
-    const size_t base = desc + (a.stage_S ? band : 0);
+    const size_t base = desc + (a.stage_S ? band : 0) + (a.stage_ivl ? ivl : 0);
     const size_t lds_dens = base + 4 * sizeof(DensStage);
     const size_t lds_pair = base + 4 * sizeof(JStagePair), lds_de = base + 4 * sizeof(JStageDE);
     FS_REQUIRE(lds_de <= 64 * 1024 && lds_pair <= 64 * 1024, FS_ERR_UNSUPPORTED,
