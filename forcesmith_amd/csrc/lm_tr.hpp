@@ -39,9 +39,59 @@ struct Opts {
   double xtol, ftol, gtol, factor;
 };
 
+// four lanes of doubles (GCC/Clang vector extension: AVX2 where the build
+// targets it, two SSE2 halves otherwise)
+typedef double v4d __attribute__((vector_size(32)));
+
+inline v4d load4(const double *p) {
+  v4d v;
+  __builtin_memcpy(&v, p, sizeof(v));
+  return v;
+}
+
+// dot products of a with four rows b0..b3 over [0, len): four-wide partial
+// sums, then the scalar tail (a fixed order: the result does not depend on
+// alignment). Measured against two partial sums per row and against eight
+// rows at a time: this form is the fastest at D = 240 (the loads bound it)
+inline void dot4(const double *a, const double *b0, const double *b1, const double *b2,
+                 const double *b3, int len, double out[4]) {
+  v4d s0 = {0, 0, 0, 0}, s1 = s0, s2 = s0, s3 = s0;
+  int k = 0;
+  for (; k + 4 <= len; k += 4) {
+    const v4d va = load4(a + k);
+    s0 += va * load4(b0 + k);
+    s1 += va * load4(b1 + k);
+    s2 += va * load4(b2 + k);
+    s3 += va * load4(b3 + k);
+  }
+  double r[4] = {(s0[0] + s0[1]) + (s0[2] + s0[3]), (s1[0] + s1[1]) + (s1[2] + s1[3]),
+                 (s2[0] + s2[1]) + (s2[2] + s2[3]), (s3[0] + s3[1]) + (s3[2] + s3[3])};
+  for (; k < len; ++k) {
+    r[0] += a[k] * b0[k];
+    r[1] += a[k] * b1[k];
+    r[2] += a[k] * b2[k];
+    r[3] += a[k] * b3[k];
+  }
+  for (int q = 0; q < 4; ++q) out[q] = r[q];
+}
+
+inline double dot1(const double *a, const double *b, int len) {
+  v4d s = {0, 0, 0, 0};
+  int k = 0;
+  for (; k + 4 <= len; k += 4) s += load4(a + k) * load4(b + k);
+  double r = (s[0] + s[1]) + (s[2] + s[3]);
+  for (; k < len; ++k) r += a[k] * b[k];
+  return r;
+}
+
 // symmetric-pivoted Cholesky P^T M P = L L^T of a D x D row-major matrix
-// (plus an optional diagonal), factor in the lower triangle of W; stops at the
-// first pivot not above tol_rel times the first (rank = that position)
+// (plus an optional diagonal), factor in the lower triangle of W (row p holds
+// L's row of pivot position p); stops at the first pivot not above tol_rel
+// times the first (rank = that position). Left-looking: column j of L is
+// (M_(perm i, perm j) - L_i,<j . L_j,<j) / L_jj, so every step is a set of
+// contiguous dot products over rows already computed (four rows at a time),
+// and the remaining diagonal, kept current, picks the pivot. M is read in
+// place (symmetric: row perm[j] holds column perm[j]).
 struct PivChol {
   int n = 0, rank = 0;
   std::vector<double> W, col;
@@ -49,43 +99,52 @@ struct PivChol {
 
   void factor(const double *M, int n_, const double *add_diag, double tol_rel) {
     n = n_;
-    W.assign(M, M + size_t(n) * n);
-    if (add_diag)
-      for (int k = 0; k < n; ++k) W[size_t(k) * n + k] += add_diag[k];
+    W.resize(size_t(n) * n);
     perm.resize(n);
-    col.resize(n);
-    for (int k = 0; k < n; ++k) perm[k] = k;
-    auto at = [&](int i, int j) -> double & { return W[size_t(i) * n + j]; };
+    col.resize(n); // the remaining diagonal of each position
+    for (int k = 0; k < n; ++k) {
+      perm[k] = k;
+      col[k] = M[size_t(k) * n + k] + (add_diag ? add_diag[k] : 0.0);
+    }
     double d0 = 0.0;
     rank = n;
     for (int j = 0; j < n; ++j) {
       int p = j;
       for (int q = j + 1; q < n; ++q)
-        if (at(q, q) > at(p, p)) p = q;
+        if (col[q] > col[p]) p = q;
       if (p != j) {
-        // swap positions j and p in the lower-triangle storage
-        for (int k = 0; k < j; ++k) std::swap(at(j, k), at(p, k));
-        std::swap(at(j, j), at(p, p));
-        for (int k = j + 1; k < p; ++k) std::swap(at(k, j), at(p, k));
-        for (int k = p + 1; k < n; ++k) std::swap(at(k, j), at(k, p));
+        // positions j and p trade places: their computed rows of L, their
+        // indices and remaining diagonals
+        double *rj = &W[size_t(j) * n], *rp = &W[size_t(p) * n];
+        for (int k = 0; k < j; ++k) std::swap(rj[k], rp[k]);
         std::swap(perm[j], perm[p]);
+        std::swap(col[j], col[p]);
       }
-      const double d = at(j, j);
+      const double d = col[j];
       if (j == 0) d0 = d;
       if (!(d > 0.0) || !(d > tol_rel * d0)) {
         rank = j;
         break;
       }
       const double l = std::sqrt(d);
-      at(j, j) = l;
-      for (int i = j + 1; i < n; ++i) {
-        at(i, j) /= l;
-        col[i] = at(i, j);
+      double *Lj = &W[size_t(j) * n];
+      Lj[j] = l;
+      const double *Mj = M + size_t(perm[j]) * n;
+      int i = j + 1;
+      for (; i + 4 <= n; i += 4) {
+        double s[4];
+        dot4(Lj, &W[size_t(i) * n], &W[size_t(i + 1) * n], &W[size_t(i + 2) * n],
+             &W[size_t(i + 3) * n], j, s);
+        for (int q = 0; q < 4; ++q) {
+          const double v = (Mj[perm[i + q]] - s[q]) / l;
+          W[size_t(i + q) * n + j] = v;
+          col[i + q] -= v * v;
+        }
       }
-      for (int i = j + 1; i < n; ++i) {
-        const double ci = col[i];
-        double *wi = &W[size_t(i) * n];
-        for (int k = j + 1; k <= i; ++k) wi[k] -= ci * col[k];
+      for (; i < n; ++i) {
+        const double v = (Mj[perm[i]] - dot1(Lj, &W[size_t(i) * n], j)) / l;
+        W[size_t(i) * n + j] = v;
+        col[i] -= v * v;
       }
     }
   }
