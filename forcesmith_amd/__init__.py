@@ -40,7 +40,7 @@ EXPORTS = (
     "fs_lm_minimize", "fs_lm_solve", "fs_normal_equations_solve", "fs_lm_step", "fs_population_cost", "fs_de_minimize", "fs_de_last_stats", "fs_device_buffer", "fs_synchronize",
     "fs_eam_set_gauge", "fs_eam_get_gauge", "fs_eam_rho_stats", "fs_eam_rescale_rho_axis",
     "fs_eam_embed_shift", "fs_eam_rescale", "fs_lm_last_timing",
-    "fs_set_launch", "fs_set_jacobian_tile", "fs_set_jacobian_form", "fs_get_launch", "fs_autotune", "fs_time_step", "fs_mlp_forward", "fs_mlp_loss_grad",
+    "fs_set_launch", "fs_set_jacobian_tile", "fs_set_jacobian_form", "fs_set_population_form", "fs_get_launch", "fs_autotune", "fs_time_step", "fs_mlp_forward", "fs_mlp_loss_grad",
 )
 
 
@@ -140,6 +140,7 @@ def load_library(path: str = LIB_PATH):
         "fs_set_launch": (ct.c_int, [vp, ct.c_int32, ct.c_int32]),
         "fs_set_jacobian_tile": (ct.c_int, [vp, ct.c_int32]),
         "fs_set_jacobian_form": (ct.c_int, [vp, ct.c_int32]),
+        "fs_set_population_form": (ct.c_int, [vp, ct.c_int32]),
         "fs_get_launch": (ct.c_int, [vp, _i32p, _i32p, _i32p, _i32p]),
         "fs_autotune": (ct.c_int, [vp, _i32p]),
         "fs_time_step": (ct.c_int, [vp, ct.c_int32, ct.c_int32, _dp]),
@@ -481,6 +482,11 @@ class GpuCalculator:
         """Form of the pair-table columns and density rows: 0 knot sums while
         they fit (else window passes), 1 window passes, 2 knot sums."""
         self._check(self._lib.fs_set_jacobian_form(self._h, int(form)))
+
+    def set_population_form(self, form: int = 0):
+        """Form of the batched population cost: 0 member-lane passes where
+        they apply (one member per lane, FP64), 1 member-block passes."""
+        self._check(self._lib.fs_set_population_form(self._h, int(form)))
 
     def launch_shape(self) -> dict:
         lpa, bpc, jt, jb = ct.c_int32(), ct.c_int32(), ct.c_int32(), ct.c_int32()

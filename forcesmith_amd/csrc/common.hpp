@@ -329,6 +329,19 @@ struct Context {
   // members per block of the population passes (1, 2 or 4); 0 until the DE
   // driver or fs_autotune has timed them (1 meanwhile)
   int pop_members = 0;
+  // member-lane population passes (pair and density tables on one grid):
+  // wave tiles (up to 32 atoms of one type in one configuration, type-major),
+  // ranges of them per work item, the configurations' tile lists; members
+  // packed 64 to a group, member-minor
+  bool pop_ml = false, pop_ml_allowed = true;
+  uint64_t pop_ml_key[4] = {~0ull, ~0ull, ~0ull, ~0ull};
+  int pop_ml_nwt = 0, pop_ml_nrange = 0, pop_ml_n = 0, pop_ml_xoff = 0, pop_ml_nemb = 0;
+  int pop_ml_ng = 0; // member groups allocated
+  DBuf<int32_t> pop_ml_wt_atom, pop_ml_wt_off, pop_ml_wt_cfg, pop_ml_rng_type, pop_ml_rng_wt0;
+  DBuf<int32_t> pop_ml_cfg_off, pop_ml_cfg_wt;
+  DBuf<double2> pop_ml_ys;
+  DBuf<double> pop_ml_Fp, pop_ml_pE, pop_ml_pF;
+  uint64_t nl_gen = 0; // bumped by every neighbour build
   fs_de_stats de_stats{};          // of the last fs_de_minimize call
   // of the last fs_lm_solve: whole solve, device calls, host algebra (ms)
   double lm_timing[3] = {0.0, 0.0, 0.0};

@@ -970,6 +970,13 @@ int fs_set_jacobian_form(fs_ctx *ctx, int32_t form) {
   });
 }
 
+int fs_set_population_form(fs_ctx *ctx, int32_t form) {
+  return guarded(ctx, [&](Context &c) {
+    FS_REQUIRE(form == 0 || form == 1, FS_ERR_ARG, "population form must be 0 or 1");
+    c.pop_ml_allowed = form == 0;
+  });
+}
+
 int fs_get_launch(fs_ctx *ctx, int32_t *lanes_per_atom, int32_t *blocks_per_cu,
                   int32_t *jacobian_tile, int32_t *jacobian_blocks_per_cu) {
   return guarded(ctx, [&](Context &c) {

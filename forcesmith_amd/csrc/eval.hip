@@ -1195,6 +1195,415 @@ __global__ void k_pop_sum(int nb, int C, const double *__restrict__ cost_mc, int
   cost[m] = s;
 }
 
+// ---------------------------------------------------------------------------
+// Member-lane population passes. When every pair and density table shares one
+// knot grid (the same abscissae, hence the same span), a pair's interval and
+// Hermite basis are the same for its three table roles and for every member.
+// A wave then holds 64 members, one per lane. It forms the sites of 64 pairs at
+// a time with lanes as pairs, then walks those pairs with each site broadcast
+// from its staging lane (v_readlane into scalar registers), every lane applying
+// its own member's knot values and slopes. Member data is member-minor, so
+// each member access is one contiguous 1 KB row: the knot values and slopes
+// ([group][knot][64] (y, s), staged in LDS), the embedding derivatives
+// ([group][atom][64], read for the partner of every pair).
+// The arithmetic is FP64 throughout, with the cached-site semantics of the
+// FP64 path (spl_locate, linear sites at the span ends, the Hermite basis
+// dotted with (y_i, y_i+1, s_i, s_i+1)). On gfx950, FP64 FMA issues at the
+// rate of unpacked FP32 FMA; the passes are bound by instruction issue and the
+// member rows read from LDS, which FP32 would halve only with hi/lo pairs
+// (the coefficient rounding of single FP32 values biases every pair of an
+// interval alike).
+// A block (16 waves, one per CU: the member rows fill the LDS) holds one
+// member group and the tables of one centre type: slots 0..nt-1 the pair
+// tables (ti, t), nt..2nt-1 the density tables, then (density pass) the
+// embedding table of ti. A work item is (member group, range of wave tiles of
+// one type); a wave tile is up to kMlTile atoms of one type in one
+// configuration, and writes its own partial sums per member, which
+// k_pop_ml_finalize sums per configuration in a fixed order. Waves take the
+// item's tiles from an LDS counter; the sums do not depend on which wave takes
+// which tile.
+// ---------------------------------------------------------------------------
+constexpr int kMlWaves = 16;
+constexpr int kMlTile = 32;
+constexpr int kMlE = 2; // density pass partials: energy (embedding, gauge shift, pair), limit
+constexpr int kMlF = 7; // force pass partials: |F - F_ref|^2, virial xx, yy, zz, xy, xz, yz
+constexpr int kMlRanges = 64; // wave-tile ranges per member group (split by pair counts)
+
+struct MlArgs {
+  int ng, nb;   // member groups, members
+  int n;        // knots of the shared grid
+  int linear;   // its tables are piecewise linear
+  int nknots;   // all tables' knots (rows of a member group)
+  int nwt, nrange;
+  const double *x;    // abscissae of the shared grid
+  int xrows;          // LDS entries (double2) before the member rows: abscissae
+  const double2 *ys;  // [ng][nknots][64] knot value and slope of each member
+  double *Fp;         // [ng][N][64] F'(rho) of each member
+  double *pE;         // [ng][nwt][kMlE][64]
+  double *pF;         // [ng][nwt][kMlF][64]
+  const int32_t *wt_atom, *wt_off, *wt_cfg; // wave tiles: atoms [wt_off[w], wt_off[w+1])
+  const int32_t *rng_type, *rng_wt0;        // range r: type, tiles [rng_wt0[r], rng_wt0[r+1])
+  const int32_t *cfg_off, *cfg_wt;          // a configuration's tiles
+};
+
+__device__ inline int ml_first(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// lane q's value in every lane
+__device__ inline int ml_lane(int v, int q) { return __builtin_amdgcn_readlane(v, q); }
+__device__ inline double ml_lane(double v, int q) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(u)), q));
+  const uint32_t hi = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(u >> 32)), q));
+  return __builtin_bit_cast(double, (uint64_t(hi) << 32) | lo);
+}
+
+// the cached site on the shared grid (site64's interval and basis); a linear
+// site as the basis (1 - w, w, 0, 0) and derivative (-1/dx, 1/dx, 0, 0)
+__device__ inline int ml_site(const MlArgs &p, const double *x, double r, double bv[4], double bd[4]) {
+  const int n = p.n;
+  int i;
+  bool lin;
+  if (r <= x[0]) {
+    i = 0;
+    lin = true;
+  } else if (r >= x[n - 1]) {
+    i = n - 2;
+    lin = true;
+  } else {
+    i = spl_locate(x, n, r);
+    lin = p.linear != 0;
+  }
+  const double dx = x[i + 1] - x[i];
+  if (lin) {
+    const double inv_dx = 1.0 / dx;
+    const double w = (r - x[i]) * inv_dx;
+    bv[0] = 1.0 - w;
+    bv[1] = w;
+    bv[2] = bv[3] = 0.0;
+    bd[0] = -inv_dx;
+    bd[1] = inv_dx;
+    bd[2] = bd[3] = 0.0;
+  } else {
+    herm_basis64(x[i], dx, r, bv, bd);
+  }
+  return i;
+}
+
+// basis . (y_k, y_k+1, s_k, s_k+1) of the member rows at r (knot k) and r + 64
+__device__ inline double ml_dot(const double2 *r, double b0, double b1, double b2, double b3) {
+  const double2 u = r[0], w = r[64];
+  return b0 * u.x + b1 * w.x + b2 * u.y + b3 * w.y;
+}
+
+// a member's embedding knots in LDS: row k at Y[64 k]
+struct MlKnots {
+  const double2 *Y;
+  __device__ double y(int k) const { return Y[64 * k].x; }
+  __device__ double s(int k) const { return Y[64 * k].y; }
+};
+
+// stage the member rows of item (g, type t): slots of 2 nt tables of n knots,
+// then nemb rows of the embedding table of t (density pass); every wave must
+// be past the previous item (the caller's barrier)
+__device__ inline void ml_stage(const EvalArgs &a, const MlArgs &p, double2 *sY, int g, int t,
+                                int nemb, int wave, int lane) {
+  const int nt = a.ntypes, n = p.n, nsl = 2 * nt * n;
+  const int eoff = a.tabs[a.npair + nt + t].off;
+  for (int row = wave; row < nsl + nemb; row += kMlWaves) {
+    int knot;
+    if (row < nsl) {
+      const int slot = row / n, k = row - slot * n;
+      const int tab = slot < nt ? ord_pair(t, slot, nt) : a.npair + slot - nt;
+      knot = a.tabs[tab].off + k;
+    } else {
+      knot = eoff + row - nsl;
+    }
+    sY[row * 64 + lane] = p.ys[(size_t(g) * p.nknots + knot) * 64 + lane];
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_pop_ml_density(EvalArgs a, MlArgs p) {
+  extern __shared__ double2 sY[];
+  __shared__ int s_next;
+  // abscissae first (the shared grid, then the item's embedding table), then
+  // the member rows
+  double *sX = reinterpret_cast<double *>(sY);
+  double2 *sR = sY + p.xrows;
+  const int wave = ml_first(int(threadIdx.x >> 6)), lane = threadIdx.x & 63;
+  const int nt = a.ntypes, n = p.n;
+  const int nitem = p.ng * p.nrange;
+  for (int item = blockIdx.x; item < nitem; item += gridDim.x) {
+    const int g = item / p.nrange, rg = item - g * p.nrange;
+    const int t = p.rng_type[rg];
+    const TableDesc Te = a.tabs[a.npair + nt + t];
+    __syncthreads(); // every wave is done with the previous item's rows
+    ml_stage(a, p, sR, g, t, Te.n, wave, lane);
+    for (int k = threadIdx.x; k < n + Te.n; k += blockDim.x)
+      sX[k] = k < n ? p.x[k] : a.tx[Te.off + k - n];
+    if (threadIdx.x == 0) s_next = p.rng_wt0[rg];
+    __syncthreads();
+    const MlKnots emb{sR + 2 * nt * n * 64 + lane};
+    const double *xe = sX + n;
+    const int w_end = p.rng_wt0[rg + 1];
+    for (;;) {
+      int w = 0;
+      if (lane == 0) w = atomicAdd(&s_next, 1);
+      w = ml_first(w);
+      if (w >= w_end) break;
+      double e_acc = 0.0, l_acc = 0.0;
+      const int a1 = p.wt_off[w + 1];
+      for (int ai = p.wt_off[w]; ai < a1; ++ai) {
+        const int64_t i = p.wt_atom[ai];
+        const int64_t p0 = a.pair_off[i], p1 = a.pair_off[i + 1];
+        double rho = 0.0, ep = 0.0;
+        for (int64_t pc = p0; pc < p1; pc += 64) {
+          const int np = int(min<int64_t>(64, p1 - pc));
+          // sites, lanes as pairs: meta = k | tj << 16 | pair-energy bit; a
+          // pair with no table role keeps a zero basis (it adds exact zeros,
+          // and the walk needs no branch)
+          int meta = 0;
+          double b0 = 0.0, b1 = 0.0, b2 = 0.0, b3 = 0.0;
+          if (lane < np) {
+            const int64_t q = pc + lane;
+            const uint32_t b = a.bits[q];
+            if (b & kFlagGj) {
+              double bv[4], bd[4];
+              const int k = ml_site(p, sX, a.r64[q], bv, bd);
+              const bool phi = (b & kFlagPhi) && !(b & kFlagTiny);
+              meta = k | (pj_type(b) << 16) | (phi ? (1 << 24) : 0);
+              b0 = bv[0];
+              b1 = bv[1];
+              b2 = bv[2];
+              b3 = bv[3];
+            }
+          }
+          // one pair ahead: pair q + 1's member rows are read while pair q is
+          // summed (two register sets, the walk unrolled by two)
+          auto rows = [&](int q, double2 (&R)[4]) {
+            const int m = ml_lane(meta, q);
+            const int k = m & 0xffff, tj = (m >> 16) & 0xff;
+            const double2 *rj = sR + ((nt + tj) * n + k) * 64 + lane;
+            const double2 *rp = sR + (tj * n + k) * 64 + lane;
+            R[0] = rj[0];
+            R[1] = rj[64];
+            R[2] = rp[0];
+            R[3] = rp[64];
+          };
+          auto sum = [&](int q, const double2 (&R)[4]) {
+            const double B0 = ml_lane(b0, q), B1 = ml_lane(b1, q), B2 = ml_lane(b2, q),
+                         B3 = ml_lane(b3, q);
+            rho += B0 * R[0].x + B1 * R[1].x + B2 * R[0].y + B3 * R[1].y;
+            if (ml_lane(meta, q) & (1 << 24)) ep += B0 * R[2].x + B1 * R[3].x + B2 * R[2].y + B3 * R[3].y;
+          };
+          // (an odd count walks the zero-basis lane np as well: the reads and
+          // sums are unconditional; scheduling barriers keep each read set
+          // ahead of the sums it does not feed)
+          const int npe = (np + 1) & ~1;
+          double2 RA[4], RB[4];
+          rows(0, RA);
+          for (int qq = 0; qq < npe; qq += 2) {
+            rows(qq + 1, RB);
+            __builtin_amdgcn_sched_barrier(0);
+            sum(qq, RA);
+            __builtin_amdgcn_sched_barrier(0);
+            rows(min(qq + 2, npe - 1), RA);
+            __builtin_amdgcn_sched_barrier(0);
+            sum(qq + 1, RB);
+            __builtin_amdgcn_sched_barrier(0);
+          }
+        }
+        // clamp and F(rho) under the gauge, as the single-member density pass
+        const GaugedRho gr = gauge_clamp(a.gauge, t, Te, rho);
+        double F, dF;
+        spl_uncached64_kv(Te, xe, emb, gr.u, F, dF);
+        p.Fp[(size_t(g) * a.N + i) * 64 + lane] = dF;
+        e_acc += F + gr.e_shift + 0.5 * ep;
+        l_acc += gr.lim;
+      }
+      double *pe = p.pE + (size_t(g) * p.nwt + w) * kMlE * 64 + lane;
+      pe[0] = e_acc;
+      pe[64] = l_acc;
+    }
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_pop_ml_force(EvalArgs a, MlArgs p) {
+  extern __shared__ double2 sY[];
+  __shared__ int s_next;
+  // abscissae first (the shared grid, then the item's embedding table), then
+  // the member rows
+  double *sX = reinterpret_cast<double *>(sY);
+  double2 *sR = sY + p.xrows;
+  const int wave = ml_first(int(threadIdx.x >> 6)), lane = threadIdx.x & 63;
+  const int nt = a.ntypes, n = p.n;
+  const int nitem = p.ng * p.nrange;
+  for (int item = blockIdx.x; item < nitem; item += gridDim.x) {
+    const int g = item / p.nrange, rg = item - g * p.nrange;
+    const int t = p.rng_type[rg];
+    __syncthreads();
+    ml_stage(a, p, sR, g, t, 0, wave, lane);
+    for (int k = threadIdx.x; k < n; k += blockDim.x) sX[k] = p.x[k];
+    if (threadIdx.x == 0) s_next = p.rng_wt0[rg];
+    __syncthreads();
+    const double *Fpg = p.Fp + size_t(g) * a.N * 64 + lane;
+    const double2 *rowi = sR + (nt + t) * n * 64 + lane; // density table of the centre type
+    const int w_end = p.rng_wt0[rg + 1];
+    for (;;) {
+      int w = 0;
+      if (lane == 0) w = atomicAdd(&s_next, 1);
+      w = ml_first(w);
+      if (w >= w_end) break;
+      const int64_t cbase = a.atom_off[p.wt_cfg[w]];
+      double r2 = 0.0, v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0, v4 = 0.0, v5 = 0.0;
+      const int a1 = p.wt_off[w + 1];
+      for (int ai = p.wt_off[w]; ai < a1; ++ai) {
+        const int64_t i = p.wt_atom[ai];
+        const int64_t p0 = a.pair_off[i], p1 = a.pair_off[i + 1];
+        const double fpi = Fpg[i * 64];
+        double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+        for (int64_t pc = p0; pc < p1; pc += 64) {
+          const int np = int(min<int64_t>(64, p1 - pc));
+          // sites, lanes as pairs: meta = k | tj << 16, the partner's index in
+          // its configuration, the derivative basis over r and the
+          // displacement. The tables share one span, so the pair,
+          // partner-density and centre-density gates agree; a pair outside it
+          // (or at r < 1e-14) keeps a zero basis and displacement and adds
+          // exact zeros.
+          int meta = 0, jl = 0;
+          double b0 = 0.0, b1 = 0.0, b2 = 0.0, b3 = 0.0, e0 = 0.0, e1 = 0.0, e2 = 0.0;
+          if (lane < np) {
+            const int64_t q = pc + lane;
+            const uint32_t b = a.bits[q];
+            if ((b & kFlagPhi) && !(b & kFlagTiny)) {
+              e0 = a.pd[3 * q];
+              e1 = a.pd[3 * q + 1];
+              e2 = a.pd[3 * q + 2];
+              double rr = e0 * e0 + e1 * e1;
+              rr = rr + e2 * e2;
+              const double r = sqrt(rr);
+              double bv[4], bd[4];
+              const int k = ml_site(p, sX, r, bv, bd);
+              const double inv_r = 1.0 / r;
+              meta = k | (pj_type(b) << 16);
+              jl = int(pj_local(b));
+              b0 = bd[0] * inv_r;
+              b1 = bd[1] * inv_r;
+              b2 = bd[2] * inv_r;
+              b3 = bd[3] * inv_r;
+            }
+          }
+          // one pair ahead: pair q + 1's member rows and partner F' are read
+          // while pair q is summed (two register sets, the walk unrolled by two)
+          auto rows = [&](int q, double2 (&R)[6], double &fj) {
+            const int m = ml_lane(meta, q);
+            const int k = m & 0xffff, tj = m >> 16;
+            const double2 *rp = sR + (tj * n + k) * 64 + lane;
+            const double2 *rj = sR + ((nt + tj) * n + k) * 64 + lane;
+            R[0] = rp[0];
+            R[1] = rp[64];
+            R[2] = rj[0];
+            R[3] = rj[64];
+            R[4] = rowi[k * 64];
+            R[5] = rowi[k * 64 + 64];
+            fj = Fpg[(cbase + ml_lane(jl, q)) * 64];
+          };
+          auto sum = [&](int q, const double2 (&R)[6], double fpj) {
+            const double B0 = ml_lane(b0, q), B1 = ml_lane(b1, q), B2 = ml_lane(b2, q),
+                         B3 = ml_lane(b3, q);
+            const double dphi = B0 * R[0].x + B1 * R[1].x + B2 * R[0].y + B3 * R[1].y;
+            const double dgj = B0 * R[2].x + B1 * R[3].x + B2 * R[2].y + B3 * R[3].y;
+            const double dgi = B0 * R[4].x + B1 * R[5].x + B2 * R[4].y + B3 * R[5].y;
+            const double fs = dphi + fpi * dgj + fpj * dgi;
+            const double D0 = ml_lane(e0, q), D1 = ml_lane(e1, q), D2 = ml_lane(e2, q);
+            const double fx = fs * D0, fy = fs * D1, fz = fs * D2;
+            f0 += fx;
+            f1 += fy;
+            f2 += fz;
+            v0 += D0 * fx;
+            v1 += D1 * fy;
+            v2 += D2 * fz;
+            v3 += D0 * fy;
+            v4 += D0 * fz;
+            v5 += D1 * fz;
+          };
+          // (an odd count walks the zero-basis lane np as well: the reads and
+          // sums are unconditional; scheduling barriers keep each read set
+          // ahead of the sums it does not feed)
+          const int npe = (np + 1) & ~1;
+          double2 RA[6], RB[6];
+          double fa = 0.0, fb = 0.0;
+          rows(0, RA, fa);
+          for (int qq = 0; qq < npe; qq += 2) {
+            rows(qq + 1, RB, fb);
+            __builtin_amdgcn_sched_barrier(0);
+            sum(qq, RA, fa);
+            __builtin_amdgcn_sched_barrier(0);
+            rows(min(qq + 2, npe - 1), RA, fa);
+            __builtin_amdgcn_sched_barrier(0);
+            sum(qq + 1, RB, fb);
+            __builtin_amdgcn_sched_barrier(0);
+          }
+        }
+        const double x0 = f0 - a.ref_force[3 * i], x1 = f1 - a.ref_force[3 * i + 1],
+                     x2 = f2 - a.ref_force[3 * i + 2];
+        r2 += x0 * x0 + x1 * x1 + x2 * x2;
+      }
+      // the virial rows take -1/2 d_p f_q per pair
+      double *pf = p.pF + (size_t(g) * p.nwt + w) * kMlF * 64 + lane;
+      pf[0] = r2;
+      pf[64] = -0.5 * v0;
+      pf[128] = -0.5 * v1;
+      pf[192] = -0.5 * v2;
+      pf[256] = -0.5 * v3;
+      pf[320] = -0.5 * v4;
+      pf[384] = -0.5 * v5;
+    }
+  }
+}
+
+// members into groups of 64, member-minor: ys[g][k][lane] = (y, s) of member
+// 64 g + lane at knot k (zeros past the batch)
+__global__ void k_pop_ml_pack(int nb, int nknots, const double *__restrict__ y,
+                              const double *__restrict__ s, double2 *__restrict__ ys) {
+  const int g = blockIdx.y, k = blockIdx.x * 4 + int(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (k >= nknots) return;
+  const int m = g * 64 + lane;
+  double2 v{0.0, 0.0};
+  if (m < nb) v = double2{y[size_t(m) * nknots + k], s[size_t(m) * nknots + k]};
+  ys[(size_t(g) * nknots + k) * 64 + lane] = v;
+}
+
+// cost of every (member, configuration) from its wave tiles' partials, in the
+// configuration's tile order; the rows as k_pop_finalize. Block (c, g), lane =
+// member of group g.
+__global__ void k_pop_ml_finalize(MlArgs p, int C, const double *__restrict__ vol,
+                                  const double *__restrict__ ref_energy,
+                                  const double *__restrict__ ref_stress, double ew, double sw,
+                                  double *__restrict__ cost_mc) {
+  const int c = blockIdx.x, g = blockIdx.y, lane = threadIdx.x;
+  const int m = g * 64 + lane;
+  if (m >= p.nb) return;
+  double e = 0.0, lim = 0.0, r2 = 0.0, vir[6] = {0, 0, 0, 0, 0, 0};
+  for (int q = p.cfg_off[c]; q < p.cfg_off[c + 1]; ++q) {
+    const size_t w = size_t(p.cfg_wt[q]);
+    const double *pe = p.pE + (size_t(g) * p.nwt + w) * kMlE * 64 + lane;
+    const double *pf = p.pF + (size_t(g) * p.nwt + w) * kMlF * 64 + lane;
+    e += pe[0];
+    lim += pe[64];
+    r2 += pf[0];
+    for (int k = 0; k < 6; ++k) vir[k] += pf[64 * (k + 1)];
+  }
+  const double er = ew * (e - ref_energy[c]);
+  double cost = r2 + er * er + lim * lim;
+  if (sw > 0.0) {
+    for (int v = 0; v < 6; ++v) {
+      const double sr = sw * (vir[v] / vol[c] - ref_stress[6 * c + v]);
+      cost += sr * sr;
+    }
+  }
+  cost_mc[size_t(m) * C + c] = cost;
+}
+
 EvalArgs make_args(Context &c) {
   EvalArgs a;
   a.N = c.N;
@@ -1322,12 +1731,128 @@ bool population_supported(const Context &c) {
   return c.kind == FS_EAM && c.ntiles > 0 && c.D > 0;
 }
 
+// LDS entries (double2) of the abscissae ahead of the member rows: the shared
+// grid's n and an embedding table's up to nemb, in whole 1 KB rows
+static size_t ml_xrows(int n, int nemb) { return (size_t(n + nemb + 127) / 128) * 64; }
+
+// LDS of the member-lane density pass (the force pass stages its first
+// 2 nt n rows), or 0 where those passes do not apply: they need an EAM model
+// whose pair and density tables share one grid (the same abscissae, so the
+// same span and interval for every table role of a pair), a neighbour list,
+// and a block's member rows within the device's LDS
+static size_t ml_lds_bytes(const Context &c) {
+  if (c.kind != FS_EAM || c.ntab == 0 || c.P == 0 || c.ntypes > kMaxTypes) return 0;
+  const int nt = c.ntypes, nsh = c.npair + nt;
+  const TableDesc &T0 = c.h_tabs[0];
+  if (T0.n < 2 || T0.n > 0xffff) return 0;
+  for (int t = 1; t < nsh; ++t) {
+    const TableDesc &T = c.h_tabs[t];
+    if (T.n != T0.n || T.linear != T0.linear) return 0;
+    for (int k = 0; k < T.n; ++k)
+      if (c.h_x[size_t(T.off) + k] != c.h_x[size_t(T0.off) + k]) return 0;
+  }
+  int nemb = 0;
+  for (int t = 0; t < nt; ++t) nemb = std::max(nemb, c.h_tabs[c.npair + nt + t].n);
+  const size_t lds = (ml_xrows(T0.n, nemb) + (size_t(2 * nt) * T0.n + nemb) * 64) * sizeof(double2);
+  int lmax = 0;
+  FS_HIP(hipDeviceGetAttribute(&lmax, hipDeviceAttributeMaxSharedMemoryPerBlock, c.device));
+  return lds + 64 <= size_t(lmax) ? lds : 0;
+}
+
 // members per batch: per member 12 N bytes of embedding derivatives, the
-// per-(config, group) partials and its tables; about 4 GiB per batch
+// per-(config, group) partials and its tables; about 4 GiB per batch. The
+// member-lane passes hold 8 N bytes of embedding derivatives and 72 bytes per
+// wave tile per member, in whole groups of 64, up to 16 GiB per batch.
 int population_batch_cap(const Context &c, int np) {
-  const size_t per = 12 * size_t(c.N) + size_t(c.C) * kPopG * 8 * (kPopE + kPopF) +
-                     size_t(c.nknots) * 8 * 7 + size_t(c.nivl) * 64 + size_t(c.C) * 8 + 8 * c.D;
+  const size_t tabs = size_t(c.nknots) * 8 * 7 + size_t(c.nivl) * 64 + size_t(c.C) * 8 + 8 * c.D;
+  if (c.pop_ml) {
+    const size_t per = 8 * size_t(c.N) + size_t(c.pop_ml_nwt) * 8 * (kMlE + kMlF) +
+                       size_t(c.nknots) * 16 + tabs;
+    size_t cap = std::max<size_t>(64, ((size_t(16) << 30) / per) & ~size_t(63));
+    return int(std::min<size_t>(size_t(std::max(np, 1)), cap));
+  }
+  const size_t per = 12 * size_t(c.N) + size_t(c.C) * kPopG * 8 * (kPopE + kPopF) + tabs;
   return int(std::max<size_t>(1, std::min<size_t>(size_t(std::max(np, 1)), (size_t(4) << 30) / per)));
+}
+
+// wave tiles of the member-lane passes (once per neighbour list and dataset):
+// the atoms of each type in atom order, split per configuration into tiles of
+// up to kMlTile; per type, ranges of tiles with about equal pair counts (about
+// kMlRanges in all); per configuration, its tiles in type order
+static void ml_layout(Context &c) {
+  const uint64_t key[4] = {c.nl_gen, uint64_t(c.N), uint64_t(c.ntypes), uint64_t(c.C)};
+  if (std::equal(key, key + 4, c.pop_ml_key)) return;
+  const int64_t N = c.N;
+  const int nt = c.ntypes;
+  std::vector<int64_t> off(size_t(N) + 1);
+  FS_HIP(hipMemcpyAsync(off.data(), c.pair_off.p, sizeof(int64_t) * (N + 1), hipMemcpyDeviceToHost,
+                        c.stream));
+  FS_HIP(hipStreamSynchronize(c.stream));
+  std::vector<int32_t> wt_atom, wt_off{0}, wt_cfg, rng_type, rng_wt0{0};
+  std::vector<int64_t> wt_pairs;
+  std::vector<std::vector<int32_t>> per_cfg(c.C);
+  const int64_t P = std::max<int64_t>(off[N], 1);
+  for (int t = 0; t < nt; ++t) {
+    const size_t w0 = wt_cfg.size();
+    int64_t tpairs = 0;
+    for (int k = 0; k < c.C; ++k) {
+      int cnt = 0;
+      int64_t pairs = 0;
+      auto close = [&]() {
+        wt_off.push_back(int32_t(wt_atom.size()));
+        per_cfg[k].push_back(int32_t(wt_cfg.size()));
+        wt_cfg.push_back(k);
+        wt_pairs.push_back(pairs);
+        tpairs += pairs;
+        cnt = 0;
+        pairs = 0;
+      };
+      for (int64_t i = c.h_atom_off[k]; i < c.h_atom_off[k + 1]; ++i) {
+        if (c.h_types[size_t(i)] != t) continue;
+        wt_atom.push_back(int32_t(i));
+        pairs += off[size_t(i) + 1] - off[size_t(i)];
+        if (++cnt == kMlTile) close();
+      }
+      if (cnt > 0) close();
+    }
+    const size_t w1 = wt_cfg.size();
+    if (w1 == w0) continue;
+    const int nr = int(std::max<int64_t>(1, std::min<int64_t>(int64_t(w1 - w0),
+                                                              (kMlRanges * tpairs + P - 1) / P)));
+    int64_t acc = 0;
+    int r = 1;
+    for (size_t w = w0; w < w1; ++w) {
+      acc += wt_pairs[w];
+      if (r < nr && acc * nr >= tpairs * r && w + 1 < w1) {
+        rng_type.push_back(t);
+        rng_wt0.push_back(int32_t(w + 1));
+        ++r;
+      }
+    }
+    rng_type.push_back(t);
+    rng_wt0.push_back(int32_t(w1));
+  }
+  std::vector<int32_t> cfg_off{0}, cfg_wt;
+  for (int k = 0; k < c.C; ++k) {
+    cfg_wt.insert(cfg_wt.end(), per_cfg[k].begin(), per_cfg[k].end());
+    cfg_off.push_back(int32_t(cfg_wt.size()));
+  }
+  auto up = [&](DBuf<int32_t> &d, const std::vector<int32_t> &h) {
+    d.alloc(std::max<size_t>(h.size(), 1));
+    if (!h.empty())
+      FS_HIP(hipMemcpyAsync(d.p, h.data(), 4 * h.size(), hipMemcpyHostToDevice, c.stream));
+  };
+  up(c.pop_ml_wt_atom, wt_atom);
+  up(c.pop_ml_wt_off, wt_off);
+  up(c.pop_ml_wt_cfg, wt_cfg);
+  up(c.pop_ml_rng_type, rng_type);
+  up(c.pop_ml_rng_wt0, rng_wt0);
+  up(c.pop_ml_cfg_off, cfg_off);
+  up(c.pop_ml_cfg_wt, cfg_wt);
+  FS_HIP(hipStreamSynchronize(c.stream));
+  c.pop_ml_nwt = int(wt_cfg.size());
+  c.pop_ml_nrange = int(rng_type.size());
+  std::copy(key, key + 4, c.pop_ml_key);
 }
 
 // every allocation of a population evaluation of np members; no launch
@@ -1342,17 +1867,89 @@ void population_prepare(Context &c, int np) {
     FS_HIP(hipStreamSynchronize(c.stream));
     c.pop_grp_tiles = c.ntiles;
   }
+  c.pop_ml = c.pop_ml_allowed && ml_lds_bytes(c) > 0;
+  if (c.pop_ml) {
+    ml_layout(c);
+    const TableDesc &T0 = c.h_tabs[0];
+    c.pop_ml_n = T0.n;
+    c.pop_ml_xoff = T0.off;
+  }
   const int cap = population_batch_cap(c, np);
   c.pop_y.alloc(size_t(cap) * c.nknots);
   c.pop_s.alloc(size_t(cap) * c.nknots);
   c.pop_S.alloc(size_t(cap) * 5 * c.nknots);
   c.pop_i32.alloc(size_t(cap) * 2 * c.nivl);
   c.pop_i64.alloc(size_t(cap) * 4 * c.nivl);
-  c.pop_Fp.alloc(size_t(cap) * c.N);
-  c.pop_Fp32.alloc(size_t(cap) * c.N);
-  c.pop_partE.alloc(size_t(cap) * c.C * G * kPopE);
-  c.pop_partF.alloc(size_t(cap) * c.C * G * kPopF);
   c.pop_cmc.alloc(size_t(cap) * c.C);
+  if (c.pop_ml) {
+    const size_t ng = (size_t(cap) + 63) / 64;
+    c.pop_ml_ng = int(ng);
+    c.pop_ml_ys.alloc(ng * c.nknots * 64);
+    c.pop_ml_Fp.alloc(ng * size_t(c.N) * 64);
+    c.pop_ml_pE.alloc(ng * size_t(c.pop_ml_nwt) * kMlE * 64);
+    c.pop_ml_pF.alloc(ng * size_t(c.pop_ml_nwt) * kMlF * 64);
+  } else {
+    c.pop_Fp.alloc(size_t(cap) * c.N);
+    c.pop_Fp32.alloc(size_t(cap) * c.N);
+    c.pop_partE.alloc(size_t(cap) * c.C * G * kPopE);
+    c.pop_partF.alloc(size_t(cap) * c.C * G * kPopF);
+  }
+}
+
+// the member-lane passes over np members (population_prepare has chosen them)
+static void population_ml_launch(Context &c, const double *d_X, int np, double *d_cost) {
+  const int cap = population_batch_cap(c, np);
+  EvalArgs a = make_args(c);
+  const size_t lds_d = ml_lds_bytes(c);
+  int nemb = 0;
+  for (int t = 0; t < c.ntypes; ++t) nemb = std::max(nemb, c.h_tabs[c.npair + c.ntypes + t].n);
+  const size_t xrows = ml_xrows(c.pop_ml_n, nemb);
+  const size_t lds_f = (xrows + size_t(2 * c.ntypes) * c.pop_ml_n * 64) * sizeof(double2);
+  for (int m0 = 0; m0 < np; m0 += cap) {
+    const int nb = std::min(cap, np - m0);
+    const int ng = (nb + 63) / 64;
+    hipLaunchKernelGGL(k_pop_tables, dim3(nb), dim3(64), 0, c.stream, c.ntab, c.D, c.nknots,
+                       c.tabs.p, c.tab_x.p, c.tab_y.p, c.knot_of.p, d_X + size_t(m0) * c.D, c.pop_y.p,
+                       c.pop_s.p, c.pop_S.p, c.pop_i32.p, c.pop_i64.p, c.nivl);
+    FS_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_pop_ml_pack, dim3(unsigned((c.nknots + 3) / 4), unsigned(ng)), dim3(256), 0,
+                       c.stream, nb, c.nknots, c.pop_y.p, c.pop_s.p, c.pop_ml_ys.p);
+    FS_HIP(hipGetLastError());
+    MlArgs p{};
+    p.ng = ng;
+    p.nb = nb;
+    p.n = c.pop_ml_n;
+    p.linear = c.h_tabs[0].linear;
+    p.nknots = c.nknots;
+    p.nwt = c.pop_ml_nwt;
+    p.nrange = c.pop_ml_nrange;
+    p.x = c.tab_x.p + c.pop_ml_xoff;
+    p.xrows = int(xrows);
+    p.ys = c.pop_ml_ys.p;
+    p.Fp = c.pop_ml_Fp.p;
+    p.pE = c.pop_ml_pE.p;
+    p.pF = c.pop_ml_pF.p;
+    p.wt_atom = c.pop_ml_wt_atom.p;
+    p.wt_off = c.pop_ml_wt_off.p;
+    p.wt_cfg = c.pop_ml_wt_cfg.p;
+    p.rng_type = c.pop_ml_rng_type.p;
+    p.rng_wt0 = c.pop_ml_rng_wt0.p;
+    p.cfg_off = c.pop_ml_cfg_off.p;
+    p.cfg_wt = c.pop_ml_cfg_wt.p;
+    const int grid = std::max(1, std::min(c.cu_count, ng * c.pop_ml_nrange));
+    hipLaunchKernelGGL(k_pop_ml_density, dim3(grid), dim3(64 * kMlWaves), lds_d, c.stream, a, p);
+    FS_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_pop_ml_force, dim3(grid), dim3(64 * kMlWaves), lds_f, c.stream, a, p);
+    FS_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_pop_ml_finalize, dim3(unsigned(c.C), unsigned(ng)), dim3(64), 0, c.stream, p,
+                       c.C, c.vol.p, c.ref_energy.p, c.ref_stress.p, c.energy_weight, c.stress_weight,
+                       c.pop_cmc.p);
+    FS_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_pop_sum, dim3(unsigned((nb + 127) / 128)), dim3(128), 0, c.stream, nb, c.C,
+                       c.pop_cmc.p, c.n_smooth, c.smooth_rows.p, c.smooth_weight, c.pop_y.p,
+                       c.nknots, d_cost + m0);
+    FS_HIP(hipGetLastError());
+  }
 }
 
 // |F(x_m)|^2 of the np device vectors d_X[np][D] into d_cost[np]: launches
@@ -1363,6 +1960,10 @@ template <int M> void population_launch_m(Context &c, const double *d_X, int np,
 // members per block: c.pop_members (1, 2 or 4; the DE driver times them on its
 // first population)
 void population_launch(Context &c, const double *d_X, int np, double *d_cost) {
+  if (c.pop_ml) {
+    population_ml_launch(c, d_X, np, d_cost);
+    return;
+  }
   switch (c.pop_members) {
   case 2: population_launch_m<2>(c, d_X, np, d_cost); break;
   case 4: population_launch_m<4>(c, d_X, np, d_cost); break;
@@ -1445,7 +2046,7 @@ template <int M> void population_launch_m(Context &c, const double *d_X, int np,
 // variant gives the same costs: a member's sums do not depend on the others
 // in its block); returns the time of the chosen variant in ms
 double population_tune_launch(Context &c, const double *d_X, int np, double *d_cost) {
-  if (c.pop_members != 0) {
+  if (c.pop_members != 0 || c.pop_ml) {
     population_launch(c, d_X, np, d_cost);
     return 0.0;
   }

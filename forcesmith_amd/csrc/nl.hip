@@ -265,6 +265,7 @@ void launch_neighbor_build(Context &c) {
   c.pair_off.alloc(size_t(N) + 1);
   FS_HIP(hipMemsetAsync(c.pair_off.p, 0, sizeof(int64_t) * (N + 1), c.stream));
   c.P = 0;
+  ++c.nl_gen;
   if (!(rc > 0.0 && std::isfinite(rc)) || N == 0) {
     c.have_nl = true;
     return;

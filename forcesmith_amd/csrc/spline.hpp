@@ -84,35 +84,49 @@ __device__ inline void spl_site64(const TableDesc &T, const double *x, const dou
 }
 
 // uncached evaluation with linear extrapolation (embedding F(rho)); also
-// the host's evaluation of the embedding for the gauge
-__host__ __device__ inline void spl_uncached64(const TableDesc &T, const double *x, const double *y,
-                                      const double *s, double r, double &v, double &d) {
+// the host's evaluation of the embedding for the gauge. KV reads the knot
+// values and slopes: y(k), s(k).
+template <class KV>
+__host__ __device__ inline void spl_uncached64_kv(const TableDesc &T, const double *x, const KV &kv,
+                                                  double r, double &v, double &d) {
   const int n = T.n;
   if (r <= x[0]) {
-    const double slope = (y[1] - y[0]) / (x[1] - x[0]);
-    v = y[0] + slope * (r - x[0]);
+    const double slope = (kv.y(1) - kv.y(0)) / (x[1] - x[0]);
+    v = kv.y(0) + slope * (r - x[0]);
     d = slope;
   } else if (r >= x[n - 1]) {
-    const double slope = (y[n - 1] - y[n - 2]) / (x[n - 1] - x[n - 2]);
-    v = y[n - 1] + slope * (r - x[n - 1]);
+    const double slope = (kv.y(n - 1) - kv.y(n - 2)) / (x[n - 1] - x[n - 2]);
+    v = kv.y(n - 1) + slope * (r - x[n - 1]);
     d = slope;
   } else if (T.linear) {
     const int i = spl_locate(x, n, r);
     const double t = (r - x[i]) / (x[i + 1] - x[i]);
-    v = y[i] + t * (y[i + 1] - y[i]);
-    d = (y[i + 1] - y[i]) / (x[i + 1] - x[i]);
+    v = kv.y(i) + t * (kv.y(i + 1) - kv.y(i));
+    d = (kv.y(i + 1) - kv.y(i)) / (x[i + 1] - x[i]);
   } else {
     const int i = spl_locate(x, n, r);
     const double x0 = x[i], dx = x[i + 1] - x[i];
     const double t = (r - x0) / dx;
     const double omt = 1.0 - t;
-    const double a = y[i] * (1.0 + 2.0 * t) + s[i] * (r - x0);
-    const double da = y[i] * 2.0 / dx + s[i];
-    const double b = y[i + 1] * (3.0 - 2.0 * t) + dx * s[i + 1] * (t - 1.0);
-    const double db = -y[i + 1] * 2.0 / dx + s[i + 1];
+    const double yi = kv.y(i), yi1 = kv.y(i + 1), si = kv.s(i), si1 = kv.s(i + 1);
+    const double a = yi * (1.0 + 2.0 * t) + si * (r - x0);
+    const double da = yi * 2.0 / dx + si;
+    const double b = yi1 * (3.0 - 2.0 * t) + dx * si1 * (t - 1.0);
+    const double db = -yi1 * 2.0 / dx + si1;
     v = omt * omt * a + t * t * b;
     d = -2.0 * omt / dx * a + omt * omt * da + 2.0 * t / dx * b + t * t * db;
   }
+}
+
+struct KnotArrays {
+  const double *ya, *sa;
+  __host__ __device__ double y(int k) const { return ya[k]; }
+  __host__ __device__ double s(int k) const { return sa[k]; }
+};
+
+__host__ __device__ inline void spl_uncached64(const TableDesc &T, const double *x, const double *y,
+                                      const double *s, double r, double &v, double &d) {
+  spl_uncached64_kv(T, x, KnotArrays{y, s}, r, v, d);
 }
 
 // FP32 bulk evaluation. Interval k of table T holds the power form

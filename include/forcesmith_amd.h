@@ -335,6 +335,14 @@ int fs_set_jacobian_tile(fs_ctx *ctx, int32_t atoms);
  * give the same columns to FP32 rounding. No reference counterpart (kernel
  * choice). */
 int fs_set_jacobian_form(fs_ctx *ctx, int32_t form);
+/* Form of the batched population cost (fs_population_cost, fs_de_minimize):
+ * 0 (default) the member-lane passes where they apply (EAM models whose pair
+ * and density tables share one knot grid, with a block's 64 members' knot
+ * rows within the LDS): one member per lane, FP64 arithmetic with the cached
+ * sites of the FP64 path in either precision mode; otherwise, and with 1, the
+ * member-block passes (1, 2 or 4 members per block, the context's precision).
+ * No reference counterpart (kernel choice). */
+int fs_set_population_form(fs_ctx *ctx, int32_t form);
 int fs_get_launch(fs_ctx *ctx, int32_t *lanes_per_atom, int32_t *blocks_per_cu,
                   int32_t *jacobian_tile, int32_t *jacobian_blocks_per_cu);
 int fs_autotune(fs_ctx *ctx, int32_t *lanes_per_atom);

@@ -55,7 +55,10 @@ def test_screened_run_trains_in_driver(ternary):
     assert 0.0 < st["last_loss"] < 0.5 * st["first_loss"]
     assert np.all(np.diff(hist) <= 0)
     assert np.all(x >= lo) and np.all(x <= hi)
-    assert abs(best - g.cost(x)) <= 1e-9 * best
+    # the batched exact cost judged the best; the single-member residual path
+    # agrees within the mixed bar
+    assert best == g.population_cost(x[None, :])[0]
+    assert abs(best - g.cost(x)) <= 1e-6 * best
 
 
 @pytest.mark.timeout(600)
