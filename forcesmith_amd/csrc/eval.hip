@@ -1776,8 +1776,8 @@ int population_batch_cap(const Context &c, int np) {
 }
 
 // wave tiles of the member-lane passes (once per neighbour list and dataset):
-// the atoms of each type in atom order, split per configuration into tiles of
-// up to kMlTile; per type, ranges of tiles with about equal pair counts (about
+// the atoms of each type, per configuration in Morton order, split into tiles
+// of up to kMlTile; per type, ranges of tiles with about equal pair counts (about
 // kMlRanges in all); per configuration, its tiles in type order
 static void ml_layout(Context &c) {
   const uint64_t key[4] = {c.nl_gen, uint64_t(c.N), uint64_t(c.ntypes), uint64_t(c.C)};
@@ -1791,6 +1791,7 @@ static void ml_layout(Context &c) {
   std::vector<int32_t> wt_atom, wt_off{0}, wt_cfg, rng_type, rng_wt0{0};
   std::vector<int64_t> wt_pairs;
   std::vector<std::vector<int32_t>> per_cfg(c.C);
+  std::vector<std::pair<uint64_t, int32_t>> sel;
   const int64_t P = std::max<int64_t>(off[N], 1);
   for (int t = 0; t < nt; ++t) {
     const size_t w0 = wt_cfg.size();
@@ -1807,9 +1808,29 @@ static void ml_layout(Context &c) {
         cnt = 0;
         pairs = 0;
       };
+      // the configuration's atoms of type t in Morton order of their cell
+      // (fractional) coordinates: a tile is then a compact cluster, whose
+      // pairs share most partners, so the partners' F' rows are re-read from
+      // cache
+      sel.clear();
+      const double *I = c.h_inv.data() + 9 * size_t(k);
       for (int64_t i = c.h_atom_off[k]; i < c.h_atom_off[k + 1]; ++i) {
         if (c.h_types[size_t(i)] != t) continue;
-        wt_atom.push_back(int32_t(i));
+        const double *r = c.h_pos.data() + 3 * size_t(i);
+        uint64_t code = 0;
+        for (int d = 0; d < 3; ++d) {
+          double v = I[3 * d] * r[0] + I[3 * d + 1] * r[1];
+          v = v + I[3 * d + 2] * r[2];
+          v -= std::floor(v);
+          const uint64_t q = std::min<uint64_t>(1023, uint64_t(std::max(0.0, v) * 1024.0));
+          for (int b = 0; b < 10; ++b) code |= ((q >> b) & 1u) << (3 * b + d);
+        }
+        sel.push_back({code, int32_t(i)});
+      }
+      std::sort(sel.begin(), sel.end());
+      for (const auto &e : sel) {
+        const int32_t i = e.second;
+        wt_atom.push_back(i);
         pairs += off[size_t(i) + 1] - off[size_t(i)];
         if (++cnt == kMlTile) close();
       }
