@@ -39,10 +39,14 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, const fl
                                                    int lda, const float *__restrict__ B, int ldb,
                                                    float *__restrict__ C, int ldc,
                                                    const float *__restrict__ bias,
-                                                   float *__restrict__ H) {
+                                                   float *__restrict__ H, int kc,
+                                                   int64_t zstride) {
   __shared__ __attribute__((aligned(16))) __bf16 sA[kBM][kBK + 8];
   __shared__ __attribute__((aligned(16))) __bf16 sB[kBN][kBK + 8];
   const int m0 = blockIdx.y * kBM, n0 = blockIdx.x * kBN;
+  // split K: block z sums k in [z kc, (z + 1) kc) into the partial tile z
+  const int kb = int(blockIdx.z) * kc, ke = min(K, kb + kc);
+  C += int64_t(blockIdx.z) * zstride;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
   f32x4 acc[2][2];
@@ -50,19 +54,19 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, const fl
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < K; k0 += kBK) {
+  for (int k0 = kb; k0 < ke; k0 += kBK) {
     for (int e = threadIdx.x; e < kBM * kBK; e += 256) {
       const int r = e / kBK, kk = e % kBK;
       const int m = m0 + r, k = k0 + kk;
       float v = 0.f;
-      if (m < M && k < K) v = TA ? A[int64_t(k) * lda + m] : A[int64_t(m) * lda + k];
+      if (m < M && k < ke) v = TA ? A[int64_t(k) * lda + m] : A[int64_t(m) * lda + k];
       sA[r][kk] = __bf16(v);
     }
     for (int e = threadIdx.x; e < kBN * kBK; e += 256) {
       const int c = e / kBK, kk = e % kBK;
       const int n = n0 + c, k = k0 + kk;
       float v = 0.f;
-      if (n < N && k < K) v = TB ? B[int64_t(n) * ldb + k] : B[int64_t(k) * ldb + n];
+      if (n < N && k < ke) v = TB ? B[int64_t(n) * ldb + k] : B[int64_t(k) * ldb + n];
       sB[c][kk] = __bf16(v);
     }
     __syncthreads();
@@ -98,6 +102,17 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, const fl
       }
 }
 
+// C = the sum of the nz split-K partial tiles, in partial order
+__global__ void k_splitk_sum(int M, int N, int nz, const float *__restrict__ P, int64_t zstride,
+                             float *__restrict__ C, int ldc) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= int64_t(M) * N) return;
+  const int m = int(e / N), n = int(e % N);
+  float s = 0.f;
+  for (int z = 0; z < nz; ++z) s += P[z * zstride + int64_t(m) * N + n];
+  C[int64_t(m) * ldc + n] = s;
+}
+
 // y[b] = H2[b] . W3 + b3  (one wave per row)
 __global__ void k_head(int B, int Hn, const float *__restrict__ H2, const float *__restrict__ W3,
                        const float *__restrict__ b3, float *__restrict__ y) {
@@ -126,14 +141,28 @@ __global__ void k_gelu_back(int64_t n, const float *__restrict__ Z, float *__res
   if (e < n) dH[e] *= gelu_df(Z[e]);
 }
 
-// column sums over rows: out[c] = sum_b A[b*ld + c] (* w[b] if w)
-__global__ void k_colsum(int B, int Ncol, const float *__restrict__ A, const float *__restrict__ w,
-                         float *__restrict__ out) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= Ncol) return;
+// column sums over rows: out[c] = sum_b A[b*ld + c] (* w[b] if w). A block
+// takes 64 columns; its 16 row groups sum rows g, g + 16, ... in order, and
+// the group partials are added in group order: a fixed order, 16 independent
+// load streams per column instead of one serial walk of the rows
+constexpr int kColGroups = 16;
+__global__ __launch_bounds__(64 * kColGroups) void k_colsum(int B, int Ncol,
+                                                            const float *__restrict__ A,
+                                                            const float *__restrict__ w,
+                                                            float *__restrict__ out) {
+  __shared__ float part[kColGroups][64];
+  const int cl = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int c = blockIdx.x * 64 + cl;
   float s = 0.f;
-  for (int b = 0; b < B; ++b) s += A[int64_t(b) * Ncol + c] * (w ? w[b] : 1.0f);
-  out[c] = s;
+  if (c < Ncol)
+    for (int b = g; b < B; b += kColGroups) s += A[int64_t(b) * Ncol + c] * (w ? w[b] : 1.0f);
+  part[g][cl] = s;
+  __syncthreads();
+  if (g == 0 && c < Ncol) {
+    float t = 0.f;
+    for (int q = 0; q < kColGroups; ++q) t += part[q][cl];
+    out[c] = t;
+  }
 }
 
 __global__ void k_loss(int B, const float *__restrict__ y, const float *__restrict__ t,
@@ -169,13 +198,34 @@ inline unsigned cdiv(int64_t a, int b) { return unsigned((a + b - 1) / b); }
 // the kernels behind the step sequence of mlp_seq.hpp, on one stream
 struct GpuOps {
   hipStream_t s;
+  DBuf<float> *ws; // split-K partial tiles
+  // a product with few output tiles and a long K (the weight gradients:
+  // K = the batch) is split along K over enough blocks to fill the GPU; the
+  // partial tiles are summed in a fixed order
   template <bool TA, bool TB, int EPI>
   void gemm(int M, int N, int K, const float *A, int lda, const float *B, int ldb, float *C, int ldc,
             const float *bias, float *H) {
     if (M <= 0 || N <= 0) return;
     dim3 grid(cdiv(N, kBN), cdiv(M, kBM));
-    hipLaunchKernelGGL((k_gemm_bf16<TA, TB, EPI>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C,
-                       ldc, bias, H);
+    const int tiles = int(grid.x * grid.y);
+    int nz = 1;
+    if (EPI == EPI_STORE && ws && tiles < 128)
+      nz = std::max(1, std::min(256 / tiles, K / (4 * kBK)));
+    if (nz <= 1) {
+      hipLaunchKernelGGL((k_gemm_bf16<TA, TB, EPI>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C,
+                         ldc, bias, H, K, int64_t(0));
+      FS_HIP(hipGetLastError());
+      return;
+    }
+    const int kc = int(cdiv(cdiv(K, nz), kBK)) * kBK;
+    nz = int(cdiv(K, kc));
+    const int64_t zs = int64_t(M) * N;
+    ws->alloc(size_t(nz) * size_t(zs));
+    grid.z = unsigned(nz);
+    hipLaunchKernelGGL((k_gemm_bf16<TA, TB, EPI>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb,
+                       ws->p, N, bias, H, kc, zs);
+    FS_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_splitk_sum, dim3(cdiv(zs, 256)), dim3(256), 0, s, M, N, nz, ws->p, zs, C, ldc);
     FS_HIP(hipGetLastError());
   }
   void head(int B, int Hn, const float *H2, const float *W3, const float *b3, float *y) {
@@ -193,7 +243,8 @@ struct GpuOps {
     FS_HIP(hipGetLastError());
   }
   void colsum(int B, int Ncol, const float *A, const float *w, float *out) {
-    hipLaunchKernelGGL(k_colsum, dim3(cdiv(Ncol, 256)), dim3(256), 0, s, B, Ncol, A, w, out);
+    hipLaunchKernelGGL(k_colsum, dim3(cdiv(Ncol, 64)), dim3(64 * kColGroups), 0, s, B, Ncol, A, w,
+                       out);
     FS_HIP(hipGetLastError());
   }
   void gelu_back(int64_t n, const float *Z, float *dH) {
@@ -213,17 +264,17 @@ struct GpuOps {
 void mlp_init(Mlp &m, int D, int Hn) { mlp_state_init(m, D, Hn); }
 
 void mlp_forward(Mlp &m, hipStream_t s, int B, const float *X, float *y_out) {
-  GpuOps ops{s};
+  GpuOps ops{s, &m.ws};
   mlp_forward_seq(ops, m, B, X, y_out);
 }
 
 void mlp_backward(Mlp &m, hipStream_t s, int B, const float *X, const float *t) {
-  GpuOps ops{s};
+  GpuOps ops{s, &m.ws};
   mlp_backward_seq(ops, m, B, X, t);
 }
 
 void mlp_adam(Mlp &m, hipStream_t s, float lr) {
-  GpuOps ops{s};
+  GpuOps ops{s, &m.ws};
   mlp_adam_seq(ops, m, lr);
 }
 

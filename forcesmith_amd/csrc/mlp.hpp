@@ -9,7 +9,9 @@ namespace fsa {
 
 // Device MLP state (the workspace layout and the step order live in
 // mlp_seq.hpp, shared with the CPU tests).
-struct Mlp : MlpState<DBuf<float>> {};
+struct Mlp : MlpState<DBuf<float>> {
+  DBuf<float> ws; // split-K partial tiles of the weight-gradient products
+};
 
 void mlp_init(Mlp &m, int D, int Hn);
 void mlp_forward(Mlp &m, hipStream_t s, int B, const float *X, float *y_out);
