@@ -201,6 +201,11 @@ def test_edge_cases_clamp_empty_and_single_atom():
         assert np.max(np.abs(Fg - Fo) / (1 + np.abs(Fo))) <= tol
         _, _, _, L = g.eval_forces()
         assert L[1] > 0.0  # lone atom: rho = 0 is clamped to the span start
+        # the batched population cost on the same configurations (member-lane
+        # passes: FP64 in either precision mode)
+        x0 = o.gather()
+        X = np.stack([x0, 1.01 * x0, 0.98 * x0])
+        assert np.allclose(g.population_cost(X), o.population_cost(X), rtol=1e-10, atol=0)
 
 
 def test_exact_edge_pairs_follow_linear_sites():
@@ -218,6 +223,10 @@ def test_exact_edge_pairs_follow_linear_sites():
     Fo = o.eval_forces()[0]
     Fg = g.eval_forces()[0]
     assert np.abs(Fg - Fo).max() <= 1e-5
+    # the population passes take the same linear site for that pair
+    x0 = o.gather()
+    X = np.stack([x0, 1.03 * x0, 0.97 * x0])
+    assert np.allclose(g.population_cost(X), o.population_cost(X), rtol=1e-10, atol=0)
 
 
 @pytest.mark.parametrize("name", ["eam3", "adp2", "sw"])
