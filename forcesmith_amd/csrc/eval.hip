@@ -1358,11 +1358,10 @@ __global__ __launch_bounds__(1024) void k_pop_ml_density(EvalArgs a, MlArgs p) {
         double rho = 0.0, ep = 0.0;
         for (int64_t pc = p0; pc < p1; pc += 64) {
           const int np = int(min<int64_t>(64, p1 - pc));
-          // sites, lanes as pairs: meta = k | tj << 16 | pair-energy bit; a
-          // pair with no table role keeps a zero basis (it adds exact zeros,
-          // and the walk needs no branch)
+          // sites, lanes as pairs: meta = k | tj << 16 | pair-energy bit 24 |
+          // density bit 25 (0 for a pair with no table role)
           int meta = 0;
-          double b0 = 0.0, b1 = 0.0, b2 = 0.0, b3 = 0.0;
+          double b1 = 0.0, b2 = 0.0, b3 = 0.0;
           if (lane < np) {
             const int64_t q = pc + lane;
             const uint32_t b = a.bits[q];
@@ -1370,8 +1369,7 @@ __global__ __launch_bounds__(1024) void k_pop_ml_density(EvalArgs a, MlArgs p) {
               double bv[4], bd[4];
               const int k = ml_site(p, sX, a.r64[q], bv, bd);
               const bool phi = (b & kFlagPhi) && !(b & kFlagTiny);
-              meta = k | (pj_type(b) << 16) | (phi ? (1 << 24) : 0);
-              b0 = bv[0];
+              meta = k | (pj_type(b) << 16) | (phi ? (1 << 24) : 0) | (1 << 25);
               b1 = bv[1];
               b2 = bv[2];
               b3 = bv[3];
@@ -1389,11 +1387,16 @@ __global__ __launch_bounds__(1024) void k_pop_ml_density(EvalArgs a, MlArgs p) {
             R[2] = rp[0];
             R[3] = rp[64];
           };
+          // the value basis sums to one on the knot values (Hermite and
+          // linear sites alike): y_k + b1 (y_k+1 - y_k) + b2 s_k + b3 s_k+1;
+          // a pair with no table role (bit 25 clear: off the span, or a
+          // padding lane) adds nothing
           auto sum = [&](int q, const double2 (&R)[4]) {
-            const double B0 = ml_lane(b0, q), B1 = ml_lane(b1, q), B2 = ml_lane(b2, q),
-                         B3 = ml_lane(b3, q);
-            rho += B0 * R[0].x + B1 * R[1].x + B2 * R[0].y + B3 * R[1].y;
-            if (ml_lane(meta, q) & (1 << 24)) ep += B0 * R[2].x + B1 * R[3].x + B2 * R[2].y + B3 * R[3].y;
+            const int m = ml_lane(meta, q);
+            if (!(m & (1 << 25))) return;
+            const double B1 = ml_lane(b1, q), B2 = ml_lane(b2, q), B3 = ml_lane(b3, q);
+            rho += R[0].x + B1 * (R[1].x - R[0].x) + B2 * R[0].y + B3 * R[1].y;
+            if (m & (1 << 24)) ep += R[2].x + B1 * (R[3].x - R[2].x) + B2 * R[2].y + B3 * R[3].y;
           };
           // (an odd count walks the zero-basis lane np as well: the reads and
           // sums are unconditional; scheduling barriers keep each read set
@@ -1463,14 +1466,16 @@ __global__ __launch_bounds__(1024) void k_pop_ml_force(EvalArgs a, MlArgs p) {
         double f0 = 0.0, f1 = 0.0, f2 = 0.0;
         for (int64_t pc = p0; pc < p1; pc += 64) {
           const int np = int(min<int64_t>(64, p1 - pc));
-          // sites, lanes as pairs: meta = k | tj << 16, the partner's index in
-          // its configuration, the derivative basis over r and the
-          // displacement. The tables share one span, so the pair,
+          // sites, lanes as pairs: meta = k | tj << 7 | (the partner's index
+          // in its configuration) << 10, the derivative basis over r and the
+          // displacement. The basis is (b0, -b0, b2, b3): the Hermite and the
+          // linear derivative bases both weigh the interval's two knot values
+          // by exact negatives. The tables share one span, so the pair,
           // partner-density and centre-density gates agree; a pair outside it
           // (or at r < 1e-14) keeps a zero basis and displacement and adds
           // exact zeros.
-          int meta = 0, jl = 0;
-          double b0 = 0.0, b1 = 0.0, b2 = 0.0, b3 = 0.0, e0 = 0.0, e1 = 0.0, e2 = 0.0;
+          int meta = 0;
+          double b0 = 0.0, b2 = 0.0, b3 = 0.0, e0 = 0.0, e1 = 0.0, e2 = 0.0;
           if (lane < np) {
             const int64_t q = pc + lane;
             const uint32_t b = a.bits[q];
@@ -1484,10 +1489,8 @@ __global__ __launch_bounds__(1024) void k_pop_ml_force(EvalArgs a, MlArgs p) {
               double bv[4], bd[4];
               const int k = ml_site(p, sX, r, bv, bd);
               const double inv_r = 1.0 / r;
-              meta = k | (pj_type(b) << 16);
-              jl = int(pj_local(b));
+              meta = k | (pj_type(b) << 7) | int(pj_local(b) << 10);
               b0 = bd[0] * inv_r;
-              b1 = bd[1] * inv_r;
               b2 = bd[2] * inv_r;
               b3 = bd[3] * inv_r;
             }
@@ -1495,8 +1498,8 @@ __global__ __launch_bounds__(1024) void k_pop_ml_force(EvalArgs a, MlArgs p) {
           // one pair ahead: pair q + 1's member rows and partner F' are read
           // while pair q is summed (two register sets, the walk unrolled by two)
           auto rows = [&](int q, double2 (&R)[6], double &fj) {
-            const int m = ml_lane(meta, q);
-            const int k = m & 0xffff, tj = m >> 16;
+            const uint32_t m = uint32_t(ml_lane(meta, q));
+            const int k = int(m & 127u), tj = int((m >> 7) & 7u);
             const double2 *rp = sR + (tj * n + k) * 64 + lane;
             const double2 *rj = sR + ((nt + tj) * n + k) * 64 + lane;
             R[0] = rp[0];
@@ -1505,15 +1508,17 @@ __global__ __launch_bounds__(1024) void k_pop_ml_force(EvalArgs a, MlArgs p) {
             R[3] = rj[64];
             R[4] = rowi[k * 64];
             R[5] = rowi[k * 64 + 64];
-            fj = Fpg[(cbase + ml_lane(jl, q)) * 64];
+            fj = Fpg[(cbase + int64_t(m >> 10)) * 64];
           };
+          // the three roles' rows weighted first (pair + F'_i density of j +
+          // F'_j density of i), then one basis dot
           auto sum = [&](int q, const double2 (&R)[6], double fpj) {
-            const double B0 = ml_lane(b0, q), B1 = ml_lane(b1, q), B2 = ml_lane(b2, q),
-                         B3 = ml_lane(b3, q);
-            const double dphi = B0 * R[0].x + B1 * R[1].x + B2 * R[0].y + B3 * R[1].y;
-            const double dgj = B0 * R[2].x + B1 * R[3].x + B2 * R[2].y + B3 * R[3].y;
-            const double dgi = B0 * R[4].x + B1 * R[5].x + B2 * R[4].y + B3 * R[5].y;
-            const double fs = dphi + fpi * dgj + fpj * dgi;
+            const double B0 = ml_lane(b0, q), B2 = ml_lane(b2, q), B3 = ml_lane(b3, q);
+            const double y0 = R[0].x + fpi * R[2].x + fpj * R[4].x;
+            const double y1 = R[1].x + fpi * R[3].x + fpj * R[5].x;
+            const double s0 = R[0].y + fpi * R[2].y + fpj * R[4].y;
+            const double s1 = R[1].y + fpi * R[3].y + fpj * R[5].y;
+            const double fs = B0 * (y0 - y1) + B2 * s0 + B3 * s1;
             const double D0 = ml_lane(e0, q), D1 = ml_lane(e1, q), D2 = ml_lane(e2, q);
             const double fx = fs * D0, fy = fs * D1, fz = fs * D2;
             f0 += fx;
@@ -1744,7 +1749,7 @@ static size_t ml_lds_bytes(const Context &c) {
   if (c.kind != FS_EAM || c.ntab == 0 || c.P == 0 || c.ntypes > kMaxTypes) return 0;
   const int nt = c.ntypes, nsh = c.npair + nt;
   const TableDesc &T0 = c.h_tabs[0];
-  if (T0.n < 2 || T0.n > 0xffff) return 0;
+  if (T0.n < 2 || T0.n > 128) return 0; // the force pass packs k in 7 bits
   for (int t = 1; t < nsh; ++t) {
     const TableDesc &T = c.h_tabs[t];
     if (T.n != T0.n || T.linear != T0.linear) return 0;
