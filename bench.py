@@ -329,11 +329,11 @@ def measured_traffic(kernel="k_jac_de"):
     return None
 
 
-def measured_limiter(kernel="k_jac_de"):
+def measured_limiter(kernel="k_jac_de", summary=None):
     """What the counters say bounds the kernel, from its committed summary
     (the line 'limiter: ...'), or None."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                        f"{kernel}_traffic_eam_ternary_500cfg.md")
+                        summary or f"{kernel}_traffic_eam_ternary_500cfg.md")
     try:
         for line in open(path):
             if line.startswith("limiter:"):
@@ -554,6 +554,9 @@ def main():
             f"{tm['host_algebra_ms']:.2f} ms ({100 * tm['host_algebra_ms'] / tm['total_ms']:.1f} %)")
     if kind == "eam" and a.de_sample_generations > 0:
         out["de"] = de_sample(w, P, a.de_sample_generations, a.de_np, a.de_lanes)
+        if ternary500:
+            # the population passes' bound, from their committed counter summary
+            out["de"]["limiter"] = measured_limiter(summary="k_pop_ml_eam_ternary_500cfg.md")
     if not a.no_cpu_baseline:
         host = host_cpu()
         threads = a.cpu_threads or baseline_threads(host)
