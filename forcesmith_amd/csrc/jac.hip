@@ -176,7 +176,7 @@ struct JacArgs {
   double *Fpp;            // [N]
   const double *mu, *lamb; // ADP per-atom dipole (3) and quadrupole (6)
   int Ddip, Dquad;        // ADP dipole / quadrupole free columns
-  float *dmu, *dlam;      // ADP moment tangents [N][Ddip][4], [N][Dquad][6]
+  float *dmu, *dlam;      // ADP moment tangents [N][Ddip][3], [N][Dquad][6]
   float *J;               // [R x D]
   double *part;           // [ntiles x kJP x D]
   const double *vol;
@@ -1368,13 +1368,14 @@ __global__ __launch_bounds__(256) void k_jac_dens_ks(JacArgs a, KsArgs k, int dc
 // ---------------------------------------------------------------------------
 enum { JP_DIP = 3, JP_QUAD = 4 };
 constexpr int kQBatch = 4; // neighbour moment tangents gathered per batch
-// moment-tangent rows: [atom][column][comp], dipole padded to a float4,
-// quadrupole as three float2, so that one lane reads its column's tangent
-// with one or three loads and a wave's gather is one contiguous span
+// moment-tangent rows: [atom][column][comp], dipole as a float3 (12 B, one
+// three-dword load), quadrupole as three float2, so that one lane reads its
+// column's tangent with one or three loads and a wave's gather is one
+// contiguous span
 
 template <bool DIP> __device__ inline void store_moment(float *base, int64_t row, const float *v) {
   if constexpr (DIP) {
-    reinterpret_cast<float4 *>(base)[row] = make_float4(v[0], v[1], v[2], 0.0f);
+    reinterpret_cast<float3 *>(base)[row] = make_float3(v[0], v[1], v[2]);
   } else {
     float2 *d = reinterpret_cast<float2 *>(base) + 3 * row;
     d[0] = make_float2(v[0], v[1]);
@@ -1384,14 +1385,14 @@ template <bool DIP> __device__ inline void store_moment(float *base, int64_t row
 }
 
 // bytes per moment-tangent entry (atom, column)
-template <bool DIP> constexpr uint32_t moment_bytes() { return DIP ? 16u : 24u; }
+template <bool DIP> constexpr uint32_t moment_bytes() { return DIP ? 12u : 24u; }
 
 // the same load from a 32-bit byte offset off the uniform table base (the
 // table is < 4 GiB): one VALU op per address in the gather loops
 template <bool DIP> __device__ inline void load_moment_at(const float *base, uint32_t off, float *v) {
   const char *b = reinterpret_cast<const char *>(base) + off;
   if constexpr (DIP) {
-    const float4 q = *reinterpret_cast<const float4 *>(b);
+    const float3 q = *reinterpret_cast<const float3 *>(b);
     v[0] = q.x;
     v[1] = q.y;
     v[2] = q.z;
@@ -1409,7 +1410,7 @@ template <bool DIP> __device__ inline void load_moment_at(const float *base, uin
 
 template <bool DIP> __device__ inline void load_moment(const float *base, int64_t row, float *v) {
   if constexpr (DIP) {
-    const float4 q = reinterpret_cast<const float4 *>(base)[row];
+    const float3 q = reinterpret_cast<const float3 *>(base)[row];
     v[0] = q.x;
     v[1] = q.y;
     v[2] = q.z;
