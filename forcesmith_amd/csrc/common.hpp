@@ -337,6 +337,7 @@ struct Context {
   uint64_t pop_ml_key[4] = {~0ull, ~0ull, ~0ull, ~0ull};
   int pop_ml_nwt = 0, pop_ml_nrange = 0, pop_ml_n = 0, pop_ml_xoff = 0, pop_ml_nemb = 0;
   int pop_ml_ng = 0; // member groups allocated
+  size_t pop_ml_lds = 0; // LDS of the member-lane density pass (0: the passes do not apply)
   DBuf<int32_t> pop_ml_wt_atom, pop_ml_wt_off, pop_ml_wt_cfg, pop_ml_rng_type, pop_ml_rng_wt0;
   DBuf<int32_t> pop_ml_cfg_off, pop_ml_cfg_wt;
   DBuf<double2> pop_ml_ys;

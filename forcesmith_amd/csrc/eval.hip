@@ -1893,7 +1893,8 @@ void population_prepare(Context &c, int np) {
     FS_HIP(hipStreamSynchronize(c.stream));
     c.pop_grp_tiles = c.ntiles;
   }
-  c.pop_ml = c.pop_ml_allowed && ml_lds_bytes(c) > 0;
+  c.pop_ml_lds = c.pop_ml_allowed ? ml_lds_bytes(c) : 0;
+  c.pop_ml = c.pop_ml_lds > 0;
   if (c.pop_ml) {
     ml_layout(c);
     const TableDesc &T0 = c.h_tabs[0];
@@ -1926,7 +1927,7 @@ void population_prepare(Context &c, int np) {
 static void population_ml_launch(Context &c, const double *d_X, int np, double *d_cost) {
   const int cap = population_batch_cap(c, np);
   EvalArgs a = make_args(c);
-  const size_t lds_d = ml_lds_bytes(c);
+  const size_t lds_d = c.pop_ml_lds; // measured by population_prepare
   int nemb = 0;
   for (int t = 0; t < c.ntypes; ++t) nemb = std::max(nemb, c.h_tabs[c.npair + c.ntypes + t].n);
   const size_t xrows = ml_xrows(c.pop_ml_n, nemb);
