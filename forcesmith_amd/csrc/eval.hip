@@ -198,16 +198,20 @@ template <int LPA, class T> __device__ inline T sub_sum(T v) {
   return v;
 }
 
-// descriptors into LDS, and the FP32 hi/lo and FP64 interval forms after
-// them when they fit the staging budget (a.stage_ivl); otherwise the lookups
-// read the interval forms from global memory (a few KB per table, L2-resident)
+// descriptors into LDS, and with ST the FP32 hi/lo and FP64 interval forms
+// after them (while they fit the staging budget, a.stage_ivl); without ST the
+// lookups read the interval forms from global memory (a few KB per table,
+// L2-resident). ST is a template argument: a pointer that may address either
+// LDS or global memory compiles to flat loads, measurably slower than LDS
+// reads on every lookup.
+template <bool ST>
 __device__ inline void stage_tables(const EvalArgs &a, TableDesc *sT, float4 *smem_ivl,
                                     const float4 *&sI, const double *&sP) {
   const float4 *src = reinterpret_cast<const float4 *>(a.tabs);
   float4 *dst = reinterpret_cast<float4 *>(sT);
   const int nq = a.ntab * int(sizeof(TableDesc) / sizeof(float4));
   for (int k = threadIdx.x; k < nq; k += blockDim.x) dst[k] = src[k];
-  if (a.stage_ivl) {
+  if constexpr (ST) {
     for (int k = threadIdx.x; k < 2 * a.nivl; k += blockDim.x) smem_ivl[k] = a.ivl32[k];
     double *p64 = reinterpret_cast<double *>(smem_ivl + 2 * a.nivl);
     for (int k = threadIdx.x; k < 4 * a.nivl; k += blockDim.x) p64[k] = a.ivl64[k];
@@ -371,7 +375,7 @@ __device__ inline void density_pair(const Tabs &tb, const EvalArgs &a, int ti, u
   }
 }
 
-template <int KIND, bool F64, int LPA>
+template <int KIND, bool F64, int LPA, bool ST>
 __global__ __launch_bounds__(256) void k_density(EvalArgs a) {
   // Densities (and ADP moments) feed the energy rows directly, whose 1e-6
   // cost bar is dominated by |E| >> |E - E_ref|: the per-pair values are
@@ -380,7 +384,7 @@ __global__ __launch_bounds__(256) void k_density(EvalArgs a) {
   TableDesc *sT = reinterpret_cast<TableDesc *>(smem);
   const float4 *sI;
   const double *sP;
-  stage_tables(a, sT, smem + a.ntab * int(sizeof(TableDesc) / sizeof(float4)), sI, sP);
+  stage_tables<ST>(a, sT, smem + a.ntab * int(sizeof(TableDesc) / sizeof(float4)), sI, sP);
   const Tabs tb{sT, sI, a.tx, a.ty, a.ts, sP};
   const int sub = threadIdx.x / LPA, lane = threadIdx.x % LPA;
   for (int tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
@@ -581,7 +585,7 @@ __device__ inline void force_pair(const Tabs &tb, const EvalArgs &a, int ti, int
 
 constexpr int kPart = 11; // energy, limit, 9 virial components
 
-template <int KIND, bool F64, int LPA>
+template <int KIND, bool F64, int LPA, bool ST>
 __global__ __launch_bounds__(256) void k_force(EvalArgs a) {
   using R = typename std::conditional<F64, double, float>::type;
   constexpr int APB = 256 / LPA;
@@ -590,7 +594,7 @@ __global__ __launch_bounds__(256) void k_force(EvalArgs a) {
   __shared__ double sRed[APB][kPart];
   const float4 *sI;
   const double *sP;
-  stage_tables(a, sT, smem + a.ntab * int(sizeof(TableDesc) / sizeof(float4)), sI, sP);
+  stage_tables<ST>(a, sT, smem + a.ntab * int(sizeof(TableDesc) / sizeof(float4)), sI, sP);
   const Tabs tb{sT, sI, a.tx, a.ty, a.ts, sP};
   const int sub = threadIdx.x / LPA, lane = threadIdx.x % LPA;
   for (int tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
@@ -1655,9 +1659,15 @@ EvalArgs make_args(Context &c) {
 
 template <int KIND, bool F64, int LPA> void run_eval(Context &c, const EvalArgs &a, size_t lds,
                                                       int grid) {
-  hipLaunchKernelGGL((k_density<KIND, F64, LPA>), dim3(grid), dim3(256), lds, c.stream, a);
-  FS_HIP(hipGetLastError());
-  hipLaunchKernelGGL((k_force<KIND, F64, LPA>), dim3(grid), dim3(256), lds, c.stream, a);
+  if (a.stage_ivl) {
+    hipLaunchKernelGGL((k_density<KIND, F64, LPA, true>), dim3(grid), dim3(256), lds, c.stream, a);
+    FS_HIP(hipGetLastError());
+    hipLaunchKernelGGL((k_force<KIND, F64, LPA, true>), dim3(grid), dim3(256), lds, c.stream, a);
+  } else {
+    hipLaunchKernelGGL((k_density<KIND, F64, LPA, false>), dim3(grid), dim3(256), lds, c.stream, a);
+    FS_HIP(hipGetLastError());
+    hipLaunchKernelGGL((k_force<KIND, F64, LPA, false>), dim3(grid), dim3(256), lds, c.stream, a);
+  }
   FS_HIP(hipGetLastError());
 }
 

@@ -189,9 +189,15 @@ struct JacArgs {
   int ivl_lo, nivl_st, stage_ivl;
 };
 
-// LDS staging of the descriptors and, when it fits the staging budget, the
-// FP32 slope bands (else sS points at their FP32 copy in global memory);
-// returns the start of the per-wave staging area that follows
+// LDS staging of the descriptors and, by TS, of the tables the lookups read:
+// TS 0 reads the FP32 slope bands (their FP32 copy) and the interval forms in
+// place from global memory, TS 1 stages the slope bands, TS 2 the slope bands
+// and the density tables' FP32 interval forms. The choice is a template
+// argument, not a run-time flag: a pointer that may address either LDS or
+// global memory compiles to flat loads, measurably slower than LDS reads on
+// every table lookup. Returns the start of the per-wave staging area that
+// follows.
+template <int TS>
 __device__ inline void *stage(const JacArgs &a, float4 *smem, TableDesc *&sT, const float *&sS,
                               int nk, const float4 **sI = nullptr) {
   sT = reinterpret_cast<TableDesc *>(smem);
@@ -199,7 +205,7 @@ __device__ inline void *stage(const JacArgs &a, float4 *smem, TableDesc *&sT, co
   const int nq = a.ntab * int(sizeof(TableDesc) / sizeof(float4));
   for (int k = threadIdx.x; k < nq; k += blockDim.x) smem[k] = src[k];
   float *after = reinterpret_cast<float *>(smem + nq);
-  if (a.stage_S) {
+  if constexpr (TS >= 1) {
     for (int k = threadIdx.x; k < nk * 5; k += blockDim.x) after[k] = float(a.tS[k]);
     sS = after;
     after += (nk * 5 + 3) & ~3;
@@ -209,7 +215,7 @@ __device__ inline void *stage(const JacArgs &a, float4 *smem, TableDesc *&sT, co
   // the FP32 interval forms of the density tables (the staging's spline
   // derivatives), intervals [ivl_lo, ivl_lo + nivl_st), while they fit; *sI
   // is indexed by interval like the global array
-  if (a.stage_ivl) {
+  if constexpr (TS >= 2) {
     float4 *d = reinterpret_cast<float4 *>(after);
     for (int k = threadIdx.x; k < 2 * a.nivl_st; k += blockDim.x) d[k] = a.ivl32[2 * a.ivl_lo + k];
     if (sI) *sI = d - 2 * a.ivl_lo;
@@ -396,11 +402,12 @@ __global__ __launch_bounds__(256) void k_emb_sites(JacArgs a) {
 }
 
 // density columns [dc0, dc0 + 256) of H and Hs, lane + 64 s per lane
+template <int TS>
 __global__ __launch_bounds__(256) void k_jac_dens(JacArgs a, int nknots, int dc0) {
   extern __shared__ float4 smem[];
   TableDesc *sT;
   const float *sS;
-  DensStage *sAll = reinterpret_cast<DensStage *>(stage(a, smem, sT, sS, nknots));
+  DensStage *sAll = reinterpret_cast<DensStage *>(stage<TS>(a, smem, sT, sS, nknots));
   const int wave = uni(threadIdx.x >> 6), lane = threadIdx.x & 63;
   DensStage &P = sAll[wave];
   const int64_t gw = int64_t(blockIdx.x) * 4 + wave;
@@ -488,7 +495,7 @@ __global__ __launch_bounds__(256) void k_jac_dens(JacArgs a, int nknots, int dc0
 //            the density-chain and per-atom terms).
 enum { JP_PAIR = 1, JP_DE = 2 };
 constexpr int kHBatch = 8; // Hs rows gathered per batch by the density-chain term
-constexpr int kWavesPair = 5, kWavesDE = 3; // waves per SIMD asked of the two passes
+constexpr int kWavesPair = 5, kWavesDE = 4; // waves per SIMD asked of the two passes
 
 struct JStagePair {
   int32_t seg[kSegPad]; // partner-type segment starts of the chunk
@@ -545,7 +552,7 @@ __device__ inline void add_pair(ColAcc &A, float *t, float c, float r, float u0,
   add_virial(t, c, r, u0, u1, u2);
 }
 
-template <int PASS, int NS>
+template <int PASS, int NS, int TS>
 __device__ __forceinline__ void jac_main_body(const JacArgs &a, int nknots, const SlotDesc *slots) {
   using Stage = typename JStageOf<PASS>::T;
   constexpr bool kPairPass = PASS == JP_PAIR;
@@ -553,7 +560,7 @@ __device__ __forceinline__ void jac_main_body(const JacArgs &a, int nknots, cons
   TableDesc *sT;
   const float *sS;
   const float4 *sI;
-  Stage *sAll = reinterpret_cast<Stage *>(stage(a, smem, sT, sS, nknots, &sI));
+  Stage *sAll = reinterpret_cast<Stage *>(stage<TS>(a, smem, sT, sS, nknots, &sI));
   // the tile-end reduction reuses the staging area of the four waves: one
   // entry per partial row and slot column (s, lane)
   static_assert(4 * sizeof(Stage) >= sizeof(double) * kJP * 64 * NS, "reduction buffer");
@@ -919,21 +926,21 @@ __device__ __forceinline__ void jac_main_body(const JacArgs &a, int nknots, cons
 }
 
 // The pass kernels. The window and gather loops are latency-bound, so each
-// pass asks for W waves per SIMD (the register budget follows): 3 for the
-// density/embedding pass (measured 1 % faster than 4, whose tighter register
-// budget puts a few values in scratch; its LDS staging would allow 4 blocks
-// of 4 waves per CU), and 5 for the lighter pair pass (4 and 6 measured the
-// same).
-template <int NS, int W>
+// pass asks for W waves per SIMD (the register budget follows): 4 for the
+// density/embedding pass (128 VGPRs, no scratch; with the density tables'
+// interval forms staged in LDS the compiler's own choice, 131 VGPRs, allows
+// only 3 and measured 0.93 -> 1.12 ms on the 500-configuration ternary
+// workload), and 5 for the lighter pair pass (4 and 6 measured the same).
+template <int NS, int W, int TS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k_jac_pair(
     JacArgs a, int nknots, const SlotDesc *slots) {
-  jac_main_body<JP_PAIR, NS>(a, nknots, slots);
+  jac_main_body<JP_PAIR, NS, TS>(a, nknots, slots);
 }
 
-template <int NS, int W>
+template <int NS, int W, int TS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k_jac_de(
     JacArgs a, int nknots, const SlotDesc *slots) {
-  jac_main_body<JP_DE, NS>(a, nknots, slots);
+  jac_main_body<JP_DE, NS, TS>(a, nknots, slots);
 }
 
 // ---------------------------------------------------------------------------
@@ -1445,14 +1452,14 @@ __device__ inline int partner_type(int pt, int ti) {
   return ti == ta ? tb : (ti == tb ? ta : -1);
 }
 
-template <int PASS, int NS>
+template <int PASS, int NS, int TS>
 __global__ __launch_bounds__(256) void k_adp_mom(JacArgs a, int nknots, const SlotDesc *slots) {
   constexpr bool kDip = PASS == JP_DIP;
   constexpr int NC = kDip ? 3 : 6;
   extern __shared__ float4 smem[];
   TableDesc *sT;
   const float *sS;
-  MomStage *sAll = reinterpret_cast<MomStage *>(stage(a, smem, sT, sS, nknots));
+  MomStage *sAll = reinterpret_cast<MomStage *>(stage<TS>(a, smem, sT, sS, nknots));
   const int wave = uni(threadIdx.x >> 6), lane = threadIdx.x & 63;
   MomStage &P = sAll[wave];
   int cm[NS], cqc[NS], cpt[NS];
@@ -1567,14 +1574,14 @@ __device__ inline void add_dvirial(float *t, float x, float y, float z, float f0
   t[6] += y * f2;
 }
 
-template <int PASS, int NS>
+template <int PASS, int NS, int TS>
 __global__ __launch_bounds__(256) void k_adp_dq(JacArgs a, int nknots, const SlotDesc *slots) {
   constexpr bool kDip = PASS == JP_DIP;
   constexpr int NC = kDip ? 3 : 6;
   extern __shared__ float4 smem[];
   TableDesc *sT;
   const float *sS;
-  DQStage *sAll = reinterpret_cast<DQStage *>(stage(a, smem, sT, sS, nknots));
+  DQStage *sAll = reinterpret_cast<DQStage *>(stage<TS>(a, smem, sT, sS, nknots));
   static_assert(4 * sizeof(DQStage) >= sizeof(double) * kJP * 64 * NS, "reduction buffer");
   double(*sPart)[64 * NS] = reinterpret_cast<double(*)[64 * NS]>(sAll);
   const int wave = uni(threadIdx.x >> 6), lane = threadIdx.x & 63;
@@ -2450,6 +2457,14 @@ void launch_jacobian(Context &c) {
       a.tS32 = c.tab_S32.p;
     }
     const size_t base = desc + (a.stage_S ? band : 0) + (a.stage_ivl ? ivl : 0);
+    // table staging as a template argument of the passes (see stage); the
+    // passes without interval lookups stage at most the slope bands
+    const int ts = a.stage_S ? (a.stage_ivl ? 2 : 1) : 0;
+    auto with_ts = [](int v, auto &&f) {
+      if (v == 2) f(std::integral_constant<int, 2>{});
+      else if (v == 1) f(std::integral_constant<int, 1>{});
+      else f(std::integral_constant<int, 0>{});
+    };
     const size_t lds_dens = base + 4 * sizeof(DensStage);
     const size_t lds_pair = base + 4 * sizeof(JStagePair), lds_de = base + 4 * sizeof(JStageDE);
     FS_REQUIRE(lds_de <= 64 * 1024 && lds_pair <= 64 * 1024, FS_ERR_UNSUPPORTED,
@@ -2535,12 +2550,17 @@ void launch_jacobian(Context &c) {
           else
             hipLaunchKernelGGL((k_jac_pair_ks<4>), dim3(g2), dim3(256), lds, c.stream, a, ks,
                                d_slots + size_t(g) * 64);
-        } else if (NSP == 2)
-          hipLaunchKernelGGL((k_jac_pair<2, kWavesPair>), dim3(g2), dim3(256), lds_pair, c.stream, a,
-                             c.nknots, d_slots + size_t(g) * 64);
-        else
-          hipLaunchKernelGGL((k_jac_pair<4, 2>), dim3(g2), dim3(256), lds_pair, c.stream, a, c.nknots,
-                             d_slots + size_t(g) * 64);
+        } else {
+          with_ts(ts, [&](auto tsc) {
+            constexpr int TS = decltype(tsc)::value;
+            if (NSP == 2)
+              hipLaunchKernelGGL((k_jac_pair<2, kWavesPair, TS>), dim3(g2), dim3(256), lds_pair,
+                                 c.stream, a, c.nknots, d_slots + size_t(g) * 64);
+            else
+              hipLaunchKernelGGL((k_jac_pair<4, 2, TS>), dim3(g2), dim3(256), lds_pair, c.stream, a,
+                                 c.nknots, d_slots + size_t(g) * 64);
+          });
+        }
         FS_HIP(hipGetLastError());
       }
       if (use_ks && TSP > 0) {
@@ -2555,20 +2575,27 @@ void launch_jacobian(Context &c) {
         if (use_ks)
           hipLaunchKernelGGL(k_jac_dens_ks, dim3(g1), dim3(256), 4 * 2 * sizeof(float) * size_t(ks.ld),
                              c.stream, a, ks, dc0);
+        else if (ts > 0)
+          hipLaunchKernelGGL(k_jac_dens<1>, dim3(g1), dim3(256), lds_dens, c.stream, a, c.nknots, dc0);
         else
-          hipLaunchKernelGGL(k_jac_dens, dim3(g1), dim3(256), lds_dens, c.stream, a, c.nknots, dc0);
+          hipLaunchKernelGGL(k_jac_dens<0>, dim3(g1), dim3(256), lds_dens, c.stream, a, c.nknots, dc0);
         FS_HIP(hipGetLastError());
       }
       c.time_main_begin(0);
       for (int g = 0; g < TSD; g += NSD) {
         const SlotDesc *sl = d_de + size_t(g) * 64;
-        if (NSD == 2)
-          hipLaunchKernelGGL((k_jac_de<2, kWavesDE>), dim3(g2), dim3(256), lds_de, c.stream, a,
-                             c.nknots, sl);
-        else if (NSD == 4)
-          hipLaunchKernelGGL((k_jac_de<4, 2>), dim3(g2), dim3(256), lds_de, c.stream, a, c.nknots, sl);
-        else
-          hipLaunchKernelGGL((k_jac_de<6, 1>), dim3(g2), dim3(256), lds_de, c.stream, a, c.nknots, sl);
+        with_ts(ts, [&](auto tsc) {
+          constexpr int TS = decltype(tsc)::value;
+          if (NSD == 2)
+            hipLaunchKernelGGL((k_jac_de<2, kWavesDE, TS>), dim3(g2), dim3(256), lds_de, c.stream, a,
+                               c.nknots, sl);
+          else if (NSD == 4)
+            hipLaunchKernelGGL((k_jac_de<4, 2, TS>), dim3(g2), dim3(256), lds_de, c.stream, a,
+                               c.nknots, sl);
+          else
+            hipLaunchKernelGGL((k_jac_de<6, 1, TS>), dim3(g2), dim3(256), lds_de, c.stream, a,
+                               c.nknots, sl);
+        });
         FS_HIP(hipGetLastError());
       }
       if (adp) {
@@ -2587,28 +2614,30 @@ void launch_jacobian(Context &c) {
         FS_REQUIRE(lds_dq <= 64 * 1024, FS_ERR_UNSUPPORTED, "ADP Jacobian LDS budget exceeded");
         const SlotDesc *d_u = d_de + size_t(TSD) * 64, *d_w = d_u + size_t(TSU) * 64;
         // every moment-tangent group before any force pass reads the tangents
-        auto pass = [&](auto kind, int ns, int total, const SlotDesc *slots0) {
-          constexpr int K = decltype(kind)::value;
+        auto pass = [&](auto kind, auto tsc, int ns, int total, const SlotDesc *slots0) {
+          constexpr int K = decltype(kind)::value, TS = decltype(tsc)::value;
           if (ns == 0) return;
           for (int g = 0; g < total; g += ns) {
             const SlotDesc *sl = slots0 + size_t(g) * 64;
-            if (ns == 1) hipLaunchKernelGGL((k_adp_mom<K, 1>), dim3(g1), dim3(256), lds_mom, c.stream, a, c.nknots, sl);
-            else if (ns == 2) hipLaunchKernelGGL((k_adp_mom<K, 2>), dim3(g1), dim3(256), lds_mom, c.stream, a, c.nknots, sl);
-            else hipLaunchKernelGGL((k_adp_mom<K, 4>), dim3(g1), dim3(256), lds_mom, c.stream, a, c.nknots, sl);
+            if (ns == 1) hipLaunchKernelGGL((k_adp_mom<K, 1, TS>), dim3(g1), dim3(256), lds_mom, c.stream, a, c.nknots, sl);
+            else if (ns == 2) hipLaunchKernelGGL((k_adp_mom<K, 2, TS>), dim3(g1), dim3(256), lds_mom, c.stream, a, c.nknots, sl);
+            else hipLaunchKernelGGL((k_adp_mom<K, 4, TS>), dim3(g1), dim3(256), lds_mom, c.stream, a, c.nknots, sl);
             FS_HIP(hipGetLastError());
           }
           if (K == JP_QUAD) c.time_main_begin(2);
           for (int g = 0; g < total; g += ns) {
             const SlotDesc *sl = slots0 + size_t(g) * 64;
-            if (ns == 1) hipLaunchKernelGGL((k_adp_dq<K, 1>), dim3(g2), dim3(256), lds_dq, c.stream, a, c.nknots, sl);
-            else if (ns == 2) hipLaunchKernelGGL((k_adp_dq<K, 2>), dim3(g2), dim3(256), lds_dq, c.stream, a, c.nknots, sl);
-            else hipLaunchKernelGGL((k_adp_dq<K, 4>), dim3(g2), dim3(256), lds_dq, c.stream, a, c.nknots, sl);
+            if (ns == 1) hipLaunchKernelGGL((k_adp_dq<K, 1, TS>), dim3(g2), dim3(256), lds_dq, c.stream, a, c.nknots, sl);
+            else if (ns == 2) hipLaunchKernelGGL((k_adp_dq<K, 2, TS>), dim3(g2), dim3(256), lds_dq, c.stream, a, c.nknots, sl);
+            else hipLaunchKernelGGL((k_adp_dq<K, 4, TS>), dim3(g2), dim3(256), lds_dq, c.stream, a, c.nknots, sl);
             FS_HIP(hipGetLastError());
           }
           if (K == JP_QUAD) c.time_main_end(2);
         };
-        pass(std::integral_constant<int, JP_DIP>{}, NSU, TSU, d_u);
-        pass(std::integral_constant<int, JP_QUAD>{}, NSW, TSW, d_w);
+        with_ts(std::min(ts, 1), [&](auto tsc) {
+          pass(std::integral_constant<int, JP_DIP>{}, tsc, NSU, TSU, d_u);
+          pass(std::integral_constant<int, JP_QUAD>{}, tsc, NSW, TSW, d_w);
+        });
       }
       c.time_main_end();
     }
