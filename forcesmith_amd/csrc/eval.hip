@@ -111,8 +111,9 @@ __device__ void interval_form(const TableDesc &T, int k, const double *x, const 
 
 // every table of one parameter set, by the threads of one block: slopes and
 // band per knot, then the interval forms (n < 4: piecewise linear, no slopes)
-__device__ void tables_block(int ntab, const TableDesc *tabs, int nknots, const double *x,
-                             const double *y, double *s, double *S, float4 *ivl, double *p64) {
+__device__ __forceinline__ void tables_block(int ntab, const TableDesc *tabs, int nknots,
+                                             const double *x, const double *y, double *s,
+                                             double *S, float4 *ivl, double *p64) {
   for (int e = threadIdx.x; e < nknots; e += blockDim.x) {
     int t = 0;
     while (t + 1 < ntab && tabs[t + 1].off <= e) ++t;
@@ -135,17 +136,48 @@ __device__ void tables_block(int ntab, const TableDesc *tabs, int nknots, const 
   }
 }
 
+// tables_block over LDS copies: the descriptors always (every knot searches
+// them for its table) and, with ST, the knots' abscissae and values (each
+// slope reads ten of them). One block does all of it, so without the copies
+// the knots' dependent global reads set the kernel's time. ST is a template
+// argument so that the reads compile to LDS loads (a pointer that may address
+// either memory compiles to flat loads); the launch passes
+// tables_lds_bytes(c, ST) of dynamic LDS.
+template <bool ST>
+__device__ __forceinline__ void tables_staged(int ntab, const TableDesc *tabs, int nknots,
+                                              const double *x, const double *y, double *s,
+                                              double *S, float4 *ivl, double *p64) {
+  extern __shared__ float4 tlds[];
+  const int nq = ntab * int(sizeof(TableDesc) / sizeof(float4));
+  for (int k = threadIdx.x; k < nq; k += blockDim.x) tlds[k] = reinterpret_cast<const float4 *>(tabs)[k];
+  const TableDesc *sT = reinterpret_cast<const TableDesc *>(tlds);
+  if constexpr (ST) {
+    double *sx = reinterpret_cast<double *>(tlds + nq);
+    for (int k = threadIdx.x; k < nknots; k += blockDim.x) {
+      sx[k] = x[k];
+      sx[nknots + k] = y[k];
+    }
+    __syncthreads();
+    tables_block(ntab, sT, nknots, sx, sx + nknots, s, S, ivl, p64);
+  } else {
+    __syncthreads();
+    tables_block(ntab, sT, nknots, x, y, s, S, ivl, p64);
+  }
+}
+
+template <bool ST>
 __global__ void k_slopes(int ntab, int nknots, const TableDesc *__restrict__ tabs,
                          const double *__restrict__ x, const double *__restrict__ y,
                          double *__restrict__ s, double *__restrict__ S, float4 *__restrict__ ivl,
                          double *__restrict__ p64) {
-  tables_block(ntab, tabs, nknots, x, y, s, S, ivl, p64);
+  tables_staged<ST>(ntab, tabs, nknots, x, y, s, S, ivl, p64);
 }
 
 // the free knots from a device parameter vector, then every table's slopes
 // and interval forms (the parameter scatter of the captured step folded into
 // the slopes launch; one block, so the knot writes are visible after the
 // barrier)
+template <bool ST>
 __global__ void k_scatter_slopes(int D, const double *__restrict__ xv,
                                  const int32_t *__restrict__ knot_of, int ntab, int nknots,
                                  const TableDesc *__restrict__ tabs, const double *__restrict__ x,
@@ -154,7 +186,7 @@ __global__ void k_scatter_slopes(int D, const double *__restrict__ xv,
                                  double *__restrict__ p64) {
   for (int c = threadIdx.x; c < D; c += blockDim.x) y[knot_of[c]] = xv[c];
   __syncthreads();
-  tables_block(ntab, tabs, nknots, x, y, s, S, ivl, p64);
+  tables_staged<ST>(ntab, tabs, nknots, x, y, s, S, ivl, p64);
 }
 
 __global__ void k_scatter(int D, const double *__restrict__ xv, const int32_t *__restrict__ knot_of,
@@ -777,6 +809,7 @@ struct PopArgs {
 };
 
 // per-member tables: knots = base knots with the free ones from X[m]
+template <bool ST>
 __global__ void k_pop_tables(int ntab, int D, int nknots, const TableDesc *__restrict__ tabs,
                              const double *__restrict__ x, const double *__restrict__ y0,
                              const int32_t *__restrict__ knot_of, const double *__restrict__ X,
@@ -788,7 +821,7 @@ __global__ void k_pop_tables(int ntab, int D, int nknots, const TableDesc *__res
   __syncthreads();
   for (int c = threadIdx.x; c < D; c += blockDim.x) ym[knot_of[c]] = X[size_t(m) * D + c];
   __syncthreads();
-  tables_block(ntab, tabs, nknots, x, ym, s + size_t(m) * nknots, S + size_t(m) * 5 * nknots,
+  tables_staged<ST>(ntab, tabs, nknots, x, ym, s + size_t(m) * nknots, S + size_t(m) * 5 * nknots,
                ivl + size_t(m) * 2 * nivl, p64 + size_t(m) * 4 * nivl);
 }
 
@@ -1690,18 +1723,30 @@ void launch_scatter_params(Context &c, const double *d_x) {
   FS_HIP(hipGetLastError());
 }
 
+// the slope kernels stage the knots' abscissae and values while they fit the
+// staging budget beside the descriptors
+static bool tables_st(const Context &c) {
+  return size_t(c.ntab) * sizeof(TableDesc) + 2 * sizeof(double) * size_t(c.nknots) <= kTableStageBytes;
+}
+static size_t tables_lds_bytes(const Context &c, bool st) {
+  return size_t(c.ntab) * sizeof(TableDesc) + (st ? 2 * sizeof(double) * size_t(c.nknots) : 0);
+}
+
 void launch_scatter_slopes(Context &c, const double *d_x) {
   if (c.kind == FS_SW || c.ntab == 0) return;
-  hipLaunchKernelGGL(k_scatter_slopes, dim3(1), dim3(256), 0, c.stream, c.D, d_x, c.knot_of.p, c.ntab,
-                     c.nknots, c.tabs.p, c.tab_x.p, c.tab_y.p, c.tab_s.p, c.tab_S.p, c.ivl32.p,
-                     c.ivl64.p);
+  const bool st = tables_st(c);
+  hipLaunchKernelGGL(st ? k_scatter_slopes<true> : k_scatter_slopes<false>, dim3(1), dim3(256),
+                     tables_lds_bytes(c, st), c.stream, c.D, d_x, c.knot_of.p, c.ntab, c.nknots,
+                     c.tabs.p, c.tab_x.p, c.tab_y.p, c.tab_s.p, c.tab_S.p, c.ivl32.p, c.ivl64.p);
   FS_HIP(hipGetLastError());
 }
 
 void launch_slopes(Context &c) {
   if (c.kind == FS_SW || c.ntab == 0) return;
-  hipLaunchKernelGGL(k_slopes, dim3(1), dim3(256), 0, c.stream, c.ntab, c.nknots, c.tabs.p,
-                     c.tab_x.p, c.tab_y.p, c.tab_s.p, c.tab_S.p, c.ivl32.p, c.ivl64.p);
+  const bool st = tables_st(c);
+  hipLaunchKernelGGL(st ? k_slopes<true> : k_slopes<false>, dim3(1), dim3(256),
+                     tables_lds_bytes(c, st), c.stream, c.ntab, c.nknots, c.tabs.p, c.tab_x.p,
+                     c.tab_y.p, c.tab_s.p, c.tab_S.p, c.ivl32.p, c.ivl64.p);
   FS_HIP(hipGetLastError());
 }
 
@@ -1945,7 +1990,8 @@ static void population_ml_launch(Context &c, const double *d_X, int np, double *
   for (int m0 = 0; m0 < np; m0 += cap) {
     const int nb = std::min(cap, np - m0);
     const int ng = (nb + 63) / 64;
-    hipLaunchKernelGGL(k_pop_tables, dim3(nb), dim3(64), 0, c.stream, c.ntab, c.D, c.nknots,
+    hipLaunchKernelGGL(tables_st(c) ? k_pop_tables<true> : k_pop_tables<false>, dim3(nb), dim3(64),
+                       tables_lds_bytes(c, tables_st(c)), c.stream, c.ntab, c.D, c.nknots,
                        c.tabs.p, c.tab_x.p, c.tab_y.p, c.knot_of.p, d_X + size_t(m0) * c.D, c.pop_y.p,
                        c.pop_s.p, c.pop_S.p, c.pop_i32.p, c.pop_i64.p, c.nivl);
     FS_HIP(hipGetLastError());
@@ -2024,7 +2070,8 @@ template <int M> void population_launch_m(Context &c, const double *d_X, int np,
   const size_t lds_d = lds_t + (stage_d ? mem_d : 0), lds_f = lds_t + (stage_f ? mem_f : 0);
   for (int m0 = 0; m0 < np; m0 += cap) {
     const int nb = std::min(cap, np - m0);
-    hipLaunchKernelGGL(k_pop_tables, dim3(nb), dim3(64), 0, c.stream, c.ntab, c.D, c.nknots,
+    hipLaunchKernelGGL(tables_st(c) ? k_pop_tables<true> : k_pop_tables<false>, dim3(nb), dim3(64),
+                       tables_lds_bytes(c, tables_st(c)), c.stream, c.ntab, c.D, c.nknots,
                        c.tabs.p, c.tab_x.p, c.tab_y.p, c.knot_of.p, d_X + size_t(m0) * c.D, c.pop_y.p,
                        c.pop_s.p, c.pop_S.p, c.pop_i32.p, c.pop_i64.p, c.nivl);
     FS_HIP(hipGetLastError());
