@@ -617,8 +617,13 @@ __device__ inline void force_pair(const Tabs &tb, const EvalArgs &a, int ti, int
 
 constexpr int kPart = 11; // energy, limit, 9 virial components
 
+// the mixed ADP force pass asks for 4 waves per SIMD (128 VGPRs, some values
+// in scratch; the compiler's own 158 allow 3): 0.231 -> 0.206 ms for the
+// residual of the binary 200-configuration workload. 5 measured slower, and
+// 6 for the EAM pass (80 VGPRs) measured no faster.
 template <int KIND, bool F64, int LPA, bool ST>
-__global__ __launch_bounds__(256) void k_force(EvalArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KIND == FS_ADP && !F64 ? 4 : 1))) void
+k_force(EvalArgs a) {
   using R = typename std::conditional<F64, double, float>::type;
   constexpr int APB = 256 / LPA;
   extern __shared__ float4 smem[];

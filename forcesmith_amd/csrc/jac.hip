@@ -1574,8 +1574,14 @@ __device__ inline void add_dvirial(float *t, float x, float y, float z, float f0
   t[6] += y * f2;
 }
 
+// waves per SIMD asked of the moment-column force passes: 6 for the dipole
+// pass (80 VGPRs, a few values in scratch) and 5 for the quadrupole pass (96,
+// none); the compiler's own choices (83 and 113 VGPRs) allow 5 and 4, and
+// measured 0.67-0.71 against 0.75 ms for the quadrupole pass of the binary
+// 200-configuration workload
 template <int PASS, int NS, int TS>
-__global__ __launch_bounds__(256) void k_adp_dq(JacArgs a, int nknots, const SlotDesc *slots) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PASS == JP_DIP ? 6 : 5))) void
+k_adp_dq(JacArgs a, int nknots, const SlotDesc *slots) {
   constexpr bool kDip = PASS == JP_DIP;
   constexpr int NC = kDip ? 3 : 6;
   extern __shared__ float4 smem[];
