@@ -407,8 +407,13 @@ __device__ inline void density_pair(const Tabs &tb, const EvalArgs &a, int ti, u
   }
 }
 
+// the mixed density passes ask for 8 (EAM, 54 VGPRs) and 5 (ADP, 92) waves per
+// SIMD, one more than the compiler's own register choices allow (68 and 106,
+// no scratch either way): residual 0.202 -> 0.200 ms (EAM, 500 configurations)
+// and 0.208 -> 0.204 ms (binary ADP, 200 configurations)
 template <int KIND, bool F64, int LPA, bool ST>
-__global__ __launch_bounds__(256) void k_density(EvalArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(F64 ? 1 : (KIND == FS_ADP ? 5 : 8)))) void
+k_density(EvalArgs a) {
   // Densities (and ADP moments) feed the energy rows directly, whose 1e-6
   // cost bar is dominated by |E| >> |E - E_ref|: the per-pair values are
   // evaluated in FP64 in both precision modes, from the FP64 distance.
