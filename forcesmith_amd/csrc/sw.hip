@@ -213,7 +213,11 @@ struct SwArgs {
 // The first kSwStage neighbours' records of each centre (unit vector, h and
 // type, -1 for a pair that takes no part in triplets) and the lambda table
 // are staged in LDS.
-__global__ __launch_bounds__(256) void k_sw_centre(SwArgs a) {
+// 5 waves per SIMD asked (96 VGPRs, two values in scratch; the compiler's own
+// 98 allow 4): residual 0.068 -> 0.065 ms on the 256-configuration Si workload.
+// The same ask of the pair-term and Jacobian centre passes spilled more and
+// measured slower.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_sw_centre(SwArgs a) {
   constexpr int kGroups = 256 / kSwLanes;
   __shared__ double4 s_u[kGroups][kSwStage];
   __shared__ int s_ty[kGroups][kSwStage];
